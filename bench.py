@@ -1,0 +1,174 @@
+#!/usr/bin/env python3
+"""Steps/s of the FP64 Boussinesq ETD2 step at 384x256x256 on one MI355X.
+
+Workload: NR=384, NTH=256, NX=256, NRCHOP=256, NTCHOP=NXCHOP=86, ELL=2,
+ZLEN=4 pi, nu=kappa=1e-4, hyperviscosity filter P=4 (nu_p=kappa_p=1e-14)
+beside CN viscosity, Omega=1, Nbar=2, ETD2 stepping, dt=5e-4.  Initial state:
+Lamb-Oseen vortex (QVORTEX, Gamma=1) plus seeded complex Gaussian noise
+1e-3 exp(-(n/16)^2-(m/4)^2-(k/4)^2) on psi, chi and b (synthetic data).
+
+A step is one full ETD2-AB/CN step (mod_march.f90:811-901) including the
+per-step DIAGNOST checks.  The Richardson start and W warm-up steps are
+untimed; K steps are timed between device synchronisations.  During the
+timed steps the library records HIP events around every launch of its
+kernel classes on its own stream; the class with the largest summed time is
+reported in `roofline` with its algorithmic work per launch.  The CPU
+baseline times the FP64 C restatement (OpenMP) on this host's cores over a
+bounded sample of steps of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+METRIC = "steps/s at 384\u00d7256\u00d7256: split-prec vs FP64 GPU (\u22641e-10 err); adjoint/fwd cost"
+PEAK_FP64 = 78.6e12      # MI355X FP64 matrix, FLOP/s
+PEAK_HBM = 8.0e12        # MI355X HBM3E, B/s
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_cfg2.json")
+
+
+def parse():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--cpu-steps", type=int, default=2, help="steps in the CPU baseline sample (0: skip)")
+    ap.add_argument("--no-timer", action="store_true", help="time without per-kernel events")
+    return ap.parse_args()
+
+
+def initial_state(H, g, cfg):
+    from mlegs_amd.configs import LAMB_OSEEN_Q, seeded_noise
+
+    psi, chi, b = g.scalar(), g.scalar(), g.scalar()
+    H.qvortex(g, LAMB_OSEEN_Q, psi, chi)
+    p, _, ln = psi.download()
+    psi.upload(p + seeded_noise(cfg, 1), H.FFF, ln)
+    chi.upload(seeded_noise(cfg, 2), H.FFF, 0.0)
+    b.upload(seeded_noise(cfg, 3), H.FFF, 0.0)
+    return psi, chi, b
+
+
+def cpu_baseline(cfg, nsteps):
+    """The oracle (FP64 C, OpenMP) on this host: steps/s over `nsteps` steps."""
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import numpy as np
+    import _oracle as O
+    from mlegs_amd.configs import LAMB_OSEEN_Q, seeded_noise
+
+    cores = min(16, os.cpu_count() or 1)
+    o = O.OGrid(cfg, threads=cores)
+    try:
+        psi, chi = o.field(O.FFF), o.field(O.FFF)
+        o.call("qvortex", LAMB_OSEEN_Q, psi, chi)
+        psi.set(psi.arr() + seeded_noise(cfg, 1), O.FFF, psi.ln)
+        chi.set(seeded_noise(cfg, 2), O.FFF)
+        b = o.field(O.FFF, np.asfortranarray(seeded_noise(cfg, 3)))
+        s = O.OSolver(o)
+        s.prepare()
+        t0 = time.perf_counter()
+        for _ in range(nsteps):
+            rc = s.step(psi, chi, b)
+            if rc:
+                raise RuntimeError(f"oracle step returned {rc}")
+        dt = time.perf_counter() - t0
+        s.close()
+    finally:
+        o.close()
+    return dict(value=nsteps / dt, unit="steps/s", cores=cores, kind="port",
+                sample=f"{nsteps} ETD2 steps of the 384x256x256 workload from the seeded state "
+                       f"(ETD operators set, no Richardson start), FP64 C restatement with OpenMP, "
+                       f"{cores} threads, {dt:.1f} s")
+
+
+def main():
+    a = parse()
+    if a.gpus != 1:
+        print("bench.py: this build runs on one GPU only; --gpus must be 1 "
+              "(the multi-GPU decomposition is a design note, not built)", file=sys.stderr)
+        sys.exit(2)
+    import torch
+
+    sys.path.insert(0, ROOT)
+    from mlegs_amd import hip as H
+    from mlegs_amd.configs import CFG2
+
+    assert torch.cuda.is_available(), "bench.py needs a HIP device"
+    torch.cuda.set_device(0)
+    cfg = dict(CFG2)
+    g = H.Grid(cfg)
+    psi, chi, b = initial_state(H, g, cfg)
+    s = H.Solver(g)
+    s.initialize(psi, chi, b)
+    if a.warmup:
+        s.time_stepping(psi, chi, b, a.warmup)
+    torch.cuda.synchronize()
+    g.synchronize()
+    print(f"bench: initialised, {a.warmup} warm-up steps done; timing {a.steps} steps", file=sys.stderr, flush=True)
+    if not a.no_timer:
+        g.timer(True)
+    t0 = time.perf_counter()
+    s.time_stepping(psi, chi, b, a.steps)
+    torch.cuda.synchronize()
+    g.synchronize()
+    el = time.perf_counter() - t0
+    g.timer(False)
+    ms_step = el / a.steps * 1e3
+
+    stages, dom = {}, None
+    if not a.no_timer:
+        for k in H.TIMED_KERNELS:
+            ms, n, work, unit = g.timer_read(k)
+            if n == 0:
+                continue
+            stages[k] = dict(ms_per_step=round(ms / a.steps, 4), launches_per_step=n / a.steps,
+                             avg_us=round(ms / n * 1e3, 2), work_per_launch=work / n, unit=unit)
+            if dom is None or ms > stages[dom]["ms_per_step"] * a.steps:
+                dom = k
+    roof = None
+    if dom:
+        st = stages[dom]
+        rate = st["work_per_launch"] / (st["avg_us"] * 1e-6)
+        flop = st["unit"] == "flop"
+        peak = PEAK_FP64 if flop else PEAK_HBM
+        traffic = None
+        if os.path.exists(TRAFFIC_FILE):
+            traffic = json.load(open(TRAFFIC_FILE)).get(dom)
+        roof = dict(bound="mfma" if flop else "hbm", kernel=dom,
+                    achieved=round(rate / (1e12 if flop else 1e9), 3), peak=(78.6 if flop else 8000.0),
+                    unit="TFLOP/s" if flop else "GB/s", frac=round(rate / peak, 4), traffic=traffic,
+                    work_per_launch=st["work_per_launch"], avg_launch_us=st["avg_us"])
+    cpu = None
+    if a.cpu_steps > 0:
+        print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
+        cpu = cpu_baseline(cfg, a.cpu_steps)
+    out = {
+        "metric": METRIC,
+        "value": round(a.steps / el, 4),
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": round(ms_step, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Lamb-Oseen vortex + seeded noise 1e-3)",
+        "config": {"workload": "384x256x256 ETD2 FP64 step (NRCHOP=256, NTCHOP=NXCHOP=86, dt=5e-4)",
+                   "nr": cfg["nr"], "nth": cfg["nth"], "nx": cfg["nx"], "nrchop": cfg["nrchop"],
+                   "ntchop": cfg["ntchop"], "nxchop": cfg["nxchop"], "scheme": "ETD2-AB/CN + P=4 filter",
+                   "precision": "fp64"},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+        "stages": stages,
+    }
+    s.close()
+    g.close()
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,204 @@
+/*
+ * mlegs_hip.h -- C ABI of libmlegs_hip.so, the MI355X (gfx950) implementation
+ * of the MLegS pseudo-spectral Boussinesq time step.
+ *
+ * The reference (Jinge-Wang/MLegS) has no FFI: its interface is a set of
+ * Fortran modules.  Every entry point below replaces one module procedure;
+ * the comment gives the reference routine (file:line) it stands in for.  A
+ * Fortran iso_c_binding shim and a Python ctypes layer bind these symbols.
+ *
+ * Conventions
+ *  - Plain C types only.  Complex values are interleaved (re, im) doubles.
+ *  - Host arrays use the reference layout (column-major E(d1,d2,d3)):
+ *      FFF  E(NRCHOPDIM, NTCHOPDIM, NXCHOPDIM)  spectral (n, m, k)
+ *      PFF  E(NDIMR,     NTCHOPDIM, NXCHOPDIM)  (r, m, k)
+ *      PFP  E(NDIMR,     NDIMTH,    NDIMX)      (r, m, z)
+ *      PPP  E(NDIMR,     NDIMTH,    NDIMX)      (r, theta slot, z)
+ *    with NDIMR=NR+3, NDIMTH=NTH+1, NDIMX=NX, NRCHOPDIM=min(NRCHOP+6,NR),
+ *    NTCHOPDIM=min(NTCHOP,NTH), NXCHOPDIM=2*NXCHOP-1 (mod_fft.f90:222-235).
+ *  - Device data stay on the GPU; every call is asynchronous on the context's
+ *    stream except the ones documented as synchronising (downloads, error
+ *    reads).  Field log-term coefficients (SCALAR%LN) live on the device too.
+ *  - Every function returns an int status using the reference's error codes
+ *    (mod_scalar3.f90:88-106): 0 ok, 1 SCALAR3, 10 NaN, 11 dealiasing,
+ *    12 divergence, 13 log term, 2 TRANSFORM, 3 LEGOPERATOR, 30 deprecated,
+ *    31 CALCULATION, 32 DIMMISMATCH, 4 MPI, 5 SOLVER; plus 100+ for
+ *    library/runtime errors (MLG_E_*).  The reference aborts on these; the
+ *    library returns them and the Fortran shim turns them into ERROR STOP.
+ *  - A context and its fields are used from one host thread at a time.
+ */
+#ifndef MLEGS_HIP_H
+#define MLEGS_HIP_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* Grid and physics parameters (read.input blocks DIMENSION, CHOPS, ELL,
+ * ZLEN, DT, VISCOSITY, BOUSSINESQ; mod_init.f90:401-710). */
+typedef struct mlg_params {
+  int nr, nth, nx;            /* collocation points: r, theta slots, z   */
+  int nrchop, ntchop, nxchop; /* truncations                            */
+  int minc, mklink;           /* azimuthal symmetry, m-k link (0)        */
+  double ell, zlen;           /* map parameter L, axial period           */
+  double nu, kappa;           /* viscosity, diffusivity                  */
+  double nup, kappap;         /* hyperviscosity / hyperdiffusivity       */
+  int visc_sw, visc_p;        /* VISC%SW (0,1,3), hyperviscosity power P */
+  double omega, bv0;          /* rotation rate, Brunt-Vaisala frequency  */
+  int adams;                  /* 1: AB2-CN, 0: ETD2-AB with CN           */
+  int hyper_cn;               /* 1: also apply the SW=3 filter with CN   */
+  double dt;
+} mlg_params;
+
+typedef struct mlg_ctx mlg_ctx;
+typedef struct mlg_field mlg_field;
+typedef struct mlg_solver mlg_solver;
+
+/* space tags: mod_scalar3.f90:82-85 */
+enum { MLG_PPP = 0, MLG_FFF = 1, MLG_PFP = 2, MLG_PFF = 3 };
+
+/* library error codes (reference codes are listed above) */
+enum {
+  MLG_E_ARG = 100,      /* bad argument / unsupported configuration */
+  MLG_E_HIP = 101,      /* HIP runtime error                        */
+  MLG_E_NOMEM = 102,    /* device allocation failed                 */
+  MLG_E_SPACE = 103     /* field in the wrong space for the call    */
+};
+
+/* ------------------------------------------------------------------ */
+/* Set-up: LEGINIT / SETUP_GRID (src/mod_fft.f90:165-367).  Builds the
+ * Gauss-Legendre grid, normalisations and the Pbar_n^m tables on the host
+ * (extended precision) and uploads them.  Uses the current HIP device. */
+int mlg_init(const mlg_params *p, mlg_ctx **ctx);
+/* Frees every device resource of the context (FINALIZE, mod_march.f90:310-353). */
+int mlg_finalize(mlg_ctx *ctx);
+/* out[12] = {NDIMR, NDIMTH, NDIMX, NRCHOPDIM, NTCHOPDIM, NXCHOPDIM, NRH,
+ *            NXCHOPH, NRCHOP (current), NPF, NNORM, NR} */
+int mlg_dims(const mlg_ctx *ctx, int *out);
+/* Host copy of a set-up table: "x","w","r","lnx","pf","lognorm","norm",
+ * "at0","at1","ak".  n = capacity in doubles; returns MLG_E_ARG if short. */
+int mlg_get_table(const mlg_ctx *ctx, const char *name, double *out, long n, long *len);
+/* Use an existing HIP stream (hipStream_t) for all work of this context;
+ * NULL selects the context's own stream. */
+int mlg_set_stream(mlg_ctx *ctx, void *hip_stream);
+/* Blocks until all queued work of the context is done; returns the first
+ * sticky device-side error code raised since the last call, then clears it. */
+int mlg_synchronize(mlg_ctx *ctx);
+/* Human-readable name of a status code. */
+const char *mlg_error_string(int code);
+
+/* CHOPSET (src/mod_scalar3.f90:165-211): NRCHOP += iof. */
+int mlg_chopset(mlg_ctx *ctx, int iof);
+int mlg_nrchop(const mlg_ctx *ctx);
+
+/* ------------------------------------------------------------------ */
+/* Fields: TYPE(SCALAR) (src/mod_scalar3.f90:41-47) with ALLOCATE/DEALLOCATE
+ * (src/mod_fft.f90:603-704).  A field owns device storage for the largest
+ * space plus its device log-term coefficient. */
+int mlg_field_create(mlg_ctx *ctx, int space, mlg_field **f);
+int mlg_field_destroy(mlg_field *f);
+int mlg_field_space(const mlg_field *f);
+/* Host <-> device copies in the reference layout of `space` (synchronising). */
+int mlg_field_upload(mlg_field *f, const double *e, int space, double ln);
+int mlg_field_download(const mlg_field *f, double *e, int *space, double *ln);
+/* Raw device pointers (element storage and the LN double) for zero-copy use
+ * from PyTorch or other device code on the same stream. */
+int mlg_field_device_ptr(const mlg_field *f, void **e, void **ln);
+/* Set the space tag and zero the field (e.g. ALLOCATE + A%E = 0). */
+int mlg_field_zero(mlg_field *f, int space);
+/* B = A (COPY0, src/mod_fft.f90:706-769). */
+int mlg_copy(mlg_field *b, const mlg_field *a);
+/* CHOPDO (src/mod_scalar3.f90:213-257). */
+int mlg_chopdo(mlg_field *a);
+
+/* ------------------------------------------------------------------ */
+/* Transforms: is = -1 physical -> spectral, +1 spectral -> physical.    */
+int mlg_horfft(mlg_field *a, int is);   /* HORFFT  src/mod_fft.f90:771-926   */
+int mlg_verfft(mlg_field *a, int is);   /* VERFFT  src/mod_fft.f90:928-1113  */
+int mlg_rtran(mlg_field *a, int is);    /* RTRAN   src/mod_fft.f90:1150-1340 */
+int mlg_toff(mlg_field *a);             /* TOFF    src/mod_fft.f90:1342-1369 */
+int mlg_tofp(mlg_field *a);             /* TOFP    src/mod_fft.f90:1371-1398 */
+int mlg_plnfft(mlg_field *a, int is);   /* PLNFFT  src/mod_fft.f90:1115-1148 */
+
+/* ------------------------------------------------------------------ */
+/* Spectral operators (src/mod_legops.f90).  Argument order follows the
+ * reference calls: (input, output). */
+int mlg_delsqh(const mlg_field *a, mlg_field *b);              /* DELSQH  :76-133    */
+int mlg_idelsqh(const mlg_field *b, mlg_field *a);             /* IDELSQH :136-239   */
+int mlg_mulxm(const mlg_field *a, mlg_field *b);               /* MULXM   :371-428   */
+int mlg_xxdx(const mlg_field *a, mlg_field *b);                /* XXDX    :680-748   */
+int mlg_del2(const mlg_field *a, mlg_field *b);                /* DEL2    :751-820   */
+int mlg_idel2(const mlg_field *b, mlg_field *a, int has_ln, double ln); /* IDEL2 :823-1010 */
+int mlg_idel2ln(const mlg_field *b, mlg_field *a);             /* IDEL2LN :1013-1176 */
+int mlg_ihelm(const mlg_field *b, mlg_field *a, double alp);   /* IHELM   :1204-1354 */
+int mlg_grad(const mlg_field *a, mlg_field *rr, mlg_field *rp, mlg_field *z); /* GRAD :1727-1786 */
+int mlg_pc2vel(const mlg_field *psi, const mlg_field *chi,
+               mlg_field *rur, mlg_field *rup, mlg_field *uz); /* PC2VEL  :1789-1853 */
+int mlg_pc2vor(const mlg_field *psi, const mlg_field *chi,
+               mlg_field *ror, mlg_field *rop, mlg_field *oz); /* PC2VOR  :1856-1919 */
+/* VPRODSUB (:1972-2133): (ror,rop,oz) <- u x omega in PPP, in place. */
+int mlg_vprodsub(mlg_field *rur, mlg_field *rup, mlg_field *uz,
+                 mlg_field *ror, mlg_field *rop, mlg_field *oz);
+/* VDOTPSUB (:2136-2232): out <- (a . b) with the r^-2 horizontal weight. */
+int mlg_vdotpsub(const mlg_field *rar, const mlg_field *rap, const mlg_field *az,
+                 const mlg_field *rbr, const mlg_field *rbp, const mlg_field *bz, mlg_field *out);
+int mlg_vel2pln(mlg_field *rup, mlg_field *psi, int has_ln, double ln); /* VEL2PLN :2277-2339 */
+/* PROJECT (:2397-2545): (r u_r, r u_theta, u_z) in PPP/PFF/FFF -> (psi, del^2 chi). */
+int mlg_project(mlg_field *rur, mlg_field *rup, mlg_field *uz,
+                mlg_field *psi, mlg_field *chi, int has_ln, double ln);
+
+/* ------------------------------------------------------------------ */
+/* Boussinesq terms (src/mod_boussinesq.f90). */
+/* CALC_BOUSSI_FORCE :317-410 -> (psin, chin, bn) in FFF. */
+int mlg_force(const mlg_field *psi, const mlg_field *chi, const mlg_field *b,
+              mlg_field *psin, mlg_field *chin, mlg_field *bn);
+/* CALC_BOUSSI_NONLIN :235-314; on exit ror/rop/oz hold (rN_r, rN_theta, N_z), bn -u.grad b, all FFF. */
+int mlg_nonlin(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_field *uz,
+               mlg_field *bn, mlg_field *ror, mlg_field *rop, mlg_field *oz);
+/* CALC_BOUSSI_ENERGY :138-232 (file output omitted): ke/pe/dke, each
+ * NTCHOP x NXCHOP column-major (synchronising). */
+int mlg_energy(const mlg_field *psi, const mlg_field *chi, const mlg_field *b,
+               double *ke, double *pe, double *dke);
+/* CALC_BOUSSI_ETD_OP (src/mod_march.f90:906-945): real parts of E and Phi, 4x4 column-major. */
+int mlg_etd_op(const mlg_ctx *ctx, double dt, double *e16, double *nl16);
+
+/* ------------------------------------------------------------------ */
+/* Time march: SOLVER_T (src/mod_march.f90:17-39). */
+int mlg_solver_create(mlg_ctx *ctx, mlg_solver **s);
+int mlg_solver_destroy(mlg_solver *s);
+/* INITIALIZE (:171-240): Richardson start (AB: RICH_BOUSSI_FE_BE :535-606;
+ * ETD: RICH_BOUSSI_ETDFE_BE :756-808); advances one step. */
+int mlg_initialize(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b);
+/* TIME_STEPPING (:243-307): one STEP_BOUSSI_AB_CN (:609-661) or
+ * STEP_BOUSSI_ETDAB_CN (:811-901) plus the per-step DIAGNOST checks
+ * (:1726-1791).  Synchronises to return the check result. */
+int mlg_time_step(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b);
+/* nsteps TIME_STEPPING calls queued back to back; the checks run on the
+ * device every step and the first failure is returned at the end. */
+int mlg_time_steps(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, int nsteps);
+double mlg_solver_time(const mlg_solver *s);
+int mlg_solver_nstep(const mlg_solver *s);
+
+/* ------------------------------------------------------------------ */
+/* Initial conditions: QVORTEX (src/mod_layout.f90:449-610) for one centred
+ * Gaussian vortex of strength q (omega_z = 2 q exp(-r^2)), chi = 0. */
+int mlg_qvortex(mlg_ctx *ctx, double q, mlg_field *psi, mlg_field *chi);
+
+/* ------------------------------------------------------------------ */
+/* Kernel event timer.  Instrumentation with no counterpart in the
+ * reference (which times whole steps with CPU_TIME).  mlg_timer_enable(1)
+ * clears and starts recording HIP events around every launch of the timed
+ * kernel classes on the context's stream; (0) stops.  mlg_timer_read
+ * synchronises and returns, for one class ("rtran_bwd", "rtran_fwd",
+ * "project", "verfft_fwd", "verfft_bwd", "horfft_fwd", "horfft_bwd",
+ * "products", "band_solve", "etd"), the summed event time in ms, the launch
+ * count and the summed algorithmic work (flops for the Legendre classes,
+ * HBM bytes read+written once for the others; *unit_is_bytes tells which). */
+int mlg_timer_enable(mlg_ctx *ctx, int on);
+int mlg_timer_read(mlg_ctx *ctx, const char *kernel, double *ms_total, long *launches,
+                   double *work_total, int *unit_is_bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -59,7 +59,7 @@ def _mix64(x):
     return x ^ (x >> np.uint64(31))
 
 
-def seeded_noise(cfg, field_id, amplitude=None, seed=NOISE_SEED, m_only=None):
+def seeded_noise(cfg, field_id, amplitude=None, seed=NOISE_SEED, m_only=None, nrchop=None):
     """Complex Gaussian noise on every kept (n,m,k) of an FFF field.
 
     Value at (n, m, k) depends only on (seed, field_id, n, m, k): a
@@ -67,6 +67,8 @@ def seeded_noise(cfg, field_id, amplitude=None, seed=NOISE_SEED, m_only=None):
     index within azimuthal mode m (degree m+n).  Amplitude envelope
     A*exp(-(n/16)^2-(m/4)^2-(k/4)^2).  m=0 is made Hermitian in k and
     c(n,0,0) real; entries beyond the triangular truncation are zero.
+    ``nrchop`` overrides the truncation (not the array shape), e.g. to fill
+    the NRCHOP+3 rows used inside the nonlinear term.
     """
     d = dims(cfg)
     amp = cfg.get("noise", 1e-4) if amplitude is None else amplitude
@@ -89,7 +91,7 @@ def seeded_noise(cfg, field_id, amplitude=None, seed=NOISE_SEED, m_only=None):
     env = amp * np.exp(-(nf / 16.0) ** 2 - (mf / 4.0) ** 2 - (kf / 4.0) ** 2)
     a = z * env
     # triangular truncation NRCHOPS(m) = max(min(NRCHOP, NRCHOP - m), 0)
-    nrchop = cfg["nrchop"]
+    nrchop = cfg["nrchop"] if nrchop is None else nrchop
     for mm in range(ntd):
         nn = max(min(nrchop, nrchop - mm * cfg.get("minc", 1)), 0)
         a[nn:, mm, :] = 0

@@ -1,0 +1,1257 @@
+// C ABI of libmlegs_hip.so: context, fields and the host-side control flow of
+// the Boussinesq step.  Kernels live in k_*.hip; this file sequences them on
+// the context's stream, mirroring the reference's call structure and CHOPSET
+// (truncation) state so results match the reference routine by routine.
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <cstring>
+
+#include "mlegs_internal.h"
+
+using namespace mlg;
+
+#define RET(x)                 \
+  do {                         \
+    int _r = (x);              \
+    if (_r) return _r;         \
+  } while (0)
+
+namespace mlg {
+KGrid kgrid(const mlg_ctx *c) {
+  const Grid &G = c->g;
+  KGrid k;
+  k.nr = G.nr;
+  k.nrh = G.nrh;
+  k.nth = G.nth;
+  k.nx = G.nx;
+  k.ntchop = G.ntchop;
+  k.nxchop = G.nxchop;
+  k.ndimr = G.ndimr;
+  k.ndimth = G.ndimth;
+  k.ndimx = G.ndimx;
+  k.nrchopdim = G.nrchopdim;
+  k.ntchopdim = G.ntchopdim;
+  k.nxchopdim = G.nxchopdim;
+  k.nxchoph = G.nxchoph;
+  k.nrchop = G.nrchop;
+  k.npf = G.npf;
+  k.nband = G.nband;
+  k.minc = G.p.minc;
+  k.ell2 = G.ell2;
+  k.norm0 = G.norm[0];
+  k.norm1 = G.norm[1];
+  k.norm2 = G.norm[2];
+  k.at1_0 = G.at1[0];
+  return k;
+}
+
+const char *const kernel_names[K_COUNT] = {"rtran_bwd", "rtran_fwd", "project", "verfft_fwd", "verfft_bwd", "horfft_fwd",
+                                           "horfft_bwd", "products", "band_solve", "etd", "spectral"};
+const char *const kernel_units[K_COUNT] = {"flop", "flop", "flop", "byte", "byte", "byte",
+                                           "byte", "byte", "byte", "byte", "byte"};
+
+static hipEvent_t take_event(mlg_ctx *c) {
+  hipEvent_t e = nullptr;
+  if (!c->timer.spare.empty()) {
+    e = c->timer.spare.back();
+    c->timer.spare.pop_back();
+  } else if (hipEventCreate(&e) != hipSuccess) {
+    e = nullptr;
+  }
+  return e;
+}
+
+KScope::KScope(mlg_ctx *c_, int id_, double work_) : c(c_), id(id_), work(work_) {
+  if (!c->timer.on) return;
+  a = take_event(c);
+  b = take_event(c);
+  if (a) (void)hipEventRecord(a, c->stream);
+}
+KScope::~KScope() {
+  if (!a || !b) return;
+  (void)hipEventRecord(b, c->stream);
+  c->timer.rec[id].push_back(EventPair{a, b, work});
+}
+
+long sum_nrchops(const mlg_ctx *c) {
+  long s = 0;
+  for (int mm = 0; mm < c->g.ntchopdim; mm++) s += nrchops_of(c->g.nrchop, mm * c->g.p.minc);
+  return s;
+}
+}  // namespace mlg
+
+static void timer_clear(mlg_ctx *c) {
+  for (auto &v : c->timer.rec) {
+    for (auto &p : v) {
+      c->timer.spare.push_back(p.a);
+      c->timer.spare.push_back(p.b);
+    }
+    v.clear();
+  }
+}
+
+extern "C" int mlg_timer_enable(mlg_ctx *c, int on) {
+  if (!c) return MLG_E_ARG;
+  MLG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  if (on) timer_clear(c);
+  c->timer.on = on != 0;
+  return 0;
+}
+
+extern "C" int mlg_timer_read(mlg_ctx *c, const char *kernel, double *ms_total, long *launches, double *work_total,
+                              int *unit_is_bytes) {
+  if (!c || !kernel) return MLG_E_ARG;
+  int id = -1;
+  for (int k = 0; k < K_COUNT; k++)
+    if (!std::strcmp(kernel, kernel_names[k])) id = k;
+  if (id < 0) return MLG_E_ARG;
+  MLG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  double ms = 0.0, work = 0.0;
+  for (auto &p : c->timer.rec[id]) {
+    float t = 0.0f;
+    MLG_CHECK_HIP(hipEventElapsedTime(&t, p.a, p.b));
+    ms += t;
+    work += p.work;
+  }
+  if (ms_total) *ms_total = ms;
+  if (launches) *launches = (long)c->timer.rec[id].size();
+  if (work_total) *work_total = work;
+  if (unit_is_bytes) *unit_is_bytes = std::strcmp(kernel_units[id], "byte") == 0;
+  return 0;
+}
+
+static long space_count(const mlg_ctx *c, int space) {
+  if (space == MLG_FFF) return c->n_fff;
+  if (space == MLG_PFF) return c->n_pff;
+  return c->n_ppp;
+}
+
+template <class T>
+static int upload(T **dst, const std::vector<T> &src) {
+  if (src.empty()) {
+    *dst = nullptr;
+    return 0;
+  }
+  MLG_CHECK_HIP(hipMalloc((void **)dst, sizeof(T) * src.size()));
+  MLG_CHECK_HIP(hipMemcpy(*dst, src.data(), sizeof(T) * src.size(), hipMemcpyHostToDevice));
+  return 0;
+}
+
+static bool pow2(int n) { return n >= 1 && (n & (n - 1)) == 0; }
+
+extern "C" const char *mlg_error_string(int code) {
+  switch (code) {
+    case 0: return "ok";
+    case 1: return "SCALAR3";
+    case 10: return "SCALAR3_ISNAN";
+    case 11: return "SCALAR3_DEALIASING";
+    case 12: return "SCALAR3_DIVERGENCE";
+    case 13: return "SCALAR3_LOGTERM";
+    case 2: return "TRANSFORM";
+    case 3: return "LEGOPERATOR";
+    case 30: return "DEPRECATED";
+    case 31: return "CALCULATION";
+    case 32: return "DIMMISMATCH";
+    case 4: return "MPIERROR";
+    case 5: return "SOLVER";
+    case MLG_E_ARG: return "invalid argument or unsupported configuration";
+    case MLG_E_HIP: return "HIP runtime error";
+    case MLG_E_NOMEM: return "device allocation failed";
+    case MLG_E_SPACE: return "field is in the wrong space for this call";
+    default: return "unknown";
+  }
+}
+
+// ------------------------------------------------------------------ context
+extern "C" int mlg_init(const mlg_params *p, mlg_ctx **out) {
+  if (!p || !out) return MLG_E_ARG;
+  *out = nullptr;
+  // supported: even NR, power-of-two NTH and NX (radix-2 kernels), the
+  // reference's chop limits (mod_fft.f90:238-244, mod_scalar3.f90:183-190)
+  if (p->nr < 4 || p->nr % 2 || !pow2(p->nth) || p->nth < 2 || !pow2(p->nx) || p->nx < 2) return MLG_E_ARG;
+  if (2 * p->nxchop > p->nx || p->nxchop < 1 || p->ntchop < 1 || p->nrchop < 1) return MLG_E_ARG;
+  if (p->nth > 4096 || p->nx > 4096) return MLG_E_ARG;
+  if (p->minc != 1 && p->minc != 0) return MLG_E_ARG;  // the path uses MINC = 1
+  if (p->mklink != 0) return MLG_E_ARG;
+  mlg_ctx *c = new mlg_ctx();
+  int rc = build_grid(*p, c->g);
+  if (rc) {
+    delete c;
+    return rc;
+  }
+  Grid &G = c->g;
+  (void)hipGetDevice(&c->device);
+  c->n_fff = (long)G.nrchopdim * G.ntchopdim * G.nxchopdim;
+  c->n_pff = (long)G.ndimr * G.ntchopdim * G.nxchopdim;
+  c->n_ppp = (long)G.ndimr * G.ndimth * G.ndimx;
+  c->n_max = std::max(c->n_fff, std::max(c->n_pff, c->n_ppp));
+  DevTables &d = c->d;
+  RET(upload(&d.pf, G.pf));
+  RET(upload(&d.vdt, G.vdt));
+  RET(upload(&d.w, G.w));
+  RET(upload(&d.r, G.r));
+  RET(upload(&d.x, G.x));
+  RET(upload(&d.lnx, G.lnx));
+  RET(upload(&d.at1, G.at1));
+  RET(upload(&d.ak, G.ak));
+  RET(upload(&d.norm, G.norm));
+  RET(upload(&d.lognorm, G.lognorm));
+  RET(upload(&d.bxm, G.bxm));
+  RET(upload(&d.bxxdx, G.bxxdx));
+  RET(upload(&d.bdel2h, G.bdel2h));
+  // r^2 for the products (the kernels divide by r_i^2)
+  {
+    std::vector<double> r2(G.nr);
+    for (int i = 0; i < G.nr; i++) r2[i] = G.r[i] * G.r[i];
+    (void)hipFree(d.r);
+    RET(upload(&d.r, r2));
+  }
+  auto tw = [&](int n, int sign, double2 **dst, int count) -> int {
+    std::vector<double> re(n), im(n);
+    twiddles(n, sign, re.data(), im.data());
+    std::vector<double2> v(count);
+    for (int j = 0; j < count; j++) v[j] = make_double2(re[j % n], im[j % n]);
+    return upload(dst, v);
+  };
+  RET(tw(G.nx, -1, &d.tw_z, G.nx));
+  RET(tw(G.nth, -1, &d.tw_t, G.nth));
+  {
+    std::vector<double> re(2 * G.nth), im(2 * G.nth);
+    twiddles(2 * G.nth, -1, re.data(), im.data());
+    std::vector<double2> v(G.nth + 1);
+    for (int j = 0; j <= G.nth; j++) v[j] = make_double2(re[j], im[j]);
+    RET(upload(&d.tw_h, v));
+  }
+  if (!G.hv_r.empty()) {
+    std::vector<double> hv;
+    hv.insert(hv.end(), G.hv_r.begin(), G.hv_r.end());
+    hv.insert(hv.end(), G.hv_t.begin(), G.hv_t.end());
+    hv.insert(hv.end(), G.hv_x.begin(), G.hv_x.end());
+    hv.insert(hv.end(), G.hd_r.begin(), G.hd_r.end());
+    hv.insert(hv.end(), G.hd_t.begin(), G.hd_t.end());
+    hv.insert(hv.end(), G.hd_x.begin(), G.hd_x.end());
+    RET(upload(&d.hv, hv));
+  }
+  d.lu_len = (size_t)(G.nrchopdim + 2) * 3 * (size_t)G.ntchopdim * G.nxchopdim;
+  MLG_CHECK_HIP(hipMalloc((void **)&d.lu, sizeof(double) * d.lu_len));
+  MLG_CHECK_HIP(hipMalloc((void **)&d.red, sizeof(double) * (4 * (size_t)G.ntchop * G.nxchopdim + 2)));
+  MLG_CHECK_HIP(hipMalloc((void **)&d.err, sizeof(int)));
+  MLG_CHECK_HIP(hipMemset(d.err, 0, sizeof(int)));
+  MLG_CHECK_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
+  c->stream = c->own_stream;
+  *out = c;
+  return 0;
+}
+
+static void free_field_storage(mlg_field *f) {
+  if (f->e) (void)hipFree(f->e);
+  if (f->ln) (void)hipFree(f->ln);
+  f->e = nullptr;
+  f->ln = nullptr;
+}
+
+extern "C" int mlg_finalize(mlg_ctx *c) {
+  if (!c) return MLG_E_ARG;
+  (void)hipStreamSynchronize(c->stream);
+  for (mlg_field *f : c->pool) {
+    free_field_storage(f);
+    delete f;
+  }
+  c->pool.clear();
+  for (mlg_field *f : c->owned) {
+    free_field_storage(f);
+    delete f;
+  }
+  c->owned.clear();
+  timer_clear(c);
+  for (hipEvent_t e : c->timer.spare) (void)hipEventDestroy(e);
+  c->timer.spare.clear();
+  DevTables &d = c->d;
+  void *ptrs[] = {d.pf, d.vdt, d.w, d.r, d.x, d.lnx, d.at1, d.ak, d.norm, d.lognorm, d.bxm, d.bxxdx, d.bdel2h,
+                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.lu, d.red, d.err};
+  for (void *p : ptrs)
+    if (p) (void)hipFree(p);
+  if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
+  delete c;
+  return 0;
+}
+
+extern "C" int mlg_dims(const mlg_ctx *c, int *o) {
+  if (!c || !o) return MLG_E_ARG;
+  const Grid &G = c->g;
+  int v[12] = {G.ndimr, G.ndimth, G.ndimx, G.nrchopdim, G.ntchopdim, G.nxchopdim, G.nrh, G.nxchoph, G.nrchop, G.npf, G.nnorm, G.nr};
+  std::memcpy(o, v, sizeof(v));
+  return 0;
+}
+
+extern "C" int mlg_get_table(const mlg_ctx *c, const char *name, double *out, long n, long *len) {
+  if (!c || !name) return MLG_E_ARG;
+  const Grid &G = c->g;
+  const std::vector<double> *v = nullptr;
+  std::string s(name);
+  if (s == "x") v = &G.x;
+  else if (s == "w") v = &G.w;
+  else if (s == "r") v = &G.r;
+  else if (s == "lnx") v = &G.lnx;
+  else if (s == "pf") v = &G.pf;
+  else if (s == "lognorm") v = &G.lognorm;
+  else if (s == "norm") v = &G.norm;
+  else if (s == "at0") v = &G.at0;
+  else if (s == "at1") v = &G.at1;
+  else if (s == "ak") v = &G.ak;
+  else if (s == "vdt") v = &G.vdt;
+  else return MLG_E_ARG;
+  if (len) *len = (long)v->size();
+  if (out) {
+    if (n < (long)v->size()) return MLG_E_ARG;
+    std::memcpy(out, v->data(), sizeof(double) * v->size());
+  }
+  return 0;
+}
+
+extern "C" int mlg_set_stream(mlg_ctx *c, void *s) {
+  if (!c) return MLG_E_ARG;
+  c->stream = s ? (hipStream_t)s : c->own_stream;
+  return 0;
+}
+
+extern "C" int mlg_synchronize(mlg_ctx *c) {
+  if (!c) return MLG_E_ARG;
+  MLG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  int e = 0;
+  MLG_CHECK_HIP(hipMemcpy(&e, c->d.err, sizeof(int), hipMemcpyDeviceToHost));
+  if (e) MLG_CHECK_HIP(hipMemset(c->d.err, 0, sizeof(int)));
+  return e;
+}
+
+extern "C" int mlg_chopset(mlg_ctx *c, int iof) {
+  if (!c) return MLG_E_ARG;
+  int v = c->g.nrchop + iof;
+  if (v > c->g.nrchopdim || v < 0) return 1;  // CHOPSET: NRCHOP TOO LARGE (mod_scalar3.f90:183-190)
+  c->g.nrchop = v;
+  return 0;
+}
+extern "C" int mlg_nrchop(const mlg_ctx *c) { return c ? c->g.nrchop : -1; }
+
+// ------------------------------------------------------------------ fields
+static int new_field(mlg_ctx *c, int space, mlg_field **out) {
+  mlg_field *f = new mlg_field();
+  f->ctx = c;
+  f->space = space;
+  f->pooled = false;
+  if (hipMalloc((void **)&f->e, sizeof(double2) * (size_t)c->n_max) != hipSuccess) {
+    delete f;
+    return MLG_E_NOMEM;
+  }
+  if (hipMalloc((void **)&f->ln, sizeof(double) * 2) != hipSuccess) {
+    (void)hipFree(f->e);
+    delete f;
+    return MLG_E_NOMEM;
+  }
+  MLG_CHECK_HIP(hipMemsetAsync(f->e, 0, sizeof(double2) * (size_t)c->n_max, c->stream));
+  MLG_CHECK_HIP(hipMemsetAsync(f->ln, 0, sizeof(double) * 2, c->stream));
+  *out = f;
+  return 0;
+}
+
+extern "C" int mlg_field_create(mlg_ctx *c, int space, mlg_field **f) {
+  if (!c || !f || space < 0 || space > 3) return MLG_E_ARG;
+  RET(new_field(c, space, f));
+  c->owned.insert(*f);
+  return 0;
+}
+extern "C" int mlg_field_destroy(mlg_field *f) {
+  if (!f) return MLG_E_ARG;
+  mlg_ctx *c = f->ctx;
+  if (!c->owned.erase(f)) return MLG_E_ARG;
+  (void)hipStreamSynchronize(c->stream);
+  free_field_storage(f);
+  delete f;
+  return 0;
+}
+extern "C" int mlg_field_space(const mlg_field *f) { return f ? f->space : -1; }
+
+// pooled temporaries (ALLOCATE of a work SCALAR); LN starts at 0
+static mlg_field *tmp(mlg_ctx *c, int space) {
+  mlg_field *f = nullptr;
+  if (!c->pool.empty()) {
+    f = c->pool.back();
+    c->pool.pop_back();
+  } else if (new_field(c, space, &f)) {
+    return nullptr;
+  }
+  f->space = space;
+  f->pooled = true;
+  launch_set_ln(c, f->ln, nullptr, 0.0, 0.0);
+  return f;
+}
+static void rel(mlg_ctx *c, mlg_field *f) {
+  if (f) c->pool.push_back(f);
+}
+struct Temps {
+  mlg_ctx *c;
+  std::vector<mlg_field *> v;
+  explicit Temps(mlg_ctx *cc) : c(cc) {}
+  mlg_field *get(int space = MLG_FFF) {
+    mlg_field *f = tmp(c, space);
+    if (f) v.push_back(f);
+    return f;
+  }
+  bool ok() const {
+    for (auto *f : v)
+      if (!f) return false;
+    return true;
+  }
+  ~Temps() {
+    for (auto *f : v) rel(c, f);
+  }
+};
+
+static void swap_storage(mlg_field *a, mlg_field *b) { std::swap(a->e, b->e); }
+
+extern "C" int mlg_field_upload(mlg_field *f, const double *e, int space, double ln) {
+  if (!f || !e || space < 0 || space > 3) return MLG_E_ARG;
+  mlg_ctx *c = f->ctx;
+  long n = space_count(c, space);
+  MLG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  MLG_CHECK_HIP(hipMemcpy(f->e, e, sizeof(double2) * (size_t)n, hipMemcpyHostToDevice));
+  MLG_CHECK_HIP(hipMemcpy(f->ln, &ln, sizeof(double), hipMemcpyHostToDevice));
+  f->space = space;
+  return 0;
+}
+
+extern "C" int mlg_field_download(const mlg_field *f, double *e, int *space, double *ln) {
+  if (!f) return MLG_E_ARG;
+  mlg_ctx *c = f->ctx;
+  MLG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  long n = space_count(c, f->space);
+  if (e) {
+    MLG_CHECK_HIP(hipMemcpy(e, f->e, sizeof(double2) * (size_t)n, hipMemcpyDeviceToHost));
+    if (f->space == MLG_PFP) {  // slots m >= NTCHOPDIM of a PFP field are zero
+      const Grid &G = c->g;
+      double2 *h = (double2 *)e;
+      for (int z = 0; z < G.ndimx; z++)
+        for (int j = G.ntchopdim; j < G.ndimth; j++)
+          for (int r = 0; r < G.ndimr; r++) h[(size_t)r + (size_t)G.ndimr * ((size_t)j + (size_t)G.ndimth * z)] = make_double2(0, 0);
+    }
+  }
+  if (space) *space = f->space;
+  if (ln) MLG_CHECK_HIP(hipMemcpy(ln, f->ln, sizeof(double), hipMemcpyDeviceToHost));
+  return 0;
+}
+
+extern "C" int mlg_field_device_ptr(const mlg_field *f, void **e, void **ln) {
+  if (!f) return MLG_E_ARG;
+  if (e) *e = (void *)f->e;
+  if (ln) *ln = (void *)f->ln;
+  return 0;
+}
+
+extern "C" int mlg_field_zero(mlg_field *f, int space) {
+  if (!f || space < 0 || space > 3) return MLG_E_ARG;
+  f->space = space;
+  RET(launch_zero(f->ctx, f->e, f->ctx->n_max));
+  return launch_set_ln(f->ctx, f->ln, nullptr, 0.0, 0.0);
+}
+
+// COPY0 (mod_fft.f90:706-769): copy, LN included; FFF copies are chopped.
+static int copy0(mlg_field *b, const mlg_field *a) {
+  mlg_ctx *c = a->ctx;
+  if (a == b) return 0;
+  RET(launch_copy(c, b->e, a->e, space_count(c, a->space)));
+  RET(launch_set_ln(c, b->ln, a->ln, 1.0, 0.0));
+  b->space = a->space;
+  if (a->space == MLG_FFF) RET(launch_chopdo(c, b->e, MLG_FFF));
+  return 0;
+}
+extern "C" int mlg_copy(mlg_field *b, const mlg_field *a) {
+  if (!a || !b) return MLG_E_ARG;
+  return copy0(b, a);
+}
+extern "C" int mlg_chopdo(mlg_field *a) {
+  if (!a) return MLG_E_ARG;
+  return launch_chopdo(a->ctx, a->e, a->space);
+}
+
+// ------------------------------------------------------------------ transforms
+extern "C" int mlg_rtran(mlg_field *a, int is) {
+  if (!a) return MLG_E_ARG;
+  mlg_ctx *c = a->ctx;
+  Temps T(c);
+  mlg_field *t = T.get();
+  if (!T.ok()) return MLG_E_NOMEM;
+  if (is == -1) {
+    if (a->space != MLG_PFF) return 2;
+    RET(launch_rtran_fwd(c, a->e, t->e));
+    swap_storage(a, t);
+    a->space = MLG_FFF;
+    RET(launch_chopdo(c, a->e, MLG_FFF));
+  } else {
+    if (a->space != MLG_FFF) return 2;
+    RET(launch_rtran_bwd(c, a->e, a->ln, t->e));
+    swap_storage(a, t);
+    a->space = MLG_PFF;
+    RET(launch_chopdo(c, a->e, MLG_PFF));
+  }
+  return launch_set_ln(c, a->ln, nullptr, 0.0, 0.0);
+}
+
+extern "C" int mlg_verfft(mlg_field *a, int is) {
+  if (!a) return MLG_E_ARG;
+  mlg_ctx *c = a->ctx;
+  Temps T(c);
+  mlg_field *t = T.get();
+  if (!T.ok()) return MLG_E_NOMEM;
+  if (is == -1) {
+    if (a->space != MLG_PFP) return 2;
+    RET(launch_verfft_fwd(c, a->e, t->e));
+    swap_storage(a, t);
+    a->space = MLG_PFF;
+  } else {
+    if (a->space != MLG_PFF) return 2;
+    RET(launch_verfft_bwd(c, a->e, t->e));
+    swap_storage(a, t);
+    a->space = MLG_PFP;
+  }
+  return 0;
+}
+
+extern "C" int mlg_horfft(mlg_field *a, int is) {
+  if (!a) return MLG_E_ARG;
+  mlg_ctx *c = a->ctx;
+  if (is == -1) {
+    if (a->space != MLG_PPP) return 2;
+    RET(launch_horfft_fwd(c, a->e));
+    a->space = MLG_PFP;
+  } else {
+    if (a->space != MLG_PFP) return 2;
+    RET(launch_horfft_bwd(c, a->e));
+    a->space = MLG_PPP;
+  }
+  return 0;
+}
+
+extern "C" int mlg_toff(mlg_field *a) {
+  if (!a) return MLG_E_ARG;
+  if (a->space == MLG_PPP) RET(mlg_horfft(a, -1));
+  if (a->space == MLG_PFP) RET(mlg_verfft(a, -1));
+  if (a->space == MLG_PFF) RET(mlg_rtran(a, -1));
+  return 0;
+}
+extern "C" int mlg_tofp(mlg_field *a) {
+  if (!a) return MLG_E_ARG;
+  if (a->space == MLG_FFF) RET(mlg_rtran(a, 1));
+  if (a->space == MLG_PFF) RET(mlg_verfft(a, 1));
+  if (a->space == MLG_PFP) RET(mlg_horfft(a, 1));
+  return 0;
+}
+extern "C" int mlg_plnfft(mlg_field *a, int is) {
+  if (!a) return MLG_E_ARG;
+  if (is == -1) {
+    if (a->space != MLG_PPP) return 2;
+    RET(mlg_horfft(a, -1));
+    return mlg_verfft(a, -1);
+  }
+  if (a->space != MLG_PFF) return 2;
+  RET(mlg_verfft(a, 1));
+  return mlg_horfft(a, 1);
+}
+
+// ------------------------------------------------------------------ operators
+// in-place calls (input == output) go through a temporary copy
+#define ALIAS_GUARD(inp, outp)                        \
+  Temps _T((inp)->ctx);                               \
+  const mlg_field *inp##_src = (inp);                 \
+  if ((const void *)(inp) == (const void *)(outp)) {  \
+    mlg_field *_t = _T.get();                         \
+    if (!_t) return MLG_E_NOMEM;                      \
+    RET(copy0(_t, (inp)));                            \
+    inp##_src = _t;                                   \
+  }
+
+extern "C" int mlg_delsqh(const mlg_field *a, mlg_field *b) {
+  if (!a || !b) return MLG_E_ARG;
+  if (a->space != MLG_FFF) return 3;
+  ALIAS_GUARD(a, b);
+  b->space = MLG_FFF;
+  return launch_delsqh(a->ctx, a_src->e, a_src->ln, b->e, b->ln);
+}
+
+extern "C" int mlg_idelsqh(const mlg_field *b, mlg_field *a) {
+  if (!a || !b) return MLG_E_ARG;
+  if (b->space != MLG_FFF) return 3;
+  ALIAS_GUARD(b, a);
+  a->space = MLG_FFF;
+  RET(launch_idelsqh(a->ctx, b_src->e, a->e, a->ln));
+  return launch_fix_at_infinity(a->ctx, a->e);
+}
+
+extern "C" int mlg_mulxm(const mlg_field *a, mlg_field *b) {
+  if (!a || !b) return MLG_E_ARG;
+  if (a->space != MLG_FFF) return 3;
+  ALIAS_GUARD(a, b);
+  b->space = MLG_FFF;
+  return launch_band3(a->ctx, 0, a_src->e, a_src->ln, b->e, b->ln);
+}
+
+extern "C" int mlg_xxdx(const mlg_field *a, mlg_field *b) {
+  if (!a || !b) return MLG_E_ARG;
+  if (a->space != MLG_FFF) return 3;
+  ALIAS_GUARD(a, b);
+  b->space = MLG_FFF;
+  return launch_band3(a->ctx, 1, a_src->e, a_src->ln, b->e, b->ln);
+}
+
+extern "C" int mlg_del2(const mlg_field *a, mlg_field *b) {
+  if (!a || !b) return MLG_E_ARG;
+  if (a->space != MLG_FFF) return 3;
+  ALIAS_GUARD(a, b);
+  b->space = MLG_FFF;
+  RET(launch_check_top_m1(a->ctx, a_src->e));
+  return launch_del2(a->ctx, a_src->e, a_src->ln, b->e, b->ln);
+}
+
+static int solve_op(const mlg_field *b, mlg_field *a, int op, double alp, int has_ln, double ln) {
+  if (!a || !b) return MLG_E_ARG;
+  if (b->space != MLG_FFF) return 3;
+  mlg_ctx *c = b->ctx;
+  Temps T(c);
+  const mlg_field *bs = b;
+  if (a == b) {
+    mlg_field *t = T.get();
+    if (!t) return MLG_E_NOMEM;
+    RET(copy0(t, b));
+    bs = t;
+  }
+  RET(copy0(a, bs));  // A = B
+  if (op != 2) RET(launch_set_ln(c, a->ln, nullptr, 0.0, 0.0));
+  RET(launch_band_solve(c, op, a->e, a->ln, bs->ln, alp, has_ln, ln));
+  RET(launch_chopdo(c, a->e, MLG_FFF));
+  if (op != 2) RET(launch_fix_at_infinity(c, a->e));
+  return 0;
+}
+extern "C" int mlg_idel2(const mlg_field *b, mlg_field *a, int has_ln, double ln) { return solve_op(b, a, 0, 0.0, has_ln, ln); }
+extern "C" int mlg_idel2ln(const mlg_field *b, mlg_field *a) { return solve_op(b, a, 1, 0.0, 0, 0.0); }
+extern "C" int mlg_ihelm(const mlg_field *b, mlg_field *a, double alp) {
+  if (alp == 0.0) return 3;  // IHELM: ALP CAN'T BE ZERO (mod_legops.f90:1243-1246)
+  return solve_op(b, a, 2, alp, 0, 0.0);
+}
+
+extern "C" int mlg_grad(const mlg_field *a, mlg_field *rr, mlg_field *rp, mlg_field *z) {
+  if (!a || !rr || !rp || !z) return MLG_E_ARG;
+  if (a->space != MLG_FFF) return 3;
+  Temps T(a->ctx);
+  const mlg_field *as = a;
+  if (a == rr || a == rp || a == z) {
+    mlg_field *t = T.get();
+    if (!t) return MLG_E_NOMEM;
+    RET(copy0(t, a));
+    as = t;
+  }
+  rr->space = rp->space = z->space = MLG_FFF;
+  return launch_grad(a->ctx, as->e, as->ln, rr->e, rr->ln, rp->e, rp->ln, z->e, z->ln);
+}
+
+extern "C" int mlg_pc2vel(const mlg_field *psi, const mlg_field *chi, mlg_field *rur, mlg_field *rup, mlg_field *uz) {
+  if (!psi || !chi || !rur || !rup || !uz) return MLG_E_ARG;
+  if (psi->space != MLG_FFF || chi->space != MLG_FFF) return 3;
+  mlg_ctx *c = psi->ctx;
+  rur->space = rup->space = uz->space = MLG_FFF;
+  return launch_pc2vel(c, psi->e, psi->ln, chi->e, chi->ln, rur->e, rup->e, uz->e, rur->ln, rup->ln, uz->ln);
+}
+
+extern "C" int mlg_pc2vor(const mlg_field *psi, const mlg_field *chi, mlg_field *ror, mlg_field *rop, mlg_field *oz) {
+  if (!psi || !chi || !ror || !rop || !oz) return MLG_E_ARG;
+  if (psi->space != MLG_FFF || chi->space != MLG_FFF) return 3;
+  mlg_ctx *c = psi->ctx;
+  RET(launch_check_top_m1(c, chi->e));
+  ror->space = rop->space = oz->space = MLG_FFF;
+  return launch_pc2vor(c, psi->e, psi->ln, chi->e, chi->ln, ror->e, rop->e, oz->e, ror->ln, rop->ln, oz->ln);
+}
+
+extern "C" int mlg_vprodsub(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *ror, mlg_field *rop, mlg_field *oz) {
+  if (!rur || !rup || !uz || !ror || !rop || !oz) return MLG_E_ARG;
+  if (rur->space != MLG_PPP || ror->space != MLG_PPP) return MLG_E_SPACE;
+  return launch_vprod(rur->ctx, rur->e, rup->e, uz->e, ror->e, rop->e, oz->e);
+}
+
+extern "C" int mlg_vdotpsub(const mlg_field *rar, const mlg_field *rap, const mlg_field *az, const mlg_field *rbr,
+                            const mlg_field *rbp, const mlg_field *bz, mlg_field *out) {
+  if (!rar || !rap || !az || !rbr || !rbp || !bz || !out) return MLG_E_ARG;
+  if (rar->space != MLG_PPP || rbr->space != MLG_PPP) return MLG_E_SPACE;
+  mlg_ctx *c = rar->ctx;
+  out->space = MLG_PPP;
+  RET(launch_zero(c, out->e, c->n_ppp));
+  RET(launch_set_ln(c, out->ln, nullptr, 0.0, 0.0));
+  return launch_vdot(c, rar->e, rap->e, az->e, rbr->e, rbp->e, bz->e, out->e, 1.0);
+}
+
+extern "C" int mlg_vel2pln(mlg_field *rup, mlg_field *psi, int has_ln, double ln) {
+  if (!rup || !psi) return MLG_E_ARG;
+  if (rup->space != MLG_PFF) return 3;
+  psi->space = MLG_FFF;
+  return launch_vel2pln(rup->ctx, rup->e, psi->ln, has_ln ? 1 : 0, ln);
+}
+
+// PROJECT (src/mod_legops.f90:2397-2648).  ln_mode 0: no LN argument; 1: LN
+// given as a value; 2: LN given as psi's own LN (the ETD steps pass PSI%LN),
+// so the device copy is kept without a host read.
+static int project_impl(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *psi, mlg_field *chi, int ln_mode, double ln) {
+  if (!(rur->space == rup->space && rup->space == uz->space)) return 2;
+  mlg_ctx *c = rur->ctx;
+  if (rur->space == MLG_PPP) {
+    RET(mlg_plnfft(rur, -1));
+    RET(mlg_plnfft(rup, -1));
+    RET(mlg_plnfft(uz, -1));
+  } else if (rur->space == MLG_FFF) {
+    RET(mlg_rtran(rur, 1));
+    RET(mlg_rtran(rup, 1));
+    RET(mlg_rtran(uz, 1));
+  } else if (rur->space != MLG_PFF) {
+    return 2;
+  }
+  if (psi->space != MLG_FFF) {
+    psi->space = MLG_FFF;
+    RET(launch_set_ln(c, psi->ln, nullptr, 0.0, 0.0));
+  }
+  if (chi->space != MLG_FFF) {
+    chi->space = MLG_FFF;
+    RET(launch_set_ln(c, chi->ln, nullptr, 0.0, 0.0));
+  }
+  if (ln_mode) RET(launch_vel2pln(c, rup->e, psi->ln, ln_mode, ln));
+  RET(launch_project(c, rur->e, rup->e, uz->e, psi->e, chi->e));
+  RET(launch_chopdo(c, psi->e, MLG_FFF));
+  RET(launch_chopdo(c, chi->e, MLG_FFF));
+  return launch_fix_at_infinity(c, psi->e);
+}
+
+extern "C" int mlg_project(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *psi, mlg_field *chi, int has_ln, double ln) {
+  if (!rur || !rup || !uz || !psi || !chi) return MLG_E_ARG;
+  return project_impl(rur, rup, uz, psi, chi, has_ln ? 1 : 0, ln);
+}
+
+// ------------------------------------------------------------------ Boussinesq terms
+// CALC_BOUSSI_FORCE (src/mod_boussinesq.f90:317-410)
+extern "C" int mlg_force(const mlg_field *psi, const mlg_field *chi, const mlg_field *b, mlg_field *psin, mlg_field *chin,
+                         mlg_field *bn) {
+  if (!psi || !chi || !b || !psin || !chin || !bn) return MLG_E_ARG;
+  mlg_ctx *c = psi->ctx;
+  Temps T(c);
+  mlg_field *rur = T.get(), *rup = T.get(), *uz = T.get(), *ror = T.get(), *rop = T.get(), *oz = T.get();
+  mlg_field *rbr = T.get(), *rbp = T.get(), *bz = T.get(), *w = T.get(), *w2 = T.get();
+  if (!T.ok()) return MLG_E_NOMEM;
+  RET(mlg_chopset(c, 3));
+  int rc = mlg_grad(b, rbr, rbp, bz);
+  if (!rc) rc = mlg_pc2vel(psi, chi, rur, rup, uz);
+  if (!rc) rc = mlg_pc2vor(psi, chi, ror, rop, oz);
+  mlg_field *fs[9] = {rur, rup, uz, ror, rop, oz, rbr, rbp, bz};
+  for (int i = 0; i < 9 && !rc; i++) rc = mlg_tofp(fs[i]);
+  mlg_chopset(c, -3);
+  RET(rc);
+  // W = TOFP(B) at NRCHOP (its value does not depend on the products, so it
+  // is formed first and fed to the fused product kernel)
+  RET(copy0(w, b));
+  RET(mlg_tofp(w));
+  bn->space = MLG_PPP;
+  RET(launch_force_products(c, rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, w->e, bn->e,
+                            2 * c->g.p.omega, std::pow(c->g.p.bv0, 2.0), 1));
+  RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
+  RET(launch_chopdo(c, bn->e, MLG_PPP));
+  ror->space = rop->space = oz->space = MLG_PPP;
+  RET(mlg_project(ror, rop, oz, psin, w2, 1, 0.0));
+  RET(mlg_idel2ln(w2, chin));
+  return mlg_toff(bn);
+}
+
+// CALC_BOUSSI_NONLIN (src/mod_boussinesq.f90:235-314)
+extern "C" int mlg_nonlin(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *bn, mlg_field *ror,
+                          mlg_field *rop, mlg_field *oz) {
+  if (!b || !rur || !rup || !uz || !bn || !ror || !rop || !oz) return MLG_E_ARG;
+  if (b->space != MLG_FFF) return 3;
+  mlg_ctx *c = b->ctx;
+  Temps T(c);
+  mlg_field *rbr = T.get(), *rbp = T.get(), *bz = T.get(), *psi = T.get(), *chi = T.get(), *w = T.get();
+  if (!T.ok()) return MLG_E_NOMEM;
+  RET(mlg_chopset(c, 1));
+  int rc = mlg_grad(b, rbr, rbp, bz);
+  mlg_chopset(c, -1);
+  RET(rc);
+  mlg_field *fs[9] = {rur, rup, uz, ror, rop, oz, rbr, rbp, bz};
+  for (int i = 0; i < 9; i++) RET(mlg_tofp(fs[i]));
+  bn->space = MLG_PPP;
+  RET(launch_force_products(c, rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, nullptr, bn->e, 0.0, 0.0,
+                            0));
+  RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
+  RET(launch_chopdo(c, bn->e, MLG_PPP));
+  RET(mlg_project(ror, rop, oz, psi, w, 1, 0.0));
+  RET(mlg_idel2(w, chi, 0, 0.0));
+  RET(mlg_chopset(c, 3));
+  rc = mlg_pc2vel(psi, chi, ror, rop, oz);
+  mlg_chopset(c, -3);
+  RET(rc);
+  return mlg_toff(bn);
+}
+
+// CALC_BOUSSI_DIAG + CALC_BOUSSI_ETD_OP (mod_boussinesq.f90:42-109, mod_march.f90:906-945)
+extern "C" int mlg_etd_op(const mlg_ctx *c, double dt, double *e16, double *nl16) {
+  if (!c || !e16 || !nl16) return MLG_E_ARG;
+  typedef std::complex<double> cd;
+  const cd I(0.0, 1.0);
+  double nf = c->g.p.bv0, om = c->g.p.omega;
+  if (nf == 0.0) return 5;  // CALC_BOUSSI_DIAG: N_FREQ = 0, CANNOT DIAGONALIZE
+  cd jd[4] = {-I * nf, I * nf, -2.0 * I * om, 2.0 * I * om};
+  cd s[16], si[16];
+  for (int q = 0; q < 16; q++) s[q] = si[q] = 0.0;
+#define M4(a, i, j) a[(i) + 4 * (j)]
+  M4(s, 0, 2) = I; M4(s, 0, 3) = -I; M4(s, 1, 2) = 1.0; M4(s, 1, 3) = 1.0;
+  M4(s, 2, 0) = -I / nf; M4(s, 2, 1) = I / nf; M4(s, 3, 0) = 1.0; M4(s, 3, 1) = 1.0;
+  M4(si, 0, 2) = I * nf / 2.0; M4(si, 0, 3) = 0.5; M4(si, 1, 2) = -I * nf / 2.0; M4(si, 1, 3) = 0.5;
+  M4(si, 2, 0) = -I / 2.0; M4(si, 2, 1) = 0.5; M4(si, 3, 0) = I / 2.0; M4(si, 3, 1) = 0.5;
+  cd ex[4], ph[4];
+  for (int i = 0; i < 4; i++) {
+    ex[i] = std::exp(jd[i] * dt);
+    ph[i] = (std::abs(jd[i]) > 1.0e-14) ? (ex[i] - 1.0) / jd[i] : cd(dt, 0.0);
+  }
+  for (int i = 0; i < 4; i++)
+    for (int j = 0; j < 4; j++) {
+      cd ae = 0.0, an = 0.0;
+      for (int l = 0; l < 4; l++) {
+        ae += M4(s, i, l) * ex[l] * M4(si, l, j);
+        an += M4(s, i, l) * ph[l] * M4(si, l, j);
+      }
+      e16[i + 4 * j] = ae.real();
+      nl16[i + 4 * j] = an.real();
+    }
+#undef M4
+  return 0;
+}
+
+// CALC_BOUSSI_ENERGY (src/mod_boussinesq.f90:138-232)
+extern "C" int mlg_energy(const mlg_field *psi, const mlg_field *chi, const mlg_field *b, double *ke, double *pe, double *dke) {
+  if (!psi || !chi || !b || !ke || !pe || !dke) return MLG_E_ARG;
+  mlg_ctx *c = psi->ctx;
+  const Grid &G = c->g;
+  double bln = 0.0;
+  MLG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  MLG_CHECK_HIP(hipMemcpy(&bln, b->ln, sizeof(double), hipMemcpyDeviceToHost));
+  if (bln != 0.0) return 13;
+  Temps T(c);
+  mlg_field *w1 = T.get(), *w2 = T.get(), *w3 = T.get();
+  if (!T.ok()) return MLG_E_NOMEM;
+  const int nt = G.ntchop, nxc = G.nxchop, nxd = G.nxchopdim;
+  const size_t na = (size_t)nt * nxd;
+  double *dall = c->d.red;  // [4][nt*nxd]
+  RET(launch_zero(c, (double2 *)dall, (long)(4 * na / 2 + 1)));
+  RET(mlg_chopset(c, 3));
+  int rc = copy0(w1, psi);
+  if (!rc) rc = launch_set_ln(c, w1->ln, psi->ln, 2.0, 0.0);
+  if (!rc) rc = mlg_delsqh(w1, w2);
+  if (!rc) rc = launch_set_ln(c, w1->ln, nullptr, 0, 0);
+  if (!rc) rc = launch_set_ln(c, w2->ln, nullptr, 0, 0);
+  if (!rc) rc = mlg_rtran(w1, 1);
+  if (!rc) rc = mlg_rtran(w2, 1);
+  if (!rc) rc = launch_energy_terms(c, w1->e, w2->e, 0, dall);
+  if (!rc) rc = mlg_del2(chi, w1);
+  if (!rc) rc = mlg_delsqh(chi, w2);
+  if (!rc) rc = launch_set_ln(c, w1->ln, nullptr, 0, 0);
+  if (!rc) rc = launch_set_ln(c, w2->ln, nullptr, 0, 0);
+  if (!rc) rc = mlg_rtran(w1, 1);
+  if (!rc) rc = mlg_rtran(w2, 1);
+  if (!rc) rc = launch_energy_terms(c, w1->e, w2->e, 0, dall + na);
+  if (!rc) rc = copy0(w3, b);
+  if (!rc) rc = launch_set_ln(c, w3->ln, nullptr, 0, 0);
+  if (!rc) rc = mlg_rtran(w3, 1);
+  if (!rc) rc = launch_energy_terms(c, w3->e, w2->e, 0, dall + 2 * na);
+  if (!rc) rc = launch_energy_terms(c, w3->e, nullptr, 1, dall + 3 * na);
+  mlg_chopset(c, -3);
+  RET(rc);
+  std::vector<double> h(4 * na);
+  MLG_CHECK_HIP(hipStreamSynchronize(c->stream));
+  MLG_CHECK_HIP(hipMemcpy(h.data(), dall, sizeof(double) * 4 * na, hipMemcpyDeviceToHost));
+  // fold +-k into the NXCHOP non-negative wavenumbers (PRODCT_MK_HALFK)
+  auto fold = [&](const double *all, double *out) {
+    for (int i = 0; i < nt * nxc; i++) out[i] = 0.0;
+    for (int mm = 0; mm < nt; mm++) out[mm] = all[mm];
+    for (int kk = 2; kk <= nxc; kk++) {
+      int kc = nxd + 2 - kk;
+      for (int mm = 0; mm < nt; mm++) out[mm + (size_t)nt * (kk - 1)] = all[mm + (size_t)nt * (kc - 1)] + all[mm + (size_t)nt * (kk - 1)];
+    }
+  };
+  std::vector<double> t1(nt * nxc), t2(nt * nxc), t4(nt * nxc);
+  fold(h.data(), t1.data());
+  fold(h.data() + na, t2.data());
+  fold(h.data() + 2 * na, dke);
+  fold(h.data() + 3 * na, t4.data());
+  for (int i = 0; i < nt * nxc; i++) {
+    ke[i] = (-t1[i] + t2[i]) / 2.0;
+    pe[i] = t4[i] / 2.0 / std::pow(G.p.bv0, 2.0);
+  }
+  return mlg_synchronize(c);
+}
+
+// ------------------------------------------------------------------ time march
+extern "C" int mlg_solver_destroy(mlg_solver *s);
+extern "C" int mlg_solver_create(mlg_ctx *c, mlg_solver **out) {
+  if (!c || !out) return MLG_E_ARG;
+  mlg_solver *s = new mlg_solver();
+  s->ctx = c;
+  mlg_field **fs[9] = {&s->n_psi, &s->n_chi, &s->n_b, &s->r_n_r, &s->r_n_p, &s->n_z, &s->r_pi_r, &s->r_pi_p, &s->pi_z};
+  for (auto f : fs) {
+    int rc = new_field(c, MLG_FFF, f);
+    if (rc) {
+      mlg_solver_destroy(s);
+      return rc;
+    }
+  }
+  *out = s;
+  return 0;
+}
+extern "C" int mlg_solver_destroy(mlg_solver *s) {
+  if (!s) return MLG_E_ARG;
+  mlg_field *fs[9] = {s->n_psi, s->n_chi, s->n_b, s->r_n_r, s->r_n_p, s->n_z, s->r_pi_r, s->r_pi_p, s->pi_z};
+  (void)hipStreamSynchronize(s->ctx->stream);
+  for (auto f : fs)
+    if (f) {
+      free_field_storage(f);
+      delete f;
+    }
+  delete s;
+  return 0;
+}
+extern "C" double mlg_solver_time(const mlg_solver *s) { return s ? s->t : 0.0; }
+extern "C" int mlg_solver_nstep(const mlg_solver *s) { return s ? s->n : -1; }
+
+// y.e = x1 + a x2 (+ y.ln likewise)
+static int euler(mlg_field *y, const mlg_field *x1, double a, const mlg_field *x2, bool ln) {
+  mlg_ctx *c = y->ctx;
+  RET(launch_axpby(c, 0, y->e, a, x1->e, x2->e, nullptr, c->n_fff));
+  if (ln) RET(launch_ln_axpby(c, 0, y->ln, a, x1->ln, x2->ln, nullptr));
+  y->space = MLG_FFF;
+  return 0;
+}
+// y = x1 + dt (1.5 x2 - 0.5 x3)
+static int ab2(mlg_field *y, double dt, const mlg_field *x2, const mlg_field *x3, bool ln) {
+  mlg_ctx *c = y->ctx;
+  RET(launch_axpby(c, 1, y->e, dt, y->e, x2->e, x3->e, c->n_fff));
+  if (ln) RET(launch_ln_axpby(c, 1, y->ln, dt, y->ln, x2->ln, x3->ln));
+  return 0;
+}
+// y = 2 x1 - y
+static int richardson(mlg_field *y, const mlg_field *x1, bool ln) {
+  mlg_ctx *c = y->ctx;
+  RET(launch_axpby(c, 2, y->e, 0.0, x1->e, nullptr, nullptr, c->n_fff));
+  if (ln) RET(launch_ln_axpby(c, 2, y->ln, 0.0, x1->ln, nullptr, nullptr));
+  return 0;
+}
+
+// VISC1 (mod_march.f90:1529-1576): backward Euler diffusion
+static int visc1(mlg_field *a, double nu, double dt, mlg_field *wk) {
+  if (nu == 0.0) return 5;
+  mlg_ctx *c = a->ctx;
+  double alp = 1 / (nu * dt);
+  wk->space = MLG_FFF;
+  RET(launch_set_ln(c, wk->ln, a->ln, -alp, 0.0));
+  RET(launch_copy(c, wk->e, a->e, c->n_fff));
+  RET(launch_scale(c, wk->e, -alp, c->n_fff));
+  return mlg_ihelm(wk, a, alp);
+}
+
+// VISC2 (mod_march.f90:1579-1636): Crank-Nicolson diffusion
+static int visc2(mlg_field *a, const mlg_field *ao, double nu, double dt, mlg_field *wk) {
+  if (nu == 0.0) return 5;
+  mlg_ctx *c = a->ctx;
+  double alp = 2 / nu / dt;
+  RET(mlg_chopset(c, 2));
+  int rc = mlg_del2(ao, wk);
+  mlg_chopset(c, -2);
+  RET(rc);
+  RET(launch_chopdo(c, wk->e, MLG_FFF));
+  // WK = -WK - ALP*A
+  RET(launch_axpby(c, 5, wk->e, alp, a->e, nullptr, nullptr, c->n_fff));
+  RET(launch_set_ln(c, wk->ln, a->ln, -alp, 0.0));
+  return mlg_ihelm(wk, a, alp);
+}
+
+static int visc_be(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, double dt) {
+  if (c->g.p.visc_sw != 1) return 0;
+  Temps T(c);
+  mlg_field *wk = T.get();
+  if (!wk) return MLG_E_NOMEM;
+  RET(visc1(psi, c->g.p.nu, dt, wk));
+  RET(visc1(chi, c->g.p.nu, dt, wk));
+  if (c->g.p.kappa != 0.0) RET(visc1(b, c->g.p.kappa, dt, wk));
+  return 0;
+}
+
+static int visc_cn(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, const mlg_field *psio, const mlg_field *chio,
+                   const mlg_field *bo) {
+  if (c->g.p.visc_sw != 1) return 0;
+  Temps T(c);
+  mlg_field *wk = T.get();
+  if (!wk) return MLG_E_NOMEM;
+  RET(visc2(psi, psio, c->g.p.nu, c->g.p.dt, wk));
+  RET(visc2(chi, chio, c->g.p.nu, c->g.p.dt, wk));
+  if (c->g.p.kappa != 0.0) RET(visc2(b, bo, c->g.p.kappa, c->g.p.dt, wk));
+  return 0;
+}
+
+static bool filter_active(const mlg_ctx *c) { return c->g.p.visc_sw == 3 || c->g.p.hyper_cn; }
+
+// CALC_BOUSSI_HYPERV, SW=3 branch (mod_march.f90:1339-1343)
+static int hyperv_state(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, double dt) {
+  if (c->g.p.visc_sw != 3) return 0;
+  double tf = dt / c->g.p.dt;
+  RET(launch_hyp3(c, psi->e, 0, tf));
+  RET(launch_hyp3(c, chi->e, 0, tf));
+  return launch_hyp3(c, b->e, 1, tf);
+}
+// CALC_BOUSSI_ETD_HYPERV (mod_march.f90:1352-1416)
+static int etd_hyperv(mlg_ctx *c, mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *b, double dt) {
+  if (!filter_active(c)) return 0;
+  double tf = dt / c->g.p.dt;
+  RET(launch_hyp3(c, rur->e, 0, tf));
+  RET(launch_hyp3(c, rup->e, 0, tf));
+  RET(launch_hyp3(c, uz->e, 0, tf));
+  return launch_hyp3(c, b->e, 1, tf);
+}
+
+// RICH_BOUSSI_FE_BE (mod_march.f90:535-606)
+static int rich_fe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
+  mlg_ctx *c = s->ctx;
+  double dt = c->g.p.dt, hdt = dt * 0.5;
+  s->t += dt;
+  s->n += 1;
+  Temps T(c);
+  mlg_field *psi2 = T.get(), *chi2 = T.get(), *b2 = T.get(), *pn = T.get(), *cn = T.get(), *bnn = T.get();
+  if (!T.ok()) return MLG_E_NOMEM;
+  RET(mlg_force(psi, chi, b, s->n_psi, s->n_chi, s->n_b));
+  RET(euler(psi2, psi, hdt, s->n_psi, true));
+  RET(euler(chi2, chi, hdt, s->n_chi, true));
+  RET(euler(b2, b, hdt, s->n_b, false));
+  RET(visc_be(c, psi2, chi2, b2, hdt));
+  RET(hyperv_state(c, psi2, chi2, b2, hdt));
+  RET(mlg_force(psi2, chi2, b2, pn, cn, bnn));
+  RET(euler(psi2, psi2, hdt, pn, true));
+  RET(euler(chi2, chi2, hdt, cn, true));
+  RET(euler(b2, b2, hdt, bnn, false));
+  RET(visc_be(c, psi2, chi2, b2, hdt));
+  RET(hyperv_state(c, psi2, chi2, b2, hdt));
+  RET(euler(psi, psi, dt, s->n_psi, true));
+  RET(euler(chi, chi, dt, s->n_chi, true));
+  RET(euler(b, b, dt, s->n_b, false));
+  RET(visc_be(c, psi, chi, b, dt));
+  RET(hyperv_state(c, psi, chi, b, dt));
+  RET(richardson(psi, psi2, true));
+  RET(richardson(chi, chi2, true));
+  return richardson(b, b2, false);
+}
+
+// STEP_BOUSSI_AB_CN (mod_march.f90:609-661)
+static int step_ab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
+  mlg_ctx *c = s->ctx;
+  double dt = c->g.p.dt;
+  s->t += dt;
+  s->n += 1;
+  Temps T(c);
+  mlg_field *psin = T.get(), *chin = T.get(), *bn = T.get(), *psi0 = T.get(), *chi0 = T.get(), *b0 = T.get();
+  if (!T.ok()) return MLG_E_NOMEM;
+  RET(mlg_force(psi, chi, b, psin, chin, bn));
+  RET(copy0(psi0, psi));
+  RET(copy0(chi0, chi));
+  RET(copy0(b0, b));
+  RET(ab2(psi, dt, psin, s->n_psi, true));
+  RET(ab2(chi, dt, chin, s->n_chi, true));
+  RET(ab2(b, dt, bn, s->n_b, false));
+  RET(copy0(s->n_psi, psin));
+  RET(copy0(s->n_chi, chin));
+  RET(copy0(s->n_b, bn));
+  RET(visc_cn(c, psi, chi, b, psi0, chi0, b0));
+  return hyperv_state(c, psi, chi, b, dt);
+}
+
+static int pc2vel3(mlg_ctx *c, const mlg_field *psi, const mlg_field *chi, mlg_field *rur, mlg_field *rup, mlg_field *uz) {
+  RET(mlg_chopset(c, 3));
+  int rc = mlg_pc2vel(psi, chi, rur, rup, uz);
+  mlg_chopset(c, -3);
+  return rc;
+}
+
+static int copy_e(mlg_field *d, const mlg_field *s) {
+  RET(launch_copy(d->ctx, d->e, s->e, d->ctx->n_fff));
+  d->space = MLG_FFF;
+  return 0;
+}
+
+// STEP_BOUSSI_ETDFE_BE (mod_march.f90:664-753)
+static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, const double *e16, const double *nl16,
+                         double dt, bool save) {
+  mlg_ctx *c = s->ctx;
+  Temps T(c);
+  mlg_field *rurn = T.get(), *rupn = T.get(), *uzn = T.get(), *rorn = T.get(), *ropn = T.get(), *ozn = T.get();
+  mlg_field *bn = T.get(), *w = T.get();
+  if (!T.ok()) return MLG_E_NOMEM;
+  RET(mlg_chopset(c, 3));
+  int rc = mlg_pc2vel(psi, chi, rurn, rupn, uzn);
+  if (!rc) rc = mlg_pc2vor(psi, chi, rorn, ropn, ozn);
+  mlg_chopset(c, -3);
+  RET(rc);
+  RET(mlg_nonlin(b, rurn, rupn, uzn, bn, rorn, ropn, ozn));
+  RET(etd_hyperv(c, rorn, ropn, ozn, bn, dt));
+  RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
+  RET(launch_etd_sub(c, e16, nl16, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, nullptr, nullptr, nullptr,
+                     nullptr, nullptr, nullptr, nullptr, 0));
+  RET(etd_hyperv(c, rurn, rupn, uzn, b, dt));
+  if (save) {
+    RET(copy_e(s->n_b, bn));
+    RET(copy_e(s->r_n_r, rorn));
+    RET(copy_e(s->r_n_p, ropn));
+    RET(copy_e(s->n_z, ozn));
+    RET(copy_e(s->r_pi_r, rurn));
+    RET(copy_e(s->r_pi_p, rupn));
+    RET(copy_e(s->pi_z, uzn));
+  }
+  RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0));  // PROJECT(..., PSI, W, PSI%LN)
+  RET(mlg_idel2ln(w, chi));
+  if (save) {
+    RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
+    RET(launch_axpby(c, 3, s->r_pi_r->e, 0.5, s->r_pi_r->e, rurn->e, nullptr, c->n_fff));
+    RET(launch_axpby(c, 3, s->r_pi_p->e, 0.5, s->r_pi_p->e, rupn->e, nullptr, c->n_fff));
+    RET(launch_axpby(c, 3, s->pi_z->e, 0.5, s->pi_z->e, uzn->e, nullptr, c->n_fff));
+  }
+  return 0;
+}
+
+// RICH_BOUSSI_ETDFE_BE (mod_march.f90:756-808)
+static int rich_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
+  mlg_ctx *c = s->ctx;
+  double dt = c->g.p.dt, hdt = dt * 0.5;
+  double he[16], hn[16];
+  RET(mlg_etd_op(c, hdt, he, hn));
+  s->t += dt;
+  s->n += 1;
+  Temps T(c);
+  mlg_field *psi2 = T.get(), *chi2 = T.get(), *b2 = T.get();
+  if (!T.ok()) return MLG_E_NOMEM;
+  RET(copy0(psi2, psi));
+  RET(copy0(chi2, chi));
+  RET(copy0(b2, b));
+  RET(step_etdfe_be(s, psi2, chi2, b2, he, hn, hdt, false));
+  RET(visc_be(c, psi2, chi2, b2, hdt));
+  RET(step_etdfe_be(s, psi2, chi2, b2, he, hn, hdt, false));
+  RET(visc_be(c, psi2, chi2, b2, hdt));
+  RET(step_etdfe_be(s, psi, chi, b, s->etd_e, s->etd_nl, dt, true));
+  RET(visc_be(c, psi, chi, b, dt));
+  RET(richardson(psi, psi2, true));
+  RET(richardson(chi, chi2, true));
+  return richardson(b, b2, false);
+}
+
+// STEP_BOUSSI_ETDAB_CN (mod_march.f90:811-901)
+static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
+  mlg_ctx *c = s->ctx;
+  double dt = c->g.p.dt;
+  s->t += dt;
+  s->n += 1;
+  Temps T(c);
+  mlg_field *rurn = T.get(), *rupn = T.get(), *uzn = T.get(), *rorn = T.get(), *ropn = T.get(), *ozn = T.get();
+  mlg_field *bn = T.get(), *w = T.get(), *psi0 = T.get(), *chi0 = T.get(), *b0 = T.get();
+  if (!T.ok()) return MLG_E_NOMEM;
+  RET(mlg_chopset(c, 3));
+  int rc = mlg_pc2vel(psi, chi, rurn, rupn, uzn);
+  if (!rc) rc = mlg_pc2vor(psi, chi, rorn, ropn, ozn);
+  mlg_chopset(c, -3);
+  RET(rc);
+  RET(mlg_nonlin(b, rurn, rupn, uzn, bn, rorn, ropn, ozn));
+  RET(etd_hyperv(c, rorn, ropn, ozn, bn, dt));
+  RET(copy0(b0, b));
+  RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
+  RET(launch_etd_sub(c, s->etd_e, s->etd_nl, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
+                     s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 1));
+  RET(copy_e(s->n_b, bn));
+  RET(copy_e(s->r_n_r, rorn));
+  RET(copy_e(s->r_n_p, ropn));
+  RET(copy_e(s->n_z, ozn));
+  RET(etd_hyperv(c, rurn, rupn, uzn, b, dt));
+  RET(copy_e(s->r_pi_r, rurn));
+  RET(copy_e(s->r_pi_p, rupn));
+  RET(copy_e(s->pi_z, uzn));
+  RET(copy0(psi0, psi));
+  RET(copy0(chi0, chi));
+  RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0));  // PROJECT(..., PSI, W, PSI%LN)
+  RET(mlg_idel2ln(w, chi));
+  RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
+  RET(launch_axpby(c, 4, s->r_pi_r->e, 0.0, s->r_pi_r->e, rurn->e, nullptr, c->n_fff));
+  RET(launch_axpby(c, 4, s->r_pi_p->e, 0.0, s->r_pi_p->e, rupn->e, nullptr, c->n_fff));
+  RET(launch_axpby(c, 4, s->pi_z->e, 0.0, s->pi_z->e, uzn->e, nullptr, c->n_fff));
+  return visc_cn(c, psi, chi, b, psi0, chi0, b0);
+}
+
+extern "C" int mlg_initialize(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
+  if (!s || !psi || !chi || !b) return MLG_E_ARG;
+  mlg_ctx *c = s->ctx;
+  int rc;
+  if (c->g.p.adams) {
+    rc = rich_fe_be(s, psi, chi, b);
+  } else {
+    RET(mlg_etd_op(c, c->g.p.dt, s->etd_e, s->etd_nl));
+    rc = rich_etdfe_be(s, psi, chi, b);
+  }
+  RET(rc);
+  return mlg_synchronize(c);
+}
+
+static int one_step(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
+  mlg_ctx *c = s->ctx;
+  RET(c->g.p.adams ? step_ab_cn(s, psi, chi, b) : step_etdab_cn(s, psi, chi, b));
+  return launch_diagnost(c, psi->e);
+}
+
+extern "C" int mlg_time_step(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
+  if (!s || !psi || !chi || !b) return MLG_E_ARG;
+  RET(one_step(s, psi, chi, b));
+  return mlg_synchronize(s->ctx);
+}
+
+extern "C" int mlg_time_steps(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, int nsteps) {
+  if (!s || !psi || !chi || !b || nsteps < 0) return MLG_E_ARG;
+  for (int i = 0; i < nsteps; i++) RET(one_step(s, psi, chi, b));
+  return mlg_synchronize(s->ctx);
+}
+
+// ------------------------------------------------------------------ initial condition
+// QVORTEX (src/mod_layout.f90:449-610) with LAY (:64-118) and GAUS (:640-673)
+extern "C" int mlg_qvortex(mlg_ctx *c, double q, mlg_field *psi, mlg_field *chi) {
+  if (!c || !psi || !chi) return MLG_E_ARG;
+  const Grid &G = c->g;
+  const double pi = std::acos(-1.0);
+  std::vector<double2> h((size_t)c->n_ppp, make_double2(0, 0));
+  std::vector<double> thr(G.nth), thi(G.nth);
+  for (int j = 0; j < G.nth; j++) {
+    thr[j] = 2 * pi / G.p.minc / G.nth * j;
+    thi[j] = 2 * pi / G.p.minc / G.nth * (j + 0.5);
+  }
+  for (int z = 0; z < G.ndimx; z++)
+    for (int j = 0; j < G.nth; j++)
+      for (int r = 0; r < G.nr; r++) {
+        double rr = G.r[r];
+        double f = std::pow(1 - G.x[r], 2);
+        auto gaus = [&](double th) {
+          double xx = rr * std::cos(th), yy = rr * std::sin(th);
+          double r2 = (std::pow(xx, 2.0) + std::pow(yy, 2.0)) / std::pow(1.0, 2.0);
+          return (q * 2.0) / std::pow(1.0, 2.0) * std::exp(-1.0 * r2);
+        };
+        h[(size_t)r + (size_t)G.ndimr * ((size_t)j + (size_t)G.ndimth * z)] = make_double2(gaus(thr[j]) / f, gaus(thi[j]) / f);
+      }
+  Temps T(c);
+  mlg_field *a = T.get(), *bb = T.get();
+  if (!T.ok()) return MLG_E_NOMEM;
+  RET(mlg_field_upload(a, (const double *)h.data(), MLG_PPP, 0.0));
+  RET(mlg_toff(a));
+  RET(launch_negate(c, a->e, c->n_fff));
+  RET(launch_set_ln(c, a->ln, nullptr, 0.0, 0.0));
+  RET(mlg_idelsqh(a, bb));
+  RET(copy0(psi, bb));
+  RET(mlg_field_zero(chi, MLG_FFF));
+  return mlg_synchronize(c);
+}

@@ -1,0 +1,360 @@
+// Mapped-Legendre radial transforms on FP64 MFMA (gfx950).
+//
+// RTRAN (src/mod_fft.f90:1150-1340) and the projection products of PROJECT
+// (src/mod_legops.f90:2476-2529, EOMUL/OEMUL :2548-2648) are, for every
+// azimuthal mode m, real x complex matrix products against the parity-folded
+// table Pbar_n^m(x_i), i < NR/2.  All modes run in one grouped launch
+// (blockIdx.y = m); each mode's K extent follows the triangular truncation
+// NRCHOPS(m).  Complex data enter as interleaved (re, im) real columns, so
+// one real MFMA product serves both parts.
+//
+// MFMA: v_mfma_f64_16x16x4f64 -- lane l supplies A[l%16][l/16] and
+// B[l/16][l%16]; the 16x16 result holds C[l/16 + 4v][l%16] in register v
+// (measured on gfx950; the f32 16x16x4 variant uses C[4(l/16)+v][l%16]).
+#include "mlegs_internal.h"
+
+namespace mlg {
+
+typedef double v4d __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c) {
+  return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+__device__ __forceinline__ int d_nrchops(int nrchop, int m) {
+  int v = nrchop - m < nrchop ? nrchop - m : nrchop;
+  return v > 0 ? v : 0;
+}
+
+// -------------------------------------------------------------------------
+// Backward: FFF (n,m,k) -> PFF (r,m,k).
+// BE(i,c) = sum_j PF(i,2j) a(2j,c), BO(i,c) = sum_j PF(i,2j+1) a(2j+1,c);
+// f(x_i) = BE+BO, f(x_{NR-1-i}) = BE-BO; LN*(-log(1-x)) added at m=0,k=0.
+// Block tile: 64 radial nodes x 64 real columns (32 complex k); 4 waves as
+// 2x2, each wave 32x32 = 2x2 MFMA tiles per parity.  K chunk: 16 pairs.
+// -------------------------------------------------------------------------
+#define BW_TI 64
+#define BW_TC 64
+#define BW_JC 16
+
+__global__ void __launch_bounds__(256) k_rtran_bwd(KGrid g, const double *__restrict__ pf, const double2 *__restrict__ in,
+                                                   const double *__restrict__ ln_in, const double *__restrict__ lnx,
+                                                   double2 *__restrict__ out) {
+  const int mm = blockIdx.y;
+  const int mv = mm * g.minc;
+  const int nn = d_nrchops(g.nrchop, mv);
+  const int nk = g.nxchopdim;
+  const int ntiles_c = (2 * nk + BW_TC - 1) / BW_TC;
+  const int ti = blockIdx.x / ntiles_c, tc = blockIdx.x % ntiles_c;
+  const int i0 = ti * BW_TI, c0 = tc * BW_TC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int wi = wave & 1, wc = wave >> 1;
+
+  __shared__ double sA[2][BW_JC][BW_TI];  // [parity][j][i]
+  __shared__ double sB[2][BW_JC][BW_TC];  // [parity][j][col]
+
+  v4d acc[2][2][2];  // [parity][sub i][sub c]
+  for (int p = 0; p < 2; p++)
+    for (int a = 0; a < 2; a++)
+      for (int b = 0; b < 2; b++) acc[p][a][b] = (v4d){0, 0, 0, 0};
+
+  const double *pfm = pf + (size_t)mm * g.npf * g.nrh;
+  const int njp = (nn + 1) / 2;  // number of (even, odd) pairs
+  for (int j0 = 0; j0 < njp; j0 += BW_JC) {
+    // table tile: 32 columns n = 2*j0 .. 2*j0+31, 64 nodes each
+    for (int q = 0; q < 8; q++) {
+      int idx = tid + 256 * q;     // 0..2047
+      int i = idx & 63, nl = idx >> 6;  // nl 0..31
+      int n = 2 * j0 + nl;
+      int ig = i0 + i;
+      double v = 0.0;
+      if (n < nn && ig < g.nrh) v = pfm[(size_t)n * g.nrh + ig];
+      sA[nl & 1][nl >> 1][i] = v;
+    }
+    // data tile: 32 rows n x 32 complex k
+    for (int q = 0; q < 4; q++) {
+      int idx = tid + 256 * q;     // 0..1023
+      int nl = idx & 31, kl = idx >> 5;
+      int n = 2 * j0 + nl;
+      int k = (c0 >> 1) + kl;
+      double2 v = make_double2(0.0, 0.0);
+      if (n < nn && k < nk) v = in[(size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k)];
+      sB[nl & 1][nl >> 1][2 * kl] = v.x;
+      sB[nl & 1][nl >> 1][2 * kl + 1] = v.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < BW_JC; ks += 4) {
+      const int kr = ks + (lane >> 4);
+#pragma unroll
+      for (int p = 0; p < 2; p++) {
+        double a0 = sA[p][kr][wi * 32 + (lane & 15)];
+        double a1 = sA[p][kr][wi * 32 + 16 + (lane & 15)];
+        double b0 = sB[p][kr][wc * 32 + (lane & 15)];
+        double b1 = sB[p][kr][wc * 32 + 16 + (lane & 15)];
+        acc[p][0][0] = mfma16x16x4(a0, b0, acc[p][0][0]);
+        acc[p][0][1] = mfma16x16x4(a0, b1, acc[p][0][1]);
+        acc[p][1][0] = mfma16x16x4(a1, b0, acc[p][1][0]);
+        acc[p][1][1] = mfma16x16x4(a1, b1, acc[p][1][1]);
+      }
+    }
+    __syncthreads();
+  }
+  // epilogue through LDS: S = BE+BO, D = BE-BO as [col][i]
+  __shared__ double sS[BW_TC][BW_TI + 1];
+  __shared__ double sD[BW_TC][BW_TI + 1];
+  for (int a = 0; a < 2; a++)
+    for (int b = 0; b < 2; b++)
+      for (int v = 0; v < 4; v++) {
+        int il = wi * 32 + a * 16 + (lane >> 4) + 4 * v;
+        int cl = wc * 32 + b * 16 + (lane & 15);
+        double e = acc[0][a][b][v], o = acc[1][a][b][v];
+        sS[cl][il] = e + o;
+        sD[cl][il] = e - o;
+      }
+  __syncthreads();
+  const double lnv = (mm == 0) ? ln_in[0] : 0.0;
+  for (int q = 0; q < 8; q++) {
+    int idx = tid + 256 * q;  // 0..2047: i fastest
+    int il = idx & 63, kl = idx >> 6;
+    int ig = i0 + il, k = (c0 >> 1) + kl;
+    if (ig < g.nrh && k < nk) {
+      size_t col = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
+      double2 s = make_double2(sS[2 * kl][il], sS[2 * kl + 1][il]);
+      double2 d = make_double2(sD[2 * kl][il], sD[2 * kl + 1][il]);
+      int ir = g.nr - 1 - ig;
+      if (mm == 0 && k == 0 && lnv != 0.0) {
+        s.x = s.x + lnv * lnx[ig];
+        d.x = d.x + lnv * lnx[ir];
+      }
+      out[col + ig] = s;
+      out[col + ir] = d;
+    }
+  }
+}
+
+int launch_rtran_bwd(mlg_ctx *c, const double2 *in, const double *ln_in, double2 *out) {
+  KGrid g = kgrid(c);
+  int nti = (g.nrh + BW_TI - 1) / BW_TI, ntc = (2 * g.nxchopdim + BW_TC - 1) / BW_TC;
+  dim3 grid(nti * ntc, g.ntchopdim);
+  // 2 NR sum_m NRCHOPS(m) NXCHOPDIM flops (real table x complex data, folded)
+  KScope ks(c, K_RTRAN_BWD, 2.0 * g.nr * (double)sum_nrchops(c) * g.nxchopdim);
+  hipLaunchKernelGGL(k_rtran_bwd, grid, dim3(256), 0, c->stream, g, c->d.pf, in, ln_in, c->d.lnx, out);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// -------------------------------------------------------------------------
+// Forward: PFF (r,m,k) -> FFF (n,m,k).
+// a_n = sum_i PF(i,n) F_p(i,c), F_e = (f(x_i)+f(x_{NR-1-i})) w_i,
+// F_o = (f(x_i)-f(x_{NR-1-i})) w_i, p = parity of n.
+// Block tile: 32 rows n (16 even + 16 odd) x 64 real columns; wave w:
+// parity w&1, column half w>>1 (2 MFMA tiles).  K chunk: 16 nodes.
+// -------------------------------------------------------------------------
+#define FW_TN 32
+#define FW_TC 64
+#define FW_IC 16
+
+__global__ void __launch_bounds__(256) k_rtran_fwd(KGrid g, const double *__restrict__ pf, const double *__restrict__ w,
+                                                   const double2 *__restrict__ in, double2 *__restrict__ out) {
+  const int mm = blockIdx.y;
+  const int mv = mm * g.minc;
+  const int nn = d_nrchops(g.nrchop, mv);
+  const int nk = g.nxchopdim;
+  const int ntc = (2 * nk + FW_TC - 1) / FW_TC;
+  const int tn = blockIdx.x / ntc, tc = blockIdx.x % ntc;
+  const int n0 = tn * FW_TN, c0 = tc * FW_TC;
+  if (n0 >= nn) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int par = wave & 1, ch = wave >> 1;
+
+  __shared__ double sA[2][FW_IC][16];      // [parity][i][n-local]
+  __shared__ double sB[2][FW_IC][FW_TC];   // [parity][i][col]
+  v4d acc[2] = {(v4d){0, 0, 0, 0}, (v4d){0, 0, 0, 0}};
+  const double *pfm = pf + (size_t)mm * g.npf * g.nrh;
+  for (int ib = 0; ib < g.nrh; ib += FW_IC) {
+    // table tile: 32 n x 16 i
+    for (int q = 0; q < 2; q++) {
+      int idx = tid + 256 * q;  // 0..511
+      int il = idx & 15, nl = idx >> 4;  // nl 0..31
+      int n = n0 + nl, i = ib + il;
+      double v = 0.0;
+      if (n < nn && i < g.nrh) v = pfm[(size_t)n * g.nrh + i];
+      sA[nl & 1][il][nl >> 1] = v;
+    }
+    // folded data: 16 nodes x 32 complex k
+    for (int q = 0; q < 2; q++) {
+      int idx = tid + 256 * q;  // 0..511
+      int il = idx & 15, kl = idx >> 4;
+      int i = ib + il, k = (c0 >> 1) + kl;
+      double2 e = make_double2(0.0, 0.0), o = make_double2(0.0, 0.0);
+      if (i < g.nrh && k < nk) {
+        size_t col = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
+        double2 p = in[col + i], r = in[col + (g.nr - 1 - i)];
+        double wi = w[i];
+        e = make_double2((p.x + r.x) * wi, (p.y + r.y) * wi);
+        o = make_double2((p.x - r.x) * wi, (p.y - r.y) * wi);
+      }
+      sB[0][il][2 * kl] = e.x;
+      sB[0][il][2 * kl + 1] = e.y;
+      sB[1][il][2 * kl] = o.x;
+      sB[1][il][2 * kl + 1] = o.y;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < FW_IC; ks += 4) {
+      const int kr = ks + (lane >> 4);
+      double a = sA[par][kr][lane & 15];
+      double b0 = sB[par][kr][ch * 32 + (lane & 15)];
+      double b1 = sB[par][kr][ch * 32 + 16 + (lane & 15)];
+      acc[0] = mfma16x16x4(a, b0, acc[0]);
+      acc[1] = mfma16x16x4(a, b1, acc[1]);
+    }
+    __syncthreads();
+  }
+  __shared__ double sC[FW_TC][FW_TN + 1];  // [col][n-local]
+  for (int b = 0; b < 2; b++)
+    for (int v = 0; v < 4; v++) {
+      int rl = (lane >> 4) + 4 * v;         // row within the parity tile
+      int nl = 2 * rl + par;                // n-local
+      int cl = ch * 32 + b * 16 + (lane & 15);
+      sC[cl][nl] = acc[b][v];
+    }
+  __syncthreads();
+  for (int q = 0; q < 4; q++) {
+    int idx = tid + 256 * q;  // 0..1023, n fastest
+    int nl = idx & 31, kl = idx >> 5;
+    int n = n0 + nl, k = (c0 >> 1) + kl;
+    if (n < g.nrchopdim && k < nk) {
+      double2 v = (n < nn) ? make_double2(sC[2 * kl][nl], sC[2 * kl + 1][nl]) : make_double2(0.0, 0.0);
+      out[(size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k)] = v;
+    }
+  }
+}
+
+int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out) {
+  KGrid g = kgrid(c);
+  int ntn = (g.nrchopdim + FW_TN - 1) / FW_TN, ntc = (2 * g.nxchopdim + FW_TC - 1) / FW_TC;
+  dim3 grid(ntn * ntc, g.ntchopdim);
+  KScope ks(c, K_RTRAN_FWD, 2.0 * g.nr * (double)sum_nrchops(c) * g.nxchopdim);
+  hipLaunchKernelGGL(k_rtran_fwd, grid, dim3(256), 0, c->stream, g, c->d.pf, c->d.w, in, out);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// -------------------------------------------------------------------------
+// PROJECT products: five parity-folded products per m,
+//   W1 = EO(V, RUR), W2 = OE(D, RUP), W3 = OE(D, RUR), W4 = EO(V, RUP),
+//   W5 = EO(T, UZ),
+// psi = -i m W1 - W2 ; del^2 chi = i k W3 + m k W4 - W5.
+// EO: even n uses the even fold (u(x_i)+u(x_{NR-1-i})), odd n the odd fold;
+// OE swaps them.  Folds here carry no weight (it is inside V, D, T).
+// Block tile: 32 rows n (16 even + 16 odd) x 32 real columns (16 complex k);
+// wave w: parity w&1, column half w>>1.  K chunk: 16 nodes.
+// -------------------------------------------------------------------------
+#define PJ_TN 32
+#define PJ_TC 32
+#define PJ_IC 16
+
+__global__ void __launch_bounds__(256) k_project(KGrid g, const double *__restrict__ vdt, const double *__restrict__ ak,
+                                                 const double2 *__restrict__ rur, const double2 *__restrict__ rup,
+                                                 const double2 *__restrict__ uz, double2 *__restrict__ psi,
+                                                 double2 *__restrict__ chi) {
+  const int mm = blockIdx.y;
+  const int mv = mm * g.minc;
+  const int nn = d_nrchops(g.nrchop, mv);
+  const int nk = g.nxchopdim;
+  const int ntc = (2 * nk + PJ_TC - 1) / PJ_TC;
+  const int tn = blockIdx.x / ntc, tc = blockIdx.x % ntc;
+  const int n0 = tn * PJ_TN, c0 = tc * PJ_TC;
+  if (n0 >= nn) return;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  const int par = wave & 1, ch = wave >> 1;
+  const size_t tstride = (size_t)g.ntchopdim * g.nrchopdim * g.nrh;
+
+  __shared__ double sT[3][2][PJ_IC][16];        // [V,D,T][parity][i][n-local]
+  __shared__ double sF[3][2][PJ_IC][PJ_TC];     // [field][fold parity][i][col]
+  v4d acc[5];
+  for (int q = 0; q < 5; q++) acc[q] = (v4d){0, 0, 0, 0};
+  const double2 *src[3] = {rur, rup, uz};
+  for (int ib = 0; ib < g.nrh; ib += PJ_IC) {
+    for (int q = 0; q < 6; q++) {
+      int idx = tid + 256 * q;  // 0..1535: [table][n 32][i 16]
+      int il = idx & 15, nl = (idx >> 4) & 31, t = idx >> 9;
+      int n = n0 + nl, i = ib + il;
+      double v = 0.0;
+      if (n < nn && i < g.nrh) v = vdt[(size_t)t * tstride + ((size_t)mm * g.nrchopdim + n) * g.nrh + i];
+      sT[t][nl & 1][il][nl >> 1] = v;
+    }
+    for (int q = 0; q < 3; q++) {
+      int idx = tid + 256 * q;  // 0..767: [field][k 16][i 16]
+      int il = idx & 15, kl = (idx >> 4) & 15, f = idx >> 8;
+      int i = ib + il, k = (c0 >> 1) + kl;
+      double2 e = make_double2(0.0, 0.0), o = make_double2(0.0, 0.0);
+      if (i < g.nrh && k < nk) {
+        size_t col = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
+        double2 p = src[f][col + i], r = src[f][col + (g.nr - 1 - i)];
+        e = make_double2(p.x + r.x, p.y + r.y);
+        o = make_double2(p.x - r.x, p.y - r.y);
+      }
+      sF[f][0][il][2 * kl] = e.x;
+      sF[f][0][il][2 * kl + 1] = e.y;
+      sF[f][1][il][2 * kl] = o.x;
+      sF[f][1][il][2 * kl + 1] = o.y;
+    }
+    __syncthreads();
+    const int eo = par, oe = par ^ 1;
+#pragma unroll
+    for (int ks = 0; ks < PJ_IC; ks += 4) {
+      const int kr = ks + (lane >> 4), cl = ch * 16 + (lane & 15);
+      double av = sT[0][par][kr][lane & 15], ad = sT[1][par][kr][lane & 15], at = sT[2][par][kr][lane & 15];
+      acc[0] = mfma16x16x4(av, sF[0][eo][kr][cl], acc[0]);
+      acc[1] = mfma16x16x4(ad, sF[1][oe][kr][cl], acc[1]);
+      acc[2] = mfma16x16x4(ad, sF[0][oe][kr][cl], acc[2]);
+      acc[3] = mfma16x16x4(av, sF[1][eo][kr][cl], acc[3]);
+      acc[4] = mfma16x16x4(at, sF[2][eo][kr][cl], acc[4]);
+    }
+    __syncthreads();
+  }
+  __shared__ double sW[5][PJ_TC][PJ_TN + 1];
+  for (int q = 0; q < 5; q++)
+    for (int v = 0; v < 4; v++) {
+      int nl = 2 * ((lane >> 4) + 4 * v) + par;
+      int cl = ch * 16 + (lane & 15);
+      sW[q][cl][nl] = acc[q][v];
+    }
+  __syncthreads();
+  for (int q = 0; q < 2; q++) {
+    int idx = tid + 256 * q;  // 0..511: n fastest, 16 complex k
+    int nl = idx & 31, kl = idx >> 5;
+    int n = n0 + nl, k = (c0 >> 1) + kl;
+    if (n < g.nrchopdim && k < nk) {
+      size_t o = (size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
+      if (n < nn) {
+        double w1r = sW[0][2 * kl][nl], w1i = sW[0][2 * kl + 1][nl];
+        double w2r = sW[1][2 * kl][nl], w2i = sW[1][2 * kl + 1][nl];
+        double w3r = sW[2][2 * kl][nl], w3i = sW[2][2 * kl + 1][nl];
+        double w4r = sW[3][2 * kl][nl], w4i = sW[3][2 * kl + 1][nl];
+        double w5r = sW[4][2 * kl][nl], w5i = sW[4][2 * kl + 1][nl];
+        double m = (double)mv, kv = ak[mm + (size_t)g.ntchopdim * k], mk = m * kv;
+        psi[o] = make_double2(m * w1i - w2r, -m * w1r - w2i);
+        chi[o] = make_double2((-kv * w3i + mk * w4r) - w5r, (kv * w3r + mk * w4i) - w5i);
+      } else {
+        psi[o] = make_double2(0.0, 0.0);
+        chi[o] = make_double2(0.0, 0.0);
+      }
+    }
+  }
+}
+
+int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *psi, double2 *chi) {
+  KGrid g = kgrid(c);
+  int ntn = (g.nrchopdim + PJ_TN - 1) / PJ_TN, ntc = (2 * g.nxchopdim + PJ_TC - 1) / PJ_TC;
+  dim3 grid(ntn * ntc, g.ntchopdim);
+  KScope ks(c, K_PROJECT, 5 * 2.0 * g.nr * (double)sum_nrchops(c) * g.nxchopdim);  // five products
+  hipLaunchKernelGGL(k_project, grid, dim3(256), 0, c->stream, g, c->d.vdt, c->d.ak, rur, rup, uz, psi, chi);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace mlg

@@ -1,0 +1,885 @@
+// Spectral-space (FFF) kernels: banded operators, banded solves, pointwise
+// updates, filters, checks and reductions.
+//
+// FFF element (n, m, k) lives at n + NRCHOPDIM*(m + NTCHOPDIM*k).  Banded
+// operators (src/mod_legops.f90) apply, per (m, k), a tri/penta-diagonal
+// matrix truncated to NRCHOPS(m) rows, exactly as BANMULC does
+// (src/mod_bandmat.f90:1199-1243): diagonal term first, then the off-diagonal
+// terms in band-column order, entries outside the truncated matrix skipped.
+// Composite operators (PC2VEL, PC2VOR, GRAD) are fused into one pass that
+// recomputes the intermediate bands per output row.
+#include "mlegs_internal.h"
+
+namespace mlg {
+
+__device__ __forceinline__ int s_nrchops(int nrchop, int m) {
+  int v = nrchop - m < nrchop ? nrchop - m : nrchop;
+  return v > 0 ? v : 0;
+}
+__device__ __forceinline__ double2 zero2() { return make_double2(0.0, 0.0); }
+__device__ __forceinline__ double2 axc(double a, double2 x) { return make_double2(a * x.x, a * x.y); }
+__device__ __forceinline__ double2 add2(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 sub2(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+__device__ __forceinline__ double2 ix(double s, double2 x) {  // (0 + i s) * x
+  return make_double2(0.0 * x.x - s * x.y, 0.0 * x.y + s * x.x);
+}
+
+struct FIdx {
+  int n, mm, k;
+};
+__device__ __forceinline__ FIdx fdecode(const KGrid &g, long idx) {
+  FIdx f;
+  f.n = (int)(idx % g.nrchopdim);
+  long r = idx / g.nrchopdim;
+  f.mm = (int)(r % g.ntchopdim);
+  f.k = (int)(r / g.ntchopdim);
+  return f;
+}
+__device__ __forceinline__ size_t fcol(const KGrid &g, int mm, int k) {
+  return (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
+}
+
+// tri-diagonal band row n of mode mm applied to column x (truncated at nn)
+__device__ __forceinline__ double2 band3_row(const double *bt, const double2 *x, int n, int nn) {
+  const double *r = bt + 3 * n;
+  double2 o = axc(r[1], x[n]);
+  if (n >= 1) o = add2(o, axc(r[0], x[n - 1]));
+  if (n + 1 < nn) o = add2(o, axc(r[2], x[n + 1]));
+  return o;
+}
+// penta-diagonal row
+__device__ __forceinline__ double2 band5_row(const double *bt, const double2 *x, int n, int nn) {
+  const double *r = bt + 5 * n;
+  double2 o = axc(r[2], x[n]);
+  if (n >= 2) o = add2(o, axc(r[0], x[n - 2]));
+  if (n >= 1) o = add2(o, axc(r[1], x[n - 1]));
+  if (n + 1 < nn) o = add2(o, axc(r[3], x[n + 1]));
+  if (n + 2 < nn) o = add2(o, axc(r[4], x[n + 2]));
+  return o;
+}
+
+__host__ __device__ static inline long fff_count(const KGrid &g) { return (long)g.nrchopdim * g.ntchopdim * g.nxchopdim; }
+static int nblocks(long total) {
+  long b = (total + 255) / 256;
+  if (b > 65536) b = 65536;
+  return (int)(b > 0 ? b : 1);
+}
+#define GRID_STRIDE(total) for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < (total); idx += (long)gridDim.x * blockDim.x)
+
+// ---------------------------------------------------------------- basics
+__global__ void k_zero(double2 *a, long n) {
+  GRID_STRIDE(n) a[idx] = zero2();
+}
+__global__ void k_copy(double2 *d, const double2 *s, long n) {
+  GRID_STRIDE(n) d[idx] = s[idx];
+}
+__global__ void k_negate(double2 *a, long n) {
+  GRID_STRIDE(n) a[idx] = make_double2(-a[idx].x, -a[idx].y);
+}
+__global__ void k_scale(double2 *a, double s, long n) {
+  GRID_STRIDE(n) a[idx] = axc(s, a[idx]);
+}
+__global__ void k_set_ln(double *dst, const double *src, double scale, double add) {
+  if (threadIdx.x == 0 && blockIdx.x == 0) dst[0] = (src ? src[0] * scale : 0.0) + add;
+}
+
+int launch_zero(mlg_ctx *c, double2 *a, long n) {
+  hipLaunchKernelGGL(k_zero, dim3(nblocks(n)), dim3(256), 0, c->stream, a, n);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+int launch_copy(mlg_ctx *c, double2 *dst, const double2 *src, long n) {
+  hipLaunchKernelGGL(k_copy, dim3(nblocks(n)), dim3(256), 0, c->stream, dst, src, n);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+int launch_negate(mlg_ctx *c, double2 *a, long n) {
+  hipLaunchKernelGGL(k_negate, dim3(nblocks(n)), dim3(256), 0, c->stream, a, n);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+int launch_scale(mlg_ctx *c, double2 *a, double s, long n) {
+  hipLaunchKernelGGL(k_scale, dim3(nblocks(n)), dim3(256), 0, c->stream, a, s, n);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+int launch_set_ln(mlg_ctx *c, double *dst, const double *src, double scale, double add) {
+  hipLaunchKernelGGL(k_set_ln, dim3(1), dim3(64), 0, c->stream, dst, src, scale, add);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- CHOPDO
+// src/mod_scalar3.f90:213-257.  PFP keeps only m < NTCHOPDIM meaningful.
+__global__ void k_chopdo_fff(KGrid g, double2 *a) {
+  GRID_STRIDE(fff_count(g)) {
+    FIdx f = fdecode(g, idx);
+    int nn = s_nrchops(g.nrchop, f.mm * g.minc);
+    if (f.mm >= g.ntchop || f.n >= nn) a[idx] = zero2();
+  }
+}
+__global__ void k_chopdo_rows(KGrid g, double2 *a, int n2, int n3, int stride2, int zero_slot) {
+  // rows NR..NDIMR-1 for every (j < n2, z < n3); optionally the theta slot `zero_slot`
+  long tot = (long)n2 * n3 * g.ndimr;
+  GRID_STRIDE(tot) {
+    int r = (int)(idx % g.ndimr);
+    long rest = idx / g.ndimr;
+    int j = (int)(rest % n2);
+    int z = (int)(rest / n2);
+    size_t id = (size_t)r + (size_t)g.ndimr * ((size_t)j + (size_t)stride2 * z);
+    if (r >= g.nr || j == zero_slot) a[id] = zero2();
+  }
+}
+
+int launch_chopdo(mlg_ctx *c, double2 *a, int space) {
+  KGrid g = kgrid(c);
+  if (space == MLG_FFF) {
+    hipLaunchKernelGGL(k_chopdo_fff, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, a);
+  } else if (space == MLG_PFF) {
+    long tot = (long)g.ndimr * g.ntchopdim * g.nxchopdim;
+    hipLaunchKernelGGL(k_chopdo_rows, dim3(nblocks(tot)), dim3(256), 0, c->stream, g, a, g.ntchopdim, g.nxchopdim,
+                       g.ntchopdim, -1);
+  } else if (space == MLG_PPP) {
+    long tot = (long)g.ndimr * g.ndimth * g.ndimx;
+    hipLaunchKernelGGL(k_chopdo_rows, dim3(nblocks(tot)), dim3(256), 0, c->stream, g, a, g.ndimth, g.ndimx, g.ndimth,
+                       g.ndimth > g.nth ? g.nth : -1);
+  } else {  // PFP: rows >= NR of the live slots m < NTCHOPDIM
+    long tot = (long)g.ndimr * g.ntchopdim * g.ndimx;
+    hipLaunchKernelGGL(k_chopdo_rows, dim3(nblocks(tot)), dim3(256), 0, c->stream, g, a, g.ntchopdim, g.ndimx, g.ndimth, -1);
+  }
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- DELSQH / IDELSQH
+// src/mod_legops.f90:76-239
+__global__ void k_delsqh(KGrid g, const double2 *a, const double *ln_a, double2 *b, double *ln_b) {
+  GRID_STRIDE(fff_count(g)) {
+    FIdx f = fdecode(g, idx);
+    int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv);
+    double2 v = zero2();
+    if (f.n < nn && f.mm < g.ntchop) {
+      int N = mv + f.n;
+      double cfac = (double)(N * (N + 1)) / g.ell2;
+      double2 x = a[idx];
+      v = make_double2(-x.x * cfac, -x.y * cfac);
+    }
+    if (idx == 0) v = make_double2(ln_a[0] / g.ell2 / g.norm0, 0.0);
+    b[idx] = v;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ln_b[0] = 0.0;
+}
+
+__global__ void k_idelsqh(KGrid g, const double2 *b, double2 *a, double *ln_a) {
+  GRID_STRIDE(fff_count(g)) {
+    FIdx f = fdecode(g, idx);
+    int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv);
+    double2 v = zero2();
+    if (f.n < nn && f.mm < g.ntchop) {
+      int N = mv + f.n;
+      if (N != 0) {
+        double cfac = g.ell2 / (double)(N * (N + 1));
+        double2 x = b[idx];
+        v = make_double2(-x.x * cfac, -x.y * cfac);
+      }
+    }
+    a[idx] = v;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ln_a[0] = b[0].x * g.norm0 * g.ell2;
+}
+
+int launch_delsqh(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, double *ln_b) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_delsqh, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, a, ln_a, b, ln_b);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+int launch_idelsqh(mlg_ctx *c, const double2 *b, double2 *a, double *ln_a) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_idelsqh, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, b, a, ln_a);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- MULXM / XXDX
+// which: 0 = MULXM (:371-428), 1 = XXDX (:680-748, adds LN/N0, LN/N1 at m0k0)
+__global__ void k_band3(KGrid g, const double *bxm, const double *bxxdx, int which, const double2 *a, const double *ln_a,
+                        double2 *b, double *ln_b) {
+  GRID_STRIDE(fff_count(g)) {
+    FIdx f = fdecode(g, idx);
+    int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv);
+    double2 v = zero2();
+    if (f.n < nn && f.mm < g.ntchop) {
+      const double *bt = (which ? bxxdx : bxm) + (size_t)f.mm * g.nband * 3;
+      v = band3_row(bt, a + fcol(g, f.mm, f.k), f.n, nn);
+    }
+    if (which == 1 && f.mm == 0 && f.k == 0) {
+      if (f.n == 0) v.x = v.x + ln_a[0] / g.norm0;
+      if (f.n == 1) v.x = v.x + ln_a[0] / g.norm1;
+    }
+    b[idx] = v;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ln_b[0] = 0.0;
+}
+
+int launch_band3(mlg_ctx *c, int which, const double2 *a, const double *ln_a, double2 *b, double *ln_b) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_band3, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bxm, c->d.bxxdx, which, a, ln_a, b,
+                     ln_b);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- DEL2
+// (:751-820): penta band - k^2, plus the log image 4/3, -2, 2/3 at m0k0.
+__device__ __forceinline__ double2 del2_row(const KGrid &g, const double *bd, const double *ak, const double2 *x, double ln,
+                                            int mm, int k, int n, int nn) {
+  double2 o = band5_row(bd + (size_t)mm * g.nband * 5, x, n, nn);
+  double a = ak[mm + (size_t)g.ntchopdim * k];
+  o = sub2(o, axc(a * a, x[n]));
+  if (mm == 0 && k == 0) {
+    if (n == 0) o.x = o.x + 4.0 / 3 * ln / g.ell2 / g.norm0;
+    if (n == 1) o.x = o.x - 2.0 * ln / g.ell2 / g.norm1;
+    if (n == 2) o.x = o.x + 2.0 / 3 * ln / g.ell2 / g.norm2;
+  }
+  return o;
+}
+
+__global__ void k_del2(KGrid g, const double *bd, const double *ak, const double2 *a, const double *ln_a, double2 *b,
+                       double *ln_b) {
+  const double ln = ln_a[0];
+  GRID_STRIDE(fff_count(g)) {
+    FIdx f = fdecode(g, idx);
+    int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv);
+    double2 v = zero2();
+    if (f.n < nn && f.mm < g.ntchop) v = del2_row(g, bd, ak, a + fcol(g, f.mm, f.k), ln, f.mm, f.k, f.n, nn);
+    b[idx] = v;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) ln_b[0] = 0.0;
+}
+
+int launch_del2(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, double *ln_b) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_del2, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bdel2h, c->d.ak, a, ln_a, b, ln_b);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// DEL2's exactness check (:778-790): the top mode of m=1 at k=0 must vanish.
+__global__ void k_check_top_m1(KGrid g, const double2 *a, int *err) {
+  if (g.ntchopdim < 2) return;
+  int nn = s_nrchops(g.nrchop, g.minc);
+  if (nn < 1) return;
+  double2 v = a[(size_t)(nn - 1) + (size_t)g.nrchopdim * 1];
+  if (hypot(v.x, v.y) > 1.0e-15) atomicCAS(err, 0, 3);
+}
+int launch_check_top_m1(mlg_ctx *c, const double2 *a) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_check_top_m1, dim3(1), dim3(1), 0, c->stream, g, a, c->d.err);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- GRAD
+// (:1727-1786): (r d/dr b, d/dtheta b, d/dz b) = (XXDX b, i m b, i k b)
+__global__ void k_grad(KGrid g, const double *bxxdx, const double *ak, const double2 *a, const double *ln_a, double2 *rr,
+                       double *ln_rr, double2 *rp, double *ln_rp, double2 *z, double *ln_z) {
+  const double ln = ln_a[0];
+  GRID_STRIDE(fff_count(g)) {
+    FIdx f = fdecode(g, idx);
+    int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv);
+    double2 vr = zero2(), vp = zero2(), vz = zero2();
+    if (f.n < nn && f.mm < g.ntchop) {
+      const double2 *col = a + fcol(g, f.mm, f.k);
+      vr = band3_row(bxxdx + (size_t)f.mm * g.nband * 3, col, f.n, nn);
+      double2 x = col[f.n];
+      vp = ix((double)mv, x);
+      vz = ix(ak[f.mm + (size_t)g.ntchopdim * f.k], x);
+    }
+    if (f.mm == 0 && f.k == 0) {
+      if (f.n == 0) vr.x = vr.x + ln / g.norm0;
+      if (f.n == 1) vr.x = vr.x + ln / g.norm1;
+    }
+    rr[idx] = vr;
+    rp[idx] = vp;
+    z[idx] = vz;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ln_rr[0] = 0.0;
+    ln_rp[0] = 0.0;
+    ln_z[0] = 0.0;
+  }
+}
+int launch_grad(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *rr, double *ln_rr, double2 *rp, double *ln_rp,
+                double2 *z, double *ln_z) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_grad, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bxxdx, c->d.ak, a, ln_a, rr, ln_rr,
+                     rp, ln_rp, z, ln_z);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// -(1-x)^2 * [(1-x)^-2 del_h^2 s] as two truncated MULXM passes over
+// DELSQH(s) (with s's log image in row 0 of m0k0): row n of the result.
+__device__ double2 uz_like_row(const KGrid &g, const double *bxm, const double2 *s, double ln, int mm, int k, int n, int nn) {
+  const int mv = mm * g.minc;
+  const double *xt = bxm + (size_t)mm * g.nband * 3;
+  auto dsq = [&](int j) -> double2 {
+    int N = mv + j;
+    double2 x = s[j];
+    double cf = (double)(N * (N + 1)) / g.ell2;
+    double2 v = make_double2(-x.x * cf, -x.y * cf);
+    if (mm == 0 && k == 0 && j == 0) v = make_double2(ln / g.ell2 / g.norm0, 0.0);
+    return v;
+  };
+  auto wrow = [&](int l) -> double2 {  // W = MULXM(DELSQH s), truncated at nn
+    const double *r = xt + 3 * l;
+    double2 o = axc(r[1], dsq(l));
+    if (l >= 1) o = add2(o, axc(r[0], dsq(l - 1)));
+    if (l + 1 < nn) o = add2(o, axc(r[2], dsq(l + 1)));
+    return o;
+  };
+  const double *r = xt + 3 * n;
+  double2 o = axc(r[1], wrow(n));
+  if (n >= 1) o = add2(o, axc(r[0], wrow(n - 1)));
+  if (n + 1 < nn) o = add2(o, axc(r[2], wrow(n + 1)));
+  return make_double2(-o.x, -o.y);
+}
+
+// ---------------------------------------------------------------- PC2VEL
+// (:1789-1853): r u_r = i m psi + i k XXDX chi ; r u_th = -XXDX psi - m k chi ;
+// u_z = -(1-x) (1-x) DELSQH chi.
+__global__ void k_pc2vel(KGrid g, const double *bxm, const double *bxxdx, const double *ak, const double2 *psi,
+                         const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *rur, double2 *rup,
+                         double2 *uz, double *l1, double *l2, double *l3) {
+  const double lp = ln_psi[0], lc = ln_chi[0];
+  GRID_STRIDE(fff_count(g)) {
+    FIdx f = fdecode(g, idx);
+    int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv);
+    double2 vr = zero2(), vp = zero2(), vz = zero2();
+    if (f.n < nn && f.mm < g.ntchop) {
+      const size_t cb = fcol(g, f.mm, f.k);
+      const double *xx = bxxdx + (size_t)f.mm * g.nband * 3;
+      double2 xc = band3_row(xx, chi + cb, f.n, nn);
+      double2 xp = band3_row(xx, psi + cb, f.n, nn);
+      if (f.mm == 0 && f.k == 0) {
+        if (f.n == 0) { xc.x = xc.x + lc / g.norm0; xp.x = xp.x + lp / g.norm0; }
+        if (f.n == 1) { xc.x = xc.x + lc / g.norm1; xp.x = xp.x + lp / g.norm1; }
+      }
+      double kv = ak[f.mm + (size_t)g.ntchopdim * f.k];
+      vr = add2(ix((double)mv, psi[cb + f.n]), ix(kv, xc));
+      vp = sub2(make_double2(-xp.x, -xp.y), axc(mv * kv, chi[cb + f.n]));
+      vz = uz_like_row(g, bxm, chi + cb, lc, f.mm, f.k, f.n, nn);
+    }
+    rur[idx] = vr;
+    rup[idx] = vp;
+    uz[idx] = vz;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    l1[0] = 0.0;
+    l2[0] = 0.0;
+    l3[0] = 0.0;
+  }
+}
+int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *rur,
+                  double2 *rup, double2 *uz, double *ln_rur, double *ln_rup, double *ln_uz) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_pc2vel, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bxm, c->d.bxxdx, c->d.ak, psi,
+                     ln_psi, chi, ln_chi, rur, rup, uz, ln_rur, ln_rup, ln_uz);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- PC2VOR
+// (:1856-1919): with Q = DEL2 chi,
+//   r om_r = -i m Q + i k XXDX psi ; r om_th = XXDX Q - k m psi ;
+//   om_z = -(1-x)(1-x) DELSQH psi.
+__global__ void k_pc2vor(KGrid g, const double *bxm, const double *bxxdx, const double *bd, const double *ak,
+                         const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *ror,
+                         double2 *rop, double2 *oz, double *l1, double *l2, double *l3) {
+  const double lp = ln_psi[0], lc = ln_chi[0];
+  GRID_STRIDE(fff_count(g)) {
+    FIdx f = fdecode(g, idx);
+    int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv);
+    double2 vr = zero2(), vp = zero2(), vz = zero2();
+    if (f.n < nn && f.mm < g.ntchop) {
+      const size_t cb = fcol(g, f.mm, f.k);
+      const double *xx = bxxdx + (size_t)f.mm * g.nband * 3;
+      double kv = ak[f.mm + (size_t)g.ntchopdim * f.k];
+      double2 q = del2_row(g, bd, ak, chi + cb, lc, f.mm, f.k, f.n, nn);
+      double2 xp = band3_row(xx, psi + cb, f.n, nn);
+      if (f.mm == 0 && f.k == 0) {
+        if (f.n == 0) xp.x = xp.x + lp / g.norm0;
+        if (f.n == 1) xp.x = xp.x + lp / g.norm1;
+      }
+      double2 imq = ix((double)mv, q);
+      vr = add2(make_double2(-imq.x, -imq.y), ix(kv, xp));
+      // XXDX of Q (Q carries no log term)
+      const double *r = xx + 3 * f.n;
+      double2 xq = axc(r[1], q);
+      if (f.n >= 1) xq = add2(xq, axc(r[0], del2_row(g, bd, ak, chi + cb, lc, f.mm, f.k, f.n - 1, nn)));
+      if (f.n + 1 < nn) xq = add2(xq, axc(r[2], del2_row(g, bd, ak, chi + cb, lc, f.mm, f.k, f.n + 1, nn)));
+      vp = sub2(xq, axc(kv * mv, psi[cb + f.n]));
+      vz = uz_like_row(g, bxm, psi + cb, lp, f.mm, f.k, f.n, nn);
+    }
+    ror[idx] = vr;
+    rop[idx] = vp;
+    oz[idx] = vz;
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    l1[0] = 0.0;
+    l2[0] = 0.0;
+    l3[0] = 0.0;
+  }
+}
+int launch_pc2vor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *ror,
+                  double2 *rop, double2 *oz, double *ln_ror, double *ln_rop, double *ln_oz) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_pc2vor, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bxm, c->d.bxxdx, c->d.bdel2h,
+                     c->d.ak, psi, ln_psi, chi, ln_chi, ror, rop, oz, ln_ror, ln_rop, ln_oz);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- value at infinity
+// A(0,0,k) -= sum_n AT1(n) A(n,0,k) / AT1(0)   (IDELSQH :222, IDEL2 :987-992,
+// IDEL2LN :1154-1159, PROJECT :2541-2543)
+__global__ void k_fix_inf(KGrid g, const double *at1, double2 *a) {
+  int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.nxchopdim) return;
+  const size_t cb = (size_t)g.nrchopdim * (size_t)g.ntchopdim * k;
+  double2 s = zero2();
+  for (int n = 0; n < g.nrchop; n++) s = add2(s, axc(at1[n], a[cb + n]));
+  a[cb] = sub2(a[cb], make_double2(s.x / g.at1_0, s.y / g.at1_0));
+}
+int launch_fix_at_infinity(mlg_ctx *c, double2 *a) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_fix_inf, dim3((g.nxchopdim + 63) / 64), dim3(64), 0, c->stream, g, c->d.at1, a);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- banded solves
+// LUDCMPB + LUBKSBB (src/mod_bandmat.f90:65-156,320-402) for one system per
+// thread, streamed: a window of the c0+1 rows below the pivot is kept in
+// registers, forward substitution follows the factorisation row by row, the
+// normalised upper rows go to scratch for the back substitution.  No
+// pivoting, 1/diagonal stored, same update order as the reference.
+struct SolveCtx {
+  KGrid g;
+  const double *bd, *bxm, *ak;
+  double alp;
+  int op;      // 0 IDEL2, 1 IDEL2LN, 2 IHELM
+  int kind;    // 0 penta, 1 IDEL2 m0k0 shifted (7-band), 2 IHELM m=0 product (7-band)
+  int mm, k, np;
+  double k2;
+};
+
+// original matrix row `q` as NE band entries (centre c0)
+__device__ void gen_row(const SolveCtx &s, int q, double *out, int NE) {
+  const KGrid &g = s.g;
+  for (int b = 0; b < NE; b++) out[b] = 0.0;
+  if (s.kind == 0) {
+    const double *r = s.bd + ((size_t)s.mm * g.nband + q) * 5;
+    out[0] = r[0];
+    out[1] = r[1];
+    out[2] = (r[2] - s.alp) - s.k2;
+    out[3] = r[3];
+    out[4] = r[4];
+    if (s.mm == 0 && s.k == 0 && s.op == 1) {
+      if (q == 0) out[2] = out[2] + 4.0 / 3 / g.ell2;
+      if (q == 1) out[1] = out[1] - 2 * g.norm0 / g.norm1 / g.ell2;
+      if (q == 2) out[0] = out[0] + 2.0 / 3 * g.norm0 / g.norm2 / g.ell2;
+    }
+  } else if (s.kind == 1) {
+    if (q == 0) {
+      out[3] = 1.0;
+    } else {
+      int p = q - 1;
+      const double *r = s.bd + ((size_t)s.mm * g.nband + p) * 5;
+      out[0] = r[0];
+      out[1] = r[1];
+      out[2] = r[2] - s.k2;
+      out[3] = r[3];
+      out[4] = r[4];
+      if (p == 0) out[2] = out[2] + 4.0 / 3 / g.ell2;
+      if (p == 1) out[1] = out[1] - 2 * g.norm0 / g.norm1 / g.ell2;
+      if (p == 2) out[0] = out[0] + 2.0 / 3 * g.norm0 / g.norm2 / g.ell2;
+    }
+  } else {
+    // (H5 * XM) row q, both truncated to np x np (BANBANMUL order)
+    const double *h = s.bd + ((size_t)s.mm * g.nband + q) * 5;
+    double h5[5] = {h[0], h[1], (h[2] - s.alp) - s.k2, h[3], h[4]};
+    for (int I = 0; I < 5; I++) {
+      int rr = q + I - 2;
+      if (rr < 0 || rr >= s.np) continue;
+      const double *xm = s.bxm + ((size_t)s.mm * g.nband + rr) * 3;
+      for (int J = 0; J < 3; J++) {
+        int cc = rr + J - 1;
+        if (cc < 0 || cc >= s.np) continue;
+        out[I + J] += h5[I] * xm[J];
+      }
+    }
+  }
+}
+
+template <int NE, int C0>
+__device__ void band_lu_solve(const SolveCtx &s, int nd, double2 *x, double *ust, long nsys, long sys) {
+  constexpr int KU = NE - 1 - C0;
+  double W[C0 + 1][NE];
+  for (int q = 0; q <= C0; q++) {
+    if (q < nd) gen_row(s, q, W[q], NE);
+    else
+      for (int b = 0; b < NE; b++) W[q][b] = 0.0;
+  }
+  double2 xh[C0 + 1];  // forward-substituted history x[nn-1], x[nn-2], ...
+  for (int q = 0; q <= C0; q++) xh[q] = zero2();
+  for (int nn = 0; nn < nd; nn++) {
+    const double piv = W[0][C0];
+    for (int b = C0 + 1; b < NE; b++)
+      if (nn + (b - C0) < nd) W[0][b] /= piv;
+    for (int b = C0 + 1; b < NE; b++)
+      for (int j = 1; j <= C0; j++) {
+        int kk = (b - C0) > j ? (b - C0) : j;
+        if (nn + kk < nd) W[j][b - j] -= W[0][b] * W[j][C0 - j];
+      }
+    for (int j = 1; j <= C0; j++)
+      if (nn + j < nd) W[j][C0 - j] /= piv;
+    // forward substitution of row nn (its multipliers were final before this pivot)
+    double2 v = x[nn];
+    int nne = nn < C0 ? nn : C0;
+    for (int j = 1; j <= nne; j++) v = sub2(v, axc(W[0][C0 - j], xh[j - 1]));
+    for (int q = C0; q >= 1; q--) xh[q] = xh[q - 1];
+    xh[0] = v;
+    const double rp = 1.0 / piv;
+    x[nn] = axc(rp, v);
+    for (int u = 0; u < KU; u++) ust[((size_t)nn * KU + u) * nsys + sys] = W[0][C0 + 1 + u];
+    for (int q = 0; q < C0; q++)
+      for (int b = 0; b < NE; b++) W[q][b] = W[q + 1][b];
+    if (nn + C0 + 1 < nd) gen_row(s, nn + C0 + 1, W[C0], NE);
+    else
+      for (int b = 0; b < NE; b++) W[C0][b] = 0.0;
+  }
+  double2 yh[KU];
+  for (int u = 0; u < KU; u++) yh[u] = zero2();
+  for (int nn = nd - 1; nn >= 0; nn--) {
+    double2 v = x[nn];
+    if (nn <= nd - 2)
+      for (int i = 1; i <= KU; i++)
+        if (nn + i < nd) v = sub2(v, axc(ust[((size_t)nn * KU + (i - 1)) * nsys + sys], yh[i - 1]));
+    x[nn] = v;
+    for (int u = KU - 1; u >= 1; u--) yh[u] = yh[u - 1];
+    yh[0] = v;
+  }
+}
+
+__global__ void __launch_bounds__(64) k_band_solve(KGrid g, const double *bd, const double *bxm, const double *ak, int op,
+                                                   double alp, int has_ln, double lnval, double2 *a, double *ln_a,
+                                                   const double *ln_b, double *ust) {
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  const long sys = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const int mm = (int)(sys % g.ntchopdim), k = (int)(sys / g.ntchopdim);
+  const int mv = mm * g.minc, nn = s_nrchops(g.nrchop, mv);
+  if (nn < 1 || mm >= g.ntchop) return;
+  double2 *x = a + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
+  SolveCtx s;
+  s.g = g;
+  s.bd = bd;
+  s.bxm = bxm;
+  s.ak = ak;
+  s.op = op;
+  s.mm = mm;
+  s.k = k;
+  s.np = nn + 1;
+  double akv = ak[mm + (size_t)g.ntchopdim * k];
+  s.k2 = akv * akv;
+  s.alp = (op == 2) ? alp : 0.0;
+  s.kind = 0;
+  if (mm == 0) {
+    if (op == 0 && k == 0) s.kind = 1;
+    if (op == 2) s.kind = 2;
+  }
+  if (s.kind == 0) {
+    band_lu_solve<5, 2>(s, nn, x, ust, nsys, sys);
+  } else if (s.kind == 1) {
+    // IDEL2 m0k0: first row is the log-term (Dirichlet) row
+    for (int i = nn - 1; i >= 1; i--) x[i] = x[i - 1];
+    x[0] = make_double2(has_ln ? lnval / g.norm0 : 0.0, 0.0);
+    band_lu_solve<7, 3>(s, nn, x, ust, nsys, sys);
+  } else {
+    // IHELM m=0: solve (del^2 - alp - k^2)(1-x) y = rhs on NN-1 rows, A = (1-x) y
+    if (k == 0) {
+      double lnA = -ln_b[0] / alp;
+      ln_a[0] = lnA;
+      x[0].x = x[0].x - lnA * (4.0 / 3 / g.ell2) / g.norm0;
+      x[1].x = x[1].x - lnA * (-2 / g.norm1 / g.ell2);
+      x[2].x = x[2].x - lnA * (+2.0 / 3 / g.norm2 / g.ell2);
+    }
+    int nm = nn - 1;
+    band_lu_solve<7, 3>(s, nm, x, ust, nsys, sys);
+    x[nn - 1] = zero2();
+    const double *xt = bxm;
+    double2 prev = zero2();
+    for (int c = 0; c < nn; c++) {
+      const double *r = xt + ((size_t)mm * g.nband + c) * 3;
+      double2 cur = x[c];
+      double2 o = axc(r[1], cur);
+      if (c >= 1) o = add2(o, axc(r[0], prev));
+      if (c + 1 < nn) o = add2(o, axc(r[2], x[c + 1]));
+      prev = cur;
+      x[c] = o;
+    }
+  }
+  if (mm == 0 && k == 0) {
+    if (op == 0) ln_a[0] = has_ln ? x[0].x * g.norm0 : 0.0;
+    if (op == 1) {
+      ln_a[0] = x[0].x * g.norm0;
+      x[0] = zero2();
+    }
+  }
+}
+
+int launch_band_solve(mlg_ctx *c, int op, double2 *a, double *ln_a, const double *ln_b, double alp, int has_ln, double ln) {
+  KGrid g = kgrid(c);
+  long nsys = (long)g.ntchopdim * g.nxchopdim;
+  size_t need = (size_t)(g.nrchopdim + 2) * 3 * nsys;
+  if (need > c->d.lu_len) return MLG_E_ARG;
+  KScope ks(c, K_BAND_SOLVE, 2.0 * 16.0 * fff_count(g));  // the coefficients read and written once
+  hipLaunchKernelGGL(k_band_solve, dim3((nsys + 63) / 64), dim3(64), 0, c->stream, g, c->d.bdel2h, c->d.bxm, c->d.ak, op, alp,
+                     has_ln, ln, a, ln_a, ln_b, c->d.lu);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ---------------------------------------------------------------- pointwise updates
+// Pointwise FFF updates with the reference's evaluation orders:
+//   mode 0: y = x1 + a*x2                      (Richardson / Euler sub-steps)
+//   mode 1: y = x1 + a*(1.5*x2 - 0.5*x3)       (AB2 update)
+//   mode 2: y = 2*x1 - y                       (Richardson extrapolation)
+//   mode 3: y = a*(x1 - x2)                    (pressure history)
+//   mode 4: y = x1 - x2
+//   mode 5: y = -y - a*x1                      (Crank-Nicolson right-hand side)
+__global__ void k_axpby(int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n) {
+  GRID_STRIDE(n) {
+    double2 v;
+    if (mode == 0) v = add2(x1[idx], axc(a, x2[idx]));
+    else if (mode == 1) v = add2(x1[idx], axc(a, sub2(axc(1.5, x2[idx]), axc(0.5, x3[idx]))));
+    else if (mode == 2) v = sub2(axc(2.0, x1[idx]), y[idx]);
+    else if (mode == 3) v = axc(a, sub2(x1[idx], x2[idx]));
+    else if (mode == 4) v = sub2(x1[idx], x2[idx]);
+    else v = sub2(make_double2(-y[idx].x, -y[idx].y), axc(a, x1[idx]));
+    y[idx] = v;
+  }
+}
+__global__ void k_ln_axpby(int mode, double *y, double a, const double *x1, const double *x2, const double *x3) {
+  if (threadIdx.x != 0 || blockIdx.x != 0) return;
+  double v;
+  if (mode == 0) v = x1[0] + a * x2[0];
+  else if (mode == 1) v = x1[0] + a * (1.5 * x2[0] - 0.5 * x3[0]);
+  else if (mode == 2) v = 2.0 * x1[0] - y[0];
+  else if (mode == 3) v = a * (x1[0] - x2[0]);
+  else v = x1[0] - x2[0];
+  y[0] = v;
+}
+int launch_axpby(mlg_ctx *c, int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n) {
+  hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(256), 0, c->stream, mode, y, a, x1, x2, x3, n);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1, const double *x2, const double *x3) {
+  hipLaunchKernelGGL(k_ln_axpby, dim3(1), dim3(64), 0, c->stream, mode, y, a, x1, x2, x3);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// ETD update (CALC_BOUSSI_ETDFE_SUB / _ETDAB_SUB, src/mod_march.f90:948-1139)
+struct Mat4x2 {
+  double e[16], nl[16];
+};
+__global__ void k_etd_sub(KGrid g, Mat4x2 M, double dt, double2 *rur, double2 *rup, double2 *uz, double2 *b, const double2 *nr_,
+                          const double2 *np_, const double2 *nz_, const double2 *nb_, const double2 *hpr, const double2 *hpp,
+                          const double2 *hpz, const double2 *hnr, const double2 *hnp, const double2 *hnz, const double2 *hnb,
+                          int ab) {
+  GRID_STRIDE(fff_count(g)) {
+    FIdx f = fdecode(g, idx);
+    double er[16], ar[16];
+    for (int q = 0; q < 16; q++) {
+      er[q] = M.e[q];
+      ar[q] = M.nl[q];
+    }
+    double notk0 = 1.0, notm0 = (f.mm == 0) ? 0.0 : 1.0;
+    if (f.k == 0) {
+      notk0 = 0.0;
+      for (int j = 0; j < 4; j++) {
+        er[0 + 4 * j] = 0.0;
+        er[1 + 4 * j] = 0.0;
+        ar[0 + 4 * j] = 0.0;
+        ar[1 + 4 * j] = 0.0;
+      }
+      er[0] = 1.0;
+      er[5] = 1.0;
+      ar[0] = dt;
+      ar[5] = dt;
+    }
+    double2 ev[4], av[4];
+    if (ab) {
+      ev[0] = sub2(rur[idx], hpr[idx]);
+      ev[1] = sub2(rup[idx], axc(notm0, hpp[idx]));
+      ev[2] = sub2(uz[idx], axc(notk0, hpz[idx]));
+      ev[3] = b[idx];
+      av[0] = sub2(axc(1.5, nr_[idx]), axc(0.5, hnr[idx]));
+      av[1] = sub2(axc(1.5, np_[idx]), axc(0.5, hnp[idx]));
+      av[2] = sub2(axc(1.5, nz_[idx]), axc(0.5, hnz[idx]));
+      av[3] = sub2(axc(1.5, nb_[idx]), axc(0.5, hnb[idx]));
+    } else {
+      ev[0] = rur[idx];
+      ev[1] = rup[idx];
+      ev[2] = uz[idx];
+      ev[3] = b[idx];
+      av[0] = nr_[idx];
+      av[1] = np_[idx];
+      av[2] = nz_[idx];
+      av[3] = nb_[idx];
+    }
+    double2 cv[4];
+    for (int i = 0; i < 4; i++) {
+      double2 s1 = zero2(), s2 = zero2();
+      for (int j = 0; j < 4; j++) {
+        s1 = add2(s1, axc(er[i + 4 * j], ev[j]));
+        s2 = add2(s2, axc(ar[i + 4 * j], av[j]));
+      }
+      cv[i] = add2(s1, s2);
+    }
+    rur[idx] = cv[0];
+    rup[idx] = cv[1];
+    uz[idx] = cv[2];
+    b[idx] = cv[3];
+  }
+}
+int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt, double2 *rur, double2 *rup, double2 *uz,
+                   double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_, const double2 *hpr,
+                   const double2 *hpp, const double2 *hpz, const double2 *hnr, const double2 *hnp, const double2 *hnz,
+                   const double2 *hnb, int ab) {
+  KGrid g = kgrid(c);
+  Mat4x2 M;
+  for (int q = 0; q < 16; q++) {
+    M.e[q] = e16[q];
+    M.nl[q] = nl16[q];
+  }
+  KScope ks(c, K_ETD, 16.0 * fff_count(g) * (ab ? 19 : 12));  // 8 (15 with history) in, 4 out
+  hipLaunchKernelGGL(k_etd_sub, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, M, dt, rur, rup, uz, b, nr_, np_, nz_,
+                     nb_, hpr, hpp, hpz, hnr, hnp, hnz, hnb, ab);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// HYPERV3 / CALC_BOUSSI_ETD_HYPERV filter (src/mod_march.f90:1352-1416,3521-3556)
+__global__ void k_hyp3(KGrid g, int nrc, const double *hr, const double *ht, const double *hx, double tf, double2 *a) {
+  GRID_STRIDE(fff_count(g)) {
+    FIdx f = fdecode(g, idx);
+    int nn = s_nrchops(g.nrchop, f.mm * g.minc);
+    if (f.n < nn && f.mm < g.ntchop) {
+      double h = hr[f.n + (size_t)nrc * f.mm] * ht[f.mm] * hx[f.k];
+      a[idx] = axc(pow(h, tf), a[idx]);
+    } else {
+      a[idx] = zero2();
+    }
+  }
+}
+int launch_hyp3(mlg_ctx *c, double2 *a, int diff, double tf) {
+  KGrid g = kgrid(c);
+  const mlg::Grid &G = c->g;
+  int nrc = G.p.nrchop;
+  size_t nr_ = (size_t)nrc * G.ntchopdim, nt = G.ntchopdim, nx = G.nxchopdim;
+  const double *base = c->d.hv + (diff ? (nr_ + nt + nx) : 0);
+  hipLaunchKernelGGL(k_hyp3, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, nrc, base, base + nr_, base + nr_ + nt, tf,
+                     a);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// DIAGNOST per-step checks (src/mod_march.f90:1750-1791): dealiasing of psi
+// beyond NRCHOPS at m = 0, 1 (error 11) and |psi| > 100 in the low modes
+// n < 5, 1 <= m < 6, k < 5 (error 12).
+__global__ void __launch_bounds__(256) k_diagnost(KGrid g, const double2 *psi, int *err) {
+  __shared__ double m0[256], m1[256];
+  __shared__ int div;
+  if (threadIdx.x == 0) div = 0;
+  double a0 = -1e300, a1 = -1e300;
+  const int n0 = s_nrchops(g.nrchop, 0), n1 = s_nrchops(g.nrchop, g.minc);
+  long tot = (long)g.nrchopdim * g.nxchopdim;
+  for (long i = threadIdx.x; i < tot; i += blockDim.x) {
+    int n = (int)(i % g.nrchopdim), k = (int)(i / g.nrchopdim);
+    if (n >= n0) {
+      double2 v = psi[(size_t)n + (size_t)g.nrchopdim * (size_t)g.ntchopdim * k];
+      a0 = fmax(a0, hypot(v.x, v.y));
+    }
+    if (g.ntchopdim >= 2 && n >= n1) {
+      double2 v = psi[(size_t)n + (size_t)g.nrchopdim * (1 + (size_t)g.ntchopdim * k)];
+      a1 = fmax(a1, hypot(v.x, v.y));
+    }
+  }
+  m0[threadIdx.x] = a0;
+  m1[threadIdx.x] = a1;
+  __syncthreads();
+  int kmax = g.nxchopdim < 5 ? g.nxchopdim : 5, mmax = g.ntchopdim < 6 ? g.ntchopdim : 6, nmax = g.nrchop < 5 ? g.nrchop : 5;
+  for (int i = threadIdx.x; i < kmax * mmax * nmax; i += blockDim.x) {
+    int n = i % nmax, mm = (i / nmax) % mmax, k = i / (nmax * mmax);
+    if (mm < 1) continue;
+    double2 v = psi[(size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k)];
+    if (hypot(v.x, v.y) > 100.) div = 1;
+  }
+  for (int s = 128; s > 0; s >>= 1) {
+    if (threadIdx.x < s) {
+      m0[threadIdx.x] = fmax(m0[threadIdx.x], m0[threadIdx.x + s]);
+      m1[threadIdx.x] = fmax(m1[threadIdx.x], m1[threadIdx.x + s]);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) {
+    double chk = (g.ntchopdim >= 2) ? (m0[0] + m1[0]) : m0[0];
+    if (chk > 1.0e-12) atomicCAS(err, 0, 11);
+    else if (div) atomicCAS(err, 0, 12);
+  }
+}
+int launch_diagnost(mlg_ctx *c, const double2 *psi) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_diagnost, dim3(1), dim3(256), 0, c->stream, g, psi, c->d.err);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// PRODCT_MK_ALLK / L2NORM_MK_ALLK (src/mod_scalar3.f90:470-526,750-803):
+// per (m, k) Gauss sum over r in PFF, x 4 pi ZLEN L^2, m = 0 halved.
+__global__ void __launch_bounds__(64) k_energy_terms(KGrid g, double fac, const double *w, const double *x, const double2 *a,
+                                                     const double2 *b, int l2, double *out) {
+  const int mm = blockIdx.x, k = blockIdx.y;
+  const size_t cb = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
+  double s = 0.0;
+  for (int r = threadIdx.x; r < g.nr; r += 64) {
+    double2 av = a[cb + r];
+    double wk;
+    if (l2) {
+      double ab = hypot(av.x, av.y), om = 1 - x[r];
+      wk = ab * ab / (om * om);
+    } else {
+      double2 bv = b[cb + r];
+      wk = av.x * bv.x + av.y * bv.y;
+    }
+    s += wk * w[r];
+  }
+  for (int off = 32; off > 0; off >>= 1) s += __shfl_down(s, off, 64);
+  if (threadIdx.x == 0) out[mm + (size_t)g.ntchop * k] = (mm == 0 ? 0.5 : 1.0) * (fac * s);
+}
+int launch_energy_terms(mlg_ctx *c, const double2 *a, const double2 *b, int l2, double *out_all) {
+  KGrid g = kgrid(c);
+  const double pi = 3.14159265358979323846;
+  double fac = 4.0 * pi * c->g.zlen * c->g.ell2;
+  hipLaunchKernelGGL(k_energy_terms, dim3(g.ntchopdim, g.nxchopdim), dim3(64), 0, c->stream, g, fac, c->d.w, c->d.x, a, b, l2,
+                     out_all);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace mlg

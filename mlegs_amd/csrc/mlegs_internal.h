@@ -1,0 +1,192 @@
+// Internal structures of libmlegs_hip.so (not part of the C ABI).
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <unordered_set>
+#include <vector>
+
+#include "mlegs_hip.h"
+
+namespace mlg {
+
+// Host-side grid and tables (LEGINIT, src/mod_fft.f90:180-367).
+struct Grid {
+  mlg_params p;
+  int nr, nrh, nth, nx, ntchop, nxchop;
+  double ell, ell2, zlen;
+  int ndimr, ndimth, ndimx, nrchopdim, ntchopdim, nxchopdim, nxchoph;
+  int nrchop;  // current truncation (CHOPSET state)
+  int npf, nnorm, nband;
+  std::vector<int> m;
+  std::vector<double> ak, x, w, r, lnx, norm, lognorm, pf, at0, at1;
+  std::vector<double> bxm, bxxdx, bdel2h;  // [mm][row][3|5]
+  std::vector<double> vdt;                 // [3][mm][n][j]
+  std::vector<double> hv_r, hv_t, hv_x, hd_r, hd_t, hd_x;
+};
+
+int build_grid(const mlg_params &p, Grid &g);
+void twiddles(int n, int sign, double *re, double *im);
+void band_xm(int ni, int m, const double *ln, double *b);
+void band_xxdx(int ni, int m, const double *ln, double *b);
+void band_del2h(int ni, int m, double ell, const double *ln, double *b);
+
+// Device copies of the tables and scratch.
+struct DevTables {
+  double *pf = nullptr;       // [mm][n][i]  (i fastest), npf rows
+  double *vdt = nullptr;      // [3][mm][n][j]
+  double *w = nullptr, *r = nullptr, *x = nullptr, *lnx = nullptr;
+  double *at1 = nullptr, *ak = nullptr, *norm = nullptr, *lognorm = nullptr;
+  double *bxm = nullptr, *bxxdx = nullptr, *bdel2h = nullptr;
+  double2 *tw_z = nullptr;    // e^{-2 pi i j / NX}, j < NX
+  double2 *tw_t = nullptr;    // e^{-2 pi i j / NTH}, j < NTH
+  double2 *tw_h = nullptr;    // e^{-2 pi i j / (2 NTH)}, j <= NTH
+  double *hv = nullptr;       // [hv_r | hv_t | hv_x | hd_r | hd_t | hd_x]
+  double *lu = nullptr;       // band-solver factor storage
+  size_t lu_len = 0;
+  double *red = nullptr;      // reduction scratch
+  int *err = nullptr;         // sticky device error code
+};
+
+}  // namespace mlg
+
+struct mlg_field {
+  mlg_ctx *ctx;
+  double2 *e;   // device elements (sized for the largest space)
+  double *ln;   // device log-term coefficient
+  int space;
+  bool pooled;
+};
+
+namespace mlg {
+// Kernel classes timed by the optional event timer.
+enum KernelId {
+  K_RTRAN_BWD = 0, K_RTRAN_FWD, K_PROJECT, K_VERFFT_FWD, K_VERFFT_BWD, K_HORFFT_FWD, K_HORFFT_BWD,
+  K_PRODUCTS, K_BAND_SOLVE, K_ETD, K_SPECTRAL, K_COUNT
+};
+extern const char *const kernel_names[K_COUNT];
+extern const char *const kernel_units[K_COUNT];  // "flop" or "byte"
+struct EventPair {
+  hipEvent_t a, b;
+  double work;
+};
+struct KTimer {
+  bool on = false;
+  std::vector<EventPair> rec[K_COUNT];
+  std::vector<hipEvent_t> spare;
+};
+}  // namespace mlg
+
+struct mlg_ctx {
+  mlg::Grid g;
+  mlg::KTimer timer;
+  mlg::DevTables d;
+  hipStream_t stream = nullptr;
+  hipStream_t own_stream = nullptr;
+  long n_fff = 0, n_pff = 0, n_ppp = 0, n_max = 0;
+  std::vector<mlg_field *> pool;  // reusable temporaries
+  std::unordered_set<mlg_field *> owned;  // fields created through the C ABI
+  int device = 0;
+};
+
+struct mlg_solver {
+  mlg_ctx *ctx;
+  double t = 0.0;
+  int n = 0;
+  mlg_field *n_psi = nullptr, *n_chi = nullptr, *n_b = nullptr;
+  mlg_field *r_n_r = nullptr, *r_n_p = nullptr, *n_z = nullptr;
+  mlg_field *r_pi_r = nullptr, *r_pi_p = nullptr, *pi_z = nullptr;
+  double etd_e[16], etd_nl[16];
+};
+
+// Kernel-side view of the current grid (passed by value to kernels).
+struct KGrid {
+  int nr, nrh, nth, nx, ntchop, nxchop;
+  int ndimr, ndimth, ndimx, nrchopdim, ntchopdim, nxchopdim, nxchoph;
+  int nrchop, npf, nband, minc;
+  double ell2, norm0, norm1, norm2, at1_0;
+};
+
+#define MLG_CHECK_HIP(expr)                                                                 \
+  do {                                                                                      \
+    hipError_t _e = (expr);                                                                 \
+    if (_e != hipSuccess) {                                                                 \
+      std::fprintf(stderr, "mlegs_hip: %s failed: %s (%s:%d)\n", #expr, hipGetErrorString(_e), \
+                   __FILE__, __LINE__);                                                     \
+      return MLG_E_HIP;                                                                     \
+    }                                                                                       \
+  } while (0)
+
+namespace mlg {
+
+KGrid kgrid(const mlg_ctx *c);
+
+// Records HIP events around one launch when the context's timer is on.
+// `work` is the launch's algorithmic flops or bytes.
+struct KScope {
+  mlg_ctx *c;
+  int id;
+  hipEvent_t a = nullptr, b = nullptr;
+  double work;
+  KScope(mlg_ctx *c_, int id_, double work_);
+  ~KScope();
+};
+// sum over m of NRCHOPS(m) at the current truncation
+long sum_nrchops(const mlg_ctx *c);
+
+static inline int nrchops_of(int nrchop, int m) {
+  int v = nrchop - m < nrchop ? nrchop - m : nrchop;
+  return v > 0 ? v : 0;
+}
+
+// ---- launchers: Legendre transforms (k_legendre.hip)
+int launch_rtran_bwd(mlg_ctx *c, const double2 *in, const double *ln_in, double2 *out);
+int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out);
+int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *psi, double2 *chi);
+// ---- FFTs (k_fft.hip)
+int launch_verfft_fwd(mlg_ctx *c, const double2 *in, double2 *out);
+int launch_verfft_bwd(mlg_ctx *c, const double2 *in, double2 *out);
+int launch_horfft_fwd(mlg_ctx *c, double2 *a);
+int launch_horfft_bwd(mlg_ctx *c, double2 *a);
+// ---- physical-space products (k_physical.hip)
+int launch_vprod(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *ror, double2 *rop, double2 *oz);
+int launch_vdot(mlg_ctx *c, const double2 *rar, const double2 *rap, const double2 *az, const double2 *rbr,
+                const double2 *rbp, const double2 *bz, double2 *out, double sign);
+int launch_force_products(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *ror, double2 *rop,
+                          double2 *oz, const double2 *rbr, const double2 *rbp, const double2 *bz, const double2 *bphys,
+                          double2 *bn, double two_omega, double n2, int linear);
+// ln_mode: 0 psi LN = the computed value, 1 psi LN = ln, 2 psi LN unchanged
+int launch_vel2pln(mlg_ctx *c, double2 *rup, double *psi_ln, int ln_mode, double ln);
+// ---- spectral-space kernels (k_spectral.hip)
+int launch_chopdo(mlg_ctx *c, double2 *a, int space);
+int launch_zero(mlg_ctx *c, double2 *a, long n);
+int launch_copy(mlg_ctx *c, double2 *dst, const double2 *src, long n);
+int launch_set_ln(mlg_ctx *c, double *dst, const double *src, double scale, double add);
+int launch_delsqh(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, double *ln_b);
+int launch_idelsqh(mlg_ctx *c, const double2 *b, double2 *a, double *ln_a);
+int launch_band3(mlg_ctx *c, int which, const double2 *a, const double *ln_a, double2 *b, double *ln_b);
+int launch_del2(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, double *ln_b);
+int launch_grad(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *rr, double *ln_rr, double2 *rp, double *ln_rp,
+                double2 *z, double *ln_z);
+int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
+                  double2 *rur, double2 *rup, double2 *uz, double *ln_rur, double *ln_rup, double *ln_uz);
+int launch_pc2vor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
+                  double2 *ror, double2 *rop, double2 *oz, double *ln_ror, double *ln_rop, double *ln_oz);
+int launch_fix_at_infinity(mlg_ctx *c, double2 *a);
+// op: 0 IDEL2, 1 IDEL2LN, 2 IHELM
+int launch_band_solve(mlg_ctx *c, int op, double2 *a, double *ln_a, const double *ln_b, double alp, int has_ln, double ln);
+int launch_axpby(mlg_ctx *c, int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n);
+int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1, const double *x2, const double *x3);
+int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt, double2 *rur, double2 *rup, double2 *uz,
+                   double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_,
+                   const double2 *hpr, const double2 *hpp, const double2 *hpz, const double2 *hnr, const double2 *hnp,
+                   const double2 *hnz, const double2 *hnb, int ab);
+int launch_hyp3(mlg_ctx *c, double2 *a, int diff, double tf);
+int launch_diagnost(mlg_ctx *c, const double2 *psi);
+int launch_energy_terms(mlg_ctx *c, const double2 *a, const double2 *b, int l2, double *out_all);
+int launch_negate(mlg_ctx *c, double2 *a, long n);
+int launch_scale(mlg_ctx *c, double2 *a, double s, long n);
+int launch_check_top_m1(mlg_ctx *c, const double2 *a);
+
+}  // namespace mlg

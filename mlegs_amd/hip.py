@@ -1,0 +1,480 @@
+"""Python interface to libmlegs_hip.so (ctypes over the C ABI in include/mlegs_hip.h).
+
+The names mirror the reference's Fortran module procedures so that a caller
+of MLegS finds the same operations with the same argument meaning:
+
+* ``Grid``   -- LEGINIT / FINALIZE (src/mod_fft.f90:165-367), the CHOPSET
+  truncation state (src/mod_scalar3.f90:174-195).
+* ``Scalar`` -- TYPE(SCALAR) (src/mod_scalar3.f90:24-74): a device field with
+  its space tag and log-term coefficient LN.
+* transforms, operators and Boussinesq terms as free functions
+  (src/mod_fft.f90, src/mod_legops.f90, src/mod_boussinesq.f90).
+* ``Solver`` -- SOLVER_T INITIALIZE / TIME_STEPPING (src/mod_march.f90:171-307).
+
+Errors: every call checks the returned status and raises ``MlegsError``
+with the reference's error code (src/mod_scalar3.f90:88-106); the reference
+aborts at the same points.  There is no CPU fallback: if the HIP library is
+missing this module fails at import of the first call.
+"""
+import ctypes as C
+import os
+
+import numpy as np
+
+PPP, FFF, PFP, PFF = 0, 1, 2, 3
+SPACE_NAMES = {PPP: "PPP", FFF: "FFF", PFP: "PFP", PFF: "PFF"}
+
+LIB_PATH = os.path.join(os.path.dirname(os.path.abspath(__file__)), "lib", "libmlegs_hip.so")
+
+
+class MlegsError(RuntimeError):
+    """A non-zero status from the library (reference error codes, 100+ for runtime errors)."""
+
+    def __init__(self, code, where):
+        self.code = int(code)
+        msg = _lib().mlg_error_string(self.code)
+        name = msg.decode() if msg else "unknown"
+        super().__init__(f"{where}: error {self.code} ({name})")
+
+
+class Params(C.Structure):
+    """mlg_params: read.input's DIMENSION/CHOPS/ELL/ZLEN/DT/VISCOSITY/BOUSSINESQ blocks."""
+
+    _fields_ = [
+        ("nr", C.c_int), ("nth", C.c_int), ("nx", C.c_int),
+        ("nrchop", C.c_int), ("ntchop", C.c_int), ("nxchop", C.c_int),
+        ("minc", C.c_int), ("mklink", C.c_int),
+        ("ell", C.c_double), ("zlen", C.c_double),
+        ("nu", C.c_double), ("kappa", C.c_double), ("nup", C.c_double), ("kappap", C.c_double),
+        ("visc_sw", C.c_int), ("visc_p", C.c_int),
+        ("omega", C.c_double), ("bv0", C.c_double),
+        ("adams", C.c_int), ("hyper_cn", C.c_int),
+        ("dt", C.c_double),
+    ]
+
+    @classmethod
+    def from_dict(cls, cfg):
+        p = cls()
+        for k, _ in cls._fields_:
+            if k in cfg:
+                setattr(p, k, cfg[k])
+        if p.minc == 0:
+            p.minc = 1
+        return p
+
+
+_LIB = None
+_V = C.c_void_p
+_D = C.c_double
+_I = C.c_int
+_PD = C.POINTER(C.c_double)
+
+# name -> (argtypes, restype); must list every symbol of include/mlegs_hip.h
+SIGNATURES = {
+    "mlg_init": ([C.POINTER(Params), C.POINTER(_V)], _I),
+    "mlg_finalize": ([_V], _I),
+    "mlg_dims": ([_V, C.POINTER(_I)], _I),
+    "mlg_get_table": ([_V, C.c_char_p, _PD, C.c_long, C.POINTER(C.c_long)], _I),
+    "mlg_set_stream": ([_V, _V], _I),
+    "mlg_synchronize": ([_V], _I),
+    "mlg_error_string": ([_I], C.c_char_p),
+    "mlg_chopset": ([_V, _I], _I),
+    "mlg_nrchop": ([_V], _I),
+    "mlg_field_create": ([_V, _I, C.POINTER(_V)], _I),
+    "mlg_field_destroy": ([_V], _I),
+    "mlg_field_space": ([_V], _I),
+    "mlg_field_upload": ([_V, _PD, _I, _D], _I),
+    "mlg_field_download": ([_V, _PD, C.POINTER(_I), _PD], _I),
+    "mlg_field_device_ptr": ([_V, C.POINTER(_V), C.POINTER(_V)], _I),
+    "mlg_field_zero": ([_V, _I], _I),
+    "mlg_copy": ([_V, _V], _I),
+    "mlg_chopdo": ([_V], _I),
+    "mlg_horfft": ([_V, _I], _I),
+    "mlg_verfft": ([_V, _I], _I),
+    "mlg_rtran": ([_V, _I], _I),
+    "mlg_toff": ([_V], _I),
+    "mlg_tofp": ([_V], _I),
+    "mlg_plnfft": ([_V, _I], _I),
+    "mlg_delsqh": ([_V, _V], _I),
+    "mlg_idelsqh": ([_V, _V], _I),
+    "mlg_mulxm": ([_V, _V], _I),
+    "mlg_xxdx": ([_V, _V], _I),
+    "mlg_del2": ([_V, _V], _I),
+    "mlg_idel2": ([_V, _V, _I, _D], _I),
+    "mlg_idel2ln": ([_V, _V], _I),
+    "mlg_ihelm": ([_V, _V, _D], _I),
+    "mlg_grad": ([_V, _V, _V, _V], _I),
+    "mlg_pc2vel": ([_V, _V, _V, _V, _V], _I),
+    "mlg_pc2vor": ([_V, _V, _V, _V, _V], _I),
+    "mlg_vprodsub": ([_V, _V, _V, _V, _V, _V], _I),
+    "mlg_vdotpsub": ([_V, _V, _V, _V, _V, _V, _V], _I),
+    "mlg_vel2pln": ([_V, _V, _I, _D], _I),
+    "mlg_project": ([_V, _V, _V, _V, _V, _I, _D], _I),
+    "mlg_force": ([_V, _V, _V, _V, _V, _V], _I),
+    "mlg_nonlin": ([_V, _V, _V, _V, _V, _V, _V, _V], _I),
+    "mlg_energy": ([_V, _V, _V, _PD, _PD, _PD], _I),
+    "mlg_etd_op": ([_V, _D, _PD, _PD], _I),
+    "mlg_solver_create": ([_V, C.POINTER(_V)], _I),
+    "mlg_solver_destroy": ([_V], _I),
+    "mlg_initialize": ([_V, _V, _V, _V], _I),
+    "mlg_time_step": ([_V, _V, _V, _V], _I),
+    "mlg_time_steps": ([_V, _V, _V, _V, _I], _I),
+    "mlg_solver_time": ([_V], _D),
+    "mlg_solver_nstep": ([_V], _I),
+    "mlg_qvortex": ([_V, _D, _V, _V], _I),
+    "mlg_timer_enable": ([_V, _I], _I),
+    "mlg_timer_read": ([_V, C.c_char_p, _PD, C.POINTER(C.c_long), _PD, C.POINTER(_I)], _I),
+}
+
+TIMED_KERNELS = ("rtran_bwd", "rtran_fwd", "project", "verfft_fwd", "verfft_bwd", "horfft_fwd", "horfft_bwd",
+                 "products", "band_solve", "etd")
+
+
+def _lib():
+    """Load the HIP library (no fallback: a missing library is an error)."""
+    global _LIB
+    if _LIB is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"libmlegs_hip.so not built ({LIB_PATH}); run `make -C mlegs_amd`")
+        L = C.CDLL(LIB_PATH)
+        for name, (args, res) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.argtypes = args
+            fn.restype = res
+        _LIB = L
+    return _LIB
+
+
+def library_path():
+    return LIB_PATH
+
+
+def _check(rc, where):
+    if rc:
+        raise MlegsError(rc, where)
+
+
+def _call(name, *args):
+    _check(getattr(_lib(), name)(*args), name)
+
+
+# ---------------------------------------------------------------------------
+class Grid:
+    """Grid, tables and truncation state on the current HIP device (LEGINIT)."""
+
+    def __init__(self, cfg):
+        self.cfg = dict(cfg)
+        self.params = Params.from_dict(cfg)
+        h = _V()
+        _call("mlg_init", C.byref(self.params), C.byref(h))
+        self.h = h
+        d = (_I * 12)()
+        _call("mlg_dims", self.h, d)
+        (self.ndimr, self.ndimth, self.ndimx, self.nrchopdim, self.ntchopdim, self.nxchopdim,
+         self.nrh, self.nxchoph, _, self.npf, self.nnorm, self.nr) = list(d)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib().mlg_finalize(self.h)
+            self.h = None
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *exc):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def shape(self, space):
+        if space == FFF:
+            return (self.nrchopdim, self.ntchopdim, self.nxchopdim)
+        if space == PFF:
+            return (self.ndimr, self.ntchopdim, self.nxchopdim)
+        return (self.ndimr, self.ndimth, self.ndimx)
+
+    def size(self, space):
+        return int(np.prod(self.shape(space)))
+
+    def table(self, name):
+        n = C.c_long(0)
+        _call("mlg_get_table", self.h, name.encode(), None, 0, C.byref(n))
+        out = np.zeros(n.value, dtype=np.float64)
+        _call("mlg_get_table", self.h, name.encode(), out.ctypes.data_as(_PD), n.value, C.byref(n))
+        return out
+
+    @property
+    def nrchop(self):
+        return _lib().mlg_nrchop(self.h)
+
+    def chopset(self, iof):
+        """CHOPSET(IOF): NRCHOP += IOF (src/mod_scalar3.f90:174-195)."""
+        _call("mlg_chopset", self.h, int(iof))
+
+    def synchronize(self):
+        """Wait for the stream; raises the first sticky device-side error."""
+        _call("mlg_synchronize", self.h)
+
+    def set_stream(self, stream_handle):
+        """Run on an external HIP stream (an int handle, e.g. torch.cuda.Stream.cuda_stream); None = own stream."""
+        _call("mlg_set_stream", self.h, _V(stream_handle) if stream_handle else None)
+
+    def etd_op(self, dt):
+        """CALC_BOUSSI_ETD_OP: real parts of S e^{J dt} S^-1 and S J^-1 (e^{J dt} - I) S^-1."""
+        e, nl = (_D * 16)(), (_D * 16)()
+        _call("mlg_etd_op", self.h, float(dt), e, nl)
+        return (np.array(e).reshape(4, 4, order="F"), np.array(nl).reshape(4, 4, order="F"))
+
+    def timer(self, on):
+        """Start (clearing earlier records) or stop the per-kernel HIP event timer."""
+        _call("mlg_timer_enable", self.h, 1 if on else 0)
+
+    def timer_read(self, kernel):
+        """(ms_total, launches, work_total, unit) for one timed kernel class."""
+        ms, n, w, ub = C.c_double(0), C.c_long(0), C.c_double(0), C.c_int(0)
+        _call("mlg_timer_read", self.h, kernel.encode(), C.byref(ms), C.byref(n), C.byref(w), C.byref(ub))
+        return ms.value, n.value, w.value, ("byte" if ub.value else "flop")
+
+    def scalar(self, space=FFF, data=None, ln=0.0):
+        s = Scalar(self, space)
+        if data is not None:
+            s.upload(data, space, ln)
+        return s
+
+
+class Scalar:
+    """TYPE(SCALAR): device coefficients E, space tag and log coefficient LN."""
+
+    def __init__(self, grid, space=FFF):
+        self.grid = grid
+        h = _V()
+        _call("mlg_field_create", grid.h, int(space), C.byref(h))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.grid.h:
+            _lib().mlg_field_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def space(self):
+        return _lib().mlg_field_space(self.h)
+
+    def upload(self, arr, space, ln=0.0):
+        shape = self.grid.shape(space)
+        a = np.asarray(arr, dtype=np.complex128)
+        if a.shape != shape:
+            raise ValueError(f"array shape {a.shape} does not match {SPACE_NAMES[space]} shape {shape}")
+        a = np.asfortranarray(a)
+        _call("mlg_field_upload", self.h, a.ctypes.data_as(_PD), int(space), float(ln))
+        return self
+
+    def download(self):
+        """Returns (E as a Fortran-ordered complex array, space, LN)."""
+        sp = C.c_int(0)
+        _check(_lib().mlg_field_download(self.h, None, C.byref(sp), None), "mlg_field_download")
+        out = np.zeros(self.grid.shape(sp.value), dtype=np.complex128, order="F")
+        ln = C.c_double(0.0)
+        _call("mlg_field_download", self.h, out.ctypes.data_as(_PD), C.byref(sp), C.byref(ln))
+        return out, sp.value, ln.value
+
+    @property
+    def e(self):
+        return self.download()[0]
+
+    @property
+    def ln(self):
+        return self.download()[2]
+
+    def device_ptrs(self):
+        e, ln = _V(), _V()
+        _call("mlg_field_device_ptr", self.h, C.byref(e), C.byref(ln))
+        return e.value, ln.value
+
+    def zero(self, space=FFF):
+        _call("mlg_field_zero", self.h, int(space))
+        return self
+
+    def as_tensor(self):
+        """A torch view (complex128, device memory of this field, current space shape)."""
+        import torch
+
+        ptr, _ = self.device_ptrs()
+        shape = self.grid.shape(self.space)
+        n = int(np.prod(shape))
+
+        class _Iface:
+            __cuda_array_interface__ = {"shape": (n,), "typestr": "<c16", "data": (ptr, False), "version": 2,
+                                        "strides": None}
+
+        t = torch.as_tensor(_Iface(), device="cuda")
+        return t.view(shape[::-1]).permute(2, 1, 0)
+
+
+# ---------------------------------------------------------------------------
+# transforms (src/mod_fft.f90)
+def chopdo(a):
+    _call("mlg_chopdo", a.h)
+
+
+def copy(b, a):
+    """B = A (with LN; FFF copies chopped)."""
+    _call("mlg_copy", b.h, a.h)
+
+
+def horfft(a, is_):
+    _call("mlg_horfft", a.h, int(is_))
+
+
+def verfft(a, is_):
+    _call("mlg_verfft", a.h, int(is_))
+
+
+def rtran(a, is_):
+    _call("mlg_rtran", a.h, int(is_))
+
+
+def plnfft(a, is_):
+    _call("mlg_plnfft", a.h, int(is_))
+
+
+def toff(a):
+    _call("mlg_toff", a.h)
+
+
+def tofp(a):
+    _call("mlg_tofp", a.h)
+
+
+# operators (src/mod_legops.f90)
+def delsqh(a, b):
+    _call("mlg_delsqh", a.h, b.h)
+
+
+def idelsqh(b, a):
+    _call("mlg_idelsqh", b.h, a.h)
+
+
+def mulxm(a, b):
+    _call("mlg_mulxm", a.h, b.h)
+
+
+def xxdx(a, b):
+    _call("mlg_xxdx", a.h, b.h)
+
+
+def del2(a, b):
+    _call("mlg_del2", a.h, b.h)
+
+
+def idel2(b, a, ln=None):
+    _call("mlg_idel2", b.h, a.h, 0 if ln is None else 1, 0.0 if ln is None else float(ln))
+
+
+def idel2ln(b, a):
+    _call("mlg_idel2ln", b.h, a.h)
+
+
+def ihelm(b, a, alp):
+    _call("mlg_ihelm", b.h, a.h, float(alp))
+
+
+def grad(a, rr, rp, z):
+    _call("mlg_grad", a.h, rr.h, rp.h, z.h)
+
+
+def pc2vel(psi, chi, rur, rup, uz):
+    _call("mlg_pc2vel", psi.h, chi.h, rur.h, rup.h, uz.h)
+
+
+def pc2vor(psi, chi, ror, rop, oz):
+    _call("mlg_pc2vor", psi.h, chi.h, ror.h, rop.h, oz.h)
+
+
+def vprodsub(rur, rup, uz, ror, rop, oz):
+    _call("mlg_vprodsub", rur.h, rup.h, uz.h, ror.h, rop.h, oz.h)
+
+
+def vdotpsub(rar, rap, az, rbr, rbp, bz, out):
+    _call("mlg_vdotpsub", rar.h, rap.h, az.h, rbr.h, rbp.h, bz.h, out.h)
+
+
+def vel2pln(rup, psi, ln=None):
+    _call("mlg_vel2pln", rup.h, psi.h, 0 if ln is None else 1, 0.0 if ln is None else float(ln))
+
+
+def project(rur, rup, uz, psi, chi, ln=None):
+    _call("mlg_project", rur.h, rup.h, uz.h, psi.h, chi.h, 0 if ln is None else 1, 0.0 if ln is None else float(ln))
+
+
+# Boussinesq terms (src/mod_boussinesq.f90)
+def calc_boussi_force(psi, chi, b, psin, chin, bn):
+    _call("mlg_force", psi.h, chi.h, b.h, psin.h, chin.h, bn.h)
+
+
+def calc_boussi_nonlin(b, rur, rup, uz, bn, ror, rop, oz):
+    _call("mlg_nonlin", b.h, rur.h, rup.h, uz.h, bn.h, ror.h, rop.h, oz.h)
+
+
+def calc_boussi_energy(psi, chi, b):
+    """Returns (KE, PE, dKE) as (NTCHOP, NXCHOP) arrays over m >= 0, |k|."""
+    cfg = psi.grid.cfg
+    nt, nxc = cfg["ntchop"], cfg["nxchop"]
+    ke, pe, dke = (_D * (nt * nxc))(), (_D * (nt * nxc))(), (_D * (nt * nxc))()
+    _call("mlg_energy", psi.h, chi.h, b.h, ke, pe, dke)
+    f = lambda a: np.array(a).reshape(nt, nxc, order="F")
+    return f(ke), f(pe), f(dke)
+
+
+def qvortex(grid, q, psi, chi):
+    """QVORTEX (src/mod_layout.f90:449-610): Gaussian vortex of strength q in psi, chi = 0."""
+    _call("mlg_qvortex", grid.h, float(q), psi.h, chi.h)
+
+
+class Solver:
+    """SOLVER_T: AB2-CN or ETD2 time march with a Richardson first step."""
+
+    def __init__(self, grid):
+        self.grid = grid
+        h = _V()
+        _call("mlg_solver_create", grid.h, C.byref(h))
+        self.h = h
+
+    def close(self):
+        if getattr(self, "h", None) and self.grid.h:
+            _lib().mlg_solver_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def initialize(self, psi, chi, b):
+        _call("mlg_initialize", self.h, psi.h, chi.h, b.h)
+
+    def time_step(self, psi, chi, b):
+        _call("mlg_time_step", self.h, psi.h, chi.h, b.h)
+
+    def time_stepping(self, psi, chi, b, nsteps):
+        """NSTEPS steps, one synchronisation at the end (device checks are sticky)."""
+        _call("mlg_time_steps", self.h, psi.h, chi.h, b.h, int(nsteps))
+
+    @property
+    def t(self):
+        return _lib().mlg_solver_time(self.h)
+
+    @property
+    def nstep(self):
+        return _lib().mlg_solver_nstep(self.h)

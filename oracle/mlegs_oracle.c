@@ -2000,6 +2000,16 @@ int og_initialize(og_solver *s, og_field *psi, og_field *chi, og_field *b) {
   return rich_etdfe_be(s, psi, chi, b);
 }
 
+/* The ETD operators of SOLVER_T%INITIALIZE without the Richardson start
+ * (used to time steps from a given state). */
+int og_solver_prepare(og_solver *s) {
+  og_grid *g = s->g;
+  if (g->p.adams) return E_OK;
+  if (g->p.bv0 == 0.0) return E_SOLVER;
+  og_etd_op(g, g->p.dt, s->etd_e, s->etd_nl);
+  return E_OK;
+}
+
 /* SOLVER_T%TIME_STEPPING: mod_march.f90:243-307 (step + per-step checks) */
 int og_time_step(og_solver *s, og_field *psi, og_field *chi, og_field *b) {
   int rc = s->g->p.adams ? step_ab_cn(s, psi, chi, b) : step_etdab_cn(s, psi, chi, b);

@@ -107,6 +107,7 @@ int og_energy(og_grid *g, const og_field *psi, const og_field *chi, const og_fie
 og_solver *og_solver_new(og_grid *g);
 void og_solver_free(og_solver *s);
 int og_initialize(og_solver *s, og_field *psi, og_field *chi, og_field *b);
+int og_solver_prepare(og_solver *s);
 int og_time_step(og_solver *s, og_field *psi, og_field *chi, og_field *b);
 int og_diagnost(const og_grid *g, const og_field *psi, const og_field *chi);
 double og_time(const og_solver *s);

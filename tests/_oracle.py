@@ -85,6 +85,7 @@ def lib():
             "og_solver_free": ([C.c_void_p], None),
             "og_initialize": ([C.c_void_p, P, P, P], C.c_int),
             "og_time_step": ([C.c_void_p, P, P, P], C.c_int),
+            "og_solver_prepare": ([C.c_void_p], C.c_int),
             "og_diagnost": ([g, P, P], C.c_int),
             "og_time": ([C.c_void_p], C.c_double),
             "og_nstep": ([C.c_void_p], C.c_int),
@@ -244,6 +245,12 @@ class OSolver:
         rc = lib().og_initialize(self.s, psi.ptr, chi.ptr, b.ptr)
         if rc:
             raise RuntimeError(f"og_initialize returned {rc}")
+
+    def prepare(self):
+        """ETD operators only (no Richardson start): for timing steps."""
+        rc = lib().og_solver_prepare(self.s)
+        if rc:
+            raise RuntimeError(f"og_solver_prepare returned {rc}")
 
     def step(self, psi, chi, b):
         return lib().og_time_step(self.s, psi.ptr, chi.ptr, b.ptr)

@@ -1,0 +1,391 @@
+"""Parity of the HIP library (through its C ABI) with the FP64 oracle.
+
+Every test drives libmlegs_hip.so through mlegs_amd.hip (ctypes over
+include/mlegs_hip.h) and the oracle on the same seeded inputs, and compares
+with the error metric
+
+    err = max |gpu - oracle| / max |oracle|      (per field)
+
+plus |LN_gpu - LN_oracle| / max(1, |LN_oracle|) for log-term coefficients.
+Tolerance: 1e-12 for single routines and one time step; a few steps of
+AB2-CN/ETD2 are held to 1e-11 (round-off differences of the two summation
+orders are amplified by the step).  Physical-space comparisons cover the
+meaningful region only (r < NR, theta slot < NTH, m < NTCHOPDIM).
+"""
+import numpy as np
+import pytest
+
+import _oracle as O
+from mlegs_amd import hip as H
+from mlegs_amd.configs import CFG1, LAMB_OSEEN_Q, seeded_noise
+
+pytestmark = pytest.mark.gpu
+
+SMALL = dict(CFG1, nr=32, nth=16, nx=16, nrchop=22, ntchop=6, nxchop=5)
+SMALL_ETD = dict(SMALL, adams=0, hyper_cn=1, nup=1e-3, kappap=1e-3, visc_p=4, omega=0.8, bv0=1.3)
+TOL = 1e-12
+
+
+def relerr(a, b):
+    s = float(np.max(np.abs(b))) if b.size else 0.0
+    d = float(np.max(np.abs(a - b))) if a.size else 0.0
+    return d / s if s > 0 else d
+
+
+class Pair:
+    def __init__(self, cfg):
+        self.cfg = dict(cfg)
+        self.g = H.Grid(cfg)
+        self.o = O.OGrid(cfg)
+
+    def close(self):
+        self.g.close()
+        self.o.close()
+
+    def both(self, arr, space, ln=0.0):
+        return self.g.scalar(space, arr, ln), self.o.field(space, arr, ln)
+
+    def empty(self, space=O.FFF):
+        return self.g.scalar(space), self.o.field(space)
+
+    def region(self, a, space):
+        if space == O.PPP:
+            return a[: self.o.nr, : self.cfg["nth"], :]
+        if space == O.PFP:
+            return a[: self.o.nr, : self.o.ntchopdim, :]
+        if space == O.PFF:
+            return a[: self.o.nr]
+        return a
+
+    def check(self, gs, of, tol=TOL, what=""):
+        ga, gsp, gln = gs.download()
+        assert gsp == of.space, f"{what}: space {gsp} != {of.space}"
+        oa = of.arr()
+        e = relerr(self.region(ga, gsp), self.region(oa, gsp))
+        eln = abs(gln - of.ln) / max(1.0, abs(of.ln))
+        assert e <= tol, f"{what}: rel err {e:.3e} > {tol:.1e}"
+        assert eln <= tol, f"{what}: LN {gln!r} vs {of.ln!r}"
+        return e
+
+    def noise(self, fid, iof=0, amp=1.0):
+        return seeded_noise(self.cfg, fid, amp, nrchop=self.cfg["nrchop"] + iof)
+
+    def rand(self, space, seed):
+        rng = np.random.default_rng(seed)
+        shp = self.o.shape(space)
+        a = rng.standard_normal(shp) + 1j * rng.standard_normal(shp)
+        z = np.zeros(shp, dtype=complex)
+        reg = self.region(z, space)
+        reg[...] = self.region(a, space)
+        return np.asfortranarray(z)
+
+
+@pytest.fixture(scope="module", params=["small", "cfg1"])
+def P(request):
+    p = Pair(SMALL if request.param == "small" else CFG1)
+    yield p
+    p.close()
+
+
+# --------------------------------------------------------------- tables
+def test_tables_match(P):
+    for name in ("x", "w", "r", "lnx", "pf", "lognorm", "norm", "at0", "at1", "ak"):
+        a, b = P.g.table(name), P.o.table(name)
+        assert a.shape == b.shape, name
+        assert relerr(a, b) <= 1e-14, name
+
+
+def test_etd_op_matches():
+    p = Pair(SMALL_ETD)
+    try:
+        for dt in (5e-4, 0.3):
+            ge, gn = p.g.etd_op(dt)
+            oe, on = p.o.etd_op(dt)
+            assert relerr(ge, oe) <= 1e-14 and relerr(gn, on) <= 1e-14
+    finally:
+        p.close()
+
+
+# --------------------------------------------------------------- transforms
+@pytest.mark.parametrize("iof", [0, 3])
+def test_rtran_backward(P, iof):
+    P.g.chopset(iof)
+    P.o.chopset(iof)
+    try:
+        g, o = P.both(P.noise(1, iof), O.FFF, ln=0.37)
+        H.rtran(g, 1)
+        P.o.call("rtran", o, 1)
+        P.check(g, o, what="RTRAN backward")
+    finally:
+        P.g.chopset(-iof)
+        P.o.chopset(-iof)
+
+
+def test_rtran_forward(P):
+    g, o = P.both(P.rand(O.PFF, 5), O.PFF)
+    H.rtran(g, -1)
+    P.o.call("rtran", o, -1)
+    P.check(g, o, what="RTRAN forward")
+
+
+def test_verfft_both_ways(P):
+    g, o = P.both(P.rand(O.PFP, 6), O.PFP)
+    H.verfft(g, -1)
+    P.o.call("verfft", o, -1)
+    P.check(g, o, what="VERFFT forward")
+    g, o = P.both(P.rand(O.PFF, 7), O.PFF)
+    H.verfft(g, 1)
+    P.o.call("verfft", o, 1)
+    P.check(g, o, what="VERFFT backward")
+
+
+def test_horfft_both_ways(P):
+    g, o = P.both(P.rand(O.PPP, 8), O.PPP)
+    H.horfft(g, -1)
+    P.o.call("horfft", o, -1)
+    P.check(g, o, what="HORFFT forward")
+    a = P.rand(O.PFP, 9)
+    a[:, 0, :] = a[:, 0, :].real
+    a[:, P.o.ntchopdim:, :] = 0
+    g, o = P.both(a, O.PFP)
+    H.horfft(g, 1)
+    P.o.call("horfft", o, 1)
+    P.check(g, o, what="HORFFT backward")
+
+
+def test_tofp_toff_chain(P):
+    g, o = P.both(P.noise(2), O.FFF, ln=-0.21)
+    H.tofp(g)
+    P.o.call("tofp", o)
+    P.check(g, o, what="TOFP")
+    H.toff(g)
+    P.o.call("toff", o)
+    P.check(g, o, what="TOFF")
+
+
+# --------------------------------------------------------------- operators
+@pytest.mark.parametrize("iof", [0, 3])
+@pytest.mark.parametrize("name", ["delsqh", "mulxm", "xxdx", "del2"])
+def test_forward_operators(P, name, iof):
+    P.g.chopset(iof)
+    P.o.chopset(iof)
+    try:
+        a = P.noise(3, iof)
+        if name == "del2":  # DEL2 needs the top m=1 mode to vanish
+            nn1 = max(min(P.o.nrchop, P.o.nrchop - 1), 0)
+            a[nn1 - 1, 1, 0] = 0
+        ga, oa = P.both(a, O.FFF, ln=0.45)
+        gb, ob = P.empty()
+        getattr(H, name)(ga, gb)
+        P.o.call(name, oa, ob)
+        P.check(gb, ob, what=name)
+    finally:
+        P.g.chopset(-iof)
+        P.o.chopset(-iof)
+
+
+@pytest.mark.parametrize("case", ["idelsqh", "idel2", "idel2_ln", "idel2ln", "ihelm"])
+def test_inverse_operators(P, case):
+    gb, ob = P.both(P.noise(4), O.FFF, ln=0.6)
+    ga, oa = P.empty()
+    if case == "idelsqh":
+        H.idelsqh(gb, ga)
+        P.o.call("idelsqh", ob, oa)
+    elif case == "idel2":
+        H.idel2(gb, ga)
+        P.o.call("idel2", ob, oa, 0, 0.0)
+    elif case == "idel2_ln":
+        H.idel2(gb, ga, ln=-0.8)
+        P.o.call("idel2", ob, oa, 1, -0.8)
+    elif case == "idel2ln":
+        H.idel2ln(gb, ga)
+        P.o.call("idel2ln", ob, oa)
+    else:
+        H.ihelm(gb, ga, 2.0 / 1e-3 / 1e-3)
+        P.o.call("ihelm", ob, oa, 2.0 / 1e-3 / 1e-3)
+    P.check(ga, oa, what=case)
+
+
+@pytest.mark.parametrize("iof", [0, 3])
+def test_composite_operators(P, iof):
+    P.g.chopset(iof)
+    P.o.chopset(iof)
+    try:
+        psi = P.noise(5, iof)
+        chi = P.noise(6, iof)
+        nn1 = max(min(P.o.nrchop, P.o.nrchop - 1), 0)
+        chi[nn1 - 1, 1, 0] = 0
+        gp, op = P.both(psi, O.FFF, ln=0.3)
+        gc, oc = P.both(chi, O.FFF, ln=0.0)
+        outs = [P.empty() for _ in range(3)]
+        H.grad(gp, *[x[0] for x in outs])
+        P.o.call("grad", op, *[x[1] for x in outs])
+        for i, (g, o) in enumerate(outs):
+            P.check(g, o, what=f"GRAD[{i}]")
+        H.pc2vel(gp, gc, *[x[0] for x in outs])
+        P.o.call("pc2vel", op, oc, *[x[1] for x in outs])
+        for i, (g, o) in enumerate(outs):
+            P.check(g, o, what=f"PC2VEL[{i}]")
+        H.pc2vor(gp, gc, *[x[0] for x in outs])
+        P.o.call("pc2vor", op, oc, *[x[1] for x in outs])
+        for i, (g, o) in enumerate(outs):
+            P.check(g, o, what=f"PC2VOR[{i}]")
+    finally:
+        P.g.chopset(-iof)
+        P.o.chopset(-iof)
+
+
+# --------------------------------------------------------------- products, projection
+def test_vprodsub_vdotpsub(P):
+    fs = [P.both(P.rand(O.PPP, 20 + i), O.PPP) for i in range(6)]
+    out = P.empty(O.PPP)
+    H.vdotpsub(*[f[0] for f in fs], out[0])
+    P.o.call("vdotpsub", *[f[1] for f in fs], out[1])
+    P.check(out[0], out[1], what="VDOTPSUB")
+    H.vprodsub(*[f[0] for f in fs])
+    P.o.call("vprodsub", *[f[1] for f in fs])
+    for i in range(3, 6):
+        P.check(fs[i][0], fs[i][1], what=f"VPRODSUB[{i}]")
+
+
+@pytest.mark.parametrize("ln", [None, 0.25])
+def test_vel2pln(P, ln):
+    grup, orup = P.both(P.rand(O.PFF, 30), O.PFF)
+    gpsi, opsi = P.empty()
+    H.vel2pln(grup, gpsi, ln)
+    P.o.call("vel2pln", orup, opsi, 0 if ln is None else 1, 0.0 if ln is None else ln)
+    P.check(grup, orup, what="VEL2PLN r u_theta")
+    P.check(gpsi, opsi, what="VEL2PLN psi")
+
+
+@pytest.mark.parametrize("src", [O.PPP, O.PFF, O.FFF])
+def test_project(P, src):
+    if src == O.FFF:
+        P.g.chopset(3)
+        P.o.chopset(3)
+        arrs = [P.noise(40 + i, 3) for i in range(3)]
+    else:
+        arrs = [P.rand(src, 40 + i) for i in range(3)]
+    try:
+        fs = [P.both(a, src) for a in arrs]
+    finally:
+        if src == O.FFF:
+            P.g.chopset(-3)
+            P.o.chopset(-3)
+    gpsi, opsi = P.both(P.noise(7), O.FFF, ln=0.11)
+    gchi, ochi = P.empty()
+    H.project(*[f[0] for f in fs], gpsi, gchi, 0.11)
+    P.o.call("project", *[f[1] for f in fs], opsi, ochi, 1, 0.11)
+    P.check(gpsi, opsi, what="PROJECT psi")
+    P.check(gchi, ochi, what="PROJECT del2 chi")
+
+
+# --------------------------------------------------------------- Boussinesq terms
+def state(P, amp=1e-4):
+    """Lamb-Oseen psi (oracle QVORTEX) plus seeded noise on psi, chi, b."""
+    psi, chi = P.o.field(O.FFF), P.o.field(O.FFF)
+    P.o.call("qvortex", LAMB_OSEEN_Q, psi, chi)
+    return (psi.arr() + seeded_noise(P.cfg, 1, amp), psi.ln), (seeded_noise(P.cfg, 2, amp), 0.0), \
+        (seeded_noise(P.cfg, 3, amp), 0.0)
+
+
+def test_qvortex(P):
+    (gp, op), (gc, oc) = P.empty(), P.empty()
+    H.qvortex(P.g, LAMB_OSEEN_Q, gp, gc)
+    P.o.call("qvortex", LAMB_OSEEN_Q, op, oc)
+    P.check(gp, op, what="QVORTEX psi")
+    P.check(gc, oc, what="QVORTEX chi")
+
+
+def test_force(P):
+    st = state(P)
+    ins = [P.both(a, O.FFF, ln) for a, ln in st]
+    outs = [P.empty() for _ in range(3)]
+    H.calc_boussi_force(*[x[0] for x in ins], *[x[0] for x in outs])
+    P.o.call("force", *[x[1] for x in ins], *[x[1] for x in outs])
+    for nm, (g, o) in zip(("psiN", "chiN", "bN"), outs):
+        P.check(g, o, what=f"FORCE {nm}")
+
+
+def test_nonlin(P):
+    st = state(P)
+    (gp, op), (gc, oc), (gb, ob) = [P.both(a, O.FFF, ln) for a, ln in st]
+    gv = [P.empty() for _ in range(7)]
+    P.g.chopset(3)
+    P.o.chopset(3)
+    H.pc2vel(gp, gc, gv[0][0], gv[1][0], gv[2][0])
+    H.pc2vor(gp, gc, gv[4][0], gv[5][0], gv[6][0])
+    P.o.call("pc2vel", op, oc, gv[0][1], gv[1][1], gv[2][1])
+    P.o.call("pc2vor", op, oc, gv[4][1], gv[5][1], gv[6][1])
+    P.g.chopset(-3)
+    P.o.chopset(-3)
+    H.calc_boussi_nonlin(gb, *[x[0] for x in gv])
+    P.o.call("nonlin", ob, *[x[1] for x in gv])
+    for i, (g, o) in enumerate(gv):
+        P.check(g, o, what=f"NONLIN[{i}]")
+
+
+def test_energy(P):
+    st = state(P, amp=1e-2)
+    (gp, op), (gc, oc), (gb, ob) = [P.both(a, O.FFF, ln) for a, ln in st]
+    gk = H.calc_boussi_energy(gp, gc, gb)
+    ok = P.o.energy(op, oc, ob)
+    for nm, a, b in zip(("KE", "PE", "dKE"), gk, ok):
+        assert relerr(a, b) <= TOL, nm
+
+
+# --------------------------------------------------------------- time march
+def run_steps(cfg, nsteps, amp=1e-4, tol=1e-11):
+    p = Pair(cfg)
+    try:
+        st = state(p, amp)
+        (gp, op), (gc, oc), (gb, ob) = [p.both(a, O.FFF, ln) for a, ln in st]
+        gs, osv = H.Solver(p.g), O.OSolver(p.o)
+        gs.initialize(gp, gc, gb)
+        osv.initialize(op, oc, ob)
+        errs = []
+        for step in range(nsteps + 1):
+            if step:
+                gs.time_step(gp, gc, gb)
+                assert osv.step(op, oc, ob) == 0
+            t = TOL if step == 0 else tol
+            errs.append([p.check(g, o, t, what=f"step {step} {nm}")
+                         for nm, g, o in (("psi", gp, op), ("chi", gc, oc), ("b", gb, ob))])
+        assert abs(gs.t - osv.t) < 1e-15 and gs.nstep == nsteps + 1
+        return errs
+    finally:
+        p.close()
+
+
+def test_ab2_cn_steps_small():
+    run_steps(SMALL, 3)
+
+
+def test_etd2_steps_small():
+    run_steps(SMALL_ETD, 3)
+
+
+def test_cfg1_ten_steps():
+    """The 64x32x32 AB2-CN workload: Richardson start + 9 steps = 10 steps."""
+    errs = run_steps(CFG1, 9)
+    assert max(max(e) for e in errs) <= 1e-11
+
+
+def test_time_steps_batch_equals_single():
+    p = Pair(SMALL_ETD)
+    try:
+        st = state(p)
+        a = [p.g.scalar(O.FFF, x, ln) for x, ln in st]
+        b = [p.g.scalar(O.FFF, x, ln) for x, ln in st]
+        s1, s2 = H.Solver(p.g), H.Solver(p.g)
+        s1.initialize(*a)
+        s2.initialize(*b)
+        for _ in range(4):
+            s1.time_step(*a)
+        s2.time_stepping(*b, 4)
+        for x, y in zip(a, b):
+            xa, _, xl = x.download()
+            ya, _, yl = y.download()
+            assert np.array_equal(xa, ya) and xl == yl
+    finally:
+        p.close()
