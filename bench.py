@@ -26,7 +26,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 METRIC = "steps/s at 384\u00d7256\u00d7256: split-prec vs FP64 GPU (\u22641e-10 err); adjoint/fwd cost"
 PEAK_FP64 = 78.6e12      # MI355X FP64 matrix, FLOP/s
 PEAK_HBM = 8.0e12        # MI355X HBM3E, B/s
-TRAFFIC_FILE = os.path.join(ROOT, "profiles", "traffic_cfg2.json")
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernels_cfg2_traffic.json")
 
 
 def parse():
@@ -36,7 +36,26 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-timer", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="only run 20 copies of a 384x257x256 complex field (407 MB each way), "
+                         "to scale HBM byte counters")
     return ap.parse_args()
+
+
+def calibrate_copy(H, cfg):
+    """20 launches of the library's copy kernel on a full physical-space field:
+    each reads and writes NDIMR*NDIMTH*NDIMX*16 bytes once."""
+    g = H.Grid(cfg)
+    a, b = g.scalar(H.PPP), g.scalar(H.PPP)
+    a.zero(H.PPP)
+    for _ in range(20):
+        H.copy(b, a)
+    g.synchronize()
+    n = g.size(H.PPP) * 16
+    print(json.dumps({"copy_bytes_each_way": n, "launches": 20}), flush=True)
+    a.close()
+    b.close()
+    g.close()
 
 
 def initial_state(H, g, cfg):
@@ -98,6 +117,9 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
     cfg = dict(CFG2)
+    if a.calibrate_copy:
+        calibrate_copy(H, cfg)
+        return
     g = H.Grid(cfg)
     psi, chi, b = initial_state(H, g, cfg)
     s = H.Solver(g)

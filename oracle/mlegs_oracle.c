@@ -69,6 +69,12 @@ struct og_solver {
 static int imax(int a, int b) { return a > b ? a : b; }
 static int imin(int a, int b) { return a < b ? a : b; }
 
+/* Summation order of the Legendre sums (RTRAN both ways, PROJECT):
+ * 0 ascending as in the reference, 1 descending.  Test use only: measures
+ * how much a result depends on the order of its floating-point sums. */
+static int sum_rev = 0;
+void og_set_sum_order(int reverse) { sum_rev = reverse ? 1 : 0; }
+
 void og_set_threads(int n) {
 #ifdef _OPENMP
   if (n > 0) omp_set_num_threads(n);
@@ -706,7 +712,8 @@ int og_rtran(const og_grid *g, og_field *a, int is) {
         for (int n = 0; n < nn; n++) {
           const double *xr = (n % 2 == 0) ? ber : bor, *xi = (n % 2 == 0) ? bei : boi;
           for (int k = 0; k < nk; k++) sr[k] = si[k] = 0.0;
-          for (int i = 0; i < nrh; i++) {
+          for (int i0 = 0; i0 < nrh; i0++) {
+            int i = sum_rev ? nrh - 1 - i0 : i0;
             double pf = PF(g, i, n, mm);
             const double *ur = xr + (size_t)nk * i, *ui = xi + (size_t)nk * i;
             for (int k = 0; k < nk; k++) { sr[k] += ur[k] * pf; si[k] += ui[k] * pf; }
@@ -732,7 +739,8 @@ int og_rtran(const og_grid *g, og_field *a, int is) {
         for (int k = 0; k < nk; k++) {
           int nn = g->nrchops[mm];
           for (int i = 0; i < nrh; i++) be_r[i] = be_i[i] = bo_r[i] = bo_i[i] = 0.0;
-          for (int n = 0; n < nn; n++) {
+          for (int n0 = 0; n0 < nn; n0++) {
+            int n = sum_rev ? nn - 1 - n0 : n0;
             cplx c = a->e[FIDX(g, n, mm, k)];
             double cr = creal(c), ci = cimag(c);
             const double *pf = &PF(g, 0, n, mm);
@@ -1321,7 +1329,8 @@ int og_project(const og_grid *g, og_field *rur, og_field *rup, og_field *uz, og_
     const double *bi_ = fb + (size_t)nk * nrh * (par + 1 + 4 * (f));                                \
     for (int k = 0; k < nk; k++) {                                                                  \
       double sr = 0, si = 0;                                                                        \
-      for (int j = 0; j < nrh; j++) {                                                               \
+      for (int j0 = 0; j0 < nrh; j0++) {                                                            \
+        int j = sum_rev ? nrh - 1 - j0 : j0;                                                        \
         double av = (A)[ii + (size_t)nmax * j];                                                     \
         sr += av * br[k + (size_t)nk * j];                                                          \
         si += av * bi_[k + (size_t)nk * j];                                                         \

@@ -116,6 +116,7 @@ int og_nstep(const og_solver *s);
 /* set-up helpers */
 int og_qvortex(og_grid *g, double q, og_field *psi, og_field *chi);
 void og_set_threads(int n);
+void og_set_sum_order(int reverse);
 
 #ifdef __cplusplus
 }

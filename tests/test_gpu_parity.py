@@ -7,9 +7,9 @@ with the error metric
     err = max |gpu - oracle| / max |oracle|      (per field)
 
 plus |LN_gpu - LN_oracle| / max(1, |LN_oracle|) for log-term coefficients.
-Tolerance: 1e-12 for single routines and one time step; a few steps of
-AB2-CN/ETD2 are held to 1e-11 (round-off differences of the two summation
-orders are amplified by the step).  Physical-space comparisons cover the
+Tolerance: 1e-12 for single routines.  Time steps: max(1e-12, 10 s) per
+field, s being the oracle's own change when its Legendre sums run in the
+opposite order (the MFMA kernels sum in another order than the oracle).  Physical-space comparisons cover the
 meaningful region only (r < NR, theta slot < NTH, m < NTCHOPDIM).
 """
 import numpy as np
@@ -335,24 +335,74 @@ def test_energy(P):
 
 
 # --------------------------------------------------------------- time march
-def run_steps(cfg, nsteps, amp=1e-4, tol=1e-11):
+def oracle_trajectory(cfg, st, nsteps, reverse=False):
+    """Oracle states (psi, chi, b as (array, LN)) after the start and each step."""
+    O.lib().og_set_sum_order(1 if reverse else 0)
+    o = O.OGrid(cfg)
+    try:
+        fs = [o.field(O.FFF, a, ln) for a, ln in st]
+        sv = O.OSolver(o)
+        sv.initialize(*fs)
+        out = [[(f.get(), f.ln) for f in fs]]
+        for _ in range(nsteps):
+            assert sv.step(*fs) == 0
+            out.append([(f.get(), f.ln) for f in fs])
+        sv.close()
+        return out
+    finally:
+        O.lib().og_set_sum_order(0)
+        o.close()
+
+
+def run_steps(cfg, nsteps, amp=1e-4):
+    """GPU vs oracle over a start and `nsteps` steps.
+
+    Bar per field and step: err <= max(1e-12, 10 s), s being the oracle's
+    own change when its Legendre sums run in the opposite order; checked on
+    all coefficients and again without the gauge entries (n=0, m=0, all k)
+    of psi and chi, which the value-at-infinity fix-up forms as a cancelling
+    sum over the whole m=0 column.  s stays below 1e-12 for psi and b on the
+    small grids; chi, an inverse Laplacian of a projected field, is the
+    ill-conditioned one (s up to ~1e-9 at 384x256x256).
+    """
     p = Pair(cfg)
     try:
         st = state(p, amp)
-        (gp, op), (gc, oc), (gb, ob) = [p.both(a, O.FFF, ln) for a, ln in st]
-        gs, osv = H.Solver(p.g), O.OSolver(p.o)
-        gs.initialize(gp, gc, gb)
-        osv.initialize(op, oc, ob)
-        errs = []
+        ref = oracle_trajectory(cfg, st, nsteps)
+        rev = oracle_trajectory(cfg, st, nsteps, reverse=True)
+        gf = [p.g.scalar(O.FFF, a, ln) for a, ln in st]
+        gs = H.Solver(p.g)
+        gs.initialize(*gf)
+        report, lines = [], []
         for step in range(nsteps + 1):
             if step:
-                gs.time_step(gp, gc, gb)
-                assert osv.step(op, oc, ob) == 0
-            t = TOL if step == 0 else tol
-            errs.append([p.check(g, o, t, what=f"step {step} {nm}")
-                         for nm, g, o in (("psi", gp, op), ("chi", gc, oc), ("b", gb, ob))])
-        assert abs(gs.t - osv.t) < 1e-15 and gs.nstep == nsteps + 1
-        return errs
+                gs.time_step(*gf)
+            row = []
+            for nm, g, (ra, rl), (va, vl) in zip(("psi", "chi", "b"), gf, ref[step], rev[step]):
+                ga, _, gl = g.download()
+                e, sens = relerr(ga, ra), relerr(va, ra)
+                mask = np.ones(ra.shape, dtype=bool)
+                if nm != "b":
+                    mask[0, 0, :] = False
+                scale = np.max(np.abs(ra))
+                ex = float(np.max(np.abs(ga - ra)[mask])) / scale
+                sens_ex = float(np.max(np.abs(va - ra)[mask])) / scale
+                eln = abs(gl - rl) / max(1.0, abs(rl))
+                lsens = abs(vl - rl) / max(1.0, abs(rl))
+                where = np.unravel_index(np.argmax(np.abs(ga - ra)), ra.shape)
+                lines.append(f"step {step} {nm}: err {e:.2e} (excl. gauge {ex:.2e}) at {tuple(int(x) for x in where)}, "
+                             f"oracle order sensitivity {sens:.2e} (excl. gauge {sens_ex:.2e}); "
+                             f"LN err {eln:.1e} (sens {lsens:.1e})")
+                row.append((e, sens, ex, sens_ex, eln, lsens))
+            report.append(row)
+        print("\n".join(lines))
+        for step, row in enumerate(report):
+            for nm, (e, sens, ex, sens_ex, eln, lsens) in zip(("psi", "chi", "b"), row):
+                assert ex <= max(TOL, 10 * sens_ex), f"step {step} {nm}: err excl. gauge {ex:.3e}\n" + "\n".join(lines)
+                assert e <= max(TOL, 10 * sens), f"step {step} {nm}: err {e:.3e} > bar\n" + "\n".join(lines)
+                assert eln <= max(TOL, 10 * lsens), f"step {step} {nm}: LN\n" + "\n".join(lines)
+        assert gs.nstep == nsteps + 1
+        return report
     finally:
         p.close()
 
@@ -366,9 +416,10 @@ def test_etd2_steps_small():
 
 
 def test_cfg1_ten_steps():
-    """The 64x32x32 AB2-CN workload: Richardson start + 9 steps = 10 steps."""
-    errs = run_steps(CFG1, 9)
-    assert max(max(e) for e in errs) <= 1e-11
+    """The 64x32x32 AB2-CN workload: Richardson start + 9 steps = 10 steps;
+    psi and b stay within 1e-12 at every step."""
+    rep = run_steps(CFG1, 9)
+    assert max(max(r[0][0], r[2][0]) for r in rep) <= TOL
 
 
 def test_time_steps_batch_equals_single():
@@ -387,5 +438,39 @@ def test_time_steps_batch_equals_single():
             xa, _, xl = x.download()
             ya, _, yl = y.download()
             assert np.array_equal(xa, ya) and xl == yl
+    finally:
+        p.close()
+
+
+@pytest.mark.timeout(600)
+def test_cfg2_first_steps():
+    """384x256x256 ETD2 workload (noise 1e-3): the Richardson start and one
+    ETD2-AB/CN step, every field within the bar of run_steps."""
+    from mlegs_amd.configs import CFG2
+
+    O.lib().og_set_threads(min(16, __import__("os").cpu_count() or 1))
+    run_steps(CFG2, 1, amp=CFG2["noise"])
+
+
+def test_cfg1_energy_100_steps():
+    """Kinetic energy and buoyancy variance of the 64x32x32 AB2-CN run over
+    100 steps agree with the oracle to 1e-10 relative (totals over m, k),
+    checked every 10 steps."""
+    p = Pair(CFG1)
+    try:
+        st = state(p, CFG1["noise"])
+        (gp, op), (gc, oc), (gb, ob) = [p.both(a, O.FFF, ln) for a, ln in st]
+        gs, osv = H.Solver(p.g), O.OSolver(p.o)
+        gs.initialize(gp, gc, gb)
+        osv.initialize(op, oc, ob)
+        for blk in range(10):
+            gs.time_stepping(gp, gc, gb, 10 if blk else 9)
+            for _ in range(10 if blk else 9):
+                assert osv.step(op, oc, ob) == 0
+            gke, gpe, _ = H.calc_boussi_energy(gp, gc, gb)
+            oke, ope, _ = p.o.energy(op, oc, ob)
+            assert abs(gke.sum() - oke.sum()) <= 1e-10 * abs(oke.sum()), (blk, gke.sum(), oke.sum())
+            assert abs(gpe.sum() - ope.sum()) <= 1e-10 * abs(ope.sum()), (blk, gpe.sum(), ope.sum())
+        assert gs.nstep == 100
     finally:
         p.close()
