@@ -233,8 +233,6 @@ extern "C" int mlg_init(const mlg_params *p, mlg_ctx **out) {
     hv.insert(hv.end(), G.hd_x.begin(), G.hd_x.end());
     RET(upload(&d.hv, hv));
   }
-  d.lu_len = (size_t)(G.nrchopdim + 2) * 3 * (size_t)G.ntchopdim * G.nxchopdim;
-  MLG_CHECK_HIP(hipMalloc((void **)&d.lu, sizeof(double) * d.lu_len));
   MLG_CHECK_HIP(hipMalloc((void **)&d.red, sizeof(double) * (4 * (size_t)G.ntchop * G.nxchopdim + 2)));
   MLG_CHECK_HIP(hipMalloc((void **)&d.err, sizeof(int)));
   MLG_CHECK_HIP(hipMemset(d.err, 0, sizeof(int)));
@@ -269,7 +267,9 @@ extern "C" int mlg_finalize(mlg_ctx *c) {
   c->timer.spare.clear();
   DevTables &d = c->d;
   void *ptrs[] = {d.pf, d.vdt, d.w, d.r, d.x, d.lnx, d.at1, d.ak, d.norm, d.lognorm, d.bxm, d.bxxdx, d.bdel2h,
-                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.lu, d.red, d.err};
+                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.red, d.err};
+  for (auto &f : c->fcache) (void)hipFree(f.f);
+  c->fcache.clear();
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);

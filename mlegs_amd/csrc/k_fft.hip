@@ -109,7 +109,7 @@ __global__ void __launch_bounds__(256) k_verfft_bwd(KGrid g, int P, int lg, cons
   }
 }
 
-int launch_verfft_fwd(mlg_ctx *c, const double2 *in, double2 *out) {
+static int launch_verfft_fwd_r2(mlg_ctx *c, const double2 *in, double2 *out) {
   KGrid g = kgrid(c);
   int P = pencils_for(g.nx), lg = ilog2(g.nx);
   dim3 grid((g.ndimr + P - 1) / P, g.ntchopdim);
@@ -120,7 +120,7 @@ int launch_verfft_fwd(mlg_ctx *c, const double2 *in, double2 *out) {
   return 0;
 }
 
-int launch_verfft_bwd(mlg_ctx *c, const double2 *in, double2 *out) {
+static int launch_verfft_bwd_r2(mlg_ctx *c, const double2 *in, double2 *out) {
   KGrid g = kgrid(c);
   int P = pencils_for(g.nx), lg = ilog2(g.nx);
   dim3 grid((g.ndimr + P - 1) / P, g.ntchopdim);
@@ -203,7 +203,7 @@ __global__ void __launch_bounds__(256) k_horfft_bwd(KGrid g, int P, int lg, cons
   }
 }
 
-int launch_horfft_fwd(mlg_ctx *c, double2 *a) {
+static int launch_horfft_fwd_r2(mlg_ctx *c, double2 *a) {
   KGrid g = kgrid(c);
   int P = pencils_for(g.nth), lg = ilog2(g.nth);
   dim3 grid((g.ndimr + P - 1) / P, g.ndimx);
@@ -214,13 +214,319 @@ int launch_horfft_fwd(mlg_ctx *c, double2 *a) {
   return 0;
 }
 
-int launch_horfft_bwd(mlg_ctx *c, double2 *a) {
+static int launch_horfft_bwd_r2(mlg_ctx *c, double2 *a) {
   KGrid g = kgrid(c);
   int P = pencils_for(g.nth + g.ntchopdim), lg = ilog2(g.nth);
   dim3 grid((g.ndimr + P - 1) / P, g.ndimx);
   size_t lds = sizeof(double2) * (size_t)P * (g.nth + g.ntchopdim);
   KScope ks(c, K_HORFFT_BWD, 16.0 * g.nr * g.nx * (double)(g.nth + g.ntchopdim));
   hipLaunchKernelGGL(k_horfft_bwd, grid, dim3(256), lds, c->stream, g, P, lg, c->d.tw_t, c->d.tw_h, a);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// =========================================================================
+// Four-step register FFTs for N = R1*R2 <= 256 (R1, R2 powers of two <= 16).
+//   X[k1 + R1 k2] = sum_n2 W_R2^{n2 k2} W_N^{n2 k1} sum_n1 x[R2 n1 + n2] W_R1^{n1 k1}
+// Pass 1: thread (pencil r, n2) holds x[R2 n1 + n2], n1 < R1, in registers
+// and does the R1-point DFT; the products with W_N^{n2 k1} go to LDS.
+// Pass 2: thread (r, k1) reads its R2 values back and does the R2-point DFT.
+// Lanes run over consecutive pencils (radial nodes), so every global access
+// of a wave is a run of consecutive r.  Block: 256 threads, P = 256/TPP
+// pencils; LDS holds P*N values.
+// =========================================================================
+template <int R>
+struct Log2 {
+  static constexpr int v = R <= 1 ? 0 : 1 + Log2<R / 2>::v;
+};
+template <>
+struct Log2<1> {
+  static constexpr int v = 0;
+};
+
+__device__ __forceinline__ constexpr int cbitrev(int i, int lg) {
+  int r = 0;
+  for (int b = 0; b < lg; b++) r |= ((i >> b) & 1) << (lg - 1 - b);
+  return r;
+}
+
+// In-register R-point DFT (natural order in and out); wR[j] = W_R^j (j < R/2),
+// already conjugated for the inverse.
+template <int R>
+__device__ __forceinline__ void dft_reg(double2 (&v)[R], const double2 (&wR)[R / 2 > 0 ? R / 2 : 1]) {
+  constexpr int LG = Log2<R>::v;
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    const int j = cbitrev(i, LG);
+    if (j > i) {
+      double2 t = v[i];
+      v[i] = v[j];
+      v[j] = t;
+    }
+  }
+#pragma unroll
+  for (int half = 1; half < R; half *= 2) {
+#pragma unroll
+    for (int g0 = 0; g0 < R; g0 += 2 * half) {
+#pragma unroll
+      for (int p = 0; p < half; p++) {
+        const double2 w = wR[p * (R / (2 * half))];
+        double2 u = v[g0 + p], x = cmul(v[g0 + p + half], w);
+        v[g0 + p] = make_double2(u.x + x.x, u.y + x.y);
+        v[g0 + p + half] = make_double2(u.x - x.x, u.y - x.y);
+      }
+    }
+  }
+}
+
+template <int R>
+__device__ __forceinline__ void load_wr(double2 (&wR)[R / 2 > 0 ? R / 2 : 1], const double2 *__restrict__ tw, int N, bool inv) {
+#pragma unroll
+  for (int j = 0; j < (R / 2 > 0 ? R / 2 : 1); j++) {
+    double2 w = (R > 1) ? tw[j * (N / R)] : make_double2(1.0, 0.0);
+    if (inv) w.y = -w.y;
+    wR[j] = w;
+  }
+}
+
+template <int R1, int R2>
+struct Fft4 {
+  static constexpr int N = R1 * R2;
+  static constexpr int TPP = R1 > R2 ? R1 : R2;
+  static constexpr int P = 256 / TPP;
+};
+
+// Pass 1 + twiddle + LDS store + pass 2.  `v` enters holding x[R2 n1 + t]
+// (thread t < R2) and leaves holding X[t + R1 k2] (thread t < R1).
+template <int R1, int R2>
+__device__ __forceinline__ void fft4_core(double2 (&v1)[R1], double2 (&v2)[R2], double2 *s, int r, int t,
+                                          const double2 *__restrict__ tw, bool inv) {
+  constexpr int N = R1 * R2, P = Fft4<R1, R2>::P;
+  if (t < R2) {
+    double2 w1[R1 / 2 > 0 ? R1 / 2 : 1];
+    load_wr<R1>(w1, tw, N, inv);
+    dft_reg<R1>(v1, w1);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; k1++) {
+      double2 w = tw[t * k1];
+      if (inv) w.y = -w.y;
+      s[(k1 * R2 + t) * P + r] = (k1 == 0 || t == 0) ? v1[k1] : cmul(v1[k1], w);
+    }
+  }
+  __syncthreads();
+  if (t < R1) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; n2++) v2[n2] = s[(t * R2 + n2) * P + r];
+    double2 w2[R2 / 2 > 0 ? R2 / 2 : 1];
+    load_wr<R2>(w2, tw, N, inv);
+    dft_reg<R2>(v2, w2);
+  }
+}
+
+// VERFFT forward: PFP (r, m, z) -> PFF (r, m, k chopped), / NX.
+template <int R1, int R2>
+__global__ void __launch_bounds__(256) k_verfft_fwd4(KGrid g, const double2 *__restrict__ tw, const double2 *__restrict__ in,
+                                                     double2 *__restrict__ out) {
+  constexpr int N = R1 * R2, P = Fft4<R1, R2>::P;
+  __shared__ double2 s[P * N];
+  const int r = threadIdx.x % P, t = threadIdx.x / P;
+  const int mm = blockIdx.y, rr = blockIdx.x * P + r;
+  double2 v1[R1], v2[R2];
+  if (t < R2) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++)
+      v1[n1] = (rr < g.nr) ? in[(size_t)rr + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * (R2 * n1 + t))]
+                           : make_double2(0.0, 0.0);
+  }
+  fft4_core<R1, R2>(v1, v2, s, r, t, tw, false);
+  if (t < R1 && rr < g.ndimr) {
+    const double inv = (double)N;
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) {
+      const int k = t + R1 * k2;
+      int kk = -1;
+      if (k < g.nxchop) kk = k;
+      else if (k > N - g.nxchop) kk = g.nxchopdim + (k - N);
+      if (kk >= 0)
+        out[(size_t)rr + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * kk)] = make_double2(v2[k2].x / inv, v2[k2].y / inv);
+    }
+  }
+}
+
+// VERFFT backward: PFF (r, m, k chopped) -> PFP (r, m, z), unscaled.
+template <int R1, int R2>
+__global__ void __launch_bounds__(256) k_verfft_bwd4(KGrid g, const double2 *__restrict__ tw, const double2 *__restrict__ in,
+                                                     double2 *__restrict__ out) {
+  constexpr int N = R1 * R2, P = Fft4<R1, R2>::P;
+  __shared__ double2 s[P * N];
+  const int r = threadIdx.x % P, t = threadIdx.x / P;
+  const int mm = blockIdx.y, rr = blockIdx.x * P + r;
+  double2 v1[R1], v2[R2];
+  if (t < R2) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) {
+      const int k = R2 * n1 + t;
+      int kk = -1;
+      if (k < g.nxchop) kk = k;
+      else if (k > N - g.nxchop) kk = g.nxchopdim + (k - N);
+      v1[n1] = (rr < g.nr && kk >= 0) ? in[(size_t)rr + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * kk)]
+                                      : make_double2(0.0, 0.0);
+    }
+  }
+  fft4_core<R1, R2>(v1, v2, s, r, t, tw, true);
+  if (t < R1 && rr < g.ndimr) {
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++)
+      out[(size_t)rr + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * (t + R1 * k2))] = v2[k2];
+  }
+}
+
+// HORFFT forward, in place: PPP slots -> PFP m < NTCHOPDIM (split of the
+// packed real transform as in k_horfft_fwd).
+template <int R1, int R2>
+__global__ void __launch_bounds__(256) k_horfft_fwd4(KGrid g, const double2 *__restrict__ tw, const double2 *__restrict__ twh,
+                                                     double2 *__restrict__ a) {
+  constexpr int N = R1 * R2, P = Fft4<R1, R2>::P;
+  __shared__ double2 s[P * N];
+  const int r = threadIdx.x % P, t = threadIdx.x / P;
+  const int z = blockIdx.y, r0 = blockIdx.x * P, rr = r0 + r;
+  double2 v1[R1], v2[R2];
+  if (t < R2) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++)
+      v1[n1] = (rr < g.nr) ? a[(size_t)rr + (size_t)g.ndimr * ((size_t)(R2 * n1 + t) + (size_t)g.ndimth * z)]
+                           : make_double2(0.0, 0.0);
+  }
+  fft4_core<R1, R2>(v1, v2, s, r, t, tw, false);
+  __syncthreads();
+  if (t < R1) {
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) s[(t + R1 * k2) * P + r] = v2[k2];
+  }
+  __syncthreads();
+  const double den = (double)(2 * N);
+  const int mmax = g.ntchopdim;
+  for (int idx = threadIdx.x; idx < P * mmax; idx += 256) {
+    const int pr = idx % P, mm = idx / P;
+    if (r0 + pr >= g.ndimr) continue;
+    double2 zm = s[(mm % N) * P + pr], zc = cconj(s[((N - mm) % N) * P + pr]);
+    double2 e = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+    double2 d = make_double2(zm.x - zc.x, zm.y - zc.y);
+    double2 wd = cmul(twh[mm], d);
+    double2 x = make_double2(e.x + 0.5 * wd.y, e.y - 0.5 * wd.x);
+    x = make_double2(x.x / den, x.y / den);
+    if (mm == 0) x.y = 0.0;
+    if (mm >= g.ntchop || r0 + pr >= g.nr) x = make_double2(0.0, 0.0);
+    a[(size_t)(r0 + pr) + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * z)] = x;
+  }
+}
+
+// HORFFT backward, in place: PFP m < NTCHOPDIM -> PPP slots (merge as in
+// k_horfft_bwd, then an inverse length-NTH transform).
+template <int R1, int R2>
+__global__ void __launch_bounds__(256) k_horfft_bwd4(KGrid g, const double2 *__restrict__ tw, const double2 *__restrict__ twh,
+                                                     double2 *__restrict__ a) {
+  constexpr int N = R1 * R2, P = Fft4<R1, R2>::P;
+  __shared__ double2 s[P * N];
+  const int r = threadIdx.x % P, t = threadIdx.x / P;
+  const int z = blockIdx.y, r0 = blockIdx.x * P, rr = r0 + r;
+  const int mmax = g.ntchopdim;
+  for (int idx = threadIdx.x; idx < P * mmax; idx += 256) {
+    const int pr = idx % P, mm = idx / P;
+    double2 v = make_double2(0.0, 0.0);
+    if (r0 + pr < g.nr && mm < g.ntchop) v = a[(size_t)(r0 + pr) + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * z)];
+    if (mm == 0) v.y = 0.0;
+    s[mm * P + pr] = v;
+  }
+  __syncthreads();
+  double2 v1[R1], v2[R2];
+  if (t < R2) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) {
+      const int j = R2 * n1 + t;
+      double2 xm = (j < mmax) ? s[j * P + r] : make_double2(0.0, 0.0);
+      const int jc = N - j;
+      double2 xc = (jc < mmax && jc > 0 && j > 0) ? cconj(s[jc * P + r]) : make_double2(0.0, 0.0);
+      double2 e = make_double2(xm.x + xc.x, xm.y + xc.y);
+      double2 d = make_double2(xm.x - xc.x, xm.y - xc.y);
+      double2 o = cmul(d, cconj(twh[j]));
+      v1[n1] = make_double2(e.x - o.y, e.y + o.x);
+    }
+  }
+  __syncthreads();
+  fft4_core<R1, R2>(v1, v2, s, r, t, tw, true);
+  if (rr < g.ndimr) {
+    const bool live = rr < g.nr;
+    if (t < R1) {
+#pragma unroll
+      for (int k2 = 0; k2 < R2; k2++)
+        a[(size_t)rr + (size_t)g.ndimr * ((size_t)(t + R1 * k2) + (size_t)g.ndimth * z)] = live ? v2[k2] : make_double2(0.0, 0.0);
+    }
+    if (t == 0) a[(size_t)rr + (size_t)g.ndimr * ((size_t)N + (size_t)g.ndimth * z)] = make_double2(0.0, 0.0);
+  }
+}
+
+#define FFT4_SWITCH(N, CALL)                \
+  switch (N) {                              \
+    case 2: CALL(2, 1); break;              \
+    case 4: CALL(4, 1); break;              \
+    case 8: CALL(8, 1); break;              \
+    case 16: CALL(16, 1); break;            \
+    case 32: CALL(16, 2); break;            \
+    case 64: CALL(8, 8); break;             \
+    case 128: CALL(16, 8); break;           \
+    case 256: CALL(16, 16); break;          \
+    default: break;                         \
+  }
+
+int launch_verfft_fwd(mlg_ctx *c, const double2 *in, double2 *out) {
+  KGrid g = kgrid(c);
+  if (g.nx > 256) return launch_verfft_fwd_r2(c, in, out);
+  KScope ks(c, K_VERFFT_FWD, 16.0 * g.nr * g.ntchopdim * (double)(g.nx + g.nxchopdim));
+#define L_VF(a, b)                                                                                             \
+  hipLaunchKernelGGL((k_verfft_fwd4<a, b>), dim3((g.ndimr + Fft4<a, b>::P - 1) / Fft4<a, b>::P, g.ntchopdim), \
+                     dim3(256), 0, c->stream, g, c->d.tw_z, in, out)
+  FFT4_SWITCH(g.nx, L_VF)
+#undef L_VF
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_verfft_bwd(mlg_ctx *c, const double2 *in, double2 *out) {
+  KGrid g = kgrid(c);
+  if (g.nx > 256) return launch_verfft_bwd_r2(c, in, out);
+  KScope ks(c, K_VERFFT_BWD, 16.0 * g.nr * g.ntchopdim * (double)(g.nx + g.nxchopdim));
+#define L_VB(a, b)                                                                                             \
+  hipLaunchKernelGGL((k_verfft_bwd4<a, b>), dim3((g.ndimr + Fft4<a, b>::P - 1) / Fft4<a, b>::P, g.ntchopdim), \
+                     dim3(256), 0, c->stream, g, c->d.tw_z, in, out)
+  FFT4_SWITCH(g.nx, L_VB)
+#undef L_VB
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_horfft_fwd(mlg_ctx *c, double2 *a) {
+  KGrid g = kgrid(c);
+  if (g.nth > 256) return launch_horfft_fwd_r2(c, a);
+  KScope ks(c, K_HORFFT_FWD, 16.0 * g.nr * g.nx * (double)(g.nth + g.ntchopdim));
+#define L_HF(a_, b_)                                                                                         \
+  hipLaunchKernelGGL((k_horfft_fwd4<a_, b_>), dim3((g.ndimr + Fft4<a_, b_>::P - 1) / Fft4<a_, b_>::P, g.ndimx), \
+                     dim3(256), 0, c->stream, g, c->d.tw_t, c->d.tw_h, a)
+  FFT4_SWITCH(g.nth, L_HF)
+#undef L_HF
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_horfft_bwd(mlg_ctx *c, double2 *a) {
+  KGrid g = kgrid(c);
+  if (g.nth > 256) return launch_horfft_bwd_r2(c, a);
+  KScope ks(c, K_HORFFT_BWD, 16.0 * g.nr * g.nx * (double)(g.nth + g.ntchopdim));
+#define L_HB(a_, b_)                                                                                         \
+  hipLaunchKernelGGL((k_horfft_bwd4<a_, b_>), dim3((g.ndimr + Fft4<a_, b_>::P - 1) / Fft4<a_, b_>::P, g.ndimx), \
+                     dim3(256), 0, c->stream, g, c->d.tw_t, c->d.tw_h, a)
+  FFT4_SWITCH(g.nth, L_HB)
+#undef L_HB
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

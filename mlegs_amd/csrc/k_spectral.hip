@@ -523,8 +523,16 @@ __device__ void gen_row(const SolveCtx &s, int q, double *out, int NE) {
   }
 }
 
+// Factor storage: row nn of class cls (= m index + NTCHOPDIM*|k| index)
+// keeps FSTRIDE doubles at F[(nn*FSTRIDE + e)*ncls + cls]:
+//   e < C0          multiplier of row nn against row nn-1-e (scaled by its pivot)
+//   C0 <= e < C0+KU normalised upper entry (nn, nn+1+e-C0)
+//   e = NE-1        1/pivot
+// so a wave of consecutive m reads consecutive addresses.
+#define FSTRIDE 7
+
 template <int NE, int C0>
-__device__ void band_lu_solve(const SolveCtx &s, int nd, double2 *x, double *ust, long nsys, long sys) {
+__device__ void band_factor(const SolveCtx &s, int nd, double *F, long ncls, long cls) {
   constexpr int KU = NE - 1 - C0;
   double W[C0 + 1][NE];
   for (int q = 0; q <= C0; q++) {
@@ -532,8 +540,6 @@ __device__ void band_lu_solve(const SolveCtx &s, int nd, double2 *x, double *ust
     else
       for (int b = 0; b < NE; b++) W[q][b] = 0.0;
   }
-  double2 xh[C0 + 1];  // forward-substituted history x[nn-1], x[nn-2], ...
-  for (int q = 0; q <= C0; q++) xh[q] = zero2();
   for (int nn = 0; nn < nd; nn++) {
     const double piv = W[0][C0];
     for (int b = C0 + 1; b < NE; b++)
@@ -545,68 +551,112 @@ __device__ void band_lu_solve(const SolveCtx &s, int nd, double2 *x, double *ust
       }
     for (int j = 1; j <= C0; j++)
       if (nn + j < nd) W[j][C0 - j] /= piv;
-    // forward substitution of row nn (its multipliers were final before this pivot)
-    double2 v = x[nn];
-    int nne = nn < C0 ? nn : C0;
-    for (int j = 1; j <= nne; j++) v = sub2(v, axc(W[0][C0 - j], xh[j - 1]));
-    for (int q = C0; q >= 1; q--) xh[q] = xh[q - 1];
-    xh[0] = v;
-    const double rp = 1.0 / piv;
-    x[nn] = axc(rp, v);
-    for (int u = 0; u < KU; u++) ust[((size_t)nn * KU + u) * nsys + sys] = W[0][C0 + 1 + u];
+    double *fr = F + (size_t)nn * FSTRIDE * ncls + cls;
+    for (int j = 1; j <= C0; j++) fr[(size_t)(j - 1) * ncls] = W[0][C0 - j];
+    for (int u = 0; u < KU; u++) fr[(size_t)(C0 + u) * ncls] = W[0][C0 + 1 + u];
+    fr[(size_t)(NE - 1) * ncls] = 1.0 / piv;
     for (int q = 0; q < C0; q++)
       for (int b = 0; b < NE; b++) W[q][b] = W[q + 1][b];
     if (nn + C0 + 1 < nd) gen_row(s, nn + C0 + 1, W[C0], NE);
     else
       for (int b = 0; b < NE; b++) W[C0][b] = 0.0;
   }
+}
+
+// Forward substitution (multipliers applied to the unscaled history, then
+// scaled by 1/pivot) and back substitution with the normalised U rows:
+// the update order of LUBKSBB.
+template <int NE, int C0>
+__device__ void band_apply(int nd, double2 *x, const double *__restrict__ F, long ncls, long cls) {
+  constexpr int KU = NE - 1 - C0;
+  double2 xh[C0];
+  for (int q = 0; q < C0; q++) xh[q] = zero2();
+  for (int nn = 0; nn < nd; nn++) {
+    const double *fr = F + (size_t)nn * FSTRIDE * ncls + cls;
+    double2 v = x[nn];
+    int nne = nn < C0 ? nn : C0;
+    for (int j = 1; j <= nne; j++) v = sub2(v, axc(fr[(size_t)(j - 1) * ncls], xh[j - 1]));
+    for (int q = C0 - 1; q >= 1; q--) xh[q] = xh[q - 1];
+    xh[0] = v;
+    x[nn] = axc(fr[(size_t)(NE - 1) * ncls], v);
+  }
   double2 yh[KU];
   for (int u = 0; u < KU; u++) yh[u] = zero2();
   for (int nn = nd - 1; nn >= 0; nn--) {
+    const double *fr = F + (size_t)nn * FSTRIDE * ncls + cls;
     double2 v = x[nn];
     if (nn <= nd - 2)
       for (int i = 1; i <= KU; i++)
-        if (nn + i < nd) v = sub2(v, axc(ust[((size_t)nn * KU + (i - 1)) * nsys + sys], yh[i - 1]));
+        if (nn + i < nd) v = sub2(v, axc(fr[(size_t)(C0 + i - 1) * ncls], yh[i - 1]));
     x[nn] = v;
     for (int u = KU - 1; u >= 1; u--) yh[u] = yh[u - 1];
     yh[0] = v;
   }
 }
 
-__global__ void __launch_bounds__(64) k_band_solve(KGrid g, const double *bd, const double *bxm, const double *ak, int op,
-                                                   double alp, int has_ln, double lnval, double2 *a, double *ln_a,
-                                                   const double *ln_b, double *ust) {
-  const long nsys = (long)g.ntchopdim * g.nxchopdim;
-  const long sys = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (sys >= nsys) return;
-  const int mm = (int)(sys % g.ntchopdim), k = (int)(sys / g.ntchopdim);
-  const int mv = mm * g.minc, nn = s_nrchops(g.nrchop, mv);
-  if (nn < 1 || mm >= g.ntchop) return;
-  double2 *x = a + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
-  SolveCtx s;
+__device__ void solve_setup(const KGrid &g, const double *bd, const double *bxm, const double *ak, int op, double alp, int mm,
+                            int kidx, SolveCtx &s) {
+  const int mv = mm * g.minc;
   s.g = g;
   s.bd = bd;
   s.bxm = bxm;
   s.ak = ak;
   s.op = op;
   s.mm = mm;
-  s.k = k;
-  s.np = nn + 1;
-  double akv = ak[mm + (size_t)g.ntchopdim * k];
+  s.k = kidx;
+  s.np = s_nrchops(g.nrchop, mv) + 1;
+  double akv = ak[mm + (size_t)g.ntchopdim * kidx];
   s.k2 = akv * akv;
   s.alp = (op == 2) ? alp : 0.0;
   s.kind = 0;
   if (mm == 0) {
-    if (op == 0 && k == 0) s.kind = 1;
+    if (op == 0 && kidx == 0) s.kind = 1;
     if (op == 2) s.kind = 2;
   }
-  if (s.kind == 0) {
-    band_lu_solve<5, 2>(s, nn, x, ust, nsys, sys);
-  } else if (s.kind == 1) {
+}
+
+// One thread per (m, |k|): factorise the operator's band matrix.
+__global__ void __launch_bounds__(64) k_band_factor(KGrid g, const double *bd, const double *bxm, const double *ak, int op,
+                                                    double alp, double *F) {
+  const long ncls = (long)g.ntchopdim * g.nxchop;
+  const long cls = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (cls >= ncls) return;
+  const int mm = (int)(cls % g.ntchopdim), kabs = (int)(cls / g.ntchopdim);
+  const int nn = s_nrchops(g.nrchop, mm * g.minc);
+  if (nn < 1 || mm >= g.ntchop) return;
+  SolveCtx s;
+  solve_setup(g, bd, bxm, ak, op, alp, mm, kabs, s);
+  if (s.kind == 0) band_factor<5, 2>(s, nn, F, ncls, cls);
+  else if (s.kind == 1) band_factor<7, 3>(s, nn, F, ncls, cls);
+  else band_factor<7, 3>(s, nn - 1, F, ncls, cls);
+}
+
+// One thread per (m, k): right-hand-side set-up, substitution, result
+// post-processing of IDEL2 / IDEL2LN / IHELM (src/mod_legops.f90:823-1354).
+__global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, int op, double alp, int has_ln, double lnval,
+                                                   double2 *a, double *ln_a, const double *ln_b, const double *__restrict__ F) {
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  const long ncls = (long)g.ntchopdim * g.nxchop;
+  const long sys = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const int mm = (int)(sys % g.ntchopdim), k = (int)(sys / g.ntchopdim);
+  const int mv = mm * g.minc, nn = s_nrchops(g.nrchop, mv);
+  if (nn < 1 || mm >= g.ntchop) return;
+  const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
+  const long cls = mm + (long)g.ntchopdim * kabs;
+  double2 *x = a + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
+  int kind = 0;
+  if (mm == 0) {
+    if (op == 0 && k == 0) kind = 1;
+    if (op == 2) kind = 2;
+  }
+  if (kind == 0) {
+    band_apply<5, 2>(nn, x, F, ncls, cls);
+  } else if (kind == 1) {
     // IDEL2 m0k0: first row is the log-term (Dirichlet) row
     for (int i = nn - 1; i >= 1; i--) x[i] = x[i - 1];
     x[0] = make_double2(has_ln ? lnval / g.norm0 : 0.0, 0.0);
-    band_lu_solve<7, 3>(s, nn, x, ust, nsys, sys);
+    band_apply<7, 3>(nn, x, F, ncls, cls);
   } else {
     // IHELM m=0: solve (del^2 - alp - k^2)(1-x) y = rhs on NN-1 rows, A = (1-x) y
     if (k == 0) {
@@ -616,13 +666,11 @@ __global__ void __launch_bounds__(64) k_band_solve(KGrid g, const double *bd, co
       x[1].x = x[1].x - lnA * (-2 / g.norm1 / g.ell2);
       x[2].x = x[2].x - lnA * (+2.0 / 3 / g.norm2 / g.ell2);
     }
-    int nm = nn - 1;
-    band_lu_solve<7, 3>(s, nm, x, ust, nsys, sys);
+    band_apply<7, 3>(nn - 1, x, F, ncls, cls);
     x[nn - 1] = zero2();
-    const double *xt = bxm;
     double2 prev = zero2();
     for (int c = 0; c < nn; c++) {
-      const double *r = xt + ((size_t)mm * g.nband + c) * 3;
+      const double *r = bxm + ((size_t)mm * g.nband + c) * 3;
       double2 cur = x[c];
       double2 o = axc(r[1], cur);
       if (c >= 1) o = add2(o, axc(r[0], prev));
@@ -640,14 +688,46 @@ __global__ void __launch_bounds__(64) k_band_solve(KGrid g, const double *bd, co
   }
 }
 
+// Factor sets are cached per (operator, alp, truncation): the matrices
+// depend on m, k^2 and those only, so repeated solves (every CN step) only
+// substitute.  Results are bitwise those of factorising on every call.
+static double *factor_set(mlg_ctx *c, int op, double alp, int *rc) {
+  KGrid g = kgrid(c);
+  double key_alp = (op == 2) ? alp : 0.0;
+  for (auto &f : c->fcache)
+    if (f.op == op && f.alp == key_alp && f.nrchop == g.nrchop) return f.f;
+  const long ncls = (long)g.ntchopdim * g.nxchop;
+  size_t n = (size_t)g.nrchopdim * FSTRIDE * ncls;
+  if (c->fcache.size() >= 8) {
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(c->fcache.front().f);
+    c->fcache.erase(c->fcache.begin());
+  }
+  double *F = nullptr;
+  if (hipMalloc((void **)&F, sizeof(double) * n) != hipSuccess) {
+    *rc = MLG_E_NOMEM;
+    return nullptr;
+  }
+  hipLaunchKernelGGL(k_band_factor, dim3((ncls + 63) / 64), dim3(64), 0, c->stream, g, c->d.bdel2h, c->d.bxm, c->d.ak, op,
+                     key_alp, F);
+  if (hipGetLastError() != hipSuccess) {
+    (void)hipFree(F);
+    *rc = MLG_E_HIP;
+    return nullptr;
+  }
+  c->fcache.push_back(FactorSet{op, key_alp, g.nrchop, F});
+  return F;
+}
+
 int launch_band_solve(mlg_ctx *c, int op, double2 *a, double *ln_a, const double *ln_b, double alp, int has_ln, double ln) {
   KGrid g = kgrid(c);
   long nsys = (long)g.ntchopdim * g.nxchopdim;
-  size_t need = (size_t)(g.nrchopdim + 2) * 3 * nsys;
-  if (need > c->d.lu_len) return MLG_E_ARG;
+  int rc = 0;
+  const double *F = factor_set(c, op, alp, &rc);
+  if (!F) return rc;
   KScope ks(c, K_BAND_SOLVE, 2.0 * 16.0 * fff_count(g));  // the coefficients read and written once
-  hipLaunchKernelGGL(k_band_solve, dim3((nsys + 63) / 64), dim3(64), 0, c->stream, g, c->d.bdel2h, c->d.bxm, c->d.ak, op, alp,
-                     has_ln, ln, a, ln_a, ln_b, c->d.lu);
+  hipLaunchKernelGGL(k_band_apply, dim3((nsys + 63) / 64), dim3(64), 0, c->stream, g, c->d.bxm, op, alp, has_ln, ln, a, ln_a,
+                     ln_b, F);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

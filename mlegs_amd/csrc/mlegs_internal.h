@@ -43,8 +43,6 @@ struct DevTables {
   double2 *tw_t = nullptr;    // e^{-2 pi i j / NTH}, j < NTH
   double2 *tw_h = nullptr;    // e^{-2 pi i j / (2 NTH)}, j <= NTH
   double *hv = nullptr;       // [hv_r | hv_t | hv_x | hd_r | hd_t | hd_x]
-  double *lu = nullptr;       // band-solver factor storage
-  size_t lu_len = 0;
   double *red = nullptr;      // reduction scratch
   int *err = nullptr;         // sticky device error code
 };
@@ -71,6 +69,13 @@ struct EventPair {
   hipEvent_t a, b;
   double work;
 };
+// Cached band-matrix factors of one solve operator (see k_spectral.hip).
+struct FactorSet {
+  int op;
+  double alp;
+  int nrchop;
+  double *f;
+};
 struct KTimer {
   bool on = false;
   std::vector<EventPair> rec[K_COUNT];
@@ -87,6 +92,7 @@ struct mlg_ctx {
   long n_fff = 0, n_pff = 0, n_ppp = 0, n_max = 0;
   std::vector<mlg_field *> pool;  // reusable temporaries
   std::unordered_set<mlg_field *> owned;  // fields created through the C ABI
+  std::vector<mlg::FactorSet> fcache;     // band-solve factors
   int device = 0;
 };
 

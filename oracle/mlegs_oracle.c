@@ -439,6 +439,9 @@ const double *og_table(const og_grid *g, const char *name, int *n) {
   if (!strcmp(name, "at1")) { *n = g->nrchopdim; return g->at1; }
   if (!strcmp(name, "ak")) { *n = g->ntchopdim * g->nxchopdim; return g->ak; }
   if (!strcmp(name, "hv_r") && g->hv_r) { *n = g->p.nrchop * g->ntchopdim; return g->hv_r; }
+  if (!strcmp(name, "hv_t") && g->hv_t) { *n = g->ntchopdim; return g->hv_t; }
+  if (!strcmp(name, "hv_x") && g->hv_x) { *n = g->nxchopdim; return g->hv_x; }
+  if (!strcmp(name, "hd_r") && g->hd_r) { *n = g->p.nrchop * g->ntchopdim; return g->hd_r; }
   *n = 0;
   return NULL;
 }

@@ -56,3 +56,12 @@ def test_params_struct_layout_matches_oracle():
     import _oracle as O
     assert [f[0] for f in H.Params._fields_] == [f[0] for f in O.OgParams._fields_]
     assert ctypes.sizeof(H.Params) == ctypes.sizeof(O.OgParams)
+
+
+def test_fortran_binding_built_and_linked():
+    drv = os.path.join(ROOT, "mlegs_amd", "fortran", "_build", "drive_bsnsq")
+    assert os.path.exists(drv), "run `make -C mlegs_amd/fortran`"
+    out = subprocess.check_output(["ldd", drv], text=True)
+    line = [l for l in out.splitlines() if "libmlegs_hip.so" in l]
+    assert line and "not found" not in line[0]
+    assert os.path.realpath(line[0].split("=>")[1].split("(")[0].strip()) == os.path.realpath(H.library_path())

@@ -170,3 +170,40 @@ def test_etd_operator_is_expm():
     assert np.allclose(E, expm(L * dt), atol=1e-14)
     assert np.allclose(PHI, np.linalg.solve(L, expm(L * dt) - np.eye(4)), atol=1e-14)
     g.close()
+
+
+def test_hyperviscosity_filter_special_values():
+    """HYP3SET/HYP3MASK (src/mod_march.f90:3320-3480): H = exp(-nu_p mask
+    (K^2/K^2_max)^(P/2)) with a tanh ramp mask that is 0 below the ramp
+    start, 1/2 at mid-ramp and 1 at the top wavenumber (ramp starts: 1/2 of
+    the top radial index, 2/3 of the top m and |k|)."""
+    cfg = dict(nr=32, nth=32, nx=32, nrchop=21, ntchop=13, nxchop=13, minc=1, mklink=0, ell=1.0,
+               zlen=2 * np.pi, nu=1e-3, kappa=1e-3, nup=0.3, kappap=0.2, visc_sw=3, visc_p=4,
+               omega=0.5, bv0=1.0, adams=1, hyper_cn=0, dt=1e-3)
+    g = O.OGrid(cfg)
+    try:
+        nu, ph = 0.3, 2
+        ht, hx = g.table("hv_t"), g.table("hv_x")
+        hr = g.table("hv_r").reshape(21, 13, order="F")
+        # azimuthal: m = 0..12, top 12, ramp from 8, midpoint 10
+        assert np.all(ht[:9] == 1.0)
+        assert ht[12] == pytest.approx(np.exp(-nu), rel=1e-14)
+        assert ht[10] == pytest.approx(np.exp(-nu * 0.5 * (100 / 144) ** ph), rel=1e-14)
+        assert np.all(np.diff(ht) <= 0)
+        # axial: |k| = 0..12 (both signs)
+        kk = np.arange(25)
+        kabs = np.where(kk < 13, kk, 25 - kk)
+        for k in range(25):
+            if kabs[k] <= 8:
+                assert hx[k] == 1.0
+        assert hx[12] == pytest.approx(np.exp(-nu), rel=1e-14) and hx[13] == hx[12]
+        assert hx[10] == pytest.approx(np.exp(-nu * 0.5 * (100 / 144) ** ph), rel=1e-14)
+        # radial at m = 0: i = 0..20, top 20, ramp from 10, midpoint 15; K^2 ~ i(i+1)
+        assert np.all(hr[:10, 0] == 1.0)
+        assert hr[20, 0] == pytest.approx(np.exp(-nu), rel=1e-14)
+        assert hr[15, 0] == pytest.approx(np.exp(-nu * 0.5 * (15 * 16 / (20 * 21)) ** ph), rel=1e-14)
+        # diffusivity filter uses kappa_p
+        hd = g.table("hd_r").reshape(21, 13, order="F")
+        assert hd[20, 0] == pytest.approx(np.exp(-0.2), rel=1e-14)
+    finally:
+        g.close()
