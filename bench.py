@@ -9,9 +9,11 @@ Lamb-Oseen vortex (QVORTEX, Gamma=1) plus seeded complex Gaussian noise
 
 A step is one full ETD2-AB/CN step (mod_march.f90:811-901) including the
 per-step DIAGNOST checks.  The Richardson start and W warm-up steps are
-untimed; K steps are timed between device synchronisations.  During the
-timed steps the library records HIP events around every launch of its
-kernel classes on its own stream; the class with the largest summed time is
+untimed; K steps are timed between device synchronisations, as replays of
+the captured step graph (--eager: launched kernel by kernel).  The
+per-kernel figures come from a second timed region of K eager steps, in
+which the library records HIP events around every launch of its kernel
+classes on its own stream; the class with the largest summed time is
 reported in `roofline` with its algorithmic work per launch.  The CPU
 baseline times the FP64 C restatement (OpenMP) on this host's cores over a
 bounded sample of steps of the same workload.
@@ -36,6 +38,8 @@ def parse():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-timer", action="store_true", help="time without per-kernel events")
+    ap.add_argument("--eager", action="store_true",
+                    help="time eager launches instead of replays of the captured step graph")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 20 copies of a 384x257x256 complex field (407 MB each way), "
                          "to scale HBM byte counters")
@@ -124,20 +128,34 @@ def main():
     psi, chi, b = initial_state(H, g, cfg)
     s = H.Solver(g)
     s.initialize(psi, chi, b)
+    use_graph = not a.eager
     if a.warmup:
-        s.time_stepping(psi, chi, b, a.warmup)
+        s.time_stepping(psi, chi, b, max(a.warmup, 2 if use_graph else 1), graph=use_graph)
     torch.cuda.synchronize()
     g.synchronize()
-    print(f"bench: initialised, {a.warmup} warm-up steps done; timing {a.steps} steps", file=sys.stderr, flush=True)
-    if not a.no_timer:
+    print(f"bench: initialised, warm-up done; timing {a.steps} steps ({'graph' if use_graph else 'eager'})",
+          file=sys.stderr, flush=True)
+    if not a.no_timer and not use_graph:
         g.timer(True)
     t0 = time.perf_counter()
-    s.time_stepping(psi, chi, b, a.steps)
+    s.time_stepping(psi, chi, b, a.steps, graph=use_graph)
     torch.cuda.synchronize()
     g.synchronize()
     el = time.perf_counter() - t0
-    g.timer(False)
     ms_step = el / a.steps * 1e3
+    if not a.no_timer and use_graph:
+        # per-kernel HIP events need eager launches: a second timed region of
+        # the same K steps, launched one kernel at a time
+        print("bench: eager timed region for the per-kernel events", file=sys.stderr, flush=True)
+        g.timer(True)
+        t1 = time.perf_counter()
+        s.time_stepping(psi, chi, b, a.steps)
+        torch.cuda.synchronize()
+        g.synchronize()
+        eager_ms = (time.perf_counter() - t1) / a.steps * 1e3
+    else:
+        eager_ms = ms_step
+    g.timer(False)
 
     stages, dom = {}, None
     if not a.no_timer:
@@ -184,6 +202,8 @@ def main():
                    "ntchop": cfg["ntchop"], "nxchop": cfg["nxchop"], "scheme": "ETD2-AB/CN + P=4 filter",
                    "precision": "fp64"},
         "roofline": roof,
+        "launch": "hip graph replay of the captured step" if use_graph else "eager",
+        "eager_ms_per_step": round(eager_ms, 3),
         "cpu_baseline": cpu,
         "stages": stages,
     }

@@ -176,6 +176,13 @@ int mlg_time_step(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b);
 /* nsteps TIME_STEPPING calls queued back to back; the checks run on the
  * device every step and the first failure is returned at the end. */
 int mlg_time_steps(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, int nsteps);
+/* The same stepping through a HIP graph: the first call on (psi, chi, b)
+ * runs one step eagerly, captures the next one and replays the captured
+ * step for the rest; later calls on the same fields only replay.  No
+ * counterpart in the reference (it replaces the per-step launch sequence of
+ * SOLVER_T%TIME_STEPPING, src/mod_march.f90:243-307).  Results equal
+ * mlg_time_steps bit for bit. */
+int mlg_time_steps_graph(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, int nsteps);
 double mlg_solver_time(const mlg_solver *s);
 int mlg_solver_nstep(const mlg_solver *s);
 

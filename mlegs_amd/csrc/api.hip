@@ -349,8 +349,9 @@ static int new_field(mlg_ctx *c, int space, mlg_field **out) {
     delete f;
     return MLG_E_NOMEM;
   }
-  MLG_CHECK_HIP(hipMemsetAsync(f->e, 0, sizeof(double2) * (size_t)c->n_max, c->stream));
-  MLG_CHECK_HIP(hipMemsetAsync(f->ln, 0, sizeof(double) * 2, c->stream));
+  // cleared with kernels (no memset nodes if this ever runs under capture)
+  RET(launch_zero(c, f->e, c->n_max));
+  RET(launch_set_ln(c, f->ln, nullptr, 0.0, 0.0));
   *out = f;
   return 0;
 }
@@ -383,25 +384,41 @@ static mlg_field *tmp(mlg_ctx *c, int space) {
   }
   f->space = space;
   f->pooled = true;
-  launch_set_ln(c, f->ln, nullptr, 0.0, 0.0);
   return f;
 }
 static void rel(mlg_ctx *c, mlg_field *f) {
   if (f) c->pool.push_back(f);
 }
+// Temporaries of one routine.  get() hands out pool buffers; ready() checks
+// them and zeroes all their LN coefficients with one launch.
 struct Temps {
   mlg_ctx *c;
   std::vector<mlg_field *> v;
+  std::vector<double *> pending;
+  bool failed = false;
   explicit Temps(mlg_ctx *cc) : c(cc) {}
   mlg_field *get(int space = MLG_FFF) {
     mlg_field *f = tmp(c, space);
-    if (f) v.push_back(f);
+    if (f) {
+      v.push_back(f);
+      pending.push_back(f->ln);
+    } else {
+      failed = true;
+    }
     return f;
   }
-  bool ok() const {
-    for (auto *f : v)
-      if (!f) return false;
-    return true;
+  // a buffer whose LN is never read (it is copied over, or only its E is used)
+  mlg_field *buffer(int space = MLG_FFF) {
+    mlg_field *f = tmp(c, space);
+    if (f) v.push_back(f);
+    else failed = true;
+    return f;
+  }
+  int ready() {
+    if (failed) return MLG_E_NOMEM;
+    int rc = pending.empty() ? 0 : launch_zero_lns(c, pending.data(), (int)pending.size());
+    pending.clear();
+    return rc;
   }
   ~Temps() {
     for (auto *f : v) rel(c, f);
@@ -479,8 +496,8 @@ extern "C" int mlg_rtran(mlg_field *a, int is) {
   if (!a) return MLG_E_ARG;
   mlg_ctx *c = a->ctx;
   Temps T(c);
-  mlg_field *t = T.get();
-  if (!T.ok()) return MLG_E_NOMEM;
+  mlg_field *t = T.buffer();
+  RET(T.ready());
   if (is == -1) {
     if (a->space != MLG_PFF) return 2;
     RET(launch_rtran_fwd(c, a->e, t->e));
@@ -501,8 +518,8 @@ extern "C" int mlg_verfft(mlg_field *a, int is) {
   if (!a) return MLG_E_ARG;
   mlg_ctx *c = a->ctx;
   Temps T(c);
-  mlg_field *t = T.get();
-  if (!T.ok()) return MLG_E_NOMEM;
+  mlg_field *t = T.buffer();
+  RET(T.ready());
   if (is == -1) {
     if (a->space != MLG_PFP) return 2;
     RET(launch_verfft_fwd(c, a->e, t->e));
@@ -564,8 +581,8 @@ extern "C" int mlg_plnfft(mlg_field *a, int is) {
   Temps _T((inp)->ctx);                               \
   const mlg_field *inp##_src = (inp);                 \
   if ((const void *)(inp) == (const void *)(outp)) {  \
-    mlg_field *_t = _T.get();                         \
-    if (!_t) return MLG_E_NOMEM;                      \
+    mlg_field *_t = _T.buffer();                      \
+    RET(_T.ready());                                  \
     RET(copy0(_t, (inp)));                            \
     inp##_src = _t;                                   \
   }
@@ -619,8 +636,8 @@ static int solve_op(const mlg_field *b, mlg_field *a, int op, double alp, int ha
   Temps T(c);
   const mlg_field *bs = b;
   if (a == b) {
-    mlg_field *t = T.get();
-    if (!t) return MLG_E_NOMEM;
+    mlg_field *t = T.buffer();
+    RET(T.ready());
     RET(copy0(t, b));
     bs = t;
   }
@@ -644,8 +661,8 @@ extern "C" int mlg_grad(const mlg_field *a, mlg_field *rr, mlg_field *rp, mlg_fi
   Temps T(a->ctx);
   const mlg_field *as = a;
   if (a == rr || a == rp || a == z) {
-    mlg_field *t = T.get();
-    if (!t) return MLG_E_NOMEM;
+    mlg_field *t = T.buffer();
+    RET(T.ready());
     RET(copy0(t, a));
     as = t;
   }
@@ -740,7 +757,7 @@ extern "C" int mlg_force(const mlg_field *psi, const mlg_field *chi, const mlg_f
   Temps T(c);
   mlg_field *rur = T.get(), *rup = T.get(), *uz = T.get(), *ror = T.get(), *rop = T.get(), *oz = T.get();
   mlg_field *rbr = T.get(), *rbp = T.get(), *bz = T.get(), *w = T.get(), *w2 = T.get();
-  if (!T.ok()) return MLG_E_NOMEM;
+  RET(T.ready());
   RET(mlg_chopset(c, 3));
   int rc = mlg_grad(b, rbr, rbp, bz);
   if (!rc) rc = mlg_pc2vel(psi, chi, rur, rup, uz);
@@ -772,7 +789,7 @@ extern "C" int mlg_nonlin(const mlg_field *b, mlg_field *rur, mlg_field *rup, ml
   mlg_ctx *c = b->ctx;
   Temps T(c);
   mlg_field *rbr = T.get(), *rbp = T.get(), *bz = T.get(), *psi = T.get(), *chi = T.get(), *w = T.get();
-  if (!T.ok()) return MLG_E_NOMEM;
+  RET(T.ready());
   RET(mlg_chopset(c, 1));
   int rc = mlg_grad(b, rbr, rbp, bz);
   mlg_chopset(c, -1);
@@ -838,7 +855,7 @@ extern "C" int mlg_energy(const mlg_field *psi, const mlg_field *chi, const mlg_
   if (bln != 0.0) return 13;
   Temps T(c);
   mlg_field *w1 = T.get(), *w2 = T.get(), *w3 = T.get();
-  if (!T.ok()) return MLG_E_NOMEM;
+  RET(T.ready());
   const int nt = G.ntchop, nxc = G.nxchop, nxd = G.nxchopdim;
   const size_t na = (size_t)nt * nxd;
   double *dall = c->d.red;  // [4][nt*nxd]
@@ -907,8 +924,10 @@ extern "C" int mlg_solver_create(mlg_ctx *c, mlg_solver **out) {
   *out = s;
   return 0;
 }
+static void drop_graph(mlg_solver *s);
 extern "C" int mlg_solver_destroy(mlg_solver *s) {
   if (!s) return MLG_E_ARG;
+  drop_graph(s);
   mlg_field *fs[9] = {s->n_psi, s->n_chi, s->n_b, s->r_n_r, s->r_n_p, s->n_z, s->r_pi_r, s->r_pi_p, s->pi_z};
   (void)hipStreamSynchronize(s->ctx->stream);
   for (auto f : fs)
@@ -977,7 +996,7 @@ static int visc_be(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, dou
   if (c->g.p.visc_sw != 1) return 0;
   Temps T(c);
   mlg_field *wk = T.get();
-  if (!wk) return MLG_E_NOMEM;
+  RET(T.ready());
   RET(visc1(psi, c->g.p.nu, dt, wk));
   RET(visc1(chi, c->g.p.nu, dt, wk));
   if (c->g.p.kappa != 0.0) RET(visc1(b, c->g.p.kappa, dt, wk));
@@ -989,7 +1008,7 @@ static int visc_cn(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, con
   if (c->g.p.visc_sw != 1) return 0;
   Temps T(c);
   mlg_field *wk = T.get();
-  if (!wk) return MLG_E_NOMEM;
+  RET(T.ready());
   RET(visc2(psi, psio, c->g.p.nu, c->g.p.dt, wk));
   RET(visc2(chi, chio, c->g.p.nu, c->g.p.dt, wk));
   if (c->g.p.kappa != 0.0) RET(visc2(b, bo, c->g.p.kappa, c->g.p.dt, wk));
@@ -1024,7 +1043,7 @@ static int rich_fe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *
   s->n += 1;
   Temps T(c);
   mlg_field *psi2 = T.get(), *chi2 = T.get(), *b2 = T.get(), *pn = T.get(), *cn = T.get(), *bnn = T.get();
-  if (!T.ok()) return MLG_E_NOMEM;
+  RET(T.ready());
   RET(mlg_force(psi, chi, b, s->n_psi, s->n_chi, s->n_b));
   RET(euler(psi2, psi, hdt, s->n_psi, true));
   RET(euler(chi2, chi, hdt, s->n_chi, true));
@@ -1055,7 +1074,7 @@ static int step_ab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *
   s->n += 1;
   Temps T(c);
   mlg_field *psin = T.get(), *chin = T.get(), *bn = T.get(), *psi0 = T.get(), *chi0 = T.get(), *b0 = T.get();
-  if (!T.ok()) return MLG_E_NOMEM;
+  RET(T.ready());
   RET(mlg_force(psi, chi, b, psin, chin, bn));
   RET(copy0(psi0, psi));
   RET(copy0(chi0, chi));
@@ -1090,7 +1109,7 @@ static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   Temps T(c);
   mlg_field *rurn = T.get(), *rupn = T.get(), *uzn = T.get(), *rorn = T.get(), *ropn = T.get(), *ozn = T.get();
   mlg_field *bn = T.get(), *w = T.get();
-  if (!T.ok()) return MLG_E_NOMEM;
+  RET(T.ready());
   RET(mlg_chopset(c, 3));
   int rc = mlg_pc2vel(psi, chi, rurn, rupn, uzn);
   if (!rc) rc = mlg_pc2vor(psi, chi, rorn, ropn, ozn);
@@ -1132,7 +1151,7 @@ static int rich_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   s->n += 1;
   Temps T(c);
   mlg_field *psi2 = T.get(), *chi2 = T.get(), *b2 = T.get();
-  if (!T.ok()) return MLG_E_NOMEM;
+  RET(T.ready());
   RET(copy0(psi2, psi));
   RET(copy0(chi2, chi));
   RET(copy0(b2, b));
@@ -1156,7 +1175,7 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   Temps T(c);
   mlg_field *rurn = T.get(), *rupn = T.get(), *uzn = T.get(), *rorn = T.get(), *ropn = T.get(), *ozn = T.get();
   mlg_field *bn = T.get(), *w = T.get(), *psi0 = T.get(), *chi0 = T.get(), *b0 = T.get();
-  if (!T.ok()) return MLG_E_NOMEM;
+  RET(T.ready());
   RET(mlg_chopset(c, 3));
   int rc = mlg_pc2vel(psi, chi, rurn, rupn, uzn);
   if (!rc) rc = mlg_pc2vor(psi, chi, rorn, ropn, ozn);
@@ -1219,6 +1238,68 @@ extern "C" int mlg_time_steps(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg
   return mlg_synchronize(s->ctx);
 }
 
+static void drop_graph(mlg_solver *s) {
+  if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
+  if (s->graph) (void)hipGraphDestroy(s->graph);
+  s->gexec = nullptr;
+  s->graph = nullptr;
+  s->gpsi = s->gchi = s->gb = nullptr;
+}
+
+// One step captured into a hipGraph and replayed.  A step keeps psi, chi, b
+// and the solver's history fields in their buffers (transforms only swap
+// buffers of temporaries), so one captured step is valid for every later
+// step on the same fields; the temporaries it uses are fixed at capture.
+// The eager step that precedes capture fills the temporary pool and the
+// band-factor cache, so the captured step allocates and clears nothing.
+extern "C" int mlg_time_steps_graph(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, int nsteps) {
+  if (!s || !psi || !chi || !b || nsteps < 0) return MLG_E_ARG;
+  if (nsteps == 0) return 0;
+  mlg_ctx *c = s->ctx;
+  int done = 0;
+  if (!s->gexec || s->gpsi != psi || s->gchi != chi || s->gb != b || s->gnrchop != c->g.nrchop ||
+      s->gstream != c->stream) {
+    drop_graph(s);
+    RET(one_step(s, psi, chi, b));
+    done = 1;
+    RET(mlg_synchronize(c));
+    if (done == nsteps) return 0;
+    const bool timer = c->timer.on;
+    c->timer.on = false;
+    const double t0 = s->t;
+    const int n0 = s->n;
+    MLG_CHECK_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+    int rc = one_step(s, psi, chi, b);
+    hipGraph_t gr = nullptr;
+    hipError_t e = hipStreamEndCapture(c->stream, &gr);
+    c->timer.on = timer;
+    s->t = t0;
+    s->n = n0;
+    if (rc || e != hipSuccess) {
+      if (gr) (void)hipGraphDestroy(gr);
+      return rc ? rc : MLG_E_HIP;
+    }
+    hipGraphExec_t ex = nullptr;
+    if (hipGraphInstantiate(&ex, gr, nullptr, nullptr, 0) != hipSuccess) {
+      (void)hipGraphDestroy(gr);
+      return MLG_E_HIP;
+    }
+    s->graph = gr;
+    s->gexec = ex;
+    s->gpsi = psi;
+    s->gchi = chi;
+    s->gb = b;
+    s->gnrchop = c->g.nrchop;
+    s->gstream = c->stream;
+  }
+  for (int i = done; i < nsteps; i++) {
+    MLG_CHECK_HIP(hipGraphLaunch(s->gexec, c->stream));
+    s->t += c->g.p.dt;
+    s->n += 1;
+  }
+  return mlg_synchronize(c);
+}
+
 // ------------------------------------------------------------------ initial condition
 // QVORTEX (src/mod_layout.f90:449-610) with LAY (:64-118) and GAUS (:640-673)
 extern "C" int mlg_qvortex(mlg_ctx *c, double q, mlg_field *psi, mlg_field *chi) {
@@ -1245,7 +1326,7 @@ extern "C" int mlg_qvortex(mlg_ctx *c, double q, mlg_field *psi, mlg_field *chi)
       }
   Temps T(c);
   mlg_field *a = T.get(), *bb = T.get();
-  if (!T.ok()) return MLG_E_NOMEM;
+  RET(T.ready());
   RET(mlg_field_upload(a, (const double *)h.data(), MLG_PPP, 0.0));
   RET(mlg_toff(a));
   RET(launch_negate(c, a->e, c->n_fff));

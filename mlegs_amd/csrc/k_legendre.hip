@@ -37,6 +37,19 @@ __device__ __forceinline__ int d_nrchops(int nrchop, int m) {
 #define BW_TC 64
 #define BW_JC 16
 
+// LDS: two K-chunk stages of the table and data tiles; the epilogue's S/D
+// tiles reuse the same memory after the K loop.
+union RtranBwdLds {
+  struct {
+    double A[2][2][BW_JC][BW_TI];  // [stage][parity][j][i]
+    double B[2][2][BW_JC][BW_TC];  // [stage][parity][j][col]
+  } k;
+  struct {
+    double S[BW_TC][BW_TI + 1];
+    double D[BW_TC][BW_TI + 1];
+  } e;
+};
+
 __global__ void __launch_bounds__(256) k_rtran_bwd(KGrid g, const double *__restrict__ pf, const double2 *__restrict__ in,
                                                    const double *__restrict__ ln_in, const double *__restrict__ lnx,
                                                    double2 *__restrict__ out) {
@@ -49,9 +62,7 @@ __global__ void __launch_bounds__(256) k_rtran_bwd(KGrid g, const double *__rest
   const int i0 = ti * BW_TI, c0 = tc * BW_TC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = wave & 1, wc = wave >> 1;
-
-  __shared__ double sA[2][BW_JC][BW_TI];  // [parity][j][i]
-  __shared__ double sB[2][BW_JC][BW_TC];  // [parity][j][col]
+  __shared__ RtranBwdLds L;
 
   v4d acc[2][2][2];  // [parity][sub i][sub c]
   for (int p = 0; p < 2; p++)
@@ -59,58 +70,79 @@ __global__ void __launch_bounds__(256) k_rtran_bwd(KGrid g, const double *__rest
       for (int b = 0; b < 2; b++) acc[p][a][b] = (v4d){0, 0, 0, 0};
 
   const double *pfm = pf + (size_t)mm * g.npf * g.nrh;
-  const int njp = (nn + 1) / 2;  // number of (even, odd) pairs
-  for (int j0 = 0; j0 < njp; j0 += BW_JC) {
-    // table tile: 32 columns n = 2*j0 .. 2*j0+31, 64 nodes each
+  const int njp = (nn + 1) / 2;  // (even, odd) pairs
+  const int nch = (njp + BW_JC - 1) / BW_JC;
+  // per-thread prefetch registers: 8 table values, 4 complex data values
+  double ra[8];
+  double2 rb[4];
+  auto fetch = [&](int ch) {
+    const int j0 = ch * BW_JC;
+#pragma unroll
     for (int q = 0; q < 8; q++) {
-      int idx = tid + 256 * q;     // 0..2047
-      int i = idx & 63, nl = idx >> 6;  // nl 0..31
-      int n = 2 * j0 + nl;
-      int ig = i0 + i;
-      double v = 0.0;
-      if (n < nn && ig < g.nrh) v = pfm[(size_t)n * g.nrh + ig];
-      sA[nl & 1][nl >> 1][i] = v;
+      int idx = tid + 256 * q;
+      int i = idx & 63, nl = idx >> 6;
+      int n = 2 * j0 + nl, ig = i0 + i;
+      ra[q] = (n < nn && ig < g.nrh) ? pfm[(size_t)n * g.nrh + ig] : 0.0;
     }
-    // data tile: 32 rows n x 32 complex k
+#pragma unroll
     for (int q = 0; q < 4; q++) {
-      int idx = tid + 256 * q;     // 0..1023
+      int idx = tid + 256 * q;
       int nl = idx & 31, kl = idx >> 5;
-      int n = 2 * j0 + nl;
-      int k = (c0 >> 1) + kl;
-      double2 v = make_double2(0.0, 0.0);
-      if (n < nn && k < nk) v = in[(size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k)];
-      sB[nl & 1][nl >> 1][2 * kl] = v.x;
-      sB[nl & 1][nl >> 1][2 * kl + 1] = v.y;
+      int n = 2 * j0 + nl, k = (c0 >> 1) + kl;
+      rb[q] = (n < nn && k < nk) ? in[(size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k)]
+                                 : make_double2(0.0, 0.0);
     }
-    __syncthreads();
+  };
+  auto stash = [&](int st) {
+#pragma unroll
+    for (int q = 0; q < 8; q++) {
+      int idx = tid + 256 * q;
+      int i = idx & 63, nl = idx >> 6;
+      L.k.A[st][nl & 1][nl >> 1][i] = ra[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      int idx = tid + 256 * q;
+      int nl = idx & 31, kl = idx >> 5;
+      L.k.B[st][nl & 1][nl >> 1][2 * kl] = rb[q].x;
+      L.k.B[st][nl & 1][nl >> 1][2 * kl + 1] = rb[q].y;
+    }
+  };
+  if (nch > 0) {
+    fetch(0);
+    stash(0);
+  }
+  __syncthreads();
+  for (int ch = 0; ch < nch; ch++) {
+    const int st = ch & 1;
+    if (ch + 1 < nch) fetch(ch + 1);
 #pragma unroll
     for (int ks = 0; ks < BW_JC; ks += 4) {
       const int kr = ks + (lane >> 4);
 #pragma unroll
       for (int p = 0; p < 2; p++) {
-        double a0 = sA[p][kr][wi * 32 + (lane & 15)];
-        double a1 = sA[p][kr][wi * 32 + 16 + (lane & 15)];
-        double b0 = sB[p][kr][wc * 32 + (lane & 15)];
-        double b1 = sB[p][kr][wc * 32 + 16 + (lane & 15)];
+        double a0 = L.k.A[st][p][kr][wi * 32 + (lane & 15)];
+        double a1 = L.k.A[st][p][kr][wi * 32 + 16 + (lane & 15)];
+        double b0 = L.k.B[st][p][kr][wc * 32 + (lane & 15)];
+        double b1 = L.k.B[st][p][kr][wc * 32 + 16 + (lane & 15)];
         acc[p][0][0] = mfma16x16x4(a0, b0, acc[p][0][0]);
         acc[p][0][1] = mfma16x16x4(a0, b1, acc[p][0][1]);
         acc[p][1][0] = mfma16x16x4(a1, b0, acc[p][1][0]);
         acc[p][1][1] = mfma16x16x4(a1, b1, acc[p][1][1]);
       }
     }
+    if (ch + 1 < nch) stash(st ^ 1);
     __syncthreads();
   }
   // epilogue through LDS: S = BE+BO, D = BE-BO as [col][i]
-  __shared__ double sS[BW_TC][BW_TI + 1];
-  __shared__ double sD[BW_TC][BW_TI + 1];
   for (int a = 0; a < 2; a++)
     for (int b = 0; b < 2; b++)
       for (int v = 0; v < 4; v++) {
         int il = wi * 32 + a * 16 + (lane >> 4) + 4 * v;
         int cl = wc * 32 + b * 16 + (lane & 15);
         double e = acc[0][a][b][v], o = acc[1][a][b][v];
-        sS[cl][il] = e + o;
-        sD[cl][il] = e - o;
+        L.e.S[cl][il] = e + o;
+        L.e.D[cl][il] = e - o;
       }
   __syncthreads();
   const double lnv = (mm == 0) ? ln_in[0] : 0.0;
@@ -120,8 +152,8 @@ __global__ void __launch_bounds__(256) k_rtran_bwd(KGrid g, const double *__rest
     int ig = i0 + il, k = (c0 >> 1) + kl;
     if (ig < g.nrh && k < nk) {
       size_t col = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
-      double2 s = make_double2(sS[2 * kl][il], sS[2 * kl + 1][il]);
-      double2 d = make_double2(sD[2 * kl][il], sD[2 * kl + 1][il]);
+      double2 s = make_double2(L.e.S[2 * kl][il], L.e.S[2 * kl + 1][il]);
+      double2 d = make_double2(L.e.D[2 * kl][il], L.e.D[2 * kl + 1][il]);
       int ir = g.nr - 1 - ig;
       if (mm == 0 && k == 0 && lnv != 0.0) {
         s.x = s.x + lnv * lnx[ig];
@@ -249,14 +281,28 @@ int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out) {
 // psi = -i m W1 - W2 ; del^2 chi = i k W3 + m k W4 - W5.
 // EO: even n uses the even fold (u(x_i)+u(x_{NR-1-i})), odd n the odd fold;
 // OE swaps them.  Folds here carry no weight (it is inside V, D, T).
-// Block tile: 32 rows n (16 even + 16 odd) x 32 real columns (16 complex k);
-// wave w: parity w&1, column half w>>1.  K chunk: 16 nodes.
 // -------------------------------------------------------------------------
-#define PJ_TN 32
+#define PJ_TN 64
 #define PJ_TC 32
 #define PJ_IC 16
 
-__global__ void __launch_bounds__(256) k_project(KGrid g, const double *__restrict__ vdt, const double *__restrict__ ak,
+union ProjectLds {
+  struct {
+    double T[3][2][PJ_IC][PJ_TN / 2];  // [V,D,T][parity][i][n-local within parity]
+    double F[3][2][PJ_IC][PJ_TC];      // [field][fold parity][i][col]
+  } k;
+  struct {
+    double P[PJ_TC / 2][PJ_TN + 1][2];  // psi   [complex k][n][re,im]
+    double C[PJ_TC / 2][PJ_TN + 1][2];  // del^2 chi
+  } e;
+};
+
+// Block tile: 64 rows n (32 even + 32 odd) x 32 real columns (16 complex k);
+// wave w: parity w&1, column half w>>1, two 16-row subtiles.  The K loop
+// (Gauss nodes, chunks of 16) stages one chunk in LDS while the next one is
+// being fetched into registers.  The epilogue forms psi and del^2 chi in
+// registers, exchanging re/im partners with a lane shuffle.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_project(KGrid g, const double *__restrict__ vdt, const double *__restrict__ ak,
                                                  const double2 *__restrict__ rur, const double2 *__restrict__ rup,
                                                  const double2 *__restrict__ uz, double2 *__restrict__ psi,
                                                  double2 *__restrict__ chi) {
@@ -271,74 +317,114 @@ __global__ void __launch_bounds__(256) k_project(KGrid g, const double *__restri
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int par = wave & 1, ch = wave >> 1;
   const size_t tstride = (size_t)g.ntchopdim * g.nrchopdim * g.nrh;
+  __shared__ ProjectLds L;
 
-  __shared__ double sT[3][2][PJ_IC][16];        // [V,D,T][parity][i][n-local]
-  __shared__ double sF[3][2][PJ_IC][PJ_TC];     // [field][fold parity][i][col]
-  v4d acc[5];
-  for (int q = 0; q < 5; q++) acc[q] = (v4d){0, 0, 0, 0};
+  v4d acc[2][5];
+  for (int a = 0; a < 2; a++)
+    for (int q = 0; q < 5; q++) acc[a][q] = (v4d){0, 0, 0, 0};
   const double2 *src[3] = {rur, rup, uz};
-  for (int ib = 0; ib < g.nrh; ib += PJ_IC) {
-    for (int q = 0; q < 6; q++) {
-      int idx = tid + 256 * q;  // 0..1535: [table][n 32][i 16]
-      int il = idx & 15, nl = (idx >> 4) & 31, t = idx >> 9;
+  double rt[12];
+  double2 rp[3], rq[3];
+  auto fetch = [&](int ib) {
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+      int idx = tid + 256 * q;  // [table 3][n 64][i 16]
+      int il = idx & 15, nl = (idx >> 4) & 63, t = idx >> 10;
       int n = n0 + nl, i = ib + il;
-      double v = 0.0;
-      if (n < nn && i < g.nrh) v = vdt[(size_t)t * tstride + ((size_t)mm * g.nrchopdim + n) * g.nrh + i];
-      sT[t][nl & 1][il][nl >> 1] = v;
+      rt[q] = (n < nn && i < g.nrh) ? vdt[(size_t)t * tstride + ((size_t)mm * g.nrchopdim + n) * g.nrh + i] : 0.0;
     }
+#pragma unroll
     for (int q = 0; q < 3; q++) {
-      int idx = tid + 256 * q;  // 0..767: [field][k 16][i 16]
+      int idx = tid + 256 * q;  // [field 3][k 16][i 16]
       int il = idx & 15, kl = (idx >> 4) & 15, f = idx >> 8;
       int i = ib + il, k = (c0 >> 1) + kl;
-      double2 e = make_double2(0.0, 0.0), o = make_double2(0.0, 0.0);
+      rp[q] = rq[q] = make_double2(0.0, 0.0);
       if (i < g.nrh && k < nk) {
         size_t col = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
-        double2 p = src[f][col + i], r = src[f][col + (g.nr - 1 - i)];
-        e = make_double2(p.x + r.x, p.y + r.y);
-        o = make_double2(p.x - r.x, p.y - r.y);
+        rp[q] = src[f][col + i];
+        rq[q] = src[f][col + (g.nr - 1 - i)];
       }
-      sF[f][0][il][2 * kl] = e.x;
-      sF[f][0][il][2 * kl + 1] = e.y;
-      sF[f][1][il][2 * kl] = o.x;
-      sF[f][1][il][2 * kl + 1] = o.y;
     }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+      int idx = tid + 256 * q;
+      int il = idx & 15, nl = (idx >> 4) & 63, t = idx >> 10;
+      L.k.T[t][nl & 1][il][nl >> 1] = rt[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 3; q++) {
+      int idx = tid + 256 * q;
+      int il = idx & 15, kl = (idx >> 4) & 15, f = idx >> 8;
+      double2 p = rp[q], r = rq[q];
+      L.k.F[f][0][il][2 * kl] = p.x + r.x;
+      L.k.F[f][0][il][2 * kl + 1] = p.y + r.y;
+      L.k.F[f][1][il][2 * kl] = p.x - r.x;
+      L.k.F[f][1][il][2 * kl + 1] = p.y - r.y;
+    }
+  };
+  const int eo = par, oe = par ^ 1;
+  fetch(0);
+  for (int ib = 0; ib < g.nrh; ib += PJ_IC) {
     __syncthreads();
-    const int eo = par, oe = par ^ 1;
+    stash();
+    __syncthreads();
+    if (ib + PJ_IC < g.nrh) fetch(ib + PJ_IC);
 #pragma unroll
     for (int ks = 0; ks < PJ_IC; ks += 4) {
       const int kr = ks + (lane >> 4), cl = ch * 16 + (lane & 15);
-      double av = sT[0][par][kr][lane & 15], ad = sT[1][par][kr][lane & 15], at = sT[2][par][kr][lane & 15];
-      acc[0] = mfma16x16x4(av, sF[0][eo][kr][cl], acc[0]);
-      acc[1] = mfma16x16x4(ad, sF[1][oe][kr][cl], acc[1]);
-      acc[2] = mfma16x16x4(ad, sF[0][oe][kr][cl], acc[2]);
-      acc[3] = mfma16x16x4(av, sF[1][eo][kr][cl], acc[3]);
-      acc[4] = mfma16x16x4(at, sF[2][eo][kr][cl], acc[4]);
+      const double f0e = L.k.F[0][eo][kr][cl], f0o = L.k.F[0][oe][kr][cl];
+      const double f1e = L.k.F[1][eo][kr][cl], f1o = L.k.F[1][oe][kr][cl];
+      const double f2e = L.k.F[2][eo][kr][cl];
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        const int rl = 16 * a + (lane & 15);
+        const double av = L.k.T[0][par][kr][rl], ad = L.k.T[1][par][kr][rl], at = L.k.T[2][par][kr][rl];
+        acc[a][0] = mfma16x16x4(av, f0e, acc[a][0]);
+        acc[a][1] = mfma16x16x4(ad, f1o, acc[a][1]);
+        acc[a][2] = mfma16x16x4(ad, f0o, acc[a][2]);
+        acc[a][3] = mfma16x16x4(av, f1e, acc[a][3]);
+        acc[a][4] = mfma16x16x4(at, f2e, acc[a][4]);
+      }
     }
-    __syncthreads();
   }
-  __shared__ double sW[5][PJ_TC][PJ_TN + 1];
-  for (int q = 0; q < 5; q++)
+  __syncthreads();
+  // epilogue: lane (row, col) holds W1..W5 at one real column; its re/im
+  // partner is lane ^ 1.
+  const int cl = ch * 16 + (lane & 15), kl = cl >> 1, isim = cl & 1;
+  const int kg = (c0 >> 1) + kl;
+  const double m = (double)mv;
+  const double kv = (kg < nk) ? ak[mm + (size_t)g.ntchopdim * kg] : 0.0, mk = m * kv;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
     for (int v = 0; v < 4; v++) {
-      int nl = 2 * ((lane >> 4) + 4 * v) + par;
-      int cl = ch * 16 + (lane & 15);
-      sW[q][cl][nl] = acc[q][v];
+      const int rl = (lane >> 4) + 4 * v + 16 * a;
+      const int nl = 2 * rl + par;
+      const double w1 = acc[a][0][v], w2 = acc[a][1][v], w3 = acc[a][2][v], w4 = acc[a][3][v], w5 = acc[a][4][v];
+      const double w1p = __shfl_xor(w1, 1, 64), w3p = __shfl_xor(w3, 1, 64);
+      double ps, cs;
+      if (!isim) {  // real parts
+        ps = m * w1p - w2;
+        cs = (-kv * w3p + mk * w4) - w5;
+      } else {      // imaginary parts
+        ps = -m * w1p - w2;
+        cs = (kv * w3p + mk * w4) - w5;
+      }
+      L.e.P[kl][nl][isim] = ps;
+      L.e.C[kl][nl][isim] = cs;
     }
   __syncthreads();
-  for (int q = 0; q < 2; q++) {
-    int idx = tid + 256 * q;  // 0..511: n fastest, 16 complex k
-    int nl = idx & 31, kl = idx >> 5;
-    int n = n0 + nl, k = (c0 >> 1) + kl;
+  for (int q = 0; q < 4; q++) {
+    int idx = tid + 256 * q;  // 0..1023: n fastest, 16 complex k
+    int nl = idx & 63, kq = idx >> 6;
+    int n = n0 + nl, k = (c0 >> 1) + kq;
     if (n < g.nrchopdim && k < nk) {
       size_t o = (size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
       if (n < nn) {
-        double w1r = sW[0][2 * kl][nl], w1i = sW[0][2 * kl + 1][nl];
-        double w2r = sW[1][2 * kl][nl], w2i = sW[1][2 * kl + 1][nl];
-        double w3r = sW[2][2 * kl][nl], w3i = sW[2][2 * kl + 1][nl];
-        double w4r = sW[3][2 * kl][nl], w4i = sW[3][2 * kl + 1][nl];
-        double w5r = sW[4][2 * kl][nl], w5i = sW[4][2 * kl + 1][nl];
-        double m = (double)mv, kv = ak[mm + (size_t)g.ntchopdim * k], mk = m * kv;
-        psi[o] = make_double2(m * w1i - w2r, -m * w1r - w2i);
-        chi[o] = make_double2((-kv * w3i + mk * w4r) - w5r, (kv * w3r + mk * w4i) - w5i);
+        psi[o] = make_double2(L.e.P[kq][nl][0], L.e.P[kq][nl][1]);
+        chi[o] = make_double2(L.e.C[kq][nl][0], L.e.C[kq][nl][1]);
       } else {
         psi[o] = make_double2(0.0, 0.0);
         chi[o] = make_double2(0.0, 0.0);

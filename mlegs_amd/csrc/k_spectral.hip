@@ -103,6 +103,24 @@ int launch_scale(mlg_ctx *c, double2 *a, double s, long n) {
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
+struct LnList {
+  double *p[32];
+  int n;
+};
+__global__ void k_zero_lns(LnList l) {
+  if ((int)threadIdx.x < l.n) l.p[threadIdx.x][0] = 0.0;
+}
+int launch_zero_lns(mlg_ctx *c, double *const *ptrs, int n) {
+  for (int i0 = 0; i0 < n; i0 += 32) {
+    LnList l;
+    l.n = n - i0 < 32 ? n - i0 : 32;
+    for (int i = 0; i < l.n; i++) l.p[i] = ptrs[i0 + i];
+    hipLaunchKernelGGL(k_zero_lns, dim3(1), dim3(32), 0, c->stream, l);
+    MLG_CHECK_HIP(hipGetLastError());
+  }
+  return 0;
+}
+
 int launch_set_ln(mlg_ctx *c, double *dst, const double *src, double scale, double add) {
   hipLaunchKernelGGL(k_set_ln, dim3(1), dim3(64), 0, c->stream, dst, src, scale, add);
   MLG_CHECK_HIP(hipGetLastError());

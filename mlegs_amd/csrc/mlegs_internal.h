@@ -104,6 +104,12 @@ struct mlg_solver {
   mlg_field *r_n_r = nullptr, *r_n_p = nullptr, *n_z = nullptr;
   mlg_field *r_pi_r = nullptr, *r_pi_p = nullptr, *pi_z = nullptr;
   double etd_e[16], etd_nl[16];
+  // graph of one captured step (mlg_time_steps_graph)
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t gexec = nullptr;
+  mlg_field *gpsi = nullptr, *gchi = nullptr, *gb = nullptr;
+  int gnrchop = -1;
+  hipStream_t gstream = nullptr;
 };
 
 // Kernel-side view of the current grid (passed by value to kernels).
@@ -169,6 +175,7 @@ int launch_chopdo(mlg_ctx *c, double2 *a, int space);
 int launch_zero(mlg_ctx *c, double2 *a, long n);
 int launch_copy(mlg_ctx *c, double2 *dst, const double2 *src, long n);
 int launch_set_ln(mlg_ctx *c, double *dst, const double *src, double scale, double add);
+int launch_zero_lns(mlg_ctx *c, double *const *ptrs, int n);  // LN = 0 for n fields, one launch per 32
 int launch_delsqh(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, double *ln_b);
 int launch_idelsqh(mlg_ctx *c, const double2 *b, double2 *a, double *ln_a);
 int launch_band3(mlg_ctx *c, int which, const double2 *a, const double *ln_a, double2 *b, double *ln_b);

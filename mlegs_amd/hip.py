@@ -119,6 +119,7 @@ SIGNATURES = {
     "mlg_initialize": ([_V, _V, _V, _V], _I),
     "mlg_time_step": ([_V, _V, _V, _V], _I),
     "mlg_time_steps": ([_V, _V, _V, _V, _I], _I),
+    "mlg_time_steps_graph": ([_V, _V, _V, _V, _I], _I),
     "mlg_solver_time": ([_V], _D),
     "mlg_solver_nstep": ([_V], _I),
     "mlg_qvortex": ([_V, _D, _V, _V], _I),
@@ -467,9 +468,10 @@ class Solver:
     def time_step(self, psi, chi, b):
         _call("mlg_time_step", self.h, psi.h, chi.h, b.h)
 
-    def time_stepping(self, psi, chi, b, nsteps):
-        """NSTEPS steps, one synchronisation at the end (device checks are sticky)."""
-        _call("mlg_time_steps", self.h, psi.h, chi.h, b.h, int(nsteps))
+    def time_stepping(self, psi, chi, b, nsteps, graph=False):
+        """NSTEPS steps, one synchronisation at the end (device checks are sticky).
+        graph=True replays a captured step (same results, fewer launches)."""
+        _call("mlg_time_steps_graph" if graph else "mlg_time_steps", self.h, psi.h, chi.h, b.h, int(nsteps))
 
     @property
     def t(self):

@@ -474,3 +474,37 @@ def test_cfg1_energy_100_steps():
         assert gs.nstep == 100
     finally:
         p.close()
+
+
+@pytest.mark.parametrize("cfg", [SMALL, SMALL_ETD], ids=["ab2", "etd2"])
+def test_graph_replays_equal_eager_steps(cfg):
+    """mlg_time_steps_graph: the captured step, replayed again and again with
+    eager work and synchronisations between replays, gives the eager steps'
+    results bit for bit every time."""
+    p = Pair(cfg)
+    try:
+        st = state(p)
+        a = [p.g.scalar(O.FFF, x, ln) for x, ln in st]
+        b = [p.g.scalar(O.FFF, x, ln) for x, ln in st]
+        s1, s2 = H.Solver(p.g), H.Solver(p.g)
+        s1.initialize(*a)
+        s2.initialize(*b)
+
+        def same():
+            for x, y in zip(a, b):
+                xa, _, xl = x.download()
+                ya, _, yl = y.download()
+                assert np.array_equal(xa, ya) and xl == yl
+
+        for _ in range(3):
+            s1.time_step(*a)
+        s2.time_stepping(*b, 3, graph=True)  # eager step, capture, replay
+        same()
+        for _ in range(4):
+            H.calc_boussi_energy(*b)  # eager work on the same pool between replays
+            s1.time_step(*a)
+            s2.time_stepping(*b, 1, graph=True)
+            same()
+        assert s1.nstep == s2.nstep and abs(s1.t - s2.t) < 1e-15
+    finally:
+        p.close()
