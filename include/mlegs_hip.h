@@ -70,6 +70,12 @@ enum {
  * Gauss-Legendre grid, normalisations and the Pbar_n^m tables on the host
  * (extended precision) and uploads them.  Uses the current HIP device. */
 int mlg_init(const mlg_params *p, mlg_ctx **ctx);
+/* Parameters from a read.input file: the DIMENSION, CHOPS, ELL, ZLEN,
+ * MINC, MKLINK, DT, VISCOSITY and BOUSSINESQ blocks of READIN0
+ * (src/mod_init.f90:401-710, line syntax of READCOM1 :165-372), with its
+ * conversions (ZLEN <= 1 is a wavenumber, NU/NUP >= 1 are Reynolds
+ * numbers).  *t_limit (may be NULL) receives DT's end time.  Host only. */
+int mlg_read_input(const char *path, mlg_params *p, double *t_limit);
 /* Frees every device resource of the context (FINALIZE, mod_march.f90:310-353). */
 int mlg_finalize(mlg_ctx *ctx);
 /* out[12] = {NDIMR, NDIMTH, NDIMX, NRCHOPDIM, NTCHOPDIM, NXCHOPDIM, NRH,

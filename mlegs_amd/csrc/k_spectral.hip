@@ -542,11 +542,12 @@ __device__ void gen_row(const SolveCtx &s, int q, double *out, int NE) {
 }
 
 // Factor storage: row nn of class cls (= m index + NTCHOPDIM*|k| index)
-// keeps FSTRIDE doubles at F[(nn*FSTRIDE + e)*ncls + cls]:
+// keeps FSTRIDE doubles at F[(nn*ncls + cls)*FSTRIDE + e]:
 //   e < C0          multiplier of row nn against row nn-1-e (scaled by its pivot)
 //   C0 <= e < C0+KU normalised upper entry (nn, nn+1+e-C0)
 //   e = NE-1        1/pivot
-// so a wave of consecutive m reads consecutive addresses.
+// so a thread reads one contiguous 56-byte record per row and a wave of
+// consecutive m reads one contiguous run.
 #define FSTRIDE 7
 
 template <int NE, int C0>
@@ -569,10 +570,10 @@ __device__ void band_factor(const SolveCtx &s, int nd, double *F, long ncls, lon
       }
     for (int j = 1; j <= C0; j++)
       if (nn + j < nd) W[j][C0 - j] /= piv;
-    double *fr = F + (size_t)nn * FSTRIDE * ncls + cls;
-    for (int j = 1; j <= C0; j++) fr[(size_t)(j - 1) * ncls] = W[0][C0 - j];
-    for (int u = 0; u < KU; u++) fr[(size_t)(C0 + u) * ncls] = W[0][C0 + 1 + u];
-    fr[(size_t)(NE - 1) * ncls] = 1.0 / piv;
+    double *fr = F + ((size_t)nn * ncls + cls) * FSTRIDE;
+    for (int j = 1; j <= C0; j++) fr[j - 1] = W[0][C0 - j];
+    for (int u = 0; u < KU; u++) fr[C0 + u] = W[0][C0 + 1 + u];
+    fr[NE - 1] = 1.0 / piv;
     for (int q = 0; q < C0; q++)
       for (int b = 0; b < NE; b++) W[q][b] = W[q + 1][b];
     if (nn + C0 + 1 < nd) gen_row(s, nn + C0 + 1, W[C0], NE);
@@ -583,32 +584,131 @@ __device__ void band_factor(const SolveCtx &s, int nd, double *F, long ncls, lon
 
 // Forward substitution (multipliers applied to the unscaled history, then
 // scaled by 1/pivot) and back substitution with the normalised U rows:
-// the update order of LUBKSBB.
-template <int NE, int C0>
+// the update order of LUBKSBB.  Rows go in register blocks of SB: all
+// loads of a block are issued before its serial arithmetic, so one memory
+// round trip serves SB rows of the dependency chain.
+template <int NE, int C0, int SB>
 __device__ void band_apply(int nd, double2 *x, const double *__restrict__ F, long ncls, long cls) {
   constexpr int KU = NE - 1 - C0;
   double2 xh[C0];
   for (int q = 0; q < C0; q++) xh[q] = zero2();
-  for (int nn = 0; nn < nd; nn++) {
-    const double *fr = F + (size_t)nn * FSTRIDE * ncls + cls;
-    double2 v = x[nn];
-    int nne = nn < C0 ? nn : C0;
-    for (int j = 1; j <= nne; j++) v = sub2(v, axc(fr[(size_t)(j - 1) * ncls], xh[j - 1]));
-    for (int q = C0 - 1; q >= 1; q--) xh[q] = xh[q - 1];
-    xh[0] = v;
-    x[nn] = axc(fr[(size_t)(NE - 1) * ncls], v);
+  for (int nb = 0; nb < nd; nb += SB) {
+    double2 rhs[SB];
+    double lm[SB][C0], rp[SB];
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int nn = nb + q;
+      if (nn < nd) {
+        const double *fr = F + ((size_t)nn * ncls + cls) * FSTRIDE;
+        rhs[q] = x[nn];
+#pragma unroll
+        for (int j = 0; j < C0; j++) lm[q][j] = fr[j];
+        rp[q] = fr[NE - 1];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int nn = nb + q;
+      if (nn < nd) {
+        double2 v = rhs[q];
+        const int nne = nn < C0 ? nn : C0;
+#pragma unroll
+        for (int j = 1; j <= C0; j++)
+          if (j <= nne) v = sub2(v, axc(lm[q][j - 1], xh[j - 1]));
+#pragma unroll
+        for (int t = C0 - 1; t >= 1; t--) xh[t] = xh[t - 1];
+        xh[0] = v;
+        rhs[q] = axc(rp[q], v);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+      if (nb + q < nd) x[nb + q] = rhs[q];
   }
   double2 yh[KU];
   for (int u = 0; u < KU; u++) yh[u] = zero2();
-  for (int nn = nd - 1; nn >= 0; nn--) {
-    const double *fr = F + (size_t)nn * FSTRIDE * ncls + cls;
-    double2 v = x[nn];
-    if (nn <= nd - 2)
-      for (int i = 1; i <= KU; i++)
-        if (nn + i < nd) v = sub2(v, axc(fr[(size_t)(C0 + i - 1) * ncls], yh[i - 1]));
-    x[nn] = v;
-    for (int u = KU - 1; u >= 1; u--) yh[u] = yh[u - 1];
-    yh[0] = v;
+  for (int nt = nd - 1; nt >= 0; nt -= SB) {
+    double2 v0[SB];
+    double um[SB][KU];
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int nn = nt - q;
+      if (nn >= 0) {
+        const double *fr = F + ((size_t)nn * ncls + cls) * FSTRIDE;
+        v0[q] = x[nn];
+#pragma unroll
+        for (int u = 0; u < KU; u++) um[q][u] = fr[C0 + u];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int nn = nt - q;
+      if (nn >= 0) {
+        double2 v = v0[q];
+        if (nn <= nd - 2) {
+#pragma unroll
+          for (int i = 1; i <= KU; i++)
+            if (nn + i < nd) v = sub2(v, axc(um[q][i - 1], yh[i - 1]));
+        }
+        v0[q] = v;
+#pragma unroll
+        for (int u = KU - 1; u >= 1; u--) yh[u] = yh[u - 1];
+        yh[0] = v;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+      if (nt - q >= 0) x[nt - q] = v0[q];
+  }
+}
+
+// x[i] = x[i-1] for i = n-1 .. 1 (top-down blocks: every block reads its
+// inputs before the block below is written)
+template <int SB>
+__device__ void shift_down_one(double2 *x, int n) {
+  for (int top = n - 1; top >= 1; top -= SB) {
+    double2 v[SB];
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+      if (top - q >= 1) v[q] = x[top - q - 1];
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+      if (top - q >= 1) x[top - q] = v[q];
+  }
+}
+
+// x <- XM x on nn rows (tri-band, ascending, from the original values)
+template <int SB>
+__device__ void mulxm_column(double2 *x, const double *__restrict__ xt, int nn) {
+  double2 prev = zero2();
+  for (int cb = 0; cb < nn; cb += SB) {
+    double2 cur[SB + 1];
+    double r[SB][3];
+#pragma unroll
+    for (int q = 0; q <= SB; q++)
+      if (cb + q < nn) cur[q] = x[cb + q];
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+      if (cb + q < nn) {
+        r[q][0] = xt[(size_t)(cb + q) * 3];
+        r[q][1] = xt[(size_t)(cb + q) * 3 + 1];
+        r[q][2] = xt[(size_t)(cb + q) * 3 + 2];
+      }
+    double2 o[SB];
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int c = cb + q;
+      if (c < nn) {
+        double2 t = axc(r[q][1], cur[q]);
+        if (c >= 1) t = add2(t, axc(r[q][0], prev));
+        if (c + 1 < nn) t = add2(t, axc(r[q][2], cur[q + 1]));
+        prev = cur[q];
+        o[q] = t;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+      if (cb + q < nn) x[cb + q] = o[q];
   }
 }
 
@@ -649,32 +749,61 @@ __global__ void __launch_bounds__(64) k_band_factor(KGrid g, const double *bd, c
   else band_factor<7, 3>(s, nn - 1, F, ncls, cls);
 }
 
-// One thread per (m, k): right-hand-side set-up, substitution, result
-// post-processing of IDEL2 / IDEL2LN / IHELM (src/mod_legops.f90:823-1354).
-__global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, int op, double alp, int has_ln, double lnval,
-                                                   double2 *a, double *ln_a, const double *ln_b, const double *__restrict__ F) {
+// IDEL2 / IDEL2LN / IHELM (src/mod_legops.f90:823-1354) on the cached
+// factors.  k_band_apply_penta: every (m, k) system solved with the
+// penta-diagonal factors (m >= 1, and m = 0 where the operator keeps the
+// penta-diagonal form), one thread each.  k_band_apply_m0: the m = 0
+// systems with their own forms (IDEL2's Dirichlet log row at k = 0, IHELM's
+// (1-x)-factorised 7-band system at every k), with the right-hand-side and
+// result fix-ups, one thread per k.  LN outputs of m = 0, k = 0 are set by
+// the thread that owns that system.
+__device__ __forceinline__ int solve_kind(int op, int mm, int k) {
+  if (mm != 0) return 0;
+  if (op == 0 && k == 0) return 1;
+  if (op == 2) return 2;
+  return 0;
+}
+
+__global__ void __launch_bounds__(64) k_band_apply_penta(KGrid g, int op, int has_ln, double2 *a, double *ln_a,
+                                                         const double *__restrict__ F) {
   const long nsys = (long)g.ntchopdim * g.nxchopdim;
   const long ncls = (long)g.ntchopdim * g.nxchop;
   const long sys = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (sys >= nsys) return;
   const int mm = (int)(sys % g.ntchopdim), k = (int)(sys / g.ntchopdim);
-  const int mv = mm * g.minc, nn = s_nrchops(g.nrchop, mv);
-  if (nn < 1 || mm >= g.ntchop) return;
+  const int nn = s_nrchops(g.nrchop, mm * g.minc);
+  if (nn < 1 || mm >= g.ntchop || solve_kind(op, mm, k) != 0) return;
   const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
-  const long cls = mm + (long)g.ntchopdim * kabs;
   double2 *x = a + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
-  int kind = 0;
-  if (mm == 0) {
-    if (op == 0 && k == 0) kind = 1;
-    if (op == 2) kind = 2;
+  band_apply<5, 2, 16>(nn, x, F, ncls, mm + (long)g.ntchopdim * kabs);
+  if (mm == 0 && k == 0) {
+    if (op == 0) ln_a[0] = has_ln ? x[0].x * g.norm0 : 0.0;
+    if (op == 1) {
+      ln_a[0] = x[0].x * g.norm0;
+      x[0] = zero2();
+    }
   }
-  if (kind == 0) {
-    band_apply<5, 2>(nn, x, F, ncls, cls);
-  } else if (kind == 1) {
+}
+
+#define M0_SB 4
+__global__ void __launch_bounds__(64) k_band_apply_m0(KGrid g, const double *bxm, int op, double alp, int has_ln,
+                                                      double lnval, double2 *a, double *ln_a, const double *ln_b,
+                                                      const double *__restrict__ F) {
+  const long ncls = (long)g.ntchopdim * g.nxchop;
+  const int k = blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= g.nxchopdim) return;
+  const int nn = s_nrchops(g.nrchop, 0);
+  const int kind = solve_kind(op, 0, k);
+  if (nn < 1 || g.ntchop < 1 || kind == 0) return;
+  const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
+  const long cls = (long)g.ntchopdim * kabs;
+  double2 *x = a + (size_t)g.nrchopdim * ((size_t)g.ntchopdim * k);
+  if (kind == 1) {
     // IDEL2 m0k0: first row is the log-term (Dirichlet) row
-    for (int i = nn - 1; i >= 1; i--) x[i] = x[i - 1];
+    shift_down_one<M0_SB>(x, nn);
     x[0] = make_double2(has_ln ? lnval / g.norm0 : 0.0, 0.0);
-    band_apply<7, 3>(nn, x, F, ncls, cls);
+    band_apply<7, 3, M0_SB>(nn, x, F, ncls, cls);
+    ln_a[0] = has_ln ? x[0].x * g.norm0 : 0.0;
   } else {
     // IHELM m=0: solve (del^2 - alp - k^2)(1-x) y = rhs on NN-1 rows, A = (1-x) y
     if (k == 0) {
@@ -684,25 +813,9 @@ __global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, i
       x[1].x = x[1].x - lnA * (-2 / g.norm1 / g.ell2);
       x[2].x = x[2].x - lnA * (+2.0 / 3 / g.norm2 / g.ell2);
     }
-    band_apply<7, 3>(nn - 1, x, F, ncls, cls);
+    band_apply<7, 3, M0_SB>(nn - 1, x, F, ncls, cls);
     x[nn - 1] = zero2();
-    double2 prev = zero2();
-    for (int c = 0; c < nn; c++) {
-      const double *r = bxm + ((size_t)mm * g.nband + c) * 3;
-      double2 cur = x[c];
-      double2 o = axc(r[1], cur);
-      if (c >= 1) o = add2(o, axc(r[0], prev));
-      if (c + 1 < nn) o = add2(o, axc(r[2], x[c + 1]));
-      prev = cur;
-      x[c] = o;
-    }
-  }
-  if (mm == 0 && k == 0) {
-    if (op == 0) ln_a[0] = has_ln ? x[0].x * g.norm0 : 0.0;
-    if (op == 1) {
-      ln_a[0] = x[0].x * g.norm0;
-      x[0] = zero2();
-    }
+    mulxm_column<M0_SB>(x, bxm, nn);
   }
 }
 
@@ -744,9 +857,13 @@ int launch_band_solve(mlg_ctx *c, int op, double2 *a, double *ln_a, const double
   const double *F = factor_set(c, op, alp, &rc);
   if (!F) return rc;
   KScope ks(c, K_BAND_SOLVE, 2.0 * 16.0 * fff_count(g));  // the coefficients read and written once
-  hipLaunchKernelGGL(k_band_apply, dim3((nsys + 63) / 64), dim3(64), 0, c->stream, g, c->d.bxm, op, alp, has_ln, ln, a, ln_a,
-                     ln_b, F);
+  hipLaunchKernelGGL(k_band_apply_penta, dim3((nsys + 63) / 64), dim3(64), 0, c->stream, g, op, has_ln, a, ln_a, F);
   MLG_CHECK_HIP(hipGetLastError());
+  if (op != 1) {
+    hipLaunchKernelGGL(k_band_apply_m0, dim3((g.nxchopdim + 63) / 64), dim3(64), 0, c->stream, g, c->d.bxm, op, alp, has_ln,
+                       ln, a, ln_a, ln_b, F);
+    MLG_CHECK_HIP(hipGetLastError());
+  }
   return 0;
 }
 

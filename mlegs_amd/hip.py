@@ -123,6 +123,7 @@ SIGNATURES = {
     "mlg_solver_time": ([_V], _D),
     "mlg_solver_nstep": ([_V], _I),
     "mlg_qvortex": ([_V, _D, _V, _V], _I),
+    "mlg_read_input": ([C.c_char_p, C.POINTER(Params), _PD], _I),
     "mlg_timer_enable": ([_V, _I], _I),
     "mlg_timer_read": ([_V, C.c_char_p, _PD, C.POINTER(C.c_long), _PD, C.POINTER(_I)], _I),
 }
@@ -148,6 +149,14 @@ def _lib():
 
 def library_path():
     return LIB_PATH
+
+
+def read_input(path):
+    """read.input -> (parameter dict for Grid, end time TIM%LIMIT) via the library's parser."""
+    p = Params()
+    lim = C.c_double(0.0)
+    _call("mlg_read_input", os.fsencode(path), C.byref(p), C.byref(lim))
+    return {k: getattr(p, k) for k, _ in Params._fields_}, lim.value
 
 
 def _check(rc, where):
