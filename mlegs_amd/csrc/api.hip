@@ -1025,16 +1025,6 @@ static int hyperv_state(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b
   RET(launch_hyp3(c, chi->e, 0, tf));
   return launch_hyp3(c, b->e, 1, tf);
 }
-// CALC_BOUSSI_ETD_HYPERV (mod_march.f90:1352-1416)
-static int etd_hyperv(mlg_ctx *c, mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *b, double dt) {
-  if (!filter_active(c)) return 0;
-  double tf = dt / c->g.p.dt;
-  RET(launch_hyp3(c, rur->e, 0, tf));
-  RET(launch_hyp3(c, rup->e, 0, tf));
-  RET(launch_hyp3(c, uz->e, 0, tf));
-  return launch_hyp3(c, b->e, 1, tf);
-}
-
 // RICH_BOUSSI_FE_BE (mod_march.f90:535-606)
 static int rich_fe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
   mlg_ctx *c = s->ctx;
@@ -1096,12 +1086,6 @@ static int pc2vel3(mlg_ctx *c, const mlg_field *psi, const mlg_field *chi, mlg_f
   return rc;
 }
 
-static int copy_e(mlg_field *d, const mlg_field *s) {
-  RET(launch_copy(d->ctx, d->e, s->e, d->ctx->n_fff));
-  d->space = MLG_FFF;
-  return 0;
-}
-
 // STEP_BOUSSI_ETDFE_BE (mod_march.f90:664-753)
 static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, const double *e16, const double *nl16,
                          double dt, bool save) {
@@ -1116,20 +1100,11 @@ static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   mlg_chopset(c, -3);
   RET(rc);
   RET(mlg_nonlin(b, rurn, rupn, uzn, bn, rorn, ropn, ozn));
-  RET(etd_hyperv(c, rorn, ropn, ozn, bn, dt));
   RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
-  RET(launch_etd_sub(c, e16, nl16, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, nullptr, nullptr, nullptr,
-                     nullptr, nullptr, nullptr, nullptr, 0));
-  RET(etd_hyperv(c, rurn, rupn, uzn, b, dt));
-  if (save) {
-    RET(copy_e(s->n_b, bn));
-    RET(copy_e(s->r_n_r, rorn));
-    RET(copy_e(s->r_n_p, ropn));
-    RET(copy_e(s->n_z, ozn));
-    RET(copy_e(s->r_pi_r, rurn));
-    RET(copy_e(s->r_pi_p, rupn));
-    RET(copy_e(s->pi_z, uzn));
-  }
+  // ETD_HYPERV(N) ; ETDFE_SUB ; ETD_HYPERV(u, b) ; history stores when saving
+  RET(launch_etd_sub(c, e16, nl16, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
+                     s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 0, save ? 1 : 0, nullptr,
+                     filter_active(c), dt / c->g.p.dt));
   RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0));  // PROJECT(..., PSI, W, PSI%LN)
   RET(mlg_idel2ln(w, chi));
   if (save) {
@@ -1182,19 +1157,13 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   mlg_chopset(c, -3);
   RET(rc);
   RET(mlg_nonlin(b, rurn, rupn, uzn, bn, rorn, ropn, ozn));
-  RET(etd_hyperv(c, rorn, ropn, ozn, bn, dt));
-  RET(copy0(b0, b));
   RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
+  // ETD_HYPERV(N) ; B0 = B ; ETDAB_SUB ; N history ; ETD_HYPERV(u, b) ; pi history
+  b0->space = MLG_FFF;
+  RET(launch_set_ln(c, b0->ln, b->ln, 1.0, 0.0));
   RET(launch_etd_sub(c, s->etd_e, s->etd_nl, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
-                     s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 1));
-  RET(copy_e(s->n_b, bn));
-  RET(copy_e(s->r_n_r, rorn));
-  RET(copy_e(s->r_n_p, ropn));
-  RET(copy_e(s->n_z, ozn));
-  RET(etd_hyperv(c, rurn, rupn, uzn, b, dt));
-  RET(copy_e(s->r_pi_r, rurn));
-  RET(copy_e(s->r_pi_p, rupn));
-  RET(copy_e(s->pi_z, uzn));
+                     s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 1, 1, b0->e,
+                     filter_active(c), dt / c->g.p.dt));
   RET(copy0(psi0, psi));
   RET(copy0(chi0, chi));
   RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0));  // PROJECT(..., PSI, W, PSI%LN)

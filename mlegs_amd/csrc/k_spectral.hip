@@ -764,11 +764,10 @@ __device__ __forceinline__ int solve_kind(int op, int mm, int k) {
   return 0;
 }
 
-__global__ void __launch_bounds__(64) k_band_apply_penta(KGrid g, int op, int has_ln, double2 *a, double *ln_a,
-                                                         const double *__restrict__ F) {
+__device__ void apply_penta(const KGrid &g, int op, int has_ln, double2 *a, double *ln_a, const double *__restrict__ F,
+                            long sys) {
   const long nsys = (long)g.ntchopdim * g.nxchopdim;
   const long ncls = (long)g.ntchopdim * g.nxchop;
-  const long sys = blockIdx.x * (long)blockDim.x + threadIdx.x;
   if (sys >= nsys) return;
   const int mm = (int)(sys % g.ntchopdim), k = (int)(sys / g.ntchopdim);
   const int nn = s_nrchops(g.nrchop, mm * g.minc);
@@ -786,11 +785,9 @@ __global__ void __launch_bounds__(64) k_band_apply_penta(KGrid g, int op, int ha
 }
 
 #define M0_SB 4
-__global__ void __launch_bounds__(64) k_band_apply_m0(KGrid g, const double *bxm, int op, double alp, int has_ln,
-                                                      double lnval, double2 *a, double *ln_a, const double *ln_b,
-                                                      const double *__restrict__ F) {
+__device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, int has_ln, double lnval, double2 *a,
+                         double *ln_a, const double *ln_b, const double *__restrict__ F, int k) {
   const long ncls = (long)g.ntchopdim * g.nxchop;
-  const int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= g.nxchopdim) return;
   const int nn = s_nrchops(g.nrchop, 0);
   const int kind = solve_kind(op, 0, k);
@@ -816,6 +813,18 @@ __global__ void __launch_bounds__(64) k_band_apply_m0(KGrid g, const double *bxm
     band_apply<7, 3, M0_SB>(nn - 1, x, F, ncls, cls);
     x[nn - 1] = zero2();
     mulxm_column<M0_SB>(x, bxm, nn);
+  }
+}
+
+// One launch: blocks [0, npenta) take the penta-diagonal systems, the
+// remaining blocks the m = 0 systems, so both sets run side by side.
+__global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, int op, double alp, int has_ln, double lnval,
+                                                   double2 *a, double *ln_a, const double *ln_b,
+                                                   const double *__restrict__ F, int npenta) {
+  if ((int)blockIdx.x < npenta) {
+    apply_penta(g, op, has_ln, a, ln_a, F, blockIdx.x * (long)blockDim.x + threadIdx.x);
+  } else {
+    apply_m0(g, bxm, op, alp, has_ln, lnval, a, ln_a, ln_b, F, (blockIdx.x - npenta) * blockDim.x + threadIdx.x);
   }
 }
 
@@ -857,13 +866,10 @@ int launch_band_solve(mlg_ctx *c, int op, double2 *a, double *ln_a, const double
   const double *F = factor_set(c, op, alp, &rc);
   if (!F) return rc;
   KScope ks(c, K_BAND_SOLVE, 2.0 * 16.0 * fff_count(g));  // the coefficients read and written once
-  hipLaunchKernelGGL(k_band_apply_penta, dim3((nsys + 63) / 64), dim3(64), 0, c->stream, g, op, has_ln, a, ln_a, F);
+  const int npenta = (int)((nsys + 63) / 64), nm0 = (op != 1) ? (g.nxchopdim + 63) / 64 : 0;
+  hipLaunchKernelGGL(k_band_apply, dim3(npenta + nm0), dim3(64), 0, c->stream, g, c->d.bxm, op, alp, has_ln, ln, a, ln_a,
+                     ln_b, F, npenta);
   MLG_CHECK_HIP(hipGetLastError());
-  if (op != 1) {
-    hipLaunchKernelGGL(k_band_apply_m0, dim3((g.nxchopdim + 63) / 64), dim3(64), 0, c->stream, g, c->d.bxm, op, alp, has_ln,
-                       ln, a, ln_a, ln_b, F);
-    MLG_CHECK_HIP(hipGetLastError());
-  }
   return 0;
 }
 
@@ -909,16 +915,42 @@ int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1,
 }
 
 // ETD update (CALC_BOUSSI_ETDFE_SUB / _ETDAB_SUB, src/mod_march.f90:948-1139)
+// fused with what surrounds it in STEP_BOUSSI_ETDFE_BE / _ETDAB_CN
+// (:664-901): the HYPERV3 filter of the nonlinear terms before the update
+// and of the updated fields after it (CALC_BOUSSI_ETD_HYPERV :1352-1416),
+// the copy of b before the update, and the stores of the filtered
+// nonlinear terms and updated velocities into the solver's history.  Every
+// value goes through the same operations as the separate kernels.
 struct Mat4x2 {
   double e[16], nl[16];
 };
-__global__ void k_etd_sub(KGrid g, Mat4x2 M, double dt, double2 *rur, double2 *rup, double2 *uz, double2 *b, const double2 *nr_,
-                          const double2 *np_, const double2 *nz_, const double2 *nb_, const double2 *hpr, const double2 *hpp,
-                          const double2 *hpz, const double2 *hnr, const double2 *hnp, const double2 *hnz, const double2 *hnb,
-                          int ab) {
+struct EtdArgs {
+  double2 *rur, *rup, *uz, *b;       // state: in, updated out
+  const double2 *nr, *np, *nz, *nb;  // nonlinear terms (before filtering)
+  double2 *hpr, *hpp, *hpz;          // pressure-gradient history: read (ab), written (save)
+  double2 *hnr, *hnp, *hnz, *hnb;    // nonlinear-term history: read (ab), written (save)
+  double2 *b0;                       // receives b before the update (or null)
+  const double *hv, *hd;             // filter tables [hr | ht | hx] (or null: no filter)
+  int nrc;
+  double tf;
+};
+
+__device__ __forceinline__ double2 filt(const KGrid &g, const FIdx &f, int nn, const double *h, int nrc, double tf,
+                                        double2 v) {
+  if (f.n < nn && f.mm < g.ntchop) {
+    const double *hr = h, *ht = h + (size_t)nrc * g.ntchopdim, *hx = ht + g.ntchopdim;
+    double hh = hr[f.n + (size_t)nrc * f.mm] * ht[f.mm] * hx[f.k];
+    return axc(pow(hh, tf), v);
+  }
+  return zero2();
+}
+
+__global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, EtdArgs A, int ab, int save) {
   GRID_STRIDE(fff_count(g)) {
     FIdx f = fdecode(g, idx);
+    const int nn = s_nrchops(g.nrchop, f.mm * g.minc);
     double er[16], ar[16];
+#pragma unroll
     for (int q = 0; q < 16; q++) {
       er[q] = M.e[q];
       ar[q] = M.nl[q];
@@ -926,6 +958,7 @@ __global__ void k_etd_sub(KGrid g, Mat4x2 M, double dt, double2 *rur, double2 *r
     double notk0 = 1.0, notm0 = (f.mm == 0) ? 0.0 : 1.0;
     if (f.k == 0) {
       notk0 = 0.0;
+#pragma unroll
       for (int j = 0; j < 4; j++) {
         er[0 + 4 * j] = 0.0;
         er[1 + 4 * j] = 0.0;
@@ -937,54 +970,87 @@ __global__ void k_etd_sub(KGrid g, Mat4x2 M, double dt, double2 *rur, double2 *r
       ar[0] = dt;
       ar[5] = dt;
     }
+    double2 nl[4] = {A.nr[idx], A.np[idx], A.nz[idx], A.nb[idx]};
+    if (A.hv) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) nl[q] = filt(g, f, nn, A.hv, A.nrc, A.tf, nl[q]);
+      nl[3] = filt(g, f, nn, A.hd, A.nrc, A.tf, nl[3]);
+    }
+    const double2 bold = A.b[idx];
+    if (A.b0) A.b0[idx] = bold;
     double2 ev[4], av[4];
     if (ab) {
-      ev[0] = sub2(rur[idx], hpr[idx]);
-      ev[1] = sub2(rup[idx], axc(notm0, hpp[idx]));
-      ev[2] = sub2(uz[idx], axc(notk0, hpz[idx]));
-      ev[3] = b[idx];
-      av[0] = sub2(axc(1.5, nr_[idx]), axc(0.5, hnr[idx]));
-      av[1] = sub2(axc(1.5, np_[idx]), axc(0.5, hnp[idx]));
-      av[2] = sub2(axc(1.5, nz_[idx]), axc(0.5, hnz[idx]));
-      av[3] = sub2(axc(1.5, nb_[idx]), axc(0.5, hnb[idx]));
+      ev[0] = sub2(A.rur[idx], A.hpr[idx]);
+      ev[1] = sub2(A.rup[idx], axc(notm0, A.hpp[idx]));
+      ev[2] = sub2(A.uz[idx], axc(notk0, A.hpz[idx]));
+      ev[3] = bold;
+      av[0] = sub2(axc(1.5, nl[0]), axc(0.5, A.hnr[idx]));
+      av[1] = sub2(axc(1.5, nl[1]), axc(0.5, A.hnp[idx]));
+      av[2] = sub2(axc(1.5, nl[2]), axc(0.5, A.hnz[idx]));
+      av[3] = sub2(axc(1.5, nl[3]), axc(0.5, A.hnb[idx]));
     } else {
-      ev[0] = rur[idx];
-      ev[1] = rup[idx];
-      ev[2] = uz[idx];
-      ev[3] = b[idx];
-      av[0] = nr_[idx];
-      av[1] = np_[idx];
-      av[2] = nz_[idx];
-      av[3] = nb_[idx];
+      ev[0] = A.rur[idx];
+      ev[1] = A.rup[idx];
+      ev[2] = A.uz[idx];
+      ev[3] = bold;
+      av[0] = nl[0];
+      av[1] = nl[1];
+      av[2] = nl[2];
+      av[3] = nl[3];
     }
     double2 cv[4];
+#pragma unroll
     for (int i = 0; i < 4; i++) {
       double2 s1 = zero2(), s2 = zero2();
+#pragma unroll
       for (int j = 0; j < 4; j++) {
         s1 = add2(s1, axc(er[i + 4 * j], ev[j]));
         s2 = add2(s2, axc(ar[i + 4 * j], av[j]));
       }
       cv[i] = add2(s1, s2);
     }
-    rur[idx] = cv[0];
-    rup[idx] = cv[1];
-    uz[idx] = cv[2];
-    b[idx] = cv[3];
+    if (A.hv) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) cv[q] = filt(g, f, nn, A.hv, A.nrc, A.tf, cv[q]);
+      cv[3] = filt(g, f, nn, A.hd, A.nrc, A.tf, cv[3]);
+    }
+    A.rur[idx] = cv[0];
+    A.rup[idx] = cv[1];
+    A.uz[idx] = cv[2];
+    A.b[idx] = cv[3];
+    if (save) {
+      A.hnr[idx] = nl[0];
+      A.hnp[idx] = nl[1];
+      A.hnz[idx] = nl[2];
+      A.hnb[idx] = nl[3];
+      A.hpr[idx] = cv[0];
+      A.hpp[idx] = cv[1];
+      A.hpz[idx] = cv[2];
+    }
   }
 }
 int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt, double2 *rur, double2 *rup, double2 *uz,
-                   double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_, const double2 *hpr,
-                   const double2 *hpp, const double2 *hpz, const double2 *hnr, const double2 *hnp, const double2 *hnz,
-                   const double2 *hnb, int ab) {
+                   double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_, double2 *hpr,
+                   double2 *hpp, double2 *hpz, double2 *hnr, double2 *hnp, double2 *hnz, double2 *hnb, int ab, int save,
+                   double2 *b0, int filter, double tf) {
   KGrid g = kgrid(c);
   Mat4x2 M;
   for (int q = 0; q < 16; q++) {
     M.e[q] = e16[q];
     M.nl[q] = nl16[q];
   }
-  KScope ks(c, K_ETD, 16.0 * fff_count(g) * (ab ? 19 : 12));  // 8 (15 with history) in, 4 out
-  hipLaunchKernelGGL(k_etd_sub, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, M, dt, rur, rup, uz, b, nr_, np_, nz_,
-                     nb_, hpr, hpp, hpz, hnr, hnp, hnz, hnb, ab);
+  EtdArgs A{rur, rup, uz, b, nr_, np_, nz_, nb_, hpr, hpp, hpz, hnr, hnp, hnz, hnb, b0, nullptr, nullptr, 0, tf};
+  if (filter) {
+    const mlg::Grid &G = c->g;
+    size_t nr1 = (size_t)G.p.nrchop * G.ntchopdim, nt = G.ntchopdim, nx = G.nxchopdim;
+    A.hv = c->d.hv;
+    A.hd = c->d.hv + (nr1 + nt + nx);
+    A.nrc = G.p.nrchop;
+  }
+  // 4 fields + 4 nonlinear terms in, 4 out; history: 7 in (ab), 7 out (save); b0 out
+  double nf = 12.0 + (ab ? 7.0 : 0.0) + (save ? 7.0 : 0.0) + (b0 ? 1.0 : 0.0);
+  KScope ks(c, K_ETD, 16.0 * fff_count(g) * nf);
+  hipLaunchKernelGGL(k_etd_sub, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, M, dt, A, ab, save);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

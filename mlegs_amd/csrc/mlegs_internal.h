@@ -191,10 +191,12 @@ int launch_fix_at_infinity(mlg_ctx *c, double2 *a);
 int launch_band_solve(mlg_ctx *c, int op, double2 *a, double *ln_a, const double *ln_b, double alp, int has_ln, double ln);
 int launch_axpby(mlg_ctx *c, int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n);
 int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1, const double *x2, const double *x3);
+// ETD update fused with the HYPERV3 filters (filter != 0), the copy of b
+// before the update (b0 != null) and the history stores (save != 0)
 int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt, double2 *rur, double2 *rup, double2 *uz,
-                   double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_,
-                   const double2 *hpr, const double2 *hpp, const double2 *hpz, const double2 *hnr, const double2 *hnp,
-                   const double2 *hnz, const double2 *hnb, int ab);
+                   double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_, double2 *hpr,
+                   double2 *hpp, double2 *hpz, double2 *hnr, double2 *hnp, double2 *hnz, double2 *hnb, int ab, int save,
+                   double2 *b0, int filter, double tf);
 int launch_hyp3(mlg_ctx *c, double2 *a, int diff, double tf);
 int launch_diagnost(mlg_ctx *c, const double2 *psi);
 int launch_energy_terms(mlg_ctx *c, const double2 *a, const double2 *b, int l2, double *out_all);
