@@ -673,6 +673,7 @@ extern "C" int mlg_grad(const mlg_field *a, mlg_field *rr, mlg_field *rp, mlg_fi
 extern "C" int mlg_pc2vel(const mlg_field *psi, const mlg_field *chi, mlg_field *rur, mlg_field *rup, mlg_field *uz) {
   if (!psi || !chi || !rur || !rup || !uz) return MLG_E_ARG;
   if (psi->space != MLG_FFF || chi->space != MLG_FFF) return 3;
+  if (rur == psi || rur == chi || rup == psi || rup == chi || uz == psi || uz == chi) return MLG_E_ARG;  // outputs must not alias inputs
   mlg_ctx *c = psi->ctx;
   rur->space = rup->space = uz->space = MLG_FFF;
   return launch_pc2vel(c, psi->e, psi->ln, chi->e, chi->ln, rur->e, rup->e, uz->e, rur->ln, rup->ln, uz->ln);
@@ -681,6 +682,7 @@ extern "C" int mlg_pc2vel(const mlg_field *psi, const mlg_field *chi, mlg_field 
 extern "C" int mlg_pc2vor(const mlg_field *psi, const mlg_field *chi, mlg_field *ror, mlg_field *rop, mlg_field *oz) {
   if (!psi || !chi || !ror || !rop || !oz) return MLG_E_ARG;
   if (psi->space != MLG_FFF || chi->space != MLG_FFF) return 3;
+  if (ror == psi || ror == chi || rop == psi || rop == chi || oz == psi || oz == chi) return MLG_E_ARG;  // outputs must not alias inputs
   mlg_ctx *c = psi->ctx;
   RET(launch_check_top_m1(c, chi->e));
   ror->space = rop->space = oz->space = MLG_FFF;
@@ -721,6 +723,10 @@ static int project_impl(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field
     RET(mlg_plnfft(rur, -1));
     RET(mlg_plnfft(rup, -1));
     RET(mlg_plnfft(uz, -1));
+  } else if (rur->space == MLG_PFP) {
+    RET(mlg_verfft(rur, -1));
+    RET(mlg_verfft(rup, -1));
+    RET(mlg_verfft(uz, -1));
   } else if (rur->space == MLG_FFF) {
     RET(mlg_rtran(rur, 1));
     RET(mlg_rtran(rup, 1));
@@ -749,6 +755,14 @@ extern "C" int mlg_project(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_fi
 }
 
 // ------------------------------------------------------------------ Boussinesq terms
+// FFF -> PFF -> PFP (TOFP without the theta transform, done by the fused
+// product kernel)
+static int to_pfp(mlg_field *a) {
+  if (a->space == MLG_FFF) RET(mlg_rtran(a, 1));
+  if (a->space == MLG_PFF) RET(mlg_verfft(a, 1));
+  return a->space == MLG_PFP ? 0 : 2;
+}
+
 // CALC_BOUSSI_FORCE (src/mod_boussinesq.f90:317-410)
 extern "C" int mlg_force(const mlg_field *psi, const mlg_field *chi, const mlg_field *b, mlg_field *psin, mlg_field *chin,
                          mlg_field *bn) {
@@ -763,28 +777,47 @@ extern "C" int mlg_force(const mlg_field *psi, const mlg_field *chi, const mlg_f
   if (!rc) rc = mlg_pc2vel(psi, chi, rur, rup, uz);
   if (!rc) rc = mlg_pc2vor(psi, chi, ror, rop, oz);
   mlg_field *fs[9] = {rur, rup, uz, ror, rop, oz, rbr, rbp, bz};
-  for (int i = 0; i < 9 && !rc; i++) rc = mlg_tofp(fs[i]);
+  const bool fused = fused_products_supported(c);
+  for (int i = 0; i < 9 && !rc; i++) rc = fused ? to_pfp(fs[i]) : mlg_tofp(fs[i]);
   mlg_chopset(c, -3);
   RET(rc);
   // W = TOFP(B) at NRCHOP (its value does not depend on the products, so it
-  // is formed first and fed to the fused product kernel)
+  // is formed first and fed to the product kernel)
   RET(copy0(w, b));
-  RET(mlg_tofp(w));
-  bn->space = MLG_PPP;
-  RET(launch_force_products(c, rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, w->e, bn->e,
-                            2 * c->g.p.omega, std::pow(c->g.p.bv0, 2.0), 1));
-  RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
-  RET(launch_chopdo(c, bn->e, MLG_PPP));
-  ror->space = rop->space = oz->space = MLG_PPP;
+  if (fused) {
+    RET(to_pfp(w));
+    const double2 *in[10] = {rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, w->e};
+    double2 *out[4] = {ror->e, rop->e, oz->e, bn->e};
+    RET(launch_fused_products(c, in, out, 2 * c->g.p.omega, std::pow(c->g.p.bv0, 2.0), 1));
+    bn->space = ror->space = rop->space = oz->space = MLG_PFP;
+    RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
+  } else {
+    RET(mlg_tofp(w));
+    bn->space = MLG_PPP;
+    RET(launch_force_products(c, rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, w->e, bn->e,
+                              2 * c->g.p.omega, std::pow(c->g.p.bv0, 2.0), 1));
+    RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
+    RET(launch_chopdo(c, bn->e, MLG_PPP));
+    ror->space = rop->space = oz->space = MLG_PPP;
+  }
   RET(mlg_project(ror, rop, oz, psin, w2, 1, 0.0));
   RET(mlg_idel2ln(w2, chin));
   return mlg_toff(bn);
 }
 
-// CALC_BOUSSI_NONLIN (src/mod_boussinesq.f90:235-314)
+// CALC_BOUSSI_NONLIN (src/mod_boussinesq.f90:235-314).  The velocity
+// arguments end in physical space, as in the reference, unless
+// `keep_velocity` (the time step, which overwrites them next) lets them stay
+// in PFP where the fused product kernel leaves them.
+static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *bn, mlg_field *ror,
+                       mlg_field *rop, mlg_field *oz, bool keep_velocity);
 extern "C" int mlg_nonlin(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *bn, mlg_field *ror,
                           mlg_field *rop, mlg_field *oz) {
   if (!b || !rur || !rup || !uz || !bn || !ror || !rop || !oz) return MLG_E_ARG;
+  return nonlin_impl(b, rur, rup, uz, bn, ror, rop, oz, false);
+}
+static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *bn, mlg_field *ror,
+                       mlg_field *rop, mlg_field *oz, bool keep_velocity) {
   if (b->space != MLG_FFF) return 3;
   mlg_ctx *c = b->ctx;
   Temps T(c);
@@ -795,12 +828,26 @@ extern "C" int mlg_nonlin(const mlg_field *b, mlg_field *rur, mlg_field *rup, ml
   mlg_chopset(c, -1);
   RET(rc);
   mlg_field *fs[9] = {rur, rup, uz, ror, rop, oz, rbr, rbp, bz};
-  for (int i = 0; i < 9; i++) RET(mlg_tofp(fs[i]));
-  bn->space = MLG_PPP;
-  RET(launch_force_products(c, rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, nullptr, bn->e, 0.0, 0.0,
-                            0));
-  RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
-  RET(launch_chopdo(c, bn->e, MLG_PPP));
+  if (fused_products_supported(c)) {
+    for (int i = 0; i < 9; i++) RET(to_pfp(fs[i]));
+    const double2 *in[10] = {rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, nullptr};
+    double2 *out[4] = {ror->e, rop->e, oz->e, bn->e};
+    RET(launch_fused_products(c, in, out, 0.0, 0.0, 0));
+    bn->space = ror->space = rop->space = oz->space = MLG_PFP;
+    RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
+    if (!keep_velocity) {
+      RET(mlg_horfft(rur, 1));
+      RET(mlg_horfft(rup, 1));
+      RET(mlg_horfft(uz, 1));
+    }
+  } else {
+    for (int i = 0; i < 9; i++) RET(mlg_tofp(fs[i]));
+    bn->space = MLG_PPP;
+    RET(launch_force_products(c, rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, nullptr, bn->e, 0.0,
+                              0.0, 0));
+    RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
+    RET(launch_chopdo(c, bn->e, MLG_PPP));
+  }
   RET(mlg_project(ror, rop, oz, psi, w, 1, 0.0));
   RET(mlg_idel2(w, chi, 0, 0.0));
   RET(mlg_chopset(c, 3));
@@ -1099,7 +1146,7 @@ static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   if (!rc) rc = mlg_pc2vor(psi, chi, rorn, ropn, ozn);
   mlg_chopset(c, -3);
   RET(rc);
-  RET(mlg_nonlin(b, rurn, rupn, uzn, bn, rorn, ropn, ozn));
+  RET(nonlin_impl(b, rurn, rupn, uzn, bn, rorn, ropn, ozn, true));
   RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
   // ETD_HYPERV(N) ; ETDFE_SUB ; ETD_HYPERV(u, b) ; history stores when saving
   RET(launch_etd_sub(c, e16, nl16, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
@@ -1156,7 +1203,7 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   if (!rc) rc = mlg_pc2vor(psi, chi, rorn, ropn, ozn);
   mlg_chopset(c, -3);
   RET(rc);
-  RET(mlg_nonlin(b, rurn, rupn, uzn, bn, rorn, ropn, ozn));
+  RET(nonlin_impl(b, rurn, rupn, uzn, bn, rorn, ropn, ozn, true));
   RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
   // ETD_HYPERV(N) ; B0 = B ; ETDAB_SUB ; N history ; ETD_HYPERV(u, b) ; pi history
   b0->space = MLG_FFF;

@@ -1,0 +1,298 @@
+// Nonlinear terms in physical space without physical-space fields in HBM.
+//
+// CALC_BOUSSI_FORCE / CALC_BOUSSI_NONLIN (src/mod_boussinesq.f90:235-410)
+// take nine fields to physical space (HORFFT backward), form u x omega and
+// u . grad b there (VPRODSUB/VDOTPSUB, src/mod_legops.f90:1972-2232, plus
+// FORCE's linear terms) and bring four results back (HORFFT forward).  This
+// kernel does all of that for two radial pencils of one z plane at a time,
+// reading the inputs in PFP (r, m, z) and writing the outputs in PFP: the
+// 512-point physical theta lines live only in LDS.
+//
+// Block: 256 threads in groups of 32 (2 pencils x 16 threads at NTH = 256);
+// a group transforms one field with the four-step FFT of k_fft.hip.  LDS:
+// eight slots of 2 x NTH complex values (64 KiB at NTH = 256).
+//   phase 1: inverse transforms of rur, rup, uz, ror, rop, oz, rbr, rbp
+//   products A (every point): X = rup oz - uz rop (+ 2 Omega rup),
+//            Y = uz ror - rur oz (- 2 Omega rur), Z = (rur rop - rup ror) / r^2,
+//            T = rur rbr + rup rbp
+//   phase 2: forward transforms of X, Y (and Z without linear terms); then
+//            inverse transforms of bz (and of W = b for FORCE)
+//   products B: N_b = -(T / r^2 + uz bz) (+ Nbar^2 uz); Z = Z - W (FORCE)
+//   phase 3: forward transforms of N_b (and Z for FORCE)
+// Two pencils per block keep the LDS within two blocks per CU; the price is
+// 32-byte runs in the PFP reads and writes.
+// Every value goes through the operations of the separate HORFFT and
+// product kernels, in the same order.
+#include "mlegs_internal.h"
+
+namespace mlg {
+
+__device__ __forceinline__ double2 fz_cmul(double2 a, double2 b) {
+  return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+}
+__device__ __forceinline__ double2 fz_conj(double2 a) { return make_double2(a.x, -a.y); }
+
+template <int R>
+struct FLog2 {
+  static constexpr int v = R <= 1 ? 0 : 1 + FLog2<R / 2>::v;
+};
+template <>
+struct FLog2<1> {
+  static constexpr int v = 0;
+};
+__device__ __forceinline__ constexpr int fz_bitrev(int i, int lg) {
+  int r = 0;
+  for (int b = 0; b < lg; b++) r |= ((i >> b) & 1) << (lg - 1 - b);
+  return r;
+}
+template <int R>
+__device__ __forceinline__ void fz_dft(double2 (&v)[R], const double2 (&wR)[R / 2 > 0 ? R / 2 : 1]) {
+  constexpr int LG = FLog2<R>::v;
+#pragma unroll
+  for (int i = 0; i < R; i++) {
+    const int j = fz_bitrev(i, LG);
+    if (j > i) {
+      double2 t = v[i];
+      v[i] = v[j];
+      v[j] = t;
+    }
+  }
+#pragma unroll
+  for (int half = 1; half < R; half *= 2) {
+#pragma unroll
+    for (int g0 = 0; g0 < R; g0 += 2 * half) {
+#pragma unroll
+      for (int p = 0; p < half; p++) {
+        const double2 w = wR[p * (R / (2 * half))];
+        double2 u = v[g0 + p], x = fz_cmul(v[g0 + p + half], w);
+        v[g0 + p] = make_double2(u.x + x.x, u.y + x.y);
+        v[g0 + p + half] = make_double2(u.x - x.x, u.y - x.y);
+      }
+    }
+  }
+}
+template <int R>
+__device__ __forceinline__ void fz_wr(double2 (&wR)[R / 2 > 0 ? R / 2 : 1], const double2 *__restrict__ tw, int N, bool inv) {
+#pragma unroll
+  for (int j = 0; j < (R / 2 > 0 ? R / 2 : 1); j++) {
+    double2 w = (R > 1) ? tw[j * (N / R)] : make_double2(1.0, 0.0);
+    if (inv) w.y = -w.y;
+    wR[j] = w;
+  }
+}
+
+constexpr int FP = 2;  // pencils per block
+
+// Four-step transform of the two pencils held by a group.  Pass 1 input in
+// v1 (thread t < R2 holds x[R2 n1 + t]); the exchange goes through `s`
+// (slot layout [index][pencil]); on return thread t < R1 holds
+// X[t + R1 k2] in v2.  All threads of the block call it (barriers).
+template <int R1, int R2>
+__device__ __forceinline__ void fz_fft(bool act, double2 (&v1)[R1], double2 (&v2)[R2], double2 *s, int p, int t,
+                                       const double2 *__restrict__ tw, bool inv) {
+  constexpr int N = R1 * R2;
+  if (act && t < R2) {
+    double2 w1[R1 / 2 > 0 ? R1 / 2 : 1];
+    fz_wr<R1>(w1, tw, N, inv);
+    fz_dft<R1>(v1, w1);
+#pragma unroll
+    for (int k1 = 0; k1 < R1; k1++) {
+      double2 w = tw[t * k1];
+      if (inv) w.y = -w.y;
+      s[(k1 * R2 + t) * FP + p] = (k1 == 0 || t == 0) ? v1[k1] : fz_cmul(v1[k1], w);
+    }
+  }
+  __syncthreads();
+  if (act && t < R1) {
+#pragma unroll
+    for (int n2 = 0; n2 < R2; n2++) v2[n2] = s[(t * R2 + n2) * FP + p];
+    double2 w2[R2 / 2 > 0 ? R2 / 2 : 1];
+    fz_wr<R2>(w2, tw, N, inv);
+    fz_dft<R2>(v2, w2);
+  }
+  __syncthreads();
+}
+
+// HORFFT backward of one PFP field into physical slots of `s`.
+template <int R1, int R2>
+__device__ void fz_inverse(bool act, const KGrid &g, const double2 *__restrict__ src, double2 *s, int r0, int z, int p, int t,
+                           const double2 *__restrict__ tw, const double2 *__restrict__ twh) {
+  constexpr int N = R1 * R2;
+  double2 v1[R1], v2[R2];
+  const int rr = r0 + p;
+  const int mmax = g.ntchopdim;
+  if (act && t < R2) {
+    auto X = [&](int m) -> double2 {
+      double2 v = make_double2(0.0, 0.0);
+      if (rr < g.nr && m < g.ntchop) v = src[(size_t)rr + (size_t)g.ndimr * ((size_t)m + (size_t)g.ndimth * z)];
+      if (m == 0) v.y = 0.0;
+      return v;
+    };
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) {
+      const int j = R2 * n1 + t;
+      double2 xm = (j < mmax) ? X(j) : make_double2(0.0, 0.0);
+      const int jc = N - j;
+      double2 xc = (jc < mmax && jc > 0 && j > 0) ? fz_conj(X(jc)) : make_double2(0.0, 0.0);
+      double2 e = make_double2(xm.x + xc.x, xm.y + xc.y);
+      double2 d = make_double2(xm.x - xc.x, xm.y - xc.y);
+      double2 o = fz_cmul(d, fz_conj(twh[j]));
+      v1[n1] = make_double2(e.x - o.y, e.y + o.x);
+    }
+  }
+  fz_fft<R1, R2>(act, v1, v2, s, p, t, tw, true);
+  if (act && t < R1) {
+    const bool live = rr < g.nr;
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) s[(t + R1 * k2) * FP + p] = live ? v2[k2] : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+}
+
+// HORFFT forward of physical slots in `s` into one PFP field.
+template <int R1, int R2>
+__device__ void fz_forward(bool act, const KGrid &g, double2 *s, double2 *__restrict__ dst, int r0, int z, int p, int t,
+                           int gt, const double2 *__restrict__ tw, const double2 *__restrict__ twh) {
+  constexpr int N = R1 * R2;
+  double2 v1[R1], v2[R2];
+  if (act && t < R2) {
+#pragma unroll
+    for (int n1 = 0; n1 < R1; n1++) v1[n1] = s[(R2 * n1 + t) * FP + p];
+  }
+  __syncthreads();
+  fz_fft<R1, R2>(act, v1, v2, s, p, t, tw, false);
+  if (act && t < R1) {
+#pragma unroll
+    for (int k2 = 0; k2 < R2; k2++) s[(t + R1 * k2) * FP + p] = v2[k2];
+  }
+  __syncthreads();
+  if (act) {
+    const double den = (double)(2 * N);
+    const int mmax = g.ntchopdim;
+    for (int idx = gt; idx < FP * mmax; idx += FP * (R1 > R2 ? R1 : R2)) {
+      const int pr = idx % FP, mm = idx / FP;
+      const int rr = r0 + pr;
+      if (rr >= g.ndimr) continue;
+      double2 zm = s[(mm % N) * FP + pr], zc = fz_conj(s[((N - mm) % N) * FP + pr]);
+      double2 e = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+      double2 d = make_double2(zm.x - zc.x, zm.y - zc.y);
+      double2 wd = fz_cmul(twh[mm], d);
+      double2 x = make_double2(e.x + 0.5 * wd.y, e.y - 0.5 * wd.x);
+      x = make_double2(x.x / den, x.y / den);
+      if (mm == 0) x.y = 0.0;
+      if (mm >= g.ntchop || rr >= g.nr) x = make_double2(0.0, 0.0);
+      dst[(size_t)rr + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * z)] = x;
+    }
+  }
+}
+
+struct FusedIO {
+  const double2 *in[10];  // rur rup uz ror rop oz rbr rbp bz [W]
+  double2 *out[4];        // r N_r, r N_theta, N_z, N_b (PFP)
+};
+
+template <int R1, int R2>
+__global__ void __launch_bounds__(256) k_fused_products(KGrid g, const double *__restrict__ rsq, const double2 *__restrict__ tw,
+                                                        const double2 *__restrict__ twh, FusedIO io, double two_omega,
+                                                        double n2, int linear) {
+  constexpr int N = R1 * R2, TPP = R1 > R2 ? R1 : R2, GS = FP * TPP, NG = 256 / GS, SL = FP * N;
+  static_assert(NG >= 8, "eight concurrent transforms");
+  __shared__ double2 S[8 * SL];
+  const int z = blockIdx.y, r0 = blockIdx.x * FP;
+  const int grp = threadIdx.x / GS, gt = threadIdx.x % GS, p = gt % FP, t = gt / FP;
+  auto slot = [&](int q) { return S + (size_t)q * SL; };
+
+  // phase 1: rur rup uz ror rop oz rbr rbp -> slots 0..7
+  {
+    const bool act = grp < 8;
+    const int q = act ? grp : 0;
+    fz_inverse<R1, R2>(act, g, io.in[q], slot(q), r0, z, p, t, tw, twh);
+  }
+  // products A
+  for (int q = threadIdx.x; q < SL; q += 256) {
+    const int pr = q % FP, rr = r0 + pr;
+    if (rr >= g.nr) continue;
+    const double r2 = rsq[rr];
+    const double2 A = slot(0)[q], B = slot(1)[q], C = slot(2)[q], D = slot(3)[q], E = slot(4)[q], F = slot(5)[q];
+    const double2 P = slot(6)[q], Q = slot(7)[q];
+    double2 xr = make_double2(B.x * F.x - C.x * E.x, B.y * F.y - C.y * E.y);
+    double2 xp = make_double2(C.x * D.x - A.x * F.x, C.y * D.y - A.y * F.y);
+    double2 xz = make_double2((A.x * E.x - B.x * D.x) / r2, (A.y * E.y - B.y * D.y) / r2);
+    double2 tt = make_double2(A.x * P.x + B.x * Q.x, A.y * P.y + B.y * Q.y);
+    if (linear) {
+      xr = make_double2(xr.x + two_omega * B.x, xr.y + two_omega * B.y);
+      xp = make_double2(xp.x - two_omega * A.x, xp.y - two_omega * A.y);
+    }
+    slot(3)[q] = xr;
+    slot(4)[q] = xp;
+    slot(5)[q] = xz;
+    slot(7)[q] = tt;
+  }
+  __syncthreads();
+  // phase 2: forward X, Y (Z if no linear terms); then inverse bz -> slot 6
+  // (and W -> slot 0, rur being no longer needed)
+  {
+    const bool fwd = grp < (linear ? 2 : 3);
+    fz_forward<R1, R2>(fwd, g, slot(3 + (fwd ? grp : 0)), io.out[fwd ? grp : 0], r0, z, p, t, gt, tw, twh);
+  }
+  {
+    const bool inv = (grp == 3) || (linear && grp == 4);
+    fz_inverse<R1, R2>(inv, g, io.in[grp == 4 ? 9 : 8], slot(grp == 4 ? 0 : 6), r0, z, p, t, tw, twh);
+  }
+  // products B
+  for (int q = threadIdx.x; q < SL; q += 256) {
+    const int pr = q % FP, rr = r0 + pr;
+    if (rr >= g.nr) continue;
+    const double r2 = rsq[rr];
+    const double2 C = slot(2)[q], R = slot(6)[q], T = slot(7)[q];
+    const double vr = T.x / r2 + C.x * R.x, vi = T.y / r2 + C.y * R.y;
+    double2 nb = make_double2(-vr, -vi);
+    if (linear) {
+      const double2 W = slot(0)[q], Z = slot(5)[q];
+      nb = make_double2(nb.x + n2 * C.x, nb.y + n2 * C.y);
+      slot(5)[q] = make_double2(Z.x - W.x, Z.y - W.y);
+    }
+    slot(7)[q] = nb;
+  }
+  __syncthreads();
+  // phase 3: forward N_b (and N_z with linear terms)
+  {
+    const bool act = (grp == 0) || (linear && grp == 1);
+    fz_forward<R1, R2>(act, g, slot(grp == 1 ? 5 : 7), io.out[grp == 1 ? 2 : 3], r0, z, p, t, gt, tw, twh);
+  }
+}
+
+bool fused_products_supported(const mlg_ctx *c) {
+  const int n = c->g.nth;
+  return n >= 2 && n <= 256 && (n & (n - 1)) == 0;
+}
+
+int launch_fused_products(mlg_ctx *c, const double2 *const in[10], double2 *const out[4], double two_omega, double n2,
+                          int linear) {
+  KGrid g = kgrid(c);
+  FusedIO io;
+  for (int i = 0; i < 10; i++) io.in[i] = in[i];
+  for (int i = 0; i < 4; i++) io.out[i] = out[i];
+  // PFP traffic: 9 (10) live input fields read, 4 written
+  KScope ks(c, K_PRODUCTS, 16.0 * g.nr * g.ntchopdim * (double)g.nx * ((linear ? 10 : 9) + 4));
+  dim3 grid((g.ndimr + FP - 1) / FP, g.ndimx);
+#define L_FP(a_, b_)                                                                                                \
+  hipLaunchKernelGGL((k_fused_products<a_, b_>), grid, dim3(256), 0, c->stream, g, c->d.r, c->d.tw_t, c->d.tw_h, io, \
+                     two_omega, n2, linear)
+  switch (g.nth) {
+    case 2: L_FP(2, 1); break;
+    case 4: L_FP(4, 1); break;
+    case 8: L_FP(8, 1); break;
+    case 16: L_FP(16, 1); break;
+    case 64: L_FP(8, 8); break;
+    case 32: L_FP(16, 2); break;
+    case 128: L_FP(16, 8); break;
+    case 256: L_FP(16, 16); break;
+    default: return MLG_E_ARG;
+  }
+#undef L_FP
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+}  // namespace mlg

@@ -17,10 +17,8 @@ __global__ void k_vprod(KGrid g, const double *__restrict__ rr, const double2 *_
                         const double2 *__restrict__ rup, const double2 *__restrict__ uz, double2 *__restrict__ ror,
                         double2 *__restrict__ rop, double2 *__restrict__ oz, long total) {
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    int r = (int)(idx % g.nr);
-    long rest = idx / g.nr;
-    int j = (int)(rest % g.nth);
-    int z = (int)(rest / g.nth);
+    const unsigned u = (unsigned)idx, q = u / (unsigned)g.nr, zz = q / (unsigned)g.nth;
+    const int r = (int)(u - q * (unsigned)g.nr), j = (int)(q - zz * (unsigned)g.nth), z = (int)zz;
     size_t id = (size_t)r + (size_t)g.ndimr * ((size_t)j + (size_t)g.ndimth * z);
     double2 A = rur[id], B = rup[id], Cc = uz[id], D = ror[id], E = rop[id], F = oz[id];
     double r2 = rr[r];  // r_i^2
@@ -35,10 +33,8 @@ __global__ void k_vdot(KGrid g, const double *__restrict__ rr, const double2 *__
                        const double2 *__restrict__ rbp, const double2 *__restrict__ bz, double2 *__restrict__ out,
                        double sign, long total) {
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    int r = (int)(idx % g.nr);
-    long rest = idx / g.nr;
-    int j = (int)(rest % g.nth);
-    int z = (int)(rest / g.nth);
+    const unsigned u = (unsigned)idx, q = u / (unsigned)g.nr, zz = q / (unsigned)g.nth;
+    const int r = (int)(u - q * (unsigned)g.nr), j = (int)(q - zz * (unsigned)g.nth), z = (int)zz;
     size_t id = (size_t)r + (size_t)g.ndimr * ((size_t)j + (size_t)g.ndimth * z);
     double2 a1 = rar[id], a2 = rap[id], a3 = az[id], b1 = rbr[id], b2 = rbp[id], b3 = bz[id];
     double r2 = rr[r];  // r_i^2
@@ -55,10 +51,8 @@ __global__ void k_force_products(KGrid g, const double *__restrict__ rr, const d
                                  const double2 *__restrict__ bz, const double2 *__restrict__ bphys,
                                  double2 *__restrict__ bn, double two_omega, double n2, int linear, long total) {
   for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < total; idx += (long)gridDim.x * blockDim.x) {
-    int r = (int)(idx % g.nr);
-    long rest = idx / g.nr;
-    int j = (int)(rest % g.nth);
-    int z = (int)(rest / g.nth);
+    const unsigned u = (unsigned)idx, q = u / (unsigned)g.nr, zz = q / (unsigned)g.nth;
+    const int r = (int)(u - q * (unsigned)g.nr), j = (int)(q - zz * (unsigned)g.nth), z = (int)zz;
     size_t id = (size_t)r + (size_t)g.ndimr * ((size_t)j + (size_t)g.ndimth * z);
     double2 A = rur[id], B = rup[id], Cc = uz[id], D = ror[id], E = rop[id], F = oz[id];
     double2 P = rbr[id], Q = rbp[id], R = bz[id];
@@ -132,17 +126,30 @@ __global__ void __launch_bounds__(256) k_vel2pln(KGrid g, const double *__restri
   const int nn = g.nrchop;         // NRCHOPS(1) at m = 0
   for (int n = threadIdx.x; n < nn; n += blockDim.x) {
     double cr = 0.0;
-    for (int i = 0; i < g.nrh; i++) {
-      double p = rup[i].x, q = rup[g.nr - 1 - i].x;
-      double f = ((n & 1) ? (p - q) : (p + q)) * w[i];
-      cr += f * pf[(size_t)n * g.nrh + i];
+    const double *pfn = pf + (size_t)n * g.nrh;
+    for (int ib = 0; ib < g.nrh; ib += 8) {  // loads in blocks of 8, sum in ascending i
+      double p[8], q[8], wv[8], t[8];
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (ib + u < g.nrh) {
+          p[u] = rup[ib + u].x;
+          q[u] = rup[g.nr - 1 - ib - u].x;
+          wv[u] = w[ib + u];
+          t[u] = pfn[ib + u];
+        }
+#pragma unroll
+      for (int u = 0; u < 8; u++)
+        if (ib + u < g.nrh) {
+          double f = ((n & 1) ? (p[u] - q[u]) : (p[u] + q[u])) * wv[u];
+          cr += f * t[u];
+        }
     }
-    crs[n] = cr;
+    crs[n] = cr * exp(lognorm[n]);
   }
   __syncthreads();
   if (threadIdx.x == 0) {
     double s = 0.0;
-    for (int n = 0; n < nn; n++) s += crs[n] * exp(lognorm[n]);  // ascending n
+    for (int n = 0; n < nn; n++) s += crs[n];  // ascending n
     crs[nn] = -s / 2.0;
   }
   __syncthreads();

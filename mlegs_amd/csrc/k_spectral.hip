@@ -27,12 +27,14 @@ __device__ __forceinline__ double2 ix(double s, double2 x) {  // (0 + i s) * x
 struct FIdx {
   int n, mm, k;
 };
+// 32-bit decode (every FFF field has fewer than 2^32 elements)
 __device__ __forceinline__ FIdx fdecode(const KGrid &g, long idx) {
   FIdx f;
-  f.n = (int)(idx % g.nrchopdim);
-  long r = idx / g.nrchopdim;
-  f.mm = (int)(r % g.ntchopdim);
-  f.k = (int)(r / g.ntchopdim);
+  const unsigned u = (unsigned)idx, nr = (unsigned)g.nrchopdim, nt = (unsigned)g.ntchopdim;
+  const unsigned r = u / nr;
+  f.n = (int)(u - r * nr);
+  f.k = (int)(r / nt);
+  f.mm = (int)(r - (unsigned)f.k * nt);
   return f;
 }
 __device__ __forceinline__ size_t fcol(const KGrid &g, int mm, int k) {
@@ -137,33 +139,41 @@ __global__ void k_chopdo_fff(KGrid g, double2 *a) {
   }
 }
 __global__ void k_chopdo_rows(KGrid g, double2 *a, int n2, int n3, int stride2, int zero_slot) {
-  // rows NR..NDIMR-1 for every (j < n2, z < n3); optionally the theta slot `zero_slot`
-  long tot = (long)n2 * n3 * g.ndimr;
-  GRID_STRIDE(tot) {
-    int r = (int)(idx % g.ndimr);
-    long rest = idx / g.ndimr;
-    int j = (int)(rest % n2);
-    int z = (int)(rest / n2);
-    size_t id = (size_t)r + (size_t)g.ndimr * ((size_t)j + (size_t)stride2 * z);
-    if (r >= g.nr || j == zero_slot) a[id] = zero2();
+  // rows NR..NDIMR-1 for every (j < n2, z < n3), then (optionally) every
+  // row of theta slot `zero_slot`; only the zeroed positions are visited
+  const unsigned pad = (unsigned)(g.ndimr - g.nr);
+  const unsigned ta = pad * (unsigned)n2 * (unsigned)n3;
+  const unsigned tb = zero_slot >= 0 ? (unsigned)g.ndimr * (unsigned)n3 : 0u;
+  GRID_STRIDE((long)ta + tb) {
+    const unsigned u = (unsigned)idx;
+    size_t id;
+    if (u < ta) {
+      const unsigned q = u / pad, r = g.nr + (u - q * pad);
+      const unsigned z = q / (unsigned)n2, j = q - z * (unsigned)n2;
+      id = (size_t)r + (size_t)g.ndimr * ((size_t)j + (size_t)stride2 * z);
+    } else {
+      const unsigned v = u - ta, z = v / (unsigned)g.ndimr, r = v - z * (unsigned)g.ndimr;
+      id = (size_t)r + (size_t)g.ndimr * ((size_t)zero_slot + (size_t)stride2 * z);
+    }
+    a[id] = zero2();
   }
 }
 
 int launch_chopdo(mlg_ctx *c, double2 *a, int space) {
   KGrid g = kgrid(c);
+  auto rows = [&](int n2, int n3, int stride2, int zero_slot) {
+    long tot = (long)(g.ndimr - g.nr) * n2 * n3 + (zero_slot >= 0 ? (long)g.ndimr * n3 : 0);
+    if (tot > 0)
+      hipLaunchKernelGGL(k_chopdo_rows, dim3(nblocks(tot)), dim3(256), 0, c->stream, g, a, n2, n3, stride2, zero_slot);
+  };
   if (space == MLG_FFF) {
     hipLaunchKernelGGL(k_chopdo_fff, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, a);
   } else if (space == MLG_PFF) {
-    long tot = (long)g.ndimr * g.ntchopdim * g.nxchopdim;
-    hipLaunchKernelGGL(k_chopdo_rows, dim3(nblocks(tot)), dim3(256), 0, c->stream, g, a, g.ntchopdim, g.nxchopdim,
-                       g.ntchopdim, -1);
+    rows(g.ntchopdim, g.nxchopdim, g.ntchopdim, -1);
   } else if (space == MLG_PPP) {
-    long tot = (long)g.ndimr * g.ndimth * g.ndimx;
-    hipLaunchKernelGGL(k_chopdo_rows, dim3(nblocks(tot)), dim3(256), 0, c->stream, g, a, g.ndimth, g.ndimx, g.ndimth,
-                       g.ndimth > g.nth ? g.nth : -1);
+    rows(g.ndimth, g.ndimx, g.ndimth, g.ndimth > g.nth ? g.nth : -1);
   } else {  // PFP: rows >= NR of the live slots m < NTCHOPDIM
-    long tot = (long)g.ndimr * g.ntchopdim * g.ndimx;
-    hipLaunchKernelGGL(k_chopdo_rows, dim3(nblocks(tot)), dim3(256), 0, c->stream, g, a, g.ntchopdim, g.ndimx, g.ndimth, -1);
+    rows(g.ntchopdim, g.ndimx, g.ndimth, -1);
   }
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
@@ -465,10 +475,22 @@ int launch_pc2vor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const do
 __global__ void k_fix_inf(KGrid g, const double *at1, double2 *a) {
   int k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= g.nxchopdim) return;
-  const size_t cb = (size_t)g.nrchopdim * (size_t)g.ntchopdim * k;
+  const double2 *col = a + (size_t)g.nrchopdim * (size_t)g.ntchopdim * k;
   double2 s = zero2();
-  for (int n = 0; n < g.nrchop; n++) s = add2(s, axc(at1[n], a[cb + n]));
-  a[cb] = sub2(a[cb], make_double2(s.x / g.at1_0, s.y / g.at1_0));
+  for (int nb = 0; nb < g.nrchop; nb += 16) {  // loads in blocks of 16, sum in ascending n
+    double2 v[16];
+    double w[16];
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      if (nb + q < g.nrchop) {
+        v[q] = col[nb + q];
+        w[q] = at1[nb + q];
+      }
+#pragma unroll
+    for (int q = 0; q < 16; q++)
+      if (nb + q < g.nrchop) s = add2(s, axc(w[q], v[q]));
+  }
+  a[(size_t)g.nrchopdim * (size_t)g.ntchopdim * k] = sub2(col[0], make_double2(s.x / g.at1_0, s.y / g.at1_0));
 }
 int launch_fix_at_infinity(mlg_ctx *c, double2 *a) {
   KGrid g = kgrid(c);
@@ -1089,18 +1111,19 @@ __global__ void __launch_bounds__(256) k_diagnost(KGrid g, const double2 *psi, i
   if (threadIdx.x == 0) div = 0;
   double a0 = -1e300, a1 = -1e300;
   const int n0 = s_nrchops(g.nrchop, 0), n1 = s_nrchops(g.nrchop, g.minc);
-  long tot = (long)g.nrchopdim * g.nxchopdim;
-  for (long i = threadIdx.x; i < tot; i += blockDim.x) {
-    int n = (int)(i % g.nrchopdim), k = (int)(i / g.nrchopdim);
-    if (n >= n0) {
-      double2 v = psi[(size_t)n + (size_t)g.nrchopdim * (size_t)g.ntchopdim * k];
-      a0 = fmax(a0, hypot(v.x, v.y));
-    }
-    if (g.ntchopdim >= 2 && n >= n1) {
-      double2 v = psi[(size_t)n + (size_t)g.nrchopdim * (1 + (size_t)g.ntchopdim * k)];
+  const int r0 = g.nrchopdim - n0, r1 = g.nrchopdim - n1;  // rows checked per k at m = 0, 1
+  const size_t colk = (size_t)g.nrchopdim * (size_t)g.ntchopdim;
+  for (int i = threadIdx.x; i < r0 * g.nxchopdim; i += blockDim.x) {
+    const int k = i / r0, n = n0 + (i - k * r0);
+    double2 v = psi[(size_t)n + colk * k];
+    a0 = fmax(a0, hypot(v.x, v.y));
+  }
+  if (g.ntchopdim >= 2)
+    for (int i = threadIdx.x; i < r1 * g.nxchopdim; i += blockDim.x) {
+      const int k = i / r1, n = n1 + (i - k * r1);
+      double2 v = psi[(size_t)n + (size_t)g.nrchopdim + colk * k];
       a1 = fmax(a1, hypot(v.x, v.y));
     }
-  }
   m0[threadIdx.x] = a0;
   m1[threadIdx.x] = a1;
   __syncthreads();

@@ -170,6 +170,11 @@ int launch_force_products(mlg_ctx *c, const double2 *rur, const double2 *rup, co
                           double2 *bn, double two_omega, double n2, int linear);
 // ln_mode: 0 psi LN = the computed value, 1 psi LN = ln, 2 psi LN unchanged
 int launch_vel2pln(mlg_ctx *c, double2 *rup, double *psi_ln, int ln_mode, double ln);
+// ---- fused theta transforms + products, PFP in and out (k_fused.hip)
+bool fused_products_supported(const mlg_ctx *c);
+// in: rur rup uz ror rop oz rbr rbp bz [W]; out: r N_r, r N_theta, N_z, N_b
+int launch_fused_products(mlg_ctx *c, const double2 *const in[10], double2 *const out[4], double two_omega, double n2,
+                          int linear);
 // ---- spectral-space kernels (k_spectral.hip)
 int launch_chopdo(mlg_ctx *c, double2 *a, int space);
 int launch_zero(mlg_ctx *c, double2 *a, long n);
