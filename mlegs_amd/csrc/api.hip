@@ -1011,55 +1011,82 @@ static int richardson(mlg_field *y, const mlg_field *x1, bool ln) {
   return 0;
 }
 
-// VISC1 (mod_march.f90:1529-1576): backward Euler diffusion
-static int visc1(mlg_field *a, double nu, double dt, mlg_field *wk) {
-  if (nu == 0.0) return 5;
-  mlg_ctx *c = a->ctx;
-  double alp = 1 / (nu * dt);
-  wk->space = MLG_FFF;
-  RET(launch_set_ln(c, wk->ln, a->ln, -alp, 0.0));
-  RET(launch_copy(c, wk->e, a->e, c->n_fff));
-  RET(launch_scale(c, wk->e, -alp, c->n_fff));
-  return mlg_ihelm(wk, a, alp);
+// IHELM of n independent fields, A_i = IHELM(B_i, ALP_i), with one band-solve
+// launch for all of them (each field as solve_op does it for op 2).
+static int ihelm_n(int n, mlg_field *const *b, mlg_field *const *a, const double *alp) {
+  mlg_ctx *c = b[0]->ctx;
+  double2 *ae[3];
+  double *aln[3];
+  const double *bln[3];
+  for (int i = 0; i < n; i++) {
+    if (alp[i] == 0.0) return 3;  // IHELM: ALP CAN'T BE ZERO (mod_legops.f90:1243-1246)
+    if (b[i]->space != MLG_FFF) return 3;
+    RET(copy0(a[i], b[i]));  // A = B
+    ae[i] = a[i]->e;
+    aln[i] = a[i]->ln;
+    bln[i] = b[i]->ln;
+  }
+  RET(launch_band_solve_n(c, 2, n, ae, aln, bln, alp, 0, 0.0));
+  for (int i = 0; i < n; i++) RET(launch_chopdo(c, a[i]->e, MLG_FFF));
+  return 0;
 }
 
-// VISC2 (mod_march.f90:1579-1636): Crank-Nicolson diffusion
-static int visc2(mlg_field *a, const mlg_field *ao, double nu, double dt, mlg_field *wk) {
+// VISC1 (mod_march.f90:1529-1576): backward Euler diffusion; the right-hand
+// side WK = -ALP*A, the solve A = IHELM(WK, ALP) follows (batched)
+static int visc1_rhs(mlg_field *a, double nu, double dt, mlg_field *wk, double *alp) {
   if (nu == 0.0) return 5;
   mlg_ctx *c = a->ctx;
-  double alp = 2 / nu / dt;
+  *alp = 1 / (nu * dt);
+  wk->space = MLG_FFF;
+  RET(launch_set_ln(c, wk->ln, a->ln, -*alp, 0.0));
+  RET(launch_copy(c, wk->e, a->e, c->n_fff));
+  return launch_scale(c, wk->e, -*alp, c->n_fff);
+}
+
+// VISC2 (mod_march.f90:1579-1636): Crank-Nicolson diffusion; the right-hand
+// side WK = -DEL2(AO) - ALP*A, then A = IHELM(WK, ALP) (batched)
+static int visc2_rhs(mlg_field *a, const mlg_field *ao, double nu, double dt, mlg_field *wk, double *alp) {
+  if (nu == 0.0) return 5;
+  mlg_ctx *c = a->ctx;
+  *alp = 2 / nu / dt;
   RET(mlg_chopset(c, 2));
   int rc = mlg_del2(ao, wk);
   mlg_chopset(c, -2);
   RET(rc);
   RET(launch_chopdo(c, wk->e, MLG_FFF));
   // WK = -WK - ALP*A
-  RET(launch_axpby(c, 5, wk->e, alp, a->e, nullptr, nullptr, c->n_fff));
-  RET(launch_set_ln(c, wk->ln, a->ln, -alp, 0.0));
-  return mlg_ihelm(wk, a, alp);
+  RET(launch_axpby(c, 5, wk->e, *alp, a->e, nullptr, nullptr, c->n_fff));
+  return launch_set_ln(c, wk->ln, a->ln, -*alp, 0.0);
 }
 
+// psi, chi (NU) and b (KAPPA, when non-zero) are independent: their right-
+// hand sides are formed in turn, then the three IHELM solves share a launch.
 static int visc_be(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, double dt) {
   if (c->g.p.visc_sw != 1) return 0;
   Temps T(c);
-  mlg_field *wk = T.get();
+  mlg_field *wk[3] = {T.get(), T.get(), T.get()};
   RET(T.ready());
-  RET(visc1(psi, c->g.p.nu, dt, wk));
-  RET(visc1(chi, c->g.p.nu, dt, wk));
-  if (c->g.p.kappa != 0.0) RET(visc1(b, c->g.p.kappa, dt, wk));
-  return 0;
+  mlg_field *f[3] = {psi, chi, b};
+  const double nu[3] = {c->g.p.nu, c->g.p.nu, c->g.p.kappa};
+  const int n = c->g.p.kappa != 0.0 ? 3 : 2;
+  double alp[3];
+  for (int i = 0; i < n; i++) RET(visc1_rhs(f[i], nu[i], dt, wk[i], &alp[i]));
+  return ihelm_n(n, wk, f, alp);
 }
 
 static int visc_cn(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, const mlg_field *psio, const mlg_field *chio,
                    const mlg_field *bo) {
   if (c->g.p.visc_sw != 1) return 0;
   Temps T(c);
-  mlg_field *wk = T.get();
+  mlg_field *wk[3] = {T.get(), T.get(), T.get()};
   RET(T.ready());
-  RET(visc2(psi, psio, c->g.p.nu, c->g.p.dt, wk));
-  RET(visc2(chi, chio, c->g.p.nu, c->g.p.dt, wk));
-  if (c->g.p.kappa != 0.0) RET(visc2(b, bo, c->g.p.kappa, c->g.p.dt, wk));
-  return 0;
+  mlg_field *f[3] = {psi, chi, b};
+  const mlg_field *fo[3] = {psio, chio, bo};
+  const double nu[3] = {c->g.p.nu, c->g.p.nu, c->g.p.kappa};
+  const int n = c->g.p.kappa != 0.0 ? 3 : 2;
+  double alp[3];
+  for (int i = 0; i < n; i++) RET(visc2_rhs(f[i], fo[i], nu[i], c->g.p.dt, wk[i], &alp[i]));
+  return ihelm_n(n, wk, f, alp);
 }
 
 static bool filter_active(const mlg_ctx *c) { return c->g.p.visc_sw == 3 || c->g.p.hyper_cn; }

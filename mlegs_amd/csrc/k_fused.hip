@@ -20,10 +20,12 @@
 //   products B: N_b = -(T / r^2 + uz bz) (+ Nbar^2 uz); Z = Z - W (FORCE)
 //   phase 3: forward transforms of N_b (and Z for FORCE)
 // Two pencils per block keep the LDS within two blocks per CU; the price is
-// 32-byte runs in the PFP reads and writes.
+// 32-byte runs in the PFP reads and writes, which the XCD-aware block order
+// turns into shared L2 lines.
 // Every value goes through the operations of the separate HORFFT and
 // product kernels, in the same order.
 #include "mlegs_internal.h"
+#include "xcd.h"
 
 namespace mlg {
 
@@ -198,7 +200,12 @@ __global__ void __launch_bounds__(256) k_fused_products(KGrid g, const double *_
   constexpr int N = R1 * R2, TPP = R1 > R2 ? R1 : R2, GS = FP * TPP, NG = 256 / GS, SL = FP * N;
   static_assert(NG >= 8, "eight concurrent transforms");
   __shared__ double2 S[8 * SL];
-  const int z = blockIdx.y, r0 = blockIdx.x * FP;
+  // logical block = (pencil pair, z) with the pair fastest: the four pairs
+  // of one 128-byte line run on one XCD
+  const int nrp = (g.ndimr + FP - 1) / FP;
+  const int lid = xcd_logical(nrp * g.ndimx);
+  if (lid < 0) return;
+  const int z = lid / nrp, r0 = (lid % nrp) * FP;
   const int grp = threadIdx.x / GS, gt = threadIdx.x % GS, p = gt % FP, t = gt / FP;
   auto slot = [&](int q) { return S + (size_t)q * SL; };
 
@@ -262,6 +269,279 @@ __global__ void __launch_bounds__(256) k_fused_products(KGrid g, const double *_
   }
 }
 
+// ---------------------------------------------------------------------------
+// NTH = 256: register-resident variant.
+//
+// 32 lanes own one theta line (two lines per wave, eight adjacent radial
+// nodes per block).  Each lane keeps 8 of the line's 256 physical points in
+// registers, at the same positions for every field, so the products are
+// formed in registers and only the transposes of the transforms use LDS.
+// The 256-point transform is 8 x 8 x 4:
+//   j = 32 a + b (lane b holds a = 0..7)      -> 8-point DFT over a, twiddle
+//   exchange to lane 4 c + f (holds e = 0..7)  -> 8-point DFT over e, twiddle
+//   exchange to lane 4 c + gh (holds f = 0..3 for g = gh, gh + 4)
+//                                               -> 4-point DFTs over f
+// so lane 4 c + gh ends with the points k = c + 8 g + 64 h, h = 0..3.  The
+// forward transform runs the same stages backwards with conjugate roots.
+// Waves never wait for one another; the LDS of a line is touched only by
+// its own 32 lanes.
+// ---------------------------------------------------------------------------
+namespace w256 {
+
+constexpr int N = 256;
+constexpr int LP = 288;  // LDS double2 per line (two exchange layouts, padded against bank conflicts)
+__device__ __forceinline__ int e1(int c, int b) { return c * 36 + b; }
+__device__ __forceinline__ int e2(int c, int f, int g) { return c * 36 + f * 9 + g; }
+
+__device__ __forceinline__ void wave_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Per-lane twiddles, copied to LDS once per block:
+//   t1[32 c + b] = W_N^{b c}, t2[4 g + f] = W_N^{8 f g}, th[k] = exp(-i pi k / N)
+struct Ctx {
+  const double2 *t1, *t2, *th;
+  double2 *s;                       // LDS of this line
+  int b;                            // lane within the line
+  double2 w8i[4], w8f[4], w4i[2], w4f[2];
+};
+
+__device__ __forceinline__ double2 cj(double2 w, bool inv) {
+  if (inv) w.y = -w.y;
+  return w;
+}
+
+// HORFFT backward of one PFP line (element m at src[lo + ndimr m], src the
+// same for the whole block); lane b ends with the physical points
+// P[c + 8 (gh + 4 s) + 64 h] in v[4 s + h], c = b >> 2, gh = b & 3.
+// Needs NTCHOPDIM <= MMAX96: only a = 0..2 hold modes, a = 5..7 partners.
+constexpr int MMAX96 = 96;
+__device__ __forceinline__ void inverse(const Ctx &x, const KGrid &g, const double2 *__restrict__ src, int lo,
+                                        bool live, double2 (&v)[8]) {
+  const int b = x.b, mmax = g.ntchopdim;
+  // branch-free: masked lanes load row 0 and discard it
+  auto X = [&](int m, bool use) -> double2 {
+    use = use && live && m < g.ntchop;
+    double2 t = src[lo + (use ? g.ndimr * m : 0)];
+    if (!use) t = make_double2(0.0, 0.0);
+    if (m == 0) t.y = 0.0;
+    return t;
+  };
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    const int j = 32 * a + b;
+    double2 xm = (a < 3) ? X(j, j < mmax) : make_double2(0.0, 0.0);
+    const int jc = N - j;
+    double2 xc = (a >= 5) ? fz_conj(X(jc, jc < mmax)) : make_double2(0.0, 0.0);
+    double2 e = make_double2(xm.x + xc.x, xm.y + xc.y);
+    double2 d = make_double2(xm.x - xc.x, xm.y - xc.y);
+    double2 o = fz_cmul(d, fz_conj(x.th[j]));
+    v[a] = make_double2(e.x - o.y, e.y + o.x);
+  }
+  fz_dft<8>(v, x.w8i);
+  // twiddles W^0 = 1 multiply exactly
+#pragma unroll
+  for (int c = 1; c < 8; c++) v[c] = fz_cmul(v[c], cj(x.t1[32 * c + b], true));
+  // exchange 1: lane b holds y_b[c]; lane 4 c + f takes y_{4 e + f}[c]
+#pragma unroll
+  for (int c = 0; c < 8; c++) x.s[e1(c, b)] = v[c];
+  wave_sync();
+  const int c0 = b >> 2, f0 = b & 3;
+#pragma unroll
+  for (int e = 0; e < 8; e++) v[e] = x.s[e1(c0, 4 * e + f0)];
+  wave_sync();
+  fz_dft<8>(v, x.w8i);
+#pragma unroll
+  for (int gg = 1; gg < 8; gg++) v[gg] = fz_cmul(v[gg], cj(x.t2[4 * gg + f0], true));
+  // exchange 2: lane 4 c + f holds z_{c,f}[g]; lane 4 c + gh takes f = 0..3 of g = gh, gh + 4
+#pragma unroll
+  for (int gg = 0; gg < 8; gg++) x.s[e2(c0, f0, gg)] = v[gg];
+  wave_sync();
+  double2 t[2][4];
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++)
+#pragma unroll
+    for (int f = 0; f < 4; f++) t[sg][f] = x.s[e2(c0, f, f0 + 4 * sg)];
+  wave_sync();
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++) {
+    fz_dft<4>(t[sg], x.w4i);
+#pragma unroll
+    for (int h = 0; h < 4; h++) v[4 * sg + h] = t[sg][h];
+  }
+}
+
+// HORFFT forward of this lane's physical points v (positions as left by
+// `inverse`) into one PFP line (element m at dst[lo + ndimr m]).
+__device__ __forceinline__ void forward(const Ctx &x, const KGrid &g, const double2 (&v)[8], double2 *__restrict__ dst,
+                                        int lo, bool inr, bool live) {
+  const int b = x.b, c0 = b >> 2, f0 = b & 3;
+  double2 t[2][4];
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++) {
+#pragma unroll
+    for (int h = 0; h < 4; h++) t[sg][h] = v[4 * sg + h];
+    fz_dft<4>(t[sg], x.w4f);
+    const int gg = f0 + 4 * sg;
+#pragma unroll
+    for (int f = 1; f < 4; f++) t[sg][f] = fz_cmul(t[sg][f], x.t2[4 * gg + f]);
+  }
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++)
+#pragma unroll
+    for (int f = 0; f < 4; f++) x.s[e2(c0, f, f0 + 4 * sg)] = t[sg][f];
+  wave_sync();
+  double2 u[8];
+#pragma unroll
+  for (int gg = 0; gg < 8; gg++) u[gg] = x.s[e2(c0, f0, gg)];
+  wave_sync();
+  fz_dft<8>(u, x.w8f);
+#pragma unroll
+  for (int e = 0; e < 8; e++) x.s[e1(c0, 4 * e + f0)] = u[e];
+  wave_sync();
+#pragma unroll
+  for (int c = 0; c < 8; c++) u[c] = x.s[e1(c, b)];
+  wave_sync();
+#pragma unroll
+  for (int c = 1; c < 8; c++) u[c] = fz_cmul(u[c], x.t1[32 * c + b]);
+  fz_dft<8>(u, x.w8f);
+  // u[a] = Y[32 a + b]; the Hermitian split pairs Y[m] with Y[N - m],
+  // which lane (32 - b) mod 32 holds at a' = 7 - a (lane 0: a' = (8 - a) mod 8)
+  const double den = (double)(2 * N);
+  const int src = (32 - b) & 31;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (32 * a >= g.ntchopdim) break;
+    const int m = 32 * a + b;
+    double2 pc;
+    pc.x = __shfl(u[7 - a].x, src, 32);
+    pc.y = __shfl(u[7 - a].y, src, 32);
+    if (b == 0) pc = u[(8 - a) & 7];
+    if (m < g.ntchopdim && inr) {
+      double2 zm = u[a], zc = fz_conj(pc);
+      double2 e = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+      double2 d = make_double2(zm.x - zc.x, zm.y - zc.y);
+      double2 wd = fz_cmul(x.th[m], d);
+      double2 o = make_double2(e.x + 0.5 * wd.y, e.y - 0.5 * wd.x);
+      o = make_double2(o.x / den, o.y / den);
+      if (m == 0) o.y = 0.0;
+      if (m >= g.ntchop || !live) o = make_double2(0.0, 0.0);
+      dst[lo + g.ndimr * m] = o;
+    }
+  }
+}
+
+}  // namespace w256
+
+// Same products as k_fused_products, in the same floating-point operations.
+// The outputs r N_r, r N_theta, N_z overwrite ror, rop, oz in the callers,
+// so all three are read before the first of them is written.  The fields go
+// to physical space one at a time; at most five stay in registers, and one
+// finished product (Y) waits in LDS.
+template <bool LINEAR>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_fused_products_256(KGrid g, const double *__restrict__ rsq, const double2 *tw, const double2 *twh, FusedIO io,
+                     double two_omega, double n2) {
+  using namespace w256;
+  __shared__ double2 S[8 * LP];
+  __shared__ double2 Y[8 * N];  // [line][point i][lane]
+  __shared__ double2 T1[256], T2[32], TH[N + 1];
+  const int nrb = (g.ndimr + 7) / 8;
+  const int lid = xcd_logical(nrb * g.ndimx);
+  if (lid < 0) return;
+  {
+    const int q = threadIdx.x;
+    T1[q] = tw[(q & 31) * (q >> 5)];
+    if (q < 32) T2[q] = tw[8 * (q & 3) * (q >> 2)];
+    TH[q] = twh[q];
+    if (q == 0) TH[N] = twh[N];
+  }
+  __syncthreads();
+  const int z = lid / nrb, line = threadIdx.x >> 5;
+  const int rr = (lid % nrb) * 8 + line;
+  const bool inr = rr < g.ndimr, live = rr < g.nr;
+  Ctx x;
+  x.t1 = T1;
+  x.t2 = T2;
+  x.th = TH;
+  x.s = S + line * LP;
+  x.b = threadIdx.x & 31;
+  double2 *ys = Y + line * N + x.b;
+  fz_wr<8>(x.w8i, tw, N, true);
+  fz_wr<8>(x.w8f, tw, N, false);
+  fz_wr<4>(x.w4i, tw, N, true);
+  fz_wr<4>(x.w4f, tw, N, false);
+  const size_t off = (size_t)(rr - line) + (size_t)g.ndimr * g.ndimth * z;  // same for the block
+  const int lo = inr ? line : 0;
+  auto in = [&](int q) { return io.in[q] + off; };
+  auto out = [&](int q) { return io.out[q] + off; };
+  const double r2 = live ? rsq[rr] : 1.0;
+
+  double2 A[8], B[8], C[8], U[8], V[8], T[8];
+  inverse(x, g, in(0), lo, live, A);  // rur
+  inverse(x, g, in(1), lo, live, B);  // rup
+  inverse(x, g, in(2), lo, live, C);  // uz
+  // oz: U = rup oz, V = rur oz
+  inverse(x, g, in(5), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    U[i] = make_double2(B[i].x * T[i].x, B[i].y * T[i].y);
+    V[i] = make_double2(A[i].x * T[i].x, A[i].y * T[i].y);
+  }
+  // ror: Y = uz ror - rur oz (to LDS); V = rup ror
+  inverse(x, g, in(3), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    ys[32 * i] = make_double2(C[i].x * T[i].x - V[i].x, C[i].y * T[i].y - V[i].y);
+    V[i] = make_double2(B[i].x * T[i].x, B[i].y * T[i].y);
+  }
+  // rop: X = rup oz - uz rop in U; Z = (rur rop - rup ror) / r^2 in V
+  inverse(x, g, in(4), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    U[i] = make_double2(U[i].x - C[i].x * T[i].x, U[i].y - C[i].y * T[i].y);
+    V[i] = make_double2((A[i].x * T[i].x - V[i].x) / r2, (A[i].y * T[i].y - V[i].y) / r2);
+  }
+  // ror, rop, oz are read: the outputs may now overwrite them
+  if (LINEAR) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) U[i] = make_double2(U[i].x + two_omega * B[i].x, U[i].y + two_omega * B[i].y);
+  }
+  forward(x, g, U, out(0), lo, inr, live);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    T[i] = ys[32 * i];
+    if (LINEAR) T[i] = make_double2(T[i].x - two_omega * A[i].x, T[i].y - two_omega * A[i].y);
+  }
+  forward(x, g, T, out(1), lo, inr, live);
+  if (LINEAR) {
+    // W: N_z = Z - W
+    inverse(x, g, in(9), lo, live, T);
+#pragma unroll
+    for (int i = 0; i < 8; i++) V[i] = make_double2(V[i].x - T[i].x, V[i].y - T[i].y);
+  }
+  forward(x, g, V, out(2), lo, inr, live);
+  // rbr, rbp: A = rur rbr + rup rbp
+  inverse(x, g, in(6), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) A[i] = make_double2(A[i].x * T[i].x, A[i].y * T[i].y);
+  inverse(x, g, in(7), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) A[i] = make_double2(A[i].x + B[i].x * T[i].x, A[i].y + B[i].y * T[i].y);
+  // bz: N_b = -(T / r^2 + uz bz) (+ Nbar^2 uz)
+  inverse(x, g, in(8), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const double vr = A[i].x / r2 + C[i].x * T[i].x, vi = A[i].y / r2 + C[i].y * T[i].y;
+    double2 nb = make_double2(-vr, -vi);
+    if (LINEAR) nb = make_double2(nb.x + n2 * C[i].x, nb.y + n2 * C[i].y);
+    A[i] = nb;
+  }
+  forward(x, g, A, out(3), lo, inr, live);
+}
+
 bool fused_products_supported(const mlg_ctx *c) {
   const int n = c->g.nth;
   return n >= 2 && n <= 256 && (n & (n - 1)) == 0;
@@ -275,7 +555,18 @@ int launch_fused_products(mlg_ctx *c, const double2 *const in[10], double2 *cons
   for (int i = 0; i < 4; i++) io.out[i] = out[i];
   // PFP traffic: 9 (10) live input fields read, 4 written
   KScope ks(c, K_PRODUCTS, 16.0 * g.nr * g.ntchopdim * (double)g.nx * ((linear ? 10 : 9) + 4));
-  dim3 grid((g.ndimr + FP - 1) / FP, g.ndimx);
+  if (g.nth == 256 && g.ntchopdim <= w256::MMAX96) {
+    dim3 grid(xcd_grid((g.ndimr + 7) / 8 * g.ndimx));
+    if (linear)
+      hipLaunchKernelGGL(k_fused_products_256<true>, grid, dim3(256), 0, c->stream, g, c->d.r, c->d.tw_t, c->d.tw_h, io,
+                         two_omega, n2);
+    else
+      hipLaunchKernelGGL(k_fused_products_256<false>, grid, dim3(256), 0, c->stream, g, c->d.r, c->d.tw_t, c->d.tw_h,
+                         io, two_omega, n2);
+    MLG_CHECK_HIP(hipGetLastError());
+    return 0;
+  }
+  dim3 grid(xcd_grid((g.ndimr + FP - 1) / FP * g.ndimx));
 #define L_FP(a_, b_)                                                                                                \
   hipLaunchKernelGGL((k_fused_products<a_, b_>), grid, dim3(256), 0, c->stream, g, c->d.r, c->d.tw_t, c->d.tw_h, io, \
                      two_omega, n2, linear)

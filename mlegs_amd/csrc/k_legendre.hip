@@ -4,14 +4,15 @@
 // (src/mod_legops.f90:2476-2529, EOMUL/OEMUL :2548-2648) are, for every
 // azimuthal mode m, real x complex matrix products against the parity-folded
 // table Pbar_n^m(x_i), i < NR/2.  All modes run in one grouped launch
-// (blockIdx.y = m); each mode's K extent follows the triangular truncation
-// NRCHOPS(m).  Complex data enter as interleaved (re, im) real columns, so
+// (every (m, tile) pair is one block, grouped per XCD); each mode's K
+// extent follows the triangular truncation NRCHOPS(m).  Complex data enter as interleaved (re, im) real columns, so
 // one real MFMA product serves both parts.
 //
 // MFMA: v_mfma_f64_16x16x4f64 -- lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; the 16x16 result holds C[l/16 + 4v][l%16] in register v
 // (measured on gfx950; the f32 16x16x4 variant uses C[4(l/16)+v][l%16]).
 #include "mlegs_internal.h"
+#include "xcd.h"
 
 namespace mlg {
 
@@ -19,6 +20,16 @@ typedef double v4d __attribute__((ext_vector_type(4)));
 
 __device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
+}
+
+// LDS tiles are [row][col] with the column index XOR-swizzled by the row:
+// row bit 0 -> column bit 4, row bits 1..3 -> column bits 0, 2, 3, and a
+// parity bit -> column bit 1.  MFMA operand reads (16 consecutive columns
+// of rows 2t and 2t+1) then fall into opposite halves of the banks, and the
+// staging writes (16 consecutive rows of one column pair, or of both
+// parities of one column) into 32 different banks.
+__device__ __forceinline__ int lsw(int r, int par) {
+  return ((r & 1) << 4) | ((r >> 1) & 1) | (((r >> 2) & 1) << 2) | (((r >> 3) & 1) << 3) | (par << 1);
 }
 
 __device__ __forceinline__ int d_nrchops(int nrchop, int m) {
@@ -50,15 +61,17 @@ union RtranBwdLds {
   } e;
 };
 
-__global__ void __launch_bounds__(256) k_rtran_bwd(KGrid g, const double *__restrict__ pf, const double2 *__restrict__ in,
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_rtran_bwd(KGrid g, const double *__restrict__ pf, const double2 *__restrict__ in,
                                                    const double *__restrict__ ln_in, const double *__restrict__ lnx,
                                                    double2 *__restrict__ out) {
-  const int mm = blockIdx.y;
+  const int nk = g.nxchopdim;
+  const int ntiles_c = (2 * nk + BW_TC - 1) / BW_TC, ntiles_i = (g.nrh + BW_TI - 1) / BW_TI;
+  const int lid = xcd_logical(ntiles_c * ntiles_i * g.ntchopdim);
+  if (lid < 0) return;
+  const int mm = lid / (ntiles_c * ntiles_i), rem = lid - mm * (ntiles_c * ntiles_i);
+  const int ti = rem / ntiles_c, tc = rem - ti * ntiles_c;
   const int mv = mm * g.minc;
   const int nn = d_nrchops(g.nrchop, mv);
-  const int nk = g.nxchopdim;
-  const int ntiles_c = (2 * nk + BW_TC - 1) / BW_TC;
-  const int ti = blockIdx.x / ntiles_c, tc = blockIdx.x % ntiles_c;
   const int i0 = ti * BW_TI, c0 = tc * BW_TC;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int wi = wave & 1, wc = wave >> 1;
@@ -98,14 +111,15 @@ __global__ void __launch_bounds__(256) k_rtran_bwd(KGrid g, const double *__rest
     for (int q = 0; q < 8; q++) {
       int idx = tid + 256 * q;
       int i = idx & 63, nl = idx >> 6;
-      L.k.A[st][nl & 1][nl >> 1][i] = ra[q];
+      L.k.A[st][nl & 1][nl >> 1][i ^ lsw(nl >> 1, 0)] = ra[q];
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       int idx = tid + 256 * q;
       int nl = idx & 31, kl = idx >> 5;
-      L.k.B[st][nl & 1][nl >> 1][2 * kl] = rb[q].x;
-      L.k.B[st][nl & 1][nl >> 1][2 * kl + 1] = rb[q].y;
+      const int sw = lsw(nl >> 1, nl & 1);
+      L.k.B[st][nl & 1][nl >> 1][(2 * kl) ^ sw] = rb[q].x;
+      L.k.B[st][nl & 1][nl >> 1][(2 * kl + 1) ^ sw] = rb[q].y;
     }
   };
   if (nch > 0) {
@@ -121,10 +135,11 @@ __global__ void __launch_bounds__(256) k_rtran_bwd(KGrid g, const double *__rest
       const int kr = ks + (lane >> 4);
 #pragma unroll
       for (int p = 0; p < 2; p++) {
-        double a0 = L.k.A[st][p][kr][wi * 32 + (lane & 15)];
-        double a1 = L.k.A[st][p][kr][wi * 32 + 16 + (lane & 15)];
-        double b0 = L.k.B[st][p][kr][wc * 32 + (lane & 15)];
-        double b1 = L.k.B[st][p][kr][wc * 32 + 16 + (lane & 15)];
+        const int sa = lsw(kr, 0), sb = lsw(kr, p);
+        double a0 = L.k.A[st][p][kr][(wi * 32 + (lane & 15)) ^ sa];
+        double a1 = L.k.A[st][p][kr][(wi * 32 + 16 + (lane & 15)) ^ sa];
+        double b0 = L.k.B[st][p][kr][(wc * 32 + (lane & 15)) ^ sb];
+        double b1 = L.k.B[st][p][kr][(wc * 32 + 16 + (lane & 15)) ^ sb];
         acc[p][0][0] = mfma16x16x4(a0, b0, acc[p][0][0]);
         acc[p][0][1] = mfma16x16x4(a0, b1, acc[p][0][1]);
         acc[p][1][0] = mfma16x16x4(a1, b0, acc[p][1][0]);
@@ -168,7 +183,7 @@ __global__ void __launch_bounds__(256) k_rtran_bwd(KGrid g, const double *__rest
 int launch_rtran_bwd(mlg_ctx *c, const double2 *in, const double *ln_in, double2 *out) {
   KGrid g = kgrid(c);
   int nti = (g.nrh + BW_TI - 1) / BW_TI, ntc = (2 * g.nxchopdim + BW_TC - 1) / BW_TC;
-  dim3 grid(nti * ntc, g.ntchopdim);
+  dim3 grid(xcd_grid(nti * ntc * g.ntchopdim));
   // 2 NR sum_m NRCHOPS(m) NXCHOPDIM flops (real table x complex data, folded)
   KScope ks(c, K_RTRAN_BWD, 2.0 * g.nr * (double)sum_nrchops(c) * g.nxchopdim);
   hipLaunchKernelGGL(k_rtran_bwd, grid, dim3(256), 0, c->stream, g, c->d.pf, in, ln_in, c->d.lnx, out);
@@ -189,18 +204,20 @@ int launch_rtran_bwd(mlg_ctx *c, const double2 *in, const double *ln_in, double2
 
 __global__ void __launch_bounds__(256) k_rtran_fwd(KGrid g, const double *__restrict__ pf, const double *__restrict__ w,
                                                    const double2 *__restrict__ in, double2 *__restrict__ out) {
-  const int mm = blockIdx.y;
+  const int nk = g.nxchopdim;
+  const int ntc = (2 * nk + FW_TC - 1) / FW_TC, ntn = (g.nrchopdim + FW_TN - 1) / FW_TN;
+  const int lid = xcd_logical(ntc * ntn * g.ntchopdim);
+  if (lid < 0) return;
+  const int mm = lid / (ntc * ntn), rem = lid - mm * (ntc * ntn);
+  const int tn = rem / ntc, tc = rem - tn * ntc;
   const int mv = mm * g.minc;
   const int nn = d_nrchops(g.nrchop, mv);
-  const int nk = g.nxchopdim;
-  const int ntc = (2 * nk + FW_TC - 1) / FW_TC;
-  const int tn = blockIdx.x / ntc, tc = blockIdx.x % ntc;
   const int n0 = tn * FW_TN, c0 = tc * FW_TC;
   if (n0 >= nn) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int par = wave & 1, ch = wave >> 1;
 
-  __shared__ double sA[2][FW_IC][16];      // [parity][i][n-local]
+  __shared__ double sA[2][FW_IC][17];      // [parity][i][n-local] (padded row)
   __shared__ double sB[2][FW_IC][FW_TC];   // [parity][i][col]
   v4d acc[2] = {(v4d){0, 0, 0, 0}, (v4d){0, 0, 0, 0}};
   const double *pfm = pf + (size_t)mm * g.npf * g.nrh;
@@ -227,18 +244,20 @@ __global__ void __launch_bounds__(256) k_rtran_fwd(KGrid g, const double *__rest
         e = make_double2((p.x + r.x) * wi, (p.y + r.y) * wi);
         o = make_double2((p.x - r.x) * wi, (p.y - r.y) * wi);
       }
-      sB[0][il][2 * kl] = e.x;
-      sB[0][il][2 * kl + 1] = e.y;
-      sB[1][il][2 * kl] = o.x;
-      sB[1][il][2 * kl + 1] = o.y;
+      const int sw = lsw(il, 0);
+      sB[0][il][(2 * kl) ^ sw] = e.x;
+      sB[0][il][(2 * kl + 1) ^ sw] = e.y;
+      sB[1][il][(2 * kl) ^ sw] = o.x;
+      sB[1][il][(2 * kl + 1) ^ sw] = o.y;
     }
     __syncthreads();
 #pragma unroll
     for (int ks = 0; ks < FW_IC; ks += 4) {
       const int kr = ks + (lane >> 4);
       double a = sA[par][kr][lane & 15];
-      double b0 = sB[par][kr][ch * 32 + (lane & 15)];
-      double b1 = sB[par][kr][ch * 32 + 16 + (lane & 15)];
+      const int sw = lsw(kr, 0);
+      double b0 = sB[par][kr][(ch * 32 + (lane & 15)) ^ sw];
+      double b1 = sB[par][kr][(ch * 32 + 16 + (lane & 15)) ^ sw];
       acc[0] = mfma16x16x4(a, b0, acc[0]);
       acc[1] = mfma16x16x4(a, b1, acc[1]);
     }
@@ -267,7 +286,7 @@ __global__ void __launch_bounds__(256) k_rtran_fwd(KGrid g, const double *__rest
 int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out) {
   KGrid g = kgrid(c);
   int ntn = (g.nrchopdim + FW_TN - 1) / FW_TN, ntc = (2 * g.nxchopdim + FW_TC - 1) / FW_TC;
-  dim3 grid(ntn * ntc, g.ntchopdim);
+  dim3 grid(xcd_grid(ntn * ntc * g.ntchopdim));
   KScope ks(c, K_RTRAN_FWD, 2.0 * g.nr * (double)sum_nrchops(c) * g.nxchopdim);
   hipLaunchKernelGGL(k_rtran_fwd, grid, dim3(256), 0, c->stream, g, c->d.pf, c->d.w, in, out);
   MLG_CHECK_HIP(hipGetLastError());
@@ -306,12 +325,14 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
                                                  const double2 *__restrict__ rur, const double2 *__restrict__ rup,
                                                  const double2 *__restrict__ uz, double2 *__restrict__ psi,
                                                  double2 *__restrict__ chi) {
-  const int mm = blockIdx.y;
+  const int nk = g.nxchopdim;
+  const int ntc = (2 * nk + PJ_TC - 1) / PJ_TC, ntn = (g.nrchopdim + PJ_TN - 1) / PJ_TN;
+  const int lid = xcd_logical(ntc * ntn * g.ntchopdim);
+  if (lid < 0) return;
+  const int mm = lid / (ntc * ntn), rem = lid - mm * (ntc * ntn);
+  const int tn = rem / ntc, tc = rem - tn * ntc;
   const int mv = mm * g.minc;
   const int nn = d_nrchops(g.nrchop, mv);
-  const int nk = g.nxchopdim;
-  const int ntc = (2 * nk + PJ_TC - 1) / PJ_TC;
-  const int tn = blockIdx.x / ntc, tc = blockIdx.x % ntc;
   const int n0 = tn * PJ_TN, c0 = tc * PJ_TC;
   if (n0 >= nn) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -351,17 +372,18 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     for (int q = 0; q < 12; q++) {
       int idx = tid + 256 * q;
       int il = idx & 15, nl = (idx >> 4) & 63, t = idx >> 10;
-      L.k.T[t][nl & 1][il][nl >> 1] = rt[q];
+      L.k.T[t][nl & 1][il][(nl >> 1) ^ lsw(il, nl & 1)] = rt[q];
     }
 #pragma unroll
     for (int q = 0; q < 3; q++) {
       int idx = tid + 256 * q;
       int il = idx & 15, kl = (idx >> 4) & 15, f = idx >> 8;
       double2 p = rp[q], r = rq[q];
-      L.k.F[f][0][il][2 * kl] = p.x + r.x;
-      L.k.F[f][0][il][2 * kl + 1] = p.y + r.y;
-      L.k.F[f][1][il][2 * kl] = p.x - r.x;
-      L.k.F[f][1][il][2 * kl + 1] = p.y - r.y;
+      const int sw = lsw(il, 0);
+      L.k.F[f][0][il][(2 * kl) ^ sw] = p.x + r.x;
+      L.k.F[f][0][il][(2 * kl + 1) ^ sw] = p.y + r.y;
+      L.k.F[f][1][il][(2 * kl) ^ sw] = p.x - r.x;
+      L.k.F[f][1][il][(2 * kl + 1) ^ sw] = p.y - r.y;
     }
   };
   const int eo = par, oe = par ^ 1;
@@ -373,13 +395,13 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     if (ib + PJ_IC < g.nrh) fetch(ib + PJ_IC);
 #pragma unroll
     for (int ks = 0; ks < PJ_IC; ks += 4) {
-      const int kr = ks + (lane >> 4), cl = ch * 16 + (lane & 15);
+      const int kr = ks + (lane >> 4), cl = (ch * 16 + (lane & 15)) ^ lsw(kr, 0), st = lsw(kr, par);
       const double f0e = L.k.F[0][eo][kr][cl], f0o = L.k.F[0][oe][kr][cl];
       const double f1e = L.k.F[1][eo][kr][cl], f1o = L.k.F[1][oe][kr][cl];
       const double f2e = L.k.F[2][eo][kr][cl];
 #pragma unroll
       for (int a = 0; a < 2; a++) {
-        const int rl = 16 * a + (lane & 15);
+        const int rl = (16 * a + (lane & 15)) ^ st;
         const double av = L.k.T[0][par][kr][rl], ad = L.k.T[1][par][kr][rl], at = L.k.T[2][par][kr][rl];
         acc[a][0] = mfma16x16x4(av, f0e, acc[a][0]);
         acc[a][1] = mfma16x16x4(ad, f1o, acc[a][1]);
@@ -436,7 +458,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *psi, double2 *chi) {
   KGrid g = kgrid(c);
   int ntn = (g.nrchopdim + PJ_TN - 1) / PJ_TN, ntc = (2 * g.nxchopdim + PJ_TC - 1) / PJ_TC;
-  dim3 grid(ntn * ntc, g.ntchopdim);
+  dim3 grid(xcd_grid(ntn * ntc * g.ntchopdim));
   KScope ks(c, K_PROJECT, 5 * 2.0 * g.nr * (double)sum_nrchops(c) * g.nxchopdim);  // five products
   hipLaunchKernelGGL(k_project, grid, dim3(256), 0, c->stream, g, c->d.vdt, c->d.ak, rur, rup, uz, psi, chi);
   MLG_CHECK_HIP(hipGetLastError());

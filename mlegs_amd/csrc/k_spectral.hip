@@ -613,6 +613,7 @@ template <int NE, int C0, int SB>
 __device__ void band_apply(int nd, double2 *x, const double *__restrict__ F, long ncls, long cls) {
   constexpr int KU = NE - 1 - C0;
   double2 xh[C0];
+#pragma unroll
   for (int q = 0; q < C0; q++) xh[q] = zero2();
   for (int nb = 0; nb < nd; nb += SB) {
     double2 rhs[SB];
@@ -648,6 +649,7 @@ __device__ void band_apply(int nd, double2 *x, const double *__restrict__ F, lon
       if (nb + q < nd) x[nb + q] = rhs[q];
   }
   double2 yh[KU];
+#pragma unroll
   for (int u = 0; u < KU; u++) yh[u] = zero2();
   for (int nt = nd - 1; nt >= 0; nt -= SB) {
     double2 v0[SB];
@@ -839,21 +841,36 @@ __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, 
 }
 
 // One launch: blocks [0, npenta) take the penta-diagonal systems, the
-// remaining blocks the m = 0 systems, so both sets run side by side.
-__global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, int op, double alp, int has_ln, double lnval,
-                                                   double2 *a, double *ln_a, const double *ln_b,
-                                                   const double *__restrict__ F, int npenta) {
+// remaining blocks the m = 0 systems, so both sets run side by side; grid
+// row y solves field y of a batch of independent solves.
+struct BandJob {
+  double2 *a;
+  double *ln_a;
+  const double *ln_b;
+  const double *F;
+  double alp;
+};
+constexpr int MAX_BAND_JOBS = 3;
+struct BandJobs {
+  BandJob j[MAX_BAND_JOBS];
+};
+
+__global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, int op, int has_ln, double lnval, BandJobs jobs,
+                                                   int npenta) {
+  const BandJob &J = jobs.j[blockIdx.y];
   if ((int)blockIdx.x < npenta) {
-    apply_penta(g, op, has_ln, a, ln_a, F, blockIdx.x * (long)blockDim.x + threadIdx.x);
+    apply_penta(g, op, has_ln, J.a, J.ln_a, J.F, blockIdx.x * (long)blockDim.x + threadIdx.x);
   } else {
-    apply_m0(g, bxm, op, alp, has_ln, lnval, a, ln_a, ln_b, F, (blockIdx.x - npenta) * blockDim.x + threadIdx.x);
+    apply_m0(g, bxm, op, J.alp, has_ln, lnval, J.a, J.ln_a, J.ln_b, J.F, (blockIdx.x - npenta) * blockDim.x + threadIdx.x);
   }
 }
 
 // Factor sets are cached per (operator, alp, truncation): the matrices
 // depend on m, k^2 and those only, so repeated solves (every CN step) only
 // substitute.  Results are bitwise those of factorising on every call.
-static double *factor_set(mlg_ctx *c, int op, double alp, int *rc) {
+// At most eight sets are kept; a miss evicts the oldest set that the
+// current batch does not use.
+static double *factor_set(mlg_ctx *c, int op, double alp, const double *const *keep, int nkeep, int *rc) {
   KGrid g = kgrid(c);
   double key_alp = (op == 2) ? alp : 0.0;
   for (auto &f : c->fcache)
@@ -861,9 +878,15 @@ static double *factor_set(mlg_ctx *c, int op, double alp, int *rc) {
   const long ncls = (long)g.ntchopdim * g.nxchop;
   size_t n = (size_t)g.nrchopdim * FSTRIDE * ncls;
   if (c->fcache.size() >= 8) {
-    (void)hipStreamSynchronize(c->stream);
-    (void)hipFree(c->fcache.front().f);
-    c->fcache.erase(c->fcache.begin());
+    for (auto it = c->fcache.begin(); it != c->fcache.end(); ++it) {
+      bool used = false;
+      for (int i = 0; i < nkeep; i++) used = used || keep[i] == it->f;
+      if (used) continue;
+      (void)hipStreamSynchronize(c->stream);
+      (void)hipFree(it->f);
+      c->fcache.erase(it);
+      break;
+    }
   }
   double *F = nullptr;
   if (hipMalloc((void **)&F, sizeof(double) * n) != hipSuccess) {
@@ -881,18 +904,29 @@ static double *factor_set(mlg_ctx *c, int op, double alp, int *rc) {
   return F;
 }
 
-int launch_band_solve(mlg_ctx *c, int op, double2 *a, double *ln_a, const double *ln_b, double alp, int has_ln, double ln) {
+int launch_band_solve_n(mlg_ctx *c, int op, int n, double2 *const *a, double *const *ln_a, const double *const *ln_b,
+                        const double *alp, int has_ln, double ln) {
+  if (n < 1 || n > MAX_BAND_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
   long nsys = (long)g.ntchopdim * g.nxchopdim;
-  int rc = 0;
-  const double *F = factor_set(c, op, alp, &rc);
-  if (!F) return rc;
-  KScope ks(c, K_BAND_SOLVE, 2.0 * 16.0 * fff_count(g));  // the coefficients read and written once
+  BandJobs jobs;
+  const double *used[MAX_BAND_JOBS];
+  for (int i = 0; i < n; i++) {
+    int rc = 0;
+    const double *F = factor_set(c, op, alp[i], used, i, &rc);
+    if (!F) return rc;
+    used[i] = F;
+    jobs.j[i] = BandJob{a[i], ln_a[i], ln_b[i], F, alp[i]};
+  }
+  KScope ks(c, K_BAND_SOLVE, n * 2.0 * 16.0 * fff_count(g));  // the coefficients read and written once
   const int npenta = (int)((nsys + 63) / 64), nm0 = (op != 1) ? (g.nxchopdim + 63) / 64 : 0;
-  hipLaunchKernelGGL(k_band_apply, dim3(npenta + nm0), dim3(64), 0, c->stream, g, c->d.bxm, op, alp, has_ln, ln, a, ln_a,
-                     ln_b, F, npenta);
+  hipLaunchKernelGGL(k_band_apply, dim3(npenta + nm0, n), dim3(64), 0, c->stream, g, c->d.bxm, op, has_ln, ln, jobs, npenta);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
+}
+
+int launch_band_solve(mlg_ctx *c, int op, double2 *a, double *ln_a, const double *ln_b, double alp, int has_ln, double ln) {
+  return launch_band_solve_n(c, op, 1, &a, &ln_a, &ln_b, &alp, has_ln, ln);
 }
 
 // ---------------------------------------------------------------- pointwise updates

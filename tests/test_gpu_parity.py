@@ -23,6 +23,9 @@ pytestmark = pytest.mark.gpu
 
 SMALL = dict(CFG1, nr=32, nth=16, nx=16, nrchop=22, ntchop=6, nxchop=5)
 SMALL_ETD = dict(SMALL, adams=0, hyper_cn=1, nup=1e-3, kappap=1e-3, visc_p=4, omega=0.8, bv0=1.3)
+# NTH = 256 with the production theta truncation: the theta transforms and the
+# fused products take their NTH = 256 kernels here
+THETA256 = dict(CFG1, nr=16, nth=256, nx=8, nrchop=12, ntchop=86, nxchop=3)
 TOL = 1e-12
 
 
@@ -80,9 +83,9 @@ class Pair:
         return np.asfortranarray(z)
 
 
-@pytest.fixture(scope="module", params=["small", "cfg1"])
+@pytest.fixture(scope="module", params=["small", "cfg1", "theta256"])
 def P(request):
-    p = Pair(SMALL if request.param == "small" else CFG1)
+    p = Pair({"small": SMALL, "cfg1": CFG1, "theta256": THETA256}[request.param])
     yield p
     p.close()
 
