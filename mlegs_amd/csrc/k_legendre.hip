@@ -48,22 +48,21 @@ __device__ __forceinline__ int d_nrchops(int nrchop, int m) {
 #define BW_TC 64
 #define BW_JC 16
 
-// LDS: two K-chunk stages of the table and data tiles; the epilogue's S/D
-// tiles reuse the same memory after the K loop.
+// LDS: one K-chunk stage of the table and data tiles (the next chunk waits
+// in registers); after the K loop the same memory stages the output, one
+// half (S = BE+BO, then D = BE-BO) at a time.  33 KB per block: four blocks
+// per CU.
 union RtranBwdLds {
   struct {
-    double A[2][2][BW_JC][BW_TI];  // [stage][parity][j][i]
-    double B[2][2][BW_JC][BW_TC];  // [stage][parity][j][col]
+    double A[2][BW_JC][BW_TI];  // [parity][j][i]
+    double B[2][BW_JC][BW_TC];  // [parity][j][col]
   } k;
-  struct {
-    double S[BW_TC][BW_TI + 1];
-    double D[BW_TC][BW_TI + 1];
-  } e;
+  double E[BW_TC][BW_TI + 1];  // [col][i]
 };
 
-__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))) k_rtran_bwd(KGrid g, const double *__restrict__ pf, const double2 *__restrict__ in,
-                                                   const double *__restrict__ ln_in, const double *__restrict__ lnx,
-                                                   double2 *__restrict__ out) {
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_rtran_bwd(KGrid g, const double *__restrict__ pf, const double2 *__restrict__ in, const double *__restrict__ ln_in,
+            const double *__restrict__ lnx, double2 *__restrict__ out) {
   const int nk = g.nxchopdim;
   const int ntiles_c = (2 * nk + BW_TC - 1) / BW_TC, ntiles_i = (g.nrh + BW_TI - 1) / BW_TI;
   const int lid = xcd_logical(ntiles_c * ntiles_i * g.ntchopdim);
@@ -78,8 +77,11 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   __shared__ RtranBwdLds L;
 
   v4d acc[2][2][2];  // [parity][sub i][sub c]
+#pragma unroll
   for (int p = 0; p < 2; p++)
+#pragma unroll
     for (int a = 0; a < 2; a++)
+#pragma unroll
       for (int b = 0; b < 2; b++) acc[p][a][b] = (v4d){0, 0, 0, 0};
 
   const double *pfm = pf + (size_t)mm * g.npf * g.nrh;
@@ -88,47 +90,45 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   // per-thread prefetch registers: 8 table values, 4 complex data values
   double ra[8];
   double2 rb[4];
+  // 32-bit element offsets: lane part fixed, chunk and q parts uniform
+  const int ta_i = i0 + (tid & 63), ta_n = tid >> 6;          // table: i, n - 2 j0 - 4 q
+  const int tb_n = tid & 31, tb_k = (c0 >> 1) + (tid >> 5);   // data: n - 2 j0, k - 8 q
+  const int kstride = g.nrchopdim * g.ntchopdim;
+  const double2 *inm = in + (size_t)g.nrchopdim * mm;
   auto fetch = [&](int ch) {
     const int j0 = ch * BW_JC;
 #pragma unroll
     for (int q = 0; q < 8; q++) {
-      int idx = tid + 256 * q;
-      int i = idx & 63, nl = idx >> 6;
-      int n = 2 * j0 + nl, ig = i0 + i;
-      ra[q] = (n < nn && ig < g.nrh) ? pfm[(size_t)n * g.nrh + ig] : 0.0;
+      const int n = 2 * j0 + ta_n + 4 * q;
+      ra[q] = (n < nn && ta_i < g.nrh) ? pfm[n * g.nrh + ta_i] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
-      int idx = tid + 256 * q;
-      int nl = idx & 31, kl = idx >> 5;
-      int n = 2 * j0 + nl, k = (c0 >> 1) + kl;
-      rb[q] = (n < nn && k < nk) ? in[(size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k)]
-                                 : make_double2(0.0, 0.0);
+      const int n = 2 * j0 + tb_n, k = tb_k + 8 * q;
+      rb[q] = (n < nn && k < nk) ? inm[n + kstride * k] : make_double2(0.0, 0.0);
     }
   };
-  auto stash = [&](int st) {
+  auto stash = [&]() {
 #pragma unroll
     for (int q = 0; q < 8; q++) {
       int idx = tid + 256 * q;
       int i = idx & 63, nl = idx >> 6;
-      L.k.A[st][nl & 1][nl >> 1][i ^ lsw(nl >> 1, 0)] = ra[q];
+      L.k.A[nl & 1][nl >> 1][i ^ lsw(nl >> 1, 0)] = ra[q];
     }
 #pragma unroll
     for (int q = 0; q < 4; q++) {
       int idx = tid + 256 * q;
       int nl = idx & 31, kl = idx >> 5;
       const int sw = lsw(nl >> 1, nl & 1);
-      L.k.B[st][nl & 1][nl >> 1][(2 * kl) ^ sw] = rb[q].x;
-      L.k.B[st][nl & 1][nl >> 1][(2 * kl + 1) ^ sw] = rb[q].y;
+      L.k.B[nl & 1][nl >> 1][(2 * kl) ^ sw] = rb[q].x;
+      L.k.B[nl & 1][nl >> 1][(2 * kl + 1) ^ sw] = rb[q].y;
     }
   };
-  if (nch > 0) {
-    fetch(0);
-    stash(0);
-  }
-  __syncthreads();
+  if (nch > 0) fetch(0);
   for (int ch = 0; ch < nch; ch++) {
-    const int st = ch & 1;
+    __syncthreads();
+    stash();
+    __syncthreads();
     if (ch + 1 < nch) fetch(ch + 1);
 #pragma unroll
     for (int ks = 0; ks < BW_JC; ks += 4) {
@@ -136,46 +136,46 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
 #pragma unroll
       for (int p = 0; p < 2; p++) {
         const int sa = lsw(kr, 0), sb = lsw(kr, p);
-        double a0 = L.k.A[st][p][kr][(wi * 32 + (lane & 15)) ^ sa];
-        double a1 = L.k.A[st][p][kr][(wi * 32 + 16 + (lane & 15)) ^ sa];
-        double b0 = L.k.B[st][p][kr][(wc * 32 + (lane & 15)) ^ sb];
-        double b1 = L.k.B[st][p][kr][(wc * 32 + 16 + (lane & 15)) ^ sb];
+        double a0 = L.k.A[p][kr][(wi * 32 + (lane & 15)) ^ sa];
+        double a1 = L.k.A[p][kr][(wi * 32 + 16 + (lane & 15)) ^ sa];
+        double b0 = L.k.B[p][kr][(wc * 32 + (lane & 15)) ^ sb];
+        double b1 = L.k.B[p][kr][(wc * 32 + 16 + (lane & 15)) ^ sb];
         acc[p][0][0] = mfma16x16x4(a0, b0, acc[p][0][0]);
         acc[p][0][1] = mfma16x16x4(a0, b1, acc[p][0][1]);
         acc[p][1][0] = mfma16x16x4(a1, b0, acc[p][1][0]);
         acc[p][1][1] = mfma16x16x4(a1, b1, acc[p][1][1]);
       }
     }
-    if (ch + 1 < nch) stash(st ^ 1);
-    __syncthreads();
   }
-  // epilogue through LDS: S = BE+BO, D = BE-BO as [col][i]
-  for (int a = 0; a < 2; a++)
-    for (int b = 0; b < 2; b++)
-      for (int v = 0; v < 4; v++) {
-        int il = wi * 32 + a * 16 + (lane >> 4) + 4 * v;
-        int cl = wc * 32 + b * 16 + (lane & 15);
-        double e = acc[0][a][b][v], o = acc[1][a][b][v];
-        L.e.S[cl][il] = e + o;
-        L.e.D[cl][il] = e - o;
-      }
-  __syncthreads();
+  // epilogue through LDS as [col][i]: S = BE+BO to f(x_i), then D = BE-BO
+  // to f(x_{NR-1-i})
   const double lnv = (mm == 0) ? ln_in[0] : 0.0;
-  for (int q = 0; q < 8; q++) {
-    int idx = tid + 256 * q;  // 0..2047: i fastest
-    int il = idx & 63, kl = idx >> 6;
-    int ig = i0 + il, k = (c0 >> 1) + kl;
-    if (ig < g.nrh && k < nk) {
-      size_t col = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
-      double2 s = make_double2(L.e.S[2 * kl][il], L.e.S[2 * kl + 1][il]);
-      double2 d = make_double2(L.e.D[2 * kl][il], L.e.D[2 * kl + 1][il]);
-      int ir = g.nr - 1 - ig;
-      if (mm == 0 && k == 0 && lnv != 0.0) {
-        s.x = s.x + lnv * lnx[ig];
-        d.x = d.x + lnv * lnx[ir];
+#pragma unroll
+  for (int h = 0; h < 2; h++) {
+    __syncthreads();
+#pragma unroll
+    for (int a = 0; a < 2; a++)
+#pragma unroll
+      for (int b = 0; b < 2; b++)
+#pragma unroll
+        for (int v = 0; v < 4; v++) {
+          int il = wi * 32 + a * 16 + (lane >> 4) + 4 * v;
+          int cl = wc * 32 + b * 16 + (lane & 15);
+          double e = acc[0][a][b][v], o = acc[1][a][b][v];
+          L.E[cl][il] = h == 0 ? e + o : e - o;
+        }
+    __syncthreads();
+    for (int q = 0; q < 8; q++) {
+      int idx = tid + 256 * q;  // 0..2047: i fastest
+      int il = idx & 63, kl = idx >> 6;
+      int ig = i0 + il, k = (c0 >> 1) + kl;
+      if (ig < g.nrh && k < nk) {
+        size_t col = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
+        double2 v = make_double2(L.E[2 * kl][il], L.E[2 * kl + 1][il]);
+        const int ir = h == 0 ? ig : g.nr - 1 - ig;
+        if (mm == 0 && k == 0 && lnv != 0.0) v.x = v.x + lnv * lnx[ir];
+        out[col + ir] = v;
       }
-      out[col + ig] = s;
-      out[col + ir] = d;
     }
   }
 }
@@ -195,15 +195,26 @@ int launch_rtran_bwd(mlg_ctx *c, const double2 *in, const double *ln_in, double2
 // Forward: PFF (r,m,k) -> FFF (n,m,k).
 // a_n = sum_i PF(i,n) F_p(i,c), F_e = (f(x_i)+f(x_{NR-1-i})) w_i,
 // F_o = (f(x_i)-f(x_{NR-1-i})) w_i, p = parity of n.
-// Block tile: 32 rows n (16 even + 16 odd) x 64 real columns; wave w:
-// parity w&1, column half w>>1 (2 MFMA tiles).  K chunk: 16 nodes.
+// Block tile: 64 rows n (32 even + 32 odd) x 64 real columns; wave w:
+// parity w&1, column half w>>1 (2 x 2 MFMA tiles).  K chunk: 16 nodes,
+// staged in LDS while the next chunk is fetched into registers.  Rows n in
+// [NRCHOPS(m), NRCHOPDIM) of the output are left to the CHOPDO that follows.
 // -------------------------------------------------------------------------
-#define FW_TN 32
+#define FW_TN 64
 #define FW_TC 64
 #define FW_IC 16
 
-__global__ void __launch_bounds__(256) k_rtran_fwd(KGrid g, const double *__restrict__ pf, const double *__restrict__ w,
-                                                   const double2 *__restrict__ in, double2 *__restrict__ out) {
+union RtranFwdLds {
+  struct {
+    double A[2][FW_IC][FW_TN / 2];  // [parity][i][n-local within parity]
+    double B[2][FW_IC][FW_TC];      // [fold parity][i][col]
+  } k;
+  double C[FW_TC][FW_TN + 1];  // [col][n-local]
+};
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_rtran_fwd(KGrid g, const double *__restrict__ pf, const double *__restrict__ w, const double2 *__restrict__ in,
+            double2 *__restrict__ out) {
   const int nk = g.nxchopdim;
   const int ntc = (2 * nk + FW_TC - 1) / FW_TC, ntn = (g.nrchopdim + FW_TN - 1) / FW_TN;
   const int lid = xcd_logical(ntc * ntn * g.ntchopdim);
@@ -216,68 +227,90 @@ __global__ void __launch_bounds__(256) k_rtran_fwd(KGrid g, const double *__rest
   if (n0 >= nn) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
   const int par = wave & 1, ch = wave >> 1;
+  __shared__ RtranFwdLds L;
 
-  __shared__ double sA[2][FW_IC][17];      // [parity][i][n-local] (padded row)
-  __shared__ double sB[2][FW_IC][FW_TC];   // [parity][i][col]
-  v4d acc[2] = {(v4d){0, 0, 0, 0}, (v4d){0, 0, 0, 0}};
+  v4d acc[2][2] = {{(v4d){0, 0, 0, 0}, (v4d){0, 0, 0, 0}}, {(v4d){0, 0, 0, 0}, (v4d){0, 0, 0, 0}}};
   const double *pfm = pf + (size_t)mm * g.npf * g.nrh;
-  for (int ib = 0; ib < g.nrh; ib += FW_IC) {
-    // table tile: 32 n x 16 i
-    for (int q = 0; q < 2; q++) {
-      int idx = tid + 256 * q;  // 0..511
-      int il = idx & 15, nl = idx >> 4;  // nl 0..31
-      int n = n0 + nl, i = ib + il;
-      double v = 0.0;
-      if (n < nn && i < g.nrh) v = pfm[(size_t)n * g.nrh + i];
-      sA[nl & 1][il][nl >> 1] = v;
+  const double2 *inm = in + (size_t)g.ndimr * mm;
+  const int kstride = g.ndimr * g.ntchopdim;
+  // lane parts of the fetch indices (table: i fastest; data: i fastest)
+  const int il = tid & 15, nl0 = tid >> 4, kl0 = tid >> 4;
+  double ra[4];
+  double2 rp[2], rq[2];
+  double rw = 0.0;
+  auto fetch = [&](int ib) {
+    const int i = ib + il;
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int n = n0 + nl0 + 16 * q;
+      ra[q] = (n < nn && i < g.nrh) ? pfm[n * g.nrh + i] : 0.0;
     }
-    // folded data: 16 nodes x 32 complex k
+#pragma unroll
     for (int q = 0; q < 2; q++) {
-      int idx = tid + 256 * q;  // 0..511
-      int il = idx & 15, kl = idx >> 4;
-      int i = ib + il, k = (c0 >> 1) + kl;
-      double2 e = make_double2(0.0, 0.0), o = make_double2(0.0, 0.0);
+      const int k = (c0 >> 1) + kl0 + 16 * q;
+      rp[q] = rq[q] = make_double2(0.0, 0.0);
       if (i < g.nrh && k < nk) {
-        size_t col = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
-        double2 p = in[col + i], r = in[col + (g.nr - 1 - i)];
-        double wi = w[i];
-        e = make_double2((p.x + r.x) * wi, (p.y + r.y) * wi);
-        o = make_double2((p.x - r.x) * wi, (p.y - r.y) * wi);
+        rp[q] = inm[kstride * k + i];
+        rq[q] = inm[kstride * k + (g.nr - 1 - i)];
       }
-      const int sw = lsw(il, 0);
-      sB[0][il][(2 * kl) ^ sw] = e.x;
-      sB[0][il][(2 * kl + 1) ^ sw] = e.y;
-      sB[1][il][(2 * kl) ^ sw] = o.x;
-      sB[1][il][(2 * kl + 1) ^ sw] = o.y;
     }
+    rw = (i < g.nrh) ? w[i] : 0.0;
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      const int nl = nl0 + 16 * q;
+      L.k.A[nl & 1][il][(nl >> 1) ^ lsw(il, nl & 1)] = ra[q];
+    }
+    const int sw = lsw(il, 0);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int kl = kl0 + 16 * q;
+      const double2 p = rp[q], r = rq[q];
+      L.k.B[0][il][(2 * kl) ^ sw] = (p.x + r.x) * rw;
+      L.k.B[0][il][(2 * kl + 1) ^ sw] = (p.y + r.y) * rw;
+      L.k.B[1][il][(2 * kl) ^ sw] = (p.x - r.x) * rw;
+      L.k.B[1][il][(2 * kl + 1) ^ sw] = (p.y - r.y) * rw;
+    }
+  };
+  fetch(0);
+  for (int ib = 0; ib < g.nrh; ib += FW_IC) {
     __syncthreads();
+    stash();
+    __syncthreads();
+    if (ib + FW_IC < g.nrh) fetch(ib + FW_IC);
 #pragma unroll
     for (int ks = 0; ks < FW_IC; ks += 4) {
-      const int kr = ks + (lane >> 4);
-      double a = sA[par][kr][lane & 15];
-      const int sw = lsw(kr, 0);
-      double b0 = sB[par][kr][(ch * 32 + (lane & 15)) ^ sw];
-      double b1 = sB[par][kr][(ch * 32 + 16 + (lane & 15)) ^ sw];
-      acc[0] = mfma16x16x4(a, b0, acc[0]);
-      acc[1] = mfma16x16x4(a, b1, acc[1]);
+      const int kr = ks + (lane >> 4), sw = lsw(kr, 0), swa = lsw(kr, par);
+      const double a0 = L.k.A[par][kr][(lane & 15) ^ swa];
+      const double a1 = L.k.A[par][kr][(16 + (lane & 15)) ^ swa];
+      const double b0 = L.k.B[par][kr][(ch * 32 + (lane & 15)) ^ sw];
+      const double b1 = L.k.B[par][kr][(ch * 32 + 16 + (lane & 15)) ^ sw];
+      acc[0][0] = mfma16x16x4(a0, b0, acc[0][0]);
+      acc[0][1] = mfma16x16x4(a0, b1, acc[0][1]);
+      acc[1][0] = mfma16x16x4(a1, b0, acc[1][0]);
+      acc[1][1] = mfma16x16x4(a1, b1, acc[1][1]);
     }
-    __syncthreads();
   }
-  __shared__ double sC[FW_TC][FW_TN + 1];  // [col][n-local]
-  for (int b = 0; b < 2; b++)
-    for (int v = 0; v < 4; v++) {
-      int rl = (lane >> 4) + 4 * v;         // row within the parity tile
-      int nl = 2 * rl + par;                // n-local
-      int cl = ch * 32 + b * 16 + (lane & 15);
-      sC[cl][nl] = acc[b][v];
-    }
   __syncthreads();
-  for (int q = 0; q < 4; q++) {
-    int idx = tid + 256 * q;  // 0..1023, n fastest
-    int nl = idx & 31, kl = idx >> 5;
-    int n = n0 + nl, k = (c0 >> 1) + kl;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int b = 0; b < 2; b++)
+#pragma unroll
+      for (int v = 0; v < 4; v++) {
+        const int rl = 16 * a + (lane >> 4) + 4 * v;  // row within the parity tile
+        const int nl = 2 * rl + par;                    // n-local
+        const int cl = ch * 32 + b * 16 + (lane & 15);
+        L.C[cl][nl] = acc[a][b][v];
+      }
+  __syncthreads();
+  for (int q = 0; q < 8; q++) {
+    const int idx = tid + 256 * q;  // 0..2047, n fastest
+    const int nl = idx & 63, kl = idx >> 6;
+    const int n = n0 + nl, k = (c0 >> 1) + kl;
     if (n < g.nrchopdim && k < nk) {
-      double2 v = (n < nn) ? make_double2(sC[2 * kl][nl], sC[2 * kl + 1][nl]) : make_double2(0.0, 0.0);
+      double2 v = (n < nn) ? make_double2(L.C[2 * kl][nl], L.C[2 * kl + 1][nl]) : make_double2(0.0, 0.0);
       out[(size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k)] = v;
     }
   }
@@ -346,13 +379,17 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   const double2 *src[3] = {rur, rup, uz};
   double rt[12];
   double2 rp[3], rq[3];
+  // 32-bit element offsets from per-m bases
+  const double *vdm = vdt + (size_t)mm * g.nrchopdim * g.nrh;
+  const int its = (int)tstride, kstride = g.ndimr * g.ntchopdim;
+  const size_t moff = (size_t)g.ndimr * mm;
   auto fetch = [&](int ib) {
 #pragma unroll
     for (int q = 0; q < 12; q++) {
       int idx = tid + 256 * q;  // [table 3][n 64][i 16]
       int il = idx & 15, nl = (idx >> 4) & 63, t = idx >> 10;
       int n = n0 + nl, i = ib + il;
-      rt[q] = (n < nn && i < g.nrh) ? vdt[(size_t)t * tstride + ((size_t)mm * g.nrchopdim + n) * g.nrh + i] : 0.0;
+      rt[q] = (n < nn && i < g.nrh) ? vdm[t * its + n * g.nrh + i] : 0.0;
     }
 #pragma unroll
     for (int q = 0; q < 3; q++) {
@@ -361,9 +398,9 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       int i = ib + il, k = (c0 >> 1) + kl;
       rp[q] = rq[q] = make_double2(0.0, 0.0);
       if (i < g.nrh && k < nk) {
-        size_t col = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
-        rp[q] = src[f][col + i];
-        rq[q] = src[f][col + (g.nr - 1 - i)];
+        const double2 *col = src[f] + moff;
+        rp[q] = col[kstride * k + i];
+        rq[q] = col[kstride * k + (g.nr - 1 - i)];
       }
     }
   };
