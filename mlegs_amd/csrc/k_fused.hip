@@ -23,7 +23,8 @@
 // 32-byte runs in the PFP reads and writes, which the XCD-aware block order
 // turns into shared L2 lines.
 // Every value goes through the operations of the separate HORFFT and
-// product kernels, in the same order.
+// product kernels, in the same order.  NTH = 256 has its own kernel (below)
+// with the same products and an 8 x 8 x 4 factorisation of the transforms.
 #include "mlegs_internal.h"
 #include "xcd.h"
 
@@ -305,8 +306,43 @@ struct Ctx {
   const double2 *t1, *t2, *th;
   double2 *s;                       // LDS of this line
   int b;                            // lane within the line
-  double2 w8i[4], w8f[4], w4i[2], w4f[2];
 };
+
+// 8- and 4-point DFTs with the roots exp(S 2 pi i k / R) written out:
+// multiplications by 1 and +-i are exact moves, exp(+-i pi/4) uses sqrt(1/2).
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
+__device__ __forceinline__ double2 csub(double2 a, double2 b) { return make_double2(a.x - b.x, a.y - b.y); }
+template <int S>
+__device__ __forceinline__ double2 mul_si(double2 d) {  // (S i) d
+  return S > 0 ? make_double2(-d.y, d.x) : make_double2(d.y, -d.x);
+}
+template <int S>
+__device__ __forceinline__ void dft4s(double2 (&y)[4]) {
+  const double2 a = cadd(y[0], y[2]), b = csub(y[0], y[2]), c = cadd(y[1], y[3]), d = mul_si<S>(csub(y[1], y[3]));
+  y[0] = cadd(a, c);
+  y[1] = cadd(b, d);
+  y[2] = csub(a, c);
+  y[3] = csub(b, d);
+}
+template <int S>
+__device__ __forceinline__ void dft8s(double2 (&x)[8]) {
+  constexpr double h = 0.70710678118654752440;  // sqrt(1/2)
+  double2 e[4] = {x[0], x[2], x[4], x[6]}, o[4] = {x[1], x[3], x[5], x[7]};
+  dft4s<S>(e);
+  dft4s<S>(o);
+  double2 t[4];
+  t[0] = o[0];
+  t[1] = S > 0 ? make_double2(h * (o[1].x - o[1].y), h * (o[1].y + o[1].x))
+               : make_double2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+  t[2] = mul_si<S>(o[2]);
+  t[3] = S > 0 ? make_double2(h * (-o[3].x - o[3].y), h * (o[3].x - o[3].y))
+               : make_double2(h * (o[3].y - o[3].x), h * (-o[3].y - o[3].x));
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    x[k] = cadd(e[k], t[k]);
+    x[k + 4] = csub(e[k], t[k]);
+  }
+}
 
 __device__ __forceinline__ double2 cj(double2 w, bool inv) {
   if (inv) w.y = -w.y;
@@ -340,7 +376,7 @@ __device__ __forceinline__ void inverse(const Ctx &x, const KGrid &g, const doub
     double2 o = fz_cmul(d, fz_conj(x.th[j]));
     v[a] = make_double2(e.x - o.y, e.y + o.x);
   }
-  fz_dft<8>(v, x.w8i);
+  dft8s<1>(v);
   // twiddles W^0 = 1 multiply exactly
 #pragma unroll
   for (int c = 1; c < 8; c++) v[c] = fz_cmul(v[c], cj(x.t1[32 * c + b], true));
@@ -352,7 +388,7 @@ __device__ __forceinline__ void inverse(const Ctx &x, const KGrid &g, const doub
 #pragma unroll
   for (int e = 0; e < 8; e++) v[e] = x.s[e1(c0, 4 * e + f0)];
   wave_sync();
-  fz_dft<8>(v, x.w8i);
+  dft8s<1>(v);
 #pragma unroll
   for (int gg = 1; gg < 8; gg++) v[gg] = fz_cmul(v[gg], cj(x.t2[4 * gg + f0], true));
   // exchange 2: lane 4 c + f holds z_{c,f}[g]; lane 4 c + gh takes f = 0..3 of g = gh, gh + 4
@@ -367,7 +403,7 @@ __device__ __forceinline__ void inverse(const Ctx &x, const KGrid &g, const doub
   wave_sync();
 #pragma unroll
   for (int sg = 0; sg < 2; sg++) {
-    fz_dft<4>(t[sg], x.w4i);
+    dft4s<1>(t[sg]);
 #pragma unroll
     for (int h = 0; h < 4; h++) v[4 * sg + h] = t[sg][h];
   }
@@ -383,7 +419,7 @@ __device__ __forceinline__ void forward(const Ctx &x, const KGrid &g, const doub
   for (int sg = 0; sg < 2; sg++) {
 #pragma unroll
     for (int h = 0; h < 4; h++) t[sg][h] = v[4 * sg + h];
-    fz_dft<4>(t[sg], x.w4f);
+    dft4s<-1>(t[sg]);
     const int gg = f0 + 4 * sg;
 #pragma unroll
     for (int f = 1; f < 4; f++) t[sg][f] = fz_cmul(t[sg][f], x.t2[4 * gg + f]);
@@ -397,7 +433,7 @@ __device__ __forceinline__ void forward(const Ctx &x, const KGrid &g, const doub
 #pragma unroll
   for (int gg = 0; gg < 8; gg++) u[gg] = x.s[e2(c0, f0, gg)];
   wave_sync();
-  fz_dft<8>(u, x.w8f);
+  dft8s<-1>(u);
 #pragma unroll
   for (int e = 0; e < 8; e++) x.s[e1(c0, 4 * e + f0)] = u[e];
   wave_sync();
@@ -406,7 +442,7 @@ __device__ __forceinline__ void forward(const Ctx &x, const KGrid &g, const doub
   wave_sync();
 #pragma unroll
   for (int c = 1; c < 8; c++) u[c] = fz_cmul(u[c], x.t1[32 * c + b]);
-  fz_dft<8>(u, x.w8f);
+  dft8s<-1>(u);
   // u[a] = Y[32 a + b]; the Hermitian split pairs Y[m] with Y[N - m],
   // which lane (32 - b) mod 32 holds at a' = 7 - a (lane 0: a' = (8 - a) mod 8)
   const double den = (double)(2 * N);
@@ -469,10 +505,6 @@ k_fused_products_256(KGrid g, const double *__restrict__ rsq, const double2 *tw,
   x.s = S + line * LP;
   x.b = threadIdx.x & 31;
   double2 *ys = Y + line * N + x.b;
-  fz_wr<8>(x.w8i, tw, N, true);
-  fz_wr<8>(x.w8f, tw, N, false);
-  fz_wr<4>(x.w4i, tw, N, true);
-  fz_wr<4>(x.w4f, tw, N, false);
   const size_t off = (size_t)(rr - line) + (size_t)g.ndimr * g.ndimth * z;  // same for the block
   const int lo = inr ? line : 0;
   auto in = [&](int q) { return io.in[q] + off; };
