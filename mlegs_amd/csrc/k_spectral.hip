@@ -686,19 +686,9 @@ __device__ void band_apply(int nd, double2 *x, const double *__restrict__ F, lon
   }
 }
 
-// x[i] = x[i-1] for i = n-1 .. 1 (top-down blocks: every block reads its
-// inputs before the block below is written)
-template <int SB>
+// x[i] = x[i-1] for i = n-1 .. 1 (x in LDS)
 __device__ void shift_down_one(double2 *x, int n) {
-  for (int top = n - 1; top >= 1; top -= SB) {
-    double2 v[SB];
-#pragma unroll
-    for (int q = 0; q < SB; q++)
-      if (top - q >= 1) v[q] = x[top - q - 1];
-#pragma unroll
-    for (int q = 0; q < SB; q++)
-      if (top - q >= 1) x[top - q] = v[q];
-  }
+  for (int i = n - 1; i >= 1; i--) x[i] = x[i - 1];
 }
 
 // x <- XM x on nn rows (tri-band, ascending, from the original values)
@@ -808,9 +798,16 @@ __device__ void apply_penta(const KGrid &g, int op, int has_ln, double2 *a, doub
   }
 }
 
+// The m = 0 systems (one per k) are the longest dependency chains of a
+// solve.  One 64-thread block per system stages the system's factor rows,
+// right-hand side and XM rows in LDS with coalesced loads; lane 0 then runs
+// the substitutions from LDS (the same arithmetic as from memory) and the
+// block writes the column back.
 #define M0_SB 4
+size_t m0_lds_bytes(const KGrid &g) { return sizeof(double) * 12 * (size_t)g.nrchopdim; }
+
 __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, int has_ln, double lnval, double2 *a,
-                         double *ln_a, const double *ln_b, const double *__restrict__ F, int k) {
+                         double *ln_a, const double *ln_b, const double *__restrict__ F, int k, double *sm) {
   const long ncls = (long)g.ntchopdim * g.nxchop;
   if (k >= g.nxchopdim) return;
   const int nn = s_nrchops(g.nrchop, 0);
@@ -818,31 +815,53 @@ __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, 
   if (nn < 1 || g.ntchop < 1 || kind == 0) return;
   const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
   const long cls = (long)g.ntchopdim * kabs;
-  double2 *x = a + (size_t)g.nrchopdim * ((size_t)g.ntchopdim * k);
-  if (kind == 1) {
-    // IDEL2 m0k0: first row is the log-term (Dirichlet) row
-    shift_down_one<M0_SB>(x, nn);
-    x[0] = make_double2(has_ln ? lnval / g.norm0 : 0.0, 0.0);
-    band_apply<7, 3, M0_SB>(nn, x, F, ncls, cls);
-    ln_a[0] = has_ln ? x[0].x * g.norm0 : 0.0;
-  } else {
-    // IHELM m=0: solve (del^2 - alp - k^2)(1-x) y = rhs on NN-1 rows, A = (1-x) y
-    if (k == 0) {
-      double lnA = -ln_b[0] / alp;
-      ln_a[0] = lnA;
-      x[0].x = x[0].x - lnA * (4.0 / 3 / g.ell2) / g.norm0;
-      x[1].x = x[1].x - lnA * (-2 / g.norm1 / g.ell2);
-      x[2].x = x[2].x - lnA * (+2.0 / 3 / g.norm2 / g.ell2);
+  double2 *xg = a + (size_t)g.nrchopdim * ((size_t)g.ntchopdim * k);
+  double2 *x = reinterpret_cast<double2 *>(sm);  // [nrchopdim]
+  double *Fs = sm + 2 * g.nrchopdim;             // [nrchopdim][FSTRIDE]
+  double *xt = Fs + FSTRIDE * g.nrchopdim;       // [nrchopdim][3]
+  const int nd = kind == 1 ? nn : nn - 1;        // factorised rows
+  for (int row = threadIdx.x; row < nn; row += blockDim.x) {
+    x[row] = xg[row];
+    if (row < nd) {
+      const double *fr = F + ((size_t)row * ncls + cls) * FSTRIDE;
+#pragma unroll
+      for (int e = 0; e < FSTRIDE; e++) Fs[row * FSTRIDE + e] = fr[e];
     }
-    band_apply<7, 3, M0_SB>(nn - 1, x, F, ncls, cls);
-    x[nn - 1] = zero2();
-    mulxm_column<M0_SB>(x, bxm, nn);
+    if (kind == 2) {
+#pragma unroll
+      for (int e = 0; e < 3; e++) xt[row * 3 + e] = bxm[(size_t)row * 3 + e];
+    }
   }
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    if (kind == 1) {
+      // IDEL2 m0k0: first row is the log-term (Dirichlet) row
+      shift_down_one(x, nn);
+      x[0] = make_double2(has_ln ? lnval / g.norm0 : 0.0, 0.0);
+      band_apply<7, 3, M0_SB>(nn, x, Fs, 1, 0);
+      ln_a[0] = has_ln ? x[0].x * g.norm0 : 0.0;
+    } else {
+      // IHELM m=0: solve (del^2 - alp - k^2)(1-x) y = rhs on NN-1 rows, A = (1-x) y
+      if (k == 0) {
+        double lnA = -ln_b[0] / alp;
+        ln_a[0] = lnA;
+        x[0].x = x[0].x - lnA * (4.0 / 3 / g.ell2) / g.norm0;
+        x[1].x = x[1].x - lnA * (-2 / g.norm1 / g.ell2);
+        x[2].x = x[2].x - lnA * (+2.0 / 3 / g.norm2 / g.ell2);
+      }
+      band_apply<7, 3, M0_SB>(nn - 1, x, Fs, 1, 0);
+      x[nn - 1] = zero2();
+      mulxm_column<M0_SB>(x, xt, nn);
+    }
+  }
+  __syncthreads();
+  for (int row = threadIdx.x; row < nn; row += blockDim.x) xg[row] = x[row];
 }
 
-// One launch: blocks [0, npenta) take the penta-diagonal systems, the
-// remaining blocks the m = 0 systems, so both sets run side by side; grid
-// row y solves field y of a batch of independent solves.
+// One launch: blocks [0, nm0) take the m = 0 systems (one block each, first
+// because they are the longest), the remaining blocks the penta-diagonal
+// systems (one thread each), so both sets run side by side; grid row y
+// solves field y of a batch of independent solves.
 struct BandJob {
   double2 *a;
   double *ln_a;
@@ -856,12 +875,13 @@ struct BandJobs {
 };
 
 __global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, int op, int has_ln, double lnval, BandJobs jobs,
-                                                   int npenta) {
+                                                   int nm0) {
+  extern __shared__ __align__(16) double m0s[];
   const BandJob &J = jobs.j[blockIdx.y];
-  if ((int)blockIdx.x < npenta) {
-    apply_penta(g, op, has_ln, J.a, J.ln_a, J.F, blockIdx.x * (long)blockDim.x + threadIdx.x);
+  if ((int)blockIdx.x < nm0) {
+    apply_m0(g, bxm, op, J.alp, has_ln, lnval, J.a, J.ln_a, J.ln_b, J.F, blockIdx.x, m0s);
   } else {
-    apply_m0(g, bxm, op, J.alp, has_ln, lnval, J.a, J.ln_a, J.ln_b, J.F, (blockIdx.x - npenta) * blockDim.x + threadIdx.x);
+    apply_penta(g, op, has_ln, J.a, J.ln_a, J.F, (blockIdx.x - nm0) * (long)blockDim.x + threadIdx.x);
   }
 }
 
@@ -919,8 +939,9 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, double2 *const *a, double *co
     jobs.j[i] = BandJob{a[i], ln_a[i], ln_b[i], F, alp[i]};
   }
   KScope ks(c, K_BAND_SOLVE, n * 2.0 * 16.0 * fff_count(g));  // the coefficients read and written once
-  const int npenta = (int)((nsys + 63) / 64), nm0 = (op != 1) ? (g.nxchopdim + 63) / 64 : 0;
-  hipLaunchKernelGGL(k_band_apply, dim3(npenta + nm0, n), dim3(64), 0, c->stream, g, c->d.bxm, op, has_ln, ln, jobs, npenta);
+  const int npenta = (int)((nsys + 63) / 64), nm0 = (op != 1) ? g.nxchopdim : 0;
+  hipLaunchKernelGGL(k_band_apply, dim3(npenta + nm0, n), dim3(64), nm0 ? m0_lds_bytes(g) : 0, c->stream, g, c->d.bxm, op,
+                     has_ln, ln, jobs, nm0);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
