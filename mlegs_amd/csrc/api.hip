@@ -763,6 +763,45 @@ static int to_pfp(mlg_field *a) {
   return a->space == MLG_PFP ? 0 : 2;
 }
 
+// to_pfp of n independent FFF fields: one RTRAN backward launch and one
+// VERFFT backward launch for all of them (each field as mlg_rtran then
+// mlg_verfft treat it)
+static int to_pfp_n(mlg_ctx *c, int n, mlg_field *const *fs) {
+  bool all_fff = n <= MAX_TRANSFORM_JOBS;
+  for (int i = 0; i < n; i++) all_fff = all_fff && fs[i]->space == MLG_FFF;
+  if (!all_fff) {
+    for (int i = 0; i < n; i++) RET(to_pfp(fs[i]));
+    return 0;
+  }
+  Temps T(c);
+  mlg_field *t[MAX_TRANSFORM_JOBS];
+  for (int i = 0; i < n; i++) t[i] = T.buffer();
+  RET(T.ready());
+  const double2 *in[MAX_TRANSFORM_JOBS];
+  const double *ln[MAX_TRANSFORM_JOBS];
+  double2 *out[MAX_TRANSFORM_JOBS];
+  for (int i = 0; i < n; i++) {
+    in[i] = fs[i]->e;
+    ln[i] = fs[i]->ln;
+    out[i] = t[i]->e;
+  }
+  RET(launch_rtran_bwd_n(c, n, in, ln, out));
+  for (int i = 0; i < n; i++) {
+    swap_storage(fs[i], t[i]);
+    fs[i]->space = MLG_PFF;
+    RET(launch_chopdo(c, fs[i]->e, MLG_PFF));
+    RET(launch_set_ln(c, fs[i]->ln, nullptr, 0.0, 0.0));
+    in[i] = fs[i]->e;
+    out[i] = t[i]->e;  // the FFF storage, free again
+  }
+  RET(launch_verfft_bwd_n(c, n, in, out));
+  for (int i = 0; i < n; i++) {
+    swap_storage(fs[i], t[i]);
+    fs[i]->space = MLG_PFP;
+  }
+  return 0;
+}
+
 // CALC_BOUSSI_FORCE (src/mod_boussinesq.f90:317-410)
 extern "C" int mlg_force(const mlg_field *psi, const mlg_field *chi, const mlg_field *b, mlg_field *psin, mlg_field *chin,
                          mlg_field *bn) {
@@ -778,7 +817,8 @@ extern "C" int mlg_force(const mlg_field *psi, const mlg_field *chi, const mlg_f
   if (!rc) rc = mlg_pc2vor(psi, chi, ror, rop, oz);
   mlg_field *fs[9] = {rur, rup, uz, ror, rop, oz, rbr, rbp, bz};
   const bool fused = fused_products_supported(c);
-  for (int i = 0; i < 9 && !rc; i++) rc = fused ? to_pfp(fs[i]) : mlg_tofp(fs[i]);
+  if (!rc && fused) rc = to_pfp_n(c, 9, fs);
+  for (int i = 0; i < 9 && !rc && !fused; i++) rc = mlg_tofp(fs[i]);
   mlg_chopset(c, -3);
   RET(rc);
   // W = TOFP(B) at NRCHOP (its value does not depend on the products, so it
@@ -829,7 +869,7 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
   RET(rc);
   mlg_field *fs[9] = {rur, rup, uz, ror, rop, oz, rbr, rbp, bz};
   if (fused_products_supported(c)) {
-    for (int i = 0; i < 9; i++) RET(to_pfp(fs[i]));
+    RET(to_pfp_n(c, 9, fs));
     const double2 *in[10] = {rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, nullptr};
     double2 *out[4] = {ror->e, rop->e, oz->e, bn->e};
     RET(launch_fused_products(c, in, out, 0.0, 0.0, 0));

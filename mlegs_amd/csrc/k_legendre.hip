@@ -61,8 +61,11 @@ union RtranBwdLds {
 };
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
-k_rtran_bwd(KGrid g, const double *__restrict__ pf, const double2 *__restrict__ in, const double *__restrict__ ln_in,
-            const double *__restrict__ lnx, double2 *__restrict__ out) {
+k_rtran_bwd(KGrid g, const double *__restrict__ pf, TransformJobs jobs, const double *__restrict__ lnx) {
+  // grid row y transforms field y of the batch
+  const double2 *__restrict__ in = jobs.in[blockIdx.y];
+  const double *__restrict__ ln_in = jobs.ln[blockIdx.y];
+  double2 *__restrict__ out = jobs.out[blockIdx.y];
   const int nk = g.nxchopdim;
   const int ntiles_c = (2 * nk + BW_TC - 1) / BW_TC, ntiles_i = (g.nrh + BW_TI - 1) / BW_TI;
   const int lid = xcd_logical(ntiles_c * ntiles_i * g.ntchopdim);
@@ -180,15 +183,26 @@ k_rtran_bwd(KGrid g, const double *__restrict__ pf, const double2 *__restrict__ 
   }
 }
 
-int launch_rtran_bwd(mlg_ctx *c, const double2 *in, const double *ln_in, double2 *out) {
+int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double *const *ln_in, double2 *const *out) {
+  if (n < 1 || n > MAX_TRANSFORM_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
+  TransformJobs jobs;
+  for (int i = 0; i < n; i++) {
+    jobs.in[i] = in[i];
+    jobs.ln[i] = ln_in[i];
+    jobs.out[i] = out[i];
+  }
   int nti = (g.nrh + BW_TI - 1) / BW_TI, ntc = (2 * g.nxchopdim + BW_TC - 1) / BW_TC;
-  dim3 grid(xcd_grid(nti * ntc * g.ntchopdim));
-  // 2 NR sum_m NRCHOPS(m) NXCHOPDIM flops (real table x complex data, folded)
-  KScope ks(c, K_RTRAN_BWD, 2.0 * g.nr * (double)sum_nrchops(c) * g.nxchopdim);
-  hipLaunchKernelGGL(k_rtran_bwd, grid, dim3(256), 0, c->stream, g, c->d.pf, in, ln_in, c->d.lnx, out);
+  dim3 grid(xcd_grid(nti * ntc * g.ntchopdim), n);
+  // 2 NR sum_m NRCHOPS(m) NXCHOPDIM flops per field (real table x complex data, folded)
+  KScope ks(c, K_RTRAN_BWD, n * 2.0 * g.nr * (double)sum_nrchops(c) * g.nxchopdim);
+  hipLaunchKernelGGL(k_rtran_bwd, grid, dim3(256), 0, c->stream, g, c->d.pf, jobs, c->d.lnx);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
+}
+
+int launch_rtran_bwd(mlg_ctx *c, const double2 *in, const double *ln_in, double2 *out) {
+  return launch_rtran_bwd_n(c, 1, &in, &ln_in, &out);
 }
 
 // -------------------------------------------------------------------------

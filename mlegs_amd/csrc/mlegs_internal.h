@@ -154,6 +154,15 @@ static inline int nrchops_of(int nrchop, int m) {
 
 // ---- launchers: Legendre transforms (k_legendre.hip)
 int launch_rtran_bwd(mlg_ctx *c, const double2 *in, const double *ln_in, double2 *out);
+// n (<= MAX_TRANSFORM_JOBS) independent fields in one launch
+constexpr int MAX_TRANSFORM_JOBS = 9;
+struct TransformJobs {
+  const double2 *in[MAX_TRANSFORM_JOBS];
+  const double *ln[MAX_TRANSFORM_JOBS];
+  double2 *out[MAX_TRANSFORM_JOBS];
+};
+int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double *const *ln_in, double2 *const *out);
+int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out);
 int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out);
 int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *psi, double2 *chi);
 // ---- FFTs (k_fft.hip)
