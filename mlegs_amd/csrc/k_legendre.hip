@@ -22,15 +22,13 @@ __device__ __forceinline__ v4d mfma16x16x4(double a, double b, v4d c) {
   return __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, c, 0, 0, 0);
 }
 
-// LDS tiles are [row][col] with the column index XOR-swizzled by the row:
-// row bit 0 -> column bit 4, row bits 1..3 -> column bits 0, 2, 3, and a
-// parity bit -> column bit 1.  MFMA operand reads (16 consecutive columns
-// of rows 2t and 2t+1) then fall into opposite halves of the banks, and the
-// staging writes (16 consecutive rows of one column pair, or of both
-// parities of one column) into 32 different banks.
-__device__ __forceinline__ int lsw(int r, int par) {
-  return ((r & 1) << 4) | ((r >> 1) & 1) | (((r >> 2) & 1) << 2) | (((r >> 3) & 1) << 3) | (par << 1);
-}
+// LDS tiles are [row][col] with the column index XOR-swizzled by the row and
+// a parity bit: column bits 0..3 ^= row bits 0..3, column bit 4 ^= row bit 0
+// ^ parity, column bit 3 ^= parity.  The MFMA operand reads (16 consecutive
+// columns of one row per 16 lanes) and the staging writes (16 rows, or 8
+// rows of both parities, of one column per 16 lanes) then hit 16 different
+// 8-byte bank pairs of a 32-bank LDS.
+__device__ __forceinline__ int lsw(int r, int par) { return (r & 15) ^ (((r & 1) ^ par) << 4) ^ (par << 3); }
 
 __device__ __forceinline__ int d_nrchops(int nrchop, int m) {
   int v = nrchop - m < nrchop ? nrchop - m : nrchop;
