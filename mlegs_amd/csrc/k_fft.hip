@@ -294,14 +294,18 @@ struct Fft4 {
   static constexpr int N = R1 * R2;
   static constexpr int TPP = R1 > R2 ? R1 : R2;
   static constexpr int P = 256 / TPP;
+  // VERFFT backward: 512 threads, 32 radial nodes per block at N = 256, so
+  // its PFF reads and PFP writes are 512-byte runs (measured faster than
+  // 256-byte runs; the forward direction measured the other way)
+  static constexpr int PB = 512 / TPP;
 };
 
 // Pass 1 + twiddle + LDS store + pass 2.  `v` enters holding x[R2 n1 + t]
 // (thread t < R2) and leaves holding X[t + R1 k2] (thread t < R1).
-template <int R1, int R2>
+template <int R1, int R2, int P = Fft4<R1, R2>::P>
 __device__ __forceinline__ void fft4_core(double2 (&v1)[R1], double2 (&v2)[R2], double2 *s, int r, int t,
                                           const double2 *__restrict__ tw, bool inv) {
-  constexpr int N = R1 * R2, P = Fft4<R1, R2>::P;
+  constexpr int N = R1 * R2;
   if (t < R2) {
     double2 w1[R1 / 2 > 0 ? R1 / 2 : 1];
     load_wr<R1>(w1, tw, N, inv);
@@ -355,11 +359,11 @@ __global__ void __launch_bounds__(256) k_verfft_fwd4(KGrid g, const double2 *__r
 
 // VERFFT backward: PFF (r, m, k chopped) -> PFP (r, m, z), unscaled.
 template <int R1, int R2>
-__global__ void __launch_bounds__(256) k_verfft_bwd4(KGrid g, const double2 *__restrict__ tw, TransformJobs jobs) {
+__global__ void __launch_bounds__(512) k_verfft_bwd4(KGrid g, const double2 *__restrict__ tw, TransformJobs jobs) {
   // grid layer z transforms field z of the batch
   const double2 *__restrict__ in = jobs.in[blockIdx.z];
   double2 *__restrict__ out = jobs.out[blockIdx.z];
-  constexpr int N = R1 * R2, P = Fft4<R1, R2>::P;
+  constexpr int N = R1 * R2, P = Fft4<R1, R2>::PB;
   __shared__ double2 s[P * N];
   const int r = threadIdx.x % P, t = threadIdx.x / P;
   const int mm = blockIdx.y, rr = blockIdx.x * P + r;
@@ -375,7 +379,7 @@ __global__ void __launch_bounds__(256) k_verfft_bwd4(KGrid g, const double2 *__r
                                       : make_double2(0.0, 0.0);
     }
   }
-  fft4_core<R1, R2>(v1, v2, s, r, t, tw, true);
+  fft4_core<R1, R2, P>(v1, v2, s, r, t, tw, true);
   if (t < R1 && rr < g.ndimr) {
 #pragma unroll
     for (int k2 = 0; k2 < R2; k2++)
@@ -512,8 +516,8 @@ int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *co
   }
   KScope ks(c, K_VERFFT_BWD, n * 16.0 * g.nr * g.ntchopdim * (double)(g.nx + g.nxchopdim));
 #define L_VB(a, b)                                                                                                \
-  hipLaunchKernelGGL((k_verfft_bwd4<a, b>), dim3((g.ndimr + Fft4<a, b>::P - 1) / Fft4<a, b>::P, g.ntchopdim, n), \
-                     dim3(256), 0, c->stream, g, c->d.tw_z, jobs)
+  hipLaunchKernelGGL((k_verfft_bwd4<a, b>), dim3((g.ndimr + Fft4<a, b>::PB - 1) / Fft4<a, b>::PB, g.ntchopdim, n), \
+                     dim3(512), 0, c->stream, g, c->d.tw_z, jobs)
   FFT4_SWITCH(g.nx, L_VB)
 #undef L_VB
   MLG_CHECK_HIP(hipGetLastError());
