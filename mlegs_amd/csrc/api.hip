@@ -506,10 +506,9 @@ extern "C" int mlg_rtran(mlg_field *a, int is) {
     RET(launch_chopdo(c, a->e, MLG_FFF));
   } else {
     if (a->space != MLG_FFF) return 2;
-    RET(launch_rtran_bwd(c, a->e, a->ln, t->e));
+    RET(launch_rtran_bwd(c, a->e, a->ln, t->e));  // writes the padding rows too
     swap_storage(a, t);
     a->space = MLG_PFF;
-    RET(launch_chopdo(c, a->e, MLG_PFF));
   }
   return launch_set_ln(c, a->ln, nullptr, 0.0, 0.0);
 }
@@ -785,15 +784,16 @@ static int to_pfp_n(mlg_ctx *c, int n, mlg_field *const *fs) {
     ln[i] = fs[i]->ln;
     out[i] = t[i]->e;
   }
-  RET(launch_rtran_bwd_n(c, n, in, ln, out));
+  RET(launch_rtran_bwd_n(c, n, in, ln, out));  // padding rows included
+  double *lns[MAX_TRANSFORM_JOBS];
   for (int i = 0; i < n; i++) {
     swap_storage(fs[i], t[i]);
     fs[i]->space = MLG_PFF;
-    RET(launch_chopdo(c, fs[i]->e, MLG_PFF));
-    RET(launch_set_ln(c, fs[i]->ln, nullptr, 0.0, 0.0));
+    lns[i] = fs[i]->ln;
     in[i] = fs[i]->e;
     out[i] = t[i]->e;  // the FFF storage, free again
   }
+  RET(launch_zero_lns(c, lns, n));
   RET(launch_verfft_bwd_n(c, n, in, out));
   for (int i = 0; i < n; i++) {
     swap_storage(fs[i], t[i]);

@@ -179,6 +179,15 @@ k_rtran_bwd(KGrid g, const double *__restrict__ pf, TransformJobs jobs, const do
       }
     }
   }
+  // the padding rows r = NR..NDIMR-1 of the block's columns are zero (CHOPDO
+  // of the PFF result)
+  if (ti == 0) {
+    const int pad = g.ndimr - g.nr;
+    for (int idx = tid; idx < pad * (BW_TC / 2); idx += 256) {
+      const int kl = idx / pad, k = (c0 >> 1) + kl;
+      if (k < nk) out[(size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k) + g.nr + (idx - kl * pad)] = make_double2(0.0, 0.0);
+    }
+  }
 }
 
 int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double *const *ln_in, double2 *const *out) {

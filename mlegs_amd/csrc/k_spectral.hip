@@ -131,12 +131,13 @@ int launch_set_ln(mlg_ctx *c, double *dst, const double *src, double scale, doub
 
 // ---------------------------------------------------------------- CHOPDO
 // src/mod_scalar3.f90:213-257.  PFP keeps only m < NTCHOPDIM meaningful.
-__global__ void k_chopdo_fff(KGrid g, double2 *a) {
-  GRID_STRIDE(fff_count(g)) {
-    FIdx f = fdecode(g, idx);
-    int nn = s_nrchops(g.nrchop, f.mm * g.minc);
-    if (f.mm >= g.ntchop || f.n >= nn) a[idx] = zero2();
-  }
+// one 64-thread block per (m, k) column: zero rows n >= NRCHOPS(m), and the
+// whole column for m >= NTCHOP (only the zeroed positions are visited)
+__global__ void __launch_bounds__(64) k_chopdo_fff(KGrid g, double2 *a) {
+  const int mm = blockIdx.x, k = blockIdx.y;
+  const int nn = mm < g.ntchop ? s_nrchops(g.nrchop, mm * g.minc) : 0;
+  double2 *col = a + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
+  for (int n = (nn > 0 ? nn : 0) + threadIdx.x; n < g.nrchopdim; n += 64) col[n] = zero2();
 }
 __global__ void k_chopdo_rows(KGrid g, double2 *a, int n2, int n3, int stride2, int zero_slot) {
   // rows NR..NDIMR-1 for every (j < n2, z < n3), then (optionally) every
@@ -167,7 +168,7 @@ int launch_chopdo(mlg_ctx *c, double2 *a, int space) {
       hipLaunchKernelGGL(k_chopdo_rows, dim3(nblocks(tot)), dim3(256), 0, c->stream, g, a, n2, n3, stride2, zero_slot);
   };
   if (space == MLG_FFF) {
-    hipLaunchKernelGGL(k_chopdo_fff, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, a);
+    hipLaunchKernelGGL(k_chopdo_fff, dim3(g.ntchopdim, g.nxchopdim), dim3(64), 0, c->stream, g, a);
   } else if (space == MLG_PFF) {
     rows(g.ntchopdim, g.nxchopdim, g.ntchopdim, -1);
   } else if (space == MLG_PPP) {
@@ -1012,14 +1013,18 @@ struct EtdArgs {
   double tf;
 };
 
-__device__ __forceinline__ double2 filt(const KGrid &g, const FIdx &f, int nn, const double *h, int nrc, double tf,
-                                        double2 v) {
+// HYPERV3 factor h^tf of one coefficient (0 outside the truncation); tf is
+// 1 on full steps, where h^1 = h
+__device__ __forceinline__ double filt_factor(const KGrid &g, const FIdx &f, int nn, const double *h, int nrc, double tf) {
   if (f.n < nn && f.mm < g.ntchop) {
     const double *hr = h, *ht = h + (size_t)nrc * g.ntchopdim, *hx = ht + g.ntchopdim;
     double hh = hr[f.n + (size_t)nrc * f.mm] * ht[f.mm] * hx[f.k];
-    return axc(pow(hh, tf), v);
+    return tf == 1.0 ? hh : pow(hh, tf);
   }
-  return zero2();
+  return 0.0;
+}
+__device__ __forceinline__ double2 filt_apply(const KGrid &g, const FIdx &f, int nn, double fac, double2 v) {
+  return (f.n < nn && f.mm < g.ntchop) ? axc(fac, v) : zero2();
 }
 
 __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, EtdArgs A, int ab, int save) {
@@ -1048,10 +1053,13 @@ __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, E
       ar[5] = dt;
     }
     double2 nl[4] = {A.nr[idx], A.np[idx], A.nz[idx], A.nb[idx]};
+    double fv = 0.0, fd = 0.0;  // filter factors (velocity, buoyancy)
     if (A.hv) {
+      fv = filt_factor(g, f, nn, A.hv, A.nrc, A.tf);
+      fd = filt_factor(g, f, nn, A.hd, A.nrc, A.tf);
 #pragma unroll
-      for (int q = 0; q < 3; q++) nl[q] = filt(g, f, nn, A.hv, A.nrc, A.tf, nl[q]);
-      nl[3] = filt(g, f, nn, A.hd, A.nrc, A.tf, nl[3]);
+      for (int q = 0; q < 3; q++) nl[q] = filt_apply(g, f, nn, fv, nl[q]);
+      nl[3] = filt_apply(g, f, nn, fd, nl[3]);
     }
     const double2 bold = A.b[idx];
     if (A.b0) A.b0[idx] = bold;
@@ -1088,8 +1096,8 @@ __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, E
     }
     if (A.hv) {
 #pragma unroll
-      for (int q = 0; q < 3; q++) cv[q] = filt(g, f, nn, A.hv, A.nrc, A.tf, cv[q]);
-      cv[3] = filt(g, f, nn, A.hd, A.nrc, A.tf, cv[3]);
+      for (int q = 0; q < 3; q++) cv[q] = filt_apply(g, f, nn, fv, cv[q]);
+      cv[3] = filt_apply(g, f, nn, fd, cv[3]);
     }
     A.rur[idx] = cv[0];
     A.rup[idx] = cv[1];
