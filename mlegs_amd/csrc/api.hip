@@ -476,6 +476,10 @@ extern "C" int mlg_field_zero(mlg_field *f, int space) {
 static int copy0(mlg_field *b, const mlg_field *a) {
   mlg_ctx *c = a->ctx;
   if (a == b) return 0;
+  if (a->space == MLG_FFF) {
+    b->space = MLG_FFF;
+    return launch_copy0_fff(c, b->e, b->ln, a->e, a->ln);
+  }
   RET(launch_copy(c, b->e, a->e, space_count(c, a->space)));
   RET(launch_set_ln(c, b->ln, a->ln, 1.0, 0.0));
   b->space = a->space;
@@ -632,17 +636,11 @@ static int solve_op(const mlg_field *b, mlg_field *a, int op, double alp, int ha
   if (!a || !b) return MLG_E_ARG;
   if (b->space != MLG_FFF) return 3;
   mlg_ctx *c = b->ctx;
-  Temps T(c);
-  const mlg_field *bs = b;
-  if (a == b) {
-    mlg_field *t = T.buffer();
-    RET(T.ready());
-    RET(copy0(t, b));
-    bs = t;
-  }
-  RET(copy0(a, bs));  // A = B
+  // A = B then the substitution in A (the reference's COPY + in-place solve):
+  // the solve reads B and writes A directly, the CHOPDO below clears the rest
+  a->space = MLG_FFF;
   if (op != 2) RET(launch_set_ln(c, a->ln, nullptr, 0.0, 0.0));
-  RET(launch_band_solve(c, op, a->e, a->ln, bs->ln, alp, has_ln, ln));
+  RET(launch_band_solve(c, op, b->e, a->e, a->ln, b->ln, alp, has_ln, ln));
   RET(launch_chopdo(c, a->e, MLG_FFF));
   if (op != 2) RET(launch_fix_at_infinity(c, a->e));
   return 0;
@@ -1055,18 +1053,20 @@ static int richardson(mlg_field *y, const mlg_field *x1, bool ln) {
 // launch for all of them (each field as solve_op does it for op 2).
 static int ihelm_n(int n, mlg_field *const *b, mlg_field *const *a, const double *alp) {
   mlg_ctx *c = b[0]->ctx;
+  const double2 *be[3];
   double2 *ae[3];
   double *aln[3];
   const double *bln[3];
   for (int i = 0; i < n; i++) {
     if (alp[i] == 0.0) return 3;  // IHELM: ALP CAN'T BE ZERO (mod_legops.f90:1243-1246)
     if (b[i]->space != MLG_FFF) return 3;
-    RET(copy0(a[i], b[i]));  // A = B
+    a[i]->space = MLG_FFF;  // A = B, solved in place: read from B, written to A
+    be[i] = b[i]->e;
     ae[i] = a[i]->e;
     aln[i] = a[i]->ln;
     bln[i] = b[i]->ln;
   }
-  RET(launch_band_solve_n(c, 2, n, ae, aln, bln, alp, 0, 0.0));
+  RET(launch_band_solve_n(c, 2, n, be, ae, aln, bln, alp, 0, 0.0));
   for (int i = 0; i < n; i++) RET(launch_chopdo(c, a[i]->e, MLG_FFF));
   return 0;
 }

@@ -95,6 +95,22 @@ int launch_copy(mlg_ctx *c, double2 *dst, const double2 *src, long n) {
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
+// COPY0 of an FFF field in one pass: E inside the truncation, 0 outside
+// (the chop of the copy), and LN; one 64-thread block per (m, k) column
+__global__ void __launch_bounds__(64) k_copy0_fff(KGrid g, double2 *dst, double *dst_ln, const double2 *src,
+                                                  const double *src_ln) {
+  const int mm = blockIdx.x, k = blockIdx.y;
+  const int nn = mm < g.ntchop ? s_nrchops(g.nrchop, mm * g.minc) : 0;
+  const size_t col = (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
+  for (int n = threadIdx.x; n < g.nrchopdim; n += 64) dst[col + n] = n < nn ? src[col + n] : zero2();
+  if (mm == 0 && k == 0 && threadIdx.x == 0) dst_ln[0] = src_ln[0] * 1.0 + 0.0;
+}
+int launch_copy0_fff(mlg_ctx *c, double2 *dst, double *dst_ln, const double2 *src, const double *src_ln) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_copy0_fff, dim3(g.ntchopdim, g.nxchopdim), dim3(64), 0, c->stream, g, dst, dst_ln, src, src_ln);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
 int launch_negate(mlg_ctx *c, double2 *a, long n) {
   hipLaunchKernelGGL(k_negate, dim3(nblocks(n)), dim3(256), 0, c->stream, a, n);
   MLG_CHECK_HIP(hipGetLastError());
@@ -611,7 +627,7 @@ __device__ void band_factor(const SolveCtx &s, int nd, double *F, long ncls, lon
 // loads of a block are issued before its serial arithmetic, so one memory
 // round trip serves SB rows of the dependency chain.
 template <int NE, int C0, int SB>
-__device__ void band_apply(int nd, double2 *x, const double *__restrict__ F, long ncls, long cls) {
+__device__ void band_apply(int nd, const double2 *xin, double2 *x, const double *__restrict__ F, long ncls, long cls) {
   constexpr int KU = NE - 1 - C0;
   double2 xh[C0];
 #pragma unroll
@@ -624,7 +640,7 @@ __device__ void band_apply(int nd, double2 *x, const double *__restrict__ F, lon
       const int nn = nb + q;
       if (nn < nd) {
         const double *fr = F + ((size_t)nn * ncls + cls) * FSTRIDE;
-        rhs[q] = x[nn];
+        rhs[q] = xin[nn];
 #pragma unroll
         for (int j = 0; j < C0; j++) lm[q][j] = fr[j];
         rp[q] = fr[NE - 1];
@@ -779,8 +795,8 @@ __device__ __forceinline__ int solve_kind(int op, int mm, int k) {
   return 0;
 }
 
-__device__ void apply_penta(const KGrid &g, int op, int has_ln, double2 *a, double *ln_a, const double *__restrict__ F,
-                            long sys) {
+__device__ void apply_penta(const KGrid &g, int op, int has_ln, const double2 *b, double2 *a, double *ln_a,
+                            const double *__restrict__ F, long sys) {
   const long nsys = (long)g.ntchopdim * g.nxchopdim;
   const long ncls = (long)g.ntchopdim * g.nxchop;
   if (sys >= nsys) return;
@@ -788,8 +804,9 @@ __device__ void apply_penta(const KGrid &g, int op, int has_ln, double2 *a, doub
   const int nn = s_nrchops(g.nrchop, mm * g.minc);
   if (nn < 1 || mm >= g.ntchop || solve_kind(op, mm, k) != 0) return;
   const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
-  double2 *x = a + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
-  band_apply<5, 2, 16>(nn, x, F, ncls, mm + (long)g.ntchopdim * kabs);
+  const size_t col = (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
+  double2 *x = a + col;
+  band_apply<5, 2, 16>(nn, b + col, x, F, ncls, mm + (long)g.ntchopdim * kabs);
   if (mm == 0 && k == 0) {
     if (op == 0) ln_a[0] = has_ln ? x[0].x * g.norm0 : 0.0;
     if (op == 1) {
@@ -807,22 +824,26 @@ __device__ void apply_penta(const KGrid &g, int op, int has_ln, double2 *a, doub
 #define M0_SB 4
 size_t m0_lds_bytes(const KGrid &g) { return sizeof(double) * 12 * (size_t)g.nrchopdim; }
 
-__device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, int has_ln, double lnval, double2 *a,
-                         double *ln_a, const double *ln_b, const double *__restrict__ F, int k, double *sm) {
+__device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, int has_ln, double lnval,
+                         const double2 *b, double2 *a, double *ln_a, const double *ln_b, const double *__restrict__ F,
+                         int k, double *sm) {
   const long ncls = (long)g.ntchopdim * g.nxchop;
   if (k >= g.nxchopdim) return;
+  // IHELM: A%LN starts as B%LN (COPY), replaced below by -B%LN/ALP
+  if (op == 2 && k == 0 && threadIdx.x == 0) ln_a[0] = ln_b[0] * 1.0 + 0.0;
   const int nn = s_nrchops(g.nrchop, 0);
   const int kind = solve_kind(op, 0, k);
   if (nn < 1 || g.ntchop < 1 || kind == 0) return;
   const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
   const long cls = (long)g.ntchopdim * kabs;
+  const double2 *xb = b + (size_t)g.nrchopdim * ((size_t)g.ntchopdim * k);
   double2 *xg = a + (size_t)g.nrchopdim * ((size_t)g.ntchopdim * k);
   double2 *x = reinterpret_cast<double2 *>(sm);  // [nrchopdim]
   double *Fs = sm + 2 * g.nrchopdim;             // [nrchopdim][FSTRIDE]
   double *xt = Fs + FSTRIDE * g.nrchopdim;       // [nrchopdim][3]
   const int nd = kind == 1 ? nn : nn - 1;        // factorised rows
   for (int row = threadIdx.x; row < nn; row += blockDim.x) {
-    x[row] = xg[row];
+    x[row] = xb[row];
     if (row < nd) {
       const double *fr = F + ((size_t)row * ncls + cls) * FSTRIDE;
 #pragma unroll
@@ -839,7 +860,7 @@ __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, 
       // IDEL2 m0k0: first row is the log-term (Dirichlet) row
       shift_down_one(x, nn);
       x[0] = make_double2(has_ln ? lnval / g.norm0 : 0.0, 0.0);
-      band_apply<7, 3, M0_SB>(nn, x, Fs, 1, 0);
+      band_apply<7, 3, M0_SB>(nn, x, x, Fs, 1, 0);
       ln_a[0] = has_ln ? x[0].x * g.norm0 : 0.0;
     } else {
       // IHELM m=0: solve (del^2 - alp - k^2)(1-x) y = rhs on NN-1 rows, A = (1-x) y
@@ -850,7 +871,7 @@ __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, 
         x[1].x = x[1].x - lnA * (-2 / g.norm1 / g.ell2);
         x[2].x = x[2].x - lnA * (+2.0 / 3 / g.norm2 / g.ell2);
       }
-      band_apply<7, 3, M0_SB>(nn - 1, x, Fs, 1, 0);
+      band_apply<7, 3, M0_SB>(nn - 1, x, x, Fs, 1, 0);
       x[nn - 1] = zero2();
       mulxm_column<M0_SB>(x, xt, nn);
     }
@@ -864,6 +885,7 @@ __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, 
 // systems (one thread each), so both sets run side by side; grid row y
 // solves field y of a batch of independent solves.
 struct BandJob {
+  const double2 *b;  // right-hand side (may be a)
   double2 *a;
   double *ln_a;
   const double *ln_b;
@@ -880,9 +902,9 @@ __global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, i
   extern __shared__ __align__(16) double m0s[];
   const BandJob &J = jobs.j[blockIdx.y];
   if ((int)blockIdx.x < nm0) {
-    apply_m0(g, bxm, op, J.alp, has_ln, lnval, J.a, J.ln_a, J.ln_b, J.F, blockIdx.x, m0s);
+    apply_m0(g, bxm, op, J.alp, has_ln, lnval, J.b, J.a, J.ln_a, J.ln_b, J.F, blockIdx.x, m0s);
   } else {
-    apply_penta(g, op, has_ln, J.a, J.ln_a, J.F, (blockIdx.x - nm0) * (long)blockDim.x + threadIdx.x);
+    apply_penta(g, op, has_ln, J.b, J.a, J.ln_a, J.F, (blockIdx.x - nm0) * (long)blockDim.x + threadIdx.x);
   }
 }
 
@@ -925,8 +947,8 @@ static double *factor_set(mlg_ctx *c, int op, double alp, const double *const *k
   return F;
 }
 
-int launch_band_solve_n(mlg_ctx *c, int op, int n, double2 *const *a, double *const *ln_a, const double *const *ln_b,
-                        const double *alp, int has_ln, double ln) {
+int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, double2 *const *a, double *const *ln_a,
+                        const double *const *ln_b, const double *alp, int has_ln, double ln) {
   if (n < 1 || n > MAX_BAND_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
   long nsys = (long)g.ntchopdim * g.nxchopdim;
@@ -937,7 +959,7 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, double2 *const *a, double *co
     const double *F = factor_set(c, op, alp[i], used, i, &rc);
     if (!F) return rc;
     used[i] = F;
-    jobs.j[i] = BandJob{a[i], ln_a[i], ln_b[i], F, alp[i]};
+    jobs.j[i] = BandJob{b[i], a[i], ln_a[i], ln_b[i], F, alp[i]};
   }
   KScope ks(c, K_BAND_SOLVE, n * 2.0 * 16.0 * fff_count(g));  // the coefficients read and written once
   const int npenta = (int)((nsys + 63) / 64), nm0 = (op != 1) ? g.nxchopdim : 0;
@@ -947,8 +969,9 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, double2 *const *a, double *co
   return 0;
 }
 
-int launch_band_solve(mlg_ctx *c, int op, double2 *a, double *ln_a, const double *ln_b, double alp, int has_ln, double ln) {
-  return launch_band_solve_n(c, op, 1, &a, &ln_a, &ln_b, &alp, has_ln, ln);
+int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *ln_a, const double *ln_b, double alp,
+                      int has_ln, double ln) {
+  return launch_band_solve_n(c, op, 1, &b, &a, &ln_a, &ln_b, &alp, has_ln, ln);
 }
 
 // ---------------------------------------------------------------- pointwise updates

@@ -188,6 +188,7 @@ int launch_fused_products(mlg_ctx *c, const double2 *const in[10], double2 *cons
 int launch_chopdo(mlg_ctx *c, double2 *a, int space);
 int launch_zero(mlg_ctx *c, double2 *a, long n);
 int launch_copy(mlg_ctx *c, double2 *dst, const double2 *src, long n);
+int launch_copy0_fff(mlg_ctx *c, double2 *dst, double *dst_ln, const double2 *src, const double *src_ln);
 int launch_set_ln(mlg_ctx *c, double *dst, const double *src, double scale, double add);
 int launch_zero_lns(mlg_ctx *c, double *const *ptrs, int n);  // LN = 0 for n fields, one launch per 32
 int launch_delsqh(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, double *ln_b);
@@ -202,10 +203,13 @@ int launch_pc2vor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const do
                   double2 *ror, double2 *rop, double2 *oz, double *ln_ror, double *ln_rop, double *ln_oz);
 int launch_fix_at_infinity(mlg_ctx *c, double2 *a);
 // op: 0 IDEL2, 1 IDEL2LN, 2 IHELM
-int launch_band_solve(mlg_ctx *c, int op, double2 *a, double *ln_a, const double *ln_b, double alp, int has_ln, double ln);
+// A = solve(B) on the FFF coefficients inside the truncation (B may be A);
+// the caller chops A afterwards
+int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *ln_a, const double *ln_b, double alp,
+                      int has_ln, double ln);
 // n (<= 3) independent solves of one operator in one launch (field i with alp[i])
-int launch_band_solve_n(mlg_ctx *c, int op, int n, double2 *const *a, double *const *ln_a, const double *const *ln_b,
-                        const double *alp, int has_ln, double ln);
+int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, double2 *const *a, double *const *ln_a,
+                        const double *const *ln_b, const double *alp, int has_ln, double ln);
 int launch_axpby(mlg_ctx *c, int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n);
 int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1, const double *x2, const double *x3);
 // ETD update fused with the HYPERV3 filters (filter != 0), the copy of b
