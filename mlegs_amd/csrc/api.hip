@@ -713,21 +713,50 @@ extern "C" int mlg_vel2pln(mlg_field *rup, mlg_field *psi, int has_ln, double ln
 // PROJECT (src/mod_legops.f90:2397-2648).  ln_mode 0: no LN argument; 1: LN
 // given as a value; 2: LN given as psi's own LN (the ETD steps pass PSI%LN),
 // so the device copy is kept without a host read.
+// n fields of one space, FFF or PFP, to PFF with one transform launch (each
+// field as mlg_rtran(., 1) or mlg_verfft(., -1) treats it)
+static int to_pff_n(mlg_ctx *c, int n, mlg_field *const *fs) {
+  const int sp = fs[0]->space;
+  if (n > MAX_TRANSFORM_JOBS || (sp != MLG_FFF && sp != MLG_PFP)) return 2;
+  for (int i = 1; i < n; i++)
+    if (fs[i]->space != sp) return 2;
+  Temps T(c);
+  mlg_field *t[MAX_TRANSFORM_JOBS];
+  for (int i = 0; i < n; i++) t[i] = T.buffer();
+  RET(T.ready());
+  const double2 *in[MAX_TRANSFORM_JOBS];
+  const double *ln[MAX_TRANSFORM_JOBS];
+  double2 *out[MAX_TRANSFORM_JOBS];
+  double *lns[MAX_TRANSFORM_JOBS];
+  for (int i = 0; i < n; i++) {
+    in[i] = fs[i]->e;
+    ln[i] = fs[i]->ln;
+    out[i] = t[i]->e;
+    lns[i] = fs[i]->ln;
+  }
+  if (sp == MLG_FFF) {
+    RET(launch_rtran_bwd_n(c, n, in, ln, out));  // padding rows included
+  } else {
+    RET(launch_verfft_fwd_n(c, n, in, out));
+  }
+  for (int i = 0; i < n; i++) {
+    swap_storage(fs[i], t[i]);
+    fs[i]->space = MLG_PFF;
+  }
+  if (sp == MLG_FFF) RET(launch_zero_lns(c, lns, n));
+  return 0;
+}
+
 static int project_impl(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *psi, mlg_field *chi, int ln_mode, double ln) {
   if (!(rur->space == rup->space && rup->space == uz->space)) return 2;
   mlg_ctx *c = rur->ctx;
+  mlg_field *const vel[3] = {rur, rup, uz};
   if (rur->space == MLG_PPP) {
     RET(mlg_plnfft(rur, -1));
     RET(mlg_plnfft(rup, -1));
     RET(mlg_plnfft(uz, -1));
-  } else if (rur->space == MLG_PFP) {
-    RET(mlg_verfft(rur, -1));
-    RET(mlg_verfft(rup, -1));
-    RET(mlg_verfft(uz, -1));
-  } else if (rur->space == MLG_FFF) {
-    RET(mlg_rtran(rur, 1));
-    RET(mlg_rtran(rup, 1));
-    RET(mlg_rtran(uz, 1));
+  } else if (rur->space == MLG_PFP || rur->space == MLG_FFF) {
+    RET(to_pff_n(c, 3, vel));
   } else if (rur->space != MLG_PFF) {
     return 2;
   }
@@ -1200,6 +1229,19 @@ static int pc2vel3(mlg_ctx *c, const mlg_field *psi, const mlg_field *chi, mlg_f
   return rc;
 }
 
+// PC2VEL at NRCHOP+3 followed by the pressure-gradient history update
+// (R_PI_R = R_PI_R - RUR, ... with hmode 4; R_PI_R = ha (R_PI_R - RUR), ...
+// with hmode 3), for velocities used by nothing else: one pass, the
+// velocities are not stored
+static int pc2vel3_hist(mlg_solver *s, const mlg_field *psi, const mlg_field *chi, int hmode, double ha) {
+  mlg_ctx *c = s->ctx;
+  if (psi->space != MLG_FFF || chi->space != MLG_FFF) return 3;
+  RET(mlg_chopset(c, 3));
+  int rc = launch_pc2vel_hist(c, psi->e, psi->ln, chi->e, chi->ln, s->r_pi_r->e, s->r_pi_p->e, s->pi_z->e, hmode, ha);
+  mlg_chopset(c, -3);
+  return rc;
+}
+
 // STEP_BOUSSI_ETDFE_BE (mod_march.f90:664-753)
 static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, const double *e16, const double *nl16,
                          double dt, bool save) {
@@ -1222,10 +1264,7 @@ static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0));  // PROJECT(..., PSI, W, PSI%LN)
   RET(mlg_idel2ln(w, chi));
   if (save) {
-    RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
-    RET(launch_axpby(c, 3, s->r_pi_r->e, 0.5, s->r_pi_r->e, rurn->e, nullptr, c->n_fff));
-    RET(launch_axpby(c, 3, s->r_pi_p->e, 0.5, s->r_pi_p->e, rupn->e, nullptr, c->n_fff));
-    RET(launch_axpby(c, 3, s->pi_z->e, 0.5, s->pi_z->e, uzn->e, nullptr, c->n_fff));
+    RET(pc2vel3_hist(s, psi, chi, 3, 0.5));
   }
   return 0;
 }
@@ -1282,10 +1321,7 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   RET(copy0(chi0, chi));
   RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0));  // PROJECT(..., PSI, W, PSI%LN)
   RET(mlg_idel2ln(w, chi));
-  RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
-  RET(launch_axpby(c, 4, s->r_pi_r->e, 0.0, s->r_pi_r->e, rurn->e, nullptr, c->n_fff));
-  RET(launch_axpby(c, 4, s->r_pi_p->e, 0.0, s->r_pi_p->e, rupn->e, nullptr, c->n_fff));
-  RET(launch_axpby(c, 4, s->pi_z->e, 0.0, s->pi_z->e, uzn->e, nullptr, c->n_fff));
+  RET(pc2vel3_hist(s, psi, chi, 4, 0.0));
   return visc_cn(c, psi, chi, b, psi0, chi0, b0);
 }
 

@@ -329,8 +329,10 @@ __device__ __forceinline__ void fft4_core(double2 (&v1)[R1], double2 (&v2)[R2], 
 
 // VERFFT forward: PFP (r, m, z) -> PFF (r, m, k chopped), / NX.
 template <int R1, int R2>
-__global__ void __launch_bounds__(256) k_verfft_fwd4(KGrid g, const double2 *__restrict__ tw, const double2 *__restrict__ in,
-                                                     double2 *__restrict__ out) {
+__global__ void __launch_bounds__(256) k_verfft_fwd4(KGrid g, const double2 *__restrict__ tw, TransformJobs jobs) {
+  // grid layer z transforms field z of the batch
+  const double2 *__restrict__ in = jobs.in[blockIdx.z];
+  double2 *__restrict__ out = jobs.out[blockIdx.z];
   constexpr int N = R1 * R2, P = Fft4<R1, R2>::P;
   __shared__ double2 s[P * N];
   const int r = threadIdx.x % P, t = threadIdx.x / P;
@@ -485,18 +487,33 @@ __global__ void __launch_bounds__(256) k_horfft_bwd4(KGrid g, const double2 *__r
     default: break;                         \
   }
 
-int launch_verfft_fwd(mlg_ctx *c, const double2 *in, double2 *out) {
+int launch_verfft_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out) {
+  if (n < 1 || n > MAX_TRANSFORM_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
-  if (g.nx > 256) return launch_verfft_fwd_r2(c, in, out);
-  KScope ks(c, K_VERFFT_FWD, 16.0 * g.nr * g.ntchopdim * (double)(g.nx + g.nxchopdim));
-#define L_VF(a, b)                                                                                             \
-  hipLaunchKernelGGL((k_verfft_fwd4<a, b>), dim3((g.ndimr + Fft4<a, b>::P - 1) / Fft4<a, b>::P, g.ntchopdim), \
-                     dim3(256), 0, c->stream, g, c->d.tw_z, in, out)
+  if (g.nx > 256) {
+    for (int i = 0; i < n; i++) {
+      const int rc = launch_verfft_fwd_r2(c, in[i], out[i]);
+      if (rc) return rc;
+    }
+    return 0;
+  }
+  TransformJobs jobs;
+  for (int i = 0; i < n; i++) {
+    jobs.in[i] = in[i];
+    jobs.ln[i] = nullptr;
+    jobs.out[i] = out[i];
+  }
+  KScope ks(c, K_VERFFT_FWD, n * 16.0 * g.nr * g.ntchopdim * (double)(g.nx + g.nxchopdim));
+#define L_VF(a, b)                                                                                                \
+  hipLaunchKernelGGL((k_verfft_fwd4<a, b>), dim3((g.ndimr + Fft4<a, b>::P - 1) / Fft4<a, b>::P, g.ntchopdim, n), \
+                     dim3(256), 0, c->stream, g, c->d.tw_z, jobs)
   FFT4_SWITCH(g.nx, L_VF)
 #undef L_VF
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
+
+int launch_verfft_fwd(mlg_ctx *c, const double2 *in, double2 *out) { return launch_verfft_fwd_n(c, 1, &in, &out); }
 
 int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out) {
   if (n < 1 || n > MAX_TRANSFORM_JOBS) return MLG_E_ARG;

@@ -394,9 +394,13 @@ __device__ double2 uz_like_row(const KGrid &g, const double *bxm, const double2 
 // ---------------------------------------------------------------- PC2VEL
 // (:1789-1853): r u_r = i m psi + i k XXDX chi ; r u_th = -XXDX psi - m k chi ;
 // u_z = -(1-x) (1-x) DELSQH chi.
+// HIST: the velocities only update the pressure-gradient histories H (in
+// rur, rup, uz): H = H - u (hmode 4) or H = a (H - u) (hmode 3), the
+// updates of the ETD steps, and are not stored
+template <bool HIST>
 __global__ void k_pc2vel(KGrid g, const double *bxm, const double *bxxdx, const double *ak, const double2 *psi,
                          const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *rur, double2 *rup,
-                         double2 *uz, double *l1, double *l2, double *l3) {
+                         double2 *uz, double *l1, double *l2, double *l3, int hmode, double ha) {
   const double lp = ln_psi[0], lc = ln_chi[0];
   GRID_STRIDE(fff_count(g)) {
     FIdx f = fdecode(g, idx);
@@ -416,11 +420,23 @@ __global__ void k_pc2vel(KGrid g, const double *bxm, const double *bxxdx, const 
       vp = sub2(make_double2(-xp.x, -xp.y), axc(mv * kv, chi[cb + f.n]));
       vz = uz_like_row(g, bxm, chi + cb, lc, f.mm, f.k, f.n, nn);
     }
-    rur[idx] = vr;
-    rup[idx] = vp;
-    uz[idx] = vz;
+    if (HIST) {
+      if (hmode == 3) {
+        rur[idx] = axc(ha, sub2(rur[idx], vr));
+        rup[idx] = axc(ha, sub2(rup[idx], vp));
+        uz[idx] = axc(ha, sub2(uz[idx], vz));
+      } else {
+        rur[idx] = sub2(rur[idx], vr);
+        rup[idx] = sub2(rup[idx], vp);
+        uz[idx] = sub2(uz[idx], vz);
+      }
+    } else {
+      rur[idx] = vr;
+      rup[idx] = vp;
+      uz[idx] = vz;
+    }
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (!HIST && blockIdx.x == 0 && threadIdx.x == 0) {
     l1[0] = 0.0;
     l2[0] = 0.0;
     l3[0] = 0.0;
@@ -429,8 +445,16 @@ __global__ void k_pc2vel(KGrid g, const double *bxm, const double *bxxdx, const 
 int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *rur,
                   double2 *rup, double2 *uz, double *ln_rur, double *ln_rup, double *ln_uz) {
   KGrid g = kgrid(c);
-  hipLaunchKernelGGL(k_pc2vel, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bxm, c->d.bxxdx, c->d.ak, psi,
-                     ln_psi, chi, ln_chi, rur, rup, uz, ln_rur, ln_rup, ln_uz);
+  hipLaunchKernelGGL(k_pc2vel<false>, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bxm, c->d.bxxdx, c->d.ak,
+                     psi, ln_psi, chi, ln_chi, rur, rup, uz, ln_rur, ln_rup, ln_uz, 0, 0.0);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+int launch_pc2vel_hist(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
+                       double2 *hr, double2 *hp, double2 *hz, int hmode, double ha) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_pc2vel<true>, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bxm, c->d.bxxdx, c->d.ak,
+                     psi, ln_psi, chi, ln_chi, hr, hp, hz, nullptr, nullptr, nullptr, hmode, ha);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

@@ -163,6 +163,7 @@ struct TransformJobs {
 };
 int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double *const *ln_in, double2 *const *out);
 int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out);
+int launch_verfft_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out);
 int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out);
 int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *psi, double2 *chi);
 // ---- FFTs (k_fft.hip)
@@ -199,6 +200,10 @@ int launch_grad(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *rr, d
                 double2 *z, double *ln_z);
 int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
                   double2 *rur, double2 *rup, double2 *uz, double *ln_rur, double *ln_rup, double *ln_uz);
+// PC2VEL whose velocities only update the histories: H = H - u (hmode 4) or
+// H = ha (H - u) (hmode 3), the AXPBY modes of the same names
+int launch_pc2vel_hist(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
+                       double2 *hr, double2 *hp, double2 *hz, int hmode, double ha);
 int launch_pc2vor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
                   double2 *ror, double2 *rop, double2 *oz, double *ln_ror, double *ln_rop, double *ln_oz);
 int launch_fix_at_infinity(mlg_ctx *c, double2 *a);
