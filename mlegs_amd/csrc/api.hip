@@ -1116,16 +1116,12 @@ static int visc1_rhs(mlg_field *a, double nu, double dt, mlg_field *wk, double *
 // side WK = -DEL2(AO) - ALP*A, then A = IHELM(WK, ALP) (batched)
 static int visc2_rhs(mlg_field *a, const mlg_field *ao, double nu, double dt, mlg_field *wk, double *alp) {
   if (nu == 0.0) return 5;
+  if (ao->space != MLG_FFF) return 3;  // DEL2's input check
   mlg_ctx *c = a->ctx;
   *alp = 2 / nu / dt;
-  RET(mlg_chopset(c, 2));
-  int rc = mlg_del2(ao, wk);
-  mlg_chopset(c, -2);
-  RET(rc);
-  RET(launch_chopdo(c, wk->e, MLG_FFF));
-  // WK = -WK - ALP*A
-  RET(launch_axpby(c, 5, wk->e, *alp, a->e, nullptr, nullptr, c->n_fff));
-  return launch_set_ln(c, wk->ln, a->ln, -*alp, 0.0);
+  // CHOPSET(2); WK = DEL2(AO); CHOPSET(-2); CHOPDO(WK); WK = -WK - ALP*A in one pass
+  wk->space = MLG_FFF;
+  return launch_visc2_rhs(c, ao->e, ao->ln, a->e, a->ln, *alp, wk->e, wk->ln);
 }
 
 // psi, chi (NU) and b (KAPPA, when non-zero) are independent: their right-

@@ -310,6 +310,44 @@ int launch_del2(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, do
   return 0;
 }
 
+// VISC2's right-hand side in one pass (mod_march.f90:1579-1636):
+//   CHOPSET(2); WK = DEL2(AO); CHOPSET(-2); CHOPDO(WK); WK = -WK - ALP*A;
+//   WK%LN = -ALP*A%LN
+// with DEL2 evaluated on the truncation NRCHOP+2 (nrchop2) at the positions
+// the base truncation keeps, and DEL2's top-mode check of AO at NRCHOP+2.
+__global__ void k_visc2_rhs(KGrid g, int nrchop2, const double *bd, const double *ak, const double2 *ao,
+                            const double *ln_ao, const double2 *a, const double *ln_a, double alp, double2 *wk,
+                            double *ln_wk, int *err) {
+  const double ln = ln_ao[0];
+  GRID_STRIDE(fff_count(g)) {
+    FIdx f = fdecode(g, idx);
+    const int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv), nn2 = s_nrchops(nrchop2, mv);
+    double2 v = zero2();
+    if (f.n < nn && f.mm < g.ntchop) v = del2_row(g, bd, ak, ao + fcol(g, f.mm, f.k), ln, f.mm, f.k, f.n, nn2);
+    wk[idx] = sub2(make_double2(-v.x, -v.y), axc(alp, a[idx]));
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    ln_wk[0] = ln_a[0] * -alp + 0.0;
+    if (g.ntchopdim >= 2) {
+      const int nt = s_nrchops(nrchop2, g.minc);
+      if (nt >= 1) {
+        double2 t = ao[(size_t)(nt - 1) + (size_t)g.nrchopdim * 1];
+        if (hypot(t.x, t.y) > 1.0e-15) atomicCAS(err, 0, 3);
+      }
+    }
+  }
+}
+int launch_visc2_rhs(mlg_ctx *c, const double2 *ao, const double *ln_ao, const double2 *a, const double *ln_a, double alp,
+                     double2 *wk, double *ln_wk) {
+  KGrid g = kgrid(c);
+  const int nrchop2 = g.nrchop + 2;
+  if (nrchop2 > g.nrchopdim) return 1;  // CHOPSET: NRCHOP TOO LARGE (mod_scalar3.f90:183-190)
+  hipLaunchKernelGGL(k_visc2_rhs, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, nrchop2, c->d.bdel2h, c->d.ak, ao,
+                     ln_ao, a, ln_a, alp, wk, ln_wk, c->d.err);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
 // DEL2's exactness check (:778-790): the top mode of m=1 at k=0 must vanish.
 __global__ void k_check_top_m1(KGrid g, const double2 *a, int *err) {
   if (g.ntchopdim < 2) return;
@@ -513,29 +551,30 @@ int launch_pc2vor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const do
 // ---------------------------------------------------------------- value at infinity
 // A(0,0,k) -= sum_n AT1(n) A(n,0,k) / AT1(0)   (IDELSQH :222, IDEL2 :987-992,
 // IDEL2LN :1154-1159, PROJECT :2541-2543)
-__global__ void k_fix_inf(KGrid g, const double *at1, double2 *a) {
-  int k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k >= g.nxchopdim) return;
-  const double2 *col = a + (size_t)g.nrchopdim * (size_t)g.ntchopdim * k;
-  double2 s = zero2();
-  for (int nb = 0; nb < g.nrchop; nb += 16) {  // loads in blocks of 16, sum in ascending n
-    double2 v[16];
-    double w[16];
-#pragma unroll
-    for (int q = 0; q < 16; q++)
-      if (nb + q < g.nrchop) {
-        v[q] = col[nb + q];
-        w[q] = at1[nb + q];
-      }
-#pragma unroll
-    for (int q = 0; q < 16; q++)
-      if (nb + q < g.nrchop) s = add2(s, axc(w[q], v[q]));
+// One 64-thread block per k: the m = 0 column and AT1 are staged in LDS with
+// coalesced loads, lane 0 sums in ascending n.
+__global__ void __launch_bounds__(64) k_fix_inf(KGrid g, const double *at1, double2 *a) {
+  extern __shared__ __align__(16) double fsm[];
+  double2 *v = reinterpret_cast<double2 *>(fsm);  // [nrchop]
+  double *w = fsm + 2 * g.nrchop;                 // [nrchop]
+  const int k = blockIdx.x;
+  double2 *col = a + (size_t)g.nrchopdim * (size_t)g.ntchopdim * k;
+  for (int n = threadIdx.x; n < g.nrchop; n += 64) {
+    v[n] = col[n];
+    w[n] = at1[n];
   }
-  a[(size_t)g.nrchopdim * (size_t)g.ntchopdim * k] = sub2(col[0], make_double2(s.x / g.at1_0, s.y / g.at1_0));
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 s = zero2();
+    for (int n = 0; n < g.nrchop; n++) s = add2(s, axc(w[n], v[n]));
+    col[0] = sub2(v[0], make_double2(s.x / g.at1_0, s.y / g.at1_0));
+  }
 }
 int launch_fix_at_infinity(mlg_ctx *c, double2 *a) {
   KGrid g = kgrid(c);
-  hipLaunchKernelGGL(k_fix_inf, dim3((g.nxchopdim + 63) / 64), dim3(64), 0, c->stream, g, c->d.at1, a);
+  if (g.nrchop < 1) return 0;
+  hipLaunchKernelGGL(k_fix_inf, dim3(g.nxchopdim), dim3(64), sizeof(double) * 3 * (size_t)g.nrchop, c->stream, g, c->d.at1,
+                     a);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
