@@ -66,6 +66,8 @@ static int nblocks(long total) {
   if (b > 65536) b = 65536;
   return (int)(b > 0 ? b : 1);
 }
+// one block per (m, k) column of an FFF field
+static inline dim3 column_grid(const KGrid &g) { return dim3(g.ntchopdim, g.nxchopdim); }
 #define GRID_STRIDE(total) for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < (total); idx += (long)gridDim.x * blockDim.x)
 
 // ---------------------------------------------------------------- basics
@@ -315,18 +317,24 @@ int launch_del2(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, do
 //   WK%LN = -ALP*A%LN
 // with DEL2 evaluated on the truncation NRCHOP+2 (nrchop2) at the positions
 // the base truncation keeps, and DEL2's top-mode check of AO at NRCHOP+2.
-__global__ void k_visc2_rhs(KGrid g, int nrchop2, const double *bd, const double *ak, const double2 *ao,
-                            const double *ln_ao, const double2 *a, const double *ln_a, double alp, double2 *wk,
-                            double *ln_wk, int *err) {
+__global__ void __launch_bounds__(64) k_visc2_rhs(KGrid g, int nrchop2, const double *bd, const double *ak,
+                                                  const double2 *ao, const double *ln_ao, const double2 *a,
+                                                  const double *ln_a, double alp, double2 *wk, double *ln_wk, int *err) {
+  extern __shared__ __align__(16) double2 csm[];
   const double ln = ln_ao[0];
-  GRID_STRIDE(fff_count(g)) {
-    FIdx f = fdecode(g, idx);
-    const int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv), nn2 = s_nrchops(nrchop2, mv);
+  const int mm = blockIdx.x, k = blockIdx.y;
+  const int mv = mm * g.minc;
+  const int nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0, nn2 = s_nrchops(nrchop2, mv);
+  const int nl = nn > 0 ? (nn2 < g.nrchopdim ? nn2 : g.nrchopdim) : 0;  // rows DEL2 reads
+  const size_t cb = fcol(g, mm, k);
+  for (int n = threadIdx.x; n < nl; n += blockDim.x) csm[n] = ao[cb + n];
+  __syncthreads();
+  for (int n = threadIdx.x; n < g.nrchopdim; n += blockDim.x) {
     double2 v = zero2();
-    if (f.n < nn && f.mm < g.ntchop) v = del2_row(g, bd, ak, ao + fcol(g, f.mm, f.k), ln, f.mm, f.k, f.n, nn2);
-    wk[idx] = sub2(make_double2(-v.x, -v.y), axc(alp, a[idx]));
+    if (n < nn) v = del2_row(g, bd, ak, csm, ln, mm, k, n, nn2);
+    wk[cb + n] = sub2(make_double2(-v.x, -v.y), axc(alp, a[cb + n]));
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (mm == 0 && k == 0 && threadIdx.x == 0) {
     ln_wk[0] = ln_a[0] * -alp + 0.0;
     if (g.ntchopdim >= 2) {
       const int nt = s_nrchops(nrchop2, g.minc);
@@ -342,8 +350,8 @@ int launch_visc2_rhs(mlg_ctx *c, const double2 *ao, const double *ln_ao, const d
   KGrid g = kgrid(c);
   const int nrchop2 = g.nrchop + 2;
   if (nrchop2 > g.nrchopdim) return 1;  // CHOPSET: NRCHOP TOO LARGE (mod_scalar3.f90:183-190)
-  hipLaunchKernelGGL(k_visc2_rhs, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, nrchop2, c->d.bdel2h, c->d.ak, ao,
-                     ln_ao, a, ln_a, alp, wk, ln_wk, c->d.err);
+  hipLaunchKernelGGL(k_visc2_rhs, column_grid(g), dim3(64), sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, nrchop2,
+                     c->d.bdel2h, c->d.ak, ao, ln_ao, a, ln_a, alp, wk, ln_wk, c->d.err);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -365,29 +373,34 @@ int launch_check_top_m1(mlg_ctx *c, const double2 *a) {
 
 // ---------------------------------------------------------------- GRAD
 // (:1727-1786): (r d/dr b, d/dtheta b, d/dz b) = (XXDX b, i m b, i k b)
-__global__ void k_grad(KGrid g, const double *bxxdx, const double *ak, const double2 *a, const double *ln_a, double2 *rr,
-                       double *ln_rr, double2 *rp, double *ln_rp, double2 *z, double *ln_z) {
+__global__ void __launch_bounds__(64) k_grad(KGrid g, const double *bxxdx, const double *ak, const double2 *a,
+                                             const double *ln_a, double2 *rr, double *ln_rr, double2 *rp, double *ln_rp,
+                                             double2 *z, double *ln_z) {
+  extern __shared__ __align__(16) double2 csm[];
   const double ln = ln_a[0];
-  GRID_STRIDE(fff_count(g)) {
-    FIdx f = fdecode(g, idx);
-    int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv);
+  const int mm = blockIdx.x, k = blockIdx.y;
+  const int mv = mm * g.minc, nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0;
+  const size_t cb = fcol(g, mm, k);
+  for (int n = threadIdx.x; n < nn; n += blockDim.x) csm[n] = a[cb + n];
+  __syncthreads();
+  const double kv = ak[mm + (size_t)g.ntchopdim * k];
+  for (int n = threadIdx.x; n < g.nrchopdim; n += blockDim.x) {
     double2 vr = zero2(), vp = zero2(), vz = zero2();
-    if (f.n < nn && f.mm < g.ntchop) {
-      const double2 *col = a + fcol(g, f.mm, f.k);
-      vr = band3_row(bxxdx + (size_t)f.mm * g.nband * 3, col, f.n, nn);
-      double2 x = col[f.n];
+    if (n < nn) {
+      vr = band3_row(bxxdx + (size_t)mm * g.nband * 3, csm, n, nn);
+      double2 x = csm[n];
       vp = ix((double)mv, x);
-      vz = ix(ak[f.mm + (size_t)g.ntchopdim * f.k], x);
+      vz = ix(kv, x);
     }
-    if (f.mm == 0 && f.k == 0) {
-      if (f.n == 0) vr.x = vr.x + ln / g.norm0;
-      if (f.n == 1) vr.x = vr.x + ln / g.norm1;
+    if (mm == 0 && k == 0) {
+      if (n == 0) vr.x = vr.x + ln / g.norm0;
+      if (n == 1) vr.x = vr.x + ln / g.norm1;
     }
-    rr[idx] = vr;
-    rp[idx] = vp;
-    z[idx] = vz;
+    rr[cb + n] = vr;
+    rp[cb + n] = vp;
+    z[cb + n] = vz;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (mm == 0 && k == 0 && threadIdx.x == 0) {
     ln_rr[0] = 0.0;
     ln_rp[0] = 0.0;
     ln_z[0] = 0.0;
@@ -396,8 +409,8 @@ __global__ void k_grad(KGrid g, const double *bxxdx, const double *ak, const dou
 int launch_grad(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *rr, double *ln_rr, double2 *rp, double *ln_rp,
                 double2 *z, double *ln_z) {
   KGrid g = kgrid(c);
-  hipLaunchKernelGGL(k_grad, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bxxdx, c->d.ak, a, ln_a, rr, ln_rr,
-                     rp, ln_rp, z, ln_z);
+  hipLaunchKernelGGL(k_grad, column_grid(g), dim3(64), sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, c->d.bxxdx,
+                     c->d.ak, a, ln_a, rr, ln_rr, rp, ln_rp, z, ln_z);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -429,6 +442,43 @@ __device__ double2 uz_like_row(const KGrid &g, const double *bxm, const double2 
   return make_double2(-o.x, -o.y);
 }
 
+// Column kernels: one 64-thread block per (m, k) column.  The column's
+// inputs and the intermediate rows (DELSQH, MULXM, DEL2) are staged in LDS,
+// and every row is formed by the same operations as the row functions above,
+// reading LDS instead of memory.
+// u_z-like rows -(1-x)^2 [(1-x)^-2 del_h^2 s] as two truncated MULXM passes
+// over DELSQH(s) (with s's log image in row 0 of m0k0):
+// D[j] = DELSQH row j of s, W[l] = MULXM row l of D, for j, l < nn
+__device__ void uz_stage(const KGrid &g, const double *bxm, const double2 *s, double ln, int mm, int k, int nn, double2 *D,
+                         double2 *W) {
+  const int mv = mm * g.minc;
+  const double *xt = bxm + (size_t)mm * g.nband * 3;
+  for (int j = threadIdx.x; j < nn; j += blockDim.x) {
+    const int N = mv + j;
+    const double2 x = s[j];
+    const double cf = (double)(N * (N + 1)) / g.ell2;
+    double2 v = make_double2(-x.x * cf, -x.y * cf);
+    if (mm == 0 && k == 0 && j == 0) v = make_double2(ln / g.ell2 / g.norm0, 0.0);
+    D[j] = v;
+  }
+  __syncthreads();
+  for (int l = threadIdx.x; l < nn; l += blockDim.x) {
+    const double *r = xt + 3 * l;
+    double2 o = axc(r[1], D[l]);
+    if (l >= 1) o = add2(o, axc(r[0], D[l - 1]));
+    if (l + 1 < nn) o = add2(o, axc(r[2], D[l + 1]));
+    W[l] = o;
+  }
+  __syncthreads();
+}
+// row n of the u_z-like result: -(MULXM row n of W)
+__device__ __forceinline__ double2 uz_from_w(const KGrid &g, const double *bxm, const double2 *W, int mm, int n, int nn) {
+  const double *r = bxm + (size_t)mm * g.nband * 3 + 3 * n;
+  double2 o = axc(r[1], W[n]);
+  if (n >= 1) o = add2(o, axc(r[0], W[n - 1]));
+  if (n + 1 < nn) o = add2(o, axc(r[2], W[n + 1]));
+  return make_double2(-o.x, -o.y);
+}
 // ---------------------------------------------------------------- PC2VEL
 // (:1789-1853): r u_r = i m psi + i k XXDX chi ; r u_th = -XXDX psi - m k chi ;
 // u_z = -(1-x) (1-x) DELSQH chi.
@@ -501,39 +551,49 @@ int launch_pc2vel_hist(mlg_ctx *c, const double2 *psi, const double *ln_psi, con
 // (:1856-1919): with Q = DEL2 chi,
 //   r om_r = -i m Q + i k XXDX psi ; r om_th = XXDX Q - k m psi ;
 //   om_z = -(1-x)(1-x) DELSQH psi.
-__global__ void k_pc2vor(KGrid g, const double *bxm, const double *bxxdx, const double *bd, const double *ak,
-                         const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *ror,
-                         double2 *rop, double2 *oz, double *l1, double *l2, double *l3) {
+__global__ void __launch_bounds__(64) k_pc2vor(KGrid g, const double *bxm, const double *bxxdx, const double *bd,
+                                               const double *ak, const double2 *psi, const double *ln_psi,
+                                               const double2 *chi, const double *ln_chi, double2 *ror, double2 *rop,
+                                               double2 *oz, double *l1, double *l2, double *l3) {
+  extern __shared__ __align__(16) double2 csm[];
+  const int mm = blockIdx.x, k = blockIdx.y;
+  const int mv = mm * g.minc, nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0;
+  const size_t cb = fcol(g, mm, k);
+  double2 *P = csm, *X = P + g.nrchopdim, *Q = X + g.nrchopdim, *D = Q + g.nrchopdim, *W = D + g.nrchopdim;
   const double lp = ln_psi[0], lc = ln_chi[0];
-  GRID_STRIDE(fff_count(g)) {
-    FIdx f = fdecode(g, idx);
-    int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv);
+  for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+    P[n] = psi[cb + n];
+    X[n] = chi[cb + n];
+  }
+  __syncthreads();
+  for (int n = threadIdx.x; n < nn; n += blockDim.x) Q[n] = del2_row(g, bd, ak, X, lc, mm, k, n, nn);
+  uz_stage(g, bxm, P, lp, mm, k, nn, D, W);  // (its first barrier also orders Q)
+  const double *xx = bxxdx + (size_t)mm * g.nband * 3;
+  const double kv = ak[mm + (size_t)g.ntchopdim * k];
+  for (int n = threadIdx.x; n < g.nrchopdim; n += blockDim.x) {
     double2 vr = zero2(), vp = zero2(), vz = zero2();
-    if (f.n < nn && f.mm < g.ntchop) {
-      const size_t cb = fcol(g, f.mm, f.k);
-      const double *xx = bxxdx + (size_t)f.mm * g.nband * 3;
-      double kv = ak[f.mm + (size_t)g.ntchopdim * f.k];
-      double2 q = del2_row(g, bd, ak, chi + cb, lc, f.mm, f.k, f.n, nn);
-      double2 xp = band3_row(xx, psi + cb, f.n, nn);
-      if (f.mm == 0 && f.k == 0) {
-        if (f.n == 0) xp.x = xp.x + lp / g.norm0;
-        if (f.n == 1) xp.x = xp.x + lp / g.norm1;
+    if (n < nn) {
+      const double2 q = Q[n];
+      double2 xp = band3_row(xx, P, n, nn);
+      if (mm == 0 && k == 0) {
+        if (n == 0) xp.x = xp.x + lp / g.norm0;
+        if (n == 1) xp.x = xp.x + lp / g.norm1;
       }
       double2 imq = ix((double)mv, q);
       vr = add2(make_double2(-imq.x, -imq.y), ix(kv, xp));
       // XXDX of Q (Q carries no log term)
-      const double *r = xx + 3 * f.n;
+      const double *r = xx + 3 * n;
       double2 xq = axc(r[1], q);
-      if (f.n >= 1) xq = add2(xq, axc(r[0], del2_row(g, bd, ak, chi + cb, lc, f.mm, f.k, f.n - 1, nn)));
-      if (f.n + 1 < nn) xq = add2(xq, axc(r[2], del2_row(g, bd, ak, chi + cb, lc, f.mm, f.k, f.n + 1, nn)));
-      vp = sub2(xq, axc(kv * mv, psi[cb + f.n]));
-      vz = uz_like_row(g, bxm, psi + cb, lp, f.mm, f.k, f.n, nn);
+      if (n >= 1) xq = add2(xq, axc(r[0], Q[n - 1]));
+      if (n + 1 < nn) xq = add2(xq, axc(r[2], Q[n + 1]));
+      vp = sub2(xq, axc(kv * mv, P[n]));
+      vz = uz_from_w(g, bxm, W, mm, n, nn);
     }
-    ror[idx] = vr;
-    rop[idx] = vp;
-    oz[idx] = vz;
+    ror[cb + n] = vr;
+    rop[cb + n] = vp;
+    oz[cb + n] = vz;
   }
-  if (blockIdx.x == 0 && threadIdx.x == 0) {
+  if (mm == 0 && k == 0 && threadIdx.x == 0) {
     l1[0] = 0.0;
     l2[0] = 0.0;
     l3[0] = 0.0;
@@ -542,8 +602,8 @@ __global__ void k_pc2vor(KGrid g, const double *bxm, const double *bxxdx, const 
 int launch_pc2vor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *ror,
                   double2 *rop, double2 *oz, double *ln_ror, double *ln_rop, double *ln_oz) {
   KGrid g = kgrid(c);
-  hipLaunchKernelGGL(k_pc2vor, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bxm, c->d.bxxdx, c->d.bdel2h,
-                     c->d.ak, psi, ln_psi, chi, ln_chi, ror, rop, oz, ln_ror, ln_rop, ln_oz);
+  hipLaunchKernelGGL(k_pc2vor, column_grid(g), dim3(64), 5 * sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, c->d.bxm,
+                     c->d.bxxdx, c->d.bdel2h, c->d.ak, psi, ln_psi, chi, ln_chi, ror, rop, oz, ln_ror, ln_rop, ln_oz);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
