@@ -122,8 +122,12 @@ int launch_force_products(mlg_ctx *c, const double2 *rur, const double2 *rup, co
 __global__ void __launch_bounds__(256) k_vel2pln(KGrid g, const double *__restrict__ pf, const double *__restrict__ w,
                                                  const double *__restrict__ lognorm, double2 *__restrict__ rup,
                                                  double *__restrict__ psi_ln, int ln_mode, double ln) {
-  extern __shared__ double crs[];  // [nn] projections, then the log value
+  extern __shared__ double crs[];  // [nn] projections, then the log value; Re rup [nr]; w [nrh]
   const int nn = g.nrchop;         // NRCHOPS(1) at m = 0
+  double *ru = crs + nn + 1, *ws = ru + g.nr;
+  for (int i = threadIdx.x; i < g.nr; i += blockDim.x) ru[i] = rup[i].x;
+  for (int i = threadIdx.x; i < g.nrh; i += blockDim.x) ws[i] = w[i];
+  __syncthreads();
   for (int n = threadIdx.x; n < nn; n += blockDim.x) {
     double cr = 0.0;
     const double *pfn = pf + (size_t)n * g.nrh;
@@ -132,9 +136,9 @@ __global__ void __launch_bounds__(256) k_vel2pln(KGrid g, const double *__restri
 #pragma unroll
       for (int u = 0; u < 8; u++)
         if (ib + u < g.nrh) {
-          p[u] = rup[ib + u].x;
-          q[u] = rup[g.nr - 1 - ib - u].x;
-          wv[u] = w[ib + u];
+          p[u] = ru[ib + u];
+          q[u] = ru[g.nr - 1 - ib - u];
+          wv[u] = ws[ib + u];
           t[u] = pfn[ib + u];
         }
 #pragma unroll
@@ -167,8 +171,8 @@ __global__ void __launch_bounds__(256) k_vel2pln(KGrid g, const double *__restri
 int launch_vel2pln(mlg_ctx *c, double2 *rup, double *psi_ln, int ln_mode, double ln) {
   KGrid g = kgrid(c);
   if (!rup || !psi_ln) return MLG_E_ARG;
-  hipLaunchKernelGGL(k_vel2pln, dim3(1), dim3(256), sizeof(double) * (g.nrchop + 1), c->stream, g, c->d.pf, c->d.w,
-                     c->d.lognorm, rup, psi_ln, ln_mode, ln);
+  hipLaunchKernelGGL(k_vel2pln, dim3(1), dim3(256), sizeof(double) * (g.nrchop + 1 + g.nr + g.nrh), c->stream, g, c->d.pf,
+                     c->d.w, c->d.lognorm, rup, psi_ln, ln_mode, ln);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
