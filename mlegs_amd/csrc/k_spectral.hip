@@ -826,6 +826,103 @@ __device__ void band_apply(int nd, const double2 *xin, double2 *x, const double 
   }
 }
 
+// band_apply for NJ right-hand sides that share one factor set: the factor
+// rows are loaded once and the NJ substitutions run side by side (each with
+// band_apply's arithmetic).
+template <int NJ, int NE, int C0, int SB>
+__device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], double2 *const (&x)[NJ], const double *__restrict__ F,
+                             long ncls, long cls) {
+  constexpr int KU = NE - 1 - C0;
+  double2 xh[NJ][C0];
+#pragma unroll
+  for (int jj = 0; jj < NJ; jj++)
+#pragma unroll
+    for (int q = 0; q < C0; q++) xh[jj][q] = zero2();
+  for (int nb = 0; nb < nd; nb += SB) {
+    double2 rhs[SB][NJ];
+    double lm[SB][C0], rp[SB];
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int nn = nb + q;
+      if (nn < nd) {
+        const double *fr = F + ((size_t)nn * ncls + cls) * FSTRIDE;
+#pragma unroll
+        for (int jj = 0; jj < NJ; jj++) rhs[q][jj] = xin[jj][nn];
+#pragma unroll
+        for (int j = 0; j < C0; j++) lm[q][j] = fr[j];
+        rp[q] = fr[NE - 1];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int nn = nb + q;
+      if (nn < nd) {
+        const int nne = nn < C0 ? nn : C0;
+#pragma unroll
+        for (int jj = 0; jj < NJ; jj++) {
+          double2 v = rhs[q][jj];
+#pragma unroll
+          for (int j = 1; j <= C0; j++)
+            if (j <= nne) v = sub2(v, axc(lm[q][j - 1], xh[jj][j - 1]));
+#pragma unroll
+          for (int t = C0 - 1; t >= 1; t--) xh[jj][t] = xh[jj][t - 1];
+          xh[jj][0] = v;
+          rhs[q][jj] = axc(rp[q], v);
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+      if (nb + q < nd)
+#pragma unroll
+        for (int jj = 0; jj < NJ; jj++) x[jj][nb + q] = rhs[q][jj];
+  }
+  double2 yh[NJ][KU];
+#pragma unroll
+  for (int jj = 0; jj < NJ; jj++)
+#pragma unroll
+    for (int u = 0; u < KU; u++) yh[jj][u] = zero2();
+  for (int nt = nd - 1; nt >= 0; nt -= SB) {
+    double2 v0[SB][NJ];
+    double um[SB][KU];
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int nn = nt - q;
+      if (nn >= 0) {
+        const double *fr = F + ((size_t)nn * ncls + cls) * FSTRIDE;
+#pragma unroll
+        for (int jj = 0; jj < NJ; jj++) v0[q][jj] = x[jj][nn];
+#pragma unroll
+        for (int u = 0; u < KU; u++) um[q][u] = fr[C0 + u];
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int nn = nt - q;
+      if (nn >= 0) {
+#pragma unroll
+        for (int jj = 0; jj < NJ; jj++) {
+          double2 v = v0[q][jj];
+          if (nn <= nd - 2) {
+#pragma unroll
+            for (int i = 1; i <= KU; i++)
+              if (nn + i < nd) v = sub2(v, axc(um[q][i - 1], yh[jj][i - 1]));
+          }
+          v0[q][jj] = v;
+#pragma unroll
+          for (int u = KU - 1; u >= 1; u--) yh[jj][u] = yh[jj][u - 1];
+          yh[jj][0] = v;
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+      if (nt - q >= 0)
+#pragma unroll
+        for (int jj = 0; jj < NJ; jj++) x[jj][nt - q] = v0[q][jj];
+  }
+}
+
 // x[i] = x[i-1] for i = n-1 .. 1 (x in LDS)
 __device__ void shift_down_one(double2 *x, int n) {
   for (int i = n - 1; i >= 1; i--) x[i] = x[i - 1];
@@ -918,8 +1015,26 @@ __device__ __forceinline__ int solve_kind(int op, int mm, int k) {
   return 0;
 }
 
-__device__ void apply_penta(const KGrid &g, int op, int has_ln, const double2 *b, double2 *a, double *ln_a,
-                            const double *__restrict__ F, long sys) {
+struct BandJob {
+  const double2 *b;  // right-hand side (may be a)
+  double2 *a;
+  double *ln_a;
+  const double *ln_b;
+  const double *F;
+  double alp;
+};
+constexpr int MAX_BAND_JOBS = 3;
+struct BandJobs {
+  BandJob j[MAX_BAND_JOBS];
+  int n;                                   // jobs
+  int ng;                                  // groups of jobs sharing a factor set
+  int gcount[MAX_BAND_JOBS];               // jobs of group g
+  int gjob[MAX_BAND_JOBS][MAX_BAND_JOBS];  // and their indices
+};
+
+// The systems of one (m, k) for the NJ jobs of group grp (one factor set).
+template <int NJ>
+__device__ void apply_penta_n(const KGrid &g, int op, int has_ln, const BandJobs &jobs, int grp, long sys) {
   const long nsys = (long)g.ntchopdim * g.nxchopdim;
   const long ncls = (long)g.ntchopdim * g.nxchop;
   if (sys >= nsys) return;
@@ -928,13 +1043,25 @@ __device__ void apply_penta(const KGrid &g, int op, int has_ln, const double2 *b
   if (nn < 1 || mm >= g.ntchop || solve_kind(op, mm, k) != 0) return;
   const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
   const size_t col = (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
-  double2 *x = a + col;
-  band_apply<5, 2, 16>(nn, b + col, x, F, ncls, mm + (long)g.ntchopdim * kabs);
+  const double2 *xin[NJ];
+  double2 *x[NJ];
+#pragma unroll
+  for (int jj = 0; jj < NJ; jj++) {
+    const BandJob &J = jobs.j[jobs.gjob[grp][jj]];
+    xin[jj] = J.b + col;
+    x[jj] = J.a + col;
+  }
+  const double *F = jobs.j[jobs.gjob[grp][0]].F;
+  band_apply_n<NJ, 5, 2, (NJ == 1 ? 16 : 8)>(nn, xin, x, F, ncls, mm + (long)g.ntchopdim * kabs);
   if (mm == 0 && k == 0) {
-    if (op == 0) ln_a[0] = has_ln ? x[0].x * g.norm0 : 0.0;
-    if (op == 1) {
-      ln_a[0] = x[0].x * g.norm0;
-      x[0] = zero2();
+#pragma unroll
+    for (int jj = 0; jj < NJ; jj++) {
+      double *ln_a = jobs.j[jobs.gjob[grp][jj]].ln_a;
+      if (op == 0) ln_a[0] = has_ln ? x[jj][0].x * g.norm0 : 0.0;
+      if (op == 1) {
+        ln_a[0] = x[jj][0].x * g.norm0;
+        x[jj][0] = zero2();
+      }
     }
   }
 }
@@ -1007,27 +1134,24 @@ __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, 
 // because they are the longest), the remaining blocks the penta-diagonal
 // systems (one thread each), so both sets run side by side; grid row y
 // solves field y of a batch of independent solves.
-struct BandJob {
-  const double2 *b;  // right-hand side (may be a)
-  double2 *a;
-  double *ln_a;
-  const double *ln_b;
-  const double *F;
-  double alp;
-};
-constexpr int MAX_BAND_JOBS = 3;
-struct BandJobs {
-  BandJob j[MAX_BAND_JOBS];
-};
-
+// blocks [0, nm0 n): the m = 0 systems, one block per (k, job); then npenta
+// blocks per group of jobs that share a factor set, one thread per (m, k)
+// solving that group's right-hand sides together
 __global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, int op, int has_ln, double lnval, BandJobs jobs,
-                                                   int nm0) {
+                                                   int nm0, int npenta) {
   extern __shared__ __align__(16) double m0s[];
-  const BandJob &J = jobs.j[blockIdx.y];
-  if ((int)blockIdx.x < nm0) {
-    apply_m0(g, bxm, op, J.alp, has_ln, lnval, J.b, J.a, J.ln_a, J.ln_b, J.F, blockIdx.x, m0s);
-  } else {
-    apply_penta(g, op, has_ln, J.b, J.a, J.ln_a, J.F, (blockIdx.x - nm0) * (long)blockDim.x + threadIdx.x);
+  const int bid = blockIdx.x;
+  if (bid < nm0 * jobs.n) {
+    const BandJob &J = jobs.j[bid / nm0];
+    apply_m0(g, bxm, op, J.alp, has_ln, lnval, J.b, J.a, J.ln_a, J.ln_b, J.F, bid % nm0, m0s);
+    return;
+  }
+  const int pb = bid - nm0 * jobs.n, grp = pb / npenta;
+  const long sys = (long)(pb - grp * npenta) * blockDim.x + threadIdx.x;
+  switch (jobs.gcount[grp]) {
+    case 1: apply_penta_n<1>(g, op, has_ln, jobs, grp, sys); break;
+    case 2: apply_penta_n<2>(g, op, has_ln, jobs, grp, sys); break;
+    default: apply_penta_n<3>(g, op, has_ln, jobs, grp, sys); break;
   }
 }
 
@@ -1084,10 +1208,18 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, doub
     used[i] = F;
     jobs.j[i] = BandJob{b[i], a[i], ln_a[i], ln_b[i], F, alp[i]};
   }
+  jobs.n = n;
+  jobs.ng = 0;
+  for (int i = 0; i < n; i++) {
+    int gi = 0;
+    while (gi < jobs.ng && jobs.j[jobs.gjob[gi][0]].F != jobs.j[i].F) gi++;
+    if (gi == jobs.ng) jobs.gcount[jobs.ng++] = 0;
+    jobs.gjob[gi][jobs.gcount[gi]++] = i;
+  }
   KScope ks(c, K_BAND_SOLVE, n * 2.0 * 16.0 * fff_count(g));  // the coefficients read and written once
   const int npenta = (int)((nsys + 63) / 64), nm0 = (op != 1) ? g.nxchopdim : 0;
-  hipLaunchKernelGGL(k_band_apply, dim3(npenta + nm0, n), dim3(64), nm0 ? m0_lds_bytes(g) : 0, c->stream, g, c->d.bxm, op,
-                     has_ln, ln, jobs, nm0);
+  hipLaunchKernelGGL(k_band_apply, dim3(nm0 * n + npenta * jobs.ng), dim3(64), nm0 ? m0_lds_bytes(g) : 0, c->stream, g,
+                     c->d.bxm, op, has_ln, ln, jobs, nm0, npenta);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
