@@ -389,6 +389,7 @@ static mlg_field *tmp(mlg_ctx *c, int space) {
 static void rel(mlg_ctx *c, mlg_field *f) {
   if (f) c->pool.push_back(f);
 }
+
 // Temporaries of one routine.  get() hands out pool buffers; ready() checks
 // them and zeroes all their LN coefficients with one launch.
 struct Temps {
@@ -792,10 +793,18 @@ static int to_pfp(mlg_field *a) {
 // to_pfp of n independent FFF fields: one RTRAN backward launch and one
 // VERFFT backward launch for all of them (each field as mlg_rtran then
 // mlg_verfft treat it)
-static int to_pfp_n(mlg_ctx *c, int n, mlg_field *const *fs) {
+// keep[i] (i < nkeep, optional) receives the FFF contents of fs[i] (E and LN)
+// instead of discarding them; it must be a temporary of the caller.
+static int to_pfp_n(mlg_ctx *c, int n, mlg_field *const *fs, mlg_field *const *keep = nullptr, int nkeep = 0) {
   bool all_fff = n <= MAX_TRANSFORM_JOBS;
   for (int i = 0; i < n; i++) all_fff = all_fff && fs[i]->space == MLG_FFF;
   if (!all_fff) {
+    for (int i = 0; i < nkeep; i++) {  // unchopped copies
+      if (fs[i]->space != MLG_FFF) return 2;
+      RET(launch_copy(c, keep[i]->e, fs[i]->e, space_count(c, MLG_FFF)));
+      RET(launch_set_ln(c, keep[i]->ln, fs[i]->ln, 1.0, 0.0));
+      keep[i]->space = MLG_FFF;
+    }
     for (int i = 0; i < n; i++) RET(to_pfp(fs[i]));
     return 0;
   }
@@ -815,10 +824,15 @@ static int to_pfp_n(mlg_ctx *c, int n, mlg_field *const *fs) {
   double *lns[MAX_TRANSFORM_JOBS];
   for (int i = 0; i < n; i++) {
     swap_storage(fs[i], t[i]);
+    if (i < nkeep) {  // the FFF storage and its LN move to keep[i]
+      std::swap(keep[i]->e, t[i]->e);
+      std::swap(keep[i]->ln, fs[i]->ln);
+      keep[i]->space = MLG_FFF;
+    }
     fs[i]->space = MLG_PFF;
     lns[i] = fs[i]->ln;
     in[i] = fs[i]->e;
-    out[i] = t[i]->e;  // the FFF storage, free again
+    out[i] = t[i]->e;  // the FFF storage (or keep[i]'s), free again
   }
   RET(launch_zero_lns(c, lns, n));
   RET(launch_verfft_bwd_n(c, n, in, out));
@@ -875,16 +889,18 @@ extern "C" int mlg_force(const mlg_field *psi, const mlg_field *chi, const mlg_f
 // CALC_BOUSSI_NONLIN (src/mod_boussinesq.f90:235-314).  The velocity
 // arguments end in physical space, as in the reference, unless
 // `keep_velocity` (the time step, which overwrites them next) lets them stay
-// in PFP where the fused product kernel leaves them.
+// in PFP where the fused product kernel leaves them.  With `vel_fff` the
+// three spectral velocities given on entry are kept there (FFF, with LN):
+// the time step uses them in place of a second PC2VEL of the same PSI, CHI.
 static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *bn, mlg_field *ror,
-                       mlg_field *rop, mlg_field *oz, bool keep_velocity);
+                       mlg_field *rop, mlg_field *oz, bool keep_velocity, mlg_field *const *vel_fff = nullptr);
 extern "C" int mlg_nonlin(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *bn, mlg_field *ror,
                           mlg_field *rop, mlg_field *oz) {
   if (!b || !rur || !rup || !uz || !bn || !ror || !rop || !oz) return MLG_E_ARG;
   return nonlin_impl(b, rur, rup, uz, bn, ror, rop, oz, false);
 }
 static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *bn, mlg_field *ror,
-                       mlg_field *rop, mlg_field *oz, bool keep_velocity) {
+                       mlg_field *rop, mlg_field *oz, bool keep_velocity, mlg_field *const *vel_fff) {
   if (b->space != MLG_FFF) return 3;
   mlg_ctx *c = b->ctx;
   Temps T(c);
@@ -896,7 +912,7 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
   RET(rc);
   mlg_field *fs[9] = {rur, rup, uz, ror, rop, oz, rbr, rbp, bz};
   if (fused_products_supported(c)) {
-    RET(to_pfp_n(c, 9, fs));
+    RET(to_pfp_n(c, 9, fs, vel_fff, vel_fff ? 3 : 0));
     const double2 *in[10] = {rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, nullptr};
     double2 *out[4] = {ror->e, rop->e, oz->e, bn->e};
     RET(launch_fused_products(c, in, out, 0.0, 0.0, 0));
@@ -908,6 +924,12 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
       RET(mlg_horfft(uz, 1));
     }
   } else {
+    if (vel_fff)  // unchopped copies: the velocities carry NRCHOP+3 rows
+      for (int i = 0; i < 3; i++) {
+        RET(launch_copy(c, vel_fff[i]->e, fs[i]->e, space_count(c, MLG_FFF)));
+        RET(launch_set_ln(c, vel_fff[i]->ln, fs[i]->ln, 1.0, 0.0));
+        vel_fff[i]->space = MLG_FFF;
+      }
     for (int i = 0; i < 9; i++) RET(mlg_tofp(fs[i]));
     bn->space = MLG_PPP;
     RET(launch_force_products(c, rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, nullptr, bn->e, 0.0,
@@ -1114,7 +1136,7 @@ static int visc1_rhs(mlg_field *a, double nu, double dt, mlg_field *wk, double *
 
 // VISC2 (mod_march.f90:1579-1636): Crank-Nicolson diffusion; the right-hand
 // side WK = -DEL2(AO) - ALP*A, then A = IHELM(WK, ALP) (batched)
-static int visc2_rhs(mlg_field *a, const mlg_field *ao, double nu, double dt, mlg_field *wk, double *alp) {
+static int visc2_rhs(const mlg_field *a, const mlg_field *ao, double nu, double dt, mlg_field *wk, double *alp) {
   if (nu == 0.0) return 5;
   if (ao->space != MLG_FFF) return 3;  // DEL2's input check
   mlg_ctx *c = a->ctx;
@@ -1139,19 +1161,26 @@ static int visc_be(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, dou
   return ihelm_n(n, wk, f, alp);
 }
 
-static int visc_cn(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, const mlg_field *psio, const mlg_field *chio,
-                   const mlg_field *bo) {
+// VISC2 on (psi, chi, b): the current values are read from fa[] (which may
+// be the outputs f[] themselves), the previous ones from fo[]; all three
+// right-hand sides are formed before the batched solves write f[].
+static int visc_cn(mlg_ctx *c, mlg_field *const *f, const mlg_field *const *fa, const mlg_field *const *fo) {
   if (c->g.p.visc_sw != 1) return 0;
   Temps T(c);
   mlg_field *wk[3] = {T.get(), T.get(), T.get()};
   RET(T.ready());
-  mlg_field *f[3] = {psi, chi, b};
-  const mlg_field *fo[3] = {psio, chio, bo};
   const double nu[3] = {c->g.p.nu, c->g.p.nu, c->g.p.kappa};
   const int n = c->g.p.kappa != 0.0 ? 3 : 2;
   double alp[3];
-  for (int i = 0; i < n; i++) RET(visc2_rhs(f[i], fo[i], nu[i], c->g.p.dt, wk[i], &alp[i]));
-  return ihelm_n(n, wk, f, alp);
+  for (int i = 0; i < n; i++) RET(visc2_rhs(fa[i], fo[i], nu[i], c->g.p.dt, wk[i], &alp[i]));
+  mlg_field *out[3] = {f[0], f[1], f[2]};
+  return ihelm_n(n, wk, out, alp);
+}
+static int visc_cn(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, const mlg_field *psio, const mlg_field *chio,
+                   const mlg_field *bo) {
+  mlg_field *const f[3] = {psi, chi, b};
+  const mlg_field *const fo[3] = {psio, chio, bo};
+  return visc_cn(c, f, f, fo);
 }
 
 static bool filter_active(const mlg_ctx *c) { return c->g.p.visc_sw == 3 || c->g.p.hyper_cn; }
@@ -1218,13 +1247,6 @@ static int step_ab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *
   return hyperv_state(c, psi, chi, b, dt);
 }
 
-static int pc2vel3(mlg_ctx *c, const mlg_field *psi, const mlg_field *chi, mlg_field *rur, mlg_field *rup, mlg_field *uz) {
-  RET(mlg_chopset(c, 3));
-  int rc = mlg_pc2vel(psi, chi, rur, rup, uz);
-  mlg_chopset(c, -3);
-  return rc;
-}
-
 // PC2VEL at NRCHOP+3 followed by the pressure-gradient history update
 // (R_PI_R = R_PI_R - RUR, ... with hmode 4; R_PI_R = ha (R_PI_R - RUR), ...
 // with hmode 3), for velocities used by nothing else: one pass, the
@@ -1244,15 +1266,18 @@ static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   mlg_ctx *c = s->ctx;
   Temps T(c);
   mlg_field *rurn = T.get(), *rupn = T.get(), *uzn = T.get(), *rorn = T.get(), *ropn = T.get(), *ozn = T.get();
-  mlg_field *bn = T.get(), *w = T.get();
+  mlg_field *bn = T.get(), *w = T.get(), *ur = T.buffer(), *up = T.buffer(), *uz = T.buffer();
   RET(T.ready());
   RET(mlg_chopset(c, 3));
   int rc = mlg_pc2vel(psi, chi, rurn, rupn, uzn);
   if (!rc) rc = mlg_pc2vor(psi, chi, rorn, ropn, ozn);
   mlg_chopset(c, -3);
   RET(rc);
-  RET(nonlin_impl(b, rurn, rupn, uzn, bn, rorn, ropn, ozn, true));
-  RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
+  // the second PC2VEL of the reference step equals the first (same PSI, CHI,
+  // NRCHOP+3): NONLIN hands the spectral velocities back in ur, up, uz
+  mlg_field *const vel[3] = {ur, up, uz};
+  RET(nonlin_impl(b, rurn, rupn, uzn, bn, rorn, ropn, ozn, true, vel));
+  rurn = ur, rupn = up, uzn = uz;
   // ETD_HYPERV(N) ; ETDFE_SUB ; ETD_HYPERV(u, b) ; history stores when saving
   RET(launch_etd_sub(c, e16, nl16, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
                      s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 0, save ? 1 : 0, nullptr,
@@ -1298,27 +1323,43 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   s->n += 1;
   Temps T(c);
   mlg_field *rurn = T.get(), *rupn = T.get(), *uzn = T.get(), *rorn = T.get(), *ropn = T.get(), *ozn = T.get();
-  mlg_field *bn = T.get(), *w = T.get(), *psi0 = T.get(), *chi0 = T.get(), *b0 = T.get();
+  mlg_field *bn = T.get(), *w = T.get(), *b0 = T.get();
+  mlg_field *ur = T.buffer(), *up = T.buffer(), *uz = T.buffer(), *psi1 = T.buffer(), *chi1 = T.buffer();
   RET(T.ready());
   RET(mlg_chopset(c, 3));
   int rc = mlg_pc2vel(psi, chi, rurn, rupn, uzn);
   if (!rc) rc = mlg_pc2vor(psi, chi, rorn, ropn, ozn);
   mlg_chopset(c, -3);
   RET(rc);
-  RET(nonlin_impl(b, rurn, rupn, uzn, bn, rorn, ropn, ozn, true));
-  RET(pc2vel3(c, psi, chi, rurn, rupn, uzn));
+  // the second PC2VEL of the reference step equals the first (same PSI, CHI,
+  // NRCHOP+3): NONLIN hands the spectral velocities back in ur, up, uz
+  mlg_field *const vel[3] = {ur, up, uz};
+  RET(nonlin_impl(b, rurn, rupn, uzn, bn, rorn, ropn, ozn, true, vel));
+  rurn = ur, rupn = up, uzn = uz;
   // ETD_HYPERV(N) ; B0 = B ; ETDAB_SUB ; N history ; ETD_HYPERV(u, b) ; pi history
   b0->space = MLG_FFF;
   RET(launch_set_ln(c, b0->ln, b->ln, 1.0, 0.0));
   RET(launch_etd_sub(c, s->etd_e, s->etd_nl, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
                      s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 1, 1, b0->e,
                      filter_active(c), dt / c->g.p.dt));
-  RET(copy0(psi0, psi));
-  RET(copy0(chi0, chi));
-  RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0));  // PROJECT(..., PSI, W, PSI%LN)
-  RET(mlg_idel2ln(w, chi));
-  RET(pc2vel3_hist(s, psi, chi, 4, 0.0));
-  return visc_cn(c, psi, chi, b, psi0, chi0, b0);
+  // PSI0 = PSI ; CHI0 = CHI ; PSI, CHI = PROJECT ; VISC2: the projection is
+  // written to psi1, chi1 so that psi, chi themselves stay the previous
+  // values VISC2 reads, and the solves write the new ones into them.
+  // Without VISC2 the projection goes to psi, chi directly.
+  const bool cn = c->g.p.visc_sw == 1;
+  mlg_field *pj = psi, *cj = chi;
+  if (cn) {
+    pj = psi1, cj = chi1;
+    pj->space = MLG_FFF;
+    RET(launch_set_ln(c, pj->ln, psi->ln, 1.0, 0.0));
+  }
+  RET(project_impl(rurn, rupn, uzn, pj, w, 2, 0.0));  // PROJECT(..., PSI, W, PSI%LN)
+  RET(mlg_idel2ln(w, cj));
+  RET(pc2vel3_hist(s, pj, cj, 4, 0.0));
+  if (!cn) return 0;
+  mlg_field *const f[3] = {psi, chi, b};
+  const mlg_field *const fa[3] = {pj, cj, b}, *const fo[3] = {psi, chi, b0};
+  return visc_cn(c, f, fa, fo);
 }
 
 extern "C" int mlg_initialize(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {

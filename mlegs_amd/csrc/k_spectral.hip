@@ -327,7 +327,8 @@ __global__ void __launch_bounds__(64) k_visc2_rhs(KGrid g, int nrchop2, const do
   const int nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0, nn2 = s_nrchops(nrchop2, mv);
   const int nl = nn > 0 ? (nn2 < g.nrchopdim ? nn2 : g.nrchopdim) : 0;  // rows DEL2 reads
   const size_t cb = fcol(g, mm, k);
-  for (int n = threadIdx.x; n < nl; n += blockDim.x) csm[n] = ao[cb + n];
+  // AO is read as its chopped copy (COPY0): rows >= NRCHOPS count as zero
+  for (int n = threadIdx.x; n < nl; n += blockDim.x) csm[n] = n < nn ? ao[cb + n] : zero2();
   __syncthreads();
   for (int n = threadIdx.x; n < g.nrchopdim; n += blockDim.x) {
     double2 v = zero2();
@@ -338,7 +339,7 @@ __global__ void __launch_bounds__(64) k_visc2_rhs(KGrid g, int nrchop2, const do
     ln_wk[0] = ln_a[0] * -alp + 0.0;
     if (g.ntchopdim >= 2) {
       const int nt = s_nrchops(nrchop2, g.minc);
-      if (nt >= 1) {
+      if (nt >= 1 && g.ntchop > 1 && nt - 1 < s_nrchops(g.nrchop, g.minc)) {
         double2 t = ao[(size_t)(nt - 1) + (size_t)g.nrchopdim * 1];
         if (hypot(t.x, t.y) > 1.0e-15) atomicCAS(err, 0, 3);
       }
