@@ -30,6 +30,10 @@
 
 namespace mlg {
 
+// The theta transforms round like any FFT (not operation for operation like
+// a reference library): products and sums may contract to FMAs here.
+#pragma clang fp contract(fast)
+
 __device__ __forceinline__ double2 fz_cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
@@ -344,6 +348,41 @@ __device__ __forceinline__ void dft8s(double2 (&x)[8]) {
   }
 }
 
+// dft8s for inputs with x[3] = x[4] = 0 (the unused slots of a spectrum
+// with fewer than 96 modes): the additions of those zeros are left out.
+template <int S>
+__device__ __forceinline__ void dft8s_z34(double2 (&x)[8]) {
+  constexpr double h = 0.70710678118654752440;
+  // even part {x0, x2, 0, x6}, odd part {x1, 0, x5, x7}
+  double2 e[4], o[4];
+  {
+    const double2 c = cadd(x[2], x[6]), d = mul_si<S>(csub(x[2], x[6]));
+    e[0] = cadd(x[0], c);
+    e[1] = cadd(x[0], d);
+    e[2] = csub(x[0], c);
+    e[3] = csub(x[0], d);
+  }
+  {
+    const double2 a = cadd(x[1], x[5]), b = csub(x[1], x[5]), d = mul_si<-S>(x[7]);
+    o[0] = cadd(a, x[7]);
+    o[1] = cadd(b, d);
+    o[2] = csub(a, x[7]);
+    o[3] = csub(b, d);
+  }
+  double2 t[4];
+  t[0] = o[0];
+  t[1] = S > 0 ? make_double2(h * (o[1].x - o[1].y), h * (o[1].y + o[1].x))
+               : make_double2(h * (o[1].x + o[1].y), h * (o[1].y - o[1].x));
+  t[2] = mul_si<S>(o[2]);
+  t[3] = S > 0 ? make_double2(h * (-o[3].x - o[3].y), h * (o[3].x - o[3].y))
+               : make_double2(h * (o[3].y - o[3].x), h * (-o[3].y - o[3].x));
+#pragma unroll
+  for (int k = 0; k < 4; k++) {
+    x[k] = cadd(e[k], t[k]);
+    x[k + 4] = csub(e[k], t[k]);
+  }
+}
+
 __device__ __forceinline__ double2 cj(double2 w, bool inv) {
   if (inv) w.y = -w.y;
   return w;
@@ -365,18 +404,23 @@ __device__ __forceinline__ void inverse(const Ctx &x, const KGrid &g, const doub
     if (m == 0) t.y = 0.0;
     return t;
   };
+  // Hermitian merge: with e = X[j] + conj X[N-j], d = X[j] - conj X[N-j],
+  // the point is e + i d conj(th[j]), th[j] = exp(-i pi j / N).  Only one of
+  // the two terms is present: X[j] (j < 96) times 1 + i conj(th[j]), or
+  // conj X[N-j] (j >= 160) times 1 - i conj(th[j]) = 1 + i th[N-j].
 #pragma unroll
   for (int a = 0; a < 8; a++) {
-    const int j = 32 * a + b;
-    double2 xm = (a < 3) ? X(j, j < mmax) : make_double2(0.0, 0.0);
-    const int jc = N - j;
-    double2 xc = (a >= 5) ? fz_conj(X(jc, jc < mmax)) : make_double2(0.0, 0.0);
-    double2 e = make_double2(xm.x + xc.x, xm.y + xc.y);
-    double2 d = make_double2(xm.x - xc.x, xm.y - xc.y);
-    double2 o = fz_cmul(d, fz_conj(x.th[j]));
-    v[a] = make_double2(e.x - o.y, e.y + o.x);
+    if (a == 3 || a == 4) {
+      v[a] = make_double2(0.0, 0.0);
+      continue;
+    }
+    const int j = 32 * a + b, jc = N - j;
+    const double2 t = x.th[a < 3 ? j : jc];
+    const double2 xv = a < 3 ? X(j, j < mmax) : fz_conj(X(jc, jc < mmax));
+    const double2 q = a < 3 ? make_double2(1.0 + t.y, t.x) : make_double2(1.0 - t.y, t.x);
+    v[a] = fz_cmul(xv, q);
   }
-  dft8s<1>(v);
+  dft8s_z34<1>(v);
   // twiddles W^0 = 1 multiply exactly
 #pragma unroll
   for (int c = 1; c < 8; c++) v[c] = fz_cmul(v[c], cj(x.t1[32 * c + b], true));

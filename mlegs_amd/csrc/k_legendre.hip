@@ -131,8 +131,10 @@ k_rtran_bwd(KGrid g, const double *__restrict__ pf, TransformJobs jobs, const do
     stash();
     __syncthreads();
     if (ch + 1 < nch) fetch(ch + 1);
+    const int jleft = njp - ch * BW_JC;  // pairs of this chunk that hold data
 #pragma unroll
     for (int ks = 0; ks < BW_JC; ks += 4) {
+      if (ks >= jleft) break;  // the rest of the chunk is zero padding
       const int kr = ks + (lane >> 4);
 #pragma unroll
       for (int p = 0; p < 2; p++) {
