@@ -505,10 +505,9 @@ extern "C" int mlg_rtran(mlg_field *a, int is) {
   RET(T.ready());
   if (is == -1) {
     if (a->space != MLG_PFF) return 2;
-    RET(launch_rtran_fwd(c, a->e, t->e));
+    RET(launch_rtran_fwd(c, a->e, t->e));  // writes the truncated rows as zeros
     swap_storage(a, t);
     a->space = MLG_FFF;
-    RET(launch_chopdo(c, a->e, MLG_FFF));
   } else {
     if (a->space != MLG_FFF) return 2;
     RET(launch_rtran_bwd(c, a->e, a->ln, t->e));  // writes the padding rows too
@@ -770,9 +769,7 @@ static int project_impl(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field
     RET(launch_set_ln(c, chi->ln, nullptr, 0.0, 0.0));
   }
   if (ln_mode) RET(launch_vel2pln(c, rup->e, psi->ln, ln_mode, ln));
-  RET(launch_project(c, rur->e, rup->e, uz->e, psi->e, chi->e));
-  RET(launch_chopdo(c, psi->e, MLG_FFF));
-  RET(launch_chopdo(c, chi->e, MLG_FFF));
+  RET(launch_project(c, rur->e, rup->e, uz->e, psi->e, chi->e));  // CHOPDO of psi, chi included
   return launch_fix_at_infinity(c, psi->e);
 }
 
@@ -1247,6 +1244,24 @@ static int step_ab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *
   return hyperv_state(c, psi, chi, b, dt);
 }
 
+// CHOPSET(3); PC2VEL(PSI, CHI, u[0..2]); PC2VOR(PSI, CHI, u[3..5]);
+// CHOPSET(-3) in one pass over psi and chi
+static int pc2velvor3(mlg_ctx *c, const mlg_field *psi, const mlg_field *chi, mlg_field *const *u) {
+  if (psi->space != MLG_FFF || chi->space != MLG_FFF) return 3;
+  VelVorOut o;
+  for (int i = 0; i < 6; i++) {
+    if (u[i] == psi || u[i] == chi) return MLG_E_ARG;  // outputs must not alias inputs
+    o.e[i] = u[i]->e;
+    o.ln[i] = u[i]->ln;
+    u[i]->space = MLG_FFF;
+  }
+  RET(mlg_chopset(c, 3));
+  int rc = launch_check_top_m1(c, chi->e);  // PC2VOR's DEL2 check
+  if (!rc) rc = launch_pc2velvor(c, psi->e, psi->ln, chi->e, chi->ln, o);
+  mlg_chopset(c, -3);
+  return rc;
+}
+
 // PC2VEL at NRCHOP+3 followed by the pressure-gradient history update
 // (R_PI_R = R_PI_R - RUR, ... with hmode 4; R_PI_R = ha (R_PI_R - RUR), ...
 // with hmode 3), for velocities used by nothing else: one pass, the
@@ -1268,11 +1283,10 @@ static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   mlg_field *rurn = T.get(), *rupn = T.get(), *uzn = T.get(), *rorn = T.get(), *ropn = T.get(), *ozn = T.get();
   mlg_field *bn = T.get(), *w = T.get(), *ur = T.buffer(), *up = T.buffer(), *uz = T.buffer();
   RET(T.ready());
-  RET(mlg_chopset(c, 3));
-  int rc = mlg_pc2vel(psi, chi, rurn, rupn, uzn);
-  if (!rc) rc = mlg_pc2vor(psi, chi, rorn, ropn, ozn);
-  mlg_chopset(c, -3);
-  RET(rc);
+  {
+    mlg_field *const u[6] = {rurn, rupn, uzn, rorn, ropn, ozn};
+    RET(pc2velvor3(c, psi, chi, u));
+  }
   // the second PC2VEL of the reference step equals the first (same PSI, CHI,
   // NRCHOP+3): NONLIN hands the spectral velocities back in ur, up, uz
   mlg_field *const vel[3] = {ur, up, uz};
@@ -1326,11 +1340,10 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   mlg_field *bn = T.get(), *w = T.get(), *b0 = T.get();
   mlg_field *ur = T.buffer(), *up = T.buffer(), *uz = T.buffer(), *psi1 = T.buffer(), *chi1 = T.buffer();
   RET(T.ready());
-  RET(mlg_chopset(c, 3));
-  int rc = mlg_pc2vel(psi, chi, rurn, rupn, uzn);
-  if (!rc) rc = mlg_pc2vor(psi, chi, rorn, ropn, ozn);
-  mlg_chopset(c, -3);
-  RET(rc);
+  {
+    mlg_field *const u[6] = {rurn, rupn, uzn, rorn, ropn, ozn};
+    RET(pc2velvor3(c, psi, chi, u));
+  }
   // the second PC2VEL of the reference step equals the first (same PSI, CHI,
   // NRCHOP+3): NONLIN hands the spectral velocities back in ur, up, uz
   mlg_field *const vel[3] = {ur, up, uz};

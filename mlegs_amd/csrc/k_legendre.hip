@@ -221,7 +221,8 @@ int launch_rtran_bwd(mlg_ctx *c, const double2 *in, const double *ln_in, double2
 // Block tile: 64 rows n (32 even + 32 odd) x 64 real columns; wave w:
 // parity w&1, column half w>>1 (2 x 2 MFMA tiles).  K chunk: 16 nodes,
 // staged in LDS while the next chunk is fetched into registers.  Rows n in
-// [NRCHOPS(m), NRCHOPDIM) of the output are left to the CHOPDO that follows.
+// [NRCHOPS(m), NRCHOPDIM) and modes m >= NTCHOP of the output are written as
+// zeros (the CHOPDO of the result).
 // -------------------------------------------------------------------------
 #define FW_TN 64
 #define FW_TC 64
@@ -245,10 +246,17 @@ k_rtran_fwd(KGrid g, const double *__restrict__ pf, const double *__restrict__ w
   const int mm = lid / (ntc * ntn), rem = lid - mm * (ntc * ntn);
   const int tn = rem / ntc, tc = rem - tn * ntc;
   const int mv = mm * g.minc;
-  const int nn = d_nrchops(g.nrchop, mv);
+  const int nn = mm < g.ntchop ? d_nrchops(g.nrchop, mv) : 0;
   const int n0 = tn * FW_TN, c0 = tc * FW_TC;
-  if (n0 >= nn) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (n0 >= nn) {  // a tile of the truncated rows: zero (CHOPDO)
+    for (int q = 0; q < 8; q++) {
+      const int idx = tid + 256 * q, n = n0 + (idx & 63), k = (c0 >> 1) + (idx >> 6);
+      if (n < g.nrchopdim && k < nk)
+        out[(size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k)] = make_double2(0.0, 0.0);
+    }
+    return;
+  }
   const int par = wave & 1, ch = wave >> 1;
   __shared__ RtranFwdLds L;
 
@@ -388,10 +396,20 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
   const int mm = lid / (ntc * ntn), rem = lid - mm * (ntc * ntn);
   const int tn = rem / ntc, tc = rem - tn * ntc;
   const int mv = mm * g.minc;
-  const int nn = d_nrchops(g.nrchop, mv);
+  const int nn = mm < g.ntchop ? d_nrchops(g.nrchop, mv) : 0;
   const int n0 = tn * PJ_TN, c0 = tc * PJ_TC;
-  if (n0 >= nn) return;
   const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (n0 >= nn) {  // a tile of the truncated rows: zero (CHOPDO)
+    for (int q = 0; q < 4; q++) {
+      const int idx = tid + 256 * q, n = n0 + (idx & 63), k = (c0 >> 1) + (idx >> 6);
+      if (n < g.nrchopdim && k < nk) {
+        const size_t o = (size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
+        psi[o] = make_double2(0.0, 0.0);
+        chi[o] = make_double2(0.0, 0.0);
+      }
+    }
+    return;
+  }
   const int par = wave & 1, ch = wave >> 1;
   const size_t tstride = (size_t)g.ntchopdim * g.nrchopdim * g.nrh;
   __shared__ ProjectLds L;

@@ -317,7 +317,7 @@ int launch_del2(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, do
 //   WK%LN = -ALP*A%LN
 // with DEL2 evaluated on the truncation NRCHOP+2 (nrchop2) at the positions
 // the base truncation keeps, and DEL2's top-mode check of AO at NRCHOP+2.
-__global__ void __launch_bounds__(64) k_visc2_rhs(KGrid g, int nrchop2, const double *bd, const double *ak,
+__global__ void __launch_bounds__(256) k_visc2_rhs(KGrid g, int nrchop2, const double *bd, const double *ak,
                                                   const double2 *ao, const double *ln_ao, const double2 *a,
                                                   const double *ln_a, double alp, double2 *wk, double *ln_wk, int *err) {
   extern __shared__ __align__(16) double2 csm[];
@@ -351,7 +351,7 @@ int launch_visc2_rhs(mlg_ctx *c, const double2 *ao, const double *ln_ao, const d
   KGrid g = kgrid(c);
   const int nrchop2 = g.nrchop + 2;
   if (nrchop2 > g.nrchopdim) return 1;  // CHOPSET: NRCHOP TOO LARGE (mod_scalar3.f90:183-190)
-  hipLaunchKernelGGL(k_visc2_rhs, column_grid(g), dim3(64), sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, nrchop2,
+  hipLaunchKernelGGL(k_visc2_rhs, column_grid(g), dim3(256), sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, nrchop2,
                      c->d.bdel2h, c->d.ak, ao, ln_ao, a, ln_a, alp, wk, ln_wk, c->d.err);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
@@ -374,7 +374,7 @@ int launch_check_top_m1(mlg_ctx *c, const double2 *a) {
 
 // ---------------------------------------------------------------- GRAD
 // (:1727-1786): (r d/dr b, d/dtheta b, d/dz b) = (XXDX b, i m b, i k b)
-__global__ void __launch_bounds__(64) k_grad(KGrid g, const double *bxxdx, const double *ak, const double2 *a,
+__global__ void __launch_bounds__(256) k_grad(KGrid g, const double *bxxdx, const double *ak, const double2 *a,
                                              const double *ln_a, double2 *rr, double *ln_rr, double2 *rp, double *ln_rp,
                                              double2 *z, double *ln_z) {
   extern __shared__ __align__(16) double2 csm[];
@@ -410,67 +410,38 @@ __global__ void __launch_bounds__(64) k_grad(KGrid g, const double *bxxdx, const
 int launch_grad(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *rr, double *ln_rr, double2 *rp, double *ln_rp,
                 double2 *z, double *ln_z) {
   KGrid g = kgrid(c);
-  hipLaunchKernelGGL(k_grad, column_grid(g), dim3(64), sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, c->d.bxxdx,
+  hipLaunchKernelGGL(k_grad, column_grid(g), dim3(256), sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, c->d.bxxdx,
                      c->d.ak, a, ln_a, rr, ln_rr, rp, ln_rp, z, ln_z);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
 
-// -(1-x)^2 * [(1-x)^-2 del_h^2 s] as two truncated MULXM passes over
-// DELSQH(s) (with s's log image in row 0 of m0k0): row n of the result.
-__device__ double2 uz_like_row(const KGrid &g, const double *bxm, const double2 *s, double ln, int mm, int k, int n, int nn) {
+// Column kernels: one block per (m, k) column.  The column's inputs and the
+// intermediate rows (DELSQH, MULXM, DEL2) are staged in LDS, and every row
+// is formed by the operations of the reference's row recurrences, reading
+// LDS instead of memory.
+// u_z-like rows -(1-x)^2 [(1-x)^-2 del_h^2 s] as two truncated MULXM passes
+// over DELSQH(s) (with s's log image in row 0 of m0k0).
+// W[l] = MULXM row l of DELSQH(s), the DELSQH rows formed where they are read
+__device__ void uz_stage_direct(const KGrid &g, const double *bxm, const double2 *s, double ln, int mm, int k, int nn,
+                                double2 *W) {
   const int mv = mm * g.minc;
   const double *xt = bxm + (size_t)mm * g.nband * 3;
   auto dsq = [&](int j) -> double2 {
-    int N = mv + j;
-    double2 x = s[j];
-    double cf = (double)(N * (N + 1)) / g.ell2;
-    double2 v = make_double2(-x.x * cf, -x.y * cf);
-    if (mm == 0 && k == 0 && j == 0) v = make_double2(ln / g.ell2 / g.norm0, 0.0);
-    return v;
-  };
-  auto wrow = [&](int l) -> double2 {  // W = MULXM(DELSQH s), truncated at nn
-    const double *r = xt + 3 * l;
-    double2 o = axc(r[1], dsq(l));
-    if (l >= 1) o = add2(o, axc(r[0], dsq(l - 1)));
-    if (l + 1 < nn) o = add2(o, axc(r[2], dsq(l + 1)));
-    return o;
-  };
-  const double *r = xt + 3 * n;
-  double2 o = axc(r[1], wrow(n));
-  if (n >= 1) o = add2(o, axc(r[0], wrow(n - 1)));
-  if (n + 1 < nn) o = add2(o, axc(r[2], wrow(n + 1)));
-  return make_double2(-o.x, -o.y);
-}
-
-// Column kernels: one 64-thread block per (m, k) column.  The column's
-// inputs and the intermediate rows (DELSQH, MULXM, DEL2) are staged in LDS,
-// and every row is formed by the same operations as the row functions above,
-// reading LDS instead of memory.
-// u_z-like rows -(1-x)^2 [(1-x)^-2 del_h^2 s] as two truncated MULXM passes
-// over DELSQH(s) (with s's log image in row 0 of m0k0):
-// D[j] = DELSQH row j of s, W[l] = MULXM row l of D, for j, l < nn
-__device__ void uz_stage(const KGrid &g, const double *bxm, const double2 *s, double ln, int mm, int k, int nn, double2 *D,
-                         double2 *W) {
-  const int mv = mm * g.minc;
-  const double *xt = bxm + (size_t)mm * g.nband * 3;
-  for (int j = threadIdx.x; j < nn; j += blockDim.x) {
     const int N = mv + j;
     const double2 x = s[j];
     const double cf = (double)(N * (N + 1)) / g.ell2;
     double2 v = make_double2(-x.x * cf, -x.y * cf);
     if (mm == 0 && k == 0 && j == 0) v = make_double2(ln / g.ell2 / g.norm0, 0.0);
-    D[j] = v;
-  }
-  __syncthreads();
+    return v;
+  };
   for (int l = threadIdx.x; l < nn; l += blockDim.x) {
     const double *r = xt + 3 * l;
-    double2 o = axc(r[1], D[l]);
-    if (l >= 1) o = add2(o, axc(r[0], D[l - 1]));
-    if (l + 1 < nn) o = add2(o, axc(r[2], D[l + 1]));
+    double2 o = axc(r[1], dsq(l));
+    if (l >= 1) o = add2(o, axc(r[0], dsq(l - 1)));
+    if (l + 1 < nn) o = add2(o, axc(r[2], dsq(l + 1)));
     W[l] = o;
   }
-  __syncthreads();
 }
 // row n of the u_z-like result: -(MULXM row n of W)
 __device__ __forceinline__ double2 uz_from_w(const KGrid &g, const double *bxm, const double2 *W, int mm, int n, int nn) {
@@ -485,65 +456,24 @@ __device__ __forceinline__ double2 uz_from_w(const KGrid &g, const double *bxm, 
 // u_z = -(1-x) (1-x) DELSQH chi.
 // HIST: the velocities only update the pressure-gradient histories H (in
 // rur, rup, uz): H = H - u (hmode 4) or H = a (H - u) (hmode 3), the
-// updates of the ETD steps, and are not stored
+// updates of the ETD steps, and are not stored (k_pc2vel_col below)
 template <bool HIST>
-__global__ void k_pc2vel(KGrid g, const double *bxm, const double *bxxdx, const double *ak, const double2 *psi,
-                         const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *rur, double2 *rup,
-                         double2 *uz, double *l1, double *l2, double *l3, int hmode, double ha) {
-  const double lp = ln_psi[0], lc = ln_chi[0];
-  GRID_STRIDE(fff_count(g)) {
-    FIdx f = fdecode(g, idx);
-    int mv = f.mm * g.minc, nn = s_nrchops(g.nrchop, mv);
-    double2 vr = zero2(), vp = zero2(), vz = zero2();
-    if (f.n < nn && f.mm < g.ntchop) {
-      const size_t cb = fcol(g, f.mm, f.k);
-      const double *xx = bxxdx + (size_t)f.mm * g.nband * 3;
-      double2 xc = band3_row(xx, chi + cb, f.n, nn);
-      double2 xp = band3_row(xx, psi + cb, f.n, nn);
-      if (f.mm == 0 && f.k == 0) {
-        if (f.n == 0) { xc.x = xc.x + lc / g.norm0; xp.x = xp.x + lp / g.norm0; }
-        if (f.n == 1) { xc.x = xc.x + lc / g.norm1; xp.x = xp.x + lp / g.norm1; }
-      }
-      double kv = ak[f.mm + (size_t)g.ntchopdim * f.k];
-      vr = add2(ix((double)mv, psi[cb + f.n]), ix(kv, xc));
-      vp = sub2(make_double2(-xp.x, -xp.y), axc(mv * kv, chi[cb + f.n]));
-      vz = uz_like_row(g, bxm, chi + cb, lc, f.mm, f.k, f.n, nn);
-    }
-    if (HIST) {
-      if (hmode == 3) {
-        rur[idx] = axc(ha, sub2(rur[idx], vr));
-        rup[idx] = axc(ha, sub2(rup[idx], vp));
-        uz[idx] = axc(ha, sub2(uz[idx], vz));
-      } else {
-        rur[idx] = sub2(rur[idx], vr);
-        rup[idx] = sub2(rup[idx], vp);
-        uz[idx] = sub2(uz[idx], vz);
-      }
-    } else {
-      rur[idx] = vr;
-      rup[idx] = vp;
-      uz[idx] = vz;
-    }
-  }
-  if (!HIST && blockIdx.x == 0 && threadIdx.x == 0) {
-    l1[0] = 0.0;
-    l2[0] = 0.0;
-    l3[0] = 0.0;
-  }
-}
+__global__ void k_pc2vel_col(KGrid g, const double *bxm, const double *bxxdx, const double *ak, const double2 *psi,
+                             const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *rur, double2 *rup,
+                             double2 *uz, double *l1, double *l2, double *l3, int hmode, double ha);
 int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *rur,
                   double2 *rup, double2 *uz, double *ln_rur, double *ln_rup, double *ln_uz) {
   KGrid g = kgrid(c);
-  hipLaunchKernelGGL(k_pc2vel<false>, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bxm, c->d.bxxdx, c->d.ak,
-                     psi, ln_psi, chi, ln_chi, rur, rup, uz, ln_rur, ln_rup, ln_uz, 0, 0.0);
+  hipLaunchKernelGGL(k_pc2vel_col<false>, column_grid(g), dim3(256), 3 * sizeof(double2) * (size_t)g.nrchopdim, c->stream, g,
+                     c->d.bxm, c->d.bxxdx, c->d.ak, psi, ln_psi, chi, ln_chi, rur, rup, uz, ln_rur, ln_rup, ln_uz, 0, 0.0);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
 int launch_pc2vel_hist(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
                        double2 *hr, double2 *hp, double2 *hz, int hmode, double ha) {
   KGrid g = kgrid(c);
-  hipLaunchKernelGGL(k_pc2vel<true>, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, c->d.bxm, c->d.bxxdx, c->d.ak,
-                     psi, ln_psi, chi, ln_chi, hr, hp, hz, nullptr, nullptr, nullptr, hmode, ha);
+  hipLaunchKernelGGL(k_pc2vel_col<true>, column_grid(g), dim3(256), 3 * sizeof(double2) * (size_t)g.nrchopdim, c->stream, g,
+                     c->d.bxm, c->d.bxxdx, c->d.ak, psi, ln_psi, chi, ln_chi, hr, hp, hz, nullptr, nullptr, nullptr, hmode, ha);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -552,7 +482,7 @@ int launch_pc2vel_hist(mlg_ctx *c, const double2 *psi, const double *ln_psi, con
 // (:1856-1919): with Q = DEL2 chi,
 //   r om_r = -i m Q + i k XXDX psi ; r om_th = XXDX Q - k m psi ;
 //   om_z = -(1-x)(1-x) DELSQH psi.
-__global__ void __launch_bounds__(64) k_pc2vor(KGrid g, const double *bxm, const double *bxxdx, const double *bd,
+__global__ void __launch_bounds__(256) k_pc2vor(KGrid g, const double *bxm, const double *bxxdx, const double *bd,
                                                const double *ak, const double2 *psi, const double *ln_psi,
                                                const double2 *chi, const double *ln_chi, double2 *ror, double2 *rop,
                                                double2 *oz, double *l1, double *l2, double *l3) {
@@ -560,7 +490,7 @@ __global__ void __launch_bounds__(64) k_pc2vor(KGrid g, const double *bxm, const
   const int mm = blockIdx.x, k = blockIdx.y;
   const int mv = mm * g.minc, nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0;
   const size_t cb = fcol(g, mm, k);
-  double2 *P = csm, *X = P + g.nrchopdim, *Q = X + g.nrchopdim, *D = Q + g.nrchopdim, *W = D + g.nrchopdim;
+  double2 *P = csm, *X = P + g.nrchopdim, *Q = X + g.nrchopdim, *W = Q + g.nrchopdim;
   const double lp = ln_psi[0], lc = ln_chi[0];
   for (int n = threadIdx.x; n < nn; n += blockDim.x) {
     P[n] = psi[cb + n];
@@ -568,7 +498,8 @@ __global__ void __launch_bounds__(64) k_pc2vor(KGrid g, const double *bxm, const
   }
   __syncthreads();
   for (int n = threadIdx.x; n < nn; n += blockDim.x) Q[n] = del2_row(g, bd, ak, X, lc, mm, k, n, nn);
-  uz_stage(g, bxm, P, lp, mm, k, nn, D, W);  // (its first barrier also orders Q)
+  uz_stage_direct(g, bxm, P, lp, mm, k, nn, W);
+  __syncthreads();
   const double *xx = bxxdx + (size_t)mm * g.nband * 3;
   const double kv = ak[mm + (size_t)g.ntchopdim * k];
   for (int n = threadIdx.x; n < g.nrchopdim; n += blockDim.x) {
@@ -603,10 +534,136 @@ __global__ void __launch_bounds__(64) k_pc2vor(KGrid g, const double *bxm, const
 int launch_pc2vor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *ror,
                   double2 *rop, double2 *oz, double *ln_ror, double *ln_rop, double *ln_oz) {
   KGrid g = kgrid(c);
-  hipLaunchKernelGGL(k_pc2vor, column_grid(g), dim3(64), 5 * sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, c->d.bxm,
+  hipLaunchKernelGGL(k_pc2vor, column_grid(g), dim3(256), 4 * sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, c->d.bxm,
                      c->d.bxxdx, c->d.bdel2h, c->d.ak, psi, ln_psi, chi, ln_chi, ror, rop, oz, ln_ror, ln_rop, ln_oz);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
+}
+
+// ---------------------------------------------------------------- PC2VEL + PC2VOR
+// Both of one (PSI, CHI) pair in one pass, as the time steps call them
+// (CHOPSET(3); PC2VEL; PC2VOR): one block per (m, k) column, the column of
+// psi and chi, DEL2 chi and the two MULXM(DELSQH) stages in LDS.  Every
+// output row is formed by the same operations as in k_pc2vel and k_pc2vor.
+__global__ void __launch_bounds__(256) k_pc2velvor(KGrid g, const double *bxm, const double *bxxdx, const double *bd,
+                                                   const double *ak, const double2 *psi, const double *ln_psi,
+                                                   const double2 *chi, const double *ln_chi, VelVorOut o) {
+  extern __shared__ __align__(16) double2 csm[];
+  const int mm = blockIdx.x, k = blockIdx.y;
+  const int mv = mm * g.minc, nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0;
+  const size_t cb = fcol(g, mm, k);
+  double2 *P = csm, *X = P + g.nrchopdim, *Q = X + g.nrchopdim, *WP = Q + g.nrchopdim, *WX = WP + g.nrchopdim;
+  const double lp = ln_psi[0], lc = ln_chi[0];
+  for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+    P[n] = psi[cb + n];
+    X[n] = chi[cb + n];
+  }
+  __syncthreads();
+  for (int n = threadIdx.x; n < nn; n += blockDim.x) Q[n] = del2_row(g, bd, ak, X, lc, mm, k, n, nn);
+  uz_stage_direct(g, bxm, P, lp, mm, k, nn, WP);
+  uz_stage_direct(g, bxm, X, lc, mm, k, nn, WX);
+  __syncthreads();
+  const double *xx = bxxdx + (size_t)mm * g.nband * 3;
+  const double kv = ak[mm + (size_t)g.ntchopdim * k];
+  for (int n = threadIdx.x; n < g.nrchopdim; n += blockDim.x) {
+    double2 ur = zero2(), up = zero2(), uz = zero2(), wr = zero2(), wp = zero2(), wz = zero2();
+    if (n < nn) {
+      double2 xc = band3_row(xx, X, n, nn);
+      double2 xp = band3_row(xx, P, n, nn);
+      if (mm == 0 && k == 0) {
+        if (n == 0) { xc.x = xc.x + lc / g.norm0; xp.x = xp.x + lp / g.norm0; }
+        if (n == 1) { xc.x = xc.x + lc / g.norm1; xp.x = xp.x + lp / g.norm1; }
+      }
+      // velocity (PC2VEL)
+      ur = add2(ix((double)mv, P[n]), ix(kv, xc));
+      up = sub2(make_double2(-xp.x, -xp.y), axc(mv * kv, X[n]));
+      uz = uz_from_w(g, bxm, WX, mm, n, nn);
+      // vorticity (PC2VOR)
+      const double2 q = Q[n];
+      const double2 imq = ix((double)mv, q);
+      wr = add2(make_double2(-imq.x, -imq.y), ix(kv, xp));
+      const double *r = xx + 3 * n;
+      double2 xq = axc(r[1], q);
+      if (n >= 1) xq = add2(xq, axc(r[0], Q[n - 1]));
+      if (n + 1 < nn) xq = add2(xq, axc(r[2], Q[n + 1]));
+      wp = sub2(xq, axc(kv * mv, P[n]));
+      wz = uz_from_w(g, bxm, WP, mm, n, nn);
+    }
+    o.e[0][cb + n] = ur;
+    o.e[1][cb + n] = up;
+    o.e[2][cb + n] = uz;
+    o.e[3][cb + n] = wr;
+    o.e[4][cb + n] = wp;
+    o.e[5][cb + n] = wz;
+  }
+  if (mm == 0 && k == 0 && threadIdx.x < 6) o.ln[threadIdx.x][0] = 0.0;
+}
+int launch_pc2velvor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
+                     const VelVorOut &o) {
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_pc2velvor, column_grid(g), dim3(256), 5 * sizeof(double2) * (size_t)g.nrchopdim, c->stream, g,
+                     c->d.bxm, c->d.bxxdx, c->d.bdel2h, c->d.ak, psi, ln_psi, chi, ln_chi, o);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+// PC2VEL as a column kernel (psi, chi and MULXM(DELSQH chi) of the column in
+// LDS), with the HIST variant of k_pc2vel; the same operations per row.
+template <bool HIST>
+__global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, const double *bxxdx, const double *ak,
+                                                    const double2 *psi, const double *ln_psi, const double2 *chi,
+                                                    const double *ln_chi, double2 *rur, double2 *rup, double2 *uz,
+                                                    double *l1, double *l2, double *l3, int hmode, double ha) {
+  extern __shared__ __align__(16) double2 csm[];
+  const int mm = blockIdx.x, k = blockIdx.y;
+  const int mv = mm * g.minc, nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0;
+  const size_t cb = fcol(g, mm, k);
+  double2 *P = csm, *X = P + g.nrchopdim, *WX = X + g.nrchopdim;
+  const double lp = ln_psi[0], lc = ln_chi[0];
+  for (int n = threadIdx.x; n < nn; n += blockDim.x) {
+    P[n] = psi[cb + n];
+    X[n] = chi[cb + n];
+  }
+  __syncthreads();
+  uz_stage_direct(g, bxm, X, lc, mm, k, nn, WX);
+  __syncthreads();
+  const double *xx = bxxdx + (size_t)mm * g.nband * 3;
+  const double kv = ak[mm + (size_t)g.ntchopdim * k];
+  for (int n = threadIdx.x; n < g.nrchopdim; n += blockDim.x) {
+    double2 vr = zero2(), vp = zero2(), vz = zero2();
+    if (n < nn) {
+      double2 xc = band3_row(xx, X, n, nn);
+      double2 xp = band3_row(xx, P, n, nn);
+      if (mm == 0 && k == 0) {
+        if (n == 0) { xc.x = xc.x + lc / g.norm0; xp.x = xp.x + lp / g.norm0; }
+        if (n == 1) { xc.x = xc.x + lc / g.norm1; xp.x = xp.x + lp / g.norm1; }
+      }
+      vr = add2(ix((double)mv, P[n]), ix(kv, xc));
+      vp = sub2(make_double2(-xp.x, -xp.y), axc(mv * kv, X[n]));
+      vz = uz_from_w(g, bxm, WX, mm, n, nn);
+    }
+    const size_t i = cb + n;
+    if (HIST) {
+      if (hmode == 3) {
+        rur[i] = axc(ha, sub2(rur[i], vr));
+        rup[i] = axc(ha, sub2(rup[i], vp));
+        uz[i] = axc(ha, sub2(uz[i], vz));
+      } else {
+        rur[i] = sub2(rur[i], vr);
+        rup[i] = sub2(rup[i], vp);
+        uz[i] = sub2(uz[i], vz);
+      }
+    } else {
+      rur[i] = vr;
+      rup[i] = vp;
+      uz[i] = vz;
+    }
+  }
+  if (!HIST && mm == 0 && k == 0 && threadIdx.x == 0) {
+    l1[0] = 0.0;
+    l2[0] = 0.0;
+    l3[0] = 0.0;
+  }
 }
 
 // ---------------------------------------------------------------- value at infinity

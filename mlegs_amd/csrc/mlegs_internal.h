@@ -207,6 +207,13 @@ int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const do
 // H = ha (H - u) (hmode 3), the AXPBY modes of the same names
 int launch_pc2vel_hist(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
                        double2 *hr, double2 *hp, double2 *hz, int hmode, double ha);
+// PC2VEL and PC2VOR of one (psi, chi): outputs rur, rup, uz, ror, rop, oz
+struct VelVorOut {
+  double2 *e[6];
+  double *ln[6];
+};
+int launch_pc2velvor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
+                     const VelVorOut &o);
 int launch_pc2vor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
                   double2 *ror, double2 *rop, double2 *oz, double *ln_ror, double *ln_rop, double *ln_oz);
 int launch_fix_at_infinity(mlg_ctx *c, double2 *a);
