@@ -715,9 +715,12 @@ extern "C" int mlg_vel2pln(mlg_field *rup, mlg_field *psi, int has_ln, double ln
 // so the device copy is kept without a host read.
 // n fields of one space, FFF or PFP, to PFF with one transform launch (each
 // field as mlg_rtran(., 1) or mlg_verfft(., -1) treats it)
-static int to_pff_n(mlg_ctx *c, int n, mlg_field *const *fs) {
+// keep[i] (optional, FFF input only) receives the FFF elements of fs[i]
+// (not its LN) instead of discarding them; it must be a temporary.
+static int to_pff_n(mlg_ctx *c, int n, mlg_field *const *fs, mlg_field *const *keep = nullptr) {
   const int sp = fs[0]->space;
   if (n > MAX_TRANSFORM_JOBS || (sp != MLG_FFF && sp != MLG_PFP)) return 2;
+  if (keep && sp != MLG_FFF) return 2;
   for (int i = 1; i < n; i++)
     if (fs[i]->space != sp) return 2;
   Temps T(c);
@@ -742,16 +745,26 @@ static int to_pff_n(mlg_ctx *c, int n, mlg_field *const *fs) {
   for (int i = 0; i < n; i++) {
     swap_storage(fs[i], t[i]);
     fs[i]->space = MLG_PFF;
+    if (keep) {
+      swap_storage(keep[i], t[i]);
+      keep[i]->space = MLG_FFF;
+    }
   }
   if (sp == MLG_FFF) RET(launch_zero_lns(c, lns, n));
   return 0;
 }
 
-static int project_impl(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *psi, mlg_field *chi, int ln_mode, double ln) {
+// keep (optional, FFF velocities only): receives the FFF elements of the
+// velocities, see to_pff_n
+static int project_impl(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *psi, mlg_field *chi, int ln_mode, double ln,
+                        mlg_field *const *keep = nullptr) {
   if (!(rur->space == rup->space && rup->space == uz->space)) return 2;
   mlg_ctx *c = rur->ctx;
   mlg_field *const vel[3] = {rur, rup, uz};
-  if (rur->space == MLG_PPP) {
+  if (keep) {
+    if (rur->space != MLG_FFF) return 2;
+    RET(to_pff_n(c, 3, vel, keep));
+  } else if (rur->space == MLG_PPP) {
     RET(mlg_plnfft(rur, -1));
     RET(mlg_plnfft(rup, -1));
     RET(mlg_plnfft(uz, -1));
@@ -1266,11 +1279,17 @@ static int pc2velvor3(mlg_ctx *c, const mlg_field *psi, const mlg_field *chi, ml
 // (R_PI_R = R_PI_R - RUR, ... with hmode 4; R_PI_R = ha (R_PI_R - RUR), ...
 // with hmode 3), for velocities used by nothing else: one pass, the
 // velocities are not stored
-static int pc2vel3_hist(mlg_solver *s, const mlg_field *psi, const mlg_field *chi, int hmode, double ha) {
+// With `base`, the histories' previous values are read from base (the
+// values the ETD update would have stored in them).
+static int pc2vel3_hist(mlg_solver *s, const mlg_field *psi, const mlg_field *chi, int hmode, double ha,
+                        mlg_field *const *base = nullptr) {
   mlg_ctx *c = s->ctx;
   if (psi->space != MLG_FFF || chi->space != MLG_FFF) return 3;
+  VelTriple bt{{nullptr, nullptr, nullptr}};
+  if (base)
+    for (int i = 0; i < 3; i++) bt.e[i] = base[i]->e;
   RET(mlg_chopset(c, 3));
-  int rc = launch_pc2vel_hist(c, psi->e, psi->ln, chi->e, chi->ln, s->r_pi_r->e, s->r_pi_p->e, s->pi_z->e, hmode, ha);
+  int rc = launch_pc2vel_hist(c, psi->e, psi->ln, chi->e, chi->ln, s->r_pi_r->e, s->r_pi_p->e, s->pi_z->e, hmode, ha, bt);
   mlg_chopset(c, -3);
   return rc;
 }
@@ -1282,6 +1301,7 @@ static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   Temps T(c);
   mlg_field *rurn = T.get(), *rupn = T.get(), *uzn = T.get(), *rorn = T.get(), *ropn = T.get(), *ozn = T.get();
   mlg_field *bn = T.get(), *w = T.get(), *ur = T.buffer(), *up = T.buffer(), *uz = T.buffer();
+  mlg_field *const hk[3] = {T.buffer(), T.buffer(), T.buffer()};
   RET(T.ready());
   {
     mlg_field *const u[6] = {rurn, rupn, uzn, rorn, ropn, ozn};
@@ -1292,14 +1312,17 @@ static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   mlg_field *const vel[3] = {ur, up, uz};
   RET(nonlin_impl(b, rurn, rupn, uzn, bn, rorn, ropn, ozn, true, vel));
   rurn = ur, rupn = up, uzn = uz;
-  // ETD_HYPERV(N) ; ETDFE_SUB ; ETD_HYPERV(u, b) ; history stores when saving
+  // ETD_HYPERV(N) ; ETDFE_SUB ; ETD_HYPERV(u, b) ; history stores when
+  // saving: the N history here; the pi history R_PI = u is kept in hk (the
+  // velocities' FFF elements, which PROJECT transforms) and combined with the
+  // second PC2VEL below
   RET(launch_etd_sub(c, e16, nl16, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
                      s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 0, save ? 1 : 0, nullptr,
-                     filter_active(c), dt / c->g.p.dt));
-  RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0));  // PROJECT(..., PSI, W, PSI%LN)
+                     nullptr, filter_active(c), dt / c->g.p.dt));
+  RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0, save ? hk : nullptr));  // PROJECT(..., PSI, W, PSI%LN)
   RET(mlg_idel2ln(w, chi));
   if (save) {
-    RET(pc2vel3_hist(s, psi, chi, 3, 0.5));
+    RET(pc2vel3_hist(s, psi, chi, 3, 0.5, hk));
   }
   return 0;
 }
@@ -1337,8 +1360,9 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   s->n += 1;
   Temps T(c);
   mlg_field *rurn = T.get(), *rupn = T.get(), *uzn = T.get(), *rorn = T.get(), *ropn = T.get(), *ozn = T.get();
-  mlg_field *bn = T.get(), *w = T.get(), *b0 = T.get();
-  mlg_field *ur = T.buffer(), *up = T.buffer(), *uz = T.buffer(), *psi1 = T.buffer(), *chi1 = T.buffer();
+  mlg_field *bn = T.get(), *w = T.get();
+  mlg_field *ur = T.buffer(), *up = T.buffer(), *uz = T.buffer(), *psi1 = T.buffer(), *chi1 = T.buffer(), *b1 = T.buffer();
+  mlg_field *const hk[3] = {T.buffer(), T.buffer(), T.buffer()};
   RET(T.ready());
   {
     mlg_field *const u[6] = {rurn, rupn, uzn, rorn, ropn, ozn};
@@ -1349,29 +1373,30 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   mlg_field *const vel[3] = {ur, up, uz};
   RET(nonlin_impl(b, rurn, rupn, uzn, bn, rorn, ropn, ozn, true, vel));
   rurn = ur, rupn = up, uzn = uz;
-  // ETD_HYPERV(N) ; B0 = B ; ETDAB_SUB ; N history ; ETD_HYPERV(u, b) ; pi history
-  b0->space = MLG_FFF;
-  RET(launch_set_ln(c, b0->ln, b->ln, 1.0, 0.0));
-  RET(launch_etd_sub(c, s->etd_e, s->etd_nl, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
-                     s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 1, 1, b0->e,
-                     filter_active(c), dt / c->g.p.dt));
-  // PSI0 = PSI ; CHI0 = CHI ; PSI, CHI = PROJECT ; VISC2: the projection is
-  // written to psi1, chi1 so that psi, chi themselves stay the previous
-  // values VISC2 reads, and the solves write the new ones into them.
-  // Without VISC2 the projection goes to psi, chi directly.
+  // B0 = B ; PSI0 = PSI ; CHI0 = CHI ; the ETD update, PROJECT ; VISC2: the
+  // new b and the projection are written to b1, psi1, chi1, so that b, psi,
+  // chi themselves stay the previous values VISC2 reads, and the solves
+  // write the new ones into them.  Without VISC2 they are updated in place.
   const bool cn = c->g.p.visc_sw == 1;
-  mlg_field *pj = psi, *cj = chi;
+  mlg_field *bj = b, *pj = psi, *cj = chi;
   if (cn) {
-    pj = psi1, cj = chi1;
-    pj->space = MLG_FFF;
+    bj = b1, pj = psi1, cj = chi1;
+    bj->space = pj->space = MLG_FFF;
+    RET(launch_set_ln(c, bj->ln, b->ln, 1.0, 0.0));
     RET(launch_set_ln(c, pj->ln, psi->ln, 1.0, 0.0));
   }
-  RET(project_impl(rurn, rupn, uzn, pj, w, 2, 0.0));  // PROJECT(..., PSI, W, PSI%LN)
+  // ETD_HYPERV(N) ; ETDAB_SUB ; N history ; ETD_HYPERV(u, b) ; pi history:
+  // R_PI = u is kept in hk (the velocities' FFF elements, which PROJECT
+  // transforms) and combined with the second PC2VEL below
+  RET(launch_etd_sub(c, s->etd_e, s->etd_nl, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
+                     s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 1, 1, nullptr, bj->e,
+                     filter_active(c), dt / c->g.p.dt));
+  RET(project_impl(rurn, rupn, uzn, pj, w, 2, 0.0, hk));  // PROJECT(..., PSI, W, PSI%LN)
   RET(mlg_idel2ln(w, cj));
-  RET(pc2vel3_hist(s, pj, cj, 4, 0.0));
+  RET(pc2vel3_hist(s, pj, cj, 4, 0.0, hk));
   if (!cn) return 0;
   mlg_field *const f[3] = {psi, chi, b};
-  const mlg_field *const fa[3] = {pj, cj, b}, *const fo[3] = {psi, chi, b0};
+  const mlg_field *const fa[3] = {pj, cj, bj}, *const fo[3] = {psi, chi, b};
   return visc_cn(c, f, fa, fo);
 }
 

@@ -460,20 +460,22 @@ __device__ __forceinline__ double2 uz_from_w(const KGrid &g, const double *bxm, 
 template <bool HIST>
 __global__ void k_pc2vel_col(KGrid g, const double *bxm, const double *bxxdx, const double *ak, const double2 *psi,
                              const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *rur, double2 *rup,
-                             double2 *uz, double *l1, double *l2, double *l3, int hmode, double ha);
+                             double2 *uz, double *l1, double *l2, double *l3, int hmode, double ha, VelTriple base);
 int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *rur,
                   double2 *rup, double2 *uz, double *ln_rur, double *ln_rup, double *ln_uz) {
   KGrid g = kgrid(c);
   hipLaunchKernelGGL(k_pc2vel_col<false>, column_grid(g), dim3(256), 3 * sizeof(double2) * (size_t)g.nrchopdim, c->stream, g,
-                     c->d.bxm, c->d.bxxdx, c->d.ak, psi, ln_psi, chi, ln_chi, rur, rup, uz, ln_rur, ln_rup, ln_uz, 0, 0.0);
+                     c->d.bxm, c->d.bxxdx, c->d.ak, psi, ln_psi, chi, ln_chi, rur, rup, uz, ln_rur, ln_rup, ln_uz, 0, 0.0,
+                     VelTriple{{nullptr, nullptr, nullptr}});
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
 int launch_pc2vel_hist(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
-                       double2 *hr, double2 *hp, double2 *hz, int hmode, double ha) {
+                       double2 *hr, double2 *hp, double2 *hz, int hmode, double ha, const VelTriple &base) {
   KGrid g = kgrid(c);
   hipLaunchKernelGGL(k_pc2vel_col<true>, column_grid(g), dim3(256), 3 * sizeof(double2) * (size_t)g.nrchopdim, c->stream, g,
-                     c->d.bxm, c->d.bxxdx, c->d.ak, psi, ln_psi, chi, ln_chi, hr, hp, hz, nullptr, nullptr, nullptr, hmode, ha);
+                     c->d.bxm, c->d.bxxdx, c->d.ak, psi, ln_psi, chi, ln_chi, hr, hp, hz, nullptr, nullptr, nullptr, hmode, ha,
+                     base);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -613,7 +615,8 @@ template <bool HIST>
 __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, const double *bxxdx, const double *ak,
                                                     const double2 *psi, const double *ln_psi, const double2 *chi,
                                                     const double *ln_chi, double2 *rur, double2 *rup, double2 *uz,
-                                                    double *l1, double *l2, double *l3, int hmode, double ha) {
+                                                    double *l1, double *l2, double *l3, int hmode, double ha,
+                                                    VelTriple base) {
   extern __shared__ __align__(16) double2 csm[];
   const int mm = blockIdx.x, k = blockIdx.y;
   const int mv = mm * g.minc, nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0;
@@ -644,14 +647,18 @@ __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, 
     }
     const size_t i = cb + n;
     if (HIST) {
+      // H is read from `base` when given (the values the histories were to
+      // receive just before), else from the histories themselves
+      const double2 h0 = base.e[0] ? base.e[0][i] : rur[i], h1 = base.e[0] ? base.e[1][i] : rup[i],
+                    h2 = base.e[0] ? base.e[2][i] : uz[i];
       if (hmode == 3) {
-        rur[i] = axc(ha, sub2(rur[i], vr));
-        rup[i] = axc(ha, sub2(rup[i], vp));
-        uz[i] = axc(ha, sub2(uz[i], vz));
+        rur[i] = axc(ha, sub2(h0, vr));
+        rup[i] = axc(ha, sub2(h1, vp));
+        uz[i] = axc(ha, sub2(h2, vz));
       } else {
-        rur[i] = sub2(rur[i], vr);
-        rup[i] = sub2(rup[i], vp);
-        uz[i] = sub2(uz[i], vz);
+        rur[i] = sub2(h0, vr);
+        rup[i] = sub2(h1, vp);
+        uz[i] = sub2(h2, vz);
       }
     } else {
       rur[i] = vr;
@@ -1339,11 +1346,12 @@ struct Mat4x2 {
   double e[16], nl[16];
 };
 struct EtdArgs {
-  double2 *rur, *rup, *uz, *b;       // state: in, updated out
+  double2 *rur, *rup, *uz, *b;       // state: in, updated out (b: in only)
   const double2 *nr, *np, *nz, *nb;  // nonlinear terms (before filtering)
-  double2 *hpr, *hpp, *hpz;          // pressure-gradient history: read (ab), written (save)
-  double2 *hnr, *hnp, *hnz, *hnb;    // nonlinear-term history: read (ab), written (save)
+  double2 *hpr, *hpp, *hpz;          // pressure-gradient history: read (ab), written (save & 2)
+  double2 *hnr, *hnp, *hnz, *hnb;    // nonlinear-term history: read (ab), written (save & 1)
   double2 *b0;                       // receives b before the update (or null)
+  double2 *bo;                       // receives the updated b (may be b)
   const double *hv, *hd;             // filter tables [hr | ht | hx] (or null: no filter)
   int nrc;
   double tf;
@@ -1438,12 +1446,14 @@ __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, E
     A.rur[idx] = cv[0];
     A.rup[idx] = cv[1];
     A.uz[idx] = cv[2];
-    A.b[idx] = cv[3];
-    if (save) {
+    A.bo[idx] = cv[3];
+    if (save & 1) {
       A.hnr[idx] = nl[0];
       A.hnp[idx] = nl[1];
       A.hnz[idx] = nl[2];
       A.hnb[idx] = nl[3];
+    }
+    if (save & 2) {
       A.hpr[idx] = cv[0];
       A.hpp[idx] = cv[1];
       A.hpz[idx] = cv[2];
@@ -1453,14 +1463,14 @@ __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, E
 int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt, double2 *rur, double2 *rup, double2 *uz,
                    double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_, double2 *hpr,
                    double2 *hpp, double2 *hpz, double2 *hnr, double2 *hnp, double2 *hnz, double2 *hnb, int ab, int save,
-                   double2 *b0, int filter, double tf) {
+                   double2 *b0, double2 *bout, int filter, double tf) {
   KGrid g = kgrid(c);
   Mat4x2 M;
   for (int q = 0; q < 16; q++) {
     M.e[q] = e16[q];
     M.nl[q] = nl16[q];
   }
-  EtdArgs A{rur, rup, uz, b, nr_, np_, nz_, nb_, hpr, hpp, hpz, hnr, hnp, hnz, hnb, b0, nullptr, nullptr, 0, tf};
+  EtdArgs A{rur, rup, uz, b, nr_, np_, nz_, nb_, hpr, hpp, hpz, hnr, hnp, hnz, hnb, b0, bout ? bout : b, nullptr, nullptr, 0, tf};
   if (filter) {
     const mlg::Grid &G = c->g;
     size_t nr1 = (size_t)G.p.nrchop * G.ntchopdim, nt = G.ntchopdim, nx = G.nxchopdim;
@@ -1468,8 +1478,8 @@ int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt,
     A.hd = c->d.hv + (nr1 + nt + nx);
     A.nrc = G.p.nrchop;
   }
-  // 4 fields + 4 nonlinear terms in, 4 out; history: 7 in (ab), 7 out (save); b0 out
-  double nf = 12.0 + (ab ? 7.0 : 0.0) + (save ? 7.0 : 0.0) + (b0 ? 1.0 : 0.0);
+  // 4 fields + 4 nonlinear terms in, 4 out; history: 7 in (ab), 4 + 3 out (save bits); b0 out
+  double nf = 12.0 + (ab ? 7.0 : 0.0) + ((save & 1) ? 4.0 : 0.0) + ((save & 2) ? 3.0 : 0.0) + (b0 ? 1.0 : 0.0);
   KScope ks(c, K_ETD, 16.0 * fff_count(g) * nf);
   hipLaunchKernelGGL(k_etd_sub, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, M, dt, A, ab, save);
   MLG_CHECK_HIP(hipGetLastError());

@@ -205,8 +205,14 @@ int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const do
                   double2 *rur, double2 *rup, double2 *uz, double *ln_rur, double *ln_rup, double *ln_uz);
 // PC2VEL whose velocities only update the histories: H = H - u (hmode 4) or
 // H = ha (H - u) (hmode 3), the AXPBY modes of the same names
+// three FFF element arrays (null e[0]: not given)
+struct VelTriple {
+  const double2 *e[3];
+};
+// H = H - u (hmode 4) or H = ha (H - u) (hmode 3) with u = PC2VEL(psi, chi);
+// H on the right is read from `base` when base.e[0] is not null
 int launch_pc2vel_hist(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
-                       double2 *hr, double2 *hp, double2 *hz, int hmode, double ha);
+                       double2 *hr, double2 *hp, double2 *hz, int hmode, double ha, const VelTriple &base);
 // PC2VEL and PC2VOR of one (psi, chi): outputs rur, rup, uz, ror, rop, oz
 struct VelVorOut {
   double2 *e[6];
@@ -228,11 +234,13 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, doub
 int launch_axpby(mlg_ctx *c, int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n);
 int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1, const double *x2, const double *x3);
 // ETD update fused with the HYPERV3 filters (filter != 0), the copy of b
-// before the update (b0 != null) and the history stores (save != 0)
+// before the update (b0 != null) and the history stores (save bit 0: the
+// nonlinear terms, bit 1: the pressure-gradient terms); the updated b goes
+// to bout (null: b itself)
 int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt, double2 *rur, double2 *rup, double2 *uz,
                    double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_, double2 *hpr,
                    double2 *hpp, double2 *hpz, double2 *hnr, double2 *hnp, double2 *hnz, double2 *hnb, int ab, int save,
-                   double2 *b0, int filter, double tf);
+                   double2 *b0, double2 *bout, int filter, double tf);
 int launch_hyp3(mlg_ctx *c, double2 *a, int diff, double tf);
 int launch_diagnost(mlg_ctx *c, const double2 *psi);
 int launch_energy_terms(mlg_ctx *c, const double2 *a, const double2 *b, int l2, double *out_all);
