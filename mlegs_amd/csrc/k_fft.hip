@@ -17,6 +17,7 @@
 //
 // Power-of-two lengths only (all configurations); radix-2 in LDS.
 #include "mlegs_internal.h"
+#include "xcd.h"
 
 namespace mlg {
 
@@ -329,14 +330,19 @@ __device__ __forceinline__ void fft4_core(double2 (&v1)[R1], double2 (&v2)[R2], 
 
 // VERFFT forward: PFP (r, m, z) -> PFF (r, m, k chopped), / NX.
 template <int R1, int R2>
-__global__ void __launch_bounds__(256) k_verfft_fwd4(KGrid g, const double2 *__restrict__ tw, TransformJobs jobs) {
-  // grid layer z transforms field z of the batch
-  const double2 *__restrict__ in = jobs.in[blockIdx.z];
-  double2 *__restrict__ out = jobs.out[blockIdx.z];
+__global__ void __launch_bounds__(256) k_verfft_fwd4(KGrid g, const double2 *__restrict__ tw, TransformJobs jobs, int nj) {
+  // logical block (radial chunk, mode m, field of the batch), radial chunks
+  // fastest and grouped per XCD, so one XCD writes whole radial lines
   constexpr int N = R1 * R2, P = Fft4<R1, R2>::P;
+  const int nrb = (g.ndimr + P - 1) / P;
+  const int lid = xcd_logical(nrb * g.ntchopdim * nj);
+  if (lid < 0) return;
+  const int bx = lid % nrb, rest = lid / nrb, mm = rest % g.ntchopdim, fj = rest / g.ntchopdim;
+  const double2 *__restrict__ in = jobs.in[fj];
+  double2 *__restrict__ out = jobs.out[fj];
   __shared__ double2 s[P * N];
   const int r = threadIdx.x % P, t = threadIdx.x / P;
-  const int mm = blockIdx.y, rr = blockIdx.x * P + r;
+  const int rr = bx * P + r;
   double2 v1[R1], v2[R2];
   if (t < R2) {
 #pragma unroll
@@ -361,14 +367,19 @@ __global__ void __launch_bounds__(256) k_verfft_fwd4(KGrid g, const double2 *__r
 
 // VERFFT backward: PFF (r, m, k chopped) -> PFP (r, m, z), unscaled.
 template <int R1, int R2>
-__global__ void __launch_bounds__(512) k_verfft_bwd4(KGrid g, const double2 *__restrict__ tw, TransformJobs jobs) {
-  // grid layer z transforms field z of the batch
-  const double2 *__restrict__ in = jobs.in[blockIdx.z];
-  double2 *__restrict__ out = jobs.out[blockIdx.z];
+__global__ void __launch_bounds__(512) k_verfft_bwd4(KGrid g, const double2 *__restrict__ tw, TransformJobs jobs, int nj) {
+  // logical block (radial chunk, mode m, field of the batch), radial chunks
+  // fastest and grouped per XCD, so one XCD writes whole radial lines
   constexpr int N = R1 * R2, P = Fft4<R1, R2>::PB;
+  const int nrb = (g.ndimr + P - 1) / P;
+  const int lid = xcd_logical(nrb * g.ntchopdim * nj);
+  if (lid < 0) return;
+  const int bx = lid % nrb, rest = lid / nrb, mm = rest % g.ntchopdim, fj = rest / g.ntchopdim;
+  const double2 *__restrict__ in = jobs.in[fj];
+  double2 *__restrict__ out = jobs.out[fj];
   __shared__ double2 s[P * N];
   const int r = threadIdx.x % P, t = threadIdx.x / P;
-  const int mm = blockIdx.y, rr = blockIdx.x * P + r;
+  const int rr = bx * P + r;
   double2 v1[R1], v2[R2];
   if (t < R2) {
 #pragma unroll
@@ -504,9 +515,9 @@ int launch_verfft_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *co
     jobs.out[i] = out[i];
   }
   KScope ks(c, K_VERFFT_FWD, n * 16.0 * g.nr * g.ntchopdim * (double)(g.nx + g.nxchopdim));
-#define L_VF(a, b)                                                                                                \
-  hipLaunchKernelGGL((k_verfft_fwd4<a, b>), dim3((g.ndimr + Fft4<a, b>::P - 1) / Fft4<a, b>::P, g.ntchopdim, n), \
-                     dim3(256), 0, c->stream, g, c->d.tw_z, jobs)
+#define L_VF(a, b)                                                                                                  \
+  hipLaunchKernelGGL((k_verfft_fwd4<a, b>), dim3(xcd_grid((g.ndimr + Fft4<a, b>::P - 1) / Fft4<a, b>::P * g.ntchopdim * n)), \
+                     dim3(256), 0, c->stream, g, c->d.tw_z, jobs, n)
   FFT4_SWITCH(g.nx, L_VF)
 #undef L_VF
   MLG_CHECK_HIP(hipGetLastError());
@@ -532,9 +543,9 @@ int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *co
     jobs.out[i] = out[i];
   }
   KScope ks(c, K_VERFFT_BWD, n * 16.0 * g.nr * g.ntchopdim * (double)(g.nx + g.nxchopdim));
-#define L_VB(a, b)                                                                                                \
-  hipLaunchKernelGGL((k_verfft_bwd4<a, b>), dim3((g.ndimr + Fft4<a, b>::PB - 1) / Fft4<a, b>::PB, g.ntchopdim, n), \
-                     dim3(512), 0, c->stream, g, c->d.tw_z, jobs)
+#define L_VB(a, b)                                                                                                    \
+  hipLaunchKernelGGL((k_verfft_bwd4<a, b>), dim3(xcd_grid((g.ndimr + Fft4<a, b>::PB - 1) / Fft4<a, b>::PB * g.ntchopdim * n)), \
+                     dim3(512), 0, c->stream, g, c->d.tw_z, jobs, n)
   FFT4_SWITCH(g.nx, L_VB)
 #undef L_VB
   MLG_CHECK_HIP(hipGetLastError());
