@@ -234,6 +234,7 @@ extern "C" int mlg_init(const mlg_params *p, mlg_ctx **out) {
     RET(upload(&d.hv, hv));
   }
   MLG_CHECK_HIP(hipMalloc((void **)&d.red, sizeof(double) * (4 * (size_t)G.ntchop * G.nxchopdim + 2)));
+  MLG_CHECK_HIP(hipMalloc((void **)&d.vproj, sizeof(double) * ((size_t)G.nrchopdim + 1)));
   MLG_CHECK_HIP(hipMalloc((void **)&d.err, sizeof(int)));
   MLG_CHECK_HIP(hipMemset(d.err, 0, sizeof(int)));
   MLG_CHECK_HIP(hipStreamCreateWithFlags(&c->own_stream, hipStreamNonBlocking));
@@ -267,7 +268,7 @@ extern "C" int mlg_finalize(mlg_ctx *c) {
   c->timer.spare.clear();
   DevTables &d = c->d;
   void *ptrs[] = {d.pf, d.vdt, d.w, d.r, d.x, d.lnx, d.at1, d.ak, d.norm, d.lognorm, d.bxm, d.bxxdx, d.bdel2h,
-                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.red, d.err};
+                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.red, d.vproj, d.err};
   for (auto &f : c->fcache) (void)hipFree(f.f);
   c->fcache.clear();
   for (void *p : ptrs)

@@ -118,46 +118,38 @@ int launch_force_products(mlg_ctx *c, const double2 *rur, const double2 *rup, co
 // VEL2PLN (src/mod_legops.f90:2277-2339): the far-field r u_theta of the
 // (m=0, k=0) column fixes psi's log coefficient, LN = -u(inf)/2 with
 // u(inf) = sum_n a_n P_n(1) of the unnormalised expansion; that log image
-// is removed from the physical column.  One block.
-__global__ void __launch_bounds__(256) k_vel2pln(KGrid g, const double *__restrict__ pf, const double *__restrict__ w,
-                                                 const double *__restrict__ lognorm, double2 *__restrict__ rup,
-                                                 double *__restrict__ psi_ln, int ln_mode, double ln) {
-  extern __shared__ double crs[];  // [nn] projections, then the log value; Re rup [nr]; w [nrh]
-  const int nn = g.nrchop;         // NRCHOPS(1) at m = 0
-  double *ru = crs + nn + 1, *ws = ru + g.nr;
-  for (int i = threadIdx.x; i < g.nr; i += blockDim.x) ru[i] = rup[i].x;
-  for (int i = threadIdx.x; i < g.nrh; i += blockDim.x) ws[i] = w[i];
-  __syncthreads();
-  for (int n = threadIdx.x; n < nn; n += blockDim.x) {
-    double cr = 0.0;
-    const double *pfn = pf + (size_t)n * g.nrh;
-    for (int ib = 0; ib < g.nrh; ib += 8) {  // loads in blocks of 8, sum in ascending i
-      double p[8], q[8], wv[8], t[8];
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-        if (ib + u < g.nrh) {
-          p[u] = ru[ib + u];
-          q[u] = ru[g.nr - 1 - ib - u];
-          wv[u] = ws[ib + u];
-          t[u] = pfn[ib + u];
-        }
-#pragma unroll
-      for (int u = 0; u < 8; u++)
-        if (ib + u < g.nrh) {
-          double f = ((n & 1) ? (p[u] - q[u]) : (p[u] + q[u])) * wv[u];
-          cr += f * t[u];
-        }
-    }
-    crs[n] = cr * exp(lognorm[n]);
+// is removed from the physical column.
+// Pass 1: one wave per n forms the projection a_n P_n(1) (the parity-folded
+// Gauss sum of Re(r u_theta) against the table row, lanes over the nodes).
+__global__ void __launch_bounds__(256) k_vel2pln_proj(KGrid g, const double *__restrict__ pf, const double *__restrict__ w,
+                                                      const double *__restrict__ lognorm, const double2 *__restrict__ rup,
+                                                      double *__restrict__ proj) {
+  const int n = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int nn = g.nrchop;  // NRCHOPS(1) at m = 0
+  if (n >= nn) return;
+  const double *pfn = pf + (size_t)n * g.nrh;
+  double cr = 0.0;
+  for (int i = lane; i < g.nrh; i += 64) {
+    const double p = rup[i].x, q = rup[g.nr - 1 - i].x;
+    cr += ((n & 1) ? (p - q) : (p + q)) * w[i] * pfn[i];
   }
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) cr += __shfl_xor(cr, o, 64);
+  if (lane == 0) proj[n] = cr * exp(lognorm[n]);
+}
+// Pass 2 (one block of 256): LN = -(sum of the projections) / 2, and
+// the log image -LN P_n(x) (n = 0, 1) leaves the physical column.
+__global__ void __launch_bounds__(256) k_vel2pln_apply(KGrid g, const double *__restrict__ pf, const double *__restrict__ proj,
+                                                       double2 *__restrict__ rup, double *__restrict__ psi_ln, int ln_mode,
+                                                       double ln) {
+  __shared__ double part[4];
+  double v = 0.0;
+  for (int n = threadIdx.x; n < g.nrchop; n += 256) v += proj[n];
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
   __syncthreads();
-  if (threadIdx.x == 0) {
-    double s = 0.0;
-    for (int n = 0; n < nn; n++) s += crs[n];  // ascending n
-    crs[nn] = -s / 2.0;
-  }
-  __syncthreads();
-  const double pln = crs[nn];
+  const double pln = -(((part[0] + part[1]) + part[2]) + part[3]) / 2.0;
   for (int i = threadIdx.x; i < g.nrh; i += blockDim.x) {
     double e = -pf[i] * pln / g.norm0;
     double o = -pf[(size_t)g.nrh + i] * pln / g.norm1;
@@ -171,8 +163,10 @@ __global__ void __launch_bounds__(256) k_vel2pln(KGrid g, const double *__restri
 int launch_vel2pln(mlg_ctx *c, double2 *rup, double *psi_ln, int ln_mode, double ln) {
   KGrid g = kgrid(c);
   if (!rup || !psi_ln) return MLG_E_ARG;
-  hipLaunchKernelGGL(k_vel2pln, dim3(1), dim3(256), sizeof(double) * (g.nrchop + 1 + g.nr + g.nrh), c->stream, g, c->d.pf,
-                     c->d.w, c->d.lognorm, rup, psi_ln, ln_mode, ln);
+  if (g.nrchop > 0)
+    hipLaunchKernelGGL(k_vel2pln_proj, dim3((g.nrchop + 3) / 4), dim3(256), 0, c->stream, g, c->d.pf, c->d.w, c->d.lognorm,
+                       rup, c->d.vproj);
+  hipLaunchKernelGGL(k_vel2pln_apply, dim3(1), dim3(256), 0, c->stream, g, c->d.pf, c->d.vproj, rup, psi_ln, ln_mode, ln);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

@@ -44,6 +44,7 @@ struct DevTables {
   double2 *tw_h = nullptr;    // e^{-2 pi i j / (2 NTH)}, j <= NTH
   double *hv = nullptr;       // [hv_r | hv_t | hv_x | hd_r | hd_t | hd_x]
   double *red = nullptr;      // reduction scratch
+  double *vproj = nullptr;    // VEL2PLN projections [NRCHOPDIM]
   int *err = nullptr;         // sticky device error code
 };
 
