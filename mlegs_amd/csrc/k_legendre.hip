@@ -125,6 +125,9 @@ k_rtran_bwd(KGrid g, const double *__restrict__ pf, TransformJobs jobs, const do
       L.k.B[nl & 1][nl >> 1][(2 * kl + 1) ^ sw] = rb[q].y;
     }
   };
+  // the wave's column subtiles that hold data (the last tile of a mode is
+  // partly padding); the others skip their products (wave-uniform)
+  const bool live0 = c0 + wc * 32 < 2 * nk, live1 = c0 + wc * 32 + 16 < 2 * nk;
   if (nch > 0) fetch(0);
   for (int ch = 0; ch < nch; ch++) {
     __syncthreads();
@@ -134,7 +137,7 @@ k_rtran_bwd(KGrid g, const double *__restrict__ pf, TransformJobs jobs, const do
     const int jleft = njp - ch * BW_JC;  // pairs of this chunk that hold data
 #pragma unroll
     for (int ks = 0; ks < BW_JC; ks += 4) {
-      if (ks >= jleft) break;  // the rest of the chunk is zero padding
+      if (ks >= jleft || !live0) break;  // zero padding (chunk tail, or all the wave's columns)
       const int kr = ks + (lane >> 4);
 #pragma unroll
       for (int p = 0; p < 2; p++) {
@@ -144,9 +147,11 @@ k_rtran_bwd(KGrid g, const double *__restrict__ pf, TransformJobs jobs, const do
         double b0 = L.k.B[p][kr][(wc * 32 + (lane & 15)) ^ sb];
         double b1 = L.k.B[p][kr][(wc * 32 + 16 + (lane & 15)) ^ sb];
         acc[p][0][0] = mfma16x16x4(a0, b0, acc[p][0][0]);
-        acc[p][0][1] = mfma16x16x4(a0, b1, acc[p][0][1]);
         acc[p][1][0] = mfma16x16x4(a1, b0, acc[p][1][0]);
-        acc[p][1][1] = mfma16x16x4(a1, b1, acc[p][1][1]);
+        if (live1) {
+          acc[p][0][1] = mfma16x16x4(a0, b1, acc[p][0][1]);
+          acc[p][1][1] = mfma16x16x4(a1, b1, acc[p][1][1]);
+        }
       }
     }
   }
@@ -304,12 +309,16 @@ k_rtran_fwd(KGrid g, const double *__restrict__ pf, const double *__restrict__ w
       L.k.B[1][il][(2 * kl + 1) ^ sw] = (p.y - r.y) * rw;
     }
   };
+  // subtiles that hold data: rows n0 + 32 a + par + 2 l, columns c0 + 32 ch + 16 b + l
+  const bool ra0 = n0 + par < nn, ra1 = n0 + 32 + par < nn;
+  const bool rc0 = c0 + ch * 32 < 2 * nk, rc1 = c0 + ch * 32 + 16 < 2 * nk;
   fetch(0);
   for (int ib = 0; ib < g.nrh; ib += FW_IC) {
     __syncthreads();
     stash();
     __syncthreads();
     if (ib + FW_IC < g.nrh) fetch(ib + FW_IC);
+    if (!(ra0 && rc0)) continue;  // all the wave's rows or columns are padding
 #pragma unroll
     for (int ks = 0; ks < FW_IC; ks += 4) {
       const int kr = ks + (lane >> 4), sw = lsw(kr, 0), swa = lsw(kr, par);
@@ -318,9 +327,11 @@ k_rtran_fwd(KGrid g, const double *__restrict__ pf, const double *__restrict__ w
       const double b0 = L.k.B[par][kr][(ch * 32 + (lane & 15)) ^ sw];
       const double b1 = L.k.B[par][kr][(ch * 32 + 16 + (lane & 15)) ^ sw];
       acc[0][0] = mfma16x16x4(a0, b0, acc[0][0]);
-      acc[0][1] = mfma16x16x4(a0, b1, acc[0][1]);
-      acc[1][0] = mfma16x16x4(a1, b0, acc[1][0]);
-      acc[1][1] = mfma16x16x4(a1, b1, acc[1][1]);
+      if (rc1) acc[0][1] = mfma16x16x4(a0, b1, acc[0][1]);
+      if (ra1) {
+        acc[1][0] = mfma16x16x4(a1, b0, acc[1][0]);
+        if (rc1) acc[1][1] = mfma16x16x4(a1, b1, acc[1][1]);
+      }
     }
   }
   __syncthreads();
@@ -465,12 +476,15 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
     }
   };
   const int eo = par, oe = par ^ 1;
+  // subtiles that hold data: rows n0 + 32 a + par + 2 l, columns c0 + 16 ch + l
+  const bool pa0 = n0 + par < nn, pa1 = n0 + 32 + par < nn, pc0 = c0 + ch * 16 < 2 * nk;
   fetch(0);
   for (int ib = 0; ib < g.nrh; ib += PJ_IC) {
     __syncthreads();
     stash();
     __syncthreads();
     if (ib + PJ_IC < g.nrh) fetch(ib + PJ_IC);
+    if (!(pa0 && pc0)) continue;  // all the wave's rows or columns are padding
 #pragma unroll
     for (int ks = 0; ks < PJ_IC; ks += 4) {
       const int kr = ks + (lane >> 4), cl = (ch * 16 + (lane & 15)) ^ lsw(kr, 0), st = lsw(kr, par);
@@ -479,6 +493,7 @@ __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2))
       const double f2e = L.k.F[2][eo][kr][cl];
 #pragma unroll
       for (int a = 0; a < 2; a++) {
+        if (a == 1 && !pa1) break;
         const int rl = (16 * a + (lane & 15)) ^ st;
         const double av = L.k.T[0][par][kr][rl], ad = L.k.T[1][par][kr][rl], at = L.k.T[2][par][kr][rl];
         acc[a][0] = mfma16x16x4(av, f0e, acc[a][0]);
