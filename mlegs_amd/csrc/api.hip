@@ -854,6 +854,59 @@ static int to_pfp_n(mlg_ctx *c, int n, mlg_field *const *fs, mlg_field *const *k
   return 0;
 }
 
+// The seven FFF fields fs to PFP (as to_pfp_n, with keep for the first
+// three), together with GRAD's i m b and i k b in PFP (rbp, bz), which come
+// from one backward Legendre transform of b (b stays in FFF): the
+// z-transform multiplies by i m or i k as it loads.  The same linear maps as
+// GRAD then RTRAN, VERFFT, in another order.
+static int to_pfp_b(mlg_ctx *c, mlg_field *const *fs, const mlg_field *b, mlg_field *rbp, mlg_field *bz,
+                    mlg_field *const *keep) {
+  constexpr int n = 7;
+  for (int i = 0; i < n; i++)
+    if (fs[i]->space != MLG_FFF) return 2;
+  Temps T(c);
+  mlg_field *t[n + 1];
+  for (int i = 0; i <= n; i++) t[i] = T.buffer();
+  RET(T.ready());
+  const double2 *in[MAX_TRANSFORM_JOBS];
+  const double *ln[MAX_TRANSFORM_JOBS];
+  double2 *out[MAX_TRANSFORM_JOBS];
+  for (int i = 0; i < n; i++) {
+    in[i] = fs[i]->e;
+    ln[i] = fs[i]->ln;
+    out[i] = t[i]->e;
+  }
+  in[n] = b->e;
+  ln[n] = b->ln;
+  out[n] = t[n]->e;
+  RET(launch_rtran_bwd_n(c, n + 1, in, ln, out));  // padding rows included
+  double *lns[n];
+  for (int i = 0; i < n; i++) {
+    swap_storage(fs[i], t[i]);
+    if (keep && i < 3) {  // the FFF storage and its LN move to keep[i]
+      std::swap(keep[i]->e, t[i]->e);
+      std::swap(keep[i]->ln, fs[i]->ln);
+      keep[i]->space = MLG_FFF;
+    }
+    fs[i]->space = MLG_PFF;
+    lns[i] = fs[i]->ln;
+    in[i] = fs[i]->e;
+    out[i] = t[i]->e;
+  }
+  in[n] = t[n]->e;  // b in PFF, transformed twice: times i m and times i k
+  out[n] = rbp->e;
+  in[n + 1] = t[n]->e;
+  out[n + 1] = bz->e;
+  RET(launch_zero_lns(c, lns, n));
+  RET(launch_verfft_bwd_n(c, n + 2, in, out, 1u << (n + 1), 1u << n));
+  for (int i = 0; i < n; i++) {
+    swap_storage(fs[i], t[i]);
+    fs[i]->space = MLG_PFP;
+  }
+  rbp->space = bz->space = MLG_PFP;
+  return 0;
+}
+
 // CALC_BOUSSI_FORCE (src/mod_boussinesq.f90:317-410)
 extern "C" int mlg_force(const mlg_field *psi, const mlg_field *chi, const mlg_field *b, mlg_field *psin, mlg_field *chin,
                          mlg_field *bn) {
@@ -917,8 +970,37 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
   Temps T(c);
   mlg_field *rbr = T.get(), *rbp = T.get(), *bz = T.get(), *psi = T.get(), *chi = T.get(), *w = T.get();
   RET(T.ready());
+  int rc;
+  if (fused_products_supported(c) && c->g.nx <= 256) {
+    // GRAD's r d/dr b is formed in FFF; its i m b and i k b components come
+    // from b's own transform (to_pfp_b)
+    RET(mlg_chopset(c, 1));
+    rc = launch_grad(c, b->e, b->ln, rbr->e, rbr->ln, nullptr, nullptr, nullptr, nullptr);
+    mlg_chopset(c, -1);
+    RET(rc);
+    rbr->space = MLG_FFF;
+    mlg_field *fs7[7] = {rur, rup, uz, ror, rop, oz, rbr};
+    RET(to_pfp_b(c, fs7, b, rbp, bz, vel_fff));
+    const double2 *in[10] = {rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, nullptr};
+    double2 *out[4] = {ror->e, rop->e, oz->e, bn->e};
+    RET(launch_fused_products(c, in, out, 0.0, 0.0, 0));
+    bn->space = ror->space = rop->space = oz->space = MLG_PFP;
+    RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
+    if (!keep_velocity) {
+      RET(mlg_horfft(rur, 1));
+      RET(mlg_horfft(rup, 1));
+      RET(mlg_horfft(uz, 1));
+    }
+    RET(mlg_project(ror, rop, oz, psi, w, 1, 0.0));
+    RET(mlg_idel2(w, chi, 0, 0.0));
+    RET(mlg_chopset(c, 3));
+    rc = mlg_pc2vel(psi, chi, ror, rop, oz);
+    mlg_chopset(c, -3);
+    RET(rc);
+    return mlg_toff(bn);
+  }
   RET(mlg_chopset(c, 1));
-  int rc = mlg_grad(b, rbr, rbp, bz);
+  rc = mlg_grad(b, rbr, rbp, bz);
   mlg_chopset(c, -1);
   RET(rc);
   mlg_field *fs[9] = {rur, rup, uz, ror, rop, oz, rbr, rbp, bz};

@@ -367,7 +367,8 @@ __global__ void __launch_bounds__(256) k_verfft_fwd4(KGrid g, const double2 *__r
 
 // VERFFT backward: PFF (r, m, k chopped) -> PFP (r, m, z), unscaled.
 template <int R1, int R2>
-__global__ void __launch_bounds__(512) k_verfft_bwd4(KGrid g, const double2 *__restrict__ tw, TransformJobs jobs, int nj) {
+__global__ void __launch_bounds__(512) k_verfft_bwd4(KGrid g, const double2 *__restrict__ tw, TransformJobs jobs, int nj,
+                                                      const double *__restrict__ ak) {
   // logical block (radial chunk, mode m, field of the batch), radial chunks
   // fastest and grouped per XCD, so one XCD writes whole radial lines
   constexpr int N = R1 * R2, P = Fft4<R1, R2>::PB;
@@ -377,6 +378,7 @@ __global__ void __launch_bounds__(512) k_verfft_bwd4(KGrid g, const double2 *__r
   const int bx = lid % nrb, rest = lid / nrb, mm = rest % g.ntchopdim, fj = rest / g.ntchopdim;
   const double2 *__restrict__ in = jobs.in[fj];
   double2 *__restrict__ out = jobs.out[fj];
+  const bool dz = (jobs.dz >> fj) & 1u, dm = (jobs.dm >> fj) & 1u;
   __shared__ double2 s[P * N];
   const int r = threadIdx.x % P, t = threadIdx.x / P;
   const int rr = bx * P + r;
@@ -390,6 +392,22 @@ __global__ void __launch_bounds__(512) k_verfft_bwd4(KGrid g, const double2 *__r
       else if (k > N - g.nxchop) kk = g.nxchopdim + (k - N);
       v1[n1] = (rr < g.nr && kk >= 0) ? in[(size_t)rr + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * kk)]
                                       : make_double2(0.0, 0.0);
+    }
+    if (dz) {  // d/dz: i k a (GRAD's third component), after all loads are issued
+#pragma unroll
+      for (int n1 = 0; n1 < R1; n1++) {
+        const int k = R2 * n1 + t;
+        int kk = -1;
+        if (k < g.nxchop) kk = k;
+        else if (k > N - g.nxchop) kk = g.nxchopdim + (k - N);
+        const double kv = kk >= 0 ? ak[mm + (size_t)g.ntchopdim * kk] : 0.0;
+        v1[n1] = make_double2(-kv * v1[n1].y, kv * v1[n1].x);
+      }
+    }
+    if (dm) {  // d/dtheta: i m a (GRAD's second component)
+      const double mv = (double)(mm * g.minc);
+#pragma unroll
+      for (int n1 = 0; n1 < R1; n1++) v1[n1] = make_double2(-mv * v1[n1].y, mv * v1[n1].x);
     }
   }
   fft4_core<R1, R2, P>(v1, v2, s, r, t, tw, true);
@@ -526,10 +544,11 @@ int launch_verfft_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *co
 
 int launch_verfft_fwd(mlg_ctx *c, const double2 *in, double2 *out) { return launch_verfft_fwd_n(c, 1, &in, &out); }
 
-int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out) {
+int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out, unsigned dz, unsigned dm) {
   if (n < 1 || n > MAX_TRANSFORM_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
   if (g.nx > 256) {
+    if (dz || dm) return MLG_E_ARG;  // derivatives only in the four-step kernel
     for (int i = 0; i < n; i++) {
       const int rc = launch_verfft_bwd_r2(c, in[i], out[i]);
       if (rc) return rc;
@@ -542,10 +561,12 @@ int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *co
     jobs.ln[i] = nullptr;
     jobs.out[i] = out[i];
   }
+  jobs.dz = dz;
+  jobs.dm = dm;
   KScope ks(c, K_VERFFT_BWD, n * 16.0 * g.nr * g.ntchopdim * (double)(g.nx + g.nxchopdim));
 #define L_VB(a, b)                                                                                                    \
   hipLaunchKernelGGL((k_verfft_bwd4<a, b>), dim3(xcd_grid((g.ndimr + Fft4<a, b>::PB - 1) / Fft4<a, b>::PB * g.ntchopdim * n)), \
-                     dim3(512), 0, c->stream, g, c->d.tw_z, jobs, n)
+                     dim3(512), 0, c->stream, g, c->d.tw_z, jobs, n, c->d.ak)
   FFT4_SWITCH(g.nx, L_VB)
 #undef L_VB
   MLG_CHECK_HIP(hipGetLastError());

@@ -398,13 +398,17 @@ __global__ void __launch_bounds__(256) k_grad(KGrid g, const double *bxxdx, cons
       if (n == 1) vr.x = vr.x + ln / g.norm1;
     }
     rr[cb + n] = vr;
-    rp[cb + n] = vp;
-    z[cb + n] = vz;
+    if (rp) {  // null rp, z: the r d/dr component only
+      rp[cb + n] = vp;
+      z[cb + n] = vz;
+    }
   }
   if (mm == 0 && k == 0 && threadIdx.x == 0) {
     ln_rr[0] = 0.0;
-    ln_rp[0] = 0.0;
-    ln_z[0] = 0.0;
+    if (rp) {
+      ln_rp[0] = 0.0;
+      ln_z[0] = 0.0;
+    }
   }
 }
 int launch_grad(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *rr, double *ln_rr, double2 *rp, double *ln_rp,

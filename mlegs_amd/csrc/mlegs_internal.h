@@ -161,9 +161,14 @@ struct TransformJobs {
   const double2 *in[MAX_TRANSFORM_JOBS];
   const double *ln[MAX_TRANSFORM_JOBS];
   double2 *out[MAX_TRANSFORM_JOBS];
+  unsigned dz = 0;  // VERFFT backward: bit j set = job j transforms i k times its input (d/dz)
+  unsigned dm = 0;  // VERFFT backward: bit j set = job j transforms i m times its input (d/dtheta)
 };
 int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double *const *ln_in, double2 *const *out);
-int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out);
+// dz, dm: bit j set = job j transforms d/dz (i k times) or d/dtheta (i m
+// times) of its input (four-step lengths, NX <= 256, only)
+int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out, unsigned dz = 0,
+                        unsigned dm = 0);
 int launch_verfft_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out);
 int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out);
 int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *psi, double2 *chi);
