@@ -1227,17 +1227,6 @@ static int visc1_rhs(mlg_field *a, double nu, double dt, mlg_field *wk, double *
   return launch_scale(c, wk->e, -*alp, c->n_fff);
 }
 
-// VISC2 (mod_march.f90:1579-1636): Crank-Nicolson diffusion; the right-hand
-// side WK = -DEL2(AO) - ALP*A, then A = IHELM(WK, ALP) (batched)
-static int visc2_rhs(const mlg_field *a, const mlg_field *ao, double nu, double dt, mlg_field *wk, double *alp) {
-  if (nu == 0.0) return 5;
-  if (ao->space != MLG_FFF) return 3;  // DEL2's input check
-  mlg_ctx *c = a->ctx;
-  *alp = 2 / nu / dt;
-  // CHOPSET(2); WK = DEL2(AO); CHOPSET(-2); CHOPDO(WK); WK = -WK - ALP*A in one pass
-  wk->space = MLG_FFF;
-  return launch_visc2_rhs(c, ao->e, ao->ln, a->e, a->ln, *alp, wk->e, wk->ln);
-}
 
 // psi, chi (NU) and b (KAPPA, when non-zero) are independent: their right-
 // hand sides are formed in turn, then the three IHELM solves share a launch.
@@ -1265,7 +1254,18 @@ static int visc_cn(mlg_ctx *c, mlg_field *const *f, const mlg_field *const *fa, 
   const double nu[3] = {c->g.p.nu, c->g.p.nu, c->g.p.kappa};
   const int n = c->g.p.kappa != 0.0 ? 3 : 2;
   double alp[3];
-  for (int i = 0; i < n; i++) RET(visc2_rhs(fa[i], fo[i], nu[i], c->g.p.dt, wk[i], &alp[i]));
+  // VISC2 (mod_march.f90:1579-1636): Crank-Nicolson diffusion, right-hand
+  // sides WK = -DEL2(AO) - ALP*A (CHOPSET(2); DEL2; CHOPSET(-2); CHOPDO; axpy
+  // in one pass) of all fields in one launch, then A = IHELM(WK, ALP)
+  Visc2Jobs jobs;
+  for (int i = 0; i < n; i++) {
+    if (nu[i] == 0.0) return 5;
+    if (fo[i]->space != MLG_FFF) return 3;  // DEL2's input check
+    alp[i] = 2 / nu[i] / c->g.p.dt;
+    wk[i]->space = MLG_FFF;
+    jobs.j[i] = Visc2Job{fo[i]->e, fo[i]->ln, fa[i]->e, fa[i]->ln, alp[i], wk[i]->e, wk[i]->ln};
+  }
+  RET(launch_visc2_rhs_n(c, n, jobs));
   mlg_field *out[3] = {f[0], f[1], f[2]};
   return ihelm_n(n, wk, out, alp);
 }

@@ -317,11 +317,17 @@ int launch_del2(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, do
 //   WK%LN = -ALP*A%LN
 // with DEL2 evaluated on the truncation NRCHOP+2 (nrchop2) at the positions
 // the base truncation keeps, and DEL2's top-mode check of AO at NRCHOP+2.
-__global__ void __launch_bounds__(256) k_visc2_rhs(KGrid g, int nrchop2, const double *bd, const double *ak,
-                                                  const double2 *ao, const double *ln_ao, const double2 *a,
-                                                  const double *ln_a, double alp, double2 *wk, double *ln_wk, int *err) {
+// grid layer z: job z of the batch (independent right-hand sides)
+__global__ void __launch_bounds__(256) k_visc2_rhs(KGrid g, int nrchop2, const double *bd, const double *ak, Visc2Jobs jobs,
+                                                  int *err) {
   extern __shared__ __align__(16) double2 csm[];
-  const double ln = ln_ao[0];
+  const Visc2Job &J = jobs.j[blockIdx.z];
+  const double2 *ao = J.ao, *a = J.a;
+  const double *ln_a = J.ln_a;
+  const double alp = J.alp;
+  double2 *wk = J.wk;
+  double *ln_wk = J.ln_wk;
+  const double ln = J.ln_ao[0];
   const int mm = blockIdx.x, k = blockIdx.y;
   const int mv = mm * g.minc;
   const int nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0, nn2 = s_nrchops(nrchop2, mv);
@@ -346,16 +352,19 @@ __global__ void __launch_bounds__(256) k_visc2_rhs(KGrid g, int nrchop2, const d
     }
   }
 }
-int launch_visc2_rhs(mlg_ctx *c, const double2 *ao, const double *ln_ao, const double2 *a, const double *ln_a, double alp,
-                     double2 *wk, double *ln_wk) {
+int launch_visc2_rhs_n(mlg_ctx *c, int n, const Visc2Jobs &jobs) {
+  if (n < 1 || n > 3) return MLG_E_ARG;
   KGrid g = kgrid(c);
   const int nrchop2 = g.nrchop + 2;
   if (nrchop2 > g.nrchopdim) return 1;  // CHOPSET: NRCHOP TOO LARGE (mod_scalar3.f90:183-190)
-  hipLaunchKernelGGL(k_visc2_rhs, column_grid(g), dim3(256), sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, nrchop2,
-                     c->d.bdel2h, c->d.ak, ao, ln_ao, a, ln_a, alp, wk, ln_wk, c->d.err);
+  dim3 grid = column_grid(g);
+  grid.z = n;
+  hipLaunchKernelGGL(k_visc2_rhs, grid, dim3(256), sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, nrchop2, c->d.bdel2h,
+                     c->d.ak, jobs, c->d.err);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
+
 
 // DEL2's exactness check (:778-790): the top mode of m=1 at k=0 must vanish.
 __global__ void k_check_top_m1(KGrid g, const double2 *a, int *err) {

@@ -197,8 +197,20 @@ int launch_zero(mlg_ctx *c, double2 *a, long n);
 int launch_copy(mlg_ctx *c, double2 *dst, const double2 *src, long n);
 int launch_copy0_fff(mlg_ctx *c, double2 *dst, double *dst_ln, const double2 *src, const double *src_ln);
 // VISC2's right-hand side WK = -CHOP(DEL2 at NRCHOP+2 of AO) - ALP*A, WK%LN = -ALP*A%LN
-int launch_visc2_rhs(mlg_ctx *c, const double2 *ao, const double *ln_ao, const double2 *a, const double *ln_a, double alp,
-                     double2 *wk, double *ln_wk);
+struct Visc2Job {
+  const double2 *ao;
+  const double *ln_ao;
+  const double2 *a;
+  const double *ln_a;
+  double alp;
+  double2 *wk;
+  double *ln_wk;
+};
+struct Visc2Jobs {
+  Visc2Job j[3];
+};
+// n (<= 3) independent right-hand sides in one launch
+int launch_visc2_rhs_n(mlg_ctx *c, int n, const Visc2Jobs &jobs);
 int launch_set_ln(mlg_ctx *c, double *dst, const double *src, double scale, double add);
 int launch_zero_lns(mlg_ctx *c, double *const *ptrs, int n);  // LN = 0 for n fields, one launch per 32
 int launch_delsqh(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, double *ln_b);
