@@ -781,17 +781,30 @@ __device__ void gen_row(const SolveCtx &s, int q, double *out, int NE) {
   }
 }
 
-// Factor storage: row nn of class cls (= m index + NTCHOPDIM*|k| index)
-// keeps FSTRIDE doubles at F[(nn*ncls + cls)*FSTRIDE + e]:
-//   e < C0          multiplier of row nn against row nn-1-e (scaled by its pivot)
-//   C0 <= e < C0+KU normalised upper entry (nn, nn+1+e-C0)
-//   e = NE-1        1/pivot
-// so a thread reads one contiguous 56-byte record per row and a wave of
-// consecutive m reads one contiguous run.
+// Factor storage of a set: three regions over (row nn, class cls), class =
+// m index + NTCHOPDIM*|k| index, NRCHOPDIM rows:
+//   penta-diagonal classes (C0 = 2, KU = 2), split by substitution pass:
+//     FF[(nn*ncls + cls)*3 + e]: the C0 multipliers of row nn against rows
+//        nn-1-e (scaled by their pivots), then 1/pivot   (forward pass)
+//     FB[(nn*ncls + cls)*2 + u]: the KU normalised upper entries (nn, nn+1+u)
+//        (back substitution)
+//   7-band classes (m = 0 forms): F7[(nn*ncls + cls)*FSTRIDE + e] with
+//     e < C0 multipliers, C0 <= e < C0+KU upper entries, e = NE-1 1/pivot.
+// Each pass reads only its part, and a wave of consecutive m reads one
+// contiguous run per row.
 #define FSTRIDE 7
+#define FSET_DOUBLES (3 + 2 + FSTRIDE)  // per (row, class)
+struct FactorView {
+  double *ff, *fb, *f7;
+};
+__host__ __device__ __forceinline__ FactorView factor_view(const double *F, int rows, long ncls) {
+  double *f = const_cast<double *>(F);
+  const size_t rc = (size_t)rows * ncls;
+  return FactorView{f, f + 3 * rc, f + 5 * rc};
+}
 
 template <int NE, int C0>
-__device__ void band_factor(const SolveCtx &s, int nd, double *F, long ncls, long cls) {
+__device__ void band_factor(const SolveCtx &s, int nd, const FactorView &FV, long ncls, long cls) {
   constexpr int KU = NE - 1 - C0;
   double W[C0 + 1][NE];
   for (int q = 0; q <= C0; q++) {
@@ -810,10 +823,17 @@ __device__ void band_factor(const SolveCtx &s, int nd, double *F, long ncls, lon
       }
     for (int j = 1; j <= C0; j++)
       if (nn + j < nd) W[j][C0 - j] /= piv;
-    double *fr = F + ((size_t)nn * ncls + cls) * FSTRIDE;
-    for (int j = 1; j <= C0; j++) fr[j - 1] = W[0][C0 - j];
-    for (int u = 0; u < KU; u++) fr[C0 + u] = W[0][C0 + 1 + u];
-    fr[NE - 1] = 1.0 / piv;
+    if (NE == 5) {
+      double *ff = FV.ff + ((size_t)nn * ncls + cls) * 3, *fb = FV.fb + ((size_t)nn * ncls + cls) * 2;
+      for (int j = 1; j <= C0; j++) ff[j - 1] = W[0][C0 - j];
+      ff[2] = 1.0 / piv;
+      for (int u = 0; u < KU; u++) fb[u] = W[0][C0 + 1 + u];
+    } else {
+      double *fr = FV.f7 + ((size_t)nn * ncls + cls) * FSTRIDE;
+      for (int j = 1; j <= C0; j++) fr[j - 1] = W[0][C0 - j];
+      for (int u = 0; u < KU; u++) fr[C0 + u] = W[0][C0 + 1 + u];
+      fr[NE - 1] = 1.0 / piv;
+    }
     for (int q = 0; q < C0; q++)
       for (int b = 0; b < NE; b++) W[q][b] = W[q + 1][b];
     if (nn + C0 + 1 < nd) gen_row(s, nn + C0 + 1, W[C0], NE);
@@ -908,8 +928,9 @@ __device__ void band_apply(int nd, const double2 *xin, double2 *x, const double 
 // rows are loaded once and the NJ substitutions run side by side (each with
 // band_apply's arithmetic).
 template <int NJ, int NE, int C0, int SB>
-__device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], double2 *const (&x)[NJ], const double *__restrict__ F,
+__device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], double2 *const (&x)[NJ], const FactorView &FV,
                              long ncls, long cls) {
+  static_assert(NE == 5 && C0 == 2, "the split factor records are penta-diagonal");
   constexpr int KU = NE - 1 - C0;
   double2 xh[NJ][C0];
 #pragma unroll
@@ -923,12 +944,12 @@ __device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], double2 *c
     for (int q = 0; q < SB; q++) {
       const int nn = nb + q;
       if (nn < nd) {
-        const double *fr = F + ((size_t)nn * ncls + cls) * FSTRIDE;
+        const double *fr = FV.ff + ((size_t)nn * ncls + cls) * 3;
 #pragma unroll
         for (int jj = 0; jj < NJ; jj++) rhs[q][jj] = xin[jj][nn];
 #pragma unroll
         for (int j = 0; j < C0; j++) lm[q][j] = fr[j];
-        rp[q] = fr[NE - 1];
+        rp[q] = fr[2];
       }
     }
 #pragma unroll
@@ -967,11 +988,11 @@ __device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], double2 *c
     for (int q = 0; q < SB; q++) {
       const int nn = nt - q;
       if (nn >= 0) {
-        const double *fr = F + ((size_t)nn * ncls + cls) * FSTRIDE;
+        const double *fr = FV.fb + ((size_t)nn * ncls + cls) * 2;
 #pragma unroll
         for (int jj = 0; jj < NJ; jj++) v0[q][jj] = x[jj][nn];
 #pragma unroll
-        for (int u = 0; u < KU; u++) um[q][u] = fr[C0 + u];
+        for (int u = 0; u < KU; u++) um[q][u] = fr[u];
       }
     }
 #pragma unroll
@@ -1073,9 +1094,10 @@ __global__ void __launch_bounds__(64) k_band_factor(KGrid g, const double *bd, c
   if (nn < 1 || mm >= g.ntchop) return;
   SolveCtx s;
   solve_setup(g, bd, bxm, ak, op, alp, mm, kabs, s);
-  if (s.kind == 0) band_factor<5, 2>(s, nn, F, ncls, cls);
-  else if (s.kind == 1) band_factor<7, 3>(s, nn, F, ncls, cls);
-  else band_factor<7, 3>(s, nn - 1, F, ncls, cls);
+  const FactorView FV = factor_view(F, g.nrchopdim, ncls);
+  if (s.kind == 0) band_factor<5, 2>(s, nn, FV, ncls, cls);
+  else if (s.kind == 1) band_factor<7, 3>(s, nn, FV, ncls, cls);
+  else band_factor<7, 3>(s, nn - 1, FV, ncls, cls);
 }
 
 // IDEL2 / IDEL2LN / IHELM (src/mod_legops.f90:823-1354) on the cached
@@ -1129,8 +1151,8 @@ __device__ void apply_penta_n(const KGrid &g, int op, int has_ln, const BandJobs
     xin[jj] = J.b + col;
     x[jj] = J.a + col;
   }
-  const double *F = jobs.j[jobs.gjob[grp][0]].F;
-  band_apply_n<NJ, 5, 2, (NJ == 1 ? 16 : 8)>(nn, xin, x, F, ncls, mm + (long)g.ntchopdim * kabs);
+  const FactorView FV = factor_view(jobs.j[jobs.gjob[grp][0]].F, g.nrchopdim, ncls);
+  band_apply_n<NJ, 5, 2, (NJ == 1 ? 16 : 8)>(nn, xin, x, FV, ncls, mm + (long)g.ntchopdim * kabs);
   if (mm == 0 && k == 0) {
 #pragma unroll
     for (int jj = 0; jj < NJ; jj++) {
@@ -1173,7 +1195,7 @@ __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, 
   for (int row = threadIdx.x; row < nn; row += blockDim.x) {
     x[row] = xb[row];
     if (row < nd) {
-      const double *fr = F + ((size_t)row * ncls + cls) * FSTRIDE;
+      const double *fr = factor_view(F, g.nrchopdim, ncls).f7 + ((size_t)row * ncls + cls) * FSTRIDE;
 #pragma unroll
       for (int e = 0; e < FSTRIDE; e++) Fs[row * FSTRIDE + e] = fr[e];
     }
@@ -1244,7 +1266,7 @@ static double *factor_set(mlg_ctx *c, int op, double alp, const double *const *k
   for (auto &f : c->fcache)
     if (f.op == op && f.alp == key_alp && f.nrchop == g.nrchop) return f.f;
   const long ncls = (long)g.ntchopdim * g.nxchop;
-  size_t n = (size_t)g.nrchopdim * FSTRIDE * ncls;
+  size_t n = (size_t)g.nrchopdim * FSET_DOUBLES * ncls;
   if (c->fcache.size() >= 8) {
     for (auto it = c->fcache.begin(); it != c->fcache.end(); ++it) {
       bool used = false;
