@@ -639,9 +639,12 @@ static int solve_op(const mlg_field *b, mlg_field *a, int op, double alp, int ha
   mlg_ctx *c = b->ctx;
   // A = B then the substitution in A (the reference's COPY + in-place solve):
   // the solve reads B and writes A directly, the CHOPDO below clears the rest
+  Temps T(c);
+  mlg_field *t = T.buffer();
+  RET(T.ready());
   a->space = MLG_FFF;
   if (op != 2) RET(launch_set_ln(c, a->ln, nullptr, 0.0, 0.0));
-  RET(launch_band_solve(c, op, b->e, a->e, a->ln, b->ln, alp, has_ln, ln));
+  RET(launch_band_solve(c, op, b->e, a->e, a->ln, b->ln, alp, has_ln, ln, t->e));
   RET(launch_chopdo(c, a->e, MLG_FFF));
   if (op != 2) RET(launch_fix_at_infinity(c, a->e));
   return 0;
@@ -1210,7 +1213,13 @@ static int ihelm_n(int n, mlg_field *const *b, mlg_field *const *a, const double
     aln[i] = a[i]->ln;
     bln[i] = b[i]->ln;
   }
-  RET(launch_band_solve_n(c, 2, n, be, ae, aln, bln, alp, 0, 0.0));
+  Temps T(c);
+  mlg_field *tf[3] = {nullptr, nullptr, nullptr};
+  for (int i = 0; i < n; i++) tf[i] = T.buffer();
+  RET(T.ready());
+  double2 *te[3] = {nullptr, nullptr, nullptr};
+  for (int i = 0; i < n; i++) te[i] = tf[i]->e;
+  RET(launch_band_solve_n(c, 2, n, be, ae, aln, bln, alp, 0, 0.0, te));
   for (int i = 0; i < n; i++) RET(launch_chopdo(c, a[i]->e, MLG_FFF));
   return 0;
 }

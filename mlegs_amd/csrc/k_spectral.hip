@@ -927,9 +927,13 @@ __device__ void band_apply(int nd, const double2 *xin, double2 *x, const double 
 // band_apply for NJ right-hand sides that share one factor set: the factor
 // rows are loaded once and the NJ substitutions run side by side (each with
 // band_apply's arithmetic).
+// The forward pass's results go to xt (row nn at xt[nn * st]: with st the
+// number of systems, consecutive systems' rows are adjacent, so a wave's
+// accesses coalesce), which the back substitution reads; only its results
+// go to the column x.
 template <int NJ, int NE, int C0, int SB>
-__device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], double2 *const (&x)[NJ], const FactorView &FV,
-                             long ncls, long cls) {
+__device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], double2 *const (&xt)[NJ], double2 *const (&x)[NJ],
+                             long st, const FactorView &FV, long ncls, long cls) {
   static_assert(NE == 5 && C0 == 2, "the split factor records are penta-diagonal");
   constexpr int KU = NE - 1 - C0;
   double2 xh[NJ][C0];
@@ -974,7 +978,7 @@ __device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], double2 *c
     for (int q = 0; q < SB; q++)
       if (nb + q < nd)
 #pragma unroll
-        for (int jj = 0; jj < NJ; jj++) x[jj][nb + q] = rhs[q][jj];
+        for (int jj = 0; jj < NJ; jj++) xt[jj][(nb + q) * st] = rhs[q][jj];
   }
   double2 yh[NJ][KU];
 #pragma unroll
@@ -990,7 +994,7 @@ __device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], double2 *c
       if (nn >= 0) {
         const double *fr = FV.fb + ((size_t)nn * ncls + cls) * 2;
 #pragma unroll
-        for (int jj = 0; jj < NJ; jj++) v0[q][jj] = x[jj][nn];
+        for (int jj = 0; jj < NJ; jj++) v0[q][jj] = xt[jj][nn * st];
 #pragma unroll
         for (int u = 0; u < KU; u++) um[q][u] = fr[u];
       }
@@ -1122,6 +1126,7 @@ struct BandJob {
   const double *ln_b;
   const double *F;
   double alp;
+  double2 *t;  // scratch of FFF size: forward-pass results, system-interleaved
 };
 constexpr int MAX_BAND_JOBS = 3;
 struct BandJobs {
@@ -1144,15 +1149,16 @@ __device__ void apply_penta_n(const KGrid &g, int op, int has_ln, const BandJobs
   const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
   const size_t col = (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
   const double2 *xin[NJ];
-  double2 *x[NJ];
+  double2 *x[NJ], *xt[NJ];
 #pragma unroll
   for (int jj = 0; jj < NJ; jj++) {
     const BandJob &J = jobs.j[jobs.gjob[grp][jj]];
     xin[jj] = J.b + col;
     x[jj] = J.a + col;
+    xt[jj] = J.t + sys;
   }
   const FactorView FV = factor_view(jobs.j[jobs.gjob[grp][0]].F, g.nrchopdim, ncls);
-  band_apply_n<NJ, 5, 2, (NJ == 1 ? 16 : 8)>(nn, xin, x, FV, ncls, mm + (long)g.ntchopdim * kabs);
+  band_apply_n<NJ, 5, 2, (NJ == 1 ? 16 : 8)>(nn, xin, xt, x, nsys, FV, ncls, mm + (long)g.ntchopdim * kabs);
   if (mm == 0 && k == 0) {
 #pragma unroll
     for (int jj = 0; jj < NJ; jj++) {
@@ -1295,7 +1301,7 @@ static double *factor_set(mlg_ctx *c, int op, double alp, const double *const *k
 }
 
 int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, double2 *const *a, double *const *ln_a,
-                        const double *const *ln_b, const double *alp, int has_ln, double ln) {
+                        const double *const *ln_b, const double *alp, int has_ln, double ln, double2 *const *t) {
   if (n < 1 || n > MAX_BAND_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
   long nsys = (long)g.ntchopdim * g.nxchopdim;
@@ -1306,7 +1312,8 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, doub
     const double *F = factor_set(c, op, alp[i], used, i, &rc);
     if (!F) return rc;
     used[i] = F;
-    jobs.j[i] = BandJob{b[i], a[i], ln_a[i], ln_b[i], F, alp[i]};
+    if (!t[i]) return MLG_E_ARG;
+    jobs.j[i] = BandJob{b[i], a[i], ln_a[i], ln_b[i], F, alp[i], t[i]};
   }
   jobs.n = n;
   jobs.ng = 0;
@@ -1325,8 +1332,8 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, doub
 }
 
 int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *ln_a, const double *ln_b, double alp,
-                      int has_ln, double ln) {
-  return launch_band_solve_n(c, op, 1, &b, &a, &ln_a, &ln_b, &alp, has_ln, ln);
+                      int has_ln, double ln, double2 *t) {
+  return launch_band_solve_n(c, op, 1, &b, &a, &ln_a, &ln_b, &alp, has_ln, ln, &t);
 }
 
 // ---------------------------------------------------------------- pointwise updates

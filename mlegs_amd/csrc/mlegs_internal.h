@@ -243,12 +243,14 @@ int launch_pc2vor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const do
 int launch_fix_at_infinity(mlg_ctx *c, double2 *a);
 // op: 0 IDEL2, 1 IDEL2LN, 2 IHELM
 // A = solve(B) on the FFF coefficients inside the truncation (B may be A);
-// the caller chops A afterwards
+// the caller chops A afterwards.  t: scratch of FFF size (no other use
+// during the solve)
 int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *ln_a, const double *ln_b, double alp,
-                      int has_ln, double ln);
-// n (<= 3) independent solves of one operator in one launch (field i with alp[i])
+                      int has_ln, double ln, double2 *t);
+// n (<= 3) independent solves of one operator in one launch (field i with
+// alp[i], scratch t[i])
 int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, double2 *const *a, double *const *ln_a,
-                        const double *const *ln_b, const double *alp, int has_ln, double ln);
+                        const double *const *ln_b, const double *alp, int has_ln, double ln, double2 *const *t);
 int launch_axpby(mlg_ctx *c, int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n);
 int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1, const double *x2, const double *x3);
 // ETD update fused with the HYPERV3 filters (filter != 0), the copy of b
