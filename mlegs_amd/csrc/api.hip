@@ -645,8 +645,7 @@ static int solve_op(const mlg_field *b, mlg_field *a, int op, double alp, int ha
   a->space = MLG_FFF;
   if (op != 2) RET(launch_set_ln(c, a->ln, nullptr, 0.0, 0.0));
   RET(launch_band_solve(c, op, b->e, a->e, a->ln, b->ln, alp, has_ln, ln, t->e));
-  RET(launch_chopdo(c, a->e, MLG_FFF));
-  if (op != 2) RET(launch_fix_at_infinity(c, a->e));
+  RET(launch_chopdo_fff_n(c, 1, &a->e, op != 2));  // CHOPDO, and the fix at infinity of IDEL2, IDEL2LN
   return 0;
 }
 extern "C" int mlg_idel2(const mlg_field *b, mlg_field *a, int has_ln, double ln) { return solve_op(b, a, 0, 0.0, has_ln, ln); }
@@ -1220,8 +1219,7 @@ static int ihelm_n(int n, mlg_field *const *b, mlg_field *const *a, const double
   double2 *te[3] = {nullptr, nullptr, nullptr};
   for (int i = 0; i < n; i++) te[i] = tf[i]->e;
   RET(launch_band_solve_n(c, 2, n, be, ae, aln, bln, alp, 0, 0.0, te));
-  for (int i = 0; i < n; i++) RET(launch_chopdo(c, a[i]->e, MLG_FFF));
-  return 0;
+  return launch_chopdo_fff_n(c, n, ae, false);
 }
 
 // VISC1 (mod_march.f90:1529-1576): backward Euler diffusion; the right-hand
@@ -1471,18 +1469,20 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   // write the new ones into them.  Without VISC2 they are updated in place.
   const bool cn = c->g.p.visc_sw == 1;
   mlg_field *bj = b, *pj = psi, *cj = chi;
-  if (cn) {
+  double *lnd[2] = {nullptr, nullptr};
+  const double *lns[2] = {nullptr, nullptr};
+  if (cn) {  // b1, psi1 take the LN of b, psi (copied by the ETD launch)
     bj = b1, pj = psi1, cj = chi1;
     bj->space = pj->space = MLG_FFF;
-    RET(launch_set_ln(c, bj->ln, b->ln, 1.0, 0.0));
-    RET(launch_set_ln(c, pj->ln, psi->ln, 1.0, 0.0));
+    lnd[0] = bj->ln, lns[0] = b->ln;
+    lnd[1] = pj->ln, lns[1] = psi->ln;
   }
   // ETD_HYPERV(N) ; ETDAB_SUB ; N history ; ETD_HYPERV(u, b) ; pi history:
   // R_PI = u is kept in hk (the velocities' FFF elements, which PROJECT
   // transforms) and combined with the second PC2VEL below
   RET(launch_etd_sub(c, s->etd_e, s->etd_nl, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
                      s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 1, 1, nullptr, bj->e,
-                     filter_active(c), dt / c->g.p.dt));
+                     filter_active(c), dt / c->g.p.dt, lnd, lns));
   RET(project_impl(rurn, rupn, uzn, pj, w, 2, 0.0, hk));  // PROJECT(..., PSI, W, PSI%LN)
   RET(mlg_idel2ln(w, cj));
   RET(pc2vel3_hist(s, pj, cj, 4, 0.0, hk));

@@ -151,11 +151,32 @@ int launch_set_ln(mlg_ctx *c, double *dst, const double *src, double scale, doub
 // src/mod_scalar3.f90:213-257.  PFP keeps only m < NTCHOPDIM meaningful.
 // one 64-thread block per (m, k) column: zero rows n >= NRCHOPS(m), and the
 // whole column for m >= NTCHOP (only the zeroed positions are visited)
-__global__ void __launch_bounds__(64) k_chopdo_fff(KGrid g, double2 *a) {
+// The value-at-infinity fix-up of one m = 0 column (one 64-thread block):
+// A(0) -= sum_n AT1(n) A(n) / AT1(0), the products formed by all lanes, the
+// sum by lane 0 in ascending n
+__device__ void fix_inf_column(const KGrid &g, const double *at1, double2 *col, double2 *p) {
+  for (int n = threadIdx.x; n < g.nrchop; n += 64) p[n] = axc(at1[n], col[n]);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double2 s = zero2();
+#pragma unroll 8
+    for (int n = 0; n < g.nrchop; n++) s = add2(s, p[n]);
+    const double2 v0 = col[0];
+    col[0] = sub2(v0, make_double2(s.x / g.at1_0, s.y / g.at1_0));
+  }
+}
+
+// CHOPDO of FFF fields (grid layer z = field z of the list); with fix set,
+// also the value-at-infinity fix-up of each field (rows below NRCHOP of the
+// m = 0 columns, which the chop does not touch)
+__global__ void __launch_bounds__(64) k_chopdo_fff(KGrid g, FieldList fl, int fix, const double *at1) {
+  extern __shared__ __align__(16) double2 fsp[];
+  double2 *a = fl.e[blockIdx.z];
   const int mm = blockIdx.x, k = blockIdx.y;
   const int nn = mm < g.ntchop ? s_nrchops(g.nrchop, mm * g.minc) : 0;
   double2 *col = a + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
   for (int n = (nn > 0 ? nn : 0) + threadIdx.x; n < g.nrchopdim; n += 64) col[n] = zero2();
+  if (fix && mm == 0 && g.nrchop >= 1) fix_inf_column(g, at1, col, fsp);
 }
 __global__ void k_chopdo_rows(KGrid g, double2 *a, int n2, int n3, int stride2, int zero_slot) {
   // rows NR..NDIMR-1 for every (j < n2, z < n3), then (optionally) every
@@ -186,7 +207,9 @@ int launch_chopdo(mlg_ctx *c, double2 *a, int space) {
       hipLaunchKernelGGL(k_chopdo_rows, dim3(nblocks(tot)), dim3(256), 0, c->stream, g, a, n2, n3, stride2, zero_slot);
   };
   if (space == MLG_FFF) {
-    hipLaunchKernelGGL(k_chopdo_fff, dim3(g.ntchopdim, g.nxchopdim), dim3(64), 0, c->stream, g, a);
+    FieldList fl;
+    fl.e[0] = a;
+    hipLaunchKernelGGL(k_chopdo_fff, dim3(g.ntchopdim, g.nxchopdim, 1), dim3(64), 0, c->stream, g, fl, 0, nullptr);
   } else if (space == MLG_PFF) {
     rows(g.ntchopdim, g.nxchopdim, g.ntchopdim, -1);
   } else if (space == MLG_PPP) {
@@ -194,6 +217,18 @@ int launch_chopdo(mlg_ctx *c, double2 *a, int space) {
   } else {  // PFP: rows >= NR of the live slots m < NTCHOPDIM
     rows(g.ntchopdim, g.ndimx, g.ndimth, -1);
   }
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_chopdo_fff_n(mlg_ctx *c, int n, double2 *const *a, bool fix) {
+  if (n < 1 || n > MAX_FIELD_LIST) return MLG_E_ARG;
+  KGrid g = kgrid(c);
+  FieldList fl;
+  for (int i = 0; i < n; i++) fl.e[i] = a[i];
+  const size_t lds = fix ? sizeof(double2) * (size_t)(g.nrchop > 0 ? g.nrchop : 1) : 0;
+  hipLaunchKernelGGL(k_chopdo_fff, dim3(g.ntchopdim, g.nxchopdim, n), dim3(64), lds, c->stream, g, fl, fix ? 1 : 0,
+                     c->d.at1);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -692,26 +727,13 @@ __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, 
 // One 64-thread block per k: the m = 0 column and AT1 are staged in LDS with
 // coalesced loads, lane 0 sums in ascending n.
 __global__ void __launch_bounds__(64) k_fix_inf(KGrid g, const double *at1, double2 *a) {
-  extern __shared__ __align__(16) double fsm[];
-  double2 *v = reinterpret_cast<double2 *>(fsm);  // [nrchop]
-  double *w = fsm + 2 * g.nrchop;                 // [nrchop]
-  const int k = blockIdx.x;
-  double2 *col = a + (size_t)g.nrchopdim * (size_t)g.ntchopdim * k;
-  for (int n = threadIdx.x; n < g.nrchop; n += 64) {
-    v[n] = col[n];
-    w[n] = at1[n];
-  }
-  __syncthreads();
-  if (threadIdx.x == 0) {
-    double2 s = zero2();
-    for (int n = 0; n < g.nrchop; n++) s = add2(s, axc(w[n], v[n]));
-    col[0] = sub2(v[0], make_double2(s.x / g.at1_0, s.y / g.at1_0));
-  }
+  extern __shared__ __align__(16) double2 fsp[];  // [nrchop] products
+  fix_inf_column(g, at1, a + (size_t)g.nrchopdim * (size_t)g.ntchopdim * blockIdx.x, fsp);
 }
 int launch_fix_at_infinity(mlg_ctx *c, double2 *a) {
   KGrid g = kgrid(c);
   if (g.nrchop < 1) return 0;
-  hipLaunchKernelGGL(k_fix_inf, dim3(g.nxchopdim), dim3(64), sizeof(double) * 3 * (size_t)g.nrchop, c->stream, g, c->d.at1,
+  hipLaunchKernelGGL(k_fix_inf, dim3(g.nxchopdim), dim3(64), sizeof(double2) * (size_t)g.nrchop, c->stream, g, c->d.at1,
                      a);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
@@ -1394,6 +1416,8 @@ struct EtdArgs {
   double2 *hnr, *hnp, *hnz, *hnb;    // nonlinear-term history: read (ab), written (save & 1)
   double2 *b0;                       // receives b before the update (or null)
   double2 *bo;                       // receives the updated b (may be b)
+  double *lnd[2];                    // LN copies made on the way: *lnd[i] = *lns[i] (null: none)
+  const double *lns[2];
   const double *hv, *hd;             // filter tables [hr | ht | hx] (or null: no filter)
   int nrc;
   double tf;
@@ -1414,6 +1438,7 @@ __device__ __forceinline__ double2 filt_apply(const KGrid &g, const FIdx &f, int
 }
 
 __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, EtdArgs A, int ab, int save) {
+  if (blockIdx.x == 0 && threadIdx.x < 2 && A.lnd[threadIdx.x]) A.lnd[threadIdx.x][0] = A.lns[threadIdx.x][0];
   GRID_STRIDE(fff_count(g)) {
     FIdx f = fdecode(g, idx);
     const int nn = s_nrchops(g.nrchop, f.mm * g.minc);
@@ -1505,14 +1530,16 @@ __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, E
 int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt, double2 *rur, double2 *rup, double2 *uz,
                    double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_, double2 *hpr,
                    double2 *hpp, double2 *hpz, double2 *hnr, double2 *hnp, double2 *hnz, double2 *hnb, int ab, int save,
-                   double2 *b0, double2 *bout, int filter, double tf) {
+                   double2 *b0, double2 *bout, int filter, double tf, double *const *lnd, const double *const *lns) {
   KGrid g = kgrid(c);
   Mat4x2 M;
   for (int q = 0; q < 16; q++) {
     M.e[q] = e16[q];
     M.nl[q] = nl16[q];
   }
-  EtdArgs A{rur, rup, uz, b, nr_, np_, nz_, nb_, hpr, hpp, hpz, hnr, hnp, hnz, hnb, b0, bout ? bout : b, nullptr, nullptr, 0, tf};
+  EtdArgs A{rur, rup, uz, b, nr_, np_, nz_, nb_, hpr, hpp, hpz, hnr, hnp, hnz, hnb, b0, bout ? bout : b,
+            {lnd ? lnd[0] : nullptr, lnd ? lnd[1] : nullptr}, {lns ? lns[0] : nullptr, lns ? lns[1] : nullptr},
+            nullptr, nullptr, 0, tf};
   if (filter) {
     const mlg::Grid &G = c->g;
     size_t nr1 = (size_t)G.p.nrchop * G.ntchopdim, nt = G.ntchopdim, nx = G.nxchopdim;

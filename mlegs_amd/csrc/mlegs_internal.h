@@ -193,6 +193,13 @@ int launch_fused_products(mlg_ctx *c, const double2 *const in[10], double2 *cons
                           int linear);
 // ---- spectral-space kernels (k_spectral.hip)
 int launch_chopdo(mlg_ctx *c, double2 *a, int space);
+constexpr int MAX_FIELD_LIST = 4;
+struct FieldList {
+  double2 *e[MAX_FIELD_LIST];
+};
+// CHOPDO of n FFF fields in one launch; fix: each field's value-at-infinity
+// fix-up (launch_fix_at_infinity) in the same launch
+int launch_chopdo_fff_n(mlg_ctx *c, int n, double2 *const *a, bool fix);
 int launch_zero(mlg_ctx *c, double2 *a, long n);
 int launch_copy(mlg_ctx *c, double2 *dst, const double2 *src, long n);
 int launch_copy0_fff(mlg_ctx *c, double2 *dst, double *dst_ln, const double2 *src, const double *src_ln);
@@ -256,11 +263,13 @@ int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1,
 // ETD update fused with the HYPERV3 filters (filter != 0), the copy of b
 // before the update (b0 != null) and the history stores (save bit 0: the
 // nonlinear terms, bit 1: the pressure-gradient terms); the updated b goes
-// to bout (null: b itself)
+// to bout (null: b itself); lnd/lns (optional): two LN copies *lnd[i] =
+// *lns[i] made by the same launch (null entries skipped)
 int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt, double2 *rur, double2 *rup, double2 *uz,
                    double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_, double2 *hpr,
                    double2 *hpp, double2 *hpz, double2 *hnr, double2 *hnp, double2 *hnz, double2 *hnb, int ab, int save,
-                   double2 *b0, double2 *bout, int filter, double tf);
+                   double2 *b0, double2 *bout, int filter, double tf, double *const *lnd = nullptr,
+                   const double *const *lns = nullptr);
 int launch_hyp3(mlg_ctx *c, double2 *a, int diff, double tf);
 int launch_diagnost(mlg_ctx *c, const double2 *psi);
 int launch_energy_terms(mlg_ctx *c, const double2 *a, const double2 *b, int l2, double *out_all);
