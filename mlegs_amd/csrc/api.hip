@@ -643,7 +643,9 @@ static int solve_op(const mlg_field *b, mlg_field *a, int op, double alp, int ha
   mlg_field *t = T.buffer();
   RET(T.ready());
   a->space = MLG_FFF;
-  if (op != 2) RET(launch_set_ln(c, a->ln, nullptr, 0.0, 0.0));
+  // IDEL2/IDEL2LN set A%LN in the solve (the thread of the (m=0, k=0)
+  // system); only a truncation without that system needs the zero here
+  if (op != 2 && (c->g.nrchop < 1 || c->g.ntchop < 1)) RET(launch_set_ln(c, a->ln, nullptr, 0.0, 0.0));
   RET(launch_band_solve(c, op, b->e, a->e, a->ln, b->ln, alp, has_ln, ln, t->e));
   RET(launch_chopdo_fff_n(c, 1, &a->e, op != 2));  // CHOPDO, and the fix at infinity of IDEL2, IDEL2LN
   return 0;
@@ -993,6 +995,10 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
       RET(mlg_horfft(rup, 1));
       RET(mlg_horfft(uz, 1));
     }
+    {  // the z-transforms of PROJECT's inputs and of N_b (TOFF's first part) in one launch
+      mlg_field *const pf4[4] = {ror, rop, oz, bn};
+      RET(to_pff_n(c, 4, pf4));
+    }
     RET(mlg_project(ror, rop, oz, psi, w, 1, 0.0));
     RET(mlg_idel2(w, chi, 0, 0.0));
     RET(mlg_chopset(c, 3));
@@ -1256,7 +1262,7 @@ static int visc_be(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, dou
 static int visc_cn(mlg_ctx *c, mlg_field *const *f, const mlg_field *const *fa, const mlg_field *const *fo) {
   if (c->g.p.visc_sw != 1) return 0;
   Temps T(c);
-  mlg_field *wk[3] = {T.get(), T.get(), T.get()};
+  mlg_field *wk[3] = {T.buffer(), T.buffer(), T.buffer()};  // the right-hand-side kernel writes E and LN
   RET(T.ready());
   const double nu[3] = {c->g.p.nu, c->g.p.nu, c->g.p.kappa};
   const int n = c->g.p.kappa != 0.0 ? 3 : 2;
