@@ -79,6 +79,14 @@ long sum_nrchops(const mlg_ctx *c) {
   for (int mm = 0; mm < c->g.ntchopdim; mm++) s += nrchops_of(c->g.nrchop, mm * c->g.p.minc);
   return s;
 }
+double sum_nrchops_sq(const mlg_ctx *c) {
+  double s = 0;
+  for (int mm = 0; mm < c->g.ntchopdim; mm++) {
+    const double v = nrchops_of(c->g.nrchop, mm * c->g.p.minc);
+    s += v * v;
+  }
+  return s;
+}
 }  // namespace mlg
 
 static void timer_clear(mlg_ctx *c) {
@@ -268,7 +276,7 @@ extern "C" int mlg_finalize(mlg_ctx *c) {
   c->timer.spare.clear();
   DevTables &d = c->d;
   void *ptrs[] = {d.pf, d.vdt, d.w, d.r, d.x, d.lnx, d.at1, d.ak, d.norm, d.lognorm, d.bxm, d.bxxdx, d.bdel2h,
-                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.red, d.vproj, d.err};
+                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.red, d.vproj, d.cvdt, d.m0vec, d.err};
   for (auto &f : c->fcache) (void)hipFree(f.f);
   c->fcache.clear();
   for (void *p : ptrs)
@@ -767,8 +775,25 @@ static int project_impl(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field
   mlg_ctx *c = rur->ctx;
   mlg_field *const vel[3] = {rur, rup, uz};
   if (keep) {
+    // spectral velocities that the caller hands over: the composed
+    // transform-and-project reads them as they are, and keep[] takes them
     if (rur->space != MLG_FFF) return 2;
-    RET(to_pff_n(c, 3, vel, keep));
+    if (psi->space != MLG_FFF) {
+      psi->space = MLG_FFF;
+      RET(launch_set_ln(c, psi->ln, nullptr, 0.0, 0.0));
+    }
+    if (chi->space != MLG_FFF) {
+      chi->space = MLG_FFF;
+      RET(launch_set_ln(c, chi->ln, nullptr, 0.0, 0.0));
+    }
+    const double *lnu[3] = {rur->ln, rup->ln, uz->ln};
+    RET(launch_project_fff(c, rur->e, rup->e, uz->e, lnu, psi->e, psi->ln, chi->e, ln_mode, ln));
+    for (int i = 0; i < 3; i++) {
+      swap_storage(keep[i], vel[i]);
+      std::swap(keep[i]->ln, vel[i]->ln);
+      keep[i]->space = MLG_FFF;
+    }
+    return launch_fix_at_infinity(c, psi->e);
   } else if (rur->space == MLG_PPP) {
     RET(mlg_plnfft(rur, -1));
     RET(mlg_plnfft(rup, -1));
@@ -1415,7 +1440,7 @@ static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   RET(launch_etd_sub(c, e16, nl16, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
                      s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 0, save ? 1 : 0, nullptr,
                      nullptr, filter_active(c), dt / c->g.p.dt));
-  RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0, save ? hk : nullptr));  // PROJECT(..., PSI, W, PSI%LN)
+  RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0, hk));  // PROJECT(..., PSI, W, PSI%LN)
   RET(mlg_idel2ln(w, chi));
   if (save) {
     RET(pc2vel3_hist(s, psi, chi, 3, 0.5, hk));

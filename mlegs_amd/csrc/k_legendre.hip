@@ -558,4 +558,336 @@ int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const dou
   return 0;
 }
 
+// -------------------------------------------------------------------------
+// PROJECT of spectral velocities.  For FFF inputs, RTRAN backward followed by
+// PROJECT's folded products is one linear map per m: with the folds
+// F_e = 2 BE, F_o = 2 BO of the backward transform,
+//   W[n] = sum_i T[n][i] F_p(i) = sum_{n' of parity p} C[n][n'] a[n'],
+//   C[n][n'] = 2 sum_i T[n][i] PF(i, n'),
+// for each table T in {V, D, T} (p = parity of n for V and T, the other
+// parity for D).  The composed tables are built once (k_compose_vdt); the
+// product kernel is PROJECT's with the spectral rows n' as its K extent
+// (NRCHOPS(m) rows instead of NR/2 nodes), and the pieces that RTRAN and
+// VEL2PLN add to the m = 0, k = 0 column only (log terms, VEL2PLN's
+// correction) are added by k_project_fff_m0.
+// Layout: cvdt[((t*NTCHOPDIM + m)*2 + p)*NH + j)*NH + h] holds C_t for the
+// output row n = 2h + p and the input row n' = 2j + q(t, p), NH =
+// ceil(NRCHOPDIM/2): 32 consecutive h of one (t, m, p, j) are one run.
+// -------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) k_compose_vdt(KGrid g, const double *__restrict__ pf, const double *__restrict__ vdt,
+                                                     double *__restrict__ cvdt) {
+  extern __shared__ double prow[];  // PF row n' [NRH]
+  const int nh = (g.nrchopdim + 1) / 2;
+  const int j = blockIdx.x, pq = blockIdx.y, mm = blockIdx.z / 3, t = blockIdx.z % 3;
+  const int p = pq;                            // parity of the output rows
+  const int q = (t == 1) ? 1 - p : p;          // parity of the input rows
+  const int np = 2 * j + q;
+  const bool have = np < g.nrchopdim;
+  if (have)
+    for (int i = threadIdx.x; i < g.nrh; i += blockDim.x) prow[i] = pf[((size_t)mm * g.npf + np) * g.nrh + i];
+  __syncthreads();
+  const double *vt = vdt + ((size_t)t * g.ntchopdim + mm) * (size_t)g.nrchopdim * g.nrh;
+  double *out = cvdt + ((((size_t)t * g.ntchopdim + mm) * 2 + p) * nh + j) * nh;
+  for (int h = threadIdx.x; h < nh; h += blockDim.x) {
+    const int n = 2 * h + p;
+    double acc = 0.0;
+    if (have && n < g.nrchopdim) {
+      const double *vr = vt + (size_t)n * g.nrh;
+      for (int i = 0; i < g.nrh; i++) acc += vr[i] * prow[i];
+    }
+    out[h] = 2.0 * acc;
+  }
+}
+
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_project_fff(KGrid g, const double *__restrict__ cvdt, const double *__restrict__ ak, const double2 *__restrict__ rur,
+              const double2 *__restrict__ rup, const double2 *__restrict__ uz, double2 *__restrict__ psi,
+              double2 *__restrict__ chi) {
+  const int nk = g.nxchopdim, nh = (g.nrchopdim + 1) / 2;
+  const int ntc = (2 * nk + PJ_TC - 1) / PJ_TC, ntn = (g.nrchopdim + PJ_TN - 1) / PJ_TN;
+  const int lid = xcd_logical(ntc * ntn * g.ntchopdim);
+  if (lid < 0) return;
+  const int mm = lid / (ntc * ntn), rem = lid - mm * (ntc * ntn);
+  const int tn = rem / ntc, tc = rem - tn * ntc;
+  const int mv = mm * g.minc;
+  const int nn = mm < g.ntchop ? d_nrchops(g.nrchop, mv) : 0;
+  const int n0 = tn * PJ_TN, c0 = tc * PJ_TC;
+  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+  if (n0 >= nn) {  // a tile of the truncated rows: zero (CHOPDO)
+    for (int q = 0; q < 4; q++) {
+      const int idx = tid + 256 * q, n = n0 + (idx & 63), k = (c0 >> 1) + (idx >> 6);
+      if (n < g.nrchopdim && k < nk) {
+        const size_t o = (size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
+        psi[o] = make_double2(0.0, 0.0);
+        chi[o] = make_double2(0.0, 0.0);
+      }
+    }
+    return;
+  }
+  const int par = wave & 1, ch = wave >> 1;
+  __shared__ ProjectLds L;
+
+  v4d acc[2][5];
+  for (int a = 0; a < 2; a++)
+    for (int q = 0; q < 5; q++) acc[a][q] = (v4d){0, 0, 0, 0};
+  const double2 *src[3] = {rur, rup, uz};
+  double rt[12];
+  double2 rd[6];
+  const int kstride = g.nrchopdim * g.ntchopdim;
+  const size_t tstride = (size_t)g.ntchopdim * 2 * nh * nh;  // one table
+  const double *cm = cvdt + (size_t)mm * 2 * nh * nh;
+  const int h0 = n0 >> 1;
+  const int njp = (nn + 1) / 2;  // input row pairs
+  auto fetch = [&](int j0) {
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+      const int idx = tid + 256 * q;  // [table 3][pair 16][parity 2][h 32]
+      const int hl = idx & 31, p = (idx >> 5) & 1, jl = (idx >> 6) & 15, t = idx >> 10;
+      const int n = n0 + 2 * hl + p, np = 2 * (j0 + jl) + ((t == 1) ? 1 - p : p);
+      rt[q] = (n < nn && np < nn) ? cm[(size_t)t * tstride + ((size_t)p * nh + j0 + jl) * nh + h0 + hl] : 0.0;
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const int idx = tid + 256 * q;  // [field 3][k 16][row 32]
+      const int rl = idx & 31, kl = (idx >> 5) & 15, f = idx >> 9;
+      const int np = 2 * j0 + rl, k = (c0 >> 1) + kl;
+      rd[q] = (np < nn && k < nk) ? src[f][(size_t)g.nrchopdim * mm + np + (size_t)kstride * k] : make_double2(0.0, 0.0);
+    }
+  };
+  auto stash = [&]() {
+#pragma unroll
+    for (int q = 0; q < 12; q++) {
+      const int idx = tid + 256 * q;
+      const int hl = idx & 31, p = (idx >> 5) & 1, jl = (idx >> 6) & 15, t = idx >> 10;
+      L.k.T[t][p][jl][hl ^ lsw(jl, p)] = rt[q];
+    }
+#pragma unroll
+    for (int q = 0; q < 6; q++) {
+      const int idx = tid + 256 * q;
+      const int rl = idx & 31, kl = (idx >> 5) & 15, f = idx >> 9;
+      const int jl = rl >> 1, qp = rl & 1;
+      const int sw = lsw(jl, 0);
+      L.k.F[f][qp][jl][(2 * kl) ^ sw] = rd[q].x;
+      L.k.F[f][qp][jl][(2 * kl + 1) ^ sw] = rd[q].y;
+    }
+  };
+  const int eo = par, oe = par ^ 1;
+  // subtiles that hold data: rows n0 + 32 a + par + 2 l, columns c0 + 16 ch + l
+  const bool pa0 = n0 + par < nn, pa1 = n0 + 32 + par < nn, pc0 = c0 + ch * 16 < 2 * nk;
+  fetch(0);
+  for (int j0 = 0; j0 < njp; j0 += PJ_IC) {
+    __syncthreads();
+    stash();
+    __syncthreads();
+    if (j0 + PJ_IC < njp) fetch(j0 + PJ_IC);
+    if (!(pa0 && pc0)) continue;  // all the wave's rows or columns are padding
+    const int jleft = njp - j0;
+#pragma unroll
+    for (int ks = 0; ks < PJ_IC; ks += 4) {
+      if (ks >= jleft) break;  // zero padding at the end of K
+      const int kr = ks + (lane >> 4), cl = (ch * 16 + (lane & 15)) ^ lsw(kr, 0), st = lsw(kr, par);
+      const double f0e = L.k.F[0][eo][kr][cl], f0o = L.k.F[0][oe][kr][cl];
+      const double f1e = L.k.F[1][eo][kr][cl], f1o = L.k.F[1][oe][kr][cl];
+      const double f2e = L.k.F[2][eo][kr][cl];
+#pragma unroll
+      for (int a = 0; a < 2; a++) {
+        if (a == 1 && !pa1) break;
+        const int rl = (16 * a + (lane & 15)) ^ st;
+        const double av = L.k.T[0][par][kr][rl], ad = L.k.T[1][par][kr][rl], at = L.k.T[2][par][kr][rl];
+        acc[a][0] = mfma16x16x4(av, f0e, acc[a][0]);
+        acc[a][1] = mfma16x16x4(ad, f1o, acc[a][1]);
+        acc[a][2] = mfma16x16x4(ad, f0o, acc[a][2]);
+        acc[a][3] = mfma16x16x4(av, f1e, acc[a][3]);
+        acc[a][4] = mfma16x16x4(at, f2e, acc[a][4]);
+      }
+    }
+  }
+  __syncthreads();
+  // epilogue as in k_project: psi = -i m W1 - W2, del^2 chi = i k W3 + m k W4 - W5
+  const int cl = ch * 16 + (lane & 15), kl = cl >> 1, isim = cl & 1;
+  const int kg = (c0 >> 1) + kl;
+  const double m = (double)mv;
+  const double kv = (kg < nk) ? ak[mm + (size_t)g.ntchopdim * kg] : 0.0, mk = m * kv;
+#pragma unroll
+  for (int a = 0; a < 2; a++)
+#pragma unroll
+    for (int v = 0; v < 4; v++) {
+      const int rl = (lane >> 4) + 4 * v + 16 * a;
+      const int nl = 2 * rl + par;
+      const double w1 = acc[a][0][v], w2 = acc[a][1][v], w3 = acc[a][2][v], w4 = acc[a][3][v], w5 = acc[a][4][v];
+      const double w1p = __shfl_xor(w1, 1, 64), w3p = __shfl_xor(w3, 1, 64);
+      double ps, cs;
+      if (!isim) {
+        ps = m * w1p - w2;
+        cs = (-kv * w3p + mk * w4) - w5;
+      } else {
+        ps = -m * w1p - w2;
+        cs = (kv * w3p + mk * w4) - w5;
+      }
+      L.e.P[kl][nl][isim] = ps;
+      L.e.C[kl][nl][isim] = cs;
+    }
+  __syncthreads();
+  for (int q = 0; q < 4; q++) {
+    int idx = tid + 256 * q;  // 0..1023: n fastest, 16 complex k
+    int nl = idx & 63, kq = idx >> 6;
+    int n = n0 + nl, k = (c0 >> 1) + kq;
+    if (n < g.nrchopdim && k < nk) {
+      size_t o = (size_t)n + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
+      if (n < nn) {
+        psi[o] = make_double2(L.e.P[kq][nl][0], L.e.P[kq][nl][1]);
+        chi[o] = make_double2(L.e.C[kq][nl][0], L.e.C[kq][nl][1]);
+      } else {
+        psi[o] = make_double2(0.0, 0.0);
+        chi[o] = make_double2(0.0, 0.0);
+      }
+    }
+  }
+}
+
+// The m = 0, k = 0 column's additions: RTRAN's log terms LN(u) (-log(1-x))
+// of r u_theta and u_z through the D and T tables, and VEL2PLN: the log
+// coefficient LN(psi) = -u(inf)/2 from the folds of Re(r u_theta) (with its
+// log term), and the removal of its log image through the D table.  At
+// m = 0, k = 0 psi = -W2 and del^2 chi = -W5.  All of it is linear in the
+// column, so it is applied with vectors built from the tables (k_m0_vectors):
+//   pln = -(sum_n LF[n] Re a[n] + LN(r u_theta) LFL) / 2,
+//   LF[n] = 2 sum_i PF(i, n) G_p(n)(i),  G_e/o(i) = sum_{n' even/odd < NN}
+//           exp(lognorm n') w_i PF(i, n'),  LFL = sum_i (lnx_e G_e + lnx_o G_o)(i),
+//   psi[n] -= LN(r u_theta) LD[n] + pln DC[n],  del^2 chi[n] -= LN(u_z) LT[n],
+// with lnx_e/o the folds of -log(1-x) and LD, LT, DC the D, T rows against
+// the folds of -log(1-x) and of VEL2PLN's log image.
+// Vector store (NRCHOPDIM each): LF, LD, LT, DC, then G_e, G_o (NRH), LFL.
+__device__ __forceinline__ double block_sum256(double v, double *part) {
+#pragma unroll
+  for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+  __syncthreads();
+  if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = v;
+  __syncthreads();
+  return ((part[0] + part[1]) + part[2]) + part[3];
+}
+// stage 1 (one block per node i): G_e(i), G_o(i) over n' < nn
+__global__ void __launch_bounds__(256) k_m0_gauss(KGrid g, int nn, const double *__restrict__ pf,
+                                                  const double *__restrict__ w, const double *__restrict__ lognorm,
+                                                  double *__restrict__ vec) {
+  __shared__ double part[4];
+  const int i = blockIdx.x, nrd = g.nrchopdim;
+  double se = 0.0, so = 0.0;
+  for (int n = threadIdx.x; n < nn; n += 256) {
+    const double v = exp(lognorm[n]) * w[i] * pf[(size_t)n * g.nrh + i];
+    if (n & 1) so += v;
+    else se += v;
+  }
+  se = block_sum256(se, part);
+  so = block_sum256(so, part);
+  if (threadIdx.x == 0) {
+    vec[4 * nrd + i] = se;
+    vec[4 * nrd + g.nrh + i] = so;
+  }
+}
+// stage 2 (one block per row n, and one more for LFL): LF, LD, LT, DC
+__global__ void __launch_bounds__(256) k_m0_rows(KGrid g, const double *__restrict__ pf, const double *__restrict__ vdt,
+                                                 const double *__restrict__ lnx, double *__restrict__ vec) {
+  __shared__ double part[4];
+  const int n = blockIdx.x, nrd = g.nrchopdim;
+  const double *ge = vec + 4 * nrd, *go = ge + g.nrh;
+  if (n == nrd) {  // LFL
+    double s = 0.0;
+    for (int i = threadIdx.x; i < g.nrh; i += 256) {
+      const double la = lnx[i], lb = lnx[g.nr - 1 - i];
+      s += (la + lb) * ge[i] + (la - lb) * go[i];
+    }
+    s = block_sum256(s, part);
+    if (threadIdx.x == 0) vec[4 * nrd + 2 * g.nrh] = s;
+    return;
+  }
+  const double *dr = vdt + ((size_t)1 * g.ntchopdim * nrd + n) * g.nrh;  // D, m = 0
+  const double *tr = vdt + ((size_t)2 * g.ntchopdim * nrd + n) * g.nrh;  // T, m = 0
+  const double *pn = pf + (size_t)n * g.nrh;
+  const bool odd = n & 1;
+  double lf = 0.0, ld = 0.0, lt = 0.0, dc = 0.0;
+  for (int i = threadIdx.x; i < g.nrh; i += 256) {
+    const double la = lnx[i], lb = lnx[g.nr - 1 - i];
+    lf += pn[i] * (odd ? go[i] : ge[i]);
+    // D pairs row n with the other parity's fold, T with its own
+    ld += dr[i] * (odd ? (la + lb) : (la - lb));
+    lt += tr[i] * (odd ? (la - lb) : (la + lb));
+    dc += dr[i] * (odd ? pf[i] / g.norm0 : pf[(size_t)g.nrh + i] / g.norm1);
+  }
+  lf = block_sum256(lf, part);
+  ld = block_sum256(ld, part);
+  lt = block_sum256(lt, part);
+  dc = block_sum256(dc, part);
+  if (threadIdx.x == 0) {
+    vec[n] = 2.0 * lf;
+    vec[nrd + n] = ld;
+    vec[2 * nrd + n] = lt;
+    vec[3 * nrd + n] = 2.0 * dc;
+  }
+}
+// the additions themselves (one block)
+__global__ void __launch_bounds__(256) k_project_fff_m0(KGrid g, const double *__restrict__ vec, const double2 *__restrict__ rup,
+                                                        const double *__restrict__ ln_rup, const double *__restrict__ ln_uz,
+                                                        double2 *__restrict__ psi, double *__restrict__ psi_ln,
+                                                        double2 *__restrict__ chi, int ln_mode, double ln) {
+  __shared__ double part[4];
+  const int nn = g.ntchop >= 1 ? g.nrchop : 0, nrd = g.nrchopdim;
+  const double lr = ln_rup[0], lz = ln_uz[0];
+  double s = 0.0;
+  for (int n = threadIdx.x; n < nn; n += 256) s += vec[n] * rup[n].x;
+  s = block_sum256(s, part);
+  const double pln = ln_mode != 0 ? -(s + lr * vec[4 * nrd + 2 * g.nrh]) / 2.0 : 0.0;  // VEL2PLN (ln_mode 0: none)
+  for (int n = threadIdx.x; n < nn; n += 256) {
+    psi[n].x = psi[n].x - (lr * vec[nrd + n] + pln * vec[3 * nrd + n]);
+    chi[n].x = chi[n].x - lz * vec[2 * nrd + n];
+  }
+  if (threadIdx.x == 0 && ln_mode != 0 && ln_mode != 2) psi_ln[0] = ln_mode == 1 ? ln : pln;
+}
+
+// The composed tables (once) and the m = 0 vectors (again when NRCHOP changes).
+static int ensure_cvdt(mlg_ctx *c) {
+  if (c->g.nrh < 1) return MLG_E_ARG;
+  KGrid g = kgrid(c);
+  if (!c->d.cvdt) {
+    const int nh = (g.nrchopdim + 1) / 2;
+    const size_t n = (size_t)3 * g.ntchopdim * 2 * nh * nh;
+    MLG_CHECK_HIP(hipMalloc((void **)&c->d.cvdt, sizeof(double) * n));
+    hipLaunchKernelGGL(k_compose_vdt, dim3(nh, 2, 3 * g.ntchopdim), dim3(256), sizeof(double) * (size_t)g.nrh, c->stream, g,
+                       c->d.pf, c->d.vdt, c->d.cvdt);
+    MLG_CHECK_HIP(hipGetLastError());
+  }
+  if (!c->d.m0vec) {
+    MLG_CHECK_HIP(hipMalloc((void **)&c->d.m0vec, sizeof(double) * (4 * (size_t)g.nrchopdim + 2 * (size_t)g.nrh + 1)));
+    c->m0vec_nrchop = -1;
+  }
+  if (c->m0vec_nrchop != g.nrchop) {
+    const int nn = g.ntchop >= 1 ? g.nrchop : 0;
+    hipLaunchKernelGGL(k_m0_gauss, dim3(g.nrh), dim3(256), 0, c->stream, g, nn, c->d.pf, c->d.w, c->d.lognorm, c->d.m0vec);
+    hipLaunchKernelGGL(k_m0_rows, dim3(g.nrchopdim + 1), dim3(256), 0, c->stream, g, c->d.pf, c->d.vdt, c->d.lnx,
+                       c->d.m0vec);
+    MLG_CHECK_HIP(hipGetLastError());
+    c->m0vec_nrchop = g.nrchop;
+  }
+  return 0;
+}
+
+int launch_project_fff(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, const double *const ln_u[3],
+                       double2 *psi, double *psi_ln, double2 *chi, int ln_mode, double ln) {
+  const int rc = ensure_cvdt(c);
+  if (rc) return rc;
+  KGrid g = kgrid(c);
+  int ntn = (g.nrchopdim + PJ_TN - 1) / PJ_TN, ntc = (2 * g.nxchopdim + PJ_TC - 1) / PJ_TC;
+  dim3 grid(xcd_grid(ntn * ntc * g.ntchopdim));
+  {
+    // five products of NRCHOPS(m) rows x NRCHOPS(m)/2 same-class input rows per real column
+    KScope ks(c, K_PROJECT, 5 * sum_nrchops_sq(c) * 2.0 * g.nxchopdim);
+    hipLaunchKernelGGL(k_project_fff, grid, dim3(256), 0, c->stream, g, c->d.cvdt, c->d.ak, rur, rup, uz, psi, chi);
+    MLG_CHECK_HIP(hipGetLastError());
+  }
+  hipLaunchKernelGGL(k_project_fff_m0, dim3(1), dim3(256), 0, c->stream, g, c->d.m0vec, rup, ln_u[1], ln_u[2], psi, psi_ln,
+                     chi, ln_mode, ln);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
 }  // namespace mlg

@@ -45,6 +45,8 @@ struct DevTables {
   double *hv = nullptr;       // [hv_r | hv_t | hv_x | hd_r | hd_t | hd_x]
   double *red = nullptr;      // reduction scratch
   double *vproj = nullptr;    // VEL2PLN projections [NRCHOPDIM]
+  double *cvdt = nullptr;     // PROJECT tables composed with RTRAN (built on first use)
+  double *m0vec = nullptr;    // their m = 0, k = 0 column vectors (built on first use)
   int *err = nullptr;         // sticky device error code
 };
 
@@ -95,6 +97,7 @@ struct mlg_ctx {
   std::unordered_set<mlg_field *> owned;  // fields created through the C ABI
   std::vector<mlg::FactorSet> fcache;     // band-solve factors
   int device = 0;
+  int m0vec_nrchop = -1;  // truncation the m = 0 vectors were built for
 };
 
 struct mlg_solver {
@@ -147,6 +150,7 @@ struct KScope {
 };
 // sum over m of NRCHOPS(m) at the current truncation
 long sum_nrchops(const mlg_ctx *c);
+double sum_nrchops_sq(const mlg_ctx *c);  // sum over m of NRCHOPS(m)^2
 
 static inline int nrchops_of(int nrchop, int m) {
   int v = nrchop - m < nrchop ? nrchop - m : nrchop;
@@ -172,6 +176,12 @@ int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *co
 int launch_verfft_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out);
 int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out);
 int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *psi, double2 *chi);
+// PROJECT of spectral (FFF) velocities: RTRAN backward, VEL2PLN and the
+// projection products in one pass with the composed tables; ln_mode 0: no
+// VEL2PLN, else as for launch_vel2pln; the velocities' LN coefficients are
+// read from ln_u.  The velocities are not changed.
+int launch_project_fff(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, const double *const ln_u[3],
+                       double2 *psi, double *psi_ln, double2 *chi, int ln_mode, double ln);
 // ---- FFTs (k_fft.hip)
 int launch_verfft_fwd(mlg_ctx *c, const double2 *in, double2 *out);
 int launch_verfft_bwd(mlg_ctx *c, const double2 *in, double2 *out);
