@@ -59,15 +59,17 @@ union RtranBwdLds {
 };
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
-k_rtran_bwd(KGrid g, const double *__restrict__ pf, TransformJobs jobs, const double *__restrict__ lnx) {
-  // grid row y transforms field y of the batch
-  const double2 *__restrict__ in = jobs.in[blockIdx.y];
-  const double *__restrict__ ln_in = jobs.ln[blockIdx.y];
-  double2 *__restrict__ out = jobs.out[blockIdx.y];
+k_rtran_bwd(KGrid g, const double *__restrict__ pf, TransformJobs jobs, int nj, const double *__restrict__ lnx) {
+  // logical block (field fastest, then column tile, radial tile, m): the
+  // fields of a batch that use one table tile run together on one XCD
   const int nk = g.nxchopdim;
   const int ntiles_c = (2 * nk + BW_TC - 1) / BW_TC, ntiles_i = (g.nrh + BW_TI - 1) / BW_TI;
-  const int lid = xcd_logical(ntiles_c * ntiles_i * g.ntchopdim);
-  if (lid < 0) return;
+  const int lidf = xcd_logical(ntiles_c * ntiles_i * g.ntchopdim * nj);
+  if (lidf < 0) return;
+  const int fj = lidf % nj, lid = lidf / nj;
+  const double2 *__restrict__ in = jobs.in[fj];
+  const double *__restrict__ ln_in = jobs.ln[fj];
+  double2 *__restrict__ out = jobs.out[fj];
   const int mm = lid / (ntiles_c * ntiles_i), rem = lid - mm * (ntiles_c * ntiles_i);
   const int ti = rem / ntiles_c, tc = rem - ti * ntiles_c;
   const int mv = mm * g.minc;
@@ -207,10 +209,10 @@ int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double
     jobs.out[i] = out[i];
   }
   int nti = (g.nrh + BW_TI - 1) / BW_TI, ntc = (2 * g.nxchopdim + BW_TC - 1) / BW_TC;
-  dim3 grid(xcd_grid(nti * ntc * g.ntchopdim), n);
+  dim3 grid(xcd_grid(nti * ntc * g.ntchopdim * n));
   // 2 NR sum_m NRCHOPS(m) NXCHOPDIM flops per field (real table x complex data, folded)
   KScope ks(c, K_RTRAN_BWD, n * 2.0 * g.nr * (double)sum_nrchops(c) * g.nxchopdim);
-  hipLaunchKernelGGL(k_rtran_bwd, grid, dim3(256), 0, c->stream, g, c->d.pf, jobs, c->d.lnx);
+  hipLaunchKernelGGL(k_rtran_bwd, grid, dim3(256), 0, c->stream, g, c->d.pf, jobs, n, c->d.lnx);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
