@@ -1390,8 +1390,7 @@ static int pc2velvor3(mlg_ctx *c, const mlg_field *psi, const mlg_field *chi, ml
     u[i]->space = MLG_FFF;
   }
   RET(mlg_chopset(c, 3));
-  int rc = launch_check_top_m1(c, chi->e);  // PC2VOR's DEL2 check
-  if (!rc) rc = launch_pc2velvor(c, psi->e, psi->ln, chi->e, chi->ln, o);
+  int rc = launch_pc2velvor(c, psi->e, psi->ln, chi->e, chi->ln, o);  // with PC2VOR's DEL2 check
   mlg_chopset(c, -3);
   return rc;
 }

@@ -597,7 +597,7 @@ int launch_pc2vor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const do
 // output row is formed by the same operations as in k_pc2vel and k_pc2vor.
 __global__ void __launch_bounds__(256) k_pc2velvor(KGrid g, const double *bxm, const double *bxxdx, const double *bd,
                                                    const double *ak, const double2 *psi, const double *ln_psi,
-                                                   const double2 *chi, const double *ln_chi, VelVorOut o) {
+                                                   const double2 *chi, const double *ln_chi, VelVorOut o, int *err) {
   extern __shared__ __align__(16) double2 csm[];
   const int mm = blockIdx.x, k = blockIdx.y;
   const int mv = mm * g.minc, nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0;
@@ -609,6 +609,12 @@ __global__ void __launch_bounds__(256) k_pc2velvor(KGrid g, const double *bxm, c
     X[n] = chi[cb + n];
   }
   __syncthreads();
+  // DEL2's exactness check of PC2VOR (k_check_top_m1): the top mode of
+  // chi at m = 1, k = 0 must vanish
+  if (mm == 1 && k == 0 && threadIdx.x == 0 && nn >= 1) {
+    const double2 t = X[nn - 1];
+    if (hypot(t.x, t.y) > 1.0e-15) atomicCAS(err, 0, 3);
+  }
   for (int n = threadIdx.x; n < nn; n += blockDim.x) Q[n] = del2_row(g, bd, ak, X, lc, mm, k, n, nn);
   uz_stage_direct(g, bxm, P, lp, mm, k, nn, WP);
   uz_stage_direct(g, bxm, X, lc, mm, k, nn, WX);
@@ -652,7 +658,7 @@ int launch_pc2velvor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const
                      const VelVorOut &o) {
   KGrid g = kgrid(c);
   hipLaunchKernelGGL(k_pc2velvor, column_grid(g), dim3(256), 5 * sizeof(double2) * (size_t)g.nrchopdim, c->stream, g,
-                     c->d.bxm, c->d.bxxdx, c->d.bdel2h, c->d.ak, psi, ln_psi, chi, ln_chi, o);
+                     c->d.bxm, c->d.bxxdx, c->d.bdel2h, c->d.ak, psi, ln_psi, chi, ln_chi, o, c->d.err);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
