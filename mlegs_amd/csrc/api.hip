@@ -1228,7 +1228,8 @@ static int richardson(mlg_field *y, const mlg_field *x1, bool ln) {
 
 // IHELM of n independent fields, A_i = IHELM(B_i, ALP_i), with one band-solve
 // launch for all of them (each field as solve_op does it for op 2).
-static int ihelm_n(int n, mlg_field *const *b, mlg_field *const *a, const double *alp) {
+// bil: the right-hand sides b are row-interleaved (launch_visc2_rhs_il)
+static int ihelm_n(int n, mlg_field *const *b, mlg_field *const *a, const double *alp, int bil = 0) {
   mlg_ctx *c = b[0]->ctx;
   const double2 *be[3];
   double2 *ae[3];
@@ -1249,7 +1250,7 @@ static int ihelm_n(int n, mlg_field *const *b, mlg_field *const *a, const double
   RET(T.ready());
   double2 *te[3] = {nullptr, nullptr, nullptr};
   for (int i = 0; i < n; i++) te[i] = tf[i]->e;
-  RET(launch_band_solve_n(c, 2, n, be, ae, aln, bln, alp, 0, 0.0, te));
+  RET(launch_band_solve_n(c, 2, n, be, ae, aln, bln, alp, 0, 0.0, te, bil));
   return launch_chopdo_fff_n(c, n, ae, false);
 }
 
@@ -1303,9 +1304,9 @@ static int visc_cn(mlg_ctx *c, mlg_field *const *f, const mlg_field *const *fa, 
     wk[i]->space = MLG_FFF;
     jobs.j[i] = Visc2Job{fo[i]->e, fo[i]->ln, fa[i]->e, fa[i]->ln, alp[i], wk[i]->e, wk[i]->ln};
   }
-  RET(launch_visc2_rhs_n(c, n, jobs));
+  RET(launch_visc2_rhs_il(c, n, jobs));  // WK row-interleaved for the solve
   mlg_field *out[3] = {f[0], f[1], f[2]};
-  return ihelm_n(n, wk, out, alp);
+  return ihelm_n(n, wk, out, alp, 1);
 }
 static int visc_cn(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, const mlg_field *psio, const mlg_field *chio,
                    const mlg_field *bo) {

@@ -352,54 +352,128 @@ int launch_del2(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, do
 //   WK%LN = -ALP*A%LN
 // with DEL2 evaluated on the truncation NRCHOP+2 (nrchop2) at the positions
 // the base truncation keeps, and DEL2's top-mode check of AO at NRCHOP+2.
-// grid layer z: job z of the batch (independent right-hand sides)
-__global__ void __launch_bounds__(256) k_visc2_rhs(KGrid g, int nrchop2, const double *bd, const double *ak, Visc2Jobs jobs,
-                                                  int *err) {
-  extern __shared__ __align__(16) double2 csm[];
+// WK is written row-interleaved over the systems: row n of system
+// s = m + NTCHOPDIM k at wk[n * NSYS + s] (NSYS = NTCHOPDIM * NXCHOPDIM), the
+// layout in which the band substitution, one thread per system, reads it
+// coalesced.  A block takes VR rows of 64 consecutive
+// systems (grid: systems, row chunks, jobs): four threads per column form
+// VR/4 consecutive rows each (DEL2's row as del2_row forms it, then
+// -WK - ALP*A), the tile goes through LDS, and each wave writes whole rows of
+// the 64 systems.
+template <int VR>
+__global__ void __launch_bounds__(256) k_visc2_rhs_il(KGrid g, int nrchop2, const double *__restrict__ bd,
+                                                      const double *__restrict__ ak, Visc2Jobs jobs, int *err) {
+  constexpr int RPT = VR / 4;
+  __shared__ double2 so[64][VR + 1];
   const Visc2Job &J = jobs.j[blockIdx.z];
-  const double2 *ao = J.ao, *a = J.a;
-  const double *ln_a = J.ln_a;
-  const double alp = J.alp;
-  double2 *wk = J.wk;
-  double *ln_wk = J.ln_wk;
-  const double ln = J.ln_ao[0];
-  const int mm = blockIdx.x, k = blockIdx.y;
-  const int mv = mm * g.minc;
-  const int nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0, nn2 = s_nrchops(nrchop2, mv);
-  const int nl = nn > 0 ? (nn2 < g.nrchopdim ? nn2 : g.nrchopdim) : 0;  // rows DEL2 reads
-  const size_t cb = fcol(g, mm, k);
-  // AO is read as its chopped copy (COPY0): rows >= NRCHOPS count as zero
-  for (int n = threadIdx.x; n < nl; n += blockDim.x) csm[n] = n < nn ? ao[cb + n] : zero2();
-  __syncthreads();
-  for (int n = threadIdx.x; n < g.nrchopdim; n += blockDim.x) {
-    double2 v = zero2();
-    if (n < nn) v = del2_row(g, bd, ak, csm, ln, mm, k, n, nn2);
-    wk[cb + n] = sub2(make_double2(-v.x, -v.y), axc(alp, a[cb + n]));
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  const long s0 = (long)blockIdx.x * 64;
+  const int t = threadIdx.x;
+  const double ln = J.ln_ao[0], alp = J.alp;
+  // column role: column cs of the block, rows sub*RPT .. sub*RPT+RPT-1 of the chunk
+  const int cs = t >> 2, sub = t & 3;
+  const long ss = s0 + cs;
+  const bool svalid = ss < nsys;
+  const int ms = svalid ? (int)(ss % g.ntchopdim) : 0, ks = svalid ? (int)(ss / g.ntchopdim) : 0;
+  const int nn = (svalid && ms < g.ntchop) ? s_nrchops(g.nrchop, ms * g.minc) : 0;  // AO rows >= NRCHOPS read as zero
+  const int nn2 = s_nrchops(nrchop2, ms * g.minc);
+  const double kv = ak[ms + (size_t)g.ntchopdim * ks], k2 = kv * kv;
+  const bool m0k0 = ms == 0 && ks == 0;
+  const size_t cb = svalid ? (size_t)g.nrchopdim * (size_t)ss : 0;  // column (m, k) = system ss
+  const double *bdm = bd + (size_t)ms * g.nband * 5;
+  // write role: system c = t & 63, rows rq*RPT .. of the chunk
+  const int c = t & 63, rq = t >> 6;
+  const long sys = s0 + c;
+  {
+    const int r0 = blockIdx.y * VR;
+    const int nb = r0 + sub * RPT;
+    // every load of the tile first, at clamped (in-bounds) rows, so that they
+    // are all in flight together; the values outside the column's rows are
+    // replaced by zeros afterwards
+    const int nlast = g.nrchopdim - 1;
+    // (a tile wholly at or below the column's truncation reads no AO rows
+    // and no band rows)
+    double2 x[RPT + 4], an[RPT];
+    double rb[RPT][5];
+#pragma unroll
+    for (int i = 0; i < RPT; i++) an[i] = J.a[cb + min(nb + i, nlast)];
+    if (nb - 2 < nn) {
+#pragma unroll
+      for (int i = 0; i < RPT + 4; i++) x[i] = J.ao[cb + min(max(nb - 2 + i, 0), nlast)];
+#pragma unroll
+      for (int i = 0; i < RPT; i++) {
+        const int n = min(nb + i, nlast);  // DEL2's table holds NRCHOPDIM+2 rows per mode
+#pragma unroll
+        for (int e = 0; e < 5; e++) rb[i][e] = bdm[(size_t)n * 5 + e];
+      }
+    } else {
+#pragma unroll
+      for (int i = 0; i < RPT + 4; i++) x[i] = zero2();
+#pragma unroll
+      for (int i = 0; i < RPT; i++)
+#pragma unroll
+        for (int e = 0; e < 5; e++) rb[i][e] = 0.0;
+    }
+#pragma unroll
+    for (int i = 0; i < RPT + 4; i++) {
+      const int n = nb - 2 + i;
+      if (!(n >= 0 && n < nn)) x[i] = zero2();
+    }
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+      const int n = nb + i;
+      double2 v = zero2();
+      if (n < nn) {  // band5_row, then -k^2 and the m0k0 log image (del2_row)
+        const double *r = rb[i];
+        double2 o = axc(r[2], x[i + 2]);
+        if (n >= 2) o = add2(o, axc(r[0], x[i]));
+        if (n >= 1) o = add2(o, axc(r[1], x[i + 1]));
+        if (n + 1 < nn2) o = add2(o, axc(r[3], x[i + 3]));
+        if (n + 2 < nn2) o = add2(o, axc(r[4], x[i + 4]));
+        o = sub2(o, axc(k2, x[i + 2]));
+        if (m0k0) {
+          if (n == 0) o.x = o.x + 4.0 / 3 * ln / g.ell2 / g.norm0;
+          if (n == 1) o.x = o.x - 2.0 * ln / g.ell2 / g.norm1;
+          if (n == 2) o.x = o.x + 2.0 / 3 * ln / g.ell2 / g.norm2;
+        }
+        v = o;
+      }
+      const double2 a = (svalid && n < g.nrchopdim) ? an[i] : zero2();
+      so[cs][sub * RPT + i] = sub2(make_double2(-v.x, -v.y), axc(alp, a));
+    }
+    __syncthreads();
+    if (sys < nsys) {
+#pragma unroll
+      for (int q = 0; q < RPT; q++) {
+        const int rl = rq * RPT + q, n = r0 + rl;
+        if (n < g.nrchopdim) J.wk[(size_t)n * nsys + sys] = so[c][rl];
+      }
+    }
   }
-  if (mm == 0 && k == 0 && threadIdx.x == 0) {
-    ln_wk[0] = ln_a[0] * -alp + 0.0;
+  if (s0 == 0 && blockIdx.y == 0 && t == 0) {
+    J.ln_wk[0] = J.ln_a[0] * -alp + 0.0;
     if (g.ntchopdim >= 2) {
       const int nt = s_nrchops(nrchop2, g.minc);
       if (nt >= 1 && g.ntchop > 1 && nt - 1 < s_nrchops(g.nrchop, g.minc)) {
-        double2 t = ao[(size_t)(nt - 1) + (size_t)g.nrchopdim * 1];
-        if (hypot(t.x, t.y) > 1.0e-15) atomicCAS(err, 0, 3);
+        double2 tv = J.ao[(size_t)(nt - 1) + (size_t)g.nrchopdim * 1];
+        if (hypot(tv.x, tv.y) > 1.0e-15) atomicCAS(err, 0, 3);
       }
     }
   }
 }
-int launch_visc2_rhs_n(mlg_ctx *c, int n, const Visc2Jobs &jobs) {
+int launch_visc2_rhs_il(mlg_ctx *c, int n, const Visc2Jobs &jobs) {
   if (n < 1 || n > 3) return MLG_E_ARG;
   KGrid g = kgrid(c);
   const int nrchop2 = g.nrchop + 2;
   if (nrchop2 > g.nrchopdim) return 1;  // CHOPSET: NRCHOP TOO LARGE (mod_scalar3.f90:183-190)
-  dim3 grid = column_grid(g);
-  grid.z = n;
-  hipLaunchKernelGGL(k_visc2_rhs, grid, dim3(256), sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, nrchop2, c->d.bdel2h,
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  constexpr int VR = 16;
+  const dim3 grid((unsigned)((nsys + 63) / 64), (unsigned)((g.nrchopdim + VR - 1) / VR), n);
+  hipLaunchKernelGGL(k_visc2_rhs_il<VR>, grid, dim3(256), 0, c->stream, g, nrchop2, c->d.bdel2h,
                      c->d.ak, jobs, c->d.err);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
-
 
 // DEL2's exactness check (:778-790): the top mode of m=1 at k=0 must vanish.
 __global__ void k_check_top_m1(KGrid g, const double2 *a, int *err) {
@@ -960,8 +1034,8 @@ __device__ void band_apply(int nd, const double2 *xin, double2 *x, const double 
 // accesses coalesce), which the back substitution reads; only its results
 // go to the column x.
 template <int NJ, int NE, int C0, int SB>
-__device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], double2 *const (&xt)[NJ], double2 *const (&x)[NJ],
-                             long st, const FactorView &FV, long ncls, long cls) {
+__device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], long sin, double2 *const (&xt)[NJ],
+                             double2 *const (&x)[NJ], long st, const FactorView &FV, long ncls, long cls) {
   static_assert(NE == 5 && C0 == 2, "the split factor records are penta-diagonal");
   constexpr int KU = NE - 1 - C0;
   double2 xh[NJ][C0];
@@ -978,7 +1052,7 @@ __device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], double2 *c
       if (nn < nd) {
         const double *fr = FV.ff + ((size_t)nn * ncls + cls) * 3;
 #pragma unroll
-        for (int jj = 0; jj < NJ; jj++) rhs[q][jj] = xin[jj][nn];
+        for (int jj = 0; jj < NJ; jj++) rhs[q][jj] = xin[jj][nn * sin];
 #pragma unroll
         for (int j = 0; j < C0; j++) lm[q][j] = fr[j];
         rp[q] = fr[2];
@@ -1159,6 +1233,7 @@ struct BandJob {
 constexpr int MAX_BAND_JOBS = 3;
 struct BandJobs {
   BandJob j[MAX_BAND_JOBS];
+  int bil;                                 // right-hand sides row-interleaved (row n of system s at b[n NSYS + s])
   int n;                                   // jobs
   int ng;                                  // groups of jobs sharing a factor set
   int gcount[MAX_BAND_JOBS];               // jobs of group g
@@ -1181,12 +1256,13 @@ __device__ void apply_penta_n(const KGrid &g, int op, int has_ln, const BandJobs
 #pragma unroll
   for (int jj = 0; jj < NJ; jj++) {
     const BandJob &J = jobs.j[jobs.gjob[grp][jj]];
-    xin[jj] = J.b + col;
+    xin[jj] = jobs.bil ? J.b + sys : J.b + col;
     x[jj] = J.a + col;
     xt[jj] = J.t + sys;
   }
   const FactorView FV = factor_view(jobs.j[jobs.gjob[grp][0]].F, g.nrchopdim, ncls);
-  band_apply_n<NJ, 5, 2, (NJ == 1 ? 16 : 8)>(nn, xin, xt, x, nsys, FV, ncls, mm + (long)g.ntchopdim * kabs);
+  band_apply_n<NJ, 5, 2, (NJ == 1 ? 16 : 8)>(nn, xin, jobs.bil ? nsys : 1L, xt, x, nsys, FV, ncls,
+                                             mm + (long)g.ntchopdim * kabs);
   if (mm == 0 && k == 0) {
 #pragma unroll
     for (int jj = 0; jj < NJ; jj++) {
@@ -1210,7 +1286,7 @@ size_t m0_lds_bytes(const KGrid &g) { return sizeof(double) * 12 * (size_t)g.nrc
 
 __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, int has_ln, double lnval,
                          const double2 *b, double2 *a, double *ln_a, const double *ln_b, const double *__restrict__ F,
-                         int k, double *sm) {
+                         int k, double *sm, int bil) {
   const long ncls = (long)g.ntchopdim * g.nxchop;
   if (k >= g.nxchopdim) return;
   // IHELM: A%LN starts as B%LN (COPY), replaced below by -B%LN/ALP
@@ -1220,14 +1296,16 @@ __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, 
   if (nn < 1 || g.ntchop < 1 || kind == 0) return;
   const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
   const long cls = (long)g.ntchopdim * kabs;
-  const double2 *xb = b + (size_t)g.nrchopdim * ((size_t)g.ntchopdim * k);
+  // the right-hand side's rows: column (m = 0, k), or row-interleaved (system NTCHOPDIM k)
+  const size_t bstride = bil ? (size_t)g.ntchopdim * g.nxchopdim : 1;
+  const double2 *xb = bil ? b + (size_t)g.ntchopdim * k : b + (size_t)g.nrchopdim * ((size_t)g.ntchopdim * k);
   double2 *xg = a + (size_t)g.nrchopdim * ((size_t)g.ntchopdim * k);
   double2 *x = reinterpret_cast<double2 *>(sm);  // [nrchopdim]
   double *Fs = sm + 2 * g.nrchopdim;             // [nrchopdim][FSTRIDE]
   double *xt = Fs + FSTRIDE * g.nrchopdim;       // [nrchopdim][3]
   const int nd = kind == 1 ? nn : nn - 1;        // factorised rows
   for (int row = threadIdx.x; row < nn; row += blockDim.x) {
-    x[row] = xb[row];
+    x[row] = xb[row * bstride];
     if (row < nd) {
       const double *fr = factor_view(F, g.nrchopdim, ncls).f7 + ((size_t)row * ncls + cls) * FSTRIDE;
 #pragma unroll
@@ -1277,7 +1355,7 @@ __global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, i
   const int bid = blockIdx.x;
   if (bid < nm0 * jobs.n) {
     const BandJob &J = jobs.j[bid / nm0];
-    apply_m0(g, bxm, op, J.alp, has_ln, lnval, J.b, J.a, J.ln_a, J.ln_b, J.F, bid % nm0, m0s);
+    apply_m0(g, bxm, op, J.alp, has_ln, lnval, J.b, J.a, J.ln_a, J.ln_b, J.F, bid % nm0, m0s, jobs.bil);
     return;
   }
   const int pb = bid - nm0 * jobs.n, grp = pb / npenta;
@@ -1329,7 +1407,7 @@ static double *factor_set(mlg_ctx *c, int op, double alp, const double *const *k
 }
 
 int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, double2 *const *a, double *const *ln_a,
-                        const double *const *ln_b, const double *alp, int has_ln, double ln, double2 *const *t) {
+                        const double *const *ln_b, const double *alp, int has_ln, double ln, double2 *const *t, int bil) {
   if (n < 1 || n > MAX_BAND_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
   long nsys = (long)g.ntchopdim * g.nxchopdim;
@@ -1344,6 +1422,7 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, doub
     jobs.j[i] = BandJob{b[i], a[i], ln_a[i], ln_b[i], F, alp[i], t[i]};
   }
   jobs.n = n;
+  jobs.bil = bil;
   jobs.ng = 0;
   for (int i = 0; i < n; i++) {
     int gi = 0;
@@ -1361,7 +1440,7 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, doub
 
 int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *ln_a, const double *ln_b, double alp,
                       int has_ln, double ln, double2 *t) {
-  return launch_band_solve_n(c, op, 1, &b, &a, &ln_a, &ln_b, &alp, has_ln, ln, &t);
+  return launch_band_solve_n(c, op, 1, &b, &a, &ln_a, &ln_b, &alp, has_ln, ln, &t, 0);
 }
 
 // ---------------------------------------------------------------- pointwise updates

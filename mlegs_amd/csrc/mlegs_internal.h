@@ -226,8 +226,10 @@ struct Visc2Job {
 struct Visc2Jobs {
   Visc2Job j[3];
 };
-// n (<= 3) independent right-hand sides in one launch
-int launch_visc2_rhs_n(mlg_ctx *c, int n, const Visc2Jobs &jobs);
+// n (<= 3) independent right-hand sides in one launch, WK written
+// row-interleaved over the systems (row n of system s = m + NTCHOPDIM k at
+// wk[n * NTCHOPDIM * NXCHOPDIM + s]), as the band solve reads it
+int launch_visc2_rhs_il(mlg_ctx *c, int n, const Visc2Jobs &jobs);
 int launch_set_ln(mlg_ctx *c, double *dst, const double *src, double scale, double add);
 int launch_zero_lns(mlg_ctx *c, double *const *ptrs, int n);  // LN = 0 for n fields, one launch per 32
 int launch_delsqh(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, double *ln_b);
@@ -265,9 +267,11 @@ int launch_fix_at_infinity(mlg_ctx *c, double2 *a);
 int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *ln_a, const double *ln_b, double alp,
                       int has_ln, double ln, double2 *t);
 // n (<= 3) independent solves of one operator in one launch (field i with
-// alp[i], scratch t[i])
+// alp[i], scratch t[i]); bil: the right-hand sides are row-interleaved as
+// written by launch_visc2_rhs_il
 int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, double2 *const *a, double *const *ln_a,
-                        const double *const *ln_b, const double *alp, int has_ln, double ln, double2 *const *t);
+                        const double *const *ln_b, const double *alp, int has_ln, double ln, double2 *const *t,
+                        int bil = 0);
 int launch_axpby(mlg_ctx *c, int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n);
 int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1, const double *x2, const double *x3);
 // ETD update fused with the HYPERV3 filters (filter != 0), the copy of b
