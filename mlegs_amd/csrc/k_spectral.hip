@@ -1527,6 +1527,17 @@ __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, E
   GRID_STRIDE(fff_count(g)) {
     FIdx f = fdecode(g, idx);
     const int nn = s_nrchops(g.nrchop, f.mm * g.minc);
+    if (A.hv && !A.b0 && !(f.n < nn && f.mm < g.ntchop)) {
+      // outside the truncation the filter's factor is zero: every value
+      // written below is the filtered one (zero), so nothing is read
+      A.rur[idx] = zero2();
+      A.rup[idx] = zero2();
+      A.uz[idx] = zero2();
+      A.bo[idx] = zero2();
+      if (save & 1) A.hnr[idx] = A.hnp[idx] = A.hnz[idx] = A.hnb[idx] = zero2();
+      if (save & 2) A.hpr[idx] = A.hpp[idx] = A.hpz[idx] = zero2();
+      continue;
+    }
     double er[16], ar[16];
 #pragma unroll
     for (int q = 0; q < 16; q++) {
