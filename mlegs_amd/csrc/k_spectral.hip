@@ -42,22 +42,27 @@ __device__ __forceinline__ size_t fcol(const KGrid &g, int mm, int k) {
 }
 
 // tri-diagonal band row n of mode mm applied to column x (truncated at nn)
-__device__ __forceinline__ double2 band3_row(const double *bt, const double2 *x, int n, int nn) {
-  const double *r = bt + 3 * n;
+// (r: the table's row n, in memory or already in registers)
+__device__ __forceinline__ double2 band3_r(const double *r, const double2 *x, int n, int nn) {
   double2 o = axc(r[1], x[n]);
   if (n >= 1) o = add2(o, axc(r[0], x[n - 1]));
   if (n + 1 < nn) o = add2(o, axc(r[2], x[n + 1]));
   return o;
 }
+__device__ __forceinline__ double2 band3_row(const double *bt, const double2 *x, int n, int nn) {
+  return band3_r(bt + 3 * n, x, n, nn);
+}
 // penta-diagonal row
-__device__ __forceinline__ double2 band5_row(const double *bt, const double2 *x, int n, int nn) {
-  const double *r = bt + 5 * n;
+__device__ __forceinline__ double2 band5_r(const double *r, const double2 *x, int n, int nn) {
   double2 o = axc(r[2], x[n]);
   if (n >= 2) o = add2(o, axc(r[0], x[n - 2]));
   if (n >= 1) o = add2(o, axc(r[1], x[n - 1]));
   if (n + 1 < nn) o = add2(o, axc(r[3], x[n + 1]));
   if (n + 2 < nn) o = add2(o, axc(r[4], x[n + 2]));
   return o;
+}
+__device__ __forceinline__ double2 band5_row(const double *bt, const double2 *x, int n, int nn) {
+  return band5_r(bt + 5 * n, x, n, nn);
 }
 
 __host__ __device__ static inline long fff_count(const KGrid &g) { return (long)g.nrchopdim * g.ntchopdim * g.nxchopdim; }
@@ -545,10 +550,10 @@ int launch_grad(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *rr, d
 // u_z-like rows -(1-x)^2 [(1-x)^-2 del_h^2 s] as two truncated MULXM passes
 // over DELSQH(s) (with s's log image in row 0 of m0k0).
 // W[l] = MULXM row l of DELSQH(s), the DELSQH rows formed where they are read
-__device__ void uz_stage_direct(const KGrid &g, const double *bxm, const double2 *s, double ln, int mm, int k, int nn,
-                                double2 *W) {
+// row l of W from the MULXM table row r (= bxm row l of mode mm)
+__device__ __forceinline__ double2 uz_stage_r(const KGrid &g, const double *r, const double2 *s, double ln, int mm, int k,
+                                              int l, int nn) {
   const int mv = mm * g.minc;
-  const double *xt = bxm + (size_t)mm * g.nband * 3;
   auto dsq = [&](int j) -> double2 {
     const int N = mv + j;
     const double2 x = s[j];
@@ -557,22 +562,50 @@ __device__ void uz_stage_direct(const KGrid &g, const double *bxm, const double2
     if (mm == 0 && k == 0 && j == 0) v = make_double2(ln / g.ell2 / g.norm0, 0.0);
     return v;
   };
-  for (int l = threadIdx.x; l < nn; l += blockDim.x) {
-    const double *r = xt + 3 * l;
-    double2 o = axc(r[1], dsq(l));
-    if (l >= 1) o = add2(o, axc(r[0], dsq(l - 1)));
-    if (l + 1 < nn) o = add2(o, axc(r[2], dsq(l + 1)));
-    W[l] = o;
-  }
+  double2 o = axc(r[1], dsq(l));
+  if (l >= 1) o = add2(o, axc(r[0], dsq(l - 1)));
+  if (l + 1 < nn) o = add2(o, axc(r[2], dsq(l + 1)));
+  return o;
+}
+__device__ void uz_stage_direct(const KGrid &g, const double *bxm, const double2 *s, double ln, int mm, int k, int nn,
+                                double2 *W) {
+  const double *xt = bxm + (size_t)mm * g.nband * 3;
+  for (int l = threadIdx.x; l < nn; l += blockDim.x) W[l] = uz_stage_r(g, xt + 3 * l, s, ln, mm, k, l, nn);
 }
 // row n of the u_z-like result: -(MULXM row n of W)
-__device__ __forceinline__ double2 uz_from_w(const KGrid &g, const double *bxm, const double2 *W, int mm, int n, int nn) {
-  const double *r = bxm + (size_t)mm * g.nband * 3 + 3 * n;
-  double2 o = axc(r[1], W[n]);
-  if (n >= 1) o = add2(o, axc(r[0], W[n - 1]));
-  if (n + 1 < nn) o = add2(o, axc(r[2], W[n + 1]));
+__device__ __forceinline__ double2 uz_from_w_r(const double *r, const double2 *W, int n, int nn) {
+  const double2 o = band3_r(r, W, n, nn);
   return make_double2(-o.x, -o.y);
 }
+__device__ __forceinline__ double2 uz_from_w(const KGrid &g, const double *bxm, const double2 *W, int mm, int n, int nn) {
+  return uz_from_w_r(bxm + (size_t)mm * g.nband * 3 + 3 * n, W, n, nn);
+}
+// A column kernel's thread handles rows n = threadIdx.x + 256 j.  The table
+// rows of its first two (j = 0, 1) are loaded into registers together with
+// the column itself, before the block's first barrier, so that a block waits
+// out one memory round trip instead of one per stage; rows from 512 on (NRCHOPDIM > 512)
+// read the tables in place.
+template <int NT>
+struct RowTables {
+  double r[2][NT];
+  __device__ __forceinline__ void load(const double *t, int nrows) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int n = threadIdx.x + 256 * j;
+#pragma unroll
+      for (int e = 0; e < NT; e++) r[j][e] = n < nrows ? t[(size_t)n * NT + e] : 0.0;
+    }
+  }
+};
+// f(n, row of table 1, row of table 2) for the thread's rows n < nrows
+#define FOR_MY_ROWS(nrows, T1, t1, T2, t2, f)                                       \
+  do {                                                                              \
+    const int n0_ = threadIdx.x;                                                    \
+    if (n0_ < (nrows)) f(n0_, T1.r[0], T2.r[0]);                                    \
+    if (n0_ + 256 < (nrows)) f(n0_ + 256, T1.r[1], T2.r[1]);                         \
+    for (int n_ = n0_ + 512; n_ < (nrows); n_ += 256)                               \
+      f(n_, (t1) + (size_t)n_ * (sizeof(T1.r[0]) / 8), (t2) + (size_t)n_ * (sizeof(T2.r[0]) / 8)); \
+  } while (0)
 // ---------------------------------------------------------------- PC2VEL
 // (:1789-1853): r u_r = i m psi + i k XXDX chi ; r u_th = -XXDX psi - m k chi ;
 // u_z = -(1-x) (1-x) DELSQH chi.
@@ -678,6 +711,14 @@ __global__ void __launch_bounds__(256) k_pc2velvor(KGrid g, const double *bxm, c
   const size_t cb = fcol(g, mm, k);
   double2 *P = csm, *X = P + g.nrchopdim, *Q = X + g.nrchopdim, *WP = Q + g.nrchopdim, *WX = WP + g.nrchopdim;
   const double lp = ln_psi[0], lc = ln_chi[0];
+  const double *bdm = bd + (size_t)mm * g.nband * 5, *xm = bxm + (size_t)mm * g.nband * 3,
+               *xx = bxxdx + (size_t)mm * g.nband * 3;
+  RowTables<5> T5;
+  RowTables<3> TM, TX;
+  T5.load(bdm, nn);
+  TM.load(xm, nn);
+  TX.load(xx, nn);
+  const double kv = ak[mm + (size_t)g.ntchopdim * k];
   for (int n = threadIdx.x; n < nn; n += blockDim.x) {
     P[n] = psi[cb + n];
     X[n] = chi[cb + n];
@@ -689,17 +730,26 @@ __global__ void __launch_bounds__(256) k_pc2velvor(KGrid g, const double *bxm, c
     const double2 t = X[nn - 1];
     if (hypot(t.x, t.y) > 1.0e-15) atomicCAS(err, 0, 3);
   }
-  for (int n = threadIdx.x; n < nn; n += blockDim.x) Q[n] = del2_row(g, bd, ak, X, lc, mm, k, n, nn);
-  uz_stage_direct(g, bxm, P, lp, mm, k, nn, WP);
-  uz_stage_direct(g, bxm, X, lc, mm, k, nn, WX);
+  // Q = DEL2 chi (del2_row), WP / WX: the MULXM(DELSQH) stages of psi / chi
+  auto stage = [&](int n, const double *r5, const double *rm) {
+    double2 q = band5_r(r5, X, n, nn);
+    q = sub2(q, axc(kv * kv, X[n]));
+    if (mm == 0 && k == 0) {
+      if (n == 0) q.x = q.x + 4.0 / 3 * lc / g.ell2 / g.norm0;
+      if (n == 1) q.x = q.x - 2.0 * lc / g.ell2 / g.norm1;
+      if (n == 2) q.x = q.x + 2.0 / 3 * lc / g.ell2 / g.norm2;
+    }
+    Q[n] = q;
+    WP[n] = uz_stage_r(g, rm, P, lp, mm, k, n, nn);
+    WX[n] = uz_stage_r(g, rm, X, lc, mm, k, n, nn);
+  };
+  FOR_MY_ROWS(nn, T5, bdm, TM, xm, stage);
   __syncthreads();
-  const double *xx = bxxdx + (size_t)mm * g.nband * 3;
-  const double kv = ak[mm + (size_t)g.ntchopdim * k];
-  for (int n = threadIdx.x; n < g.nrchopdim; n += blockDim.x) {
+  auto out = [&](int n, const double *rx, const double *rm) {
     double2 ur = zero2(), up = zero2(), uz = zero2(), wr = zero2(), wp = zero2(), wz = zero2();
     if (n < nn) {
-      double2 xc = band3_row(xx, X, n, nn);
-      double2 xp = band3_row(xx, P, n, nn);
+      double2 xc = band3_r(rx, X, n, nn);
+      double2 xp = band3_r(rx, P, n, nn);
       if (mm == 0 && k == 0) {
         if (n == 0) { xc.x = xc.x + lc / g.norm0; xp.x = xp.x + lp / g.norm0; }
         if (n == 1) { xc.x = xc.x + lc / g.norm1; xp.x = xp.x + lp / g.norm1; }
@@ -707,17 +757,14 @@ __global__ void __launch_bounds__(256) k_pc2velvor(KGrid g, const double *bxm, c
       // velocity (PC2VEL)
       ur = add2(ix((double)mv, P[n]), ix(kv, xc));
       up = sub2(make_double2(-xp.x, -xp.y), axc(mv * kv, X[n]));
-      uz = uz_from_w(g, bxm, WX, mm, n, nn);
-      // vorticity (PC2VOR)
+      uz = uz_from_w_r(rm, WX, n, nn);
+      // vorticity (PC2VOR); XXDX of Q (Q carries no log term)
       const double2 q = Q[n];
       const double2 imq = ix((double)mv, q);
       wr = add2(make_double2(-imq.x, -imq.y), ix(kv, xp));
-      const double *r = xx + 3 * n;
-      double2 xq = axc(r[1], q);
-      if (n >= 1) xq = add2(xq, axc(r[0], Q[n - 1]));
-      if (n + 1 < nn) xq = add2(xq, axc(r[2], Q[n + 1]));
+      const double2 xq = band3_r(rx, Q, n, nn);
       wp = sub2(xq, axc(kv * mv, P[n]));
-      wz = uz_from_w(g, bxm, WP, mm, n, nn);
+      wz = uz_from_w_r(rm, WP, n, nn);
     }
     o.e[0][cb + n] = ur;
     o.e[1][cb + n] = up;
@@ -725,7 +772,8 @@ __global__ void __launch_bounds__(256) k_pc2velvor(KGrid g, const double *bxm, c
     o.e[3][cb + n] = wr;
     o.e[4][cb + n] = wp;
     o.e[5][cb + n] = wz;
-  }
+  };
+  FOR_MY_ROWS(g.nrchopdim, TX, xx, TM, xm, out);
   if (mm == 0 && k == 0 && threadIdx.x < 6) o.ln[threadIdx.x][0] = 0.0;
 }
 int launch_pc2velvor(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
@@ -751,27 +799,31 @@ __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, 
   const size_t cb = fcol(g, mm, k);
   double2 *P = csm, *X = P + g.nrchopdim, *WX = X + g.nrchopdim;
   const double lp = ln_psi[0], lc = ln_chi[0];
+  const double *xm = bxm + (size_t)mm * g.nband * 3, *xx = bxxdx + (size_t)mm * g.nband * 3;
+  RowTables<3> TM, TX;
+  TM.load(xm, nn);
+  TX.load(xx, nn);
+  const double kv = ak[mm + (size_t)g.ntchopdim * k];
   for (int n = threadIdx.x; n < nn; n += blockDim.x) {
     P[n] = psi[cb + n];
     X[n] = chi[cb + n];
   }
   __syncthreads();
-  uz_stage_direct(g, bxm, X, lc, mm, k, nn, WX);
+  auto stage = [&](int n, const double *rm, const double *) { WX[n] = uz_stage_r(g, rm, X, lc, mm, k, n, nn); };
+  FOR_MY_ROWS(nn, TM, xm, TM, xm, stage);
   __syncthreads();
-  const double *xx = bxxdx + (size_t)mm * g.nband * 3;
-  const double kv = ak[mm + (size_t)g.ntchopdim * k];
-  for (int n = threadIdx.x; n < g.nrchopdim; n += blockDim.x) {
+  auto out = [&](int n, const double *rx, const double *rm) {
     double2 vr = zero2(), vp = zero2(), vz = zero2();
     if (n < nn) {
-      double2 xc = band3_row(xx, X, n, nn);
-      double2 xp = band3_row(xx, P, n, nn);
+      double2 xc = band3_r(rx, X, n, nn);
+      double2 xp = band3_r(rx, P, n, nn);
       if (mm == 0 && k == 0) {
         if (n == 0) { xc.x = xc.x + lc / g.norm0; xp.x = xp.x + lp / g.norm0; }
         if (n == 1) { xc.x = xc.x + lc / g.norm1; xp.x = xp.x + lp / g.norm1; }
       }
       vr = add2(ix((double)mv, P[n]), ix(kv, xc));
       vp = sub2(make_double2(-xp.x, -xp.y), axc(mv * kv, X[n]));
-      vz = uz_from_w(g, bxm, WX, mm, n, nn);
+      vz = uz_from_w_r(rm, WX, n, nn);
     }
     const size_t i = cb + n;
     if (HIST) {
@@ -793,7 +845,8 @@ __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, 
       rup[i] = vp;
       uz[i] = vz;
     }
-  }
+  };
+  FOR_MY_ROWS(g.nrchopdim, TX, xx, TM, xm, out);
   if (!HIST && mm == 0 && k == 0 && threadIdx.x == 0) {
     l1[0] = 0.0;
     l2[0] = 0.0;
