@@ -64,6 +64,32 @@ __device__ __forceinline__ double2 band5_r(const double *r, const double2 *x, in
 __device__ __forceinline__ double2 band5_row(const double *bt, const double2 *x, int n, int nn) {
   return band5_r(bt + 5 * n, x, n, nn);
 }
+// A column kernel's thread handles rows n = threadIdx.x + 256 j.  The table
+// rows of its first two (j = 0, 1) are loaded into registers together with
+// the column itself, before the block's first barrier, so that a block waits
+// out one memory round trip instead of one per stage; rows from 512 on (NRCHOPDIM > 512)
+// read the tables in place.
+template <int NT>
+struct RowTables {
+  double r[2][NT];
+  __device__ __forceinline__ void load(const double *t, int nrows) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int n = threadIdx.x + 256 * j;
+#pragma unroll
+      for (int e = 0; e < NT; e++) r[j][e] = n < nrows ? t[(size_t)n * NT + e] : 0.0;
+    }
+  }
+};
+// f(n, row of table 1, row of table 2) for the thread's rows n < nrows
+#define FOR_MY_ROWS(nrows, T1, t1, T2, t2, f)                                       \
+  do {                                                                              \
+    const int n0_ = threadIdx.x;                                                    \
+    if (n0_ < (nrows)) f(n0_, T1.r[0], T2.r[0]);                                    \
+    if (n0_ + 256 < (nrows)) f(n0_ + 256, T1.r[1], T2.r[1]);                         \
+    for (int n_ = n0_ + 512; n_ < (nrows); n_ += 256)                               \
+      f(n_, (t1) + (size_t)n_ * (sizeof(T1.r[0]) / 8), (t2) + (size_t)n_ * (sizeof(T2.r[0]) / 8)); \
+  } while (0)
 
 __host__ __device__ static inline long fff_count(const KGrid &g) { return (long)g.nrchopdim * g.ntchopdim * g.nxchopdim; }
 static int nblocks(long total) {
@@ -505,13 +531,16 @@ __global__ void __launch_bounds__(256) k_grad(KGrid g, const double *bxxdx, cons
   const int mm = blockIdx.x, k = blockIdx.y;
   const int mv = mm * g.minc, nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0;
   const size_t cb = fcol(g, mm, k);
+  const double *xx = bxxdx + (size_t)mm * g.nband * 3;
+  RowTables<3> TX;
+  TX.load(xx, nn);
+  const double kv = ak[mm + (size_t)g.ntchopdim * k];
   for (int n = threadIdx.x; n < nn; n += blockDim.x) csm[n] = a[cb + n];
   __syncthreads();
-  const double kv = ak[mm + (size_t)g.ntchopdim * k];
-  for (int n = threadIdx.x; n < g.nrchopdim; n += blockDim.x) {
+  auto out = [&](int n, const double *rx, const double *) {
     double2 vr = zero2(), vp = zero2(), vz = zero2();
     if (n < nn) {
-      vr = band3_row(bxxdx + (size_t)mm * g.nband * 3, csm, n, nn);
+      vr = band3_r(rx, csm, n, nn);
       double2 x = csm[n];
       vp = ix((double)mv, x);
       vz = ix(kv, x);
@@ -525,7 +554,8 @@ __global__ void __launch_bounds__(256) k_grad(KGrid g, const double *bxxdx, cons
       rp[cb + n] = vp;
       z[cb + n] = vz;
     }
-  }
+  };
+  FOR_MY_ROWS(g.nrchopdim, TX, xx, TX, xx, out);
   if (mm == 0 && k == 0 && threadIdx.x == 0) {
     ln_rr[0] = 0.0;
     if (rp) {
@@ -580,32 +610,6 @@ __device__ __forceinline__ double2 uz_from_w_r(const double *r, const double2 *W
 __device__ __forceinline__ double2 uz_from_w(const KGrid &g, const double *bxm, const double2 *W, int mm, int n, int nn) {
   return uz_from_w_r(bxm + (size_t)mm * g.nband * 3 + 3 * n, W, n, nn);
 }
-// A column kernel's thread handles rows n = threadIdx.x + 256 j.  The table
-// rows of its first two (j = 0, 1) are loaded into registers together with
-// the column itself, before the block's first barrier, so that a block waits
-// out one memory round trip instead of one per stage; rows from 512 on (NRCHOPDIM > 512)
-// read the tables in place.
-template <int NT>
-struct RowTables {
-  double r[2][NT];
-  __device__ __forceinline__ void load(const double *t, int nrows) {
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-      const int n = threadIdx.x + 256 * j;
-#pragma unroll
-      for (int e = 0; e < NT; e++) r[j][e] = n < nrows ? t[(size_t)n * NT + e] : 0.0;
-    }
-  }
-};
-// f(n, row of table 1, row of table 2) for the thread's rows n < nrows
-#define FOR_MY_ROWS(nrows, T1, t1, T2, t2, f)                                       \
-  do {                                                                              \
-    const int n0_ = threadIdx.x;                                                    \
-    if (n0_ < (nrows)) f(n0_, T1.r[0], T2.r[0]);                                    \
-    if (n0_ + 256 < (nrows)) f(n0_ + 256, T1.r[1], T2.r[1]);                         \
-    for (int n_ = n0_ + 512; n_ < (nrows); n_ += 256)                               \
-      f(n_, (t1) + (size_t)n_ * (sizeof(T1.r[0]) / 8), (t2) + (size_t)n_ * (sizeof(T2.r[0]) / 8)); \
-  } while (0)
 // ---------------------------------------------------------------- PC2VEL
 // (:1789-1853): r u_r = i m psi + i k XXDX chi ; r u_th = -XXDX psi - m k chi ;
 // u_z = -(1-x) (1-x) DELSQH chi.
@@ -804,6 +808,19 @@ __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, 
   TM.load(xm, nn);
   TX.load(xx, nn);
   const double kv = ak[mm + (size_t)g.ntchopdim * k];
+  // H is read from `base` when given (the values the histories were to
+  // receive just before), else from the histories themselves; the first two
+  // rows' values are loaded with the column
+  const double2 *hs[3] = {base.e[0] ? base.e[0] : rur, base.e[0] ? base.e[1] : rup, base.e[0] ? base.e[2] : uz};
+  double2 hv[2][3];
+  if (HIST) {
+#pragma unroll
+    for (int j = 0; j < 2; j++) {
+      const int n = threadIdx.x + 256 * j;
+#pragma unroll
+      for (int q = 0; q < 3; q++) hv[j][q] = n < g.nrchopdim ? hs[q][cb + n] : zero2();
+    }
+  }
   for (int n = threadIdx.x; n < nn; n += blockDim.x) {
     P[n] = psi[cb + n];
     X[n] = chi[cb + n];
@@ -812,7 +829,7 @@ __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, 
   auto stage = [&](int n, const double *rm, const double *) { WX[n] = uz_stage_r(g, rm, X, lc, mm, k, n, nn); };
   FOR_MY_ROWS(nn, TM, xm, TM, xm, stage);
   __syncthreads();
-  auto out = [&](int n, const double *rx, const double *rm) {
+  auto out = [&](int n, const double *rx, const double *rm, const double2 *h) {
     double2 vr = zero2(), vp = zero2(), vz = zero2();
     if (n < nn) {
       double2 xc = band3_r(rx, X, n, nn);
@@ -827,10 +844,7 @@ __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, 
     }
     const size_t i = cb + n;
     if (HIST) {
-      // H is read from `base` when given (the values the histories were to
-      // receive just before), else from the histories themselves
-      const double2 h0 = base.e[0] ? base.e[0][i] : rur[i], h1 = base.e[0] ? base.e[1][i] : rup[i],
-                    h2 = base.e[0] ? base.e[2][i] : uz[i];
+      const double2 h0 = h[0], h1 = h[1], h2 = h[2];
       if (hmode == 3) {
         rur[i] = axc(ha, sub2(h0, vr));
         rup[i] = axc(ha, sub2(h1, vp));
@@ -846,7 +860,17 @@ __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, 
       uz[i] = vz;
     }
   };
-  FOR_MY_ROWS(g.nrchopdim, TX, xx, TM, xm, out);
+  const int n0 = threadIdx.x;
+  if (n0 < g.nrchopdim) out(n0, TX.r[0], TM.r[0], hv[0]);
+  if (n0 + 256 < g.nrchopdim) out(n0 + 256, TX.r[1], TM.r[1], hv[1]);
+  for (int n = n0 + 512; n < g.nrchopdim; n += 256) {
+    double2 hl[3];
+    if (HIST) {
+#pragma unroll
+      for (int q = 0; q < 3; q++) hl[q] = hs[q][cb + n];
+    }
+    out(n, xx + (size_t)n * 3, xm + (size_t)n * 3, hl);
+  }
   if (!HIST && mm == 0 && k == 0 && threadIdx.x == 0) {
     l1[0] = 0.0;
     l2[0] = 0.0;
