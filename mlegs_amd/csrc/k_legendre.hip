@@ -60,18 +60,20 @@ union RtranBwdLds {
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_rtran_bwd(KGrid g, const double *__restrict__ pf, TransformJobs jobs, int nj, const double *__restrict__ lnx) {
-  // logical block (field fastest, then column tile, radial tile, m): the
-  // fields of a batch that use one table tile run together on one XCD
+  // logical block (radial tile fastest, then field, column tile, m): the
+  // radial tiles that read one data tile run one after another on one XCD
+  // (the data come from L2 after the first), and the mode's table tiles
+  // (all radial tiles of m) stay in that L2 for every field and column tile
   const int nk = g.nxchopdim;
   const int ntiles_c = (2 * nk + BW_TC - 1) / BW_TC, ntiles_i = (g.nrh + BW_TI - 1) / BW_TI;
-  const int lidf = xcd_logical(ntiles_c * ntiles_i * g.ntchopdim * nj);
-  if (lidf < 0) return;
-  const int fj = lidf % nj, lid = lidf / nj;
+  const int lid = xcd_logical(ntiles_c * ntiles_i * g.ntchopdim * nj);
+  if (lid < 0) return;
+  const int ti = lid % ntiles_i, r1 = lid / ntiles_i;
+  const int fj = r1 % nj, r2 = r1 / nj;
+  const int tc = r2 % ntiles_c, mm = r2 / ntiles_c;
   const double2 *__restrict__ in = jobs.in[fj];
   const double *__restrict__ ln_in = jobs.ln[fj];
   double2 *__restrict__ out = jobs.out[fj];
-  const int mm = lid / (ntiles_c * ntiles_i), rem = lid - mm * (ntiles_c * ntiles_i);
-  const int ti = rem / ntiles_c, tc = rem - ti * ntiles_c;
   const int mv = mm * g.minc;
   const int nn = d_nrchops(g.nrchop, mv);
   const int i0 = ti * BW_TI, c0 = tc * BW_TC;
