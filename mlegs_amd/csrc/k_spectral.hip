@@ -1720,9 +1720,12 @@ int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt,
     A.hd = c->d.hv + (nr1 + nt + nx);
     A.nrc = G.p.nrchop;
   }
-  // 4 fields + 4 nonlinear terms in, 4 out; history: 7 in (ab), 4 + 3 out (save bits); b0 out
-  double nf = 12.0 + (ab ? 7.0 : 0.0) + ((save & 1) ? 4.0 : 0.0) + ((save & 2) ? 3.0 : 0.0) + (b0 ? 1.0 : 0.0);
-  KScope ks(c, K_ETD, 16.0 * fff_count(g) * nf);
+  // 4 fields + 4 nonlinear terms in, 4 out; history: 7 in (ab), 4 + 3 out
+  // (save bits); b0 out.  With the filter (and no b0) only the coefficients
+  // inside the truncation are read.
+  const double nin = 8.0 + (ab ? 7.0 : 0.0), nout = 4.0 + ((save & 1) ? 4.0 : 0.0) + ((save & 2) ? 3.0 : 0.0) + (b0 ? 1.0 : 0.0);
+  const double cin = (A.hv && !b0) ? (double)sum_nrchops(c) * g.nxchopdim : (double)fff_count(g);
+  KScope ks(c, K_ETD, 16.0 * (cin * nin + (double)fff_count(g) * nout));
   hipLaunchKernelGGL(k_etd_sub, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, M, dt, A, ab, save);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
