@@ -874,8 +874,7 @@ static int to_pfp_n(mlg_ctx *c, int n, mlg_field *const *fs, mlg_field *const *k
     in[i] = fs[i]->e;
     out[i] = t[i]->e;  // the FFF storage (or keep[i]'s), free again
   }
-  RET(launch_zero_lns(c, lns, n));
-  RET(launch_verfft_bwd_n(c, n, in, out));
+  RET(launch_verfft_bwd_n(c, n, in, out, 0, 0, lns, n));  // the PFF fields' LN = 0
   for (int i = 0; i < n; i++) {
     swap_storage(fs[i], t[i]);
     fs[i]->space = MLG_PFP;
@@ -926,8 +925,7 @@ static int to_pfp_b(mlg_ctx *c, mlg_field *const *fs, const mlg_field *b, mlg_fi
   out[n] = rbp->e;
   in[n + 1] = t[n]->e;
   out[n + 1] = bz->e;
-  RET(launch_zero_lns(c, lns, n));
-  RET(launch_verfft_bwd_n(c, n + 2, in, out, 1u << (n + 1), 1u << n));
+  RET(launch_verfft_bwd_n(c, n + 2, in, out, 1u << (n + 1), 1u << n, lns, n));  // the PFF fields' LN = 0
   for (int i = 0; i < n; i++) {
     swap_storage(fs[i], t[i]);
     fs[i]->space = MLG_PFP;
@@ -1479,9 +1477,12 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   double dt = c->g.p.dt;
   s->t += dt;
   s->n += 1;
+  // No LN is cleared here: PC2VEL+PC2VOR writes those of rurn .. ozn, NONLIN
+  // that of bn, and the ETD launch sets W%LN = 0 before PROJECT
   Temps T(c);
-  mlg_field *rurn = T.get(), *rupn = T.get(), *uzn = T.get(), *rorn = T.get(), *ropn = T.get(), *ozn = T.get();
-  mlg_field *bn = T.get(), *w = T.get();
+  mlg_field *rurn = T.buffer(), *rupn = T.buffer(), *uzn = T.buffer(), *rorn = T.buffer(), *ropn = T.buffer(),
+            *ozn = T.buffer();
+  mlg_field *bn = T.buffer(), *w = T.buffer();
   mlg_field *ur = T.buffer(), *up = T.buffer(), *uz = T.buffer(), *psi1 = T.buffer(), *chi1 = T.buffer(), *b1 = T.buffer();
   mlg_field *const hk[3] = {T.buffer(), T.buffer(), T.buffer()};
   RET(T.ready());
@@ -1500,8 +1501,8 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   // write the new ones into them.  Without VISC2 they are updated in place.
   const bool cn = c->g.p.visc_sw == 1;
   mlg_field *bj = b, *pj = psi, *cj = chi;
-  double *lnd[2] = {nullptr, nullptr};
-  const double *lns[2] = {nullptr, nullptr};
+  double *lnd[3] = {nullptr, nullptr, w->ln};  // W%LN = 0
+  const double *lns[3] = {nullptr, nullptr, nullptr};
   if (cn) {  // b1, psi1 take the LN of b, psi (copied by the ETD launch)
     bj = b1, pj = psi1, cj = chi1;
     bj->space = pj->space = MLG_FFF;

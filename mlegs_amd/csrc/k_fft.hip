@@ -379,6 +379,7 @@ __global__ void __launch_bounds__(512) k_verfft_bwd4(KGrid g, const double2 *__r
   const double2 *__restrict__ in = jobs.in[fj];
   double2 *__restrict__ out = jobs.out[fj];
   const bool dz = (jobs.dz >> fj) & 1u, dm = (jobs.dm >> fj) & 1u;
+  if (lid == 0 && (int)threadIdx.x < jobs.nzl) jobs.zl[threadIdx.x][0] = 0.0;
   __shared__ double2 s[P * N];
   const int r = threadIdx.x % P, t = threadIdx.x / P;
   const int rr = bx * P + r;
@@ -544,11 +545,16 @@ int launch_verfft_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *co
 
 int launch_verfft_fwd(mlg_ctx *c, const double2 *in, double2 *out) { return launch_verfft_fwd_n(c, 1, &in, &out); }
 
-int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out, unsigned dz, unsigned dm) {
-  if (n < 1 || n > MAX_TRANSFORM_JOBS) return MLG_E_ARG;
+int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out, unsigned dz, unsigned dm,
+                        double *const *zl, int nzl) {
+  if (n < 1 || n > MAX_TRANSFORM_JOBS || nzl < 0 || nzl > MAX_TRANSFORM_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
   if (g.nx > 256) {
     if (dz || dm) return MLG_E_ARG;  // derivatives only in the four-step kernel
+    if (nzl > 0) {
+      const int rc = launch_zero_lns(c, zl, nzl);
+      if (rc) return rc;
+    }
     for (int i = 0; i < n; i++) {
       const int rc = launch_verfft_bwd_r2(c, in[i], out[i]);
       if (rc) return rc;
@@ -563,6 +569,8 @@ int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *co
   }
   jobs.dz = dz;
   jobs.dm = dm;
+  jobs.nzl = nzl;
+  for (int i = 0; i < nzl; i++) jobs.zl[i] = zl[i];
   KScope ks(c, K_VERFFT_BWD, n * 16.0 * g.nr * g.ntchopdim * (double)(g.nx + g.nxchopdim));
 #define L_VB(a, b)                                                                                                    \
   hipLaunchKernelGGL((k_verfft_bwd4<a, b>), dim3(xcd_grid((g.ndimr + Fft4<a, b>::PB - 1) / Fft4<a, b>::PB * g.ntchopdim * n)), \

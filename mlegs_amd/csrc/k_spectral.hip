@@ -1578,8 +1578,8 @@ struct EtdArgs {
   double2 *hnr, *hnp, *hnz, *hnb;    // nonlinear-term history: read (ab), written (save & 1)
   double2 *b0;                       // receives b before the update (or null)
   double2 *bo;                       // receives the updated b (may be b)
-  double *lnd[2];                    // LN copies made on the way: *lnd[i] = *lns[i] (null: none)
-  const double *lns[2];
+  double *lnd[3];                    // LN values set on the way: *lnd[i] = *lns[i], or 0 for a null
+  const double *lns[3];              // lns[i] (null lnd[i]: none)
   const double *hv, *hd;             // filter tables [hr | ht | hx] (or null: no filter)
   int nrc;
   double tf;
@@ -1600,7 +1600,8 @@ __device__ __forceinline__ double2 filt_apply(const KGrid &g, const FIdx &f, int
 }
 
 __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, EtdArgs A, int ab, int save) {
-  if (blockIdx.x == 0 && threadIdx.x < 2 && A.lnd[threadIdx.x]) A.lnd[threadIdx.x][0] = A.lns[threadIdx.x][0];
+  if (blockIdx.x == 0 && threadIdx.x < 3 && A.lnd[threadIdx.x])
+    A.lnd[threadIdx.x][0] = A.lns[threadIdx.x] ? A.lns[threadIdx.x][0] : 0.0;
   GRID_STRIDE(fff_count(g)) {
     FIdx f = fdecode(g, idx);
     const int nn = s_nrchops(g.nrchop, f.mm * g.minc);
@@ -1711,7 +1712,8 @@ int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt,
     M.nl[q] = nl16[q];
   }
   EtdArgs A{rur, rup, uz, b, nr_, np_, nz_, nb_, hpr, hpp, hpz, hnr, hnp, hnz, hnb, b0, bout ? bout : b,
-            {lnd ? lnd[0] : nullptr, lnd ? lnd[1] : nullptr}, {lns ? lns[0] : nullptr, lns ? lns[1] : nullptr},
+            {lnd ? lnd[0] : nullptr, lnd ? lnd[1] : nullptr, lnd ? lnd[2] : nullptr},
+            {lns ? lns[0] : nullptr, lns ? lns[1] : nullptr, lns ? lns[2] : nullptr},
             nullptr, nullptr, 0, tf};
   if (filter) {
     const mlg::Grid &G = c->g;

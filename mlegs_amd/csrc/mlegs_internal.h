@@ -167,12 +167,15 @@ struct TransformJobs {
   double2 *out[MAX_TRANSFORM_JOBS];
   unsigned dz = 0;  // VERFFT backward: bit j set = job j transforms i k times its input (d/dz)
   unsigned dm = 0;  // VERFFT backward: bit j set = job j transforms i m times its input (d/dtheta)
+  double *zl[MAX_TRANSFORM_JOBS];  // VERFFT backward: LN coefficients set to zero by the launch
+  int nzl = 0;
 };
 int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double *const *ln_in, double2 *const *out);
 // dz, dm: bit j set = job j transforms d/dz (i k times) or d/dtheta (i m
-// times) of its input (four-step lengths, NX <= 256, only)
+// times) of its input (four-step lengths, NX <= 256, only); zl[0..nzl-1]:
+// LN coefficients the launch sets to zero (nzl <= MAX_TRANSFORM_JOBS)
 int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out, unsigned dz = 0,
-                        unsigned dm = 0);
+                        unsigned dm = 0, double *const *zl = nullptr, int nzl = 0);
 int launch_verfft_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out);
 int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out);
 int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *psi, double2 *chi);
@@ -283,7 +286,7 @@ int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt,
                    double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_, double2 *hpr,
                    double2 *hpp, double2 *hpz, double2 *hnr, double2 *hnp, double2 *hnz, double2 *hnb, int ab, int save,
                    double2 *b0, double2 *bout, int filter, double tf, double *const *lnd = nullptr,
-                   const double *const *lns = nullptr);
+                   const double *const *lns = nullptr);  // lnd, lns: 3 entries, *lnd[i] = *lns[i] (null lns[i]: 0)
 int launch_hyp3(mlg_ctx *c, double2 *a, int diff, double tf);
 int launch_diagnost(mlg_ctx *c, const double2 *psi);
 int launch_energy_terms(mlg_ctx *c, const double2 *a, const double2 *b, int l2, double *out_all);
