@@ -793,7 +793,7 @@ static int project_impl(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field
       std::swap(keep[i]->ln, vel[i]->ln);
       keep[i]->space = MLG_FFF;
     }
-    return launch_fix_at_infinity(c, psi->e);
+    return 0;  // the value-at-infinity fix-up of psi is part of the launch
   } else if (rur->space == MLG_PPP) {
     RET(mlg_plnfft(rur, -1));
     RET(mlg_plnfft(rup, -1));

@@ -830,22 +830,45 @@ __global__ void __launch_bounds__(256) k_m0_rows(KGrid g, const double *__restri
   }
 }
 // the additions themselves (one block)
+// Block k (grid: NXCHOPDIM blocks) then applies PROJECT's value-at-infinity
+// fix-up to psi's (m = 0, k) column, after block 0's additions to (0, 0):
+// A(0,0,k) -= sum_n AT1(n) A(n,0,k) / AT1(0) (:2541-2543), the products
+// formed in parallel and summed by one thread in ascending n.
 __global__ void __launch_bounds__(256) k_project_fff_m0(KGrid g, const double *__restrict__ vec, const double2 *__restrict__ rup,
                                                         const double *__restrict__ ln_rup, const double *__restrict__ ln_uz,
                                                         double2 *__restrict__ psi, double *__restrict__ psi_ln,
-                                                        double2 *__restrict__ chi, int ln_mode, double ln) {
+                                                        double2 *__restrict__ chi, int ln_mode, double ln,
+                                                        const double *__restrict__ at1) {
   __shared__ double part[4];
-  const int nn = g.ntchop >= 1 ? g.nrchop : 0, nrd = g.nrchopdim;
-  const double lr = ln_rup[0], lz = ln_uz[0];
-  double s = 0.0;
-  for (int n = threadIdx.x; n < nn; n += 256) s += vec[n] * rup[n].x;
-  s = block_sum256(s, part);
-  const double pln = ln_mode != 0 ? -(s + lr * vec[4 * nrd + 2 * g.nrh]) / 2.0 : 0.0;  // VEL2PLN (ln_mode 0: none)
-  for (int n = threadIdx.x; n < nn; n += 256) {
-    psi[n].x = psi[n].x - (lr * vec[nrd + n] + pln * vec[3 * nrd + n]);
-    chi[n].x = chi[n].x - lz * vec[2 * nrd + n];
+  extern __shared__ __align__(16) double2 fsp[];  // [NRCHOP] products of the fix-up
+  if (blockIdx.x == 0) {
+    const int nn = g.ntchop >= 1 ? g.nrchop : 0, nrd = g.nrchopdim;
+    const double lr = ln_rup[0], lz = ln_uz[0];
+    double s = 0.0;
+    for (int n = threadIdx.x; n < nn; n += 256) s += vec[n] * rup[n].x;
+    s = block_sum256(s, part);
+    const double pln = ln_mode != 0 ? -(s + lr * vec[4 * nrd + 2 * g.nrh]) / 2.0 : 0.0;  // VEL2PLN (ln_mode 0: none)
+    for (int n = threadIdx.x; n < nn; n += 256) {
+      psi[n].x = psi[n].x - (lr * vec[nrd + n] + pln * vec[3 * nrd + n]);
+      chi[n].x = chi[n].x - lz * vec[2 * nrd + n];
+    }
+    if (threadIdx.x == 0 && ln_mode != 0 && ln_mode != 2) psi_ln[0] = ln_mode == 1 ? ln : pln;
+    __syncthreads();
   }
-  if (threadIdx.x == 0 && ln_mode != 0 && ln_mode != 2) psi_ln[0] = ln_mode == 1 ? ln : pln;
+  if (g.nrchop < 1) return;
+  double2 *col = psi + (size_t)g.nrchopdim * (size_t)g.ntchopdim * blockIdx.x;
+  for (int n = threadIdx.x; n < g.nrchop; n += 256) fsp[n] = make_double2(at1[n] * col[n].x, at1[n] * col[n].y);
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double sx = 0.0, sy = 0.0;
+#pragma unroll 8
+    for (int n = 0; n < g.nrchop; n++) {
+      sx = sx + fsp[n].x;
+      sy = sy + fsp[n].y;
+    }
+    const double2 v0 = col[0];
+    col[0] = make_double2(v0.x - sx / g.at1_0, v0.y - sy / g.at1_0);
+  }
 }
 
 // The composed tables (once) and the m = 0 vectors (again when NRCHOP changes).
@@ -888,8 +911,9 @@ int launch_project_fff(mlg_ctx *c, const double2 *rur, const double2 *rup, const
     hipLaunchKernelGGL(k_project_fff, grid, dim3(256), 0, c->stream, g, c->d.cvdt, c->d.ak, rur, rup, uz, psi, chi);
     MLG_CHECK_HIP(hipGetLastError());
   }
-  hipLaunchKernelGGL(k_project_fff_m0, dim3(1), dim3(256), 0, c->stream, g, c->d.m0vec, rup, ln_u[1], ln_u[2], psi, psi_ln,
-                     chi, ln_mode, ln);
+  // with the value-at-infinity fix-up of psi (one block per k)
+  hipLaunchKernelGGL(k_project_fff_m0, dim3(g.nxchopdim), dim3(256), sizeof(double2) * (size_t)(g.nrchop > 0 ? g.nrchop : 1),
+                     c->stream, g, c->d.m0vec, rup, ln_u[1], ln_u[2], psi, psi_ln, chi, ln_mode, ln, c->d.at1);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

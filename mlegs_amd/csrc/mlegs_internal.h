@@ -182,7 +182,8 @@ int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const dou
 // PROJECT of spectral (FFF) velocities: RTRAN backward, VEL2PLN and the
 // projection products in one pass with the composed tables; ln_mode 0: no
 // VEL2PLN, else as for launch_vel2pln; the velocities' LN coefficients are
-// read from ln_u.  The velocities are not changed.
+// read from ln_u.  The velocities are not changed.  PROJECT's value-at-
+// infinity fix-up of psi is applied by the same launch.
 int launch_project_fff(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, const double *const ln_u[3],
                        double2 *psi, double *psi_ln, double2 *chi, int ln_mode, double ln);
 // ---- FFTs (k_fft.hip)
