@@ -514,9 +514,10 @@ extern "C" int mlg_rtran(mlg_field *a, int is) {
   RET(T.ready());
   if (is == -1) {
     if (a->space != MLG_PFF) return 2;
-    RET(launch_rtran_fwd(c, a->e, t->e));  // writes the truncated rows as zeros
+    RET(launch_rtran_fwd(c, a->e, t->e, a->ln));  // writes the truncated rows as zeros, and LN = 0
     swap_storage(a, t);
     a->space = MLG_FFF;
+    return 0;
   } else {
     if (a->space != MLG_FFF) return 2;
     RET(launch_rtran_bwd(c, a->e, a->ln, t->e));  // writes the padding rows too
@@ -960,9 +961,8 @@ extern "C" int mlg_force(const mlg_field *psi, const mlg_field *chi, const mlg_f
     RET(to_pfp(w));
     const double2 *in[10] = {rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, w->e};
     double2 *out[4] = {ror->e, rop->e, oz->e, bn->e};
-    RET(launch_fused_products(c, in, out, 2 * c->g.p.omega, std::pow(c->g.p.bv0, 2.0), 1));
+    RET(launch_fused_products(c, in, out, 2 * c->g.p.omega, std::pow(c->g.p.bv0, 2.0), 1, bn->ln));  // and N_b's LN = 0
     bn->space = ror->space = rop->space = oz->space = MLG_PFP;
-    RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
   } else {
     RET(mlg_tofp(w));
     bn->space = MLG_PPP;
@@ -1010,9 +1010,8 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
     RET(to_pfp_b(c, fs7, b, rbp, bz, vel_fff));
     const double2 *in[10] = {rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, nullptr};
     double2 *out[4] = {ror->e, rop->e, oz->e, bn->e};
-    RET(launch_fused_products(c, in, out, 0.0, 0.0, 0));
+    RET(launch_fused_products(c, in, out, 0.0, 0.0, 0, bn->ln));  // and N_b's LN = 0
     bn->space = ror->space = rop->space = oz->space = MLG_PFP;
-    RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
     if (!keep_velocity) {
       RET(mlg_horfft(rur, 1));
       RET(mlg_horfft(rup, 1));
@@ -1039,9 +1038,8 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
     RET(to_pfp_n(c, 9, fs, vel_fff, vel_fff ? 3 : 0));
     const double2 *in[10] = {rur->e, rup->e, uz->e, ror->e, rop->e, oz->e, rbr->e, rbp->e, bz->e, nullptr};
     double2 *out[4] = {ror->e, rop->e, oz->e, bn->e};
-    RET(launch_fused_products(c, in, out, 0.0, 0.0, 0));
+    RET(launch_fused_products(c, in, out, 0.0, 0.0, 0, bn->ln));  // and N_b's LN = 0
     bn->space = ror->space = rop->space = oz->space = MLG_PFP;
-    RET(launch_set_ln(c, bn->ln, nullptr, 0.0, 0.0));
     if (!keep_velocity) {
       RET(mlg_horfft(rur, 1));
       RET(mlg_horfft(rup, 1));

@@ -196,12 +196,14 @@ __device__ void fz_forward(bool act, const KGrid &g, double2 *s, double2 *__rest
 struct FusedIO {
   const double2 *in[10];  // rur rup uz ror rop oz rbr rbp bz [W]
   double2 *out[4];        // r N_r, r N_theta, N_z, N_b (PFP)
+  double *zl;             // an LN coefficient the launch sets to zero (N_b's), or null
 };
 
 template <int R1, int R2>
 __global__ void __launch_bounds__(256) k_fused_products(KGrid g, const double *__restrict__ rsq, const double2 *__restrict__ tw,
                                                         const double2 *__restrict__ twh, FusedIO io, double two_omega,
                                                         double n2, int linear) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && io.zl) io.zl[0] = 0.0;
   constexpr int N = R1 * R2, TPP = R1 > R2 ? R1 : R2, GS = FP * TPP, NG = 256 / GS, SL = FP * N;
   static_assert(NG >= 8, "eight concurrent transforms");
   __shared__ double2 S[8 * SL];
@@ -524,6 +526,7 @@ template <bool LINEAR>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_fused_products_256(KGrid g, const double *__restrict__ rsq, const double2 *tw, const double2 *twh, FusedIO io,
                      double two_omega, double n2) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && io.zl) io.zl[0] = 0.0;
   using namespace w256;
   __shared__ double2 S[8 * LP];
   __shared__ double2 Y[8 * N];  // [line][point i][lane]
@@ -624,11 +627,12 @@ bool fused_products_supported(const mlg_ctx *c) {
 }
 
 int launch_fused_products(mlg_ctx *c, const double2 *const in[10], double2 *const out[4], double two_omega, double n2,
-                          int linear) {
+                          int linear, double *zl) {
   KGrid g = kgrid(c);
   FusedIO io;
   for (int i = 0; i < 10; i++) io.in[i] = in[i];
   for (int i = 0; i < 4; i++) io.out[i] = out[i];
+  io.zl = zl;
   // PFP traffic: 9 (10) live input fields read, 4 written
   KScope ks(c, K_PRODUCTS, 16.0 * g.nr * g.ntchopdim * (double)g.nx * ((linear ? 10 : 9) + 4));
   if (g.nth == 256 && g.ntchopdim <= w256::MMAX96) {

@@ -247,7 +247,8 @@ union RtranFwdLds {
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
 k_rtran_fwd(KGrid g, const double *__restrict__ pf, const double *__restrict__ w, const double2 *__restrict__ in,
-            double2 *__restrict__ out) {
+            double2 *__restrict__ out, double *zl) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && zl) zl[0] = 0.0;  // the result's LN
   const int nk = g.nxchopdim;
   const int ntc = (2 * nk + FW_TC - 1) / FW_TC, ntn = (g.nrchopdim + FW_TN - 1) / FW_TN;
   const int lid = xcd_logical(ntc * ntn * g.ntchopdim);
@@ -362,12 +363,12 @@ k_rtran_fwd(KGrid g, const double *__restrict__ pf, const double *__restrict__ w
   }
 }
 
-int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out) {
+int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out, double *zl) {
   KGrid g = kgrid(c);
   int ntn = (g.nrchopdim + FW_TN - 1) / FW_TN, ntc = (2 * g.nxchopdim + FW_TC - 1) / FW_TC;
   dim3 grid(xcd_grid(ntn * ntc * g.ntchopdim));
   KScope ks(c, K_RTRAN_FWD, 2.0 * g.nr * (double)sum_nrchops(c) * g.nxchopdim);
-  hipLaunchKernelGGL(k_rtran_fwd, grid, dim3(256), 0, c->stream, g, c->d.pf, c->d.w, in, out);
+  hipLaunchKernelGGL(k_rtran_fwd, grid, dim3(256), 0, c->stream, g, c->d.pf, c->d.w, in, out, zl);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

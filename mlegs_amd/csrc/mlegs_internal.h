@@ -177,7 +177,7 @@ int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double
 int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out, unsigned dz = 0,
                         unsigned dm = 0, double *const *zl = nullptr, int nzl = 0);
 int launch_verfft_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out);
-int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out);
+int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out, double *zl = nullptr);  // zl: an LN set to 0
 int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *psi, double2 *chi);
 // PROJECT of spectral (FFF) velocities: RTRAN backward, VEL2PLN and the
 // projection products in one pass with the composed tables; ln_mode 0: no
@@ -202,9 +202,10 @@ int launch_force_products(mlg_ctx *c, const double2 *rur, const double2 *rup, co
 int launch_vel2pln(mlg_ctx *c, double2 *rup, double *psi_ln, int ln_mode, double ln);
 // ---- fused theta transforms + products, PFP in and out (k_fused.hip)
 bool fused_products_supported(const mlg_ctx *c);
-// in: rur rup uz ror rop oz rbr rbp bz [W]; out: r N_r, r N_theta, N_z, N_b
+// in: rur rup uz ror rop oz rbr rbp bz [W]; out: r N_r, r N_theta, N_z, N_b;
+// zl: an LN coefficient the launch sets to zero (or null)
 int launch_fused_products(mlg_ctx *c, const double2 *const in[10], double2 *const out[4], double two_omega, double n2,
-                          int linear);
+                          int linear, double *zl = nullptr);
 // ---- spectral-space kernels (k_spectral.hip)
 int launch_chopdo(mlg_ctx *c, double2 *a, int space);
 constexpr int MAX_FIELD_LIST = 4;
