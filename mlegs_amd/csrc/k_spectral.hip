@@ -395,6 +395,7 @@ template <int VR>
 __global__ void __launch_bounds__(256) k_visc2_rhs_il(KGrid g, int nrchop2, const double *__restrict__ bd,
                                                       const double *__restrict__ ak, Visc2Jobs jobs, int *err) {
   constexpr int RPT = VR / 4;
+  static_assert(RPT * 5 % 2 == 0, "row coefficients in pairs");
   __shared__ double2 so[64][VR + 1];
   const Visc2Job &J = jobs.j[blockIdx.z];
   const long nsys = (long)g.ntchopdim * g.nxchopdim;
@@ -431,11 +432,22 @@ __global__ void __launch_bounds__(256) k_visc2_rhs_il(KGrid g, int nrchop2, cons
     if (nb - 2 < nn) {
 #pragma unroll
       for (int i = 0; i < RPT + 4; i++) x[i] = J.ao[cb + min(max(nb - 2 + i, 0), nlast)];
+      const double *b0 = bdm + (size_t)nb * 5;  // the RPT rows' 5 RPT coefficients are contiguous
+      if (nb + RPT <= g.nband && (reinterpret_cast<uintptr_t>(b0) & 15) == 0) {
+        const double2 *bv = reinterpret_cast<const double2 *>(b0);
 #pragma unroll
-      for (int i = 0; i < RPT; i++) {
-        const int n = min(nb + i, nlast);  // DEL2's table holds NRCHOPDIM+2 rows per mode
+        for (int q = 0; q < RPT * 5 / 2; q++) {
+          const double2 v = bv[q];
+          rb[(2 * q) / 5][(2 * q) % 5] = v.x;
+          rb[(2 * q + 1) / 5][(2 * q + 1) % 5] = v.y;
+        }
+      } else {
 #pragma unroll
-        for (int e = 0; e < 5; e++) rb[i][e] = bdm[(size_t)n * 5 + e];
+        for (int i = 0; i < RPT; i++) {
+          const int n = min(nb + i, nlast);  // DEL2's table holds NRCHOPDIM+2 rows per mode
+#pragma unroll
+          for (int e = 0; e < 5; e++) rb[i][e] = bdm[(size_t)n * 5 + e];
+        }
       }
     } else {
 #pragma unroll
