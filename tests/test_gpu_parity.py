@@ -418,6 +418,14 @@ def test_etd2_steps_small():
     run_steps(SMALL_ETD, 3)
 
 
+@pytest.mark.parametrize("cfg", [dict(SMALL, nrchop=21), dict(SMALL_ETD, nrchop=21)], ids=["ab2cn", "etd2"])
+def test_steps_odd_truncation(cfg):
+    """Odd NRCHOP: NRCHOPDIM and the band tables' row count are odd, so every
+    second mode's DEL2 rows are not 16-byte aligned (the VISC2 right-hand
+    sides take their scalar loads), and the row tiles end ragged."""
+    run_steps(cfg, 3)
+
+
 def test_cfg1_ten_steps():
     """The 64x32x32 AB2-CN workload: Richardson start + 9 steps = 10 steps;
     psi and b stay within 1e-12 at every step."""
