@@ -67,8 +67,8 @@ __device__ __forceinline__ double2 band5_row(const double *bt, const double2 *x,
 // A column kernel's thread handles rows n = threadIdx.x + 256 j.  The table
 // rows of its first two (j = 0, 1) are loaded into registers together with
 // the column itself, before the block's first barrier, so that a block waits
-// out one memory round trip instead of one per stage; rows from 512 on (NRCHOPDIM > 512)
-// read the tables in place.
+// out one memory round trip instead of one per stage; rows from 512 on
+// (NRCHOPDIM > 512) read the tables in place.
 template <int NT>
 struct RowTables {
   double r[2][NT];
@@ -81,7 +81,9 @@ struct RowTables {
     }
   }
 };
-// f(n, row of table 1, row of table 2) for the thread's rows n < nrows
+// f(n, row n of table 1, row n of table 2) for the thread's rows n < nrows
+// (t1, t2: the tables T1, T2 hold the first two rows of; NT is recovered
+// from the size of a held row)
 #define FOR_MY_ROWS(nrows, T1, t1, T2, t2, f)                                       \
   do {                                                                              \
     const int n0_ = threadIdx.x;                                                    \
