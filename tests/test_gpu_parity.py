@@ -426,6 +426,12 @@ def test_steps_odd_truncation(cfg):
     run_steps(cfg, 3)
 
 
+def test_steps_long_columns():
+    """NRCHOPDIM > 512 (NR = 1024, NRCHOP = 510): the column kernels' threads
+    take a third row per column, read from the tables in place."""
+    run_steps(dict(SMALL_ETD, nr=1024, nth=8, nx=4, nrchop=510, ntchop=3, nxchop=2), 2)
+
+
 def test_cfg1_ten_steps():
     """The 64x32x32 AB2-CN workload: Richardson start + 9 steps = 10 steps;
     psi and b stay within 1e-12 at every step."""
