@@ -994,15 +994,18 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
                        mlg_field *rop, mlg_field *oz, bool keep_velocity, mlg_field *const *vel_fff) {
   if (b->space != MLG_FFF) return 3;
   mlg_ctx *c = b->ctx;
+  // LN coefficients: GRAD writes those of rbr (rbp, bz: through the fused
+  // path only their E is used), VEL2PLN psi's, IDEL2 chi's; W's (and chi's,
+  // before IDEL2) are cleared by the first launch
   Temps T(c);
-  mlg_field *rbr = T.get(), *rbp = T.get(), *bz = T.get(), *psi = T.get(), *chi = T.get(), *w = T.get();
+  mlg_field *rbr = T.buffer(), *rbp = T.buffer(), *bz = T.buffer(), *psi = T.buffer(), *chi = T.buffer(), *w = T.buffer();
   RET(T.ready());
   int rc;
   if (fused_products_supported(c) && c->g.nx <= 256) {
     // GRAD's r d/dr b is formed in FFF; its i m b and i k b components come
     // from b's own transform (to_pfp_b)
     RET(mlg_chopset(c, 1));
-    rc = launch_grad(c, b->e, b->ln, rbr->e, rbr->ln, nullptr, nullptr, nullptr, nullptr);
+    rc = launch_grad(c, b->e, b->ln, rbr->e, rbr->ln, nullptr, nullptr, nullptr, nullptr, w->ln, chi->ln);
     mlg_chopset(c, -1);
     RET(rc);
     rbr->space = MLG_FFF;
@@ -1028,6 +1031,10 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
     mlg_chopset(c, -3);
     RET(rc);
     return mlg_toff(bn);
+  }
+  {
+    double *lz[3] = {psi->ln, chi->ln, w->ln};
+    RET(launch_zero_lns(c, lz, 3));
   }
   RET(mlg_chopset(c, 1));
   rc = mlg_grad(b, rbr, rbp, bz);

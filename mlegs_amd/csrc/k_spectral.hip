@@ -539,7 +539,7 @@ int launch_check_top_m1(mlg_ctx *c, const double2 *a) {
 // (:1727-1786): (r d/dr b, d/dtheta b, d/dz b) = (XXDX b, i m b, i k b)
 __global__ void __launch_bounds__(256) k_grad(KGrid g, const double *bxxdx, const double *ak, const double2 *a,
                                              const double *ln_a, double2 *rr, double *ln_rr, double2 *rp, double *ln_rp,
-                                             double2 *z, double *ln_z) {
+                                             double2 *z, double *ln_z, double *zl0, double *zl1) {
   extern __shared__ __align__(16) double2 csm[];
   const double ln = ln_a[0];
   const int mm = blockIdx.x, k = blockIdx.y;
@@ -576,13 +576,15 @@ __global__ void __launch_bounds__(256) k_grad(KGrid g, const double *bxxdx, cons
       ln_rp[0] = 0.0;
       ln_z[0] = 0.0;
     }
+    if (zl0) zl0[0] = 0.0;
+    if (zl1) zl1[0] = 0.0;
   }
 }
 int launch_grad(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *rr, double *ln_rr, double2 *rp, double *ln_rp,
-                double2 *z, double *ln_z) {
+                double2 *z, double *ln_z, double *zl0, double *zl1) {
   KGrid g = kgrid(c);
   hipLaunchKernelGGL(k_grad, column_grid(g), dim3(256), sizeof(double2) * (size_t)g.nrchopdim, c->stream, g, c->d.bxxdx,
-                     c->d.ak, a, ln_a, rr, ln_rr, rp, ln_rp, z, ln_z);
+                     c->d.ak, a, ln_a, rr, ln_rr, rp, ln_rp, z, ln_z, zl0, zl1);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

@@ -241,8 +241,9 @@ int launch_delsqh(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, 
 int launch_idelsqh(mlg_ctx *c, const double2 *b, double2 *a, double *ln_a);
 int launch_band3(mlg_ctx *c, int which, const double2 *a, const double *ln_a, double2 *b, double *ln_b);
 int launch_del2(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, double *ln_b);
+// zl0, zl1: further LN coefficients the launch sets to zero (or null)
 int launch_grad(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *rr, double *ln_rr, double2 *rp, double *ln_rp,
-                double2 *z, double *ln_z);
+                double2 *z, double *ln_z, double *zl0 = nullptr, double *zl1 = nullptr);
 int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
                   double2 *rur, double2 *rup, double2 *uz, double *ln_rur, double *ln_rup, double *ln_uz);
 // PC2VEL whose velocities only update the histories: H = H - u (hmode 4) or
