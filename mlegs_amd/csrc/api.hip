@@ -681,13 +681,19 @@ extern "C" int mlg_grad(const mlg_field *a, mlg_field *rr, mlg_field *rp, mlg_fi
   return launch_grad(a->ctx, as->e, as->ln, rr->e, rr->ln, rp->e, rp->ln, z->e, z->ln);
 }
 
-extern "C" int mlg_pc2vel(const mlg_field *psi, const mlg_field *chi, mlg_field *rur, mlg_field *rup, mlg_field *uz) {
+// fix_nrchop >= 1: with PROJECT's fix-up at infinity of psi (rows below
+// fix_nrchop) applied to the values read (launch_pc2vel)
+static int pc2vel_impl(const mlg_field *psi, const mlg_field *chi, mlg_field *rur, mlg_field *rup, mlg_field *uz,
+                       int fix_nrchop) {
   if (!psi || !chi || !rur || !rup || !uz) return MLG_E_ARG;
   if (psi->space != MLG_FFF || chi->space != MLG_FFF) return 3;
   if (rur == psi || rur == chi || rup == psi || rup == chi || uz == psi || uz == chi) return MLG_E_ARG;  // outputs must not alias inputs
   mlg_ctx *c = psi->ctx;
   rur->space = rup->space = uz->space = MLG_FFF;
-  return launch_pc2vel(c, psi->e, psi->ln, chi->e, chi->ln, rur->e, rup->e, uz->e, rur->ln, rup->ln, uz->ln);
+  return launch_pc2vel(c, psi->e, psi->ln, chi->e, chi->ln, rur->e, rup->e, uz->e, rur->ln, rup->ln, uz->ln, fix_nrchop);
+}
+extern "C" int mlg_pc2vel(const mlg_field *psi, const mlg_field *chi, mlg_field *rur, mlg_field *rup, mlg_field *uz) {
+  return pc2vel_impl(psi, chi, rur, rup, uz, 0);
 }
 
 extern "C" int mlg_pc2vor(const mlg_field *psi, const mlg_field *chi, mlg_field *ror, mlg_field *rop, mlg_field *oz) {
@@ -770,8 +776,10 @@ static int to_pff_n(mlg_ctx *c, int n, mlg_field *const *fs, mlg_field *const *k
 
 // keep (optional, FFF velocities only): receives the FFF elements of the
 // velocities, see to_pff_n
+// fix false (non-FFF velocities only): psi's value-at-infinity fix-up is
+// left to the caller
 static int project_impl(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *psi, mlg_field *chi, int ln_mode, double ln,
-                        mlg_field *const *keep = nullptr) {
+                        mlg_field *const *keep = nullptr, bool fix = true) {
   if (!(rur->space == rup->space && rup->space == uz->space)) return 2;
   mlg_ctx *c = rur->ctx;
   mlg_field *const vel[3] = {rur, rup, uz};
@@ -814,7 +822,7 @@ static int project_impl(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field
   }
   if (ln_mode) RET(launch_vel2pln(c, rup->e, psi->ln, ln_mode, ln));
   RET(launch_project(c, rur->e, rup->e, uz->e, psi->e, chi->e));  // CHOPDO of psi, chi included
-  return launch_fix_at_infinity(c, psi->e);
+  return fix ? launch_fix_at_infinity(c, psi->e) : 0;
 }
 
 extern "C" int mlg_project(mlg_field *rur, mlg_field *rup, mlg_field *uz, mlg_field *psi, mlg_field *chi, int has_ln, double ln) {
@@ -1024,10 +1032,13 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
       mlg_field *const pf4[4] = {ror, rop, oz, bn};
       RET(to_pff_n(c, 4, pf4));
     }
-    RET(mlg_project(ror, rop, oz, psi, w, 1, 0.0));
+    // psi is read by PC2VEL only: PROJECT's fix-up at infinity of psi is
+    // made there, on the columns it stages
+    const int nrchop0 = c->g.nrchop;
+    RET(project_impl(ror, rop, oz, psi, w, 1, 0.0, nullptr, false));
     RET(mlg_idel2(w, chi, 0, 0.0));
     RET(mlg_chopset(c, 3));
-    rc = mlg_pc2vel(psi, chi, ror, rop, oz);
+    rc = pc2vel_impl(psi, chi, ror, rop, oz, nrchop0);
     mlg_chopset(c, -3);
     RET(rc);
     return mlg_toff(bn);

@@ -635,13 +635,16 @@ __device__ __forceinline__ double2 uz_from_w(const KGrid &g, const double *bxm, 
 template <bool HIST>
 __global__ void k_pc2vel_col(KGrid g, const double *bxm, const double *bxxdx, const double *ak, const double2 *psi,
                              const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *rur, double2 *rup,
-                             double2 *uz, double *l1, double *l2, double *l3, int hmode, double ha, VelTriple base);
+                             double2 *uz, double *l1, double *l2, double *l3, int hmode, double ha, VelTriple base,
+                             const double *fat1, int fnr);
 int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi, double2 *rur,
-                  double2 *rup, double2 *uz, double *ln_rur, double *ln_rup, double *ln_uz) {
+                  double2 *rup, double2 *uz, double *ln_rur, double *ln_rup, double *ln_uz, int fix_nrchop) {
   KGrid g = kgrid(c);
-  hipLaunchKernelGGL(k_pc2vel_col<false>, column_grid(g), dim3(256), 3 * sizeof(double2) * (size_t)g.nrchopdim, c->stream, g,
-                     c->d.bxm, c->d.bxxdx, c->d.ak, psi, ln_psi, chi, ln_chi, rur, rup, uz, ln_rur, ln_rup, ln_uz, 0, 0.0,
-                     VelTriple{{nullptr, nullptr, nullptr}});
+  const bool fix = fix_nrchop >= 1;
+  if (fix && fix_nrchop > g.nrchop) return MLG_E_ARG;  // the fix-up reads staged rows only
+  hipLaunchKernelGGL(k_pc2vel_col<false>, column_grid(g), dim3(256), (fix ? 4 : 3) * sizeof(double2) * (size_t)g.nrchopdim,
+                     c->stream, g, c->d.bxm, c->d.bxxdx, c->d.ak, psi, ln_psi, chi, ln_chi, rur, rup, uz, ln_rur, ln_rup,
+                     ln_uz, 0, 0.0, VelTriple{{nullptr, nullptr, nullptr}}, fix ? c->d.at1 : nullptr, fix_nrchop);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -650,7 +653,7 @@ int launch_pc2vel_hist(mlg_ctx *c, const double2 *psi, const double *ln_psi, con
   KGrid g = kgrid(c);
   hipLaunchKernelGGL(k_pc2vel_col<true>, column_grid(g), dim3(256), 3 * sizeof(double2) * (size_t)g.nrchopdim, c->stream, g,
                      c->d.bxm, c->d.bxxdx, c->d.ak, psi, ln_psi, chi, ln_chi, hr, hp, hz, nullptr, nullptr, nullptr, hmode, ha,
-                     base);
+                     base, nullptr, 0);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -812,7 +815,7 @@ __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, 
                                                     const double2 *psi, const double *ln_psi, const double2 *chi,
                                                     const double *ln_chi, double2 *rur, double2 *rup, double2 *uz,
                                                     double *l1, double *l2, double *l3, int hmode, double ha,
-                                                    VelTriple base) {
+                                                    VelTriple base, const double *fat1, int fnr) {
   extern __shared__ __align__(16) double2 csm[];
   const int mm = blockIdx.x, k = blockIdx.y;
   const int mv = mm * g.minc, nn = mm < g.ntchop ? s_nrchops(g.nrchop, mv) : 0;
@@ -842,6 +845,23 @@ __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, 
     X[n] = chi[cb + n];
   }
   __syncthreads();
+  if (fat1 && mm == 0 && fnr >= 1) {
+    // PROJECT's value-at-infinity fix-up of the m = 0 column of psi
+    // (:2541-2543), made here on the staged column when this kernel is the
+    // only reader of psi: P(0) -= sum_{n < fnr} AT1(n) P(n) / AT1(0), the
+    // products in parallel, one thread summing in ascending n
+    double2 *F = WX + g.nrchopdim;
+    for (int n = threadIdx.x; n < fnr; n += blockDim.x) F[n] = make_double2(fat1[n] * P[n].x, fat1[n] * P[n].y);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      double sx = 0.0, sy = 0.0;
+      for (int n = 0; n < fnr; n++) {
+        sx = sx + F[n].x;
+        sy = sy + F[n].y;
+      }
+      P[0] = make_double2(P[0].x - sx / g.at1_0, P[0].y - sy / g.at1_0);
+    }
+  }
   auto stage = [&](int n, const double *rm, const double *) { WX[n] = uz_stage_r(g, rm, X, lc, mm, k, n, nn); };
   FOR_MY_ROWS(nn, TM, xm, TM, xm, stage);
   __syncthreads();
