@@ -642,7 +642,9 @@ extern "C" int mlg_del2(const mlg_field *a, mlg_field *b) {
   return launch_del2(a->ctx, a_src->e, a_src->ln, b->e, b->ln);
 }
 
-static int solve_op(const mlg_field *b, mlg_field *a, int op, double alp, int has_ln, double ln) {
+// chop false: A is left without its CHOPDO and fix-up at infinity (for a
+// caller whose only reader of A applies them)
+static int solve_op(const mlg_field *b, mlg_field *a, int op, double alp, int has_ln, double ln, bool chop = true) {
   if (!a || !b) return MLG_E_ARG;
   if (b->space != MLG_FFF) return 3;
   mlg_ctx *c = b->ctx;
@@ -656,7 +658,7 @@ static int solve_op(const mlg_field *b, mlg_field *a, int op, double alp, int ha
   // system); only a truncation without that system needs the zero here
   if (op != 2 && (c->g.nrchop < 1 || c->g.ntchop < 1)) RET(launch_set_ln(c, a->ln, nullptr, 0.0, 0.0));
   RET(launch_band_solve(c, op, b->e, a->e, a->ln, b->ln, alp, has_ln, ln, t->e));
-  RET(launch_chopdo_fff_n(c, 1, &a->e, op != 2));  // CHOPDO, and the fix at infinity of IDEL2, IDEL2LN
+  if (chop) RET(launch_chopdo_fff_n(c, 1, &a->e, op != 2));  // CHOPDO, and the fix at infinity of IDEL2, IDEL2LN
   return 0;
 }
 extern "C" int mlg_idel2(const mlg_field *b, mlg_field *a, int has_ln, double ln) { return solve_op(b, a, 0, 0.0, has_ln, ln); }
@@ -681,8 +683,9 @@ extern "C" int mlg_grad(const mlg_field *a, mlg_field *rr, mlg_field *rp, mlg_fi
   return launch_grad(a->ctx, as->e, as->ln, rr->e, rr->ln, rp->e, rp->ln, z->e, z->ln);
 }
 
-// fix_nrchop >= 1: with PROJECT's fix-up at infinity of psi (rows below
-// fix_nrchop) applied to the values read (launch_pc2vel)
+// fix_nrchop >= 1: psi and chi as PROJECT and IDEL2 left them without
+// IDEL2's CHOPDO and the fix-ups at infinity, which are applied to the
+// values read (launch_pc2vel)
 static int pc2vel_impl(const mlg_field *psi, const mlg_field *chi, mlg_field *rur, mlg_field *rup, mlg_field *uz,
                        int fix_nrchop) {
   if (!psi || !chi || !rur || !rup || !uz) return MLG_E_ARG;
@@ -1032,11 +1035,12 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
       mlg_field *const pf4[4] = {ror, rop, oz, bn};
       RET(to_pff_n(c, 4, pf4));
     }
-    // psi is read by PC2VEL only: PROJECT's fix-up at infinity of psi is
-    // made there, on the columns it stages
+    // psi and chi are read by PC2VEL only: PROJECT's fix-up at infinity of
+    // psi, and IDEL2's CHOPDO and fix-up of chi, are made there, on the
+    // columns it stages
     const int nrchop0 = c->g.nrchop;
     RET(project_impl(ror, rop, oz, psi, w, 1, 0.0, nullptr, false));
-    RET(mlg_idel2(w, chi, 0, 0.0));
+    RET(solve_op(w, chi, 0, 0.0, 0, 0.0, false));  // IDEL2
     RET(mlg_chopset(c, 3));
     rc = pc2vel_impl(psi, chi, ror, rop, oz, nrchop0);
     mlg_chopset(c, -3);

@@ -642,7 +642,7 @@ int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const do
   KGrid g = kgrid(c);
   const bool fix = fix_nrchop >= 1;
   if (fix && fix_nrchop > g.nrchop) return MLG_E_ARG;  // the fix-up reads staged rows only
-  hipLaunchKernelGGL(k_pc2vel_col<false>, column_grid(g), dim3(256), (fix ? 4 : 3) * sizeof(double2) * (size_t)g.nrchopdim,
+  hipLaunchKernelGGL(k_pc2vel_col<false>, column_grid(g), dim3(256), 3 * sizeof(double2) * (size_t)g.nrchopdim,
                      c->stream, g, c->d.bxm, c->d.bxxdx, c->d.ak, psi, ln_psi, chi, ln_chi, rur, rup, uz, ln_rur, ln_rup,
                      ln_uz, 0, 0.0, VelTriple{{nullptr, nullptr, nullptr}}, fix ? c->d.at1 : nullptr, fix_nrchop);
   MLG_CHECK_HIP(hipGetLastError());
@@ -840,26 +840,34 @@ __global__ void __launch_bounds__(256) k_pc2vel_col(KGrid g, const double *bxm, 
       for (int q = 0; q < 3; q++) hv[j][q] = n < g.nrchopdim ? hs[q][cb + n] : zero2();
     }
   }
+  // with fat1: psi and chi come straight from PROJECT and IDEL2 at the
+  // truncation fnr, and this kernel is their only reader.  IDEL2's CHOPDO of
+  // chi is made as the column is staged (rows >= NRCHOPS(fnr, m) read as
+  // zero), and the value-at-infinity fix-ups of both m = 0 columns
+  // (PROJECT :2541-2543, IDEL2 :987-992) on the staged columns:
+  // A(0) -= sum_{n < fnr} AT1(n) A(n) / AT1(0), the products in parallel,
+  // one thread per field summing in ascending n.
+  const int nchi = !fat1 ? nn : (mm < g.ntchop ? s_nrchops(fnr, mv) : 0);
   for (int n = threadIdx.x; n < nn; n += blockDim.x) {
     P[n] = psi[cb + n];
-    X[n] = chi[cb + n];
+    X[n] = n < nchi ? chi[cb + n] : zero2();
   }
   __syncthreads();
   if (fat1 && mm == 0 && fnr >= 1) {
-    // PROJECT's value-at-infinity fix-up of the m = 0 column of psi
-    // (:2541-2543), made here on the staged column when this kernel is the
-    // only reader of psi: P(0) -= sum_{n < fnr} AT1(n) P(n) / AT1(0), the
-    // products in parallel, one thread summing in ascending n
-    double2 *F = WX + g.nrchopdim;
-    for (int n = threadIdx.x; n < fnr; n += blockDim.x) F[n] = make_double2(fat1[n] * P[n].x, fat1[n] * P[n].y);
-    __syncthreads();
-    if (threadIdx.x == 0) {
-      double sx = 0.0, sy = 0.0;
-      for (int n = 0; n < fnr; n++) {
-        sx = sx + F[n].x;
-        sy = sy + F[n].y;
+    double2 *F = WX;  // scratch for the products until the stage below writes WX
+    for (int f = 0; f < 2; f++) {
+      double2 *A = f == 0 ? P : X;
+      for (int n = threadIdx.x; n < fnr; n += blockDim.x) F[n] = make_double2(fat1[n] * A[n].x, fat1[n] * A[n].y);
+      __syncthreads();
+      if (threadIdx.x == 0) {
+        double sx = 0.0, sy = 0.0;
+        for (int n = 0; n < fnr; n++) {
+          sx = sx + F[n].x;
+          sy = sy + F[n].y;
+        }
+        A[0] = make_double2(A[0].x - sx / g.at1_0, A[0].y - sy / g.at1_0);
       }
-      P[0] = make_double2(P[0].x - sx / g.at1_0, P[0].y - sy / g.at1_0);
+      __syncthreads();
     }
   }
   auto stage = [&](int n, const double *rm, const double *) { WX[n] = uz_stage_r(g, rm, X, lc, mm, k, n, nn); };

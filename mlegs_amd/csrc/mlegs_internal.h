@@ -244,9 +244,10 @@ int launch_del2(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, do
 // zl0, zl1: further LN coefficients the launch sets to zero (or null)
 int launch_grad(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *rr, double *ln_rr, double2 *rp, double *ln_rp,
                 double2 *z, double *ln_z, double *zl0 = nullptr, double *zl1 = nullptr);
-// fix_nrchop >= 1: psi first gets PROJECT's value-at-infinity fix-up over
-// its rows n < fix_nrchop (<= NRCHOP), on the staged columns only (for a psi
-// that nothing else reads)
+// fix_nrchop >= 1: psi and chi come straight from PROJECT and IDEL2 at the
+// truncation fix_nrchop (<= NRCHOP), without IDEL2's CHOPDO and either
+// value-at-infinity fix-up, which are applied to the staged columns only
+// (for a psi and chi that nothing else reads)
 int launch_pc2vel(mlg_ctx *c, const double2 *psi, const double *ln_psi, const double2 *chi, const double *ln_chi,
                   double2 *rur, double2 *rup, double2 *uz, double *ln_rur, double *ln_rup, double *ln_uz,
                   int fix_nrchop = 0);
