@@ -189,6 +189,11 @@ int mlg_time_steps(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, 
  * SOLVER_T%TIME_STEPPING, src/mod_march.f90:243-307).  Results equal
  * mlg_time_steps bit for bit. */
 int mlg_time_steps_graph(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, int nsteps);
+/* DIAGNOST (src/mod_march.f90:1726-1791), its per-step checks on given
+ * fields: returns 11 (dealiasing: psi non-zero beyond NRCHOPS(m) at m = 0,
+ * 1) or 12 (divergence: |psi| > 100 at n < 5, 1 <= m < 6, k < 5), else 0.
+ * Synchronising; an earlier unread device-side error is returned first. */
+int mlg_diagnost(const mlg_field *psi, const mlg_field *chi);
 double mlg_solver_time(const mlg_solver *s);
 int mlg_solver_nstep(const mlg_solver *s);
 

@@ -181,6 +181,7 @@ extern "C" int mlg_init(const mlg_params *p, mlg_ctx **out) {
   if (2 * p->nxchop > p->nx || p->nxchop < 1 || p->ntchop < 1 || p->nrchop < 1) return MLG_E_ARG;
   if (p->nth > 4096 || p->nx > 4096) return MLG_E_ARG;
   if (p->minc != 1 && p->minc != 0) return MLG_E_ARG;  // the path uses MINC = 1
+  if (p->visc_sw < 0 || p->visc_sw > 3) return MLG_E_ARG;  // VISC%SW: 0 none, 1 CN, 2 IHELMP, 3 filter
   if (p->mklink != 0) return MLG_E_ARG;
   mlg_ctx *c = new mlg_ctx();
   int rc = build_grid(*p, c->g);
@@ -276,9 +277,11 @@ extern "C" int mlg_finalize(mlg_ctx *c) {
   c->timer.spare.clear();
   DevTables &d = c->d;
   void *ptrs[] = {d.pf, d.vdt, d.w, d.r, d.x, d.lnx, d.at1, d.ak, d.norm, d.lognorm, d.bxm, d.bxxdx, d.bdel2h,
-                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.red, d.vproj, d.cvdt, d.m0vec, d.err};
+                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.red, d.vproj, d.cvdt, d.err};
   for (auto &f : c->fcache) (void)hipFree(f.f);
   c->fcache.clear();
+  for (auto &v : c->m0vecs) (void)hipFree(v.second);
+  c->m0vecs.clear();
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -1547,6 +1550,10 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
 extern "C" int mlg_initialize(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
   if (!s || !psi || !chi || !b) return MLG_E_ARG;
   mlg_ctx *c = s->ctx;
+  // SOLVER_T%INITIALIZE (mod_march.f90:185-201): ETD takes only the
+  // exponential hyperviscosity; VISC%SW = 2 aborts with ERR_FLAGS%SOLVER
+  if (!c->g.p.adams && c->g.p.visc_sw == 2) return 5;
+  if (c->g.p.visc_sw == 2) return MLG_E_ARG;
   int rc;
   if (c->g.p.adams) {
     rc = rich_fe_be(s, psi, chi, b);
@@ -1564,6 +1571,19 @@ static int one_step(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b)
   return launch_diagnost(c, psi->e);
 }
 
+// DIAGNOST's per-step checks (src/mod_march.f90:1726-1791) on the given
+// fields: 11 when psi has coefficients beyond NRCHOPS(m) at m = 0, 1; 12
+// when |psi| > 100 in the lowest modes (n < 5, 1 <= m < 6, k < 5).  An error
+// raised earlier on the device and not yet read is returned first.
+extern "C" int mlg_diagnost(const mlg_field *psi, const mlg_field *chi) {
+  if (!psi || !chi) return MLG_E_ARG;
+  if (psi->space != MLG_FFF || chi->space != MLG_FFF) return 3;
+  mlg_ctx *c = psi->ctx;
+  RET(mlg_synchronize(c));
+  RET(launch_diagnost(c, psi->e));
+  return mlg_synchronize(c);
+}
+
 extern "C" int mlg_time_step(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
   if (!s || !psi || !chi || !b) return MLG_E_ARG;
   RET(one_step(s, psi, chi, b));
@@ -1576,26 +1596,49 @@ extern "C" int mlg_time_steps(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg
   return mlg_synchronize(s->ctx);
 }
 
+// The graph's temporaries go back to the pool and its factor sets are
+// unpinned (after the queued replays have finished with them).
 static void drop_graph(mlg_solver *s) {
+  mlg_ctx *c = s->ctx;
+  if (s->gexec || !s->gpool.empty()) (void)hipStreamSynchronize(c->stream);
   if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
   if (s->graph) (void)hipGraphDestroy(s->graph);
   s->gexec = nullptr;
   s->graph = nullptr;
   s->gpsi = s->gchi = s->gb = nullptr;
+  for (auto &a : s->gaddr) a = nullptr;
+  for (mlg_field *f : s->gpool) c->pool.push_back(f);
+  s->gpool.clear();
+  for (double *p : s->gfactors)
+    for (auto &f : c->fcache)
+      if (f.f == p && f.pins > 0) f.pins--;
+  s->gfactors.clear();
+}
+
+static void field_addrs(const mlg_field *psi, const mlg_field *chi, const mlg_field *b, const void **a) {
+  a[0] = psi->e, a[1] = chi->e, a[2] = b->e, a[3] = psi->ln, a[4] = chi->ln, a[5] = b->ln;
 }
 
 // One step captured into a hipGraph and replayed.  A step keeps psi, chi, b
 // and the solver's history fields in their buffers (transforms only swap
 // buffers of temporaries), so one captured step is valid for every later
-// step on the same fields; the temporaries it uses are fixed at capture.
-// The eager step that precedes capture fills the temporary pool and the
-// band-factor cache, so the captured step allocates and clears nothing.
+// step on the same fields while their device buffers stay the same (a
+// transform of psi, chi or b between calls may swap them: then the step is
+// captured again).  The eager step that precedes capture fills the temporary
+// pool and the band-factor cache, so the captured step allocates and clears
+// nothing.  After capture the graph owns the pooled temporaries (they leave
+// the pool, so no later call can take one over as its own storage) and pins
+// the factor sets in the cache.
 extern "C" int mlg_time_steps_graph(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, int nsteps) {
   if (!s || !psi || !chi || !b || nsteps < 0) return MLG_E_ARG;
   if (nsteps == 0) return 0;
   mlg_ctx *c = s->ctx;
   int done = 0;
-  if (!s->gexec || s->gpsi != psi || s->gchi != chi || s->gb != b || s->gnrchop != c->g.nrchop ||
+  const void *addr[6];
+  field_addrs(psi, chi, b, addr);
+  bool same_buffers = true;
+  for (int i = 0; i < 6; i++) same_buffers = same_buffers && addr[i] == s->gaddr[i];
+  if (!s->gexec || s->gpsi != psi || s->gchi != chi || s->gb != b || !same_buffers || s->gnrchop != c->g.nrchop ||
       s->gstream != c->stream) {
     drop_graph(s);
     RET(one_step(s, psi, chi, b));
@@ -1627,6 +1670,12 @@ extern "C" int mlg_time_steps_graph(mlg_solver *s, mlg_field *psi, mlg_field *ch
     s->gpsi = psi;
     s->gchi = chi;
     s->gb = b;
+    field_addrs(psi, chi, b, s->gaddr);
+    s->gpool.swap(c->pool);  // every temporary the captured step used is in the pool now
+    for (auto &f : c->fcache) {
+      f.pins++;
+      s->gfactors.push_back(f.f);
+    }
     s->gnrchop = c->g.nrchop;
     s->gstream = c->stream;
   }

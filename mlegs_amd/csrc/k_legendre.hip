@@ -872,8 +872,10 @@ __global__ void __launch_bounds__(256) k_project_fff_m0(KGrid g, const double *_
   }
 }
 
-// The composed tables (once) and the m = 0 vectors (again when NRCHOP changes).
-static int ensure_cvdt(mlg_ctx *c) {
+// The composed tables (once) and the m = 0 vectors (once per truncation: a
+// buffer per NRCHOP value, kept until the context ends, so that a captured
+// step that reads one set is never handed another truncation's vectors).
+static int ensure_cvdt(mlg_ctx *c, const double **m0vec) {
   if (c->g.nrh < 1) return MLG_E_ARG;
   KGrid g = kgrid(c);
   if (!c->d.cvdt) {
@@ -884,24 +886,26 @@ static int ensure_cvdt(mlg_ctx *c) {
                        c->d.pf, c->d.vdt, c->d.cvdt);
     MLG_CHECK_HIP(hipGetLastError());
   }
-  if (!c->d.m0vec) {
-    MLG_CHECK_HIP(hipMalloc((void **)&c->d.m0vec, sizeof(double) * (4 * (size_t)g.nrchopdim + 2 * (size_t)g.nrh + 1)));
-    c->m0vec_nrchop = -1;
-  }
-  if (c->m0vec_nrchop != g.nrchop) {
-    const int nn = g.ntchop >= 1 ? g.nrchop : 0;
-    hipLaunchKernelGGL(k_m0_gauss, dim3(g.nrh), dim3(256), 0, c->stream, g, nn, c->d.pf, c->d.w, c->d.lognorm, c->d.m0vec);
-    hipLaunchKernelGGL(k_m0_rows, dim3(g.nrchopdim + 1), dim3(256), 0, c->stream, g, c->d.pf, c->d.vdt, c->d.lnx,
-                       c->d.m0vec);
-    MLG_CHECK_HIP(hipGetLastError());
-    c->m0vec_nrchop = g.nrchop;
-  }
+  for (auto &v : c->m0vecs)
+    if (v.first == g.nrchop) {
+      *m0vec = v.second;
+      return 0;
+    }
+  double *m0 = nullptr;
+  MLG_CHECK_HIP(hipMalloc((void **)&m0, sizeof(double) * (4 * (size_t)g.nrchopdim + 2 * (size_t)g.nrh + 1)));
+  c->m0vecs.emplace_back(g.nrchop, m0);
+  const int nn = g.ntchop >= 1 ? g.nrchop : 0;
+  hipLaunchKernelGGL(k_m0_gauss, dim3(g.nrh), dim3(256), 0, c->stream, g, nn, c->d.pf, c->d.w, c->d.lognorm, m0);
+  hipLaunchKernelGGL(k_m0_rows, dim3(g.nrchopdim + 1), dim3(256), 0, c->stream, g, c->d.pf, c->d.vdt, c->d.lnx, m0);
+  MLG_CHECK_HIP(hipGetLastError());
+  *m0vec = m0;
   return 0;
 }
 
 int launch_project_fff(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, const double *const ln_u[3],
                        double2 *psi, double *psi_ln, double2 *chi, int ln_mode, double ln) {
-  const int rc = ensure_cvdt(c);
+  const double *m0vec = nullptr;
+  const int rc = ensure_cvdt(c, &m0vec);
   if (rc) return rc;
   KGrid g = kgrid(c);
   int ntn = (g.nrchopdim + PJ_TN - 1) / PJ_TN, ntc = (2 * g.nxchopdim + PJ_TC - 1) / PJ_TC;
@@ -914,7 +918,7 @@ int launch_project_fff(mlg_ctx *c, const double2 *rur, const double2 *rup, const
   }
   // with the value-at-infinity fix-up of psi (one block per k)
   hipLaunchKernelGGL(k_project_fff_m0, dim3(g.nxchopdim), dim3(256), sizeof(double2) * (size_t)(g.nrchop > 0 ? g.nrchop : 1),
-                     c->stream, g, c->d.m0vec, rup, ln_u[1], ln_u[2], psi, psi_ln, chi, ln_mode, ln, c->d.at1);
+                     c->stream, g, m0vec, rup, ln_u[1], ln_u[2], psi, psi_ln, chi, ln_mode, ln, c->d.at1);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

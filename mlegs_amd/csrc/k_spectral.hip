@@ -1491,8 +1491,9 @@ __global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, i
 // Factor sets are cached per (operator, alp, truncation): the matrices
 // depend on m, k^2 and those only, so repeated solves (every CN step) only
 // substitute.  Results are bitwise those of factorising on every call.
-// At most eight sets are kept; a miss evicts the oldest set that the
-// current batch does not use.
+// At most eight sets are kept; a miss evicts the oldest set that neither
+// the current batch nor a captured step uses (pinned sets are never
+// evicted; with all sets pinned the cache grows past eight).
 static double *factor_set(mlg_ctx *c, int op, double alp, const double *const *keep, int nkeep, int *rc) {
   KGrid g = kgrid(c);
   double key_alp = (op == 2) ? alp : 0.0;
@@ -1504,7 +1505,7 @@ static double *factor_set(mlg_ctx *c, int op, double alp, const double *const *k
     for (auto it = c->fcache.begin(); it != c->fcache.end(); ++it) {
       bool used = false;
       for (int i = 0; i < nkeep; i++) used = used || keep[i] == it->f;
-      if (used) continue;
+      if (used || it->pins > 0) continue;
       (void)hipStreamSynchronize(c->stream);
       (void)hipFree(it->f);
       c->fcache.erase(it);

@@ -46,7 +46,6 @@ struct DevTables {
   double *red = nullptr;      // reduction scratch
   double *vproj = nullptr;    // VEL2PLN projections [NRCHOPDIM]
   double *cvdt = nullptr;     // PROJECT tables composed with RTRAN (built on first use)
-  double *m0vec = nullptr;    // their m = 0, k = 0 column vectors (built on first use)
   int *err = nullptr;         // sticky device error code
 };
 
@@ -78,6 +77,7 @@ struct FactorSet {
   double alp;
   int nrchop;
   double *f;
+  int pins = 0;  // captured steps that use this set: not evicted while > 0
 };
 struct KTimer {
   bool on = false;
@@ -97,7 +97,8 @@ struct mlg_ctx {
   std::unordered_set<mlg_field *> owned;  // fields created through the C ABI
   std::vector<mlg::FactorSet> fcache;     // band-solve factors
   int device = 0;
-  int m0vec_nrchop = -1;  // truncation the m = 0 vectors were built for
+  // PROJECT's m = 0, k = 0 column vectors, one buffer per truncation (built on first use)
+  std::vector<std::pair<int, double *>> m0vecs;
 };
 
 struct mlg_solver {
@@ -108,10 +109,17 @@ struct mlg_solver {
   mlg_field *r_n_r = nullptr, *r_n_p = nullptr, *n_z = nullptr;
   mlg_field *r_pi_r = nullptr, *r_pi_p = nullptr, *pi_z = nullptr;
   double etd_e[16], etd_nl[16];
-  // graph of one captured step (mlg_time_steps_graph)
+  // graph of one captured step (mlg_time_steps_graph).  The graph holds the
+  // device addresses it was captured with: the temporaries it used are taken
+  // out of the context's pool into gpool and the band-factor sets it reads
+  // are pinned in the cache, both until the graph is dropped; gaddr records
+  // the element and LN buffers of psi, chi, b at capture.
   hipGraph_t graph = nullptr;
   hipGraphExec_t gexec = nullptr;
   mlg_field *gpsi = nullptr, *gchi = nullptr, *gb = nullptr;
+  const void *gaddr[6] = {nullptr, nullptr, nullptr, nullptr, nullptr, nullptr};
+  std::vector<mlg_field *> gpool;
+  std::vector<double *> gfactors;
   int gnrchop = -1;
   hipStream_t gstream = nullptr;
 };

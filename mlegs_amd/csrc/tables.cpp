@@ -116,23 +116,37 @@ void band_xxdx(int ni, int m, const double *ln, double *b) {
   }
 }
 
+// L^2 times the horizontal Laplacian, (1-x)^-2 del_h^2, on the mapped
+// Legendre functions (Matsushima & Marcus 1997): with n the degree, the image
+// of P_n^m spans degrees n-2 .. n+2, with coefficients
+//   d-2: -(n-m-1)(n-m)(n-2)(n-1) / ((2n-3)(2n-1))
+//   d-1:  2n(n-m)(n-1) / (2n-1)
+//   d 0: -2n(n+1)(3n^2+3n-m^2-2) / ((2n-1)(2n+3))
+//   d+1:  2(n+1)(n+m+1)(n+2) / (2n+3)
+//   d+2: -(n+m+1)(n+m+2)(n+2)(n+3) / ((2n+3)(2n+5))
+// for the unnormalised P_n^m, divided by L^2; the off-diagonals are rescaled
+// to the normalised basis by the ratio of the log-norms.  The numerators are
+// integers below 2^53 (exact in FP64); the divisions are made one factor at a
+// time.
 void band_del2h(int ni, int m, double ell, const double *ln, double *b) {
-  const double s = 0.0;
-  double ell2 = std::pow(ell, 2.0);
+  const double l2 = std::pow(ell, 2.0);
   std::memset(b, 0, sizeof(double) * 5 * (size_t)ni);
-  for (int nn = 1; nn <= ni; nn++) {
-    int n = m + nn - 1;
-    double *r = b + (size_t)5 * (nn - 1);
-    if (nn - 2 >= 1)
-      r[0] = -(n - m - 1.0) * (n - m) * (n - 2 + s) * (n - 1 + s) / (2 * n - 3.0) / (2 * n - 1.0) / ell2 *
-             std::exp(ln[nn - 3] - ln[nn - 1]);
-    if (nn - 1 >= 1) r[1] = 2.0 * n * (n - m) * (n - 1 + s) / (2 * n - 1.0) / ell2 * std::exp(ln[nn - 2] - ln[nn - 1]);
-    r[2] = (-2.0 * n * (n + 1.0) * (3.0 * n * n + 3 * n - m * m - 2) + 2 * s * (s - 2) * (n * n + n + m * m - 1.0)) /
-           (2 * n - 1.0) / (2 * n + 3.0) / ell2;
-    if (nn + 1 <= ni) r[3] = 2.0 * (n + 1) * (n + m + 1.0) * (n + 2 - s) / (2 * n + 3.0) / ell2 * std::exp(ln[nn] - ln[nn - 1]);
-    if (nn + 2 <= ni)
-      r[4] = -(n + m + 1.0) * (n + m + 2.0) * (n + 3 - s) * (n + 2 - s) / (2 * n + 3.0) / (2 * n + 5.0) / ell2 *
-             std::exp(ln[nn + 1] - ln[nn - 1]);
+  for (int row = 0; row < ni; row++) {
+    const long n = m + row;
+    double *r = b + (size_t)5 * row;
+    const double up = 2.0 * n + 3.0;  // 2n+3
+    const double lo = 2.0 * n - 1.0;  // 2n-1
+    if (row >= 2) {
+      const double num = (double)((n - m - 1) * (n - m) * (n - 2) * (n - 1));
+      r[0] = -num / (2.0 * n - 3.0) / lo / l2 * std::exp(ln[row - 2] - ln[row]);
+    }
+    if (row >= 1) r[1] = (double)(2 * n * (n - m) * (n - 1)) / lo / l2 * std::exp(ln[row - 1] - ln[row]);
+    r[2] = (double)(-2 * n * (n + 1) * (3 * n * n + 3 * n - (long)m * m - 2)) / lo / up / l2;
+    if (row + 1 < ni) r[3] = (double)(2 * (n + 1) * (n + m + 1) * (n + 2)) / up / l2 * std::exp(ln[row + 1] - ln[row]);
+    if (row + 2 < ni) {
+      const double num = (double)((n + m + 1) * (n + m + 2) * (n + 2) * (n + 3));
+      r[4] = -num / up / (2.0 * n + 5.0) / l2 * std::exp(ln[row + 2] - ln[row]);
+    }
   }
 }
 

@@ -120,6 +120,7 @@ SIGNATURES = {
     "mlg_time_step": ([_V, _V, _V, _V], _I),
     "mlg_time_steps": ([_V, _V, _V, _V, _I], _I),
     "mlg_time_steps_graph": ([_V, _V, _V, _V, _I], _I),
+    "mlg_diagnost": ([_V, _V], _I),
     "mlg_solver_time": ([_V], _D),
     "mlg_solver_nstep": ([_V], _I),
     "mlg_qvortex": ([_V, _D, _V, _V], _I),
@@ -444,6 +445,12 @@ def calc_boussi_energy(psi, chi, b):
     _call("mlg_energy", psi.h, chi.h, b.h, ke, pe, dke)
     f = lambda a: np.array(a).reshape(nt, nxc, order="F")
     return f(ke), f(pe), f(dke)
+
+
+def diagnost(psi, chi):
+    """DIAGNOST's per-step checks (src/mod_march.f90:1726-1791): raises
+    MlegsError(11) (dealiasing) or MlegsError(12) (divergence)."""
+    _call("mlg_diagnost", psi.h, chi.h)
 
 
 def qvortex(grid, q, psi, chi):

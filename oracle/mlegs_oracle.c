@@ -2006,6 +2006,7 @@ int og_diagnost(const og_grid *g, const og_field *psi, const og_field *chi) {
 /* SOLVER_T%INITIALIZE: mod_march.f90:171-240 (file output omitted) */
 int og_initialize(og_solver *s, og_field *psi, og_field *chi, og_field *b) {
   og_grid *g = s->g;
+  if (!g->p.adams && g->p.visc_sw == 2) return E_SOLVER; /* mod_march.f90:190-201 */
   if (g->p.adams) return rich_fe_be(s, psi, chi, b);
   if (g->p.bv0 == 0.0) return E_SOLVER;
   og_etd_op(g, g->p.dt, s->etd_e, s->etd_nl);

@@ -7,7 +7,7 @@ with the error metric
     err = max |gpu - oracle| / max |oracle|      (per field)
 
 plus |LN_gpu - LN_oracle| / max(1, |LN_oracle|) for log-term coefficients.
-Tolerance: 1e-12 for single routines.  Time steps: max(1e-12, 10 s) per
+Tolerance: 1e-12 for single routines.  Time steps: max(1e-12, 6 s) per
 field, s being the oracle's own change when its Legendre sums run in the
 opposite order (the MFMA kernels sum in another order than the oracle).  Physical-space comparisons cover the
 meaningful region only (r < NR, theta slot < NTH, m < NTCHOPDIM).
@@ -26,7 +26,16 @@ SMALL_ETD = dict(SMALL, adams=0, hyper_cn=1, nup=1e-3, kappap=1e-3, visc_p=4, om
 # NTH = 256 with the production theta truncation: the theta transforms and the
 # fused products take their NTH = 256 kernels here
 THETA256 = dict(CFG1, nr=16, nth=256, nx=8, nrchop=12, ntchop=86, nxchop=3)
+# NTH and NX above 256: the general radix-2 transforms, the separate product
+# kernels and the non-fused NONLIN sequence
+WIDE_THETA = dict(CFG1, nr=16, nth=512, nx=8, nrchop=12, ntchop=100, nxchop=3)
+WIDE_Z = dict(CFG1, nr=16, nth=8, nx=512, nrchop=12, ntchop=3, nxchop=100)
 TOL = 1e-12
+# step bar: the GPU error may reach this multiple of the oracle's own change
+# under reversed Legendre summation.  Both are single samples of a change of
+# summation order; their measured ratio reaches 4.75 (chi, ETD2 with the SW=3
+# filter, step 3) and 4.6 (chi, NR=1024), 3.9 (psi, 384x256x256, step 0).
+SENS_FACTOR = 6
 
 
 def relerr(a, b):
@@ -83,9 +92,10 @@ class Pair:
         return np.asfortranarray(z)
 
 
-@pytest.fixture(scope="module", params=["small", "cfg1", "theta256"])
+@pytest.fixture(scope="module", params=["small", "cfg1", "theta256", "wide_theta", "wide_z"])
 def P(request):
-    p = Pair({"small": SMALL, "cfg1": CFG1, "theta256": THETA256}[request.param])
+    p = Pair({"small": SMALL, "cfg1": CFG1, "theta256": THETA256, "wide_theta": WIDE_THETA,
+              "wide_z": WIDE_Z}[request.param])
     yield p
     p.close()
 
@@ -360,7 +370,7 @@ def oracle_trajectory(cfg, st, nsteps, reverse=False):
 def run_steps(cfg, nsteps, amp=1e-4):
     """GPU vs oracle over a start and `nsteps` steps.
 
-    Bar per field and step: err <= max(1e-12, 10 s), s being the oracle's
+    Bar per field and step: err <= max(1e-12, 6 s), s being the oracle's
     own change when its Legendre sums run in the opposite order; checked on
     all coefficients and again without the gauge entries (n=0, m=0, all k)
     of psi and chi, which the value-at-infinity fix-up forms as a cancelling
@@ -401,9 +411,9 @@ def run_steps(cfg, nsteps, amp=1e-4):
         print("\n".join(lines))
         for step, row in enumerate(report):
             for nm, (e, sens, ex, sens_ex, eln, lsens) in zip(("psi", "chi", "b"), row):
-                assert ex <= max(TOL, 10 * sens_ex), f"step {step} {nm}: err excl. gauge {ex:.3e}\n" + "\n".join(lines)
-                assert e <= max(TOL, 10 * sens), f"step {step} {nm}: err {e:.3e} > bar\n" + "\n".join(lines)
-                assert eln <= max(TOL, 10 * lsens), f"step {step} {nm}: LN\n" + "\n".join(lines)
+                assert ex <= max(TOL, SENS_FACTOR * sens_ex), f"step {step} {nm}: err excl. gauge {ex:.3e}\n" + "\n".join(lines)
+                assert e <= max(TOL, SENS_FACTOR * sens), f"step {step} {nm}: err {e:.3e} > bar\n" + "\n".join(lines)
+                assert eln <= max(TOL, SENS_FACTOR * lsens), f"step {step} {nm}: LN\n" + "\n".join(lines)
         assert gs.nstep == nsteps + 1
         return report
     finally:
@@ -416,6 +426,22 @@ def test_ab2_cn_steps_small():
 
 def test_etd2_steps_small():
     run_steps(SMALL_ETD, 3)
+
+
+@pytest.mark.parametrize("cfg", [dict(SMALL, nth=512, ntchop=40), dict(SMALL_ETD, nx=512, nxchop=40)],
+                         ids=["ab2cn_nth512", "etd2_nx512"])
+def test_steps_wide_transforms(cfg):
+    """NTH or NX = 512: the steps run the general transforms, the separate
+    product kernels and the non-fused NONLIN sequence."""
+    run_steps(cfg, 2)
+
+
+@pytest.mark.parametrize("cfg", [dict(SMALL, visc_sw=3, nup=1e-3, kappap=1e-3, visc_p=4),
+                                 dict(SMALL_ETD, visc_sw=3, hyper_cn=0)], ids=["ab2", "etd2"])
+def test_steps_exponential_hyperviscosity_only(cfg):
+    """VISC%SW = 3 alone: the HYPERV3 filter after each AB2 step
+    (CALC_BOUSSI_HYPERV) or inside the ETD update, no CN viscosity."""
+    run_steps(cfg, 3)
 
 
 @pytest.mark.parametrize("cfg", [dict(SMALL, nrchop=21), dict(SMALL_ETD, nrchop=21)], ids=["ab2cn", "etd2"])
@@ -461,12 +487,12 @@ def test_time_steps_batch_equals_single():
 
 @pytest.mark.timeout(600)
 def test_cfg2_first_steps():
-    """384x256x256 ETD2 workload (noise 1e-3): the Richardson start and one
-    ETD2-AB/CN step, every field within the bar of run_steps."""
+    """384x256x256 ETD2 workload (noise 1e-3): the Richardson start and three
+    ETD2-AB/CN steps, every field within the bar of run_steps."""
     from mlegs_amd.configs import CFG2
 
     O.lib().og_set_threads(min(16, __import__("os").cpu_count() or 1))
-    run_steps(CFG2, 1, amp=CFG2["noise"])
+    run_steps(CFG2, 3, amp=CFG2["noise"])
 
 
 def test_cfg1_energy_100_steps():
@@ -523,5 +549,149 @@ def test_graph_replays_equal_eager_steps(cfg):
             s2.time_stepping(*b, 1, graph=True)
             same()
         assert s1.nstep == s2.nstep and abs(s1.t - s2.t) < 1e-15
+    finally:
+        p.close()
+
+
+@pytest.mark.parametrize("cfg", [SMALL, SMALL_ETD], ids=["ab2", "etd2"])
+def test_graph_survives_transforms_of_other_fields(cfg):
+    """Transforms between replays swap a field's device buffer with a pooled
+    temporary.  The captured step owns its temporaries, so a user field that
+    is transformed and destroyed between replays never takes over (or frees)
+    a buffer the graph writes; a transform round trip of psi itself moves
+    psi's buffer, and the step is then captured again.  Every replay equals
+    the eager steps bit for bit."""
+    p = Pair(cfg)
+    try:
+        st = state(p)
+        a = [p.g.scalar(O.FFF, x, ln) for x, ln in st]
+        b = [p.g.scalar(O.FFF, x, ln) for x, ln in st]
+        s1, s2 = H.Solver(p.g), H.Solver(p.g)
+        s1.initialize(*a)
+        s2.initialize(*b)
+
+        def same():
+            for x, y in zip(a, b):
+                xa, _, xl = x.download()
+                ya, _, yl = y.download()
+                assert np.array_equal(xa, ya) and xl == yl
+
+        for _ in range(3):
+            s1.time_step(*a)
+        s2.time_stepping(*b, 3, graph=True)
+        same()
+        for rnd in range(3):
+            other = p.g.scalar(O.FFF, st[1][0], 0.0)
+            H.tofp(other)  # swaps other's buffer with pooled temporaries
+            H.toff(other)
+            other.close()  # frees the buffer other ended with
+            if rnd == 1:  # psi's own buffer moves: the step must be captured again
+                H.tofp(b[0])
+                H.toff(b[0])
+                H.tofp(a[0])
+                H.toff(a[0])
+            s1.time_step(*a)
+            s2.time_stepping(*b, 2 if rnd else 1, graph=True)
+            if rnd:
+                s1.time_step(*a)
+            same()
+        assert s1.nstep == s2.nstep
+    finally:
+        p.close()
+
+
+# --------------------------------------------------------------- error paths
+def test_diagnost_codes():
+    """DIAGNOST (mod_march.f90:1726-1791) on host-made fields: a coefficient
+    beyond NRCHOPS(1) gives 11, |psi| > 100 in a low mode gives 12, as in
+    the oracle; a clean state gives 0."""
+    p = Pair(SMALL)
+    try:
+        st = state(p)
+        chi = p.g.scalar(O.FFF, st[1][0])
+        och = p.o.field(O.FFF, st[1][0])
+        nn1 = p.cfg["nrchop"] - 1  # NRCHOPS(m = 1): the first row beyond it is nn1
+        cases = [(None, 0), ((nn1, 1, 2, 1e-9), 11), ((0, 0, 0, 0.0), 0), ((2, 1, 1, 150.0), 12)]
+        for change, code in cases:
+            a = st[0][0].copy(order="F")
+            if change:
+                a[change[:3]] = change[3]
+            psi = p.g.scalar(O.FFF, a, st[0][1])
+            ops = p.o.field(O.FFF, a, st[0][1])
+            assert O.lib().og_diagnost(p.o.g, ops.ptr, och.ptr) == code
+            if code:
+                with pytest.raises(H.MlegsError) as ei:
+                    H.diagnost(psi, chi)
+                assert ei.value.code == code
+            else:
+                H.diagnost(psi, chi)
+            psi.close()
+        p.g.synchronize()  # nothing left pending
+    finally:
+        p.close()
+
+
+def test_step_reports_divergence():
+    """A state with |psi| > 100 in a low mode: the step's device-side check
+    returns 12 (SCALAR3_DIVERGENCE) at the synchronisation, as the oracle's
+    step does."""
+    p = Pair(SMALL)
+    try:
+        st = list(state(p))
+        a = st[0][0].copy(order="F")
+        a[2, 1, 1] = 150.0
+        st[0] = (a, st[0][1])
+        gf = [p.g.scalar(O.FFF, x, ln) for x, ln in st]
+        of = [p.o.field(O.FFF, x, ln) for x, ln in st]
+        gs, osv = H.Solver(p.g), O.OSolver(p.o)
+        try:
+            with pytest.raises(H.MlegsError) as ei:
+                gs.initialize(*gf)
+                gs.time_step(*gf)
+            assert ei.value.code == 12
+            osv.initialize(*of)
+            assert osv.step(*of) == 12
+        finally:
+            gs.close()
+            osv.close()
+    finally:
+        p.close()
+
+
+def test_del2_top_mode_abort():
+    """DEL2 (mod_legops.f90:778-790): a non-zero top mode of m = 1 at k = 0
+    is an error (code 3) in the library and in the oracle."""
+    p = Pair(SMALL)
+    try:
+        a = np.zeros(p.o.shape(O.FFF), dtype=complex, order="F")
+        a[p.cfg["nrchop"] - 2, 1, 0] = 1.0  # row NRCHOPS(1) - 1
+        (ga, oa), (gb, ob) = p.both(a, O.FFF), p.empty()
+        H.del2(ga, gb)
+        with pytest.raises(H.MlegsError) as ei:
+            p.g.synchronize()
+        assert ei.value.code == 3
+        assert O.lib().og_del2(p.o.g, oa.ptr, ob.ptr) == 3
+    finally:
+        p.close()
+
+
+def test_etd_rejects_visc_sw2():
+    """SOLVER_T%INITIALIZE (mod_march.f90:190-201): ETD with VISC%SW = 2
+    aborts with ERR_FLAGS%SOLVER (5), in the library and in the oracle."""
+    cfg = dict(SMALL_ETD, visc_sw=2, hyper_cn=0)
+    p = Pair(cfg)
+    try:
+        st = state(p)
+        gf = [p.g.scalar(O.FFF, x, ln) for x, ln in st]
+        of = [p.o.field(O.FFF, x, ln) for x, ln in st]
+        gs = H.Solver(p.g)
+        with pytest.raises(H.MlegsError) as ei:
+            gs.initialize(*gf)
+        assert ei.value.code == 5
+        gs.close()
+        osv = O.OSolver(p.o)
+        with pytest.raises(RuntimeError, match="returned 5"):
+            osv.initialize(*of)
+        osv.close()
     finally:
         p.close()
