@@ -75,6 +75,16 @@ static int imin(int a, int b) { return a < b ? a : b; }
 static int sum_rev = 0;
 void og_set_sum_order(int reverse) { sum_rev = reverse ? 1 : 0; }
 
+/* Test-only planted mistakes (0: none).  Each value breaks one sign or index
+ * of the restatement, so a test can show that it detects that mistake:
+ *  1 buoyancy force +b z (FORCE)      2 Coriolis theta component +2 Omega u_r
+ *  3 N^2 u_z term sign (FORCE)        4 sign of KE's chi term
+ *  5 sign of DKE                      6 sign of PC2VOR's i m del^2 chi term
+ *  7 sign of GRAD's i m b             8 ETD k = 0 row replacement left out
+ *  9 CHOPDO keeps one row too many    10 sign of GRAD's i k b */
+static int mutation = 0;
+void og_set_mutation(int k) { mutation = k; }
+
 void og_set_threads(int n) {
 #ifdef _OPENMP
   if (n > 0) omp_set_num_threads(n);
@@ -534,7 +544,7 @@ void og_chopdo(const og_grid *g, og_field *a) {
 #pragma omp parallel for
     for (int kk = 0; kk < g->nxchopdim; kk++) {
       for (int mm = 0; mm < lim; mm++)
-        for (int n = g->nrchops[mm]; n < g->nrchopdim; n++) a->e[FIDX(g, n, mm, kk)] = 0;
+        for (int n = g->nrchops[mm] + (mutation == 9); n < g->nrchopdim; n++) a->e[FIDX(g, n, mm, kk)] = 0;
       for (int mm = imax(0, g->ntchop); mm < g->ntchopdim; mm++)
         for (int n = 0; n < g->nrchopdim; n++) a->e[FIDX(g, n, mm, kk)] = 0;
     }
@@ -1121,8 +1131,8 @@ int og_grad(const og_grid *g, const og_field *a, og_field *rr, og_field *rp, og_
       double mv = g->m[mm], kv = AK(g, mm, kk);
       for (int n = 0; n < g->nrchops[mm]; n++) {
         cplx v = a->e[FIDX(g, n, mm, kk)];
-        rp->e[FIDX(g, n, mm, kk)] = (I * mv) * v;
-        z->e[FIDX(g, n, mm, kk)] = (I * kv) * v;
+        rp->e[FIDX(g, n, mm, kk)] = (mutation == 7 ? -I * mv : I * mv) * v;
+        z->e[FIDX(g, n, mm, kk)] = (mutation == 10 ? -I * kv : I * kv) * v;
       }
     }
   return E_OK;
@@ -1165,7 +1175,7 @@ int og_pc2vor(const og_grid *g, const og_field *psi, const og_field *chi, og_fie
       double mv = g->m[mm], kv = AK(g, mm, kk);
       for (int n = 0; n < g->nrchops[mm]; n++) {
         size_t id = FIDX(g, n, mm, kk);
-        ror->e[id] = -(I * mv) * oz->e[id] + (I * kv) * ror->e[id];
+        ror->e[id] = (mutation == 6 ? (I * mv) : -(I * mv)) * oz->e[id] + (I * kv) * ror->e[id];
         rop->e[id] = rop->e[id] - (kv * mv) * psi->e[id];
       }
     }
@@ -1409,9 +1419,9 @@ int og_force(og_grid *g, const og_field *psi, const og_field *chi, const og_fiel
 #pragma omp parallel for
     for (long i = 0; i < n; i++) {
       ror->e[i] = ror->e[i] + o2 * rup->e[i];
-      rop->e[i] = rop->e[i] - o2 * rur->e[i];
-      oz->e[i] = oz->e[i] - w->e[i];
-      bn->e[i] = bn->e[i] + n2 * uz->e[i];
+      rop->e[i] = mutation == 2 ? rop->e[i] + o2 * rur->e[i] : rop->e[i] - o2 * rur->e[i];
+      oz->e[i] = mutation == 1 ? oz->e[i] + w->e[i] : oz->e[i] - w->e[i];
+      bn->e[i] = mutation == 3 ? bn->e[i] - n2 * uz->e[i] : bn->e[i] + n2 * uz->e[i];
     }
   }
   realloc_space(g, w, OG_FFF);
@@ -1544,11 +1554,13 @@ int og_energy(og_grid *g, const og_field *psi, const og_field *chi, const og_fie
   w1->ln = 0.0; w2->ln = 0.0;
   og_rtran(g, w1, 1); og_rtran(g, w2, 1);
   prodct_halfk(g, w1, w2, 0, tmp);
-  for (int i = 0; i < nt * nxc; i++) ke[i] = (ke[i] + tmp[i]) / 2.0;
+  for (int i = 0; i < nt * nxc; i++) ke[i] = (mutation == 4 ? ke[i] - tmp[i] : ke[i] + tmp[i]) / 2.0;
   og_copy(g, w3, b);
   w3->ln = 0.0;
   og_rtran(g, w3, 1);
   prodct_halfk(g, w3, w2, 0, dke);
+  if (mutation == 5)
+    for (int i = 0; i < nt * nxc; i++) dke[i] = -dke[i];
   prodct_halfk(g, w3, w3, 1, tmp);
   for (int i = 0; i < nt * nxc; i++) pe[i] = tmp[i] / 2.0 / pow(g->p.bv0, 2.0);
   og_chopset(g, -3);
@@ -1816,7 +1828,7 @@ static void etd_sub(const og_grid *g, const double *e16, const double *nl16, dou
     memcpy(er, e16, sizeof(er));
     memcpy(ar, nl16, sizeof(ar));
     double notk0 = 1.0;
-    if (kk == 0) {
+    if (kk == 0 && mutation != 8) {
       notk0 = 0.0;
       for (int j = 0; j < 4; j++) { er[0 + 4 * j] = 0; er[1 + 4 * j] = 0; ar[0 + 4 * j] = 0; ar[1 + 4 * j] = 0; }
       er[0] = 1.0; er[1 + 4] = 1.0; ar[0] = dt; ar[1 + 4] = dt;

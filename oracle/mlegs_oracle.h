@@ -117,6 +117,7 @@ int og_nstep(const og_solver *s);
 int og_qvortex(og_grid *g, double q, og_field *psi, og_field *chi);
 void og_set_threads(int n);
 void og_set_sum_order(int reverse);
+void og_set_mutation(int k);   /* test-only planted mistakes, see mlegs_oracle.c */
 
 #ifdef __cplusplus
 }

@@ -92,6 +92,7 @@ def lib():
             "og_qvortex": ([g, C.c_double, P, P], C.c_int),
             "og_set_threads": ([C.c_int], None),
             "og_set_sum_order": ([C.c_int], None),
+            "og_set_mutation": ([C.c_int], None),
         }
         for name, (args, res) in sigs.items():
             fn = getattr(L, name)
