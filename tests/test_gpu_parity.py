@@ -32,9 +32,13 @@ WIDE_THETA = dict(CFG1, nr=16, nth=512, nx=8, nrchop=12, ntchop=100, nxchop=3)
 WIDE_Z = dict(CFG1, nr=16, nth=8, nx=512, nrchop=12, ntchop=3, nxchop=100)
 TOL = 1e-12
 # step bar: the GPU error may reach this multiple of the oracle's own change
-# under reversed Legendre summation.  Both are single samples of a change of
-# summation order; their measured ratio reaches 4.75 (chi, ETD2 with the SW=3
-# filter, step 3) and 4.6 (chi, NR=1024), 3.9 (psi, 384x256x256, step 0).
+# under reversed Legendre summation, taken as the largest change over the
+# steps so far (round-off made at one step is carried into the next ones,
+# while a single reversed-order sample at a later step can cancel to a small
+# value: chi's gauge entry under the SW=3 filter alone, 3.1e-12 at step 2
+# after 1.5e-11 at step 0).  Measured ratios against that running maximum
+# reach 4.75 (chi without gauge entries, ETD2 with the SW=3 filter, step 3),
+# 4.6 (chi, NR=1024) and 3.9 (psi, 384x256x256, step 0).
 SENS_FACTOR = 6
 
 
@@ -370,8 +374,9 @@ def oracle_trajectory(cfg, st, nsteps, reverse=False):
 def run_steps(cfg, nsteps, amp=1e-4):
     """GPU vs oracle over a start and `nsteps` steps.
 
-    Bar per field and step: err <= max(1e-12, 6 s), s being the oracle's
-    own change when its Legendre sums run in the opposite order; checked on
+    Bar per field and step: err <= max(1e-12, 6 s), s being the largest
+    change of the oracle, over this and the earlier steps, when its Legendre
+    sums run in the opposite order; checked on
     all coefficients and again without the gauge entries (n=0, m=0, all k)
     of psi and chi, which the value-at-infinity fix-up forms as a cancelling
     sum over the whole m=0 column.  s stays below 1e-12 for psi and b on the
@@ -409,8 +414,12 @@ def run_steps(cfg, nsteps, amp=1e-4):
                 row.append((e, sens, ex, sens_ex, eln, lsens))
             report.append(row)
         print("\n".join(lines))
+        smax = [[0.0, 0.0, 0.0] for _ in range(3)]
         for step, row in enumerate(report):
-            for nm, (e, sens, ex, sens_ex, eln, lsens) in zip(("psi", "chi", "b"), row):
+            for f, (nm, (e, sens, ex, sens_ex, eln, lsens)) in enumerate(zip(("psi", "chi", "b"), row)):
+                run = smax[f]
+                run[0], run[1], run[2] = max(run[0], sens), max(run[1], sens_ex), max(run[2], lsens)
+                sens, sens_ex, lsens = run
                 assert ex <= max(TOL, SENS_FACTOR * sens_ex), f"step {step} {nm}: err excl. gauge {ex:.3e}\n" + "\n".join(lines)
                 assert e <= max(TOL, SENS_FACTOR * sens), f"step {step} {nm}: err {e:.3e} > bar\n" + "\n".join(lines)
                 assert eln <= max(TOL, SENS_FACTOR * lsens), f"step {step} {nm}: LN\n" + "\n".join(lines)
