@@ -62,7 +62,9 @@ enum {
   MLG_E_ARG = 100,      /* bad argument / unsupported configuration */
   MLG_E_HIP = 101,      /* HIP runtime error                        */
   MLG_E_NOMEM = 102,    /* device allocation failed                 */
-  MLG_E_SPACE = 103     /* field in the wrong space for the call    */
+  MLG_E_SPACE = 103,    /* field in the wrong space for the call    */
+  MLG_E_SIZE = 104      /* input larger than the fixed scratch of an
+                           allocation-free entry point              */
 };
 
 /* ------------------------------------------------------------------ */
@@ -149,6 +151,13 @@ int mlg_vprodsub(mlg_field *rur, mlg_field *rup, mlg_field *uz,
 int mlg_vdotpsub(const mlg_field *rar, const mlg_field *rap, const mlg_field *az,
                  const mlg_field *rbr, const mlg_field *rbp, const mlg_field *bz, mlg_field *out);
 int mlg_vel2pln(mlg_field *rup, mlg_field *psi, int has_ln, double ln); /* VEL2PLN :2277-2339 */
+/* mlg_vel2pln without allocation.  No counterpart in the reference: it is
+ * for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops).  Same arguments and results as mlg_vel2pln
+ * (bitwise); its scratch (the NRCHOP projections) is a fixed array of 1024
+ * doubles per GPU, so NRCHOP > 1024 returns MLG_E_SIZE.  Every call on one
+ * GPU shares that array: calls that overlap on one GPU race. */
+int mlg_vel2pln_noalloc(mlg_field *rup, mlg_field *psi, int has_ln, double ln);
 /* PROJECT (:2397-2545): (r u_r, r u_theta, u_z) in PPP/PFF/FFF -> (psi, del^2 chi). */
 int mlg_project(mlg_field *rur, mlg_field *rup, mlg_field *uz,
                 mlg_field *psi, mlg_field *chi, int has_ln, double ln);

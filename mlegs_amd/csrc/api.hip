@@ -167,6 +167,7 @@ extern "C" const char *mlg_error_string(int code) {
     case MLG_E_HIP: return "HIP runtime error";
     case MLG_E_NOMEM: return "device allocation failed";
     case MLG_E_SPACE: return "field is in the wrong space for this call";
+    case MLG_E_SIZE: return "size exceeds the fixed scratch of an allocation-free entry point";
     default: return "unknown";
   }
 }
@@ -191,6 +192,12 @@ extern "C" int mlg_init(const mlg_params *p, mlg_ctx **out) {
   }
   Grid &G = c->g;
   (void)hipGetDevice(&c->device);
+  // the allocation-free entry points' fixed scratch: its address on this device
+  rc = vel2pln_scratch_setup(c->device);
+  if (rc) {
+    delete c;
+    return rc;
+  }
   c->n_fff = (long)G.nrchopdim * G.ntchopdim * G.nxchopdim;
   c->n_pff = (long)G.ndimr * G.ntchopdim * G.nxchopdim;
   c->n_ppp = (long)G.ndimr * G.ndimth * G.ndimx;
@@ -734,6 +741,14 @@ extern "C" int mlg_vel2pln(mlg_field *rup, mlg_field *psi, int has_ln, double ln
   if (rup->space != MLG_PFF) return 3;
   psi->space = MLG_FFF;
   return launch_vel2pln(rup->ctx, rup->e, psi->ln, has_ln ? 1 : 0, ln);
+}
+
+extern "C" int mlg_vel2pln_noalloc(mlg_field *rup, mlg_field *psi, int has_ln, double ln) {
+  if (!rup || !psi) return MLG_E_ARG;
+  if (rup->space != MLG_PFF) return 3;
+  if (rup->ctx->g.nrchop > VEL2PLN_SCRATCH_MAX) return MLG_E_SIZE;
+  psi->space = MLG_FFF;
+  return launch_vel2pln_noalloc(rup->ctx, rup->e, psi->ln, has_ln ? 1 : 0, ln);
 }
 
 // PROJECT (src/mod_legops.f90:2397-2648).  ln_mode 0: no LN argument; 1: LN

@@ -9,6 +9,8 @@
 // so the nine velocity/vorticity/gradient fields are read once and the four
 // right-hand sides written once.  The free-stream/strain additions and the
 // CFL monitor of VPRODSUB are zero/off on this path.
+#include <mutex>
+
 #include "mlegs_internal.h"
 
 namespace mlg {
@@ -160,15 +162,54 @@ __global__ void __launch_bounds__(256) k_vel2pln_apply(KGrid g, const double *__
   if (threadIdx.x == 0 && ln_mode != 2) psi_ln[0] = ln_mode == 1 ? ln : pln;
 }
 
-int launch_vel2pln(mlg_ctx *c, double2 *rup, double *psi_ln, int ln_mode, double ln) {
+static int vel2pln_passes(mlg_ctx *c, double2 *rup, double *psi_ln, int ln_mode, double ln, double *proj) {
   KGrid g = kgrid(c);
-  if (!rup || !psi_ln) return MLG_E_ARG;
   if (g.nrchop > 0)
     hipLaunchKernelGGL(k_vel2pln_proj, dim3((g.nrchop + 3) / 4), dim3(256), 0, c->stream, g, c->d.pf, c->d.w, c->d.lognorm,
-                       rup, c->d.vproj);
-  hipLaunchKernelGGL(k_vel2pln_apply, dim3(1), dim3(256), 0, c->stream, g, c->d.pf, c->d.vproj, rup, psi_ln, ln_mode, ln);
+                       rup, proj);
+  hipLaunchKernelGGL(k_vel2pln_apply, dim3(1), dim3(256), 0, c->stream, g, c->d.pf, proj, rup, psi_ln, ln_mode, ln);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
+}
+
+int launch_vel2pln(mlg_ctx *c, double2 *rup, double *psi_ln, int ln_mode, double ln) {
+  if (!rup || !psi_ln) return MLG_E_ARG;
+  return vel2pln_passes(c, rup, psi_ln, ln_mode, ln, c->d.vproj);
+}
+
+// VEL2PLN without allocation.  Its one scratch buffer, the NRCHOP
+// projections of pass 1, is this fixed device array instead of the context's
+// buffer.  Every GPU holds its own copy until the process ends; the copy's
+// address is looked up once per device, when the first context is created on
+// it, and only that address is handed to kernels.  Pass 1 writes every entry
+// that pass 2 reads, so nothing needs clearing between calls; the array keeps
+// the last call's projections.  Calls that overlap on one GPU (two streams,
+// two threads) share the array and race.
+static __device__ __align__(256) double scratch[VEL2PLN_SCRATCH_MAX];
+static double *scratch_dev[MLG_MAX_DEVICES];
+static std::mutex scratch_mu;
+
+int vel2pln_scratch_setup(int device) {
+  if (device < 0 || device >= MLG_MAX_DEVICES) return MLG_E_ARG;
+  std::lock_guard<std::mutex> lock(scratch_mu);
+  if (scratch_dev[device]) return 0;
+  void *ptr = nullptr;
+  // the caller has made `device` current
+  if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return MLG_E_HIP;
+  scratch_dev[device] = (double *)ptr;
+  return 0;
+}
+
+int launch_vel2pln_noalloc(mlg_ctx *c, double2 *rup, double *psi_ln, int ln_mode, double ln) {
+  if (!rup || !psi_ln) return MLG_E_ARG;
+  if (c->g.nrchop > VEL2PLN_SCRATCH_MAX) return MLG_E_SIZE;
+  double *proj = nullptr;
+  {
+    std::lock_guard<std::mutex> lock(scratch_mu);
+    if (c->device >= 0 && c->device < MLG_MAX_DEVICES) proj = scratch_dev[c->device];
+  }
+  if (!proj) return MLG_E_HIP;
+  return vel2pln_passes(c, rup, psi_ln, ln_mode, ln, proj);
 }
 
 }  // namespace mlg

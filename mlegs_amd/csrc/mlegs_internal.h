@@ -208,6 +208,13 @@ int launch_force_products(mlg_ctx *c, const double2 *rur, const double2 *rup, co
                           double2 *bn, double two_omega, double n2, int linear);
 // ln_mode: 0 psi LN = the computed value, 1 psi LN = ln, 2 psi LN unchanged
 int launch_vel2pln(mlg_ctx *c, double2 *rup, double *psi_ln, int ln_mode, double ln);
+// the same with the projections in a fixed per-device array of at most
+// VEL2PLN_SCRATCH_MAX entries (no allocation); vel2pln_scratch_setup looks up
+// the array's address on the current device, once per device
+constexpr int VEL2PLN_SCRATCH_MAX = 1024;
+constexpr int MLG_MAX_DEVICES = 64;
+int vel2pln_scratch_setup(int device);
+int launch_vel2pln_noalloc(mlg_ctx *c, double2 *rup, double *psi_ln, int ln_mode, double ln);
 // ---- fused theta transforms + products, PFP in and out (k_fused.hip)
 bool fused_products_supported(const mlg_ctx *c);
 // in: rur rup uz ror rop oz rbr rbp bz [W]; out: r N_r, r N_theta, N_z, N_b;

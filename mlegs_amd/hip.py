@@ -109,6 +109,7 @@ SIGNATURES = {
     "mlg_vprodsub": ([_V, _V, _V, _V, _V, _V], _I),
     "mlg_vdotpsub": ([_V, _V, _V, _V, _V, _V, _V], _I),
     "mlg_vel2pln": ([_V, _V, _I, _D], _I),
+    "mlg_vel2pln_noalloc": ([_V, _V, _I, _D], _I),
     "mlg_project": ([_V, _V, _V, _V, _V, _I, _D], _I),
     "mlg_force": ([_V, _V, _V, _V, _V, _V], _I),
     "mlg_nonlin": ([_V, _V, _V, _V, _V, _V, _V, _V], _I),
@@ -422,6 +423,11 @@ def vdotpsub(rar, rap, az, rbr, rbp, bz, out):
 
 def vel2pln(rup, psi, ln=None):
     _call("mlg_vel2pln", rup.h, psi.h, 0 if ln is None else 1, 0.0 if ln is None else float(ln))
+
+
+def vel2pln_noalloc(rup, psi, ln=None):
+    """VEL2PLN with its scratch in a fixed per-GPU array (no device allocation; NRCHOP <= 1024)."""
+    _call("mlg_vel2pln_noalloc", rup.h, psi.h, 0 if ln is None else 1, 0.0 if ln is None else float(ln))
 
 
 def project(rur, rup, uz, psi, chi, ln=None):
