@@ -34,12 +34,15 @@ TRAFFIC_FILE = os.path.join(ROOT, "profiles", "step_kernels_cfg2_traffic.json")
 def parse():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--steps", type=int, default=20,
+                    help="timed steps; with the warm-up at most 60 (the workload diverges at step 68)")
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--cpu-steps", type=int, default=2, help="steps in the CPU baseline sample (0: skip)")
     ap.add_argument("--no-timer", action="store_true", help="time without per-kernel events")
     ap.add_argument("--eager", action="store_true",
                     help="time eager launches instead of replays of the captured step graph")
+    ap.add_argument("--no-split", action="store_true",
+                    help="skip the split hi/lo FP32 Legendre measurement (timing and error against FP64)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 20 copies of a 384x257x256 complex field (407 MB each way), "
                          "to scale HBM byte counters")
@@ -72,6 +75,80 @@ def initial_state(H, g, cfg):
     chi.upload(seeded_noise(cfg, 2), H.FFF, 0.0)
     b.upload(seeded_noise(cfg, 3), H.FFF, 0.0)
     return psi, chi, b
+
+
+def field_errors(H, xs, ys):
+    """max|x - y| / max|y| per field (psi, chi, b) and |LN_x - LN_y| / max(1, |LN_y|)."""
+    import numpy as np
+
+    out = {}
+    for nm, x, y in zip(("psi", "chi", "b"), xs, ys):
+        xa, _, xl = x.download()
+        ya, _, yl = y.download()
+        out[nm] = float(np.max(np.abs(xa - ya)) / np.max(np.abs(ya)))
+        out[nm + "_ln"] = abs(xl - yl) / max(1.0, abs(yl))
+    return {k: float(f"{v:.3e}") for k, v in out.items()}
+
+
+def split_measurement(H, cfg, steps, warmup, ms_fp64):
+    """The same workload with the Legendre products in split hi/lo FP32: its
+    steps/s (graph replay, K steps after W warm-up steps), its per-kernel
+    Legendre times, and its difference from the FP64 GPU step after the
+    start + 1 and + 60 steps (per field, and in total kinetic and potential
+    energy).  60, not 100: the workload's per-step DIAGNOST check stops it
+    with code 12 (divergence) at step 68, in the FP64 oracle as here.  Two
+    contexts, one per precision, march from the same state."""
+    import torch
+
+    gd, gs = H.Grid(cfg), H.Grid(cfg)
+    gs.legendre_precision = "split32"
+    fd, fs = initial_state(H, gd, cfg), initial_state(H, gs, cfg)
+    sd, ss = H.Solver(gd), H.Solver(gs)
+    sd.initialize(*fd)
+    ss.initialize(*fs)
+    sd.time_stepping(*fd, 1, graph=False)
+    ss.time_stepping(*fs, 1, graph=False)
+    err1 = field_errors(H, fs, fd)
+    sd.time_stepping(*fd, 59, graph=True)
+    ss.time_stepping(*fs, 59, graph=True)
+    err100 = field_errors(H, fs, fd)
+    ked, ped, _ = H.calc_boussi_energy(*fd)
+    kes, pes, _ = H.calc_boussi_energy(*fs)
+    err100["ke_total"] = float(f"{abs(kes.sum() - ked.sum()) / abs(ked.sum()):.3e}")
+    err100["pe_total"] = float(f"{abs(pes.sum() - ped.sum()) / abs(ped.sum()):.3e}")
+    sd.close()
+    gd.close()
+    # timing: the split context only, from a fresh start (the timed steps
+    # must end before the workload's divergence at step 68)
+    ss.close()
+    fs = initial_state(H, gs, cfg)
+    ss = H.Solver(gs)
+    ss.initialize(*fs)
+    ss.time_stepping(*fs, max(warmup, 2), graph=True)
+    torch.cuda.synchronize()
+    gs.synchronize()
+    t0 = time.perf_counter()
+    ss.time_stepping(*fs, steps, graph=True)
+    torch.cuda.synchronize()
+    gs.synchronize()
+    ms = (time.perf_counter() - t0) / steps * 1e3
+    gs.timer(True)
+    ss.time_stepping(*fs, steps)
+    gs.synchronize()
+    gs.timer(False)
+    stages = {}
+    for k in ("rtran_bwd", "rtran_fwd", "project"):
+        t, n, work, unit = gs.timer_read(k)
+        if n:
+            stages[k] = dict(ms_per_step=round(t / steps, 4), avg_us=round(t / n * 1e3, 2),
+                             fp64_equiv_tflops=round(work / n / (t / n * 1e-3) / 1e12, 2))
+    ss.close()
+    gs.close()
+    return dict(ms_per_step=round(ms, 3), steps_per_s=round(1e3 / ms, 4), vs_fp64=round(ms_fp64 / ms, 4),
+                err_vs_fp64_gpu_after_start_plus_1=err1, err_vs_fp64_gpu_after_start_plus_60=err100,
+                legendre_stages=stages,
+                note="split/FP64 steps/s ratio of the same graph-replayed step; errors are max|split-fp64|/max|fp64| "
+                     "per field (the FP64 path itself is held to the oracle by the parity tests)")
 
 
 def cpu_baseline(cfg, nsteps):
@@ -121,6 +198,10 @@ def main():
     assert torch.cuda.is_available(), "bench.py needs a HIP device"
     torch.cuda.set_device(0)
     cfg = dict(CFG2)
+    if a.warmup + a.steps > 60:
+        print("bench.py: --warmup + --steps must be at most 60: the workload's per-step DIAGNOST check stops it "
+              "with code 12 (divergence) at step 68, in the FP64 oracle too", file=sys.stderr)
+        sys.exit(2)
     if a.calibrate_copy:
         calibrate_copy(H, cfg)
         return
@@ -147,6 +228,13 @@ def main():
         # per-kernel HIP events need eager launches: a second timed region of
         # the same K steps, launched one kernel at a time
         print("bench: eager timed region for the per-kernel events", file=sys.stderr, flush=True)
+        # from a fresh start, so that no trajectory reaches the divergence at step 68
+        s.close()
+        psi, chi, b = initial_state(H, g, cfg)
+        s = H.Solver(g)
+        s.initialize(psi, chi, b)
+        s.time_stepping(psi, chi, b, a.warmup)
+        g.synchronize()
         g.timer(True)
         t1 = time.perf_counter()
         s.time_stepping(psi, chi, b, a.steps)
@@ -180,6 +268,12 @@ def main():
                     achieved=round(rate / (1e12 if flop else 1e9), 3), peak=(78.6 if flop else 8000.0),
                     unit="TFLOP/s" if flop else "GB/s", frac=round(rate / peak, 4), traffic=traffic,
                     work_per_launch=st["work_per_launch"], avg_launch_us=st["avg_us"])
+    s.close()
+    g.close()
+    split = None
+    if not a.no_split:
+        print("bench: split hi/lo FP32 Legendre products", file=sys.stderr, flush=True)
+        split = split_measurement(H, cfg, a.steps, a.warmup, ms_step)
     cpu = None
     if a.cpu_steps > 0:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
@@ -206,9 +300,8 @@ def main():
         "eager_ms_per_step": round(eager_ms, 3),
         "cpu_baseline": cpu,
         "stages": stages,
+        "split_precision": split,
     }
-    s.close()
-    g.close()
     print(json.dumps(out), flush=True)
 
 

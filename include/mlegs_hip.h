@@ -207,6 +207,23 @@ double mlg_solver_time(const mlg_solver *s);
 int mlg_solver_nstep(const mlg_solver *s);
 
 /* ------------------------------------------------------------------ */
+/* Precision of the Legendre products (RTRAN, PROJECT).  No counterpart in
+ * the reference, whose products are FP64 DGEMMs (src/mod_eig.f90:502-656
+ * behind src/mod_fft.f90:1208-1307 and src/mod_legops.f90:2496-2648).
+ * MLG_LEG_SPLIT32: table and data operands split into hi/lo FP32 on FP32
+ * MFMA, exact hi x hi sums within 16-deep K chunks, FP32 cross terms, the
+ * chunks combined in FP64 (relative error ~1e-10 of the operands' scales).
+ * The lowest azimuthal modes keep FP64 products (m < 5 unless set with
+ * mlg_set_split_fp64_modes, at least 2): their projection tables divide by
+ * (1 - x^2) and their products cancel in psi and chi.  Band solves, log
+ * terms, FFTs and diagnostics stay FP64.  The first switch to SPLIT32 builds
+ * the split tables (synchronous allocation). */
+enum { MLG_LEG_FP64 = 0, MLG_LEG_SPLIT32 = 1 };
+int mlg_set_legendre_precision(mlg_ctx *ctx, int mode);
+int mlg_legendre_precision(const mlg_ctx *ctx);
+int mlg_set_split_fp64_modes(mlg_ctx *ctx, int nmodes);
+
+/* ------------------------------------------------------------------ */
 /* Initial conditions: QVORTEX (src/mod_layout.f90:449-610) for one centred
  * Gaussian vortex of strength q (omega_z = 2 q exp(-r^2)), chi = 0. */
 int mlg_qvortex(mlg_ctx *ctx, double q, mlg_field *psi, mlg_field *chi);

@@ -5,7 +5,8 @@ dictionaries whose keys match the fields of ``mlg_params``:
 
 * ``CFG1`` -- 64x32x32 AB2-CN run (the reference's own CPU-sized case).
 * ``CFG2`` -- 384x256x256 ETD2 production case (the headline workload).
-* ``CFG3`` -- CFG2 with split hi/lo FP32 Legendre products.
+* ``CFG3`` -- CFG2 with split hi/lo FP32 Legendre products (``split=1``:
+  ``mlegs_amd.hip.Grid`` selects ``legendre_precision = "split32"``).
 * ``CFG5`` -- 128x64x64 AB2-CN case used for the adjoint/forward cost.
 
 Mapping of the production case's viscosity: the reference offers CN
@@ -28,6 +29,9 @@ CFG2 = dict(nr=384, nth=256, nx=256, nrchop=256, ntchop=86, nxchop=86, minc=1, m
             ell=2.0, zlen=4 * math.pi, nu=1e-4, kappa=1e-4, nup=1e-14, kappap=1e-14,
             visc_sw=1, visc_p=4, omega=1.0, bv0=2.0, adams=0, hyper_cn=1, dt=5e-4,
             nsteps=100, noise=1e-3)
+# CFG2's state does not last its 100 steps: the per-step DIAGNOST check
+# stops it with code 12 (divergence) at step 68, in the FP64 oracle as on
+# the GPU (tests/golden/cfg2_energy.json); measurements stop at step 60.
 
 CFG3 = dict(CFG2, split=1)
 

@@ -38,6 +38,8 @@ KGrid kgrid(const mlg_ctx *c) {
   k.npf = G.npf;
   k.nband = G.nband;
   k.minc = G.p.minc;
+  k.mlo = 0;
+  k.mcnt = G.ntchopdim;
   k.ell2 = G.ell2;
   k.norm0 = G.norm[0];
   k.norm1 = G.norm[1];
@@ -284,7 +286,8 @@ extern "C" int mlg_finalize(mlg_ctx *c) {
   c->timer.spare.clear();
   DevTables &d = c->d;
   void *ptrs[] = {d.pf, d.vdt, d.w, d.r, d.x, d.lnx, d.at1, d.ak, d.norm, d.lognorm, d.bxm, d.bxxdx, d.bdel2h,
-                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.red, d.vproj, d.cvdt, d.err};
+                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.red, d.vproj, d.cvdt, d.err,
+                  d.pf_sb, d.pf_eb, d.pf_sf, d.pf_ef, d.vdt_s, d.vdt_e, d.cvdt_s, d.cvdt_e};
   for (auto &f : c->fcache) (void)hipFree(f.f);
   c->fcache.clear();
   for (auto &v : c->m0vecs) (void)hipFree(v.second);
@@ -293,6 +296,21 @@ extern "C" int mlg_finalize(mlg_ctx *c) {
     if (p) (void)hipFree(p);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
   delete c;
+  return 0;
+}
+
+extern "C" int mlg_set_legendre_precision(mlg_ctx *c, int mode) {
+  if (!c || (mode != MLG_LEG_FP64 && mode != MLG_LEG_SPLIT32)) return MLG_E_ARG;
+  if (mode == MLG_LEG_SPLIT32 && !c->d.pf_sb) RET(build_split_tables(c));
+  c->leg_split = mode == MLG_LEG_SPLIT32;
+  return 0;
+}
+
+extern "C" int mlg_legendre_precision(const mlg_ctx *c) { return c ? (c->leg_split ? MLG_LEG_SPLIT32 : MLG_LEG_FP64) : -1; }
+
+extern "C" int mlg_set_split_fp64_modes(mlg_ctx *c, int nmodes) {
+  if (!c || nmodes < 2) return MLG_E_ARG;
+  c->split_fp64_modes = nmodes;
   return 0;
 }
 
@@ -1653,8 +1671,9 @@ extern "C" int mlg_time_steps_graph(mlg_solver *s, mlg_field *psi, mlg_field *ch
   field_addrs(psi, chi, b, addr);
   bool same_buffers = true;
   for (int i = 0; i < 6; i++) same_buffers = same_buffers && addr[i] == s->gaddr[i];
+  const int split_key = c->leg_split ? c->split_fp64_modes : 0;
   if (!s->gexec || s->gpsi != psi || s->gchi != chi || s->gb != b || !same_buffers || s->gnrchop != c->g.nrchop ||
-      s->gstream != c->stream) {
+      s->gstream != c->stream || s->gsplit != split_key) {
     drop_graph(s);
     RET(one_step(s, psi, chi, b));
     done = 1;
@@ -1692,6 +1711,7 @@ extern "C" int mlg_time_steps_graph(mlg_solver *s, mlg_field *psi, mlg_field *ch
       s->gfactors.push_back(f.f);
     }
     s->gnrchop = c->g.nrchop;
+    s->gsplit = split_key;
     s->gstream = c->stream;
   }
   for (int i = done; i < nsteps; i++) {

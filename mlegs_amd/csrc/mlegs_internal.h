@@ -46,6 +46,18 @@ struct DevTables {
   double *red = nullptr;      // reduction scratch
   double *vproj = nullptr;    // VEL2PLN projections [NRCHOPDIM]
   double *cvdt = nullptr;     // PROJECT tables composed with RTRAN (built on first use)
+  // Split hi/lo FP32 forms of the Legendre tables (built when the split
+  // precision is first selected): every row of a product's table operand is
+  // scaled by 2^-E, E the row's exponent (|entries| < 2^E), and split into
+  // hi (a multiple of 2^-10) and lo (the FP32 rounding of the rest).
+  float2 *pf_sb = nullptr;    // PF, rows (m, node i): RTRAN backward
+  int *pf_eb = nullptr;
+  float2 *pf_sf = nullptr;    // PF, rows (m, n): RTRAN forward
+  int *pf_ef = nullptr;
+  float2 *vdt_s = nullptr;    // V, D, T, rows (table, m, n): PROJECT
+  int *vdt_e = nullptr;
+  float2 *cvdt_s = nullptr;   // composed V, D, T, rows (table, m, parity, h)
+  int *cvdt_e = nullptr;
   int *err = nullptr;         // sticky device error code
 };
 
@@ -97,6 +109,8 @@ struct mlg_ctx {
   std::unordered_set<mlg_field *> owned;  // fields created through the C ABI
   std::vector<mlg::FactorSet> fcache;     // band-solve factors
   int device = 0;
+  int leg_split = 0;  // Legendre products in split hi/lo FP32 (mlg_set_legendre_precision)
+  int split_fp64_modes = 5;  // split mode: modes m < this keep FP64 products
   // PROJECT's m = 0, k = 0 column vectors, one buffer per truncation (built on first use)
   std::vector<std::pair<int, double *>> m0vecs;
 };
@@ -121,6 +135,7 @@ struct mlg_solver {
   std::vector<mlg_field *> gpool;
   std::vector<double *> gfactors;
   int gnrchop = -1;
+  int gsplit = -1;
   hipStream_t gstream = nullptr;
 };
 
@@ -129,6 +144,7 @@ struct KGrid {
   int nr, nrh, nth, nx, ntchop, nxchop;
   int ndimr, ndimth, ndimx, nrchopdim, ntchopdim, nxchopdim, nxchoph;
   int nrchop, npf, nband, minc;
+  int mlo, mcnt;  // Legendre product launches: the modes mlo .. mlo+mcnt-1 (all by default)
   double ell2, norm0, norm1, norm2, at1_0;
 };
 
@@ -194,6 +210,9 @@ int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const dou
 // infinity fix-up of psi is applied by the same launch.
 int launch_project_fff(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, const double *const ln_u[3],
                        double2 *psi, double *psi_ln, double2 *chi, int ln_mode, double ln);
+// Split hi/lo FP32 tables for RTRAN and PROJECT (the composed PROJECT tables
+// are split when first used)
+int build_split_tables(mlg_ctx *c);
 // ---- FFTs (k_fft.hip)
 int launch_verfft_fwd(mlg_ctx *c, const double2 *in, double2 *out);
 int launch_verfft_bwd(mlg_ctx *c, const double2 *in, double2 *out);

@@ -125,6 +125,9 @@ SIGNATURES = {
     "mlg_solver_time": ([_V], _D),
     "mlg_solver_nstep": ([_V], _I),
     "mlg_qvortex": ([_V, _D, _V, _V], _I),
+    "mlg_set_legendre_precision": ([_V, _I], _I),
+    "mlg_legendre_precision": ([_V], _I),
+    "mlg_set_split_fp64_modes": ([_V, _I], _I),
     "mlg_read_input": ([C.c_char_p, C.POINTER(Params), _PD], _I),
     "mlg_timer_enable": ([_V, _I], _I),
     "mlg_timer_read": ([_V, C.c_char_p, _PD, C.POINTER(C.c_long), _PD, C.POINTER(_I)], _I),
@@ -184,6 +187,8 @@ class Grid:
         _call("mlg_dims", self.h, d)
         (self.ndimr, self.ndimth, self.ndimx, self.nrchopdim, self.ntchopdim, self.nxchopdim,
          self.nrh, self.nxchoph, _, self.npf, self.nnorm, self.nr) = list(d)
+        if cfg.get("split"):
+            self.legendre_precision = "split32"
 
     def close(self):
         if getattr(self, "h", None):
@@ -240,6 +245,21 @@ class Grid:
         e, nl = (_D * 16)(), (_D * 16)()
         _call("mlg_etd_op", self.h, float(dt), e, nl)
         return (np.array(e).reshape(4, 4, order="F"), np.array(nl).reshape(4, 4, order="F"))
+
+    @property
+    def legendre_precision(self):
+        """'fp64' or 'split32': the arithmetic of the Legendre products (RTRAN, PROJECT)."""
+        return "split32" if _lib().mlg_legendre_precision(self.h) == 1 else "fp64"
+
+    @legendre_precision.setter
+    def legendre_precision(self, mode):
+        if mode not in ("fp64", "split32"):
+            raise ValueError("legendre_precision is 'fp64' or 'split32'")
+        _call("mlg_set_legendre_precision", self.h, 1 if mode == "split32" else 0)
+
+    def set_split_fp64_modes(self, nmodes):
+        """Split precision: the modes m < nmodes (>= 2; default 5) keep FP64 products."""
+        _call("mlg_set_split_fp64_modes", self.h, int(nmodes))
 
     def timer(self, on):
         """Start (clearing earlier records) or stop the per-kernel HIP event timer."""
