@@ -43,7 +43,7 @@ typedef struct mlg_params {
   double ell, zlen;           /* map parameter L, axial period           */
   double nu, kappa;           /* viscosity, diffusivity                  */
   double nup, kappap;         /* hyperviscosity / hyperdiffusivity       */
-  int visc_sw, visc_p;        /* VISC%SW (0,1,3), hyperviscosity power P */
+  int visc_sw, visc_p;        /* VISC%SW (0..3), hyperviscosity power P  */
   double omega, bv0;          /* rotation rate, Brunt-Vaisala frequency  */
   int adams;                  /* 1: AB2-CN, 0: ETD2-AB with CN           */
   int hyper_cn;               /* 1: also apply the SW=3 filter with CN   */
@@ -139,6 +139,10 @@ int mlg_del2(const mlg_field *a, mlg_field *b);                /* DEL2    :751-8
 int mlg_idel2(const mlg_field *b, mlg_field *a, int has_ln, double ln); /* IDEL2 :823-1010 */
 int mlg_idel2ln(const mlg_field *b, mlg_field *a);             /* IDEL2LN :1013-1176 */
 int mlg_ihelm(const mlg_field *b, mlg_field *a, double alp);   /* IHELM   :1204-1354 */
+/* IHELMP :1409-1592: a = (del^P + nu del^2 - alp)^-1 b, even P in 4..14,
+ * alp != 0 (else 3), with the log term at m = 0, k = 0, the value at
+ * infinity of m = 0 set to zero, and CHOPDO.  VISC%SW = 2's HYPERV step. */
+int mlg_ihelmp(const mlg_field *b, mlg_field *a, int p, double alp, double nu);
 int mlg_grad(const mlg_field *a, mlg_field *rr, mlg_field *rp, mlg_field *z); /* GRAD :1727-1786 */
 int mlg_pc2vel(const mlg_field *psi, const mlg_field *chi,
                mlg_field *rur, mlg_field *rup, mlg_field *uz); /* PC2VEL  :1789-1853 */

@@ -696,6 +696,14 @@ extern "C" int mlg_ihelm(const mlg_field *b, mlg_field *a, double alp) {
   return solve_op(b, a, 2, alp, 0, 0.0);
 }
 
+extern "C" int mlg_ihelmp(const mlg_field *b, mlg_field *a, int p, double alp, double nu) {
+  if (!b || !a) return MLG_E_ARG;
+  if (b->space != MLG_FFF) return 3;
+  mlg_ctx *c = b->ctx;
+  a->space = MLG_FFF;
+  return launch_ihelmp(c, p, b->e, b->ln, a->e, a->ln, alp, nu);
+}
+
 extern "C" int mlg_grad(const mlg_field *a, mlg_field *rr, mlg_field *rp, mlg_field *z) {
   if (!a || !rr || !rp || !z) return MLG_E_ARG;
   if (a->space != MLG_FFF) return 3;
@@ -1372,7 +1380,32 @@ static int visc_cn(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, con
 static bool filter_active(const mlg_ctx *c) { return c->g.p.visc_sw == 3 || c->g.p.hyper_cn; }
 
 // CALC_BOUSSI_HYPERV, SW=3 branch (mod_march.f90:1339-1343)
+// HYPERV (src/mod_march.f90:1639-1686): implicit Euler step of dA/dt =
+// nu del^2 A - (-1)^(P/2) nup del^P A, A <- IHELMP(P, alp A, -alp, bet)
+// with alp = (-1)^(P/2) / (dt nup), bet = -(-1)^(P/2) nu / nup.
+static int hyperv2(mlg_ctx *c, mlg_field *a, double nu, double nup, int p, double dt) {
+  if (nup == 0.0 || a->space != MLG_FFF) return 5;
+  const double sg = ((p / 2) & 1) ? -1.0 : 1.0;
+  const double alp = 1 / dt / nup * sg, bet = -sg * nu / nup;
+  Temps T(c);
+  mlg_field *wk = T.buffer();
+  RET(T.ready());
+  RET(launch_copy(c, wk->e, a->e, space_count(c, MLG_FFF)));
+  RET(launch_scale(c, wk->e, alp, space_count(c, MLG_FFF)));
+  RET(launch_set_ln(c, wk->ln, a->ln, alp, 0.0));
+  return launch_ihelmp(c, p, wk->e, wk->ln, a->e, a->ln, -alp, bet);
+}
+
+// CALC_BOUSSI_HYPERV (src/mod_march.f90:1305-1349): SW = 2, HYPERV of psi,
+// chi and (with a hyperdiffusivity) b; SW = 3, the exponential filter.
 static int hyperv_state(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, double dt) {
+  if (c->g.p.visc_sw == 2) {
+    const mlg_params &P = c->g.p;
+    RET(hyperv2(c, psi, P.nu, P.nup, P.visc_p, dt));
+    RET(hyperv2(c, chi, P.nu, P.nup, P.visc_p, dt));
+    if (P.kappap != 0.0) RET(hyperv2(c, b, P.kappa, P.kappap, P.visc_p, dt));
+    return 0;
+  }
   if (c->g.p.visc_sw != 3) return 0;
   double tf = dt / c->g.p.dt;
   RET(launch_hyp3(c, psi->e, 0, tf));
@@ -1585,8 +1618,8 @@ extern "C" int mlg_initialize(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg
   mlg_ctx *c = s->ctx;
   // SOLVER_T%INITIALIZE (mod_march.f90:185-201): ETD takes only the
   // exponential hyperviscosity; VISC%SW = 2 aborts with ERR_FLAGS%SOLVER
-  if (!c->g.p.adams && c->g.p.visc_sw == 2) return 5;
-  if (c->g.p.visc_sw == 2) return MLG_E_ARG;
+  if (!c->g.p.adams && c->g.p.visc_sw == 2) return 5;  // mod_march.f90:190-201
+  if (c->g.p.visc_sw == 2 && (c->g.p.visc_p < 4 || c->g.p.visc_p > 14 || c->g.p.visc_p % 2)) return 3;
   int rc;
   if (c->g.p.adams) {
     rc = rich_fe_be(s, psi, chi, b);

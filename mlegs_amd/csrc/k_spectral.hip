@@ -1565,6 +1565,228 @@ int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *
   return launch_band_solve_n(c, op, 1, &b, &a, &ln_a, &ln_b, &alp, has_ln, ln, &t, 0);
 }
 
+// ---------------------------------------------------------------- IHELMP (VISC%SW = 2)
+// A = (del^P + nu del^2 - alp)^-1 B, even P in 4..14 (src/mod_legops.f90:
+// 1409-1592).  Per m the band of del_h^2 over NP = NRCHOPS(m) + P rows
+// (host-built, BAND_LOGLEG_RAT_DEL2H) is raised to the powers del_h^(2j),
+// j <= P/2 (one block per m, one thread per row, BANMUL's band x band order);
+// per (m, k) the (2P+1)-band matrix (del_h^2 - k^2)^(P/2) + nu (del_h^2 -
+// k^2) - alp is assembled from the powers with the binomial terms in the
+// reference's order and factorised without pivoting (LUDCMPB) over the
+// NRCHOPS(m) rows, one thread per system.  The factors are cached like the
+// other band solves'.  This path is off the benchmark workloads (VISC%SW = 2
+// runs AB2 only), so its kernels are plain one-thread-per-system loops.
+constexpr int HELMP_WMAX = 29;  // 2P+1 for P = 14
+
+__global__ void __launch_bounds__(256) k_helmp_powers(int q, int npmax, int nrchop, int minc, const double *__restrict__ d1,
+                                                      double *__restrict__ dp) {
+  const int mm = blockIdx.x, w = 4 * q + 1;
+  const int n = s_nrchops(nrchop, mm * minc) + 2 * q;
+  const double *b = d1 + (size_t)mm * npmax * 5;
+  double *lev = dp + (size_t)mm * (q + 1) * npmax * w;  // level j: del_h^(2j), width 4j+1, centre 2j
+  for (int r = threadIdx.x; r < npmax; r += blockDim.x)
+    for (int c = 0; c < w; c++) lev[((size_t)npmax + r) * w + c] = (c < 5 && r < n) ? b[(size_t)r * 5 + c] : 0.0;
+  __syncthreads();
+  for (int j = 2; j <= q; j++) {
+    const double *a = lev + (size_t)(j - 1) * npmax * w;
+    double *out = lev + (size_t)j * npmax * w;
+    const int aw = 4 * j - 3, ac = 2 * j - 2;
+    for (int r = threadIdx.x; r < npmax; r += blockDim.x) {
+      double acc[HELMP_WMAX];
+      for (int c = 0; c < w; c++) acc[c] = 0.0;
+      if (r < n)
+        for (int i = 0; i < aw; i++) {
+          const int as = i - ac;
+          if (r < max(0, -as) || r > min(n - 1, n - 1 - as)) continue;
+          for (int jj = 0; jj < 5; jj++) {
+            const int bs = jj - 2, rb = r + as;
+            if (rb < max(0, -bs) || rb > min(n - 1, n - 1 - bs)) continue;
+            acc[i + jj] += a[(size_t)r * w + i] * b[(size_t)rb * 5 + jj];
+          }
+        }
+      for (int c = 0; c < w; c++) out[(size_t)r * w + c] = acc[c];
+    }
+    __syncthreads();
+  }
+}
+
+__constant__ double c_binom[8][8] = {{1}, {1, 1}, {1, 2, 1}, {1, 3, 3, 1}, {1, 4, 6, 4, 1}, {1, 5, 10, 10, 5, 1},
+                                    {1, 6, 15, 20, 15, 6, 1}, {1, 7, 21, 35, 35, 21, 7, 1}};
+
+// One thread per (m, k): the system's factors F[((sys NRCHOP) + row) W + col]
+__global__ void __launch_bounds__(64) k_helmp_factor(KGrid g, int p, int npmax, const double *__restrict__ dp,
+                                                     const double *__restrict__ ak, double alp, double nu,
+                                                     double *__restrict__ F) {
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  const long sys = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const int mm = (int)(sys % g.ntchopdim), kk = (int)(sys / g.ntchopdim);
+  const int nn = s_nrchops(g.nrchop, mm * g.minc);
+  if (nn < 1 || mm >= g.ntchop) return;
+  const int q = p / 2, w = 2 * p + 1;
+  const double *lev = dp + (size_t)mm * (q + 1) * npmax * w;
+  const double akv = ak[mm + (size_t)g.ntchopdim * kk], k2 = akv * akv;
+  double *h = F + (size_t)sys * g.nrchop * w;
+  for (int r = 0; r < nn; r++) {
+    double *hr = h + (size_t)r * w;
+    const double *dq = lev + ((size_t)q * npmax + r) * w;
+    for (int c = 0; c < w; c++) hr[c] = dq[c];
+    for (int t = 1; t < q; t++) {
+      const int j = q - t, wj = 4 * j + 1, off = 2 * (q - j);
+      double kp = k2;
+      for (int e = 1; e < t; e++) kp = kp * k2;
+      const double cf = (t & 1) ? -(c_binom[q][t] * kp) : c_binom[q][t] * kp;
+      const double *dj = lev + ((size_t)j * npmax + r) * w;
+      for (int c = 0; c < wj; c++) hr[off + c] = hr[off + c] + cf * dj[c];
+    }
+    double kp = k2;
+    for (int e = 1; e < q; e++) kp = kp * k2;
+    hr[p] = (q & 1) ? hr[p] - kp : hr[p] + kp;
+    const double *d1 = lev + ((size_t)npmax + r) * w;
+    for (int c = 0; c < 5; c++) {
+      double dv = d1[c];
+      if (c == 2) dv = dv - k2;
+      hr[p - 2 + c] = hr[p - 2 + c] + nu * dv;
+    }
+    hr[p] = hr[p] - alp;
+  }
+  // LUDCMPB (src/mod_bandmat.f90:65-156) over nn rows, centre column p
+  const int c0 = p;
+  for (int r = 0; r < nn; r++) {
+    double *hr = h + (size_t)r * w;
+    for (int b = c0 + 1; b < w; b++)
+      if (r + (b - c0) < nn) hr[b] /= hr[c0];
+    for (int b = c0 + 1; b < w; b++)
+      for (int j = 1; j <= c0; j++)
+        if (r + max(b - c0, j) < nn) h[(size_t)(r + j) * w + (b - j)] -= hr[b] * h[(size_t)(r + j) * w + (c0 - j)];
+    for (int j = 1; j <= c0; j++)
+      if (r + j < nn) h[(size_t)(r + j) * w + (c0 - j)] /= hr[c0];
+  }
+  for (int r = 0; r < nn; r++) h[(size_t)r * w + c0] = 1.0 / h[(size_t)r * w + c0];
+}
+
+// LUBKSBB (src/mod_bandmat.f90:320-402) on each (m, k) column of A, in place
+__global__ void __launch_bounds__(64) k_helmp_solve(KGrid g, int p, const double *__restrict__ F, double2 *__restrict__ a) {
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  const long sys = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const int mm = (int)(sys % g.ntchopdim), kk = (int)(sys / g.ntchopdim);
+  const int nn = s_nrchops(g.nrchop, mm * g.minc);
+  if (nn < 1 || mm >= g.ntchop) return;
+  const int w = 2 * p + 1, c0 = p;
+  const double *h = F + (size_t)sys * g.nrchop * w;
+  double2 *x = a + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * kk);
+  for (int r = 1; r < nn; r++) {
+    const int ne = min(r, c0);
+    double2 v = x[r];
+    for (int j = 1; j <= ne; j++) v = sub2(v, axc(h[(size_t)r * w + (c0 - j)], x[r - j]));
+    x[r] = v;
+  }
+  for (int r = 0; r < nn; r++) x[r] = axc(h[(size_t)r * w + c0], x[r]);
+  for (int r = nn - 2; r >= 0; r--) {
+    double2 v = x[r];
+    for (int i = 1; i <= w - 1 - c0; i++)
+      if (r + i < nn) v = sub2(v, axc(h[(size_t)r * w + (c0 + i)], x[r + i]));
+    x[r] = v;
+  }
+}
+
+// A%LN = -B%LN / alp and the log term's contribution at m = 0, k = 0:
+// AL2 = LN (4/(3 L^2) / NORM(0), -2 / NORM(1) / L^2, 2/3 / NORM(2) / L^2),
+// AL = (del_h^2)^(P/2-1) AL2 (BANMUL), A(:,0,0) -= AL + nu AL2.  One block.
+__global__ void __launch_bounds__(256) k_helmp_logterm(KGrid g, int p, int npmax, const double *__restrict__ d1,
+                                                       const double *__restrict__ ln_b, double *__restrict__ ln_a,
+                                                       double alp, double nu, double2 *__restrict__ a) {
+  extern __shared__ double hl[];  // al2[nrchop], al[nrchop], t[nrchop]
+  const int nr = g.nrchop;
+  double *al2 = hl, *al = hl + nr, *t = hl + 2 * nr;
+  const double ln = -ln_b[0] / alp;
+  for (int n = threadIdx.x; n < nr; n += blockDim.x) al2[n] = 0.0;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    ln_a[0] = ln;
+    if (nr > 0) al2[0] = ln * (4.0 / 3 / g.ell2) / g.norm0;
+    if (nr > 1) al2[1] = ln * (-2 / g.norm1 / g.ell2);
+    if (nr > 2) al2[2] = ln * (+2.0 / 3 / g.norm2 / g.ell2);
+  }
+  __syncthreads();
+  for (int n = threadIdx.x; n < nr; n += blockDim.x) al[n] = al2[n];
+  __syncthreads();
+  for (int it = 4; it <= p; it += 2) {
+    for (int cc = threadIdx.x; cc < nr; cc += blockDim.x) {
+      const double *br = d1 + (size_t)cc * 5;  // m = 0 rows
+      double v = br[2] * al[cc];
+      for (int bi = 0; bi < 5; bi++) {
+        if (bi == 2) continue;
+        const int is = bi - 2;
+        if (cc + is >= 0 && cc + is <= nr - 1) v += br[bi] * al[cc + is];
+      }
+      t[cc] = v;
+    }
+    __syncthreads();
+    for (int n = threadIdx.x; n < nr; n += blockDim.x) al[n] = t[n];
+    __syncthreads();
+  }
+  for (int n = threadIdx.x; n < nr; n += blockDim.x) a[n].x = (a[n].x - al[n]) - nu * al2[n];
+  (void)npmax;  // the m = 0 band rows are the first of the table
+}
+
+int launch_ihelmp(mlg_ctx *c, int p, const double2 *b, const double *ln_b, double2 *a, double *ln_a, double alp, double nu) {
+  KGrid g = kgrid(c);
+  if (p < 4 || p > 14 || (p & 1) || alp == 0.0) return 3;
+  // the log-term block stages 3 NRCHOP doubles in LDS (at most 64 KB)
+  if (g.nrchop < 1 || g.nrchop + p > c->g.nnorm || 3L * g.nrchop * 8 > 65536) return MLG_E_ARG;
+  const int q = p / 2, w = 2 * p + 1, npmax = g.nrchop + p;
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  const size_t nfac = (size_t)nsys * g.nrchop * w;  // the set: factors, then the del_h^2 band rows [m][NP][5]
+  const double *F = nullptr;
+  for (auto &f : c->fcache)
+    if (f.op == 100 + p && f.alp == alp && f.nu == nu && f.nrchop == g.nrchop) F = f.f;
+  if (!F) {
+    std::vector<double> d1((size_t)g.ntchopdim * npmax * 5, 0.0);
+    for (int mm = 0; mm < g.ntchopdim; mm++) {
+      const int np = nrchops_of(g.nrchop, mm * g.minc) + p;
+      band_del2h(np, c->g.m[mm], c->g.p.ell, &c->g.lognorm[(size_t)c->g.nnorm * mm], &d1[(size_t)mm * npmax * 5]);
+    }
+    double *dp = nullptr, *Fn = nullptr;
+    if (hipMalloc((void **)&Fn, sizeof(double) * (nfac + d1.size())) != hipSuccess) return MLG_E_NOMEM;
+    if (hipMalloc((void **)&dp, sizeof(double) * (size_t)g.ntchopdim * (q + 1) * npmax * w) != hipSuccess) {
+      (void)hipFree(Fn);
+      return MLG_E_NOMEM;
+    }
+    int rc = 0;
+    if (hipMemcpyAsync(Fn + nfac, d1.data(), sizeof(double) * d1.size(), hipMemcpyHostToDevice, c->stream) != hipSuccess)
+      rc = MLG_E_HIP;
+    if (!rc) {
+      hipLaunchKernelGGL(k_helmp_powers, dim3(g.ntchopdim), dim3(256), 0, c->stream, q, npmax, g.nrchop, g.minc, Fn + nfac,
+                         dp);
+      hipLaunchKernelGGL(k_helmp_factor, dim3((unsigned)((nsys + 63) / 64)), dim3(64), 0, c->stream, g, p, npmax, dp,
+                         c->d.ak, alp, nu, Fn);
+      if (hipGetLastError() != hipSuccess) rc = MLG_E_HIP;
+    }
+    (void)hipStreamSynchronize(c->stream);
+    (void)hipFree(dp);
+    if (rc) {
+      (void)hipFree(Fn);
+      return rc;
+    }
+    FactorSet fs{100 + p, alp, g.nrchop, Fn};
+    fs.nu = nu;
+    c->fcache.push_back(fs);
+    F = Fn;
+  }
+  if (a != b) {
+    const int rc = launch_copy(c, a, b, fff_count(g));
+    if (rc) return rc;
+  }
+  hipLaunchKernelGGL(k_helmp_logterm, dim3(1), dim3(256), sizeof(double) * 3 * (size_t)g.nrchop, c->stream, g, p, npmax,
+                     F + nfac, ln_b, ln_a, alp, nu, a);
+  hipLaunchKernelGGL(k_helmp_solve, dim3((unsigned)((nsys + 63) / 64)), dim3(64), 0, c->stream, g, p, F, a);
+  MLG_CHECK_HIP(hipGetLastError());
+  const int rc = launch_fix_at_infinity(c, a);
+  return rc ? rc : launch_chopdo(c, a, MLG_FFF);
+}
+
 // ---------------------------------------------------------------- pointwise updates
 // Pointwise FFF updates with the reference's evaluation orders:
 //   mode 0: y = x1 + a*x2                      (Richardson / Euler sub-steps)

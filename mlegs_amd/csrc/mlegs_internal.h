@@ -90,6 +90,7 @@ struct FactorSet {
   int nrchop;
   double *f;
   int pins = 0;  // captured steps that use this set: not evicted while > 0
+  double nu = 0.0;  // IHELMP sets: the del^2 coefficient (op = 100 + P)
 };
 struct KTimer {
   bool on = false;
@@ -309,6 +310,10 @@ int launch_fix_at_infinity(mlg_ctx *c, double2 *a);
 // A = solve(B) on the FFF coefficients inside the truncation (B may be A);
 // the caller chops A afterwards.  t: scratch of FFF size (no other use
 // during the solve)
+// IHELMP (src/mod_legops.f90:1409-1592): a = (del^P + nu del^2 - alp)^-1 b
+// for even P in 4..14, with its log term, value at infinity and CHOPDO; a
+// may equal b
+int launch_ihelmp(mlg_ctx *c, int p, const double2 *b, const double *ln_b, double2 *a, double *ln_a, double alp, double nu);
 int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *ln_a, const double *ln_b, double alp,
                       int has_ln, double ln, double2 *t);
 // n (<= 3) independent solves of one operator in one launch (field i with

@@ -103,6 +103,7 @@ SIGNATURES = {
     "mlg_idel2": ([_V, _V, _I, _D], _I),
     "mlg_idel2ln": ([_V, _V], _I),
     "mlg_ihelm": ([_V, _V, _D], _I),
+    "mlg_ihelmp": ([_V, _V, _I, _D, _D], _I),
     "mlg_grad": ([_V, _V, _V, _V], _I),
     "mlg_pc2vel": ([_V, _V, _V, _V, _V], _I),
     "mlg_pc2vor": ([_V, _V, _V, _V, _V], _I),
@@ -419,6 +420,11 @@ def idel2ln(b, a):
 
 def ihelm(b, a, alp):
     _call("mlg_ihelm", b.h, a.h, float(alp))
+
+
+def ihelmp(p, b, a, alp, nu=0.0):
+    """IHELMP(P, B, A, ALP, NUIN): A = (del^P + nu del^2 - alp)^-1 B (src/mod_legops.f90:1409-1592)."""
+    _call("mlg_ihelmp", b.h, a.h, int(p), float(alp), float(nu))
 
 
 def grad(a, rr, rp, z):

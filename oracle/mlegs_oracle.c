@@ -1116,6 +1116,138 @@ int og_ihelm(const og_grid *g, const og_field *b, og_field *a, double alp) {
   return E_OK;
 }
 
+/* BANMULR: mod_bandmat.f90 (band x real vector), as banmul. */
+static void banmul_r(const double *a, int lda, int nb, int nc, const double *x, int n, double *out) {
+  for (int c = 0; c < n; c++) out[c] = a[c + (size_t)lda * nc] * x[c];
+  for (int bi = 0; bi < nb; bi++) {
+    if (bi == nc) continue;
+    int is = bi - nc;
+    int c0 = imax(0, -is), c1 = imin(n - 1, n - 1 - is);
+    for (int c = c0; c <= c1; c++) out[c] += a[c + (size_t)lda * bi] * x[c + is];
+  }
+}
+
+/* The (2P+1)-band matrix of (del^P + nu del^2 - alp) for mode mm and axial
+ * wavenumber squared k2 (mod_legops.f90:1518-1572): h[np x (2p+1)], rows
+ * < nn modified by nu and alp.  d[j] holds the band of del_h^(2j) (width
+ * 4j+1, centre 2j) over np rows; the binomial terms are added in the
+ * reference's order. */
+static void helmp_matrix(int p, int np, int nn, double *const *d, double k2, double nu, double alp, double *h) {
+  const int q = p / 2, w = 2 * p + 1;
+  static const double binom[8][8] = {{1}, {1, 1}, {1, 2, 1}, {1, 3, 3, 1}, {1, 4, 6, 4, 1}, {1, 5, 10, 10, 5, 1},
+                                     {1, 6, 15, 20, 15, 6, 1}, {1, 7, 21, 35, 35, 21, 7, 1}};
+  memset(h, 0, sizeof(double) * (size_t)np * w);
+  for (int c = 0; c < w; c++)
+    for (int r = 0; r < np; r++) h[r + (size_t)np * c] = d[q][r + (size_t)np * c];
+  /* - C(q,1) K2 del^(P-2) + C(q,2) K2^2 del^(P-4) - ... */
+  for (int t = 1; t < q; t++) {
+    const int j = q - t, wj = 4 * j + 1, off = 2 * (q - j);
+    double kp = k2;
+    for (int e = 1; e < t; e++) kp = kp * k2;
+    double cf = (t & 1) ? -(binom[q][t] * kp) : binom[q][t] * kp;
+    if (mutation == 20) cf = -cf;
+    for (int c = 0; c < wj; c++)
+      for (int r = 0; r < np; r++) h[r + (size_t)np * (off + c)] = h[r + (size_t)np * (off + c)] + cf * d[j][r + (size_t)np * c];
+  }
+  {
+    double kp = k2;
+    for (int e = 1; e < q; e++) kp = kp * k2;
+    for (int r = 0; r < np; r++) h[r + (size_t)np * p] = (q & 1) ? h[r + (size_t)np * p] - kp : h[r + (size_t)np * p] + kp;
+  }
+  for (int c = 0; c < 5; c++) {
+    const int cc = p - 2 + c;
+    for (int r = 0; r < nn; r++) {
+      double dv = d[1][r + (size_t)np * c];
+      if (c == 2) dv = dv - k2;
+      h[r + (size_t)np * cc] = h[r + (size_t)np * cc] + (mutation == 21 ? 0.0 : nu) * dv;
+    }
+  }
+  for (int r = 0; r < nn; r++) h[r + (size_t)np * p] = h[r + (size_t)np * p] - alp;
+}
+
+/* IHELMP: mod_legops.f90:1409-1592, A = (del^P + nu del^2 - alp)^-1 B for
+ * even P in 4..14: the log term's contribution at m = 0, k = 0, then one
+ * (2P+1)-band LU solve per (m, k) over NRCHOPS(m) rows (the band powers
+ * formed over NRCHOPS(m)+P rows), the value at infinity of m = 0, CHOPDO. */
+int og_ihelmp(const og_grid *g, int p, const og_field *b, og_field *a, double alp, double nu) {
+  if (b->space != OG_FFF) return E_LEGOP;
+  if (g->m[0] != 0 || alp == 0.0) return E_LEGOP;
+  if (p < 4 || p > 14 || (p & 1)) return E_LEGOP;
+  const double bln = b->ln;
+  og_copy(g, a, b);
+  a->ln = -bln / alp;
+  {
+    const int nr = g->nrchop;
+    double *al2 = (double *)calloc((size_t)nr + 3, sizeof(double)), *al = (double *)calloc((size_t)nr + 3, sizeof(double));
+    double *t = (double *)calloc((size_t)nr + 3, sizeof(double)), *h = (double *)malloc(sizeof(double) * 5 * (size_t)nr);
+    al2[0] = a->ln * (4.0 / 3 / g->ell2) / NORM(g, 0, 0);
+    al2[1] = a->ln * (-2 / NORM(g, 1, 0) / g->ell2);
+    al2[2] = a->ln * (+2.0 / 3 / NORM(g, 2, 0) / g->ell2);
+    memcpy(al, al2, sizeof(double) * (size_t)nr);
+    band_del2h(nr, g->m[0], g->ell, &LOGNORM(g, 0, 0), h);
+    for (int i = 4; i <= p; i += 2) {
+      banmul_r(h, nr, 5, 2, al, nr, t);
+      memcpy(al, t, sizeof(double) * (size_t)nr);
+    }
+    for (int n = 0; n < nr; n++) a->e[FIDX(g, n, 0, 0)] = a->e[FIDX(g, n, 0, 0)] - al[n] - (mutation == 22 ? 0.0 : nu) * al2[n];
+    free(al2);
+    free(al);
+    free(t);
+    free(h);
+  }
+  const int w = 2 * p + 1, q = p / 2;
+#pragma omp parallel for schedule(dynamic)
+  for (int mm = 0; mm < g->ntchopdim; mm++) {
+    const int nn = g->nrchops[mm], np = nn + p;
+    double *d[8] = {0};
+    for (int j = 1; j <= q; j++) d[j] = (double *)malloc(sizeof(double) * (size_t)np * (4 * j + 1));
+    band_del2h(np, g->m[mm], g->ell, &LOGNORM(g, 0, mm), d[1]);
+    for (int j = 2; j <= q; j++) banbanmul(d[j - 1], 4 * j - 3, 2 * j - 2, d[1], 5, 2, np, d[j]);
+    double *h = (double *)malloc(sizeof(double) * (size_t)np * w);
+    cplx *x = (cplx *)malloc(sizeof(cplx) * (size_t)(nn > 0 ? nn : 1));
+    for (int kk = 0; kk < g->nxchopdim; kk++) {
+      const double k2 = pow(AK(g, mm, kk), 2.0);
+      helmp_matrix(p, np, nn, d, k2, nu, alp, h);
+      if (nn < 1) continue;
+      lub(h, np, nn, w, p);
+      for (int n = 0; n < nn; n++) x[n] = a->e[FIDX(g, n, mm, kk)];
+      solveb(h, np, nn, w, p, x);
+      for (int n = 0; n < nn; n++) a->e[FIDX(g, n, mm, kk)] = x[n];
+    }
+    for (int j = 1; j <= q; j++) free(d[j]);
+    free(h);
+    free(x);
+  }
+  /* zero at infinity: A(1,1,:) -= CALCAT1(A) / PF(1,1,1) */
+  for (int kk = 0; kk < g->nxchopdim; kk++)
+    a->e[FIDX(g, 0, 0, kk)] = a->e[FIDX(g, 0, 0, kk)] - calcat1(g, a, kk) / PF(g, 0, 0, 0);
+  og_chopdo(g, a);
+  return E_OK;
+}
+
+/* HELMP: mod_legops.f90:1357-1406, B = (del^P + nu del^2 + alp) A (the
+ * forward operator, for checks of IHELMP): repeated DEL2. */
+int og_helmp(const og_grid *g, int p, const og_field *a, og_field *b, double alp, double nu) {
+  if (a->space != OG_FFF) return E_LEGOP;
+  og_field *w = og_field_new((og_grid *)g, OG_FFF), *d2 = og_field_new((og_grid *)g, OG_FFF);
+  int rc = og_del2(g, a, d2);
+  if (!rc) og_copy(g, b, d2);
+  for (int i = 4; !rc && i <= p; i += 2) {
+    rc = og_del2(g, b, w);
+    if (!rc) og_copy(g, b, w);
+  }
+  if (!rc) {
+    for (size_t i = 0; i < (size_t)g->nrchopdim * g->ntchopdim * g->nxchopdim; i++) {
+      b->e[i] = b->e[i] - alp * a->e[i];
+      b->e[i] = b->e[i] + nu * d2->e[i];
+    }
+    b->ln = -alp * a->ln;
+  }
+  og_field_free(w);
+  og_field_free(d2);
+  return rc;
+}
+
 /* GRAD: mod_legops.f90:1727-1786 */
 int og_grad(const og_grid *g, const og_field *a, og_field *rr, og_field *rp, og_field *z) {
   if (rr->space != OG_FFF) { realloc_space(g, rr, OG_FFF); realloc_space(g, rp, OG_FFF); realloc_space(g, z, OG_FFF); }
@@ -1712,13 +1844,37 @@ static int visc_cn(og_grid *g, og_field *psi, og_field *chi, og_field *b, const 
   return rc;
 }
 
-/* CALC_BOUSSI_HYPERV (SW=3 branch): mod_march.f90:1305-1349 */
-static void hyperv_state(og_grid *g, og_field *psi, og_field *chi, og_field *b, double dt) {
-  if (g->p.visc_sw != 3) return;
+/* HYPERV: mod_march.f90:1639-1686, implicit Euler step of
+ * dA/dt = nu del^2 A -(-1)^(P/2) nup del^P A through IHELMP. */
+static int hyperv2(og_grid *g, og_field *a, double nu, double nup, int p, double dt) {
+  if (nup == 0.0) return E_SOLVER;
+  if (a->space != OG_FFF) return E_SOLVER;
+  const int ph = p / 2;
+  const double sg = (ph & 1) ? -1.0 : 1.0;
+  const double alp = 1 / dt / nup * sg, bet = -sg * nu / nup;
+  og_field *wk = og_field_new(g, OG_FFF);
+  for (size_t i = 0; i < (size_t)g->nrchopdim * g->ntchopdim * g->nxchopdim; i++) wk->e[i] = alp * a->e[i];
+  wk->ln = alp * a->ln;
+  int rc = og_ihelmp(g, p, wk, a, -alp, bet);
+  og_field_free(wk);
+  return rc;
+}
+
+/* CALC_BOUSSI_HYPERV: mod_march.f90:1305-1349 (SW=2: HYPERV of psi, chi and,
+ * with a hyperdiffusivity, b; SW=3: the exponential filter) */
+static int hyperv_state(og_grid *g, og_field *psi, og_field *chi, og_field *b, double dt) {
+  if (g->p.visc_sw == 2) {
+    int rc = hyperv2(g, psi, g->p.nu, g->p.nup, g->p.visc_p, dt);
+    if (!rc) rc = hyperv2(g, chi, g->p.nu, g->p.nup, g->p.visc_p, dt);
+    if (!rc && g->p.kappap != 0.0) rc = hyperv2(g, b, g->p.kappa, g->p.kappap, g->p.visc_p, dt);
+    return rc;
+  }
+  if (g->p.visc_sw != 3) return E_OK;
   double tf = dt / g->p.dt;
   hyp3_apply(g, psi, 0, tf);
   hyp3_apply(g, chi, 0, tf);
   hyp3_apply(g, b, 1, tf);
+  return E_OK;
 }
 
 /* CALC_BOUSSI_ETD_HYPERV: mod_march.f90:1352-1416 (SW=3; with hyper_cn the

@@ -81,6 +81,8 @@ int og_del2(const og_grid *g, const og_field *a, og_field *b);
 int og_idel2(const og_grid *g, const og_field *b, og_field *a, int has_ln, double ln);
 int og_idel2ln(const og_grid *g, const og_field *b, og_field *a);
 int og_ihelm(const og_grid *g, const og_field *b, og_field *a, double alp);
+int og_ihelmp(const og_grid *g, int p, const og_field *b, og_field *a, double alp, double nu);
+int og_helmp(const og_grid *g, int p, const og_field *a, og_field *b, double alp, double nu);
 int og_grad(const og_grid *g, const og_field *a, og_field *rr, og_field *rp, og_field *z);
 int og_pc2vel(const og_grid *g, const og_field *psi, const og_field *chi,
               og_field *rur, og_field *rup, og_field *uz);

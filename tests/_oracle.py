@@ -70,6 +70,8 @@ def lib():
             "og_idel2": ([g, P, P, C.c_int, C.c_double], C.c_int),
             "og_idel2ln": ([g, P, P], C.c_int),
             "og_ihelm": ([g, P, P, C.c_double], C.c_int),
+            "og_ihelmp": ([g, C.c_int, P, P, C.c_double, C.c_double], C.c_int),
+            "og_helmp": ([g, C.c_int, P, P, C.c_double, C.c_double], C.c_int),
             "og_grad": ([g, P, P, P, P], C.c_int),
             "og_pc2vel": ([g, P, P, P, P, P], C.c_int),
             "og_pc2vor": ([g, P, P, P, P, P], C.c_int),

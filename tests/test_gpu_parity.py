@@ -275,6 +275,16 @@ def test_vel2pln(P, ln):
     P.check(gpsi, opsi, what="VEL2PLN psi")
 
 
+@pytest.mark.parametrize("p,alp,nu", [(4, -250.0, 0.0), (6, 4e3, 0.8), (8, -1e5, -2.0), (14, 3e6, 1.5)])
+def test_ihelmp(P, p, alp, nu):
+    """IHELMP (VISC%SW = 2's solve) against the oracle, P = 4 .. 14."""
+    gb, ob = P.both(P.noise(11), O.FFF, ln=0.29)
+    ga, oa = P.empty()
+    H.ihelmp(p, gb, ga, alp, nu)
+    P.o.call("ihelmp", p, ob, oa, alp, nu)
+    P.check(ga, oa, what=f"IHELMP P={p}")
+
+
 @pytest.mark.parametrize("src", [O.PPP, O.PFF, O.FFF])
 def test_project(P, src):
     if src == O.FFF:
@@ -443,6 +453,14 @@ def test_steps_wide_transforms(cfg):
     """NTH or NX = 512: the steps run the general transforms, the separate
     product kernels and the non-fused NONLIN sequence."""
     run_steps(cfg, 2)
+
+
+@pytest.mark.parametrize("p", [4, 6])
+def test_steps_implicit_hyperviscosity_sw2(p):
+    """VISC%SW = 2 (AB2-CN march): HYPERV, an implicit Euler step of the
+    nu del^2 and P-th order hyperviscous terms through IHELMP, after the
+    Richardson start's sub-steps and every step (CALC_BOUSSI_HYPERV)."""
+    run_steps(dict(SMALL, visc_sw=2, visc_p=p, nup=1e-6, kappap=2e-6), 3)
 
 
 @pytest.mark.parametrize("cfg", [dict(SMALL, visc_sw=3, nup=1e-3, kappap=1e-3, visc_p=4),
