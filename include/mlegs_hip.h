@@ -63,8 +63,9 @@ enum {
   MLG_E_HIP = 101,      /* HIP runtime error                        */
   MLG_E_NOMEM = 102,    /* device allocation failed                 */
   MLG_E_SPACE = 103,    /* field in the wrong space for the call    */
-  MLG_E_SIZE = 104      /* input larger than the fixed scratch of an
+  MLG_E_SIZE = 104,     /* input larger than the fixed scratch of an
                            allocation-free entry point              */
+  MLG_E_IO = 105        /* a file could not be opened, read or written */
 };
 
 /* ------------------------------------------------------------------ */
@@ -231,6 +232,43 @@ int mlg_set_split_fp64_modes(mlg_ctx *ctx, int nmodes);
 /* Initial conditions: QVORTEX (src/mod_layout.f90:449-610) for one centred
  * Gaussian vortex of strength q (omega_z = 2 q exp(-r^2)), chi = 0. */
 int mlg_qvortex(mlg_ctx *ctx, double q, mlg_field *psi, mlg_field *chi);
+
+/* ------------------------------------------------------------------ */
+/* Field files and run settings (the drivers' I/O). */
+/* MPISAVEX (src/mod_fft.f90:2304-2377): FN.info (grid, space, LN records)
+ * and FN (raw complex128, one process block).  Synchronising. */
+int mlg_field_save(const mlg_field *f, const char *fn);
+/* MPILOADX (src/mod_fft.f90:2510-2642): fills f from FN.info + FN written
+ * by a run on n1 x n2 processes (1, 1 for files of this library); a grid
+ * other than the file's returns 32 (DIMMISMATCH), a bad file MLG_E_IO. */
+int mlg_field_load(mlg_field *f, const char *fn, int n1, int n2);
+/* The m = 0 columns E(:,1,k), k < nk, in the field's current space, as
+ * INSPECT (src/mod_march.f90:3743-3775) prints them; *nrad = first extent.
+ * out = NULL only reports *nrad.  Synchronising. */
+int mlg_field_m0_columns(const mlg_field *f, int nk, double *out, int *nrad);
+
+/* The run settings of read.input that the drivers use besides mlg_params
+ * (READIN0, src/mod_init.f90:401-710): file names, output schedule,
+ * diagnostics cadence, vortex removal and hyperviscosity adjustment
+ * switches, monitored modes.  Names are blank-padded, not terminated. */
+#define MLG_NAME_LEN 72
+#define MLG_MAX_OUTPUTS 512
+#define MLG_MAX_MONITOR 32
+typedef struct mlg_run_settings {
+  char savedir[MLG_NAME_LEN], psi0[MLG_NAME_LEN], chi0[MLG_NAME_LEN], b0[MLG_NAME_LEN];
+  char psii[MLG_NAME_LEN], chii[MLG_NAME_LEN], bi[MLG_NAME_LEN];
+  int it1, it2;          /* DIAGNOST (defaults 40, 40)            */
+  int rmv_sw, rmv_int;   /* REMOVE (defaults 0, 10000)            */
+  int adj_sw, adj_int;   /* NUP_ADJUST (default switch 0)         */
+  int noutputs;          /* OUTPUTS: snapshot times and names     */
+  double out_t[MLG_MAX_OUTPUTS];
+  char out_psi[MLG_MAX_OUTPUTS][MLG_NAME_LEN], out_chi[MLG_MAX_OUTPUTS][MLG_NAME_LEN];
+  char out_b[MLG_MAX_OUTPUTS][MLG_NAME_LEN];
+  int nmonitor;          /* MONITOR_MK: (m, k) pairs, start, interval */
+  int monitor_mk[MLG_MAX_MONITOR][2];
+  int monitor_start, monitor_interval;
+} mlg_run_settings;
+int mlg_read_run_settings(const char *path, mlg_run_settings *s);
 
 /* ------------------------------------------------------------------ */
 /* Kernel event timer.  Instrumentation with no counterpart in the

@@ -170,6 +170,7 @@ extern "C" const char *mlg_error_string(int code) {
     case MLG_E_NOMEM: return "device allocation failed";
     case MLG_E_SPACE: return "field is in the wrong space for this call";
     case MLG_E_SIZE: return "size exceeds the fixed scratch of an allocation-free entry point";
+    case MLG_E_IO: return "file could not be opened, read or written";
     default: return "unknown";
   }
 }

@@ -189,3 +189,121 @@ extern "C" int mlg_read_input(const char *path, mlg_params *p, double *t_limit) 
   if (t_limit) *t_limit = limit;
   return 0;
 }
+
+// The run settings besides the grid and physics: READIN0's SAVEDIR, PSI0,
+// CHI0, B0, PSII, CHII, BI, OUTPUTS, DIAGNOST, REMOVE, NUP_ADJUST and
+// MONITOR_MK blocks (src/mod_init.f90:551-660), with READIN0's defaults
+// (:428-447).  Names are stored blank-padded like CHARACTER(LEN=72).
+namespace {
+void put_name(char *dst, const std::string &s) {
+  std::memset(dst, ' ', MLG_NAME_LEN);
+  std::memcpy(dst, s.data(), s.size() < (size_t)MLG_NAME_LEN ? s.size() : (size_t)MLG_NAME_LEN);
+}
+}  // namespace
+
+extern "C" int mlg_read_run_settings(const char *path, mlg_run_settings *s) {
+  if (!path || !s) return MLG_E_ARG;
+  Reader r;
+  r.in.open(path);
+  if (!r.in) return MLG_E_IO;
+  std::memset(s, 0, sizeof(*s));
+  const char *blank_names[] = {s->savedir, s->psi0, s->chi0, s->b0, s->psii, s->chii, s->bi};
+  for (const char *p : blank_names) std::memset((char *)p, ' ', MLG_NAME_LEN);
+  s->it1 = 40;
+  s->it2 = 40;
+  s->rmv_sw = 0;
+  s->rmv_int = 10000;
+  s->adj_sw = 0;
+  int nvortex = 1;
+  std::vector<std::string> f, v;
+  int lines = 0;
+  auto row = [&]() {
+    if (!r.next(v)) v.clear();
+    v.resize(8);
+  };
+  while (r.next(f) && lines++ < 1000) {
+    if (f.empty()) continue;
+    const std::string &key = f[0];
+    if (key == "SAVEDIR") {
+      row();
+      put_name(s->savedir, v[0]);
+    } else if (key == "PSI0") {
+      row();
+      put_name(s->psi0, v[0]);
+    } else if (key == "CHI0") {
+      row();
+      put_name(s->chi0, v[0]);
+    } else if (key == "B0") {
+      row();
+      put_name(s->b0, v[0]);
+    } else if (key == "PSII") {
+      row();
+      put_name(s->psii, v[0]);
+    } else if (key == "CHII") {
+      row();
+      put_name(s->chii, v[0]);
+    } else if (key == "BI") {
+      row();
+      put_name(s->bi, v[0]);
+    } else if (key == "OUTPUTS") {
+      row();
+      const int n = atoi_f(v[0]);
+      if (n < 0 || n > MLG_MAX_OUTPUTS) return MLG_E_ARG;
+      s->noutputs = n;
+      for (int i = 0; i < n; i++) {
+        row();
+        s->out_t[i] = atof_f(v[0]);
+        put_name(s->out_psi[i], v[1]);
+        put_name(s->out_chi[i], v[2]);
+        put_name(s->out_b[i], v[3]);
+      }
+    } else if (key == "DIAGNOST") {
+      row();
+      s->it1 = atoi_f(v[0]);
+      s->it2 = atoi_f(v[1]);
+    } else if (key == "REMOVE") {
+      row();
+      s->rmv_sw = atoi_f(v[0]);
+      s->rmv_int = atoi_f(v[1]);
+    } else if (key == "NUP_ADJUST") {
+      row();
+      s->adj_sw = atoi_f(v[0]);
+      s->adj_int = atoi_f(v[1]);
+    } else if (key == "MONITOR_MK") {
+      row();
+      const int n = atoi_f(v[0]);
+      if (n < 0 || n > MLG_MAX_MONITOR) return MLG_E_ARG;
+      s->nmonitor = n;
+      for (int i = 0; i < n; i++) {
+        row();
+        s->monitor_mk[i][0] = atoi_f(v[0]);
+        s->monitor_mk[i][1] = atoi_f(v[1]);
+      }
+      row();
+      s->monitor_start = atoi_f(v[0]);
+      s->monitor_interval = atoi_f(v[1]);
+    } else if (key == "VORTEXPAIR") {
+      row();
+      nvortex = atoi_f(v[0]);
+      for (int i = 0; i < nvortex; i++) row();
+    } else if (key == "VORTEXPAIR_CROW" || key == "VORTEXPAIR_SECTION") {
+      for (int i = 0; i < nvortex; i++) row();
+    } else if (key == "POSTPROCESS") {
+      row();
+      const int n = atoi_f(v[0]);
+      for (int i = 0; i < n; i++) row();
+    } else if (key == "END") {
+      break;
+    } else {
+      static const char *one[] = {"DIMENSION", "CHOPS", "ELL", "ZLEN", "MINC", "MKLINK", "DT", "VISCOSITY",
+                                  "BOUSSINESQ", "VORTEXPAIR_NOISE", "TIME_INTEG", "FREESTREAM", "STRAIN", "LINEAR",
+                                  "DEBUG"};
+      for (const char *k : one)
+        if (key == k) {
+          row();
+          break;
+        }
+    }
+  }
+  return 0;
+}

@@ -67,7 +67,8 @@ module mlegs_hip
   public :: leginit, legfinal, allocate, deallocate, to_host, to_device, chopset, chopdo, nrchop_now
   public :: horfft, verfft, rtran, toff, tofp, plnfft
   public :: delsqh, idelsqh, mulxm, xxdx, del2, idel2, idel2ln, ihelm, grad, pc2vel, pc2vor, project
-  public :: calc_boussi_force, calc_boussi_nonlin, calc_boussi_energy, qvortex, copy_scalar
+  public :: calc_boussi_force, calc_boussi_nonlin, calc_boussi_energy, qvortex, copy_scalar, ihelmp
+  public :: check, mlegs_ctx
 
   interface
     integer(c_int) function c_init(p, c) bind(c, name='mlg_init')
@@ -195,6 +196,12 @@ module mlegs_hip
       type(c_ptr), value :: b, a
       real(c_double), value :: alp
     end function
+    integer(c_int) function mlg_ihelmp(b, a, p, alp, nu) bind(c)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: b, a
+      integer(c_int), value :: p
+      real(c_double), value :: alp, nu
+    end function
     integer(c_int) function mlg_grad(a, rr, rp, z) bind(c)
       import :: c_int, c_ptr
       type(c_ptr), value :: a, rr, rp, z
@@ -272,6 +279,11 @@ contains
     ndimr = d(1); ndimth = d(2); ndimx = d(3)
     nrchopdim = d(4); ntchopdim = d(5); nxchopdim = d(6)
   end subroutine leginit
+
+  ! the context handle, for modules that call the C ABI directly
+  type(c_ptr) function mlegs_ctx()
+    mlegs_ctx = ctx
+  end function mlegs_ctx
 
   subroutine legfinal()
     if (c_associated(ctx)) call check(c_finalize(ctx), 'FINALIZE')
@@ -442,6 +454,18 @@ contains
     real(c_double), intent(in) :: alp
     call check(mlg_ihelm(b%h, a%h, alp), 'IHELM')
   end subroutine ihelm
+
+  subroutine ihelmp(p, b, a, alp, nuin)
+    integer, intent(in) :: p
+    type(scalar), intent(in) :: b
+    type(scalar), intent(inout) :: a
+    real(c_double), intent(in) :: alp
+    real(c_double), intent(in), optional :: nuin
+    real(c_double) :: nu
+    nu = 0.0_c_double
+    if (present(nuin)) nu = nuin
+    call check(mlg_ihelmp(b%h, a%h, int(p, c_int), alp, nu), 'IHELMP')
+  end subroutine ihelmp
 
   subroutine grad(a, rr, rp, z)
     type(scalar), intent(in) :: a

@@ -63,6 +63,21 @@ class Params(C.Structure):
         return p
 
 
+NAME_LEN, MAX_OUTPUTS, MAX_MONITOR = 72, 512, 32
+
+
+class RunSettings(C.Structure):
+    """mlg_run_settings: read.input's file names, outputs and cadences."""
+    _fields_ = [("savedir", C.c_char * NAME_LEN), ("psi0", C.c_char * NAME_LEN), ("chi0", C.c_char * NAME_LEN),
+                ("b0", C.c_char * NAME_LEN), ("psii", C.c_char * NAME_LEN), ("chii", C.c_char * NAME_LEN),
+                ("bi", C.c_char * NAME_LEN), ("it1", C.c_int), ("it2", C.c_int), ("rmv_sw", C.c_int),
+                ("rmv_int", C.c_int), ("adj_sw", C.c_int), ("adj_int", C.c_int), ("noutputs", C.c_int),
+                ("out_t", C.c_double * MAX_OUTPUTS), ("out_psi", (C.c_char * NAME_LEN) * MAX_OUTPUTS),
+                ("out_chi", (C.c_char * NAME_LEN) * MAX_OUTPUTS), ("out_b", (C.c_char * NAME_LEN) * MAX_OUTPUTS),
+                ("nmonitor", C.c_int), ("monitor_mk", (C.c_int * 2) * MAX_MONITOR), ("monitor_start", C.c_int),
+                ("monitor_interval", C.c_int)]
+
+
 _LIB = None
 _V = C.c_void_p
 _D = C.c_double
@@ -130,6 +145,10 @@ SIGNATURES = {
     "mlg_legendre_precision": ([_V], _I),
     "mlg_set_split_fp64_modes": ([_V, _I], _I),
     "mlg_read_input": ([C.c_char_p, C.POINTER(Params), _PD], _I),
+    "mlg_read_run_settings": ([C.c_char_p, C.POINTER(RunSettings)], _I),
+    "mlg_field_save": ([_V, C.c_char_p], _I),
+    "mlg_field_load": ([_V, C.c_char_p, _I, _I], _I),
+    "mlg_field_m0_columns": ([_V, _I, _PD, C.POINTER(_I)], _I),
     "mlg_timer_enable": ([_V, _I], _I),
     "mlg_timer_read": ([_V, C.c_char_p, _PD, C.POINTER(C.c_long), _PD, C.POINTER(_I)], _I),
 }
@@ -163,6 +182,23 @@ def read_input(path):
     lim = C.c_double(0.0)
     _call("mlg_read_input", os.fsencode(path), C.byref(p), C.byref(lim))
     return {k: getattr(p, k) for k, _ in Params._fields_}, lim.value
+
+
+def read_run_settings(path):
+    """read.input's run settings (names, OUTPUTS, DIAGNOST, REMOVE, NUP_ADJUST, MONITOR_MK) as a dict."""
+    st = RunSettings()
+    _call("mlg_read_run_settings", os.fsencode(path), C.byref(st))
+
+    def name(b):
+        return bytes(b).decode(errors="replace").rstrip()
+
+    out = {k: name(getattr(st, k)) for k in ("savedir", "psi0", "chi0", "b0", "psii", "chii", "bi")}
+    out.update({k: getattr(st, k) for k in ("it1", "it2", "rmv_sw", "rmv_int", "adj_sw", "adj_int",
+                                            "monitor_start", "monitor_interval")})
+    out["outputs"] = [(st.out_t[i], name(st.out_psi[i]), name(st.out_chi[i]), name(st.out_b[i]))
+                      for i in range(st.noutputs)]
+    out["monitor_mk"] = [(st.monitor_mk[i][0], st.monitor_mk[i][1]) for i in range(st.nmonitor)]
+    return out
 
 
 def _check(rc, where):
@@ -328,6 +364,23 @@ class Scalar:
     @property
     def ln(self):
         return self.download()[2]
+
+    def save(self, fn):
+        """MSAVE / MPISAVEX (src/mod_fft.f90:2304-2377): FN.info and FN, written by the library."""
+        _call("mlg_field_save", self.h, os.fsencode(fn))
+
+    def load(self, fn, procs=(1, 1)):
+        """MLOAD / MPILOADX (src/mod_fft.f90:2510-2642) of a file written on procs=(N1, N2) processes."""
+        _call("mlg_field_load", self.h, os.fsencode(fn), int(procs[0]), int(procs[1]))
+        return self
+
+    def m0_columns(self, nk):
+        """E(:, m=0, k) for k < nk in the current space (what INSPECT prints)."""
+        nrad = C.c_int(0)
+        _call("mlg_field_m0_columns", self.h, int(nk), None, C.byref(nrad))
+        out = np.zeros((nrad.value, nk), dtype=np.complex128, order="F")
+        _call("mlg_field_m0_columns", self.h, int(nk), out.ctypes.data_as(_PD), C.byref(nrad))
+        return out
 
     def device_ptrs(self):
         e, ln = _V(), _V()

@@ -79,3 +79,32 @@ def test_reference_read_input_if_present():
 def test_missing_file_is_an_error(tmp_path):
     with pytest.raises(H.MlegsError):
         H.read_input(str(tmp_path / "nope"))
+
+
+def test_run_settings(tmp_path):
+    """READIN0's name, OUTPUTS, DIAGNOST, REMOVE, NUP_ADJUST and MONITOR_MK
+    blocks (src/mod_init.f90:551-660) and its defaults (:428-447)."""
+    f = tmp_path / "read.input"
+    f.write_text("DIMENSION\n 32 16 16\nSAVEDIR\n  ./run/\nPSI0\n  p0.dat\nOUTPUTS\n 2\n# t psi chi b\n 1.5 a1 b1 c1\n"
+                 " 3.0, 'a 2', b2, c2\nDIAGNOST\n 7 9\nNUP_ADJUST\n 1 50\nMONITOR_MK\n 2\n 1 -3\n 0 4\n 5. 10.\n%END\n")
+    s = H.read_run_settings(str(f))
+    assert s["savedir"] == "./run/" and s["psi0"] == "p0.dat" and s["chi0"] == ""
+    assert s["outputs"] == [(1.5, "a1", "b1", "c1"), (3.0, "a 2", "b2", "c2")]
+    assert (s["it1"], s["it2"], s["adj_sw"], s["adj_int"]) == (7, 9, 1, 50)
+    assert (s["rmv_sw"], s["rmv_int"]) == (0, 10000)                    # defaults
+    assert s["monitor_mk"] == [(1, -3), (0, 4)] and (s["monitor_start"], s["monitor_interval"]) == (5, 10)
+    d = tmp_path / "empty.input"
+    d.write_text("DIMENSION\n 8 8 8\n")
+    s = H.read_run_settings(str(d))
+    assert (s["it1"], s["it2"], s["outputs"], s["monitor_mk"]) == (40, 40, [], [])
+
+
+def test_reference_run_settings_if_present():
+    ref = "/root/reference/read.input"
+    if not os.path.exists(ref):
+        pytest.skip("reference tree not mounted")
+    s = H.read_run_settings(ref)
+    assert s["savedir"] == "./output/" and (s["psi0"], s["chi0"], s["b0"]) == ("psi0.dat", "chi0.dat", "b0.dat")
+    assert len(s["outputs"]) == 150 and s["outputs"][0] == (2.0, "psi2.dat", "chi2.dat", "b2.dat")
+    assert (s["rmv_sw"], s["rmv_int"], s["adj_sw"], s["adj_int"]) == (2, 1000, 1, 100)
+    assert s["monitor_mk"] == [(2, -10), (1, 12), (3, 2), (0, 0)]
