@@ -43,6 +43,8 @@ def parse():
                     help="time eager launches instead of replays of the captured step graph")
     ap.add_argument("--no-split", action="store_true",
                     help="skip the split hi/lo FP32 Legendre measurement (timing and error against FP64)")
+    ap.add_argument("--no-adjoint", action="store_true",
+                    help="skip the adjoint measurement (128x64x64 AB2-CN, 40-step gradient)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 20 copies of a 384x257x256 complex field (407 MB each way), "
                          "to scale HBM byte counters")
@@ -151,6 +153,67 @@ def split_measurement(H, cfg, steps, warmup, ms_fp64):
                      "per field (the FP64 path itself is held to the oracle by the parity tests)")
 
 
+def adjoint_measurement(H):
+    """Adjoint/forward cost on the 128x64x64 AB2-CN workload (CFG5): the
+    gradient of the perturbation kinetic energy after 40 steps (mlegs_amd.
+    adjoint).  Per step: the library's forward step (mlg_time_step) and its
+    adjoint (mlg_step_adjoint, which recomputes the state's products), each
+    timed over 40 calls on the same state; per trajectory: the checkpointed
+    gradient (forward with a state kept every 5 steps, reverse sweep with
+    each segment recomputed) against the plain 40-step forward march."""
+    import torch
+    from mlegs_amd import adjoint as A
+    from mlegs_amd.configs import CFG5, LAMB_OSEEN_Q, seeded_noise
+
+    cfg = CFG5
+    g = H.Grid(cfg)
+    m = A.March(g)
+    try:
+        psi, chi = g.scalar(), g.scalar()
+        H.qvortex(g, LAMB_OSEEN_Q, psi, chi)
+        p0, _, ln = psi.download()
+        amp = cfg["noise"]
+        x0 = m.state_from(p0 + seeded_noise(cfg, 1, amp, m_only=1), seeded_noise(cfg, 2, amp, m_only=1),
+                          seeded_noise(cfg, 3, amp, m_only=1), psi_ln=ln)
+        n = cfg["nsteps"]
+        A.gradient(m, x0, n, every=5)  # warm-up: pools, factor sets
+        x1 = m.step(x0, True)
+        # per step, on the library's fields directly
+        m.load(x1)
+        ybar = torch.zeros_like(x1)
+        ybar[: 2 * m.nfff] = 1.0
+        m.load(ybar, adj=True)
+
+        def timed(fn, reps):
+            torch.cuda.synchronize()
+            t0 = time.perf_counter()
+            for _ in range(reps):
+                fn()
+            torch.cuda.synchronize()
+            return (time.perf_counter() - t0) / reps * 1e3
+
+        m.load(x1)
+        fwd = timed(lambda: m.solver.time_step(*m.u), n)
+        m.load(x1)
+        adj = timed(lambda: m.solver.step_adjoint(*m.u, m.ub, m.hb), n)
+        run_ms = timed(lambda: A.run(m, x0, n), 3)
+        grad_ms = timed(lambda: A.gradient(m, x0, n, every=5), 3)
+        J, _ = A.gradient(m, x0, n, every=5)
+        return dict(workload="128x64x64 AB2-CN (NRCHOP=96, NTCHOP=NXCHOP=22, dt=1e-3), 40 steps, "
+                             "Lamb-Oseen + m=1 noise 1e-5; J = perturbation KE (m >= 1)",
+                    fwd_ms_per_step=round(fwd, 4), adjoint_ms_per_step=round(adj, 4),
+                    adjoint_over_forward_step=round(adj / fwd, 3),
+                    forward_40_steps_ms=round(run_ms, 3), gradient_40_steps_ms=round(grad_ms, 3),
+                    gradient_over_forward_trajectory=round(grad_ms / run_ms, 3), J=J,
+                    note="adjoint step includes recomputing the state's velocity, vorticity and buoyancy "
+                         "gradient; the trajectory gradient adds the checkpointed forward and the "
+                         "segment recomputation (every 5 steps); both trajectories include the flat-tensor "
+                         "state copies of mlegs_amd.adjoint")
+    finally:
+        m.close()
+        g.close()
+
+
 def cpu_baseline(cfg, nsteps):
     """The oracle (FP64 C, OpenMP) on this host: steps/s over `nsteps` steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -158,7 +221,10 @@ def cpu_baseline(cfg, nsteps):
     import _oracle as O
     from mlegs_amd.configs import LAMB_OSEEN_Q, seeded_noise
 
-    cores = min(16, os.cpu_count() or 1)
+    # the host's CPUs as the scheduler reports them; one GPU's share of the
+    # host is 16 cores, so the oracle runs on at most that many threads
+    nproc = len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else (os.cpu_count() or 1)
+    cores = min(16, nproc)
     o = O.OGrid(cfg, threads=cores)
     try:
         psi, chi = o.field(O.FFF), o.field(O.FFF)
@@ -177,10 +243,10 @@ def cpu_baseline(cfg, nsteps):
         s.close()
     finally:
         o.close()
-    return dict(value=nsteps / dt, unit="steps/s", cores=cores, kind="port",
+    return dict(value=nsteps / dt, unit="steps/s", cores=cores, host_nproc=nproc, kind="port",
                 sample=f"{nsteps} ETD2 steps of the 384x256x256 workload from the seeded state "
                        f"(ETD operators set, no Richardson start), FP64 C restatement with OpenMP, "
-                       f"{cores} threads, {dt:.1f} s")
+                       f"{cores} threads (host nproc {nproc}, one GPU's share 16 cores), {dt:.1f} s")
 
 
 def main():
@@ -274,6 +340,10 @@ def main():
     if not a.no_split:
         print("bench: split hi/lo FP32 Legendre products", file=sys.stderr, flush=True)
         split = split_measurement(H, cfg, a.steps, a.warmup, ms_step)
+    adjoint = None
+    if not a.no_adjoint:
+        print("bench: adjoint/forward cost (128x64x64 AB2-CN)", file=sys.stderr, flush=True)
+        adjoint = adjoint_measurement(H)
     cpu = None
     if a.cpu_steps > 0:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
@@ -301,6 +371,7 @@ def main():
         "cpu_baseline": cpu,
         "stages": stages,
         "split_precision": split,
+        "adjoint": adjoint,
     }
     print(json.dumps(out), flush=True)
 

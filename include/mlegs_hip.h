@@ -271,6 +271,39 @@ typedef struct mlg_run_settings {
 int mlg_read_run_settings(const char *path, mlg_run_settings *s);
 
 /* ------------------------------------------------------------------ */
+/* Discrete adjoint of the AB2-CN march (no counterpart in the reference,
+ * which has no adjoint; for optimal-perturbation and sensitivity callers).
+ * Exact transposes, for the inner product sum Re(conj(a) b) over stored
+ * coefficients plus a%LN b%LN (psi, chi), of what mlg_initialize and
+ * mlg_time_step compute with ADAMS = 1, VISC%SW = 0 or 1, no
+ * hyperviscosity and FP64 Legendre products (else MLG_E_ARG).  All fields
+ * are FFF; adjoint fields are ordinary fields.  States are passed in, not
+ * kept: a caller that checkpoints recomputes them.
+ *   mlg_force_adjoint: *_bar += CALC_BOUSSI_FORCE'(psi, chi, b)^T (psin_bar,
+ *     chin_bar, bn_bar).
+ *   mlg_step_adjoint: (psi, chi, b) the state before one STEP_BOUSSI_AB_CN;
+ *     in: *_bar the adjoint of the state after it, h*_bar of the history
+ *     (N_psi, N_chi, N_b) after it; out: the same before it.
+ *   mlg_start_adjoint: the same for RICH_BOUSSI_FE_BE (mlg_initialize) from
+ *     the initial state; the history adjoint comes out zero.
+ *   mlg_ke_gradient: d/d(psi, chi) of the sum over m >= mmin (mmin = 1) and
+ *     all k of CALC_BOUSSI_ENERGY's KE(m, k), written to psi_bar, chi_bar.
+ *   mlg_solver_history: the solver's own history fields. */
+int mlg_force_adjoint(const mlg_field *psi, const mlg_field *chi, const mlg_field *b, const mlg_field *psin_bar,
+                      const mlg_field *chin_bar, const mlg_field *bn_bar, mlg_field *psi_bar, mlg_field *chi_bar,
+                      mlg_field *b_bar);
+int mlg_step_adjoint(mlg_solver *s, const mlg_field *psi, const mlg_field *chi, const mlg_field *b, mlg_field *psi_bar,
+                     mlg_field *chi_bar, mlg_field *b_bar, mlg_field *hpsi_bar, mlg_field *hchi_bar, mlg_field *hb_bar);
+int mlg_start_adjoint(mlg_solver *s, const mlg_field *psi, const mlg_field *chi, const mlg_field *b, mlg_field *psi_bar,
+                      mlg_field *chi_bar, mlg_field *b_bar, mlg_field *hpsi_bar, mlg_field *hchi_bar, mlg_field *hb_bar);
+int mlg_ke_gradient(const mlg_field *psi, const mlg_field *chi, int mmin, mlg_field *psi_bar, mlg_field *chi_bar);
+int mlg_solver_history(mlg_solver *s, mlg_field **hpsi, mlg_field **hchi, mlg_field **hb);
+/* Transposes of single operators (op codes in adjoint.hip): 0 TOFP, 1 TOFF,
+ * 2 PROJECT from PPP, 3 IDEL2LN, 4 IHELM(alp), 5 DEL2, 6 PC2VEL+PC2VOR+GRAD,
+ * 7 CALC_BOUSSI_FORCE's pointwise terms.  No reference counterpart. */
+int mlg_adjoint_op(int op, mlg_field *const *f, int nf, double alp);
+
+/* ------------------------------------------------------------------ */
 /* Kernel event timer.  Instrumentation with no counterpart in the
  * reference (which times whole steps with CPU_TIME).  mlg_timer_enable(1)
  * clears and starts recording HIP events around every launch of the timed

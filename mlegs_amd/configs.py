@@ -7,7 +7,8 @@ dictionaries whose keys match the fields of ``mlg_params``:
 * ``CFG2`` -- 384x256x256 ETD2 production case (the headline workload).
 * ``CFG3`` -- CFG2 with split hi/lo FP32 Legendre products (``split=1``:
   ``mlegs_amd.hip.Grid`` selects ``legendre_precision = "split32"``).
-* ``CFG5`` -- 128x64x64 AB2-CN case used for the adjoint/forward cost.
+* ``CFG5`` -- 128x64x64 AB2-CN case of the adjoint (gradient of the
+  perturbation kinetic energy after 40 steps; mlegs_amd.adjoint).
 
 Mapping of the production case's viscosity: the reference offers CN
 viscosity (``VISC%SW=1``, mod_march.f90:1233-1265) or the exponential
@@ -108,3 +109,15 @@ def seeded_noise(cfg, field_id, amplitude=None, seed=NOISE_SEED, m_only=None, nr
     for kpos in range(1, nxc):
         a[:, 0, nxd - kpos] = np.conj(a[:, 0, kpos])
     return np.asfortranarray(a)
+
+
+FD_SEED = 20261203
+
+
+def fd_direction(cfg, i):
+    """The i-th seeded direction (psi, chi, b) of the adjoint's finite-difference
+    checks: seeded noise over every kept coefficient (all m), with the
+    amplitude, envelope, truncation and m = 0 symmetry of the workload's
+    initial noise, so that a step e along it is a relative change e of the
+    perturbation."""
+    return tuple(seeded_noise(cfg, 3 * i + f + 1, seed=FD_SEED) for f in range(3))

@@ -412,7 +412,8 @@ extern "C" int mlg_field_destroy(mlg_field *f) {
 extern "C" int mlg_field_space(const mlg_field *f) { return f ? f->space : -1; }
 
 // pooled temporaries (ALLOCATE of a work SCALAR); LN starts at 0
-static mlg_field *tmp(mlg_ctx *c, int space) {
+namespace mlg {
+mlg_field *pool_take(mlg_ctx *c, int space) {
   mlg_field *f = nullptr;
   if (!c->pool.empty()) {
     f = c->pool.back();
@@ -424,45 +425,10 @@ static mlg_field *tmp(mlg_ctx *c, int space) {
   f->pooled = true;
   return f;
 }
-static void rel(mlg_ctx *c, mlg_field *f) {
+void pool_give(mlg_ctx *c, mlg_field *f) {
   if (f) c->pool.push_back(f);
 }
-
-// Temporaries of one routine.  get() hands out pool buffers; ready() checks
-// them and zeroes all their LN coefficients with one launch.
-struct Temps {
-  mlg_ctx *c;
-  std::vector<mlg_field *> v;
-  std::vector<double *> pending;
-  bool failed = false;
-  explicit Temps(mlg_ctx *cc) : c(cc) {}
-  mlg_field *get(int space = MLG_FFF) {
-    mlg_field *f = tmp(c, space);
-    if (f) {
-      v.push_back(f);
-      pending.push_back(f->ln);
-    } else {
-      failed = true;
-    }
-    return f;
-  }
-  // a buffer whose LN is never read (it is copied over, or only its E is used)
-  mlg_field *buffer(int space = MLG_FFF) {
-    mlg_field *f = tmp(c, space);
-    if (f) v.push_back(f);
-    else failed = true;
-    return f;
-  }
-  int ready() {
-    if (failed) return MLG_E_NOMEM;
-    int rc = pending.empty() ? 0 : launch_zero_lns(c, pending.data(), (int)pending.size());
-    pending.clear();
-    return rc;
-  }
-  ~Temps() {
-    for (auto *f : v) rel(c, f);
-  }
-};
+}  // namespace mlg
 
 static void swap_storage(mlg_field *a, mlg_field *b) { std::swap(a->e, b->e); }
 
@@ -512,7 +478,7 @@ extern "C" int mlg_field_zero(mlg_field *f, int space) {
 }
 
 // COPY0 (mod_fft.f90:706-769): copy, LN included; FFF copies are chopped.
-static int copy0(mlg_field *b, const mlg_field *a) {
+int mlg::copy_field(mlg_field *b, const mlg_field *a) {
   mlg_ctx *c = a->ctx;
   if (a == b) return 0;
   if (a->space == MLG_FFF) {
@@ -525,6 +491,7 @@ static int copy0(mlg_field *b, const mlg_field *a) {
   if (a->space == MLG_FFF) RET(launch_chopdo(c, b->e, MLG_FFF));
   return 0;
 }
+static int copy0(mlg_field *b, const mlg_field *a) { return copy_field(b, a); }
 extern "C" int mlg_copy(mlg_field *b, const mlg_field *a) {
   if (!a || !b) return MLG_E_ARG;
   return copy0(b, a);

@@ -1565,6 +1565,110 @@ int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *
   return launch_band_solve_n(c, op, 1, &b, &a, &ln_a, &ln_b, &alp, has_ln, ln, &t, 0);
 }
 
+// ---------------------------------------------------------------- transposed solves
+// Adjoint of IDEL2LN / IHELM: G = A^-T H for every system, on the factors
+// the forward solve caches.  A = L D U with L, U unit triangular (L the
+// stored multipliers, U the stored normalised upper entries, D the pivots),
+// so A^T = U^T D L^T: U^T w = h (forward over the rows), v = D^-1 w, then
+// L^T g = v (backward over the rows).  One thread per system, in place on
+// the FFF column.
+template <int NE, int C0>
+__device__ void band_apply_t(int nd, double2 *x, const FactorView &FV, long ncls, long cls) {
+  constexpr int KU = NE - 1 - C0;
+  auto lmul = [&](int row, int j) -> double {  // L(row, row - j)
+    if (NE == 5) return FV.ff[((size_t)row * ncls + cls) * 3 + (j - 1)];
+    return FV.f7[((size_t)row * ncls + cls) * FSTRIDE + (j - 1)];
+  };
+  auto uup = [&](int row, int u) -> double {  // U(row, row + u)
+    if (NE == 5) return FV.fb[((size_t)row * ncls + cls) * 2 + (u - 1)];
+    return FV.f7[((size_t)row * ncls + cls) * FSTRIDE + C0 + (u - 1)];
+  };
+  auto rpiv = [&](int row) -> double {
+    if (NE == 5) return FV.ff[((size_t)row * ncls + cls) * 3 + 2];
+    return FV.f7[((size_t)row * ncls + cls) * FSTRIDE + NE - 1];
+  };
+  double2 wh[KU];
+  for (int u = 0; u < KU; u++) wh[u] = zero2();
+  for (int nn = 0; nn < nd; nn++) {
+    double2 w = x[nn];
+    for (int u = 1; u <= KU; u++)
+      if (nn - u >= 0) w = sub2(w, axc(uup(nn - u, u), wh[u - 1]));
+    for (int u = KU - 1; u >= 1; u--) wh[u] = wh[u - 1];
+    wh[0] = w;
+    x[nn] = axc(rpiv(nn), w);
+  }
+  double2 gh[C0];
+  for (int j = 0; j < C0; j++) gh[j] = zero2();
+  for (int nn = nd - 1; nn >= 0; nn--) {
+    double2 v = x[nn];
+    for (int j = 1; j <= C0; j++)
+      if (nn + j < nd) v = sub2(v, axc(lmul(nn + j, j), gh[j - 1]));
+    for (int j = C0 - 1; j >= 1; j--) gh[j] = gh[j - 1];
+    gh[0] = v;
+    x[nn] = v;
+  }
+}
+
+// op 1 (IDEL2LN): the (m = 0, k = 0) system's first unknown became LN
+// (LN = x(0) NORM(0,0), x(0) = 0), so its adjoint enters as x(0) =
+// NORM(0,0) LNbar.  op 2 (IHELM): the m = 0 systems solve (del^2 - alp -
+// k^2)(1-x) y on NN-1 rows and return XM y, so XM^T comes first; at k = 0
+// the right-hand side had LN terms removed (A%LN = -B%LN/ALP), whose
+// adjoint gives B's LNbar.  ln_bar: LNbar of the solution (in), of the
+// right-hand side (out, op 2).
+__global__ void __launch_bounds__(64) k_band_apply_t(KGrid g, const double *bxm, int op, double alp, const double *F,
+                                                     double2 *x, double *ln_bar) {
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  const long ncls = (long)g.ntchopdim * g.nxchop;
+  const long sys = blockIdx.x * (long)blockDim.x + threadIdx.x;
+  if (sys >= nsys) return;
+  const int mm = (int)(sys % g.ntchopdim), k = (int)(sys / g.ntchopdim);
+  const int nn = s_nrchops(g.nrchop, mm * g.minc);
+  if (nn < 1 || mm >= g.ntchop) return;
+  const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
+  const long cls = mm + (long)g.ntchopdim * kabs;
+  double2 *col = x + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
+  const FactorView FV = factor_view(F, g.nrchopdim, ncls);
+  const int kind = solve_kind(op, mm, k);
+  if (kind == 0) {
+    if (op == 1 && mm == 0 && k == 0) col[0] = make_double2(g.norm0 * ln_bar[0], 0.0);
+    band_apply_t<5, 2>(nn, col, FV, ncls, cls);
+    return;
+  }
+  // IHELM, m = 0: x <- XM^T x on NN rows (ascending, from the original values)
+  double2 prev = zero2();
+  for (int c = 0; c < nn; c++) {
+    const double *r = bxm + (size_t)c * 3;
+    double2 t = axc(r[1], col[c]);
+    if (c + 1 < nn) t = add2(t, axc(bxm[(size_t)(c + 1) * 3 + 0], col[c + 1]));
+    if (c >= 1) t = add2(t, axc(bxm[(size_t)(c - 1) * 3 + 2], prev));
+    prev = col[c];
+    col[c] = t;
+  }
+  band_apply_t<7, 3>(nn - 1, col, FV, ncls, cls);
+  col[nn - 1] = zero2();
+  if (k == 0) {
+    const double lbar = ln_bar[0] - (col[0].x * ((4.0 / 3 / g.ell2) / g.norm0) + col[1].x * (-2 / g.norm1 / g.ell2) +
+                                     col[2].x * (+2.0 / 3 / g.norm2 / g.ell2));
+    ln_bar[0] = -lbar / alp;
+  }
+}
+
+int launch_band_solve_t(mlg_ctx *c, int op, double alp, double2 *x, double *ln_bar) {
+  if (op != 1 && op != 2) return MLG_E_ARG;
+  KGrid g = kgrid(c);
+  if (op == 2 && g.nrchop < 4) return MLG_E_ARG;  // the m = 0 form needs the three LN rows
+  int rc = 0;
+  const double *F = factor_set(c, op, alp, nullptr, 0, &rc);
+  if (!F) return rc;
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  KScope ks(c, K_BAND_SOLVE, 2.0 * 16.0 * fff_count(g));
+  hipLaunchKernelGGL(k_band_apply_t, dim3((unsigned)((nsys + 63) / 64)), dim3(64), 0, c->stream, g, c->d.bxm, op, alp, F, x,
+                     ln_bar);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
 // ---------------------------------------------------------------- IHELMP (VISC%SW = 2)
 // A = (del^P + nu del^2 - alp)^-1 B, even P in 4..14 (src/mod_legops.f90:
 // 1409-1592).  Per m the band of del_h^2 over NP = NRCHOPS(m) + P rows

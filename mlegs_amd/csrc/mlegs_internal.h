@@ -322,6 +322,12 @@ int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *
 int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, double2 *const *a, double *const *ln_a,
                         const double *const *ln_b, const double *alp, int has_ln, double ln, double2 *const *t,
                         int bil = 0);
+// Adjoint of the op 1 (IDEL2LN) or op 2 (IHELM) solve, in place on the FFF
+// field x (the solution's adjoint in, the right-hand side's out); ln_bar:
+// LNbar of the solution (op 1: read; op 2: read, then replaced by the
+// right-hand side's).  CHOPDO and IDEL2LN's fix at infinity are the
+// caller's.
+int launch_band_solve_t(mlg_ctx *c, int op, double alp, double2 *x, double *ln_bar);
 int launch_axpby(mlg_ctx *c, int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n);
 int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1, const double *x2, const double *x3);
 // ETD update fused with the HYPERV3 filters (filter != 0), the copy of b
@@ -340,5 +346,46 @@ int launch_energy_terms(mlg_ctx *c, const double2 *a, const double2 *b, int l2, 
 int launch_negate(mlg_ctx *c, double2 *a, long n);
 int launch_scale(mlg_ctx *c, double2 *a, double s, long n);
 int launch_check_top_m1(mlg_ctx *c, const double2 *a);
+
+// ---- pooled temporaries and COPY0 (api.hip)
+mlg_field *pool_take(mlg_ctx *c, int space);
+void pool_give(mlg_ctx *c, mlg_field *f);
+int copy_field(mlg_field *b, const mlg_field *a);  // COPY0: LN included, FFF copies chopped
+
+// Temporaries of one routine.  get() hands out pool buffers; ready() checks
+// them and zeroes all their LN coefficients with one launch.
+struct Temps {
+  mlg_ctx *c;
+  std::vector<mlg_field *> v;
+  std::vector<double *> pending;
+  bool failed = false;
+  explicit Temps(mlg_ctx *cc) : c(cc) {}
+  mlg_field *get(int space = MLG_FFF) {
+    mlg_field *f = pool_take(c, space);
+    if (f) {
+      v.push_back(f);
+      pending.push_back(f->ln);
+    } else {
+      failed = true;
+    }
+    return f;
+  }
+  // a buffer whose LN is never read (it is copied over, or only its E is used)
+  mlg_field *buffer(int space = MLG_FFF) {
+    mlg_field *f = pool_take(c, space);
+    if (f) v.push_back(f);
+    else failed = true;
+    return f;
+  }
+  int ready() {
+    if (failed) return MLG_E_NOMEM;
+    int rc = pending.empty() ? 0 : launch_zero_lns(c, pending.data(), (int)pending.size());
+    pending.clear();
+    return rc;
+  }
+  ~Temps() {
+    for (auto *f : v) pool_give(c, f);
+  }
+};
 
 }  // namespace mlg

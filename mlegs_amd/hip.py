@@ -149,6 +149,12 @@ SIGNATURES = {
     "mlg_field_save": ([_V, C.c_char_p], _I),
     "mlg_field_load": ([_V, C.c_char_p, _I, _I], _I),
     "mlg_field_m0_columns": ([_V, _I, _PD, C.POINTER(_I)], _I),
+    "mlg_force_adjoint": ([_V] * 9, _I),
+    "mlg_step_adjoint": ([_V] * 10, _I),
+    "mlg_start_adjoint": ([_V] * 10, _I),
+    "mlg_ke_gradient": ([_V, _V, _I, _V, _V], _I),
+    "mlg_solver_history": ([_V, C.POINTER(_V), C.POINTER(_V), C.POINTER(_V)], _I),
+    "mlg_adjoint_op": ([_I, C.POINTER(_V), _I, _D], _I),
     "mlg_timer_enable": ([_V, _I], _I),
     "mlg_timer_read": ([_V, C.c_char_p, _PD, C.POINTER(C.c_long), _PD, C.POINTER(_I)], _I),
 }
@@ -407,6 +413,17 @@ class Scalar:
         return t.view(shape[::-1]).permute(2, 1, 0)
 
 
+class _Borrowed(Scalar):
+    """A field owned by the library (a solver's history field): not destroyed here."""
+
+    def __init__(self, grid, handle):
+        self.grid = grid
+        self.h = handle
+
+    def close(self):
+        self.h = None
+
+
 # ---------------------------------------------------------------------------
 # transforms (src/mod_fft.f90)
 def chopdo(a):
@@ -532,6 +549,25 @@ def calc_boussi_energy(psi, chi, b):
     return f(ke), f(pe), f(dke)
 
 
+def force_adjoint(psi, chi, b, nbar, ubar):
+    """ubar (3 fields) += CALC_BOUSSI_FORCE'(psi, chi, b)^T nbar (3 fields)."""
+    _call("mlg_force_adjoint", psi.h, chi.h, b.h, *[f.h for f in nbar], *[f.h for f in ubar])
+
+
+ADJ_TOFP, ADJ_TOFF, ADJ_PROJECT, ADJ_IDEL2LN, ADJ_IHELM, ADJ_DEL2, ADJ_VELVORGRAD, ADJ_PRODUCTS = range(8)
+
+
+def adjoint_op(op, fields, alp=0.0):
+    """Transpose of one operator (mlg_adjoint_op) on the given fields."""
+    arr = (_V * len(fields))(*[f.h for f in fields])
+    _call("mlg_adjoint_op", int(op), arr, len(fields), float(alp))
+
+
+def ke_gradient(psi, chi, psi_bar, chi_bar, mmin=1):
+    """d/d(psi, chi) of the sum over m >= mmin of CALC_BOUSSI_ENERGY's KE."""
+    _call("mlg_ke_gradient", psi.h, chi.h, int(mmin), psi_bar.h, chi_bar.h)
+
+
 def diagnost(psi, chi):
     """DIAGNOST's per-step checks (src/mod_march.f90:1726-1791): raises
     MlegsError(11) (dealiasing) or MlegsError(12) (divergence)."""
@@ -573,6 +609,22 @@ class Solver:
         """NSTEPS steps, one synchronisation at the end (device checks are sticky).
         graph=True replays a captured step (same results, fewer launches)."""
         _call("mlg_time_steps_graph" if graph else "mlg_time_steps", self.h, psi.h, chi.h, b.h, int(nsteps))
+
+    def history(self):
+        """The solver's own AB2 history fields (N_psi, N_chi, N_b), borrowed."""
+        a, b, c = _V(), _V(), _V()
+        _call("mlg_solver_history", self.h, C.byref(a), C.byref(b), C.byref(c))
+        return _Borrowed(self.grid, a), _Borrowed(self.grid, b), _Borrowed(self.grid, c)
+
+    def step_adjoint(self, psi, chi, b, ub, hb):
+        """Transpose of one AB2-CN step from the state (psi, chi, b): ub (3
+        fields) in: adjoint of the state after the step, out: before it; hb
+        the same for the history."""
+        _call("mlg_step_adjoint", self.h, psi.h, chi.h, b.h, *[f.h for f in ub], *[f.h for f in hb])
+
+    def start_adjoint(self, psi, chi, b, ub, hb):
+        """Transpose of the Richardson start (initialize) from the initial state."""
+        _call("mlg_start_adjoint", self.h, psi.h, chi.h, b.h, *[f.h for f in ub], *[f.h for f in hb])
 
     @property
     def t(self):

@@ -1,0 +1,483 @@
+"""The discrete adjoint of the AB2-CN march (mlg_force_adjoint,
+mlg_step_adjoint, mlg_start_adjoint, mlg_ke_gradient) on the GPU.
+
+Dot-product tests: for a map S and its adjoint S^T at a state x,
+<S'(x) d, y> = <d, S'(x)^T y> for random d, y, with the inner product
+sum Re(conj(a) b) + LN products.  S'(x) d comes from the forward library
+calls themselves by central differences, which are exact up to round-off
+for CALC_BOUSSI_FORCE and the AB2-CN step (quadratic in the state) and the
+energy (quadratic); the Richardson start is a quartic, for which the
+combination (4 D(e/2) - D(e)) / 3 of central differences D is exact.  The
+two sides are held to 1e-10 of the larger term.
+
+Trajectory tests (the 128x64x64 workload): the gradient of the final
+perturbation kinetic energy after 40 steps against central finite
+differences (e = 1e-6 along seeded directions), <= 1e-6.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from mlegs_amd import adjoint as A
+from mlegs_amd import hip as H
+from mlegs_amd.configs import CFG5, LAMB_OSEEN_Q, fd_direction, seeded_noise
+
+pytestmark = pytest.mark.gpu
+
+SMALL_AB = dict(CFG5, nr=32, nth=16, nx=16, nrchop=22, ntchop=6, nxchop=5, noise=1e-3)
+# b without diffusion (no CN for b), odd truncation, another box
+SMALL_AB2 = dict(SMALL_AB, nrchop=21, kappa=0.0, omega=0.9, bv0=1.7, ell=1.3, zlen=5.0)
+DOT_TOL = 1e-10
+
+
+@pytest.fixture(params=["ab", "ab_kappa0"])
+def M(request):
+    import torch
+
+    torch.cuda.set_device(0)
+    cfg = {"ab": SMALL_AB, "ab_kappa0": SMALL_AB2}[request.param]
+    g = H.Grid(cfg)
+    m = A.March(g)
+    m.cfg = cfg
+    yield m
+    m.close()
+    g.close()
+
+
+def state(m, amp, seed0=1, history=True):
+    """Flat march state: Lamb-Oseen psi plus seeded noise on every field
+    (and LN values), histories seeded too when asked."""
+    g, cfg = m.grid, m.cfg
+    psi, chi = g.scalar(), g.scalar()
+    H.qvortex(g, LAMB_OSEEN_Q, psi, chi)
+    p0, _, ln = psi.download()
+    x = m.state_from(p0 + seeded_noise(cfg, seed0, amp), seeded_noise(cfg, seed0 + 1, amp),
+                     seeded_noise(cfg, seed0 + 2, amp), psi_ln=ln)
+    n2 = 2 * m.nfff
+    x[6 * n2 + 1] = 0.3 * amp
+    if history:
+        import torch
+
+        for i in range(3):
+            h = np.asfortranarray(seeded_noise(cfg, seed0 + 3 + i, amp))
+            x[(3 + i) * n2:(4 + i) * n2] = torch.from_numpy(h.reshape(-1, order="F").view(np.float64).copy()).cuda()
+        x[6 * n2 + 2] = 0.7 * amp
+        x[6 * n2 + 3] = -0.2 * amp
+    return x
+
+
+def direction(m, seed0, amp=1.0, history=True):
+    """A seeded direction over every degree of freedom (LN values included)."""
+    import torch
+
+    cfg = m.cfg
+    d = torch.zeros(m.size, dtype=torch.float64, device="cuda")
+    n2 = 2 * m.nfff
+    for i in range(6 if history else 3):
+        a = np.asfortranarray(seeded_noise(cfg, seed0 + i, amp, seed=777))
+        d[i * n2:(i + 1) * n2] = torch.from_numpy(a.reshape(-1, order="F").view(np.float64).copy()).cuda()
+    d[6 * n2:6 * n2 + 2] = torch.tensor([0.5 * amp, -0.4 * amp], dtype=torch.float64)
+    if history:
+        d[6 * n2 + 2:6 * n2 + 4] = torch.tensor([0.25 * amp, 0.6 * amp], dtype=torch.float64)
+    return d
+
+
+def dot(a, b):
+    return float((a * b).sum())
+
+
+def check_dot(lhs, rhs, what):
+    e = abs(lhs - rhs) / max(abs(lhs), abs(rhs))
+    print(f"{what}: <S'd, y> = {lhs:.15e}, <d, S'^T y> = {rhs:.15e}, rel {e:.2e}", flush=True)
+    assert e <= DOT_TOL, f"{what}: rel {e:.3e}"
+
+
+# ---------------------------------------------------------------- single operators
+def fdot(a, b):
+    """sum Re(conj(a) b) over two host arrays"""
+    return float(np.sum(a.real * b.real + a.imag * b.imag))
+
+
+def rand_ppp(g, cfg, seed):
+    """Random physical-space field: entries on r < NR, theta slots < NTH."""
+    rng = np.random.default_rng(seed)
+    a = np.zeros(g.shape(H.PPP), dtype=np.complex128, order="F")
+    a[:g.nr, :cfg["nth"], :] = rng.standard_normal((g.nr, cfg["nth"], g.ndimx)) + \
+        1j * rng.standard_normal((g.nr, cfg["nth"], g.ndimx))
+    return a
+
+
+def fff(g, cfg, seed, amp=1.0, nrchop=None):
+    return seeded_noise(cfg, seed, amp, seed=4242, nrchop=nrchop)
+
+
+def test_transposed_transforms(M):
+    """TOFP^T and TOFF^T against mlg_tofp / mlg_toff."""
+    g, cfg = M.grid, M.cfg
+    x = fff(g, cfg, 1)
+    y = rand_ppp(g, cfg, 2)
+    a = g.scalar(H.FFF, x)
+    H.tofp(a)
+    yb = g.scalar(H.PPP, y)
+    H.adjoint_op(H.ADJ_TOFP, [yb])
+    assert yb.space == H.FFF
+    check_dot(fdot(a.download()[0], y), fdot(x, yb.download()[0]), "TOFP")
+    c = g.scalar(H.PPP, y)
+    H.toff(c)
+    xb = g.scalar(H.FFF, x)
+    H.adjoint_op(H.ADJ_TOFF, [xb])
+    assert xb.space == H.PPP
+    check_dot(fdot(c.download()[0], x), fdot(y, xb.download()[0]), "TOFF")
+
+
+def test_transposed_project(M):
+    g, cfg = M.grid, M.cfg
+    us = [rand_ppp(g, cfg, 10 + i) for i in range(3)]
+    ins = [g.scalar(H.PPP, u) for u in us]
+    psi, chi = g.scalar(), g.scalar()
+    H.project(*ins, psi, chi, ln=0.0)
+    pb, cb = fff(g, cfg, 20), fff(g, cfg, 21)
+    fs = [g.scalar(H.FFF, pb), g.scalar(H.FFF, cb)] + [g.scalar(H.PPP) for _ in range(3)]
+    H.adjoint_op(H.ADJ_PROJECT, fs)
+    lhs = fdot(psi.download()[0], pb) + fdot(chi.download()[0], cb)
+    rhs = sum(fdot(u, f.download()[0]) for u, f in zip(us, fs[2:]))
+    check_dot(lhs, rhs, "PROJECT")
+
+
+def test_transposed_solves(M):
+    g, cfg = M.grid, M.cfg
+    b = fff(g, cfg, 30)
+    ab, abln = fff(g, cfg, 31), 0.37
+    # IDEL2LN: A (with LN) from B
+    gb, ga = g.scalar(H.FFF, b, 0.0), g.scalar()
+    H.idel2ln(gb, ga)
+    a, _, aln = ga.download()
+    fb = [g.scalar(H.FFF, ab, abln), g.scalar()]
+    H.adjoint_op(H.ADJ_IDEL2LN, fb)
+    check_dot(fdot(a, ab) + aln * abln, fdot(b, fb[1].download()[0]), "IDEL2LN")
+    # IHELM: both sides carry LN
+    alp = 2 / cfg["nu"] / cfg["dt"]
+    bln = -0.21
+    gb, ga = g.scalar(H.FFF, b, bln), g.scalar()
+    H.ihelm(gb, ga, alp)
+    a, _, aln = ga.download()
+    fb = [g.scalar(H.FFF, ab, abln), g.scalar()]
+    H.adjoint_op(H.ADJ_IHELM, fb, alp)
+    bb, _, bbln = fb[1].download()
+    check_dot(fdot(a, ab) + aln * abln, fdot(b, bb) + bln * bbln, "IHELM")
+
+
+def test_transposed_del2(M):
+    g, cfg = M.grid, M.cfg
+    a, aln = fff(g, cfg, 40, nrchop=cfg["nrchop"] - 1), 0.45  # DEL2 needs the top mode of m = 1 zero
+    yb = fff(g, cfg, 41)
+    ga, gb = g.scalar(H.FFF, a, aln), g.scalar()
+    H.del2(ga, gb)
+    fb = [g.scalar(H.FFF, yb), g.scalar()]
+    H.adjoint_op(H.ADJ_DEL2, fb)
+    ab, _, abln = fb[1].download()
+    check_dot(fdot(gb.download()[0], yb), fdot(a, ab) + aln * abln, "DEL2")
+
+
+def test_transposed_velvorgrad(M):
+    g, cfg = M.grid, M.cfg
+    psi, pln = fff(g, cfg, 50), 0.8
+    chi, cln = fff(g, cfg, 51, nrchop=cfg["nrchop"] - 1), -0.3
+    b = fff(g, cfg, 52)
+    gp, gc, gb = g.scalar(H.FFF, psi, pln), g.scalar(H.FFF, chi, cln), g.scalar(H.FFF, b)
+    outs = [g.scalar() for _ in range(9)]
+    H.pc2vel(gp, gc, *outs[0:3])
+    H.pc2vor(gp, gc, *outs[3:6])
+    H.grad(gb, *outs[6:9])
+    ys = [fff(g, cfg, 60 + i) for i in range(9)]
+    fs = [g.scalar(H.FFF, y) for y in ys] + [g.scalar() for _ in range(3)]
+    for f in fs[9:]:
+        f.zero(H.FFF)
+    H.adjoint_op(H.ADJ_VELVORGRAD, fs)
+    lhs = sum(fdot(o.download()[0], y) for o, y in zip(outs, ys))
+    (pb, _, pbln), (cb, _, cbln), (bb, _, _) = [f.download() for f in fs[9:]]
+    rhs = fdot(psi, pb) + pln * pbln + fdot(chi, cb) + cln * cbln + fdot(b, bb)
+    check_dot(lhs, rhs, "PC2VEL+PC2VOR+GRAD")
+
+
+def test_transposed_products(M):
+    """CALC_BOUSSI_FORCE's pointwise terms: VPRODSUB and VDOTPSUB from the
+    library, the Coriolis, buoyancy and stratification terms added on the
+    host; central differences of this bilinear map are exact."""
+    g, cfg = M.grid, M.cfg
+    two_om, n2 = 2 * cfg["omega"], cfg["bv0"] ** 2
+    x = [rand_ppp(g, cfg, 70 + i) for i in range(10)]  # u, om, gb, w
+    d = [rand_ppp(g, cfg, 80 + i) for i in range(10)]
+    y = [rand_ppp(g, cfg, 90 + i) for i in range(4)]  # adjoints of o1..o3, bn
+
+    def G(v):
+        u = [g.scalar(H.PPP, a) for a in v[0:3]]
+        om = [g.scalar(H.PPP, a) for a in v[3:6]]
+        gb = [g.scalar(H.PPP, a) for a in v[6:9]]
+        H.vprodsub(*u, *om)
+        dot_ = g.scalar(H.PPP)
+        H.vdotpsub(*u, *gb, dot_)
+        o = [f.download()[0] for f in om]
+        o[0] = o[0] + two_om * v[1]
+        o[1] = o[1] - two_om * v[0]
+        o[2] = o[2] - v[9]
+        bn = -dot_.download()[0] + n2 * v[2]
+        mask = np.zeros(g.shape(H.PPP), dtype=bool)
+        mask[:g.nr, :cfg["nth"], :] = True
+        return [np.where(mask, a, 0) for a in o + [bn]]
+
+    gp, gm = G([a + b for a, b in zip(x, d)]), G([a - b for a, b in zip(x, d)])
+    lhs = sum(fdot((p - m) / 2, yy) for p, m, yy in zip(gp, gm, y))
+    fs = [g.scalar(H.PPP, a) for a in x[0:9]] + [g.scalar(H.PPP, a) for a in y[0:3]] + [g.scalar(H.PPP, y[3])]
+    H.adjoint_op(H.ADJ_PRODUCTS, fs)
+    bars = [f.download()[0] for f in fs[0:9]] + [fs[12].download()[0]]
+    rhs = sum(fdot(dd, bb) for dd, bb in zip(d, bars))
+    check_dot(lhs, rhs, "FORCE products")
+
+
+def test_force_adjoint_dot(M):
+    g = M.grid
+    x = state(M, 1e-2, history=False)
+    d = direction(M, 11, 1e-2, history=False)
+    ybar = direction(M, 21, 1.0, history=False)
+    eps = 0.5
+
+    def force(xx):
+        M.load(xx, state_only=True)
+        outs = [g.scalar() for _ in range(3)]
+        H.calc_boussi_force(*M.u, *outs)
+        n2 = 2 * M.nfff
+        import torch
+
+        y = torch.zeros(M.size, dtype=torch.float64, device="cuda")
+        for i, f in enumerate(outs):
+            a, _, ln = f.download()
+            y[i * n2:(i + 1) * n2] = torch.from_numpy(np.asfortranarray(a).reshape(-1, order="F").view(np.float64).copy()).cuda()
+            if i < 2:
+                y[6 * n2 + i] = ln
+        return y
+
+    lin = (force(x + eps * d) - force(x - eps * d)) / (2 * eps)
+    # adjoint: outputs' adjoint ybar (psiN, chiN, bN) into the state's
+    M.load(x, state_only=True)
+    M.load(ybar, adj=True, state_only=True)  # psin_bar, chin_bar, bn_bar in ub
+    nb = [g.scalar() for _ in range(3)]
+    for f, src in zip(nb, M.ub):
+        H.copy(f, src)
+    for f in M.ub:
+        f.zero(H.FFF)
+    H.force_adjoint(*M.u, nb, M.ub)
+    xbar = M.dump(adj=True)
+    n2 = 2 * M.nfff
+    keep = np.zeros(M.size, dtype=bool)
+    keep[:3 * n2] = True
+    keep[6 * n2:6 * n2 + 2] = True
+    import torch
+
+    kt = torch.from_numpy(keep).cuda()
+    check_dot(dot(lin * kt, ybar * kt), dot(d * kt, xbar * kt), "CALC_BOUSSI_FORCE")
+
+
+def test_step_adjoint_dot(M):
+    x = state(M, 1e-2)
+    d = direction(M, 31, 1e-2)
+    ybar = direction(M, 41, 1.0)
+    eps = 0.5
+    lin = (M.step(x + eps * d, False) - M.step(x - eps * d, False)) / (2 * eps)
+    xbar = M.step_adjoint(x, ybar, False)
+    check_dot(dot(lin, ybar), dot(d, xbar), "STEP_BOUSSI_AB_CN")
+
+
+def test_start_adjoint_dot(M):
+    x = state(M, 1e-2, history=False)
+    d = direction(M, 51, 1e-2, history=False)
+    ybar = direction(M, 61, 1.0)
+
+    def D(e):
+        return (M.step(x + e * d, True) - M.step(x - e * d, True)) / (2 * e)
+
+    eps = 0.25
+    lin = (4 * D(eps / 2) - D(eps)) / 3
+    xbar = M.step_adjoint(x, ybar, True)
+    n2 = 2 * M.nfff
+    assert float(xbar[3 * n2:6 * n2].abs().max()) == 0.0  # the start reads no history
+    check_dot(dot(lin, ybar), dot(d, xbar), "RICH_BOUSSI_FE_BE")
+
+
+def test_ke_gradient_dot(M):
+    x = state(M, 1e-2, history=False)
+    d = direction(M, 71, 1e-2, history=False)
+    eps = 0.5
+    lin = (M.energy(x + eps * d) - M.energy(x - eps * d)) / (2 * eps)
+    gJ = M.energy_gradient(x)
+    check_dot(lin, dot(d, gJ), "KE(m >= 1)")
+
+
+# ---------------------------------------------------------------- 128x64x64, 40 steps
+@pytest.fixture(scope="module")
+def M5():
+    import torch
+
+    torch.cuda.set_device(0)
+    g = H.Grid(CFG5)
+    m = A.March(g)
+    m.cfg = CFG5
+    yield m
+    m.close()
+    g.close()
+
+
+def cfg5_state(m):
+    """Lamb-Oseen vortex plus seeded noise of amplitude 1e-5 restricted to m = 1."""
+    cfg = m.cfg
+    psi, chi = m.grid.scalar(), m.grid.scalar()
+    H.qvortex(m.grid, LAMB_OSEEN_Q, psi, chi)
+    p0, _, ln = psi.download()
+    amp = cfg["noise"]
+    return m.state_from(p0 + seeded_noise(cfg, 1, amp, m_only=1), seeded_noise(cfg, 2, amp, m_only=1),
+                        seeded_noise(cfg, 3, amp, m_only=1), psi_ln=ln)
+
+
+def cfg5_direction(m, i):
+    import torch
+
+    psi, chi, b = fd_direction(m.cfg, i)
+    return m.state_from(psi, chi, b).to(torch.float64)
+
+
+def test_trajectory_gradient_fd():
+    """dJ/dx0 . d against (J(x0 + e d) - J(x0 - e d)) / 2e, e = 1e-6, along
+    16 seeded directions: noise of the workload's amplitude (1e-5) over
+    every kept coefficient of psi, chi and b, all m.  The central
+    difference's own error, e^2/6 of J's third derivative along d, is then
+    far below the bar; along directions of unit amplitude (1e5 times the
+    perturbation) it reaches 4e-5 at e = 1e-6, through the coupling of the
+    base flow to the perturbation."""
+    import torch
+
+    torch.cuda.set_device(0)
+    g = H.Grid(CFG5)
+    m = A.March(g)
+    m.cfg = CFG5
+    try:
+        x0 = cfg5_state(m)
+        nsteps = CFG5["nsteps"]
+        J, gJ = A.gradient(m, x0, nsteps, every=5)
+        print(f"J = {J:.15e}", flush=True)
+        eps = 1e-6
+        worst = 0.0
+        for i in range(16):
+            d = cfg5_direction(m, i)
+            jp = m.energy(A.run(m, x0 + eps * d, nsteps))
+            jm = m.energy(A.run(m, x0 - eps * d, nsteps))
+            fd = (jp - jm) / (2 * eps)
+            ad = dot(gJ, d)
+            e = abs(fd - ad) / abs(fd)
+            worst = max(worst, e)
+            print(f"direction {i}: FD {fd:.12e} adjoint {ad:.12e} rel {e:.2e}", flush=True)
+        assert worst <= 1e-6, worst
+    finally:
+        m.close()
+        g.close()
+
+
+def test_autograd_checkpointed_equals_reverse_sweep():
+    """torch.autograd through StepFunction/EnergyFunction with checkpointed
+    segments gives the reverse sweep's gradient bit for bit."""
+    import torch
+
+    torch.cuda.set_device(0)
+    cfg = dict(SMALL_AB, nsteps=12, noise=1e-4)
+    g = H.Grid(cfg)
+    m = A.March(g)
+    m.cfg = cfg
+    try:
+        x0 = state(m, 1e-4, history=False)
+        J, gJ = A.gradient(m, x0, 12, every=5)
+        xt = x0.clone().requires_grad_(True)
+        Jt = A.trajectory_energy(m, xt, 12, every=5)
+        Jt.backward()
+        assert float(Jt) == J
+        assert torch.equal(xt.grad, gJ)
+        # and a different checkpoint spacing leaves the gradient unchanged
+        _, gJ3 = A.gradient(m, x0, 12, every=3)
+        assert torch.equal(gJ3, gJ)
+    finally:
+        m.close()
+        g.close()
+
+
+def _golden():
+    import json
+    import os
+
+    with open(os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "adjoint_fd.json")) as fh:
+        return json.load(fh)
+
+
+def test_gradient_vs_oracle_tiny():
+    """The whole gradient on a 16x8x8 grid (6 steps) against the oracle's
+    central difference of every real degree of freedom
+    (tests/golden/adjoint_fd.json, make_adjoint_fd.py): max |gpu - oracle| /
+    max |oracle| <= 1e-6, and J itself to 1e-12."""
+    import torch
+
+    torch.cuda.set_device(0)
+    gold = _golden()
+    cfg = gold["tiny_config"]
+    ref = gold["tiny"]
+    g = H.Grid(cfg)
+    m = A.March(g)
+    m.cfg = cfg
+    try:
+        psi, chi = g.scalar(), g.scalar()
+        H.qvortex(g, LAMB_OSEEN_Q, psi, chi)
+        p0, _, ln = psi.download()
+        amp = cfg["noise"]
+        x0 = m.state_from(p0 + seeded_noise(cfg, 1, amp), seeded_noise(cfg, 2, amp), seeded_noise(cfg, 3, amp), psi_ln=ln)
+        J, gJ = A.gradient(m, x0, cfg["nsteps"], every=4)
+        print(f"J gpu {J:.15e} oracle {ref['J']:.15e}", flush=True)
+        assert abs(J - ref["J"]) <= 1e-12 * abs(ref["J"])
+        shape = tuple(ref["shape"])
+        (gp, gc, gb), (glp, glc) = m.fields_of(gJ)
+        worst, scale = 0.0, 0.0
+        for f, got in enumerate((gp, gc, gb)):
+            want = (np.array(ref["grad_re"][f]) + 1j * np.array(ref["grad_im"][f])).reshape(shape, order="F")
+            worst = max(worst, float(np.max(np.abs(got - want))))
+            scale = max(scale, float(np.max(np.abs(want))))
+        for got, want in zip((glp, glc), ref["grad_ln"]):
+            worst = max(worst, abs(got - want))
+        print(f"gradient: max |gpu - oracle FD| / max |oracle FD| = {worst / scale:.2e}", flush=True)
+        assert worst <= 1e-6 * scale
+    finally:
+        m.close()
+        g.close()
+
+
+def test_cfg5_gradient_vs_oracle_fd():
+    """<dJ/dx0, d_i> on the 128x64x64 workload against the oracle's central
+    differences along the same 16 directions (FP64 CPU reference), <= 1e-6;
+    J to 1e-10 of the oracle's."""
+    import torch
+
+    torch.cuda.set_device(0)
+    ref = _golden()["cfg5"]
+    g = H.Grid(CFG5)
+    m = A.March(g)
+    m.cfg = CFG5
+    try:
+        x0 = cfg5_state(m)
+        J, gJ = A.gradient(m, x0, CFG5["nsteps"], every=5)
+        print(f"J gpu {J:.15e} oracle {ref['J']:.15e}", flush=True)
+        assert abs(J - ref["J"]) <= 1e-10 * abs(ref["J"])
+        worst = 0.0
+        for i, fd in enumerate(ref["dJ"]):
+            ad = dot(gJ, cfg5_direction(m, i))
+            e = abs(ad - fd) / abs(fd)
+            worst = max(worst, e)
+            print(f"direction {i}: oracle FD {fd:.12e} gpu adjoint {ad:.12e} rel {e:.2e}", flush=True)
+        assert worst <= 1e-6, worst
+    finally:
+        m.close()
+        g.close()
