@@ -30,20 +30,6 @@ using namespace mlg;
     if (_r) return _r;         \
   } while (0)
 
-namespace mlg {
-int launch_adj_scale_pff(mlg_ctx *c, double2 *a, int mode, double s, int skip_m0);
-int launch_adj_scale_pfp(mlg_ctx *c, double2 *a, double s0, double s1);
-int launch_adj_products(mlg_ctx *c, double2 *const u[9], const double2 *const o[3], double2 *bn);
-int launch_adj_project(mlg_ctx *c, const double2 *psib, const double2 *chib, double2 *vr, double2 *vp, double2 *vz);
-int launch_adj_vel2pln(mlg_ctx *c, double2 *vp);
-int launch_adj_fixinf(mlg_ctx *c, double2 *a);
-int launch_adj_columns(mlg_ctx *c, int nrchop_out, const double2 *const in[9], double2 *psib, double *psib_ln, double2 *chib,
-                       double *chib_ln, double2 *bb);
-int launch_adj_del2t(mlg_ctx *c, int nrchop_out, const double2 *y, double s, double2 *out, double *ln_out);
-int launch_adj_lincomb(mlg_ctx *c, double2 *y, double a, const double2 *x1, double b1, const double2 *x2, double b2, long n);
-int launch_adj_ln_lincomb(mlg_ctx *c, double *y, double a, const double *x1, double b1, const double *x2, double b2);
-}  // namespace mlg
-
 namespace {
 
 // CHOPSET(iof) for the lifetime of the object
@@ -66,47 +52,89 @@ int check_supported(const mlg_ctx *c) {
   return 0;
 }
 
-// transpose of TOFP (RTRAN, VERFFT, HORFFT backward): a PPP adjoint becomes
-// the FFF adjoint at the current truncation.  zln: an LN of value 0.
-int tofp_adj(mlg_field *f, mlg_field *t) {
-  mlg_ctx *c = f->ctx;
+// Fields of one batch swap storage with scratch after an out-of-place
+// transform (only the element buffers move; spaces are set by the caller).
+void swap_n(int n, mlg_field *const *a, mlg_field *const *t) {
+  for (int i = 0; i < n; i++) swap_e(a[i], t[i]);
+}
+
+// transpose of TOFP (RTRAN, VERFFT, HORFFT backward) for n PPP adjoints:
+// HORFFT bwd^T = 2 NTH D2 HORFFT fwd (D2 = 1 at m = 0, 2 above), VERFFT
+// bwd^T = NX VERFFT fwd, RTRAN bwd^T = the Gauss sums without weights.
+// nrchop[i]: the truncation of field i's RTRAN (its TOFP's); t: n scratch.
+int tofp_adj_n(mlg_ctx *c, int n, mlg_field *const *f, mlg_field *const *t, const int *nrchop) {
   const Grid &G = c->g;
-  RET(launch_horfft_fwd(c, f->e));  // HORFFT bwd^T = 2 NTH D2 HORFFT fwd
-  const double s = 2.0 * G.nth * G.nx;  // and VERFFT bwd^T = NX VERFFT fwd
-  RET(launch_adj_scale_pfp(c, f->e, s, 2.0 * s));
-  RET(launch_verfft_fwd(c, f->e, t->e));
-  swap_e(f, t);
-  RET(launch_adj_scale_pff(c, f->e, 1, 1.0, 0));  // RTRAN bwd^T = RTRAN fwd of f / w
-  RET(launch_rtran_fwd(c, f->e, t->e, nullptr));
-  swap_e(f, t);
-  f->space = MLG_FFF;
+  double2 *e[MAX_HOR_JOBS];
+  for (int i = 0; i < n; i++) e[i] = f[i]->e;
+  const double s = 2.0 * G.nth * G.nx;
+  RET(launch_horfft_fwd_n(c, n, e, s, 2.0 * s));
+  for (int i0 = 0; i0 < n; i0 += MAX_TRANSFORM_JOBS) {
+    const int m = n - i0 < MAX_TRANSFORM_JOBS ? n - i0 : MAX_TRANSFORM_JOBS;
+    const double2 *in[MAX_TRANSFORM_JOBS];
+    double2 *out[MAX_TRANSFORM_JOBS];
+    for (int i = 0; i < m; i++) in[i] = f[i0 + i]->e, out[i] = t[i0 + i]->e;
+    RET(launch_verfft_fwd_n(c, m, in, out));
+  }
+  swap_n(n, f, t);
+  // RTRAN forward without weights, fields of one truncation per launch
+  for (int i0 = 0; i0 < n;) {
+    int m = 1;
+    while (i0 + m < n && m < MAX_TRANSFORM_JOBS && nrchop[i0 + m] == nrchop[i0]) m++;
+    const double2 *in[MAX_TRANSFORM_JOBS];
+    double2 *out[MAX_TRANSFORM_JOBS];
+    for (int i = 0; i < m; i++) in[i] = f[i0 + i]->e, out[i] = t[i0 + i]->e;
+    const int d = nrchop[i0] - c->g.nrchop;
+    if (d) RET(mlg_chopset(c, d));
+    const int rc = launch_rtran_fwd_n(c, m, in, out, c->d.ones);
+    if (d) mlg_chopset(c, -d);
+    RET(rc);
+    i0 += m;
+  }
+  for (int i = 0; i < n; i++) f[i]->space = MLG_FFF;
+  swap_n(n, f, t);
   return 0;
 }
 
-// transpose of TOFF (HORFFT, VERFFT, RTRAN forward): an FFF adjoint becomes
-// the PPP adjoint
-int toff_adj(mlg_field *f, mlg_field *t, const double *zln) {
-  mlg_ctx *c = f->ctx;
+// transpose of TOFF (HORFFT, VERFFT, RTRAN forward) for n FFF adjoints into
+// the PPP fields out: RTRAN fwd^T = w RTRAN bwd (w folded into the HORFFT
+// load), VERFFT fwd^T = VERFFT bwd / NX, HORFFT fwd^T = HORFFT bwd D(1/2) /
+// (2 NTH).  t: n scratch; zln: an LN of value 0.
+int toff_adj_n(mlg_ctx *c, int n, const mlg_field *const *in, mlg_field *const *out, mlg_field *const *t, const double *zln) {
   const Grid &G = c->g;
-  RET(launch_rtran_bwd(c, f->e, zln, t->e));  // RTRAN fwd^T = w RTRAN bwd
-  swap_e(f, t);
-  RET(launch_adj_scale_pff(c, f->e, 0, 1.0, 0));
-  RET(launch_verfft_bwd(c, f->e, t->e));  // VERFFT fwd^T = VERFFT bwd / NX
-  swap_e(f, t);
-  const double s = 1.0 / (2.0 * G.nth * G.nx);  // HORFFT fwd^T = HORFFT bwd D(1/2) / (2 NTH)
-  RET(launch_adj_scale_pfp(c, f->e, s, 0.5 * s));
-  RET(launch_horfft_bwd(c, f->e));
-  f->space = MLG_PPP;
+  const double2 *ie[MAX_TRANSFORM_JOBS];
+  const double *lz[MAX_TRANSFORM_JOBS];
+  double2 *te[MAX_TRANSFORM_JOBS], *oe[MAX_TRANSFORM_JOBS];
+  if (n > MAX_TRANSFORM_JOBS) return MLG_E_ARG;
+  for (int i = 0; i < n; i++) ie[i] = in[i]->e, lz[i] = zln, te[i] = t[i]->e, oe[i] = out[i]->e;
+  RET(launch_rtran_bwd_n(c, n, ie, lz, te));
+  const double2 *tc[MAX_TRANSFORM_JOBS];
+  for (int i = 0; i < n; i++) tc[i] = te[i];
+  RET(launch_verfft_bwd_n(c, n, tc, oe));
+  const double s = 1.0 / (2.0 * G.nth * G.nx);
+  RET(launch_horfft_bwd_n(c, n, oe, s, 0.5 * s, c->d.w));
+  for (int i = 0; i < n; i++) out[i]->space = MLG_PPP;
   return 0;
 }
 
-// x <- IHELM^T(x) with LN (x: adjoint of IHELM's result in, of its
-// right-hand side out), at the current truncation
+// transpose of PROJECT from PPP with VEL2PLN (psi%LN set, as
+// CALC_BOUSSI_FORCE calls it): psi, chi2 adjoints (FFF) into three PPP
+// adjoints; v: three PFF scratch fields
+int project_adj(mlg_ctx *c, const mlg_field *psib, const mlg_field *chib, mlg_field *const *out, mlg_field *const *v) {
+  RET(launch_adj_project(c, psib->e, chib->e, v[0]->e, v[1]->e, v[2]->e));
+  RET(launch_adj_vel2pln(c, v[1]->e));
+  const double2 *ve[3] = {v[0]->e, v[1]->e, v[2]->e};
+  double2 *oe[3] = {out[0]->e, out[1]->e, out[2]->e};
+  RET(launch_verfft_bwd_n(c, 3, ve, oe));
+  const double s = 1.0 / (2.0 * c->g.nth * c->g.nx);
+  RET(launch_horfft_bwd_n(c, 3, oe, s, 0.5 * s, nullptr));
+  for (int i = 0; i < 3; i++) out[i]->space = MLG_PPP;
+  return 0;
+}
+
+// x <- IHELM^T(x) with LN, at the current truncation
 int ihelm_adj(mlg_field *x, double alp) {
-  mlg_ctx *c = x->ctx;
-  RET(launch_chopdo(c, x->e, MLG_FFF));
-  RET(launch_band_solve_t(c, 2, alp, x->e, x->ln));
-  return launch_chopdo(c, x->e, MLG_FFF);
+  BandTJob j{x->e, x->e, x->ln, x->ln, alp, nullptr};
+  return launch_band_solve_t_n(x->ctx, 2, 1, &j);
 }
 
 // x <- VISC1^T(x): VISC1 is A = IHELM(-ALP A, ALP), ALP = 1/(NU DT)
@@ -124,68 +152,56 @@ int force_adj(const mlg_field *psi, const mlg_field *chi, const mlg_field *b, co
               const mlg_field *bnb, mlg_field *psib, mlg_field *chib, mlg_field *bb) {
   mlg_ctx *c = psi->ctx;
   Temps T(c);
-  mlg_field *u[9];
-  for (auto &f : u) f = T.get();
-  mlg_field *o[3] = {T.get(), T.get(), T.get()};
-  mlg_field *bn = T.get(), *t = T.buffer(), *z = T.get();
+  mlg_field *u[10], *t[10];
+  for (auto &f : u) f = T.get();  // u[0..8]: u, om, grad b; u[9]: the adjoint of bn's and W's physical field
+  for (auto &f : t) f = T.buffer();
+  mlg_field *w2 = T.buffer(), *z = T.get();
   RET(T.ready());
   const int nrchop = c->g.nrchop;
   // the products' inputs at the state, in physical space (CHOPSET(3) as in
-  // CALC_BOUSSI_FORCE)
+  // CALC_BOUSSI_FORCE): PC2VEL + PC2VOR, GRAD, then TOFP of all nine
   {
     Chop ch(c, 3);
-    RET(mlg_grad(b, u[6], u[7], u[8]));
-    RET(mlg_pc2vel(psi, chi, u[0], u[1], u[2]));
-    RET(mlg_pc2vor(psi, chi, u[3], u[4], u[5]));
-    for (auto *f : u) RET(mlg_tofp(f));
+    VelVorOut vo;
+    for (int i = 0; i < 6; i++) vo.e[i] = u[i]->e, vo.ln[i] = u[i]->ln;
+    RET(launch_pc2velvor(c, psi->e, psi->ln, chi->e, chi->ln, vo));
+    RET(launch_grad(c, b->e, b->ln, u[6]->e, u[6]->ln, u[7]->e, u[7]->ln, u[8]->e, u[8]->ln));
+    const double2 *in[9];
+    const double *ln[9];
+    double2 *out[9];
+    for (int i = 0; i < 9; i++) in[i] = u[i]->e, ln[i] = u[i]->ln, out[i] = t[i]->e;
+    RET(launch_rtran_bwd_n(c, 9, in, ln, out));
+    swap_n(9, u, t);
+    for (int i = 0; i < 9; i++) in[i] = u[i]->e, out[i] = t[i]->e;
+    RET(launch_verfft_bwd_n(c, 9, in, out));
+    swap_n(9, u, t);
+    double2 *ue9[9];
+    for (int i = 0; i < 9; i++) ue9[i] = u[i]->e;
+    RET(launch_horfft_bwd_n(c, 9, ue9, 1.0, 1.0, nullptr));
+    for (int i = 0; i < 9; i++) u[i]->space = MLG_PPP;
   }
   // bn = TOFF(bn_phys)
-  RET(copy_field(bn, bnb));
-  RET(toff_adj(bn, t, z->ln));
-  // chin = IDEL2LN(w2): CHOPDO then the fix at infinity, after the solve
-  mlg_field *x = o[0];
-  RET(copy_field(x, chinb));
-  RET(launch_adj_fixinf(c, x->e));
-  RET(launch_chopdo(c, x->e, MLG_FFF));
-  RET(launch_band_solve_t(c, 1, 0.0, x->e, x->ln));
-  RET(launch_chopdo(c, x->e, MLG_FFF));
+  RET(toff_adj_n(c, 1, &bnb, &u[9], &t[0], z->ln));
+  // chin = IDEL2LN(w2), then CHOPDO and the fix at infinity
+  BandTJob j{chinb->e, w2->e, chinb->ln, nullptr, 0.0, nullptr};
+  RET(launch_band_solve_t_n(c, 1, 1, &j));
   // (psin, w2) = PROJECT(N_r, N_theta, N_z) from PPP, psi's fix at infinity last
-  mlg_field *p = o[1];
-  RET(copy_field(p, psinb));
-  RET(launch_adj_fixinf(c, p->e));
-  RET(launch_chopdo(c, p->e, MLG_FFF));
-  mlg_field *v[3] = {t, o[2], T.get()};
-  if (!v[2]) return MLG_E_NOMEM;
-  RET(launch_adj_project(c, p->e, x->e, v[0]->e, v[1]->e, v[2]->e));
-  RET(launch_adj_vel2pln(c, v[1]->e));
-  // VERFFT and HORFFT forward of each of the three
-  const Grid &G = c->g;
-  const double s = 1.0 / (2.0 * G.nth * G.nx);
-  mlg_field *ob[3] = {x, p, nullptr};
-  {
-    // results into x, p and a third buffer; v[0] (= t) is the first input
-    ob[2] = T.get();
-    RET(T.ready());
-    for (int i = 0; i < 3; i++) {
-      RET(launch_verfft_bwd(c, v[i]->e, ob[i]->e));
-      RET(launch_adj_scale_pfp(c, ob[i]->e, s, 0.5 * s));
-      RET(launch_horfft_bwd(c, ob[i]->e));
-      ob[i]->space = MLG_PPP;
-    }
-  }
+  mlg_field *ob[3] = {t[3], t[4], t[5]};
+  RET(project_adj(c, psinb, w2, ob, &t[0]));
   // the pointwise terms
   double2 *ue[9];
   for (int i = 0; i < 9; i++) ue[i] = u[i]->e;
   const double2 *oe[3] = {ob[0]->e, ob[1]->e, ob[2]->e};
-  RET(launch_adj_products(c, ue, oe, bn->e));
-  // W = TOFP(B) at NRCHOP: its adjoint (in bn) back to b
-  RET(tofp_adj(bn, t));
-  RET(launch_adj_lincomb(c, bb->e, 1.0, bn->e, 1.0, nullptr, 0.0, c->n_fff));
-  // the velocity, vorticity and gradient adjoints back through TOFP and
+  RET(launch_adj_products(c, ue, oe, u[9]->e));
+  // TOFP^T of the nine (at NRCHOP+3) and of W (at NRCHOP)
+  int chops[10];
+  for (int i = 0; i < 9; i++) chops[i] = nrchop + 3;
+  chops[9] = nrchop;
+  RET(tofp_adj_n(c, 10, u, t, chops));
+  RET(launch_adj_lincomb(c, bb->e, 1.0, u[9]->e, 1.0, nullptr, 0.0, c->n_fff));
   // PC2VEL, PC2VOR, GRAD at NRCHOP+3, into psi, chi, b rows of NRCHOP
   {
     Chop ch(c, 3);
-    for (auto *f : u) RET(tofp_adj(f, t));
     const double2 *ie[9];
     for (int i = 0; i < 9; i++) ie[i] = u[i]->e;
     RET(launch_adj_columns(c, nrchop, ie, psib->e, psib->ln, chib->e, chib->ln, bb->e));
@@ -225,43 +241,38 @@ extern "C" int mlg_step_adjoint(mlg_solver *s, const mlg_field *psi, const mlg_f
   RET(check_supported(c));
   const mlg_params &P = c->g.p;
   const double dt = P.dt;
-  const long n = c->n_fff;
   mlg_field *ub[3] = {psi_bar, chi_bar, b_bar}, *hb[3] = {hpsi_bar, hchi_bar, hb_bar};
   const double nu[3] = {P.nu, P.nu, P.kappa};
   const bool cn[3] = {P.visc_sw == 1, P.visc_sw == 1, P.visc_sw == 1 && P.kappa != 0.0};
   Temps T(c);
-  mlg_field *st[3] = {T.get(), T.get(), T.get()}, *nb[3] = {T.get(), T.get(), T.get()};
+  mlg_field *wk[3] = {T.get(), T.get(), T.get()}, *nb[3] = {T.get(), T.get(), T.get()};
   RET(T.ready());
-  RET(launch_set_ln(c, b_bar->ln, nullptr, 0.0, 0.0));
+  // wk = IHELM^T(ubar) for the fields with CN, one launch
+  BandTJob jobs[3];
+  int nj = 0;
+  double alp[3] = {0.0, 0.0, 0.0};
   for (int f = 0; f < 3; f++) {
-    if (cn[f]) {
-      if (nu[f] == 0.0) return 5;
-      const double alp = 2 / nu[f] / dt;
-      // wk = IHELM^T(ub) in ub; u*bar = -ALP wk; ubar = u*bar - DEL2^T(wk) (CHOPSET(2), rows of NRCHOP)
-      RET(ihelm_adj(ub[f], alp));
-      RET(copy_field(st[f], ub[f]));
-      RET(launch_adj_lincomb(c, st[f]->e, -alp, nullptr, 0.0, nullptr, 0.0, n));
-      RET(launch_adj_ln_lincomb(c, st[f]->ln, -alp, nullptr, 0.0, nullptr, 0.0));
-      mlg_field *wk = nb[f];
-      RET(copy_field(wk, ub[f]));
-      RET(copy_field(ub[f], st[f]));
-      const int nrchop = c->g.nrchop;
-      Chop ch(c, 2);
-      RET(launch_adj_del2t(c, nrchop, wk->e, -1.0, ub[f]->e, f < 2 ? ub[f]->ln : nullptr));
-    } else {
-      RET(copy_field(st[f], ub[f]));
-    }
-    // N bar = 1.5 dt u*bar + CHOPDO(hbar); N_old bar = -0.5 dt u*bar
-    RET(copy_field(nb[f], hb[f]));
-    RET(launch_adj_lincomb(c, nb[f]->e, 1.0, st[f]->e, 1.5 * dt, nullptr, 0.0, n));
-    RET(launch_adj_ln_lincomb(c, nb[f]->ln, 1.0, st[f]->ln, 1.5 * dt, nullptr, 0.0));
-    RET(launch_adj_lincomb(c, hb[f]->e, 0.0, st[f]->e, -0.5 * dt, nullptr, 0.0, n));
-    RET(launch_adj_ln_lincomb(c, hb[f]->ln, 0.0, st[f]->ln, -0.5 * dt, nullptr, 0.0));
+    if (!cn[f]) continue;
+    if (nu[f] == 0.0) return 5;
+    alp[f] = 2 / nu[f] / dt;
+    jobs[nj++] = BandTJob{ub[f]->e, wk[f]->e, ub[f]->ln, wk[f]->ln, alp[f], nullptr};
   }
-  RET(launch_set_ln(c, nb[2]->ln, nullptr, 0.0, 0.0));
-  RET(launch_set_ln(c, hb_bar->ln, nullptr, 0.0, 0.0));
+  if (nj) RET(launch_band_solve_t_n(c, 2, nj, jobs));
+  // ubar = u*bar - DEL2^T(wk), Nbar = CHOPDO(hbar) + 1.5 dt u*bar, hbar = -0.5 dt u*bar, one launch
+  AdjCnField fl[3];
+  for (int f = 0; f < 3; f++) {
+    const bool ln = f < 2;
+    fl[f] = AdjCnField{wk[f]->e, wk[f]->ln, ub[f]->e, hb[f]->e, nb[f]->e, ln ? ub[f]->ln : nullptr,
+                       ln ? hb[f]->ln : nullptr, ln ? nb[f]->ln : nullptr, alp[f], cn[f] ? 1 : 0};
+  }
+  {
+    const int nrchop = c->g.nrchop;
+    Chop ch(c, 2);
+    RET(launch_adj_cn(c, nrchop, 3, fl, dt));
+  }
   RET(force_adj(psi, chi, b, nb[0], nb[1], nb[2], psi_bar, chi_bar, b_bar));
-  RET(launch_set_ln(c, b_bar->ln, nullptr, 0.0, 0.0));
+  double *lz[2] = {b_bar->ln, hb_bar->ln};
+  RET(launch_zero_lns(c, lz, 2));
   return mlg_synchronize(c);
 }
 
@@ -435,46 +446,33 @@ extern "C" int mlg_adjoint_op(int op, mlg_field *const *f, int nf, double alp) {
   if (c->leg_split) return MLG_E_ARG;
   Temps T(c);
   mlg_field *t = T.buffer(), *z = T.get();
+  mlg_field *v[3] = {T.buffer(), T.buffer(), T.buffer()};
   RET(T.ready());
   switch (op) {
-    case 0:
+    case 0: {
       if (f[0]->space != MLG_PPP) return 2;
-      RET(tofp_adj(f[0], t));
-      break;
-    case 1:
-      if (f[0]->space != MLG_FFF) return 2;
-      RET(toff_adj(f[0], t, z->ln));
-      break;
-    case 2: {
-      Temps W(c);
-      mlg_field *p = W.get(), *x = W.get(), *v1 = W.buffer(), *v2 = W.buffer();
-      RET(W.ready());
-      RET(copy_field(p, f[0]));
-      RET(copy_field(x, f[1]));
-      RET(launch_adj_fixinf(c, p->e));
-      RET(launch_chopdo(c, p->e, MLG_FFF));
-      RET(launch_chopdo(c, x->e, MLG_FFF));
-      RET(launch_adj_project(c, p->e, x->e, t->e, v1->e, v2->e));
-      RET(launch_adj_vel2pln(c, v1->e));
-      const double s = 1.0 / (2.0 * c->g.nth * c->g.nx);
-      mlg_field *v[3] = {t, v1, v2};
-      for (int i = 0; i < 3; i++) {
-        RET(launch_verfft_bwd(c, v[i]->e, f[2 + i]->e));
-        RET(launch_adj_scale_pfp(c, f[2 + i]->e, s, 0.5 * s));
-        RET(launch_horfft_bwd(c, f[2 + i]->e));
-        f[2 + i]->space = MLG_PPP;
-        RET(launch_set_ln(c, f[2 + i]->ln, nullptr, 0.0, 0.0));
-      }
+      const int ch = c->g.nrchop;
+      RET(tofp_adj_n(c, 1, &f[0], &t, &ch));
       break;
     }
-    case 3:
-      RET(copy_field(f[1], f[0]));
-      RET(launch_adj_fixinf(c, f[1]->e));
-      RET(launch_chopdo(c, f[1]->e, MLG_FFF));
-      RET(launch_band_solve_t(c, 1, 0.0, f[1]->e, f[1]->ln));
-      RET(launch_chopdo(c, f[1]->e, MLG_FFF));
+    case 1: {
+      if (f[0]->space != MLG_FFF) return 2;
+      RET(copy_field(v[0], f[0]));
+      const mlg_field *in = v[0];
+      RET(toff_adj_n(c, 1, &in, &f[0], &t, z->ln));
+      break;
+    }
+    case 2:
+      RET(project_adj(c, f[0], f[1], &f[2], v));
+      for (int i = 0; i < 3; i++) RET(launch_set_ln(c, f[2 + i]->ln, nullptr, 0.0, 0.0));
+      break;
+    case 3: {
+      BandTJob j{f[0]->e, f[1]->e, f[0]->ln, nullptr, 0.0, nullptr};
+      RET(launch_band_solve_t_n(c, 1, 1, &j));
+      f[1]->space = MLG_FFF;
       RET(launch_set_ln(c, f[1]->ln, nullptr, 0.0, 0.0));
       break;
+    }
     case 4:
       if (alp == 0.0) return 3;
       RET(copy_field(f[1], f[0]));

@@ -209,6 +209,7 @@ extern "C" int mlg_init(const mlg_params *p, mlg_ctx **out) {
   RET(upload(&d.pf, G.pf));
   RET(upload(&d.vdt, G.vdt));
   RET(upload(&d.w, G.w));
+  RET(upload(&d.ones, std::vector<double>(G.nr, 1.0)));
   RET(upload(&d.r, G.r));
   RET(upload(&d.x, G.x));
   RET(upload(&d.lnx, G.lnx));
@@ -286,13 +287,15 @@ extern "C" int mlg_finalize(mlg_ctx *c) {
   for (hipEvent_t e : c->timer.spare) (void)hipEventDestroy(e);
   c->timer.spare.clear();
   DevTables &d = c->d;
-  void *ptrs[] = {d.pf, d.vdt, d.w, d.r, d.x, d.lnx, d.at1, d.ak, d.norm, d.lognorm, d.bxm, d.bxxdx, d.bdel2h,
+  void *ptrs[] = {d.pf, d.vdt, d.w, d.ones, d.r, d.x, d.lnx, d.at1, d.ak, d.norm, d.lognorm, d.bxm, d.bxxdx, d.bdel2h,
                   d.tw_z, d.tw_t, d.tw_h, d.hv, d.red, d.vproj, d.cvdt, d.err,
                   d.pf_sb, d.pf_eb, d.pf_sf, d.pf_ef, d.vdt_s, d.vdt_e, d.cvdt_s, d.cvdt_e};
   for (auto &f : c->fcache) (void)hipFree(f.f);
   c->fcache.clear();
   for (auto &v : c->m0vecs) (void)hipFree(v.second);
   c->m0vecs.clear();
+  for (auto &v : c->adjv) (void)hipFree(v.second);
+  c->adjv.clear();
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);

@@ -12,6 +12,8 @@
 // k_adj_scale_pfp), because the forward transforms are a weighted Gauss sum
 // (RTRAN), an FFT scaled by 1/NX (VERFFT) and a real FFT of 2 NTH samples
 // scaled by 1/(2 NTH) that keeps only m < NTCHOP (HORFFT).
+#include <cmath>
+
 #include "mlegs_internal.h"
 
 namespace mlg {
@@ -134,58 +136,97 @@ __global__ void k_adj_products(KGrid g, const double *r2, double two_omega, doub
 //   chi2 = i k OE(D, ru_r) + m k EO(V, ru_theta) - EO(T, u_z)
 // In: psi, chi2 adjoints (FFF, chopped); out: the three PFF adjoints (all
 // rows written).  V, D, T: vdt[(t NTCHOPDIM + m) NRCHOPDIM + n][j], j < NRH.
-__global__ void __launch_bounds__(64) k_adj_project(KGrid g, const double *vdt, const double *ak, const double2 *psib,
-                                                    const double2 *chib, double2 *vr, double2 *vp, double2 *vz) {
-  const int mm = blockIdx.x, k = blockIdx.y;
+// The psi adjoint is read through the transposes of CHOPDO and of PROJECT's
+// fix at infinity of m = 0 (psibar(n) - AT1(n)/AT1(0) psibar(0)), chi2's
+// through CHOPDO's.  Per (m, k) and node pair (j, NR-1-j), with the
+// coefficient factors a1 = i m psibar, a2 = -i k chi2bar, a3 = -psibar,
+// a4 = m k chi2bar, a5 = -chi2bar and S_e / S_o sums over even / odd n - m:
+//   ru_r:  row j  Ve a1 + Vo a1 + De a2 + Do a2,   row NR-1-j  (Ve - Vo) a1 + (Do - De) a2
+//   ru_t:  row j  De a3 + Do a3 + Ve a4 + Vo a4,   row NR-1-j  (Do - De) a3 + (Ve - Vo) a4
+//   u_z:   row j  Te a5 + To a5,                   row NR-1-j  (Te - To) a5
+// A block takes one m and ADJ_PKB consecutive k: the factors go to LDS, each
+// thread a node j, and a table row is loaded once for the ADJ_PKB columns.
+#define ADJ_PKB 4
+__global__ void __launch_bounds__(64) k_adj_project(KGrid g, const double *vdt, const double *ak, const double *at1,
+                                                    const double2 *psib, const double2 *chib, double2 *vr, double2 *vp,
+                                                    double2 *vz) {
+  extern __shared__ __align__(16) double2 fs[];  // [ADJ_PKB][5][nn]
+  const int mm = blockIdx.x, k0 = blockIdx.y * ADJ_PKB;
   const int nn = mm < g.ntchop ? a_nrchops(g.nrchop, mm * g.minc) : 0;
-  const double mv = (double)(mm * g.minc), kv = ak[mm + (size_t)g.ntchopdim * k];
-  const size_t fcol = (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
-  const size_t pcol = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
+  const double mv = (double)(mm * g.minc);
   const size_t tstride = (size_t)g.ntchopdim * g.nrchopdim * g.nrh;
-  for (int r = g.nr + threadIdx.x; r < g.ndimr; r += blockDim.x) vr[pcol + r] = vp[pcol + r] = vz[pcol + r] = z2();
-  for (int j = threadIdx.x; j < g.nrh; j += blockDim.x) {
-    // p: row j, q: row NR-1-j
-    double2 pr = z2(), qr = z2(), pp = z2(), qp = z2(), pz = z2(), qz = z2();
-    for (int ii = 0; ii < nn; ii++) {
-      const size_t tb = ((size_t)mm * g.nrchopdim + ii) * g.nrh + j;
-      const double V = vdt[tb], D = vdt[tstride + tb], T = vdt[2 * tstride + tb];
-      const double2 P = psib[fcol + ii], X = chib[fcol + ii];
-      const double se = (ii & 1) ? -1.0 : 1.0;  // EO's sign of the mirrored node
-      // ru_r: EO(V) with i m P, OE(D) with -i k X
-      const double2 e1 = sc(V, isc(mv, P)), o1 = sc(D, isc(-kv, X));
-      pr = pl(pr, pl(e1, o1));
-      qr = pl(qr, mi(sc(se, e1), sc(se, o1)));
-      // ru_theta: OE(D) with -P, EO(V) with m k X
-      const double2 o2 = sc(-D, P), e2 = sc(V * mv * kv, X);
-      pp = pl(pp, pl(o2, e2));
-      qp = pl(qp, mi(sc(se, e2), sc(se, o2)));
-      // u_z: EO(T) with -X
-      const double2 e3 = sc(-T, X);
-      pz = pl(pz, e3);
-      qz = pl(qz, sc(se, e3));
+  const int nk = g.nxchopdim - k0 < ADJ_PKB ? g.nxchopdim - k0 : ADJ_PKB;
+  for (int i = threadIdx.x; i < ADJ_PKB * nn; i += blockDim.x) {
+    const int q = i / (nn > 0 ? nn : 1), ii = i - q * nn;
+    double2 P = z2(), X = z2();
+    double kv = 0.0;
+    if (q < nk) {
+      const size_t fcol = (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * (k0 + q));
+      P = mm == 0 ? mi(psib[fcol + ii], sc(at1[ii] / at1[0], psib[fcol])) : psib[fcol + ii];
+      X = chib[fcol + ii];
+      kv = ak[mm + (size_t)g.ntchopdim * (k0 + q)];
     }
-    vr[pcol + j] = pr, vr[pcol + g.nr - 1 - j] = qr;
-    vp[pcol + j] = pp, vp[pcol + g.nr - 1 - j] = qp;
-    vz[pcol + j] = pz, vz[pcol + g.nr - 1 - j] = qz;
+    double2 *f = fs + (size_t)q * 5 * nn;
+    f[ii] = isc(mv, P);
+    f[nn + ii] = isc(-kv, X);
+    f[2 * nn + ii] = sc(-1.0, P);
+    f[3 * nn + ii] = sc(mv * kv, X);
+    f[4 * nn + ii] = sc(-1.0, X);
+  }
+  __syncthreads();
+  for (int q = 0; q < nk; q++) {
+    const size_t pcol = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * (k0 + q));
+    for (int r = g.nr + threadIdx.x; r < g.ndimr; r += blockDim.x) vr[pcol + r] = vp[pcol + r] = vz[pcol + r] = z2();
+  }
+  for (int j = threadIdx.x; j < g.nrh; j += blockDim.x) {
+    // [column][e/o] sums: r1 = V a1, r2 = D a2, p3 = D a3, p4 = V a4, z5 = T a5
+    double2 r1[ADJ_PKB][2], r2[ADJ_PKB][2], p3[ADJ_PKB][2], p4[ADJ_PKB][2], z5[ADJ_PKB][2];
+#pragma unroll
+    for (int q = 0; q < ADJ_PKB; q++)
+#pragma unroll
+      for (int e = 0; e < 2; e++) r1[q][e] = r2[q][e] = p3[q][e] = p4[q][e] = z5[q][e] = z2();
+    // rows in pairs (even n - m, then odd), so the parity index is static
+    for (int i2 = 0; i2 < nn; i2 += 2) {
+#pragma unroll
+      for (int e = 0; e < 2; e++) {
+        const int ii = i2 + e;
+        if (ii >= nn) break;
+        const size_t tb = ((size_t)mm * g.nrchopdim + ii) * g.nrh + j;
+        const double V = vdt[tb], D = vdt[tstride + tb], T = vdt[2 * tstride + tb];
+#pragma unroll
+        for (int q = 0; q < ADJ_PKB; q++) {
+          const double2 *f = fs + (size_t)q * 5 * nn;
+          r1[q][e] = pl(r1[q][e], sc(V, f[ii]));
+          r2[q][e] = pl(r2[q][e], sc(D, f[nn + ii]));
+          p3[q][e] = pl(p3[q][e], sc(D, f[2 * nn + ii]));
+          p4[q][e] = pl(p4[q][e], sc(V, f[3 * nn + ii]));
+          z5[q][e] = pl(z5[q][e], sc(T, f[4 * nn + ii]));
+        }
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < ADJ_PKB; q++) {
+      if (q >= nk) break;
+      const size_t pcol = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * (k0 + q));
+      vr[pcol + j] = pl(pl(r1[q][0], r1[q][1]), pl(r2[q][0], r2[q][1]));
+      vr[pcol + g.nr - 1 - j] = pl(mi(r1[q][0], r1[q][1]), mi(r2[q][1], r2[q][0]));
+      vp[pcol + j] = pl(pl(p3[q][0], p3[q][1]), pl(p4[q][0], p4[q][1]));
+      vp[pcol + g.nr - 1 - j] = pl(mi(p3[q][1], p3[q][0]), mi(p4[q][0], p4[q][1]));
+      vz[pcol + j] = pl(z5[q][0], z5[q][1]);
+      vz[pcol + g.nr - 1 - j] = mi(z5[q][0], z5[q][1]);
+    }
   }
 }
 
 // Transpose of VEL2PLN (src/mod_legops.f90:2277-2339) on the (m = 0,
 // k = 0) column of ru_theta in PFF.  Forward: LN = -s/2 with
-// s = sum_j sigma(j) Re ru(j), then ru(j) += LN c(j), where for i < NRH
-//   c(i) = PF(i,0)/N(0) + PF(i,1)/N(1),   c(NR-1-i) = PF(i,0)/N(0) - PF(i,1)/N(1)
-//   sigma(i) = w(i) sum_n e^{LOGNORM(n)} PF(i,n),
-//   sigma(NR-1-i) = w(i) sum_n (-1)^n e^{LOGNORM(n)} PF(i,n)   (n < NRCHOPS(0)).
+// s = sum_j sigma(j) Re ru(j), then ru(j) += LN c(j) (the vectors c, sigma
+// are cs[0..NR-1], cs[NR..2NR-1], built on the host for the truncation).
 // Adjoint: Re rubar(j) -= sigma(j)/2 * sum_l c(l) Re rubar(l).  One block.
-__global__ void __launch_bounds__(256) k_adj_vel2pln(KGrid g, const double *pf, const double *w, const double *lognorm,
-                                                     double2 *vp) {
+__global__ void __launch_bounds__(256) k_adj_vel2pln(KGrid g, const double *cs, double2 *vp) {
   __shared__ double red[256];
-  const int nn = a_nrchops(g.nrchop, 0);
   double part = 0.0;
-  for (int i = threadIdx.x; i < g.nrh; i += blockDim.x) {
-    const double p0 = pf[i], p1 = pf[(size_t)g.nrh + i];
-    part += (p0 / g.norm0 + p1 / g.norm1) * vp[i].x + (p0 / g.norm0 - p1 / g.norm1) * vp[g.nr - 1 - i].x;
-  }
+  for (int i = threadIdx.x; i < g.nr; i += blockDim.x) part += cs[i] * vp[i].x;
   red[threadIdx.x] = part;
   __syncthreads();
   for (int s = blockDim.x / 2; s > 0; s >>= 1) {
@@ -193,16 +234,7 @@ __global__ void __launch_bounds__(256) k_adj_vel2pln(KGrid g, const double *pf, 
     __syncthreads();
   }
   const double cdot = red[0];
-  for (int i = threadIdx.x; i < g.nrh; i += blockDim.x) {
-    double se = 0.0, so = 0.0;
-    for (int n = 0; n < nn; n++) {
-      const double t = exp(lognorm[n]) * pf[(size_t)n * g.nrh + i];
-      se += t;
-      so += (n & 1) ? -t : t;
-    }
-    vp[i].x -= 0.5 * w[i] * se * cdot;
-    vp[g.nr - 1 - i].x -= 0.5 * w[i] * so * cdot;
-  }
+  for (int i = threadIdx.x; i < g.nr; i += blockDim.x) vp[i].x -= 0.5 * cs[g.nr + i] * cdot;
 }
 
 // Transpose of the value-at-infinity fix-up of m = 0 (CALCAT1,
@@ -344,6 +376,67 @@ __global__ void k_adj_del2t(KGrid g, int nrchop_out, const double *bdel2h, const
   }
 }
 
+// Transpose of the AB2 update and CN viscosity of STEP_BOUSSI_AB_CN for up
+// to three fields in one launch (grid row = field).  Forward, per field:
+//   u* = u + dt (1.5 N - 0.5 N_old);  N_old' = N;
+//   u' = IHELM(-DEL2(u) - ALP u*, ALP) (DEL2 at NRCHOP+2, chopped to NRCHOP), or u' = u* without CN.
+// In: wk = IHELM^T(u' bar) (CN fields), ub = u' bar (others), hb = N_old' bar.
+// Out: ub = u bar = u* bar - DEL2^T(wk), nb = N bar = CHOPDO(hb) + 1.5 dt u* bar,
+// hb = N_old bar = -0.5 dt u* bar, with u* bar = -ALP wk (CN) or u' bar.
+// g carries the truncation NRCHOP+2; rows are kept below NRCHOPS at nrchop_out.
+struct AdjCnJobs {
+  AdjCnField f[3];
+};
+__global__ void k_adj_cn(KGrid g, int nrchop_out, const double *bdel2h, const double *ak, AdjCnJobs jobs, double dt) {
+  const AdjCnField F = jobs.f[blockIdx.y];
+  const long n = (long)g.nrchopdim * g.ntchopdim * g.nxchopdim;
+  A_GRID_STRIDE(n) {
+    const int c = (int)(idx % g.nrchopdim);
+    const int mm = (int)((idx / g.nrchopdim) % g.ntchopdim);
+    const int k = (int)(idx / ((long)g.nrchopdim * g.ntchopdim));
+    const int no = mm < g.ntchop ? a_nrchops(nrchop_out, mm * g.minc) : 0;
+    const bool in = c < no;
+    double2 st = z2(), u = z2();
+    if (in) {
+      if (F.cn) {
+        const int nn = a_nrchops(g.nrchop, mm * g.minc);
+        const size_t col = idx - c;
+        const double kv = ak[mm + (size_t)g.ntchopdim * k];
+        const double *Lb = bdel2h + (size_t)mm * g.nband * 5;
+        double2 t = sc(-kv * kv, F.wk[idx]);
+#pragma unroll
+        for (int j = 0; j < 5; j++) {
+          const int row = c - j + 2;
+          if (row >= 0 && row < nn) t = pl(t, sc(Lb[(size_t)row * 5 + j], F.wk[col + row]));
+        }
+        st = sc(-F.alp, F.wk[idx]);
+        u = mi(st, t);
+      } else {
+        st = F.ub[idx];
+        u = st;
+      }
+    }
+    const double2 h = in ? F.hb[idx] : z2();
+    F.nb[idx] = pl(h, sc(1.5 * dt, st));
+    F.hb[idx] = sc(-0.5 * dt, st);
+    F.ub[idx] = u;
+    if (idx == 0) {
+      if (F.ub_ln) {
+        const int nn = a_nrchops(g.nrchop, 0);
+        double stl = F.cn ? -F.alp * F.wk_ln[0] : F.ub_ln[0], ul = stl;
+        if (F.cn) {
+          ul -= F.wk[0].x * (4.0 / 3 / g.ell2 / g.norm0);
+          if (nn > 1) ul -= F.wk[1].x * (-2.0 / g.ell2 / g.norm1);
+          if (nn > 2) ul -= F.wk[2].x * (2.0 / 3 / g.ell2 / g.norm2);
+        }
+        F.nb_ln[0] = F.hb_ln[0] + 1.5 * dt * stl;
+        F.hb_ln[0] = -0.5 * dt * stl;
+        F.ub_ln[0] = ul;
+      }
+    }
+  }
+}
+
 // y = a y + b1 x1 + b2 x2 (x1, x2 may be null), n entries
 __global__ void k_adj_lincomb(double2 *y, double a, const double2 *x1, double b1, const double2 *x2, double b2, long n) {
   A_GRID_STRIDE(n) {
@@ -387,15 +480,54 @@ int launch_adj_products(mlg_ctx *c, double2 *const u[9], const double2 *const o[
 }
 int launch_adj_project(mlg_ctx *c, const double2 *psib, const double2 *chib, double2 *vr, double2 *vp, double2 *vz) {
   KGrid g = kgrid(c);
-  hipLaunchKernelGGL(k_adj_project, dim3(g.ntchopdim, g.nxchopdim), dim3(64), 0, c->stream, g, c->d.vdt, c->d.ak, psib, chib,
-                     vr, vp, vz);
+  const size_t lds = sizeof(double2) * 5 * ADJ_PKB * (size_t)(g.nrchop > 0 ? g.nrchop : 1);
+  hipLaunchKernelGGL(k_adj_project, dim3(g.ntchopdim, (g.nxchopdim + ADJ_PKB - 1) / ADJ_PKB), dim3(64), lds, c->stream, g,
+                     c->d.vdt, c->d.ak, c->d.at1, psib, chib, vr, vp, vz);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
+// VEL2PLN's vectors c and sigma at the current truncation (cached per
+// truncation): for i < NRH,
+//   c(i) = PF(i,0)/N(0) + PF(i,1)/N(1),   c(NR-1-i) = PF(i,0)/N(0) - PF(i,1)/N(1),
+//   sigma(i) = w(i) sum_n e^{LOGNORM(n)} PF(i,n),
+//   sigma(NR-1-i) = w(i) sum_n (-1)^n e^{LOGNORM(n)} PF(i,n),   n < NRCHOPS(0)
+static const double *vel2pln_vectors(mlg_ctx *c, int *rc) {
+  const Grid &G = c->g;
+  for (auto &v : c->adjv)
+    if (v.first == G.nrchop) return v.second;
+  const int nr = G.nr, nrh = G.nrh, nn = nrchops_of(G.nrchop, 0);
+  std::vector<double> h(2 * (size_t)nr, 0.0);
+  const double n0 = G.norm[0], n1 = G.norm[1];
+  for (int i = 0; i < nrh; i++) {
+    const double p0 = G.pf[i], p1 = G.pf[(size_t)nrh + i];
+    h[i] = p0 / n0 + p1 / n1;
+    h[nr - 1 - i] = p0 / n0 - p1 / n1;
+    double se = 0.0, so = 0.0;
+    for (int n = 0; n < nn; n++) {
+      const double t = std::exp(G.lognorm[n]) * G.pf[(size_t)n * nrh + i];
+      se += t;
+      so += (n & 1) ? -t : t;
+    }
+    h[nr + i] = G.w[i] * se;
+    h[nr + nr - 1 - i] = G.w[i] * so;
+  }
+  double *d = nullptr;
+  if (hipMalloc((void **)&d, sizeof(double) * h.size()) != hipSuccess ||
+      hipMemcpy(d, h.data(), sizeof(double) * h.size(), hipMemcpyHostToDevice) != hipSuccess) {
+    *rc = MLG_E_HIP;
+    return nullptr;
+  }
+  c->adjv.push_back({G.nrchop, d});
+  return d;
+}
+
 int launch_adj_vel2pln(mlg_ctx *c, double2 *vp) {
   KGrid g = kgrid(c);
   if (g.ntchop < 1 || g.nrchop < 1) return 0;
-  hipLaunchKernelGGL(k_adj_vel2pln, dim3(1), dim3(256), 0, c->stream, g, c->d.pf, c->d.w, c->d.lognorm, vp);
+  int rc = 0;
+  const double *cs = vel2pln_vectors(c, &rc);
+  if (!cs) return rc;
+  hipLaunchKernelGGL(k_adj_vel2pln, dim3(1), dim3(256), 0, c->stream, g, cs, vp);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -422,6 +554,16 @@ int launch_adj_del2t(mlg_ctx *c, int nrchop_out, const double2 *y, double s, dou
   const long n = (long)g.nrchopdim * g.ntchopdim * g.nxchopdim;
   hipLaunchKernelGGL(k_adj_del2t, dim3(grid_for(n, 256)), dim3(256), 0, c->stream, g, nrchop_out, c->d.bdel2h, c->d.ak, y, s,
                      out, ln_out);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+int launch_adj_cn(mlg_ctx *c, int nrchop_out, int nf, const AdjCnField *f, double dt) {
+  KGrid g = kgrid(c);
+  AdjCnJobs jobs;
+  for (int i = 0; i < nf; i++) jobs.f[i] = f[i];
+  const long n = (long)g.nrchopdim * g.ntchopdim * g.nxchopdim;
+  hipLaunchKernelGGL(k_adj_cn, dim3(grid_for(n, 256), nf), dim3(256), 0, c->stream, g, nrchop_out, c->d.bdel2h, c->d.ak, jobs,
+                     dt);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

@@ -301,6 +301,36 @@ struct Fft4 {
   static constexpr int PB = 512 / TPP;
 };
 
+#define RET_FFT(x)             \
+  do {                         \
+    int _r = (x);              \
+    if (_r) return _r;         \
+  } while (0)
+
+// PFP slots m < NTCHOPDIM of a(r, m, z), r < NR, times s0 (m = 0) or s1
+// (m >= 1), and times wr(min(r, NR-1-r)) when wr is given (the scalings of
+// the adjoint transforms on the general-length theta path)
+__global__ void k_scale_pfp_m(KGrid g, double2 *a, double s0, double s1, const double *wr) {
+  const long n = (long)g.nr * g.ntchopdim * g.ndimx;
+  for (long idx = blockIdx.x * (long)blockDim.x + threadIdx.x; idx < n; idx += (long)gridDim.x * blockDim.x) {
+    const int r = (int)(idx % g.nr);
+    const int j = (int)((idx / g.nr) % g.ntchopdim);
+    const int z = (int)(idx / ((long)g.nr * g.ntchopdim));
+    double sm = j == 0 ? s0 : s1;
+    if (wr) sm *= wr[r < g.nr - 1 - r ? r : g.nr - 1 - r];
+    double2 &v = a[(size_t)r + (size_t)g.ndimr * ((size_t)j + (size_t)g.ndimth * z)];
+    v = make_double2(v.x * sm, v.y * sm);
+  }
+}
+static int launch_scale_pfp_m(mlg_ctx *c, double2 *a, double s0, double s1, const double *wr) {
+  KGrid g = kgrid(c);
+  const long n = (long)g.nr * g.ntchopdim * g.ndimx;
+  long b = (n + 255) / 256;
+  hipLaunchKernelGGL(k_scale_pfp_m, dim3((unsigned)(b < 4096 ? b : 4096)), dim3(256), 0, c->stream, g, a, s0, s1, wr);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
 // Pass 1 + twiddle + LDS store + pass 2.  `v` enters holding x[R2 n1 + t]
 // (thread t < R2) and leaves holding X[t + R1 k2] (thread t < R1).
 template <int R1, int R2, int P = Fft4<R1, R2>::P>
@@ -423,8 +453,9 @@ __global__ void __launch_bounds__(512) k_verfft_bwd4(KGrid g, const double2 *__r
 // packed real transform as in k_horfft_fwd).
 template <int R1, int R2>
 __global__ void __launch_bounds__(256) k_horfft_fwd4(KGrid g, const double2 *__restrict__ tw, const double2 *__restrict__ twh,
-                                                     double2 *__restrict__ a) {
+                                                     HorJobs jobs) {
   constexpr int N = R1 * R2, P = Fft4<R1, R2>::P;
+  double2 *__restrict__ a = jobs.a[blockIdx.z];
   __shared__ double2 s[P * N];
   const int r = threadIdx.x % P, t = threadIdx.x / P;
   const int z = blockIdx.y, r0 = blockIdx.x * P, rr = r0 + r;
@@ -453,6 +484,8 @@ __global__ void __launch_bounds__(256) k_horfft_fwd4(KGrid g, const double2 *__r
     double2 wd = cmul(twh[mm], d);
     double2 x = make_double2(e.x + 0.5 * wd.y, e.y - 0.5 * wd.x);
     x = make_double2(x.x / den, x.y / den);
+    const double sm = mm == 0 ? jobs.s0 : jobs.s1;
+    x = make_double2(x.x * sm, x.y * sm);
     if (mm == 0) x.y = 0.0;
     if (mm >= g.ntchop || r0 + pr >= g.nr) x = make_double2(0.0, 0.0);
     a[(size_t)(r0 + pr) + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * z)] = x;
@@ -463,8 +496,9 @@ __global__ void __launch_bounds__(256) k_horfft_fwd4(KGrid g, const double2 *__r
 // k_horfft_bwd, then an inverse length-NTH transform).
 template <int R1, int R2>
 __global__ void __launch_bounds__(256) k_horfft_bwd4(KGrid g, const double2 *__restrict__ tw, const double2 *__restrict__ twh,
-                                                     double2 *__restrict__ a) {
+                                                     HorJobs jobs) {
   constexpr int N = R1 * R2, P = Fft4<R1, R2>::P;
+  double2 *__restrict__ a = jobs.a[blockIdx.z];
   __shared__ double2 s[P * N];
   const int r = threadIdx.x % P, t = threadIdx.x / P;
   const int z = blockIdx.y, r0 = blockIdx.x * P, rr = r0 + r;
@@ -472,7 +506,15 @@ __global__ void __launch_bounds__(256) k_horfft_bwd4(KGrid g, const double2 *__r
   for (int idx = threadIdx.x; idx < P * mmax; idx += 256) {
     const int pr = idx % P, mm = idx / P;
     double2 v = make_double2(0.0, 0.0);
-    if (r0 + pr < g.nr && mm < g.ntchop) v = a[(size_t)(r0 + pr) + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * z)];
+    if (r0 + pr < g.nr && mm < g.ntchop) {
+      v = a[(size_t)(r0 + pr) + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * z)];
+      double sm = mm == 0 ? jobs.s0 : jobs.s1;
+      if (jobs.wr) {
+        const int rr = r0 + pr, rf = rr < g.nr - 1 - rr ? rr : g.nr - 1 - rr;
+        sm = sm * jobs.wr[rf];
+      }
+      v = make_double2(v.x * sm, v.y * sm);
+    }
     if (mm == 0) v.y = 0.0;
     s[mm * P + pr] = v;
   }
@@ -583,30 +625,52 @@ int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *co
 
 int launch_verfft_bwd(mlg_ctx *c, const double2 *in, double2 *out) { return launch_verfft_bwd_n(c, 1, &in, &out); }
 
-int launch_horfft_fwd(mlg_ctx *c, double2 *a) {
+int launch_horfft_fwd_n(mlg_ctx *c, int n, double2 *const *a, double s0, double s1) {
+  if (n < 1 || n > MAX_HOR_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
-  if (g.nth > 256) return launch_horfft_fwd_r2(c, a);
-  KScope ks(c, K_HORFFT_FWD, 16.0 * g.nr * g.nx * (double)(g.nth + g.ntchopdim));
-#define L_HF(a_, b_)                                                                                         \
-  hipLaunchKernelGGL((k_horfft_fwd4<a_, b_>), dim3((g.ndimr + Fft4<a_, b_>::P - 1) / Fft4<a_, b_>::P, g.ndimx), \
-                     dim3(256), 0, c->stream, g, c->d.tw_t, c->d.tw_h, a)
+  if (g.nth > 256) {
+    for (int i = 0; i < n; i++) {
+      RET_FFT(launch_horfft_fwd_r2(c, a[i]));
+      if (s0 != 1.0 || s1 != 1.0) RET_FFT(launch_scale_pfp_m(c, a[i], s0, s1, nullptr));
+    }
+    return 0;
+  }
+  HorJobs jobs;
+  for (int i = 0; i < n; i++) jobs.a[i] = a[i];
+  jobs.s0 = s0, jobs.s1 = s1, jobs.wr = nullptr;
+  KScope ks(c, K_HORFFT_FWD, n * 16.0 * g.nr * g.nx * (double)(g.nth + g.ntchopdim));
+#define L_HF(a_, b_)                                                                                            \
+  hipLaunchKernelGGL((k_horfft_fwd4<a_, b_>), dim3((g.ndimr + Fft4<a_, b_>::P - 1) / Fft4<a_, b_>::P, g.ndimx, n), \
+                     dim3(256), 0, c->stream, g, c->d.tw_t, c->d.tw_h, jobs)
   FFT4_SWITCH(g.nth, L_HF)
 #undef L_HF
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
+int launch_horfft_fwd(mlg_ctx *c, double2 *a) { return launch_horfft_fwd_n(c, 1, &a, 1.0, 1.0); }
 
-int launch_horfft_bwd(mlg_ctx *c, double2 *a) {
+int launch_horfft_bwd_n(mlg_ctx *c, int n, double2 *const *a, double s0, double s1, const double *wr) {
+  if (n < 1 || n > MAX_HOR_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
-  if (g.nth > 256) return launch_horfft_bwd_r2(c, a);
-  KScope ks(c, K_HORFFT_BWD, 16.0 * g.nr * g.nx * (double)(g.nth + g.ntchopdim));
-#define L_HB(a_, b_)                                                                                         \
-  hipLaunchKernelGGL((k_horfft_bwd4<a_, b_>), dim3((g.ndimr + Fft4<a_, b_>::P - 1) / Fft4<a_, b_>::P, g.ndimx), \
-                     dim3(256), 0, c->stream, g, c->d.tw_t, c->d.tw_h, a)
+  if (g.nth > 256) {
+    for (int i = 0; i < n; i++) {
+      if (s0 != 1.0 || s1 != 1.0 || wr) RET_FFT(launch_scale_pfp_m(c, a[i], s0, s1, wr));
+      RET_FFT(launch_horfft_bwd_r2(c, a[i]));
+    }
+    return 0;
+  }
+  HorJobs jobs;
+  for (int i = 0; i < n; i++) jobs.a[i] = a[i];
+  jobs.s0 = s0, jobs.s1 = s1, jobs.wr = wr;
+  KScope ks(c, K_HORFFT_BWD, n * 16.0 * g.nr * g.nx * (double)(g.nth + g.ntchopdim));
+#define L_HB(a_, b_)                                                                                            \
+  hipLaunchKernelGGL((k_horfft_bwd4<a_, b_>), dim3((g.ndimr + Fft4<a_, b_>::P - 1) / Fft4<a_, b_>::P, g.ndimx, n), \
+                     dim3(256), 0, c->stream, g, c->d.tw_t, c->d.tw_h, jobs)
   FFT4_SWITCH(g.nth, L_HB)
 #undef L_HB
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
+int launch_horfft_bwd(mlg_ctx *c, double2 *a) { return launch_horfft_bwd_n(c, 1, &a, 1.0, 1.0, nullptr); }
 
 }  // namespace mlg

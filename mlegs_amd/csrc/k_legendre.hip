@@ -652,6 +652,13 @@ k_rtran_fwd(KGrid g, const double *__restrict__ pf, const double *__restrict__ w
   rtran_fwd_body<false>(g, pf, w, in, out, zl, nullptr, L);
 }
 
+// n fields in one launch (grid row = field), FP64 products
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(4, 4)))
+k_rtran_fwd_n(KGrid g, const double *__restrict__ pf, const double *__restrict__ w, TransformJobs jobs) {
+  __shared__ RtranFwdLds<false> L;
+  rtran_fwd_body<false>(g, pf, w, jobs.in[blockIdx.y], jobs.out[blockIdx.y], nullptr, nullptr, L);
+}
+
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_rtran_fwd_split(KGrid g, const float2 *__restrict__ pf, const double *__restrict__ w, const double2 *__restrict__ in,
                   double2 *__restrict__ out, double *zl, const int *__restrict__ rowe) {
@@ -659,14 +666,32 @@ k_rtran_fwd_split(KGrid g, const float2 *__restrict__ pf, const double *__restri
   rtran_fwd_body<true>(g, pf, w, in, out, zl, rowe, L);
 }
 
-int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out, double *zl) {
+int launch_rtran_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out, const double *w) {
+  if (n < 1 || n > MAX_TRANSFORM_JOBS || c->leg_split) return MLG_E_ARG;
+  KGrid g = kgrid(c);
+  int ntn = (g.nrchopdim + FW_TN - 1) / FW_TN, ntc = (2 * g.nxchopdim + FW_TC - 1) / FW_TC;
+  TransformJobs jobs;
+  for (int i = 0; i < n; i++) {
+    jobs.in[i] = in[i];
+    jobs.ln[i] = nullptr;
+    jobs.out[i] = out[i];
+  }
+  KScope ks(c, K_RTRAN_FWD, n * 2.0 * g.nr * g.nxchopdim * (double)sum_nrchops(c));
+  hipLaunchKernelGGL(k_rtran_fwd_n, dim3(xcd_grid(ntn * ntc * g.mcnt), n), dim3(256), 0, c->stream, g, c->d.pf,
+                     w ? w : c->d.w, jobs);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
+int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out, double *zl, const double *w) {
   KGrid g = kgrid(c);
   int ntn = (g.nrchopdim + FW_TN - 1) / FW_TN, ntc = (2 * g.nxchopdim + FW_TC - 1) / FW_TC;
   const double per_k = 2.0 * g.nr * g.nxchopdim;
+  if (w && c->leg_split) return MLG_E_ARG;
   if (!c->leg_split) {
     KScope ks(c, K_RTRAN_FWD, per_k * (double)sum_nrchops(c));
-    hipLaunchKernelGGL(k_rtran_fwd, dim3(xcd_grid(ntn * ntc * g.mcnt)), dim3(256), 0, c->stream, g, c->d.pf, c->d.w, in, out,
-                       zl);
+    hipLaunchKernelGGL(k_rtran_fwd, dim3(xcd_grid(ntn * ntc * g.mcnt)), dim3(256), 0, c->stream, g, c->d.pf, w ? w : c->d.w,
+                       in, out, zl);
     MLG_CHECK_HIP(hipGetLastError());
     return 0;
   }

@@ -1572,7 +1572,7 @@ int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *
 // so A^T = U^T D L^T: U^T w = h (forward over the rows), v = D^-1 w, then
 // L^T g = v (backward over the rows).  One thread per system, in place on
 // the FFF column.
-template <int NE, int C0>
+template <int NE, int C0, int SB>
 __device__ void band_apply_t(int nd, double2 *x, const FactorView &FV, long ncls, long cls) {
   constexpr int KU = NE - 1 - C0;
   auto lmul = [&](int row, int j) -> double {  // L(row, row - j)
@@ -1587,52 +1587,94 @@ __device__ void band_apply_t(int nd, double2 *x, const FactorView &FV, long ncls
     if (NE == 5) return FV.ff[((size_t)row * ncls + cls) * 3 + 2];
     return FV.f7[((size_t)row * ncls + cls) * FSTRIDE + NE - 1];
   };
+  // rows in blocks of SB: a block's factor entries are loaded before its
+  // serial arithmetic (out-of-range entries as 0, which leaves the sums exact)
   double2 wh[KU];
+#pragma unroll
   for (int u = 0; u < KU; u++) wh[u] = zero2();
-  for (int nn = 0; nn < nd; nn++) {
-    double2 w = x[nn];
-    for (int u = 1; u <= KU; u++)
-      if (nn - u >= 0) w = sub2(w, axc(uup(nn - u, u), wh[u - 1]));
-    for (int u = KU - 1; u >= 1; u--) wh[u] = wh[u - 1];
-    wh[0] = w;
-    x[nn] = axc(rpiv(nn), w);
+  for (int nb = 0; nb < nd; nb += SB) {
+    double um[SB][KU], rp[SB];
+    double2 h[SB];
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int nn = nb + q;
+      if (nn < nd) {
+        h[q] = x[nn];
+        rp[q] = rpiv(nn);
+#pragma unroll
+        for (int u = 1; u <= KU; u++) um[q][u - 1] = nn - u >= 0 ? uup(nn - u, u) : 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      if (nb + q < nd) {
+        double2 w = h[q];
+#pragma unroll
+        for (int u = 1; u <= KU; u++) w = sub2(w, axc(um[q][u - 1], wh[u - 1]));
+#pragma unroll
+        for (int u = KU - 1; u >= 1; u--) wh[u] = wh[u - 1];
+        wh[0] = w;
+        h[q] = axc(rp[q], w);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+      if (nb + q < nd) x[nb + q] = h[q];
   }
   double2 gh[C0];
+#pragma unroll
   for (int j = 0; j < C0; j++) gh[j] = zero2();
-  for (int nn = nd - 1; nn >= 0; nn--) {
-    double2 v = x[nn];
-    for (int j = 1; j <= C0; j++)
-      if (nn + j < nd) v = sub2(v, axc(lmul(nn + j, j), gh[j - 1]));
-    for (int j = C0 - 1; j >= 1; j--) gh[j] = gh[j - 1];
-    gh[0] = v;
-    x[nn] = v;
+  for (int nt = nd - 1; nt >= 0; nt -= SB) {
+    double lm[SB][C0];
+    double2 v[SB];
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      const int nn = nt - q;
+      if (nn >= 0) {
+        v[q] = x[nn];
+#pragma unroll
+        for (int j = 1; j <= C0; j++) lm[q][j - 1] = nn + j < nd ? lmul(nn + j, j) : 0.0;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      if (nt - q >= 0) {
+        double2 g = v[q];
+#pragma unroll
+        for (int j = 1; j <= C0; j++) g = sub2(g, axc(lm[q][j - 1], gh[j - 1]));
+#pragma unroll
+        for (int j = C0 - 1; j >= 1; j--) gh[j] = gh[j - 1];
+        gh[0] = g;
+        v[q] = g;
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+      if (nt - q >= 0) x[nt - q] = v[q];
   }
 }
 
-// op 1 (IDEL2LN): the (m = 0, k = 0) system's first unknown became LN
-// (LN = x(0) NORM(0,0), x(0) = 0), so its adjoint enters as x(0) =
-// NORM(0,0) LNbar.  op 2 (IHELM): the m = 0 systems solve (del^2 - alp -
-// k^2)(1-x) y on NN-1 rows and return XM y, so XM^T comes first; at k = 0
-// the right-hand side had LN terms removed (A%LN = -B%LN/ALP), whose
-// adjoint gives B's LNbar.  ln_bar: LNbar of the solution (in), of the
-// right-hand side (out, op 2).
-__global__ void __launch_bounds__(64) k_band_apply_t(KGrid g, const double *bxm, int op, double alp, const double *F,
-                                                     double2 *x, double *ln_bar) {
-  const long nsys = (long)g.ntchopdim * g.nxchopdim;
-  const long ncls = (long)g.ntchopdim * g.nxchop;
-  const long sys = blockIdx.x * (long)blockDim.x + threadIdx.x;
-  if (sys >= nsys) return;
-  const int mm = (int)(sys % g.ntchopdim), k = (int)(sys / g.ntchopdim);
-  const int nn = s_nrchops(g.nrchop, mm * g.minc);
-  if (nn < 1 || mm >= g.ntchop) return;
+// One system of the transposed solve, on its column x in LDS (rows >= NN
+// already zero).  op 1 (IDEL2LN): the m = 0 columns first take the
+// transpose of the fix at infinity (x(n) -= AT1(n)/AT1(0) x(0)); the
+// (m = 0, k = 0) system's first unknown became LN (LN = x(0) NORM(0,0),
+// x(0) = 0), so its adjoint enters as x(0) = NORM(0,0) LNbar.  op 2 (IHELM):
+// the m = 0 systems solve (del^2 - alp - k^2)(1-x) y on NN-1 rows and
+// return XM y, so XM^T comes first; at k = 0 the right-hand side had LN
+// terms removed (A%LN = -B%LN/ALP), whose adjoint gives B's LNbar.  ln:
+// LNbar of the solution; ln_out: of the right-hand side (op 2).
+__device__ void solve_t_system(const KGrid &g, const double *bxm, const double *at1, int op, double alp, const FactorView &FV,
+                               long ncls, int mm, int k, int nn, double2 *col, const double *ln, double *ln_out) {
   const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
   const long cls = mm + (long)g.ntchopdim * kabs;
-  double2 *col = x + (size_t)g.nrchopdim * ((size_t)mm + (size_t)g.ntchopdim * k);
-  const FactorView FV = factor_view(F, g.nrchopdim, ncls);
   const int kind = solve_kind(op, mm, k);
   if (kind == 0) {
-    if (op == 1 && mm == 0 && k == 0) col[0] = make_double2(g.norm0 * ln_bar[0], 0.0);
-    band_apply_t<5, 2>(nn, col, FV, ncls, cls);
+    if (op == 1 && mm == 0) {
+      const double2 a0 = col[0];
+      for (int n = 0; n < g.nrchop; n++) col[n] = sub2(col[n], axc(at1[n] / at1[0], a0));
+      if (k == 0) col[0] = make_double2(g.norm0 * ln[0], 0.0);
+    }
+    band_apply_t<5, 2, 8>(nn, col, FV, ncls, cls);
     return;
   }
   // IHELM, m = 0: x <- XM^T x on NN rows (ascending, from the original values)
@@ -1645,26 +1687,80 @@ __global__ void __launch_bounds__(64) k_band_apply_t(KGrid g, const double *bxm,
     prev = col[c];
     col[c] = t;
   }
-  band_apply_t<7, 3>(nn - 1, col, FV, ncls, cls);
+  band_apply_t<7, 3, 8>(nn - 1, col, FV, ncls, cls);
   col[nn - 1] = zero2();
   if (k == 0) {
-    const double lbar = ln_bar[0] - (col[0].x * ((4.0 / 3 / g.ell2) / g.norm0) + col[1].x * (-2 / g.norm1 / g.ell2) +
-                                     col[2].x * (+2.0 / 3 / g.norm2 / g.ell2));
-    ln_bar[0] = -lbar / alp;
+    const double lbar = ln[0] - (col[0].x * ((4.0 / 3 / g.ell2) / g.norm0) + col[1].x * (-2 / g.norm1 / g.ell2) +
+                                 col[2].x * (+2.0 / 3 / g.norm2 / g.ell2));
+    ln_out[0] = -lbar / alp;
   }
 }
 
-int launch_band_solve_t(mlg_ctx *c, int op, double alp, double2 *x, double *ln_bar) {
-  if (op != 1 && op != 2) return MLG_E_ARG;
+struct BandTJobs {
+  BandTJob j[MAX_BAND_JOBS];
+};
+// Block (group of spb consecutive systems, job): the group's columns (one
+// contiguous run of the FFF array) are staged in LDS with coalesced loads,
+// chopped to NRCHOPS(m) on the way in; one thread per system substitutes
+// from LDS; the block writes the columns back, zeros beyond NRCHOPS(m)
+// included (the solve's CHOPDO).  in may equal out.
+__global__ void __launch_bounds__(64) k_band_apply_t(KGrid g, const double *bxm, const double *at1, int op, BandTJobs jobs,
+                                                     int spb) {
+  extern __shared__ __align__(16) double2 xs[];
+  const BandTJob &J = jobs.j[blockIdx.y];
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  const long ncls = (long)g.ntchopdim * g.nxchop;
+  const int R = g.nrchopdim;
+  const long sys0 = blockIdx.x * (long)spb;
+  const size_t base = (size_t)R * sys0;  // column of system s = m + NTCHOPDIM k starts at R s
+  const long nv = (long)spb * R;
+  for (long i = threadIdx.x; i < nv; i += blockDim.x) {
+    const long sys = sys0 + i / R;
+    const int row = (int)(i % R);
+    double2 v = zero2();
+    if (sys < nsys) {
+      const int mm = (int)(sys % g.ntchopdim);
+      const int nn = mm < g.ntchop ? s_nrchops(g.nrchop, mm * g.minc) : 0;
+      if (row < nn) v = J.in[base + i];
+    }
+    xs[i] = v;
+  }
+  __syncthreads();
+  if ((int)threadIdx.x < spb) {
+    const long sys = sys0 + threadIdx.x;
+    if (sys < nsys) {
+      const int mm = (int)(sys % g.ntchopdim), k = (int)(sys / g.ntchopdim);
+      const int nn = mm < g.ntchop ? s_nrchops(g.nrchop, mm * g.minc) : 0;
+      if (nn >= 1)
+        solve_t_system(g, bxm, at1, op, J.alp, factor_view(J.F, R, ncls), ncls, mm, k, nn, xs + (size_t)threadIdx.x * R,
+                       J.ln_in, J.ln_out);
+    }
+  }
+  __syncthreads();
+  for (long i = threadIdx.x; i < nv; i += blockDim.x)
+    if (sys0 + i / R < nsys) J.out[base + i] = xs[i];
+}
+
+int launch_band_solve_t_n(mlg_ctx *c, int op, int n, const BandTJob *in) {
+  if ((op != 1 && op != 2) || n < 1 || n > MAX_BAND_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
   if (op == 2 && g.nrchop < 4) return MLG_E_ARG;  // the m = 0 form needs the three LN rows
-  int rc = 0;
-  const double *F = factor_set(c, op, alp, nullptr, 0, &rc);
-  if (!F) return rc;
+  BandTJobs jobs;
+  const double *used[MAX_BAND_JOBS];
+  for (int i = 0; i < n; i++) {
+    int rc = 0;
+    const double *F = factor_set(c, op, in[i].alp, used, i, &rc);
+    if (!F) return rc;
+    used[i] = F;
+    jobs.j[i] = in[i];
+    jobs.j[i].F = F;
+  }
   const long nsys = (long)g.ntchopdim * g.nxchopdim;
-  KScope ks(c, K_BAND_SOLVE, 2.0 * 16.0 * fff_count(g));
-  hipLaunchKernelGGL(k_band_apply_t, dim3((unsigned)((nsys + 63) / 64)), dim3(64), 0, c->stream, g, c->d.bxm, op, alp, F, x,
-                     ln_bar);
+  int spb = (int)((48 * 1024) / ((size_t)g.nrchopdim * sizeof(double2)));
+  spb = spb < 1 ? 1 : (spb > 64 ? 64 : spb);
+  KScope ks(c, K_BAND_SOLVE, n * 2.0 * 16.0 * fff_count(g));
+  hipLaunchKernelGGL(k_band_apply_t, dim3((unsigned)((nsys + spb - 1) / spb), n), dim3(64),
+                     sizeof(double2) * (size_t)spb * g.nrchopdim, c->stream, g, c->d.bxm, c->d.at1, op, jobs, spb);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

@@ -37,6 +37,7 @@ struct DevTables {
   double *pf = nullptr;       // [mm][n][i]  (i fastest), npf rows
   double *vdt = nullptr;      // [3][mm][n][j]
   double *w = nullptr, *r = nullptr, *x = nullptr, *lnx = nullptr;
+  double *ones = nullptr;     // NR ones: RTRAN forward's Gauss sums without weights (the adjoint's RTRAN backward^T)
   double *at1 = nullptr, *ak = nullptr, *norm = nullptr, *lognorm = nullptr;
   double *bxm = nullptr, *bxxdx = nullptr, *bdel2h = nullptr;
   double2 *tw_z = nullptr;    // e^{-2 pi i j / NX}, j < NX
@@ -114,6 +115,8 @@ struct mlg_ctx {
   int split_fp64_modes = 5;  // split mode: modes m < this keep FP64 products
   // PROJECT's m = 0, k = 0 column vectors, one buffer per truncation (built on first use)
   std::vector<std::pair<int, double *>> m0vecs;
+  // the adjoint's VEL2PLN vectors, one buffer per truncation (built on first use)
+  std::vector<std::pair<int, double *>> adjv;
 };
 
 struct mlg_solver {
@@ -202,7 +205,11 @@ int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double
 int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out, unsigned dz = 0,
                         unsigned dm = 0, double *const *zl = nullptr, int nzl = 0);
 int launch_verfft_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out);
-int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out, double *zl = nullptr);  // zl: an LN set to 0
+// zl: an LN set to 0; w: the Gauss weights of the sums (null: the grid's;
+// the adjoint passes ones, FP64 products only)
+int launch_rtran_fwd(mlg_ctx *c, const double2 *in, double2 *out, double *zl = nullptr, const double *w = nullptr);
+// n (<= MAX_TRANSFORM_JOBS) fields in one launch (FP64 products only)
+int launch_rtran_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out, const double *w = nullptr);
 int launch_project(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *psi, double2 *chi);
 // PROJECT of spectral (FFF) velocities: RTRAN backward, VEL2PLN and the
 // projection products in one pass with the composed tables; ln_mode 0: no
@@ -219,6 +226,18 @@ int launch_verfft_fwd(mlg_ctx *c, const double2 *in, double2 *out);
 int launch_verfft_bwd(mlg_ctx *c, const double2 *in, double2 *out);
 int launch_horfft_fwd(mlg_ctx *c, double2 *a);
 int launch_horfft_bwd(mlg_ctx *c, double2 *a);
+// n (<= MAX_HOR_JOBS) fields in place in one launch.  Forward: the results
+// of m = 0 times s0, of m >= 1 times s1.  Backward: the inputs of m = 0 times
+// s0, of m >= 1 times s1, and each radial row r times wr(min(r, NR-1-r))
+// when wr is given.  (s0 = s1 = 1, wr = null: HORFFT itself.)
+constexpr int MAX_HOR_JOBS = 10;
+struct HorJobs {
+  double2 *a[MAX_HOR_JOBS];
+  double s0, s1;
+  const double *wr;
+};
+int launch_horfft_fwd_n(mlg_ctx *c, int n, double2 *const *a, double s0, double s1);
+int launch_horfft_bwd_n(mlg_ctx *c, int n, double2 *const *a, double s0, double s1, const double *wr);
 // ---- physical-space products (k_physical.hip)
 int launch_vprod(mlg_ctx *c, const double2 *rur, const double2 *rup, const double2 *uz, double2 *ror, double2 *rop, double2 *oz);
 int launch_vdot(mlg_ctx *c, const double2 *rar, const double2 *rap, const double2 *az, const double2 *rbr,
@@ -322,12 +341,21 @@ int launch_band_solve(mlg_ctx *c, int op, const double2 *b, double2 *a, double *
 int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, double2 *const *a, double *const *ln_a,
                         const double *const *ln_b, const double *alp, int has_ln, double ln, double2 *const *t,
                         int bil = 0);
-// Adjoint of the op 1 (IDEL2LN) or op 2 (IHELM) solve, in place on the FFF
-// field x (the solution's adjoint in, the right-hand side's out); ln_bar:
-// LNbar of the solution (op 1: read; op 2: read, then replaced by the
-// right-hand side's).  CHOPDO and IDEL2LN's fix at infinity are the
-// caller's.
-int launch_band_solve_t(mlg_ctx *c, int op, double alp, double2 *x, double *ln_bar);
+// Adjoint of n (<= 3) op 1 (IDEL2LN) or op 2 (IHELM) solves in one launch:
+// job i reads the solution's adjoint from in (FFF, chopped on the way in)
+// and writes the right-hand side's adjoint to out (may equal in; CHOPDO
+// included); ln_in: LNbar of the solution; ln_out (op 2): LNbar of the
+// right-hand side (may equal ln_in).  op 1 includes the transpose of
+// IDEL2LN's fix at infinity.  F is filled in by the launcher.
+struct BandTJob {
+  const double2 *in;
+  double2 *out;
+  const double *ln_in;
+  double *ln_out;
+  double alp;
+  const double *F;
+};
+int launch_band_solve_t_n(mlg_ctx *c, int op, int n, const BandTJob *jobs);
 int launch_axpby(mlg_ctx *c, int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n);
 int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1, const double *x2, const double *x3);
 // ETD update fused with the HYPERV3 filters (filter != 0), the copy of b
@@ -346,6 +374,29 @@ int launch_energy_terms(mlg_ctx *c, const double2 *a, const double2 *b, int l2, 
 int launch_negate(mlg_ctx *c, double2 *a, long n);
 int launch_scale(mlg_ctx *c, double2 *a, double s, long n);
 int launch_check_top_m1(mlg_ctx *c, const double2 *a);
+
+// ---- adjoint kernels (k_adjoint.hip)
+int launch_adj_scale_pff(mlg_ctx *c, double2 *a, int mode, double s, int skip_m0);
+int launch_adj_scale_pfp(mlg_ctx *c, double2 *a, double s0, double s1);
+int launch_adj_products(mlg_ctx *c, double2 *const u[9], const double2 *const o[3], double2 *bn);
+int launch_adj_project(mlg_ctx *c, const double2 *psib, const double2 *chib, double2 *vr, double2 *vp, double2 *vz);
+int launch_adj_vel2pln(mlg_ctx *c, double2 *vp);
+int launch_adj_fixinf(mlg_ctx *c, double2 *a);
+int launch_adj_columns(mlg_ctx *c, int nrchop_out, const double2 *const in[9], double2 *psib, double *psib_ln, double2 *chib,
+                       double *chib_ln, double2 *bb);
+int launch_adj_del2t(mlg_ctx *c, int nrchop_out, const double2 *y, double s, double2 *out, double *ln_out);
+int launch_adj_lincomb(mlg_ctx *c, double2 *y, double a, const double2 *x1, double b1, const double2 *x2, double b2, long n);
+int launch_adj_ln_lincomb(mlg_ctx *c, double *y, double a, const double *x1, double b1, const double *x2, double b2);
+// one field of the AB2-CN update's transpose (k_adj_cn)
+struct AdjCnField {
+  const double2 *wk;
+  const double *wk_ln;
+  double2 *ub, *hb, *nb;
+  double *ub_ln, *hb_ln, *nb_ln;  // null: the field has no log term
+  double alp;
+  int cn;
+};
+int launch_adj_cn(mlg_ctx *c, int nrchop_out, int nf, const AdjCnField *f, double dt);
 
 // ---- pooled temporaries and COPY0 (api.hip)
 mlg_field *pool_take(mlg_ctx *c, int space);
