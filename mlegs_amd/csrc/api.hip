@@ -179,9 +179,10 @@ extern "C" const char *mlg_error_string(int code) {
 extern "C" int mlg_init(const mlg_params *p, mlg_ctx **out) {
   if (!p || !out) return MLG_E_ARG;
   *out = nullptr;
-  // supported: even NR, power-of-two NTH and NX (radix-2 kernels), the
-  // reference's chop limits (mod_fft.f90:238-244, mod_scalar3.f90:183-190)
-  if (p->nr < 4 || p->nr % 2 || !pow2(p->nth) || p->nth < 2 || !pow2(p->nx) || p->nx < 2) return MLG_E_ARG;
+  // supported: even NR, NTH and NX from 2 to 4096 (powers of two take the
+  // FFT kernels, other lengths direct DFTs), the reference's chop limits
+  // (mod_fft.f90:238-244, mod_scalar3.f90:183-190)
+  if (p->nr < 4 || p->nr % 2 || p->nth < 2 || p->nx < 2) return MLG_E_ARG;
   if (2 * p->nxchop > p->nx || p->nxchop < 1 || p->ntchop < 1 || p->nrchop < 1) return MLG_E_ARG;
   if (p->nth > 4096 || p->nx > 4096) return MLG_E_ARG;
   if (p->minc != 1 && p->minc != 0) return MLG_E_ARG;  // the path uses MINC = 1
@@ -242,6 +243,9 @@ extern "C" int mlg_init(const mlg_params *p, mlg_ctx **out) {
     std::vector<double2> v(G.nth + 1);
     for (int j = 0; j <= G.nth; j++) v[j] = make_double2(re[j], im[j]);
     RET(upload(&d.tw_h, v));
+    std::vector<double2> v2(2 * (size_t)G.nth);
+    for (int j = 0; j < 2 * G.nth; j++) v2[j] = make_double2(re[j], im[j]);
+    RET(upload(&d.tw_h2, v2));
   }
   if (!G.hv_r.empty()) {
     std::vector<double> hv;
@@ -288,7 +292,7 @@ extern "C" int mlg_finalize(mlg_ctx *c) {
   c->timer.spare.clear();
   DevTables &d = c->d;
   void *ptrs[] = {d.pf, d.vdt, d.w, d.ones, d.r, d.x, d.lnx, d.at1, d.ak, d.norm, d.lognorm, d.bxm, d.bxxdx, d.bdel2h,
-                  d.tw_z, d.tw_t, d.tw_h, d.hv, d.red, d.vproj, d.cvdt, d.err,
+                  d.tw_z, d.tw_t, d.tw_h, d.tw_h2, d.hv, d.red, d.vproj, d.cvdt, d.err,
                   d.pf_sb, d.pf_eb, d.pf_sf, d.pf_ef, d.vdt_s, d.vdt_e, d.cvdt_s, d.cvdt_e};
   for (auto &f : c->fcache) (void)hipFree(f.f);
   c->fcache.clear();
@@ -1027,7 +1031,7 @@ static int nonlin_impl(const mlg_field *b, mlg_field *rur, mlg_field *rup, mlg_f
   mlg_field *rbr = T.buffer(), *rbp = T.buffer(), *bz = T.buffer(), *psi = T.buffer(), *chi = T.buffer(), *w = T.buffer();
   RET(T.ready());
   int rc;
-  if (fused_products_supported(c) && c->g.nx <= 256) {
+  if (fused_products_supported(c) && c->g.nx <= 256 && pow2(c->g.nx)) {
     // GRAD's r d/dr b is formed in FFF; its i m b and i k b components come
     // from b's own transform (to_pfp_b)
     RET(mlg_chopset(c, 1));

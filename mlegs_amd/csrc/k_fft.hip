@@ -15,7 +15,10 @@
 // into a complex length-N one, so the slots are transformed as they lie and
 // the half-spectrum is split/merged with the e^{-i pi m/NTH} twiddles.
 //
-// Power-of-two lengths only (all configurations); radix-2 in LDS.
+// Power-of-two lengths: four-step register kernels up to 256, radix-2 in
+// LDS above.  Other lengths (the reference plans FFTW at any length,
+// src/mod_fft.f90:833-920,1006-1091): direct DFT sums over pencils staged in
+// LDS, the phase index k*z mod N stepped through a table e^{-2 pi i j/N}.
 #include "mlegs_internal.h"
 #include "xcd.h"
 
@@ -25,6 +28,7 @@ __device__ __forceinline__ double2 cmul(double2 a, double2 b) {
   return make_double2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
 }
 __device__ __forceinline__ double2 cconj(double2 a) { return make_double2(a.x, -a.y); }
+__device__ __forceinline__ double2 cadd(double2 a, double2 b) { return make_double2(a.x + b.x, a.y + b.y); }
 
 __device__ __forceinline__ int bitrev(int v, int lg) { return (int)(__brev((unsigned)v) >> (32 - lg)); }
 
@@ -301,6 +305,181 @@ struct Fft4 {
   static constexpr int PB = 512 / TPP;
 };
 
+// ---- general lengths: DFT sums from LDS
+static bool pow2_len(int n) { return n >= 1 && (n & (n - 1)) == 0; }
+static int dft_pencils(int n) {
+  int p = 4096 / n;
+  return p < 1 ? 1 : (p > 32 ? 32 : p);
+}
+// the wavenumber of chopped index kk (k = 0..NXCHOP-1, then -(NXCHOP-1)..-1 as N + k)
+__device__ __forceinline__ int kval_of(const KGrid &g, int kk) { return kk < g.nxchop ? kk : g.nx - (g.nxchopdim - kk); }
+
+// VERFFT forward: PFP (r, m, z) -> PFF (r, m, k), / NX
+__global__ void __launch_bounds__(256) k_verfft_fwd_dft(KGrid g, int P, const double2 *__restrict__ tw,
+                                                        const double2 *__restrict__ in, double2 *__restrict__ out) {
+  extern __shared__ double2 s[];
+  const int mm = blockIdx.y, r0 = blockIdx.x * P, N = g.nx;
+  for (int idx = threadIdx.x; idx < P * N; idx += blockDim.x) {
+    const int r = idx % P, z = idx / P;
+    s[r * N + z] = (r0 + r < g.nr) ? in[(size_t)(r0 + r) + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * z)]
+                                   : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < P * g.nxchopdim; idx += blockDim.x) {
+    const int r = idx % P, kk = idx / P;
+    if (r0 + r >= g.ndimr) continue;
+    const int k = kval_of(g, kk);
+    double2 acc = make_double2(0.0, 0.0);
+    int ph = 0;
+    for (int z = 0; z < N; z++) {
+      acc = cadd(acc, cmul(s[r * N + z], tw[ph]));
+      ph += k;
+      if (ph >= N) ph -= N;
+    }
+    out[(size_t)(r0 + r) + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * kk)] = make_double2(acc.x / N, acc.y / N);
+  }
+}
+
+// VERFFT backward: PFF (r, m, k) -> PFP (r, m, z), unscaled
+__global__ void __launch_bounds__(256) k_verfft_bwd_dft(KGrid g, int P, const double2 *__restrict__ tw,
+                                                        const double2 *__restrict__ in, double2 *__restrict__ out) {
+  extern __shared__ double2 s[];
+  const int mm = blockIdx.y, r0 = blockIdx.x * P, N = g.nx, K = g.nxchopdim;
+  for (int idx = threadIdx.x; idx < P * K; idx += blockDim.x) {
+    const int r = idx % P, kk = idx / P;
+    s[r * K + kk] = (r0 + r < g.nr) ? in[(size_t)(r0 + r) + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * kk)]
+                                    : make_double2(0.0, 0.0);
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < P * N; idx += blockDim.x) {
+    const int r = idx % P, z = idx / P;
+    if (r0 + r >= g.ndimr) continue;
+    double2 acc = make_double2(0.0, 0.0);
+    int ph = 0;  // k z mod N, k = kk for kk < NXCHOP
+    for (int kk = 0; kk < K; kk++) {
+      if (kk == g.nxchop) ph = (int)(((long)kval_of(g, kk) * z) % N);
+      acc = cadd(acc, cmul(s[r * K + kk], cconj(tw[ph])));
+      ph += z;
+      if (ph >= N) ph -= N;
+    }
+    out[(size_t)(r0 + r) + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * z)] = acc;
+  }
+}
+
+// HORFFT forward, in place: the 2 NTH real samples of a PPP row (slot j: Re at
+// 2j, Im at 2j+1) -> X_m = sum_t s_t e^{-2 pi i m t/(2 NTH)} / (2 NTH) for
+// m < NTCHOPDIM (zero from NTCHOP), X_0 real; results times s0 (m = 0) or s1.
+__global__ void __launch_bounds__(256) k_horfft_fwd_dft(KGrid g, int P, const double2 *__restrict__ tw2, double2 *a,
+                                                        double s0, double s1) {
+  extern __shared__ double sr[];
+  const int z = blockIdx.y, r0 = blockIdx.x * P, N = g.nth, N2 = 2 * N;
+  for (int idx = threadIdx.x; idx < P * N; idx += blockDim.x) {
+    const int r = idx % P, j = idx / P;
+    double2 v = (r0 + r < g.nr) ? a[(size_t)(r0 + r) + (size_t)g.ndimr * ((size_t)j + (size_t)g.ndimth * z)]
+                                : make_double2(0.0, 0.0);
+    sr[r * N2 + 2 * j] = v.x;
+    sr[r * N2 + 2 * j + 1] = v.y;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < P * g.ntchopdim; idx += blockDim.x) {
+    const int r = idx % P, mm = idx / P;
+    if (r0 + r >= g.ndimr) continue;
+    double2 x = make_double2(0.0, 0.0);
+    if (mm < g.ntchop && r0 + r < g.nr) {
+      int ph = 0;
+      for (int t = 0; t < N2; t++) {
+        const double v = sr[r * N2 + t];
+        x = make_double2(x.x + v * tw2[ph].x, x.y + v * tw2[ph].y);
+        ph += mm;
+        if (ph >= N2) ph -= N2;
+      }
+      const double sm = (mm == 0 ? s0 : s1) / N2;
+      x = make_double2(x.x * sm, x.y * sm);
+      if (mm == 0) x.y = 0.0;
+    }
+    a[(size_t)(r0 + r) + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * z)] = x;
+  }
+}
+
+// HORFFT backward, in place: s_t = Re X_0 + 2 sum_{1 <= m < NTCHOP} Re(X_m
+// e^{+2 pi i m t/(2 NTH)}), slot j = (s_2j, s_2j+1), slot NTH = 0; inputs times
+// s0 (m = 0) or s1 and, with wr, times wr(min(r, NR-1-r)).
+__global__ void __launch_bounds__(256) k_horfft_bwd_dft(KGrid g, int P, const double2 *__restrict__ tw2, double2 *a,
+                                                        double s0, double s1, const double *wr) {
+  extern __shared__ double2 sc[];
+  const int z = blockIdx.y, r0 = blockIdx.x * P, N = g.nth, N2 = 2 * N, M = g.ntchopdim;
+  for (int idx = threadIdx.x; idx < P * M; idx += blockDim.x) {
+    const int r = idx % P, mm = idx / P;
+    double2 v = make_double2(0.0, 0.0);
+    if (r0 + r < g.nr && mm < g.ntchop) {
+      v = a[(size_t)(r0 + r) + (size_t)g.ndimr * ((size_t)mm + (size_t)g.ndimth * z)];
+      double sm = mm == 0 ? s0 : s1;
+      if (wr) sm *= wr[r0 + r < g.nr - 1 - (r0 + r) ? r0 + r : g.nr - 1 - (r0 + r)];
+      v = make_double2(v.x * sm, v.y * sm);
+    }
+    if (mm == 0) v.y = 0.0;
+    sc[r * M + mm] = v;
+  }
+  __syncthreads();
+  for (int idx = threadIdx.x; idx < P * (N + 1); idx += blockDim.x) {
+    const int r = idx % P, j = idx / P;
+    if (r0 + r >= g.ndimr) continue;
+    double2 out = make_double2(0.0, 0.0);
+    if (j < N && r0 + r < g.nr) {
+      double se = sc[r * M].x, so = sc[r * M].x;
+      int pe = 0, po = 0;  // m (2j) and m (2j+1) mod 2N
+      for (int mm = 1; mm < M; mm++) {
+        pe += 2 * j;
+        while (pe >= N2) pe -= N2;
+        po += 2 * j + 1;
+        while (po >= N2) po -= N2;
+        const double2 c = sc[r * M + mm];
+        se += 2.0 * (c.x * tw2[pe].x + c.y * tw2[pe].y);
+        so += 2.0 * (c.x * tw2[po].x + c.y * tw2[po].y);
+      }
+      out = make_double2(se, so);
+    }
+    a[(size_t)(r0 + r) + (size_t)g.ndimr * ((size_t)j + (size_t)g.ndimth * z)] = out;
+  }
+}
+
+static int launch_verfft_fwd_dft(mlg_ctx *c, const double2 *in, double2 *out) {
+  KGrid g = kgrid(c);
+  const int P = dft_pencils(g.nx);
+  KScope ks(c, K_VERFFT_FWD, 16.0 * g.nr * g.ntchopdim * (double)(g.nx + g.nxchopdim));
+  hipLaunchKernelGGL(k_verfft_fwd_dft, dim3((g.ndimr + P - 1) / P, g.ntchopdim), dim3(256),
+                     sizeof(double2) * (size_t)P * g.nx, c->stream, g, P, c->d.tw_z, in, out);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+static int launch_verfft_bwd_dft(mlg_ctx *c, const double2 *in, double2 *out) {
+  KGrid g = kgrid(c);
+  const int P = dft_pencils(g.nx);
+  KScope ks(c, K_VERFFT_BWD, 16.0 * g.nr * g.ntchopdim * (double)(g.nx + g.nxchopdim));
+  hipLaunchKernelGGL(k_verfft_bwd_dft, dim3((g.ndimr + P - 1) / P, g.ntchopdim), dim3(256),
+                     sizeof(double2) * (size_t)P * g.nxchopdim, c->stream, g, P, c->d.tw_z, in, out);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+static int launch_horfft_fwd_dft(mlg_ctx *c, double2 *a, double s0, double s1) {
+  KGrid g = kgrid(c);
+  const int P = dft_pencils(2 * g.nth);
+  KScope ks(c, K_HORFFT_FWD, 16.0 * g.nr * g.nx * (double)(g.nth + g.ntchopdim));
+  hipLaunchKernelGGL(k_horfft_fwd_dft, dim3((g.ndimr + P - 1) / P, g.ndimx), dim3(256), sizeof(double) * 2 * (size_t)P * g.nth,
+                     c->stream, g, P, c->d.tw_h2, a, s0, s1);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+static int launch_horfft_bwd_dft(mlg_ctx *c, double2 *a, double s0, double s1, const double *wr) {
+  KGrid g = kgrid(c);
+  const int P = dft_pencils(2 * g.nth);
+  KScope ks(c, K_HORFFT_BWD, 16.0 * g.nr * g.nx * (double)(g.nth + g.ntchopdim));
+  hipLaunchKernelGGL(k_horfft_bwd_dft, dim3((g.ndimr + P - 1) / P, g.ndimx), dim3(256),
+                     sizeof(double2) * (size_t)P * g.ntchopdim, c->stream, g, P, c->d.tw_h2, a, s0, s1, wr);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
 #define RET_FFT(x)             \
   do {                         \
     int _r = (x);              \
@@ -562,9 +741,9 @@ __global__ void __launch_bounds__(256) k_horfft_bwd4(KGrid g, const double2 *__r
 int launch_verfft_fwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *const *out) {
   if (n < 1 || n > MAX_TRANSFORM_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
-  if (g.nx > 256) {
+  if (g.nx > 256 || !pow2_len(g.nx)) {
     for (int i = 0; i < n; i++) {
-      const int rc = launch_verfft_fwd_r2(c, in[i], out[i]);
+      const int rc = pow2_len(g.nx) ? launch_verfft_fwd_r2(c, in[i], out[i]) : launch_verfft_fwd_dft(c, in[i], out[i]);
       if (rc) return rc;
     }
     return 0;
@@ -591,14 +770,14 @@ int launch_verfft_bwd_n(mlg_ctx *c, int n, const double2 *const *in, double2 *co
                         double *const *zl, int nzl) {
   if (n < 1 || n > MAX_TRANSFORM_JOBS || nzl < 0 || nzl > MAX_TRANSFORM_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
-  if (g.nx > 256) {
+  if (g.nx > 256 || !pow2_len(g.nx)) {
     if (dz || dm) return MLG_E_ARG;  // derivatives only in the four-step kernel
     if (nzl > 0) {
       const int rc = launch_zero_lns(c, zl, nzl);
       if (rc) return rc;
     }
     for (int i = 0; i < n; i++) {
-      const int rc = launch_verfft_bwd_r2(c, in[i], out[i]);
+      const int rc = pow2_len(g.nx) ? launch_verfft_bwd_r2(c, in[i], out[i]) : launch_verfft_bwd_dft(c, in[i], out[i]);
       if (rc) return rc;
     }
     return 0;
@@ -628,6 +807,10 @@ int launch_verfft_bwd(mlg_ctx *c, const double2 *in, double2 *out) { return laun
 int launch_horfft_fwd_n(mlg_ctx *c, int n, double2 *const *a, double s0, double s1) {
   if (n < 1 || n > MAX_HOR_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
+  if (!pow2_len(g.nth)) {
+    for (int i = 0; i < n; i++) RET_FFT(launch_horfft_fwd_dft(c, a[i], s0, s1));
+    return 0;
+  }
   if (g.nth > 256) {
     for (int i = 0; i < n; i++) {
       RET_FFT(launch_horfft_fwd_r2(c, a[i]));
@@ -652,6 +835,10 @@ int launch_horfft_fwd(mlg_ctx *c, double2 *a) { return launch_horfft_fwd_n(c, 1,
 int launch_horfft_bwd_n(mlg_ctx *c, int n, double2 *const *a, double s0, double s1, const double *wr) {
   if (n < 1 || n > MAX_HOR_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
+  if (!pow2_len(g.nth)) {
+    for (int i = 0; i < n; i++) RET_FFT(launch_horfft_bwd_dft(c, a[i], s0, s1, wr));
+    return 0;
+  }
   if (g.nth > 256) {
     for (int i = 0; i < n; i++) {
       if (s0 != 1.0 || s1 != 1.0 || wr) RET_FFT(launch_scale_pfp_m(c, a[i], s0, s1, wr));

@@ -43,6 +43,7 @@ struct DevTables {
   double2 *tw_z = nullptr;    // e^{-2 pi i j / NX}, j < NX
   double2 *tw_t = nullptr;    // e^{-2 pi i j / NTH}, j < NTH
   double2 *tw_h = nullptr;    // e^{-2 pi i j / (2 NTH)}, j <= NTH
+  double2 *tw_h2 = nullptr;   // e^{-2 pi i j / (2 NTH)}, j < 2 NTH (theta DFTs of other lengths)
   double *hv = nullptr;       // [hv_r | hv_t | hv_x | hd_r | hd_t | hd_x]
   double *red = nullptr;      // reduction scratch
   double *vproj = nullptr;    // VEL2PLN projections [NRCHOPDIM]

@@ -30,6 +30,10 @@ THETA256 = dict(CFG1, nr=16, nth=256, nx=8, nrchop=12, ntchop=86, nxchop=3)
 # kernels and the non-fused NONLIN sequence
 WIDE_THETA = dict(CFG1, nr=16, nth=512, nx=8, nrchop=12, ntchop=100, nxchop=3)
 WIDE_Z = dict(CFG1, nr=16, nth=8, nx=512, nrchop=12, ntchop=3, nxchop=100)
+# lengths that are not powers of two (direct DFT kernels): the reference's
+# own read.input uses NTH = 204 and NX = 608 with NTCHOP = 136, NXCHOP = 304
+THETA204 = dict(CFG1, nr=16, nth=204, nx=8, nrchop=12, ntchop=136, nxchop=3)
+Z608 = dict(CFG1, nr=16, nth=8, nx=608, nrchop=12, ntchop=3, nxchop=304)
 TOL = 1e-12
 # step bar: the GPU error may reach this multiple of the oracle's own change
 # under reversed Legendre summation, taken as the largest change over the
@@ -96,10 +100,10 @@ class Pair:
         return np.asfortranarray(z)
 
 
-@pytest.fixture(scope="module", params=["small", "cfg1", "theta256", "wide_theta", "wide_z"])
+@pytest.fixture(scope="module", params=["small", "cfg1", "theta256", "wide_theta", "wide_z", "theta204", "z608"])
 def P(request):
     p = Pair({"small": SMALL, "cfg1": CFG1, "theta256": THETA256, "wide_theta": WIDE_THETA,
-              "wide_z": WIDE_Z}[request.param])
+              "wide_z": WIDE_Z, "theta204": THETA204, "z608": Z608}[request.param])
     yield p
     p.close()
 
@@ -447,11 +451,15 @@ def test_etd2_steps_small():
     run_steps(SMALL_ETD, 3)
 
 
-@pytest.mark.parametrize("cfg", [dict(SMALL, nth=512, ntchop=40), dict(SMALL_ETD, nx=512, nxchop=40)],
-                         ids=["ab2cn_nth512", "etd2_nx512"])
+@pytest.mark.parametrize("cfg", [dict(SMALL, nth=512, ntchop=40), dict(SMALL_ETD, nx=512, nxchop=40),
+                                 dict(SMALL, nth=12, ntchop=8), dict(SMALL_ETD, nth=12, ntchop=8, nx=20, nxchop=7),
+                                 dict(SMALL_ETD, nth=16, nx=24, nxchop=8)],
+                         ids=["ab2cn_nth512", "etd2_nx512", "ab2cn_nth12", "etd2_nth12_nx20", "etd2_nx24"])
 def test_steps_wide_transforms(cfg):
     """NTH or NX = 512: the steps run the general transforms, the separate
-    product kernels and the non-fused NONLIN sequence."""
+    product kernels and the non-fused NONLIN sequence.  NTH = 12, NX = 20 or
+    24 (not powers of two): the direct DFT kernels, with the fused theta
+    products where NTH is a power of two."""
     run_steps(cfg, 2)
 
 
