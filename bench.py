@@ -45,6 +45,13 @@ def parse():
                     help="skip the split hi/lo FP32 Legendre measurement (timing and error against FP64)")
     ap.add_argument("--no-adjoint", action="store_true",
                     help="skip the adjoint measurement (128x64x64 AB2-CN, 40-step gradient)")
+    ap.add_argument("--no-sweep", action="store_true",
+                    help="skip the batched uneven sweep measurement (48 AB2-CN runs)")
+    ap.add_argument("--sweep-only", action="store_true",
+                    help="only run the sweep measurement (the modes of --sweep-modes) and print its JSON")
+    ap.add_argument("--sweep-steps", type=int, default=50, help="steps per run of the sweep (2 untimed)")
+    ap.add_argument("--sweep-modes", default=",".join(SWEEP_MODES),
+                    help="comma-separated sweep modes to time (--sweep-only)")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="only run 20 copies of a 384x257x256 complex field (407 MB each way), "
                          "to scale HBM byte counters")
@@ -214,6 +221,122 @@ def adjoint_measurement(H):
         g.close()
 
 
+SWEEP_MODES = ("alone", "ungrouped", "grouped", "ungrouped_streams", "grouped_streams", "grouped_by_grid_streams")
+
+
+def sweep_measurement(H, nsteps, warm=2, modes=SWEEP_MODES):
+    """The batched uneven sweep (48 AB2-CN runs of mlegs_amd.configs.
+    cfg4_runs, 50 steps each): aggregate run-steps/s of the last
+    nsteps - warm steps of every run, (a) each run alone, replaying its own
+    captured step graph, runs one after another; (b) one sweep graph per step
+    for all runs, the runs' segments independent; (c) the same with the RTRAN
+    backward products of all runs grouped into one persistent launch per place
+    in the step, tile shapes from the tuner.  The first `warm` steps (eager
+    step, recording, tuning) are untimed.  Every mode starts from the same
+    initial states, and its states after the last step are compared with
+    mode (a)'s (bitwise equal)."""
+    import numpy as np
+    import torch
+    from mlegs_amd.configs import LAMB_OSEEN_Q, cfg4_runs, seeded_noise
+
+    cfgs = cfg4_runs()
+    grids, fields, init = [], [], []
+    for cfg in cfgs:
+        g = H.Grid(cfg)
+        f = [g.scalar() for _ in range(3)]
+        H.qvortex(g, LAMB_OSEEN_Q, f[0], f[1])
+        p, _, ln = f[0].download()
+        init.append([(p + seeded_noise(cfg, 1), ln), (seeded_noise(cfg, 2), 0.0), (seeded_noise(cfg, 3), 0.0)])
+        grids.append(g)
+        fields.append(f)
+
+    def start():
+        solvers = []
+        for g, f, st in zip(grids, fields, init):
+            for x, (a, ln) in zip(f, st):
+                x.upload(a, H.FFF, ln)
+            s = H.Solver(g)
+            s.initialize(*f)
+            solvers.append(s)
+        return solvers
+
+    def alone(s, f, n, codes, i):
+        # a run whose DIAGNOST check fails has still taken its n steps
+        try:
+            s.time_stepping(*f, n, graph=True)
+        except H.MlegsError as e:
+            if e.code >= 100:
+                raise
+            codes[i] = codes[i] or e.code
+
+    res, final, info, codes = {}, {}, None, {}
+    timed = nsteps - warm
+    for mode in modes:
+        solvers = start()
+        sw = None
+        codes[mode] = [0] * len(cfgs)
+        if mode == "alone":
+            for i, (s, f) in enumerate(zip(solvers, fields)):
+                alone(s, f, warm, codes[mode], i)
+        else:
+            sw = H.Sweep([(s, *f) for s, f in zip(solvers, fields)], grouped=mode.startswith("grouped"), tune=True,
+                         streams=mode.endswith("streams"), by_grid="by_grid" in mode)
+            sw.steps(warm, check=False)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        if sw is None:
+            for i, (s, f) in enumerate(zip(solvers, fields)):
+                alone(s, f, timed, codes[mode], i)
+        else:
+            codes[mode] = sw.steps(timed, check=False)
+        torch.cuda.synchronize()
+        el = time.perf_counter() - t0
+        res[mode] = len(cfgs) * timed / el
+        final[mode] = [[np.ascontiguousarray(x.download()[0]).view(np.int64) for x in f] for f in fields]
+        if sw is not None:
+            if mode.startswith("grouped") and info is None:
+                info = dict(sw.info(), mode=mode)
+            sw.close()
+        for s in solvers:
+            s.close()
+        print(f"bench: sweep {mode}: {res[mode]:.1f} run-steps/s", file=sys.stderr, flush=True)
+    same = {m: all(np.array_equal(a, b) for fa, fb in zip(final[m], final["alone"]) for a, b in zip(fa, fb))
+            for m in final if m != "alone"} if "alone" in final else {}
+    for g, f in zip(grids, fields):
+        for x in f:
+            x.close()
+        g.close()
+    out = dict(workload="48 AB2-CN runs, NR in {64,96,128,192} x NTH=NX in {32,64,128}, NRCHOP=2NR/3, "
+                        "NTCHOP=NXCHOP=NTH/3+1, seeded ELL/OMEGA/BV0, nu=kappa=5e-4, dt=1e-3; "
+                        f"{nsteps} steps per run, the last {timed} timed",
+               unit="run-steps/s")
+    swept = {m: v for m, v in res.items() if m != "alone"}
+    if swept:
+        best = max(swept, key=swept.get)
+        out["run_steps_per_s"] = round(swept[best], 1)
+        out["best_mode"] = best
+    for m in res:
+        out[f"{m}_run_steps_per_s"] = round(res[m], 1)
+    if "alone" in res and swept:
+        out["best_over_alone"] = round(max(swept.values()) / res["alone"], 3)
+    if "alone" in res and "grouped" in res:
+        out["grouped_over_alone"] = round(res["grouped"] / res["alone"], 3)
+    out["bitwise_equal_to_alone"] = same
+    out["grouped_graph"] = info
+    last = codes[modes[-1]]
+    out["runs_failing_diagnost"] = [i for i, c in enumerate(last) if c]
+    out["modes"] = ("alone: each run's captured step graph replayed, runs one after another; ungrouped/grouped: "
+                    "one graph launch per step for all runs (grouped: the RTRAN backward products of all runs "
+                    "in one persistent launch per place in the step, tuned tiles); *_streams: each run's "
+                    "segments launched on its own stream (4 hardware queues), grouped launches joined by "
+                    "events; by_grid: one grouped launch per class of runs of one grid")
+    out["diagnost_codes_same_in_every_mode"] = all(c == last for c in codes.values())
+    out["note"] = ("runs with small ELL or large NTH fail the per-step DIAGNOST divergence check (code 12) "
+                   "within 50 steps at dt=1e-3 -- the FP64 oracle fails them at the same step -- and keep "
+                   "stepping with the others; the timing is of all 48 runs")
+    return out
+
+
 def cpu_baseline(cfg, nsteps):
     """The oracle (FP64 C, OpenMP) on this host: steps/s over `nsteps` steps."""
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -270,6 +393,9 @@ def main():
         sys.exit(2)
     if a.calibrate_copy:
         calibrate_copy(H, cfg)
+        return
+    if a.sweep_only:
+        print(json.dumps(sweep_measurement(H, a.sweep_steps, modes=tuple(a.sweep_modes.split(",")))), flush=True)
         return
     g = H.Grid(cfg)
     psi, chi, b = initial_state(H, g, cfg)
@@ -344,6 +470,10 @@ def main():
     if not a.no_adjoint:
         print("bench: adjoint/forward cost (128x64x64 AB2-CN)", file=sys.stderr, flush=True)
         adjoint = adjoint_measurement(H)
+    sweep = None
+    if not a.no_sweep:
+        print("bench: batched uneven sweep (48 runs)", file=sys.stderr, flush=True)
+        sweep = sweep_measurement(H, a.sweep_steps)
     cpu = None
     if a.cpu_steps > 0:
         print("bench: timing the CPU baseline", file=sys.stderr, flush=True)
@@ -372,6 +502,7 @@ def main():
         "stages": stages,
         "split_precision": split,
         "adjoint": adjoint,
+        "sweep": sweep,
     }
     print(json.dumps(out), flush=True)
 

@@ -304,6 +304,65 @@ int mlg_solver_history(mlg_solver *s, mlg_field **hpsi, mlg_field **hchi, mlg_fi
 int mlg_adjoint_op(int op, mlg_field *const *f, int nf, double alp);
 
 /* ------------------------------------------------------------------ */
+/* Batched sweep of independent runs (no counterpart in the reference, which
+ * runs one case per job through SOLVER_T%TIME_STEPPING,
+ * src/mod_march.f90:244-300; for parameter-sweep callers).  Each run is a
+ * solver with its own context (grid, parameters) and its state psi, chi, b
+ * (FFF), already started (mlg_initialize).  mlg_sweep_steps advances every
+ * run by nsteps steps; every run's results are bitwise those of
+ * mlg_time_steps on that run alone.  The first call takes one eager step of
+ * every run, then records each run's step into one hipGraph for all runs:
+ * the runs' segments are independent; with MLG_SWEEP_GROUPED, the RTRAN
+ * backward products at the same place of every run's step are one grouped
+ * persistent launch over (run, mode bucket) jobs, with tile shapes chosen
+ * by the tuner (MLG_SWEEP_TUNE) or given as (shape + 1) << 4 (shapes as
+ * mlg_tune_rtran_bwd's).  MLG_SWEEP_STREAMS: instead of one graph, each
+ * run's segments are graphs of their own launched on the run's stream (runs
+ * on different streams run at once), the grouped launches on the sweep's
+ * stream, joined to the runs by events.  MLG_SWEEP_BY_GRID (streams mode
+ * only): one grouped launch per place and class of runs of one grid (NR,
+ * NTH, NX and truncations), so that a launch waits only for runs of equal
+ * cost.
+ * Grouping needs FP64 Legendre
+ * products.  The
+ * graph is recorded again when a run's field buffers or truncation change.
+ * Forced shapes take bits 4..6 of flags.
+ * The sweep owns the temporaries its graph uses until destroyed.
+ *
+ * A run whose per-step DIAGNOST check fails keeps stepping with the others
+ * (runs are independent jobs): mlg_sweep_steps returns the code of the
+ * lowest-numbered run that has one (after all steps), mlg_sweep_codes each
+ * run's first code since the sweep was created (0: none).  Library and HIP
+ * errors (MLG_E_*) are returned at once.
+ *
+ * mlg_sweep_info: out[0] grouped positions per step, [1] graph nodes,
+ * [2] grouped jobs per step, [3..5] jobs per tile shape, [6] grouped
+ * launches per step, [7] runs.
+ *
+ * mlg_tune_rtran_bwd: for each mode bucket of the context's current
+ * truncation (by NRCHOPS(m): 0: > 128, 1: 65..128, 2: 33..64, 3: <= 32),
+ * times a backward product of nfields fields over the bucket's modes with
+ * each tile shape (0: 64 nodes x 64 real columns, 1: 32x64, 2: 64x32) on
+ * scratch buffers: us[4 bucket + shape] in microseconds (median of 5
+ * launches; us[4 bucket + 3] = 0), shape[bucket] the fastest (-1: no
+ * modes).  The choice is kept per grid and used by later sweeps. */
+typedef struct mlg_sweep mlg_sweep;
+#define MLG_SWEEP_GROUPED 1
+#define MLG_SWEEP_TUNE 2
+#define MLG_SWEEP_STREAMS 4
+#define MLG_SWEEP_BY_GRID 8
+int mlg_sweep_create(int nruns, mlg_solver *const *solvers, mlg_field *const *psi, mlg_field *const *chi,
+                     mlg_field *const *b, int flags, mlg_sweep **out);
+int mlg_sweep_steps(mlg_sweep *sw, int nsteps);
+int mlg_sweep_info(const mlg_sweep *sw, int out[8]);
+int mlg_sweep_codes(const mlg_sweep *sw, int *codes);
+/* algorithmic GFLOP of one step's grouped launches: 2 NR NXCHOPDIM x the
+ * sum over the job's modes of NRCHOPS(m), per field, summed over jobs */
+int mlg_sweep_work(const mlg_sweep *sw, double *gflop_per_step);
+int mlg_sweep_destroy(mlg_sweep *sw);
+int mlg_tune_rtran_bwd(mlg_ctx *ctx, int nfields, int shape[4], double us[16]);
+
+/* ------------------------------------------------------------------ */
 /* Kernel event timer.  Instrumentation with no counterpart in the
  * reference (which times whole steps with CPU_TIME).  mlg_timer_enable(1)
  * clears and starts recording HIP events around every launch of the timed

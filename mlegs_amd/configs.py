@@ -7,6 +7,7 @@ dictionaries whose keys match the fields of ``mlg_params``:
 * ``CFG2`` -- 384x256x256 ETD2 production case (the headline workload).
 * ``CFG3`` -- CFG2 with split hi/lo FP32 Legendre products (``split=1``:
   ``mlegs_amd.hip.Grid`` selects ``legendre_precision = "split32"``).
+* ``cfg4_runs()`` -- the 48 runs of uneven resolution of the batched sweep.
 * ``CFG5`` -- 128x64x64 AB2-CN case of the adjoint (gradient of the
   perturbation kinetic energy after 40 steps; mlegs_amd.adjoint).
 
@@ -42,6 +43,28 @@ CFG5 = dict(nr=128, nth=64, nx=64, nrchop=96, ntchop=22, nxchop=22, minc=1, mkli
             nsteps=40, noise=1e-5)
 
 CONFIGS = {"cfg1": CFG1, "cfg2": CFG2, "cfg3": CFG3, "cfg5": CFG5}
+
+# CFG4: the batched uneven sweep -- 48 AB2-CN runs of 50 steps, stepped
+# together (mlegs_amd.hip.Sweep).  NR cycles through CFG4_NR and NTH = NX
+# through CFG4_NTH (each of the 12 grids 4 times), NRCHOP = floor(2 NR / 3),
+# NTCHOP = NXCHOP = floor(NTH / 3) + 1; ELL, OMEGA and the buoyancy frequency
+# BV0 are seeded uniform draws on [0.5, 3], [0, 2], [0, 4].
+CFG4_NR = (64, 96, 128, 192)
+CFG4_NTH = (32, 64, 128)
+CFG4_SEED = 20261204
+CFG4_BASE = dict(CFG1, nu=5e-4, kappa=5e-4, adams=1, dt=1e-3, nsteps=50, noise=1e-4)
+
+
+def cfg4_runs(nruns=48, seed=CFG4_SEED):
+    """The sweep's run configurations (list of dicts)."""
+    rng = np.random.default_rng(seed)
+    runs = []
+    for i in range(nruns):
+        nr, nth = CFG4_NR[i % len(CFG4_NR)], CFG4_NTH[(i // len(CFG4_NR)) % len(CFG4_NTH)]
+        ell, omega, bv0 = rng.uniform(0.5, 3.0), rng.uniform(0.0, 2.0), rng.uniform(0.0, 4.0)
+        runs.append(dict(CFG4_BASE, nr=nr, nth=nth, nx=nth, nrchop=(2 * nr) // 3, ntchop=nth // 3 + 1,
+                         nxchop=nth // 3 + 1, ell=float(ell), omega=float(omega), bv0=float(bv0)))
+    return runs
 
 LAMB_OSEEN_Q = 1.0 / (2.0 * math.pi)   # Gamma = 1, reference Q = Gamma/(2 pi)
 NOISE_SEED = 20261202

@@ -1625,6 +1625,8 @@ extern "C" int mlg_diagnost(const mlg_field *psi, const mlg_field *chi) {
   return mlg_synchronize(c);
 }
 
+int mlg::step_once(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) { return one_step(s, psi, chi, b); }
+
 extern "C" int mlg_time_step(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
   if (!s || !psi || !chi || !b) return MLG_E_ARG;
   RET(one_step(s, psi, chi, b));

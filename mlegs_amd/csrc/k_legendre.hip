@@ -11,6 +11,8 @@
 // MFMA: v_mfma_f64_16x16x4f64 -- lane l supplies A[l%16][l/16] and
 // B[l/16][l%16]; the 16x16 result holds C[l/16 + 4v][l%16] in register v
 // (measured on gfx950; the f32 16x16x4 variant uses C[4(l/16)+v][l%16]).
+#include <algorithm>
+
 #include "mlegs_internal.h"
 #include "xcd.h"
 
@@ -189,42 +191,52 @@ int build_split_tables(mlg_ctx *c) {
 // Backward: FFF (n,m,k) -> PFF (r,m,k).
 // BE(i,c) = sum_j PF(i,2j) a(2j,c), BO(i,c) = sum_j PF(i,2j+1) a(2j+1,c);
 // f(x_i) = BE+BO, f(x_{NR-1-i}) = BE-BO; LN*(-log(1-x)) added at m=0,k=0.
-// Block tile: 64 radial nodes x 64 real columns (32 complex k); 4 waves as
-// 2x2, each wave 32x32 = 2x2 MFMA tiles per parity.  K chunk: 16 pairs.
+// Block tile: TI radial nodes x TC real columns (TC/2 complex k); the 4
+// waves as (TI/32) x (128/TI), each wave 32 nodes (2 MFMA tiles) by
+// TC TI/128 columns per parity.  K chunk: 16 pairs.  Every tile shape sums
+// each output over the same sequence (chunks of 16 pairs, 4-deep MFMA steps
+// in order), so all shapes give bitwise equal results.
 // -------------------------------------------------------------------------
-#define BW_TI 64
-#define BW_TC 64
 #define BW_JC 16
 
 // LDS: one K-chunk stage of the table and data tiles (the next chunk waits
 // in registers); after the K loop the same memory stages the output, one
-// half (S = BE+BO, then D = BE-BO) at a time.  33 KB per block: four blocks
-// per CU.  Split mode: (hi, lo) pairs in place of the doubles, and the
-// chunk's grid exponents per complex column and per table row.
-template <bool SPLIT>
+// half (S = BE+BO, then D = BE-BO) at a time.  (64, 64): 33 KB per block,
+// four blocks per CU.  Split mode: (hi, lo) pairs in place of the doubles,
+// and the chunk's grid exponents per complex column and per table row.
+template <bool SPLIT, int TI, int TC>
 union RtranBwdLds {
   struct {
-    typename LegElem<SPLIT>::T A[2][BW_JC][BW_TI];  // [parity][j][i]
-    typename LegElem<SPLIT>::T B[2][BW_JC][BW_TC];  // [parity][j][col]
-    int ce[SPLIT ? BW_TC / 2 : 1];
-    __align__(16) int re[SPLIT ? BW_TI : 1];
+    typename LegElem<SPLIT>::T A[2][BW_JC][TI];  // [parity][j][i]
+    typename LegElem<SPLIT>::T B[2][BW_JC][TC];  // [parity][j][col]
+    int ce[SPLIT ? TC / 2 : 1];
+    __align__(16) int re[SPLIT ? TI : 1];
   } k;
-  double E[BW_TC][BW_TI + 1];  // [col][i]
+  double E[TC][TI + 1];  // [col][i]
 };
 
-template <bool SPLIT>
-__device__ __forceinline__ void rtran_bwd_body(const KGrid &g, const typename LegElem<SPLIT>::T *__restrict__ pf,
+// logical tiles of one backward product over the modes g.mlo .. g.mlo+g.mcnt-1
+template <int TI, int TC>
+__host__ __device__ inline int rtran_bwd_tiles(const KGrid &g, int nj) {
+  return ((2 * g.nxchopdim + TC - 1) / TC) * ((g.nrh + TI - 1) / TI) * g.mcnt * nj;
+}
+
+// One block tile (logical id lid) of a backward product.  Logical order:
+// radial tile fastest, then field, column tile, m: the radial tiles that
+// read one data tile run one after another on one XCD (the data come from
+// L2 after the first), and the mode's table tiles (all radial tiles of m)
+// stay in that L2 for every field and column tile.
+template <bool SPLIT, int TI, int TC>
+__device__ __forceinline__ void rtran_bwd_tile(const KGrid &g, const typename LegElem<SPLIT>::T *__restrict__ pf,
                                                const TransformJobs &jobs, int nj, const double *__restrict__ lnx,
-                                               const int *__restrict__ rowe, RtranBwdLds<SPLIT> &L) {
+                                               const int *__restrict__ rowe, int lid, RtranBwdLds<SPLIT, TI, TC> &L) {
   typedef typename LegElem<SPLIT>::T TE;
-  // logical block (radial tile fastest, then field, column tile, m): the
-  // radial tiles that read one data tile run one after another on one XCD
-  // (the data come from L2 after the first), and the mode's table tiles
-  // (all radial tiles of m) stay in that L2 for every field and column tile
+  constexpr int WI = TI / 32, WC = 4 / WI, CW = TC / WC, SC = CW / 16;
+  constexpr int QA = TI / 8, SA = 256 / TI, QB = TC / 16, QE = TI * TC / 512;
+  static_assert((TI == 32 || TI == 64) && SC >= 1 && SC <= 2 && CW % 16 == 0, "tile shape");
+  static_assert(QA * SA == 2 * BW_JC && QB * 8 == TC / 2 && QE * 512 == TI * TC, "staging covers the tile");
   const int nk = g.nxchopdim;
-  const int ntiles_c = (2 * nk + BW_TC - 1) / BW_TC, ntiles_i = (g.nrh + BW_TI - 1) / BW_TI;
-  const int lid = xcd_logical(ntiles_c * ntiles_i * g.mcnt * nj);
-  if (lid < 0) return;
+  const int ntiles_c = (2 * nk + TC - 1) / TC, ntiles_i = (g.nrh + TI - 1) / TI;
   const int ti = lid % ntiles_i, r1 = lid / ntiles_i;
   const int fj = r1 % nj, r2 = r1 / nj;
   const int tc = r2 % ntiles_c, mm = g.mlo + r2 / ntiles_c;
@@ -233,18 +245,23 @@ __device__ __forceinline__ void rtran_bwd_body(const KGrid &g, const typename Le
   double2 *__restrict__ out = jobs.out[fj];
   const int mv = mm * g.minc;
   const int nn = d_nrchops(g.nrchop, mv);
-  const int i0 = ti * BW_TI, c0 = tc * BW_TC;
-  const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
-  const int wi = wave & 1, wc = wave >> 1;
+  const int i0 = ti * TI, c0 = tc * TC;
+  // the thread id through an empty asm: in the grouped kernel's tile loop
+  // the compiler then keeps no thread-dependent offsets of every tile shape
+  // live across tiles (they would spill)
+  int tid = threadIdx.x;
+  asm volatile("" : "+v"(tid));
+  const int lane = tid & 63, wave = tid >> 6;
+  const int wi = wave % WI, wc = wave / WI;
 
-  v4d acc[2][2][2];  // [parity][sub i][sub c]
-  v4f hh[2][2][2], xx[2][2][2];
+  v4d acc[2][2][SC];  // [parity][sub i][sub c]
+  v4f hh[2][2][SC], xx[2][2][SC];
 #pragma unroll
   for (int p = 0; p < 2; p++)
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int b = 0; b < 2; b++) {
+      for (int b = 0; b < SC; b++) {
         acc[p][a][b] = (v4d){0, 0, 0, 0};
         hh[p][a][b] = xx[p][a][b] = (v4f){0.f, 0.f, 0.f, 0.f};
       }
@@ -253,42 +270,42 @@ __device__ __forceinline__ void rtran_bwd_body(const KGrid &g, const typename Le
   const int njp = (nn + 1) / 2;  // (even, odd) pairs
   const int nch = (njp + BW_JC - 1) / BW_JC;
   const int nchr = (g.npf + 2 * BW_JC - 1) / (2 * BW_JC);  // chunks of a split table row
-  // per-thread prefetch registers: 8 table values, 4 complex data values
+  // per-thread prefetch registers: QA table values, QB complex data values
   // (split: and one table row's grid exponent)
-  TE ra[8];
-  double2 rb[4];
+  TE ra[QA];
+  double2 rb[QB];
   int rre = -1022;
   // 32-bit element offsets: lane part fixed, chunk and q parts uniform
-  const int ta_i = i0 + (tid & 63), ta_n = tid >> 6;          // table: i, n - 2 j0 - 4 q
+  const int ta_i = i0 + (tid % TI), ta_n = tid / TI;          // table: i, n - 2 j0 - SA q
   const int tb_n = tid & 31, tb_k = (c0 >> 1) + (tid >> 5);   // data: n - 2 j0, k - 8 q
   const int kstride = g.nrchopdim * g.ntchopdim;
   const double2 *inm = in + (size_t)g.nrchopdim * mm;
   auto fetch = [&](int ch) {
     const int j0 = ch * BW_JC;
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
-      const int n = 2 * j0 + ta_n + 4 * q;
+    for (int q = 0; q < QA; q++) {
+      const int n = 2 * j0 + ta_n + SA * q;
       ra[q] = (n < nn && ta_i < g.nrh) ? pfm[n * g.nrh + ta_i] : TE{};
     }
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < QB; q++) {
       const int n = 2 * j0 + tb_n, k = tb_k + 8 * q;
       rb[q] = (n < nn && k < nk) ? inm[n + kstride * k] : make_double2(0.0, 0.0);
     }
     if constexpr (SPLIT)
-      if (tid < BW_TI) rre = ta_i < g.nrh ? rowe[((size_t)mm * g.nrh + ta_i) * nchr + ch] : -1022;
+      if (tid < TI) rre = ta_i < g.nrh ? rowe[((size_t)mm * g.nrh + ta_i) * nchr + ch] : -1022;
   };
   auto stash = [&]() {
 #pragma unroll
-    for (int q = 0; q < 8; q++) {
+    for (int q = 0; q < QA; q++) {
       int idx = tid + 256 * q;
-      int i = idx & 63, nl = idx >> 6;
+      int i = idx % TI, nl = idx / TI;
       L.k.A[nl & 1][nl >> 1][i ^ lsw(nl >> 1, 0)] = ra[q];
     }
     if constexpr (SPLIT)
-      if (tid < BW_TI) L.k.re[tid] = rre;
+      if (tid < TI) L.k.re[tid] = rre;
 #pragma unroll
-    for (int q = 0; q < 4; q++) {
+    for (int q = 0; q < QB; q++) {
       int idx = tid + 256 * q;
       int nl = idx & 31, kl = idx >> 5;
       const int sw = lsw(nl >> 1, nl & 1);
@@ -307,7 +324,9 @@ __device__ __forceinline__ void rtran_bwd_body(const KGrid &g, const typename Le
   };
   // the wave's column subtiles that hold data (the last tile of a mode is
   // partly padding); the others skip their products (wave-uniform)
-  const bool live0 = c0 + wc * 32 < 2 * nk, live1 = c0 + wc * 32 + 16 < 2 * nk;
+  bool live[SC];
+#pragma unroll
+  for (int t = 0; t < SC; t++) live[t] = c0 + wc * CW + 16 * t < 2 * nk;
   if (nch > 0) fetch(0);
   for (int ch = 0; ch < nch; ch++) {
     __syncthreads();
@@ -317,42 +336,36 @@ __device__ __forceinline__ void rtran_bwd_body(const KGrid &g, const typename Le
     const int jleft = njp - ch * BW_JC;  // pairs of this chunk that hold data
 #pragma unroll
     for (int ks = 0; ks < BW_JC; ks += 4) {
-      if (ks >= jleft || !live0) break;  // zero padding (chunk tail, or all the wave's columns)
+      if (ks >= jleft || !live[0]) break;  // zero padding (chunk tail, or all the wave's columns)
       const int kr = ks + (lane >> 4);
 #pragma unroll
       for (int p = 0; p < 2; p++) {
         const int sa = lsw(kr, 0), sb = lsw(kr, p);
         const TE a0 = L.k.A[p][kr][(wi * 32 + (lane & 15)) ^ sa];
         const TE a1 = L.k.A[p][kr][(wi * 32 + 16 + (lane & 15)) ^ sa];
-        const TE b0 = L.k.B[p][kr][(wc * 32 + (lane & 15)) ^ sb];
-        const TE b1 = L.k.B[p][kr][(wc * 32 + 16 + (lane & 15)) ^ sb];
-        if constexpr (SPLIT) {
-          mfma_split(a0, b0, hh[p][0][0], xx[p][0][0]);
-          mfma_split(a1, b0, hh[p][1][0], xx[p][1][0]);
-          if (live1) {
-            mfma_split(a0, b1, hh[p][0][1], xx[p][0][1]);
-            mfma_split(a1, b1, hh[p][1][1], xx[p][1][1]);
-          }
-        } else {
-          acc[p][0][0] = mfma16x16x4(a0, b0, acc[p][0][0]);
-          acc[p][1][0] = mfma16x16x4(a1, b0, acc[p][1][0]);
-          if (live1) {
-            acc[p][0][1] = mfma16x16x4(a0, b1, acc[p][0][1]);
-            acc[p][1][1] = mfma16x16x4(a1, b1, acc[p][1][1]);
+#pragma unroll
+        for (int t = 0; t < SC; t++) {
+          if (t > 0 && !live[t]) break;
+          const TE bt = L.k.B[p][kr][(wc * CW + 16 * t + (lane & 15)) ^ sb];
+          if constexpr (SPLIT) {
+            mfma_split(a0, bt, hh[p][0][t], xx[p][0][t]);
+            mfma_split(a1, bt, hh[p][1][t], xx[p][1][t]);
+          } else {
+            acc[p][0][t] = mfma16x16x4(a0, bt, acc[p][0][t]);
+            acc[p][1][t] = mfma16x16x4(a1, bt, acc[p][1][t]);
           }
         }
       }
     }
     if constexpr (SPLIT) {
-      const int ec0 = L.k.ce[(wc * 32 + (lane & 15)) >> 1];
-      const int ec1 = L.k.ce[(wc * 32 + 16 + (lane & 15)) >> 1];
 #pragma unroll
-      for (int a = 0; a < 2; a++) {
-        const int4 er = *(const int4 *)&L.k.re[wi * 32 + a * 16 + 4 * (lane >> 4)];
+      for (int t = 0; t < SC; t++) {
+        const int ec = L.k.ce[(wc * CW + 16 * t + (lane & 15)) >> 1];
 #pragma unroll
-        for (int p = 0; p < 2; p++) {
-          fold_split(acc[p][a][0], hh[p][a][0], xx[p][a][0], er, ec0);
-          fold_split(acc[p][a][1], hh[p][a][1], xx[p][a][1], er, ec1);
+        for (int a = 0; a < 2; a++) {
+          const int4 er = *(const int4 *)&L.k.re[wi * 32 + a * 16 + 4 * (lane >> 4)];
+#pragma unroll
+          for (int p = 0; p < 2; p++) fold_split(acc[p][a][t], hh[p][a][t], xx[p][a][t], er, ec);
         }
       }
     }
@@ -366,18 +379,19 @@ __device__ __forceinline__ void rtran_bwd_body(const KGrid &g, const typename Le
 #pragma unroll
     for (int a = 0; a < 2; a++)
 #pragma unroll
-      for (int b = 0; b < 2; b++)
+      for (int b = 0; b < SC; b++)
 #pragma unroll
         for (int v = 0; v < 4; v++) {
           int il = wi * 32 + a * 16 + acc_row<SPLIT>(lane, v);
-          int cl = wc * 32 + b * 16 + (lane & 15);
+          int cl = wc * CW + b * 16 + (lane & 15);
           double e = acc[0][a][b][v], o = acc[1][a][b][v];
           L.E[cl][il] = h == 0 ? e + o : e - o;
         }
     __syncthreads();
-    for (int q = 0; q < 8; q++) {
-      int idx = tid + 256 * q;  // 0..2047: i fastest
-      int il = idx & 63, kl = idx >> 6;
+#pragma unroll
+    for (int q = 0; q < QE; q++) {
+      int idx = tid + 256 * q;  // i fastest
+      int il = idx % TI, kl = idx / TI;
       int ig = i0 + il, k = (c0 >> 1) + kl;
       if (ig < g.nrh && k < nk) {
         size_t col = (size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k);
@@ -392,7 +406,7 @@ __device__ __forceinline__ void rtran_bwd_body(const KGrid &g, const typename Le
   // of the PFF result)
   if (ti == 0) {
     const int pad = g.ndimr - g.nr;
-    for (int idx = tid; idx < pad * (BW_TC / 2); idx += 256) {
+    for (int idx = tid; idx < pad * (TC / 2); idx += 256) {
       const int kl = idx / pad, k = (c0 >> 1) + kl;
       if (k < nk) out[(size_t)g.ndimr * ((size_t)mm + (size_t)g.ntchopdim * k) + g.nr + (idx - kl * pad)] = make_double2(0.0, 0.0);
     }
@@ -401,15 +415,61 @@ __device__ __forceinline__ void rtran_bwd_body(const KGrid &g, const typename Le
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
 k_rtran_bwd(KGrid g, const double *__restrict__ pf, TransformJobs jobs, int nj, const double *__restrict__ lnx) {
-  __shared__ RtranBwdLds<false> L;
-  rtran_bwd_body<false>(g, pf, jobs, nj, lnx, nullptr, L);
+  __shared__ RtranBwdLds<false, 64, 64> L;
+  const int lid = xcd_logical(rtran_bwd_tiles<64, 64>(g, nj));
+  if (lid < 0) return;
+  rtran_bwd_tile<false, 64, 64>(g, pf, jobs, nj, lnx, nullptr, lid, L);
 }
 
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_rtran_bwd_split(KGrid g, const float2 *__restrict__ pf, TransformJobs jobs, int nj, const double *__restrict__ lnx,
                   const int *__restrict__ rowe) {
-  __shared__ RtranBwdLds<true> L;
-  rtran_bwd_body<true>(g, pf, jobs, nj, lnx, rowe, L);
+  __shared__ RtranBwdLds<true, 64, 64> L;
+  const int lid = xcd_logical(rtran_bwd_tiles<64, 64>(g, nj));
+  if (lid < 0) return;
+  rtran_bwd_tile<true, 64, 64>(g, pf, jobs, nj, lnx, rowe, lid, L);
+}
+
+// ---- grouped backward products: any number of (run, mode range) jobs in
+// one persistent launch, each job with its own tile shape.
+constexpr size_t cmax(size_t a, size_t b) { return a > b ? a : b; }
+constexpr size_t BWD_LDS_BYTES =
+    cmax(sizeof(RtranBwdLds<false, 64, 64>), cmax(sizeof(RtranBwdLds<false, 32, 64>), sizeof(RtranBwdLds<false, 64, 32>)));
+template <int TI, int TC>
+__device__ __forceinline__ void bwd_job_tile(const BwdJob &j, int lid, unsigned char *lds) {
+  rtran_bwd_tile<false, TI, TC>(j.g, j.pf, j.jobs, j.nj, j.lnx, nullptr, lid,
+                                *reinterpret_cast<RtranBwdLds<false, TI, TC> *>(lds));
+}
+
+// Logical tiles 0 .. total-1, job d holding tiles b0[d] .. b0[d+1]-1
+// (b0[njobs] = total).  The tiles split into 8 contiguous parts, one per
+// XCD; the blocks of one XCD (linear ids = x mod 8; gridDim.x a multiple of
+// 8) walk its part with stride gridDim.x / 8, a barrier between tiles.
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(3, 3)))
+k_rtran_bwd_grouped(const BwdJob *__restrict__ jb, const int *__restrict__ b0, int njobs, int total) {
+  __shared__ __align__(16) unsigned char lds[BWD_LDS_BYTES];
+  const int per = (total + 7) >> 3;
+  const int x = blockIdx.x & 7, stride = gridDim.x >> 3;
+  const int lo = x * per, hi = min(total, lo + per);
+  const int first = lo + (int)(blockIdx.x >> 3);
+  // the job of the block's first tile: the last d with b0[d] <= first
+  int d = 0;
+  for (int span = njobs; span > 1;) {
+    const int h = span >> 1;
+    if (b0[d + h] <= first) d += h;
+    span -= h;
+  }
+  for (int lid = first; lid < hi; lid += stride) {
+    while (d + 1 < njobs && b0[d + 1] <= lid) d++;  // tiles of a block only increase
+    const BwdJob &j = jb[d];
+    const int t = lid - b0[d];
+    switch (j.shape) {
+      case BWD_TILE_32x64: bwd_job_tile<32, 64>(j, t, lds); break;
+      case BWD_TILE_64x32: bwd_job_tile<64, 32>(j, t, lds); break;
+      default: bwd_job_tile<64, 64>(j, t, lds); break;
+    }
+    __syncthreads();
+  }
 }
 
 int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double *const *ln_in, double2 *const *out) {
@@ -421,13 +481,13 @@ int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double
     jobs.ln[i] = ln_in[i];
     jobs.out[i] = out[i];
   }
-  int nti = (g.nrh + BW_TI - 1) / BW_TI, ntc = (2 * g.nxchopdim + BW_TC - 1) / BW_TC;
   // 2 NR sum_m NRCHOPS(m) NXCHOPDIM flops per field (real table x complex data, folded)
   const double per_k = n * 2.0 * g.nr * g.nxchopdim;
   if (!c->leg_split) {
+    if (c->rec) return sweep_defer_bwd(c, g, jobs, n);
     KScope ks(c, K_RTRAN_BWD, per_k * (double)sum_nrchops(c));
-    hipLaunchKernelGGL(k_rtran_bwd, dim3(xcd_grid(nti * ntc * g.mcnt * n)), dim3(256), 0, c->stream, g, c->d.pf, jobs, n,
-                       c->d.lnx);
+    hipLaunchKernelGGL(k_rtran_bwd, dim3(xcd_grid(rtran_bwd_tiles<64, 64>(g, n))), dim3(256), 0, c->stream, g, c->d.pf,
+                       jobs, n, c->d.lnx);
     MLG_CHECK_HIP(hipGetLastError());
     return 0;
   }
@@ -435,14 +495,14 @@ int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double
   split_modes(c, g, g0, g1);
   {
     KScope ks(c, K_RTRAN_BWD, per_k * split_fp64_sum(g0, 1));
-    hipLaunchKernelGGL(k_rtran_bwd, dim3(xcd_grid(nti * ntc * g0.mcnt * n)), dim3(256), 0, c->stream, g0, c->d.pf, jobs, n,
-                       c->d.lnx);
+    hipLaunchKernelGGL(k_rtran_bwd, dim3(xcd_grid(rtran_bwd_tiles<64, 64>(g0, n))), dim3(256), 0, c->stream, g0, c->d.pf,
+                       jobs, n, c->d.lnx);
     MLG_CHECK_HIP(hipGetLastError());
   }
   if (g1.mcnt > 0) {
     KScope ks(c, K_RTRAN_BWD, per_k * ((double)sum_nrchops(c) - split_fp64_sum(g0, 1)));
-    hipLaunchKernelGGL(k_rtran_bwd_split, dim3(xcd_grid(nti * ntc * g1.mcnt * n)), dim3(256), 0, c->stream, g1, c->d.pf_sb,
-                       jobs, n, c->d.lnx, c->d.pf_eb);
+    hipLaunchKernelGGL(k_rtran_bwd_split, dim3(xcd_grid(rtran_bwd_tiles<64, 64>(g1, n))), dim3(256), 0, c->stream, g1,
+                       c->d.pf_sb, jobs, n, c->d.lnx, c->d.pf_eb);
     MLG_CHECK_HIP(hipGetLastError());
   }
   return 0;
@@ -450,6 +510,30 @@ int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double
 
 int launch_rtran_bwd(mlg_ctx *c, const double2 *in, const double *ln_in, double2 *out) {
   return launch_rtran_bwd_n(c, 1, &in, &ln_in, &out);
+}
+
+int bwd_job_tiles(const BwdJob &j) {
+  switch (j.shape) {
+    case BWD_TILE_32x64: return rtran_bwd_tiles<32, 64>(j.g, j.nj);
+    case BWD_TILE_64x32: return rtran_bwd_tiles<64, 32>(j.g, j.nj);
+    default: return rtran_bwd_tiles<64, 64>(j.g, j.nj);
+  }
+}
+
+// three resident blocks per CU (3 waves per SIMD at 256 threads), rounded
+// to the 8 XCDs; no more blocks than tiles
+int bwd_grouped_grid(int total, int cus) {
+  const int cap = (3 * cus + 7) & ~7;
+  return std::max(8, std::min(xcd_grid(total), cap));
+}
+
+void *bwd_grouped_kernel() { return (void *)k_rtran_bwd_grouped; }
+
+int launch_rtran_bwd_grouped(hipStream_t st, const BwdJob *jobs, const int *b0, int njobs, int total, int grid) {
+  if (njobs < 1 || grid < 8 || (grid & 7)) return MLG_E_ARG;
+  hipLaunchKernelGGL(k_rtran_bwd_grouped, dim3(grid), dim3(256), 0, st, jobs, b0, njobs, total);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
 }
 
 // -------------------------------------------------------------------------

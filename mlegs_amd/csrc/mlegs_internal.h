@@ -101,6 +101,10 @@ struct KTimer {
 };
 }  // namespace mlg
 
+namespace mlg {
+struct SweepRec;
+}
+
 struct mlg_ctx {
   mlg::Grid g;
   mlg::KTimer timer;
@@ -118,6 +122,9 @@ struct mlg_ctx {
   std::vector<std::pair<int, double *>> m0vecs;
   // the adjoint's VEL2PLN vectors, one buffer per truncation (built on first use)
   std::vector<std::pair<int, double *>> adjv;
+  // set while a sweep records this context's step (RTRAN backward products
+  // are then deferred to the sweep's grouped launches)
+  mlg::SweepRec *rec = nullptr;
 };
 
 struct mlg_solver {
@@ -200,6 +207,30 @@ struct TransformJobs {
   int nzl = 0;
 };
 int launch_rtran_bwd_n(mlg_ctx *c, int n, const double2 *const *in, const double *const *ln_in, double2 *const *out);
+// Grouped backward products (k_legendre.hip): any number of jobs, each one
+// run's product over a range of its modes with its own tile shape (radial
+// nodes x real columns), in one persistent launch.  Every shape gives the
+// same bits.
+enum BwdTile { BWD_TILE_64x64 = 0, BWD_TILE_32x64, BWD_TILE_64x32, BWD_TILE_COUNT };
+struct BwdJob {
+  KGrid g;  // the run's grid; the job covers its modes g.mlo .. g.mlo + g.mcnt - 1
+  const double *pf, *lnx;
+  TransformJobs jobs;
+  int nj, shape;  // fields of the product, tile shape (BwdTile)
+};
+int bwd_job_tiles(const BwdJob &j);
+// blocks of a grouped launch of `total` tiles on a device with `cus` CUs
+int bwd_grouped_grid(int total, int cus);
+void *bwd_grouped_kernel();  // the grouped kernel (graph kernel nodes)
+// jobs[0..njobs-1] and b0[0..njobs] (first tile of each job; b0[njobs] =
+// total) in device memory
+int launch_rtran_bwd_grouped(hipStream_t st, const BwdJob *jobs, const int *b0, int njobs, int total, int grid);
+// A step recorded for a sweep (sweep.hip): ends the capture segment at this
+// backward product, keeps the product for the sweep's grouped launch and
+// starts the next segment.
+int sweep_defer_bwd(mlg_ctx *c, const KGrid &g, const TransformJobs &jobs, int nj);
+// one AB2-CN / ETD2 step of the solver (api.hip), launches only
+int step_once(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b);
 // dz, dm: bit j set = job j transforms d/dz (i k times) or d/dtheta (i m
 // times) of its input (four-step lengths, NX <= 256, only); zl[0..nzl-1]:
 // LN coefficients the launch sets to zero (nzl <= MAX_TRANSFORM_JOBS)

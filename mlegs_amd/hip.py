@@ -155,6 +155,13 @@ SIGNATURES = {
     "mlg_ke_gradient": ([_V, _V, _I, _V, _V], _I),
     "mlg_solver_history": ([_V, C.POINTER(_V), C.POINTER(_V), C.POINTER(_V)], _I),
     "mlg_adjoint_op": ([_I, C.POINTER(_V), _I, _D], _I),
+    "mlg_sweep_create": ([_I, C.POINTER(_V), C.POINTER(_V), C.POINTER(_V), C.POINTER(_V), _I, C.POINTER(_V)], _I),
+    "mlg_sweep_steps": ([_V, _I], _I),
+    "mlg_sweep_info": ([_V, C.POINTER(_I)], _I),
+    "mlg_sweep_codes": ([_V, C.POINTER(_I)], _I),
+    "mlg_sweep_work": ([_V, _PD], _I),
+    "mlg_sweep_destroy": ([_V], _I),
+    "mlg_tune_rtran_bwd": ([_V, _I, C.POINTER(_I), _PD], _I),
     "mlg_timer_enable": ([_V, _I], _I),
     "mlg_timer_read": ([_V, C.c_char_p, _PD, C.POINTER(C.c_long), _PD, C.POINTER(_I)], _I),
 }
@@ -633,3 +640,89 @@ class Solver:
     @property
     def nstep(self):
         return _lib().mlg_solver_nstep(self.h)
+
+
+# ---------------------------------------------------------------------------
+SWEEP_GROUPED = 1
+SWEEP_TUNE = 2
+SWEEP_STREAMS = 4
+SWEEP_BY_GRID = 8
+BWD_TILE_SHAPES = ("64x64", "32x64", "64x32")   # radial nodes x real columns
+
+
+class Sweep:
+    """Independent runs of different resolution stepped together: one graph
+    launch per step for all runs (mlg_sweep_*).  ``runs`` is a sequence of
+    (solver, psi, chi, b), each solver on its own Grid and already started.
+    grouped: the RTRAN backward products of all runs at each place of the
+    step run as one grouped persistent launch; tune: tile shapes from the
+    tuner; shape: force one tile shape (index into BWD_TILE_SHAPES);
+    streams: each run's segments launched on the run's own stream instead of
+    one graph launch per step; by_grid: one grouped launch per class of runs
+    of one grid."""
+
+    def __init__(self, runs, grouped=True, tune=True, shape=None, streams=False, by_grid=False):
+        runs = list(runs)
+        n = len(runs)
+        self.runs = runs
+        arr = [(_V * n)(*[r[i].h for r in runs]) for i in range(4)]
+        flags = (SWEEP_GROUPED if grouped else 0) | (SWEEP_TUNE if tune and shape is None else 0)
+        if shape is not None:
+            flags |= (int(shape) + 1) << 4
+        if streams:
+            flags |= SWEEP_STREAMS
+        if by_grid:
+            flags |= SWEEP_BY_GRID
+        h = _V()
+        _call("mlg_sweep_create", n, arr[0], arr[1], arr[2], arr[3], flags, C.byref(h))
+        self.h = h
+
+    def steps(self, nsteps, check=True):
+        """Advance every run by nsteps.  A run's DIAGNOST failure raises
+        MlegsError (check=True) after all runs have stepped; with
+        check=False the runs' codes are returned (codes())."""
+        rc = _lib().mlg_sweep_steps(self.h, int(nsteps))
+        if rc >= 100 or (rc and check):
+            raise MlegsError(rc, "mlg_sweep_steps")
+        return self.codes()
+
+    def codes(self):
+        """Each run's first DIAGNOST code since the sweep was created (0: none)."""
+        o = (_I * len(self.runs))()
+        _call("mlg_sweep_codes", self.h, o)
+        return list(o)
+
+    def info(self):
+        o = (_I * 8)()
+        _call("mlg_sweep_info", self.h, o)
+        o = list(o)
+        gf = C.c_double(0.0)
+        _call("mlg_sweep_work", self.h, C.byref(gf))
+        return dict(grouped_positions=o[0], graph_nodes=o[1], grouped_jobs=o[2],
+                    jobs_per_shape=dict(zip(BWD_TILE_SHAPES, o[3:6])), grouped_launches=o[6], runs=o[7],
+                    grouped_gflop_per_step=gf.value)
+
+    def close(self):
+        if getattr(self, "h", None):
+            _lib().mlg_sweep_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def tune_rtran_bwd(grid, nfields):
+    """Tile tuner of the backward products at the grid's current truncation:
+    {bucket: (chosen shape name, {shape name: microseconds})} per mode bucket
+    (0: NRCHOPS(m) > 128, 1: 65..128, 2: 33..64, 3: <= 32) that has modes."""
+    shape = (_I * 4)()
+    us = (_D * 16)()
+    _call("mlg_tune_rtran_bwd", grid.h, int(nfields), shape, us)
+    out = {}
+    for b in range(4):
+        if shape[b] >= 0:
+            out[b] = (BWD_TILE_SHAPES[shape[b]], {BWD_TILE_SHAPES[s]: us[4 * b + s] for s in range(3)})
+    return out
