@@ -304,6 +304,38 @@ int mlg_solver_history(mlg_solver *s, mlg_field **hpsi, mlg_field **hchi, mlg_fi
 int mlg_adjoint_op(int op, mlg_field *const *f, int nf, double alp);
 
 /* ------------------------------------------------------------------ */
+/* Vorticity removal, axial sponge and adaptive hyperviscosity
+ * (src/mod_march.f90), with TIME_STEPPING's cadence.
+ *   mlg_set_march_controls: RMV%SW, RMV%INT, VISC%ADJSW, VISC%ADJINT
+ *     (READIN0's REMOVE and NUP_ADJUST blocks).  Every RMV%INT steps
+ *     (RMV%SW != 0) mlg_time_step(s) and mlg_time_steps_graph apply REMOVE,
+ *     the Richardson restart and DIAGNOST; every ADJINT steps (ADJSW != 0)
+ *     HYP3ADJ and HYP3SET (TIME_STEPPING :264-287).  RMV%SW = 2 also turns on
+ *     the axial sponge of the ETD steps (CALC_BOUSSI_ETD_SPONGE :1436-1469;
+ *     SPNGINIT :2292-2323): set it before mlg_initialize, as INITIALIZE
+ *     makes SPNGINIT before its Richardson start.
+ *   mlg_remove: REMOVE (:2503-2551), b may be NULL.
+ *   mlg_ringcut: RINGCUT (:2403-2445) on a PFF field, ns0/ns 1-based.
+ *   mlg_sponge: SPONGE (:2380-2420) of psi, chi (and b unless NULL) over dt;
+ *     a no-op unless RMV%SW = 2.
+ *   mlg_enemon: ENEMON (:2581-2617) at the current truncation: em[NTCHOP],
+ *     ek[NXCHOP].
+ *   mlg_hyp3adj: HYP3ADJ (:3000-3182) then HYP3SET of the new NUP_R/T/X and
+ *     KAPPAP_R/T/X; b may be NULL (no KAPPAP update).
+ *   mlg_hyperv_coeffs / mlg_set_hyperv_coeffs: NUP_R, NUP_T, NUP_X,
+ *     KAPPAP_R, KAPPAP_T, KAPPAP_X (set: HYP3SET of the given values).
+ *   mlg_sponge_table: TOPHAT_X over the NDIMX z points (zeros when off). */
+int mlg_set_march_controls(mlg_solver *s, int rmv_sw, int rmv_int, int adj_sw, int adj_int);
+int mlg_remove(mlg_field *psi, mlg_field *chi, mlg_field *b);
+int mlg_ringcut(mlg_field *wk, int ns0, int ns, int divxm2);
+int mlg_sponge(mlg_field *psi, mlg_field *chi, mlg_field *b, double dt);
+int mlg_enemon(const mlg_field *psi, const mlg_field *chi, double *em, double *ek);
+int mlg_hyp3adj(mlg_ctx *ctx, const mlg_field *psi, const mlg_field *chi, const mlg_field *b);
+int mlg_hyperv_coeffs(const mlg_ctx *ctx, double out[6]);
+int mlg_set_hyperv_coeffs(mlg_ctx *ctx, const double in[6]);
+int mlg_sponge_table(const mlg_ctx *ctx, double *tophat, int n);
+
+/* ------------------------------------------------------------------ */
 /* Batched sweep of independent runs (no counterpart in the reference, which
  * runs one case per job through SOLVER_T%TIME_STEPPING,
  * src/mod_march.f90:244-300; for parameter-sweep callers).  Each run is a
@@ -329,6 +361,8 @@ int mlg_adjoint_op(int op, mlg_field *const *f, int nf, double alp);
  * Forced shapes take bits 4..6 of flags.
  * The sweep owns the temporaries its graph uses until destroyed.
  *
+ * The runs step without TIME_STEPPING's REMOVE / HYP3ADJ events: a solver
+ * with an active cadence (mlg_set_march_controls) is rejected.
  * A run whose per-step DIAGNOST check fails keeps stepping with the others
  * (runs are independent jobs): mlg_sweep_steps returns the code of the
  * lowest-numbered run that has one (after all steps), mlg_sweep_codes each

@@ -195,6 +195,11 @@ extern "C" int mlg_init(const mlg_params *p, mlg_ctx **out) {
     return rc;
   }
   Grid &G = c->g;
+  // HYP3INIT (mod_march.f90:3463-3480): every direction at NUP and KAPPAP
+  for (int i = 0; i < 3; i++) {
+    c->hv_coef[i] = p->nup;
+    c->hv_coef[3 + i] = p->kappap;
+  }
   (void)hipGetDevice(&c->device);
   // the allocation-free entry points' fixed scratch: its address on this device
   rc = vel2pln_scratch_setup(c->device);
@@ -300,6 +305,7 @@ extern "C" int mlg_finalize(mlg_ctx *c) {
   c->m0vecs.clear();
   for (auto &v : c->adjv) (void)hipFree(v.second);
   c->adjv.clear();
+  if (c->d_sponge) (void)hipFree(c->d_sponge);
   for (void *p : ptrs)
     if (p) (void)hipFree(p);
   if (c->own_stream) (void)hipStreamDestroy(c->own_stream);
@@ -1477,6 +1483,20 @@ static int pc2vel3_hist(mlg_solver *s, const mlg_field *psi, const mlg_field *ch
   return rc;
 }
 
+// With the axial sponge on (RMV%SW = 2) the ETD update leaves u and b
+// unfiltered: CALC_BOUSSI_ETD_SPONGE(UZN, B) then CALC_BOUSSI_ETD_HYPERV(
+// RURN, RUPN, UZN, B), in the reference's order (mod_march.f90:708-709,
+// 865-866)
+static int sponge_then_filter(mlg_ctx *c, mlg_field *rurn, mlg_field *rupn, mlg_field *uzn, mlg_field *b, double dt) {
+  RET(etd_sponge(c, uzn, b, dt));
+  if (!filter_active(c)) return 0;
+  const double tf = dt / c->g.p.dt;
+  RET(launch_hyp3(c, rurn->e, 0, tf));
+  RET(launch_hyp3(c, rupn->e, 0, tf));
+  RET(launch_hyp3(c, uzn->e, 0, tf));
+  return launch_hyp3(c, b->e, 1, tf);
+}
+
 // STEP_BOUSSI_ETDFE_BE (mod_march.f90:664-753)
 static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, const double *e16, const double *nl16,
                          double dt, bool save) {
@@ -1499,9 +1519,11 @@ static int step_etdfe_be(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   // saving: the N history here; the pi history R_PI = u is kept in hk (the
   // velocities' FFF elements, which PROJECT transforms) and combined with the
   // second PC2VEL below
+  const bool sponge = c->rmv_sw == 2;
   RET(launch_etd_sub(c, e16, nl16, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
                      s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 0, save ? 1 : 0, nullptr,
-                     nullptr, filter_active(c), dt / c->g.p.dt));
+                     nullptr, filter_active(c), dt / c->g.p.dt, nullptr, nullptr, sponge ? 0 : 1));
+  if (sponge) RET(sponge_then_filter(c, rurn, rupn, uzn, b, dt));
   RET(project_impl(rurn, rupn, uzn, psi, w, 2, 0.0, hk));  // PROJECT(..., PSI, W, PSI%LN)
   RET(mlg_idel2ln(w, chi));
   if (save) {
@@ -1576,9 +1598,11 @@ static int step_etdab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_fiel
   // ETD_HYPERV(N) ; ETDAB_SUB ; N history ; ETD_HYPERV(u, b) ; pi history:
   // R_PI = u is kept in hk (the velocities' FFF elements, which PROJECT
   // transforms) and combined with the second PC2VEL below
+  const bool sponge = c->rmv_sw == 2;
   RET(launch_etd_sub(c, s->etd_e, s->etd_nl, dt, rurn->e, rupn->e, uzn->e, b->e, rorn->e, ropn->e, ozn->e, bn->e, s->r_pi_r->e,
                      s->r_pi_p->e, s->pi_z->e, s->r_n_r->e, s->r_n_p->e, s->n_z->e, s->n_b->e, 1, 1, nullptr, bj->e,
-                     filter_active(c), dt / c->g.p.dt, lnd, lns));
+                     filter_active(c), dt / c->g.p.dt, lnd, lns, sponge ? 0 : 1));
+  if (sponge) RET(sponge_then_filter(c, rurn, rupn, uzn, bj, dt));
   RET(project_impl(rurn, rupn, uzn, pj, w, 2, 0.0, hk));  // PROJECT(..., PSI, W, PSI%LN)
   RET(mlg_idel2ln(w, cj));
   RET(pc2vel3_hist(s, pj, cj, 4, 0.0, hk));
@@ -1627,15 +1651,49 @@ extern "C" int mlg_diagnost(const mlg_field *psi, const mlg_field *chi) {
 
 int mlg::step_once(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) { return one_step(s, psi, chi, b); }
 
+// TIME_STEPPING's events after a step (src/mod_march.f90:264-287): REMOVE
+// and the Richardson restart every RMV%INT steps, then HYP3ADJ and HYP3SET
+// every ADJINT steps (mlg_set_march_controls).  The steps' DIAGNOST codes
+// are read first: the reference stops there.
+static bool rmv_due(const mlg_solver *s) { return s->rmv_sw != 0 && s->rmv_int > 0 && s->n % s->rmv_int == 0; }
+static bool adj_due(const mlg_solver *s) { return s->adj_sw != 0 && s->adj_int > 0 && s->n % s->adj_int == 0; }
+static int march_events(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
+  mlg_ctx *c = s->ctx;
+  const bool rmv = rmv_due(s), adj = adj_due(s);
+  if (!rmv && !adj) return 0;
+  RET(mlg_synchronize(c));
+  if (rmv) {
+    RET(mlg_remove(psi, chi, b));
+    RET(c->g.p.adams ? rich_fe_be(s, psi, chi, b) : rich_etdfe_be(s, psi, chi, b));
+    RET(launch_diagnost(c, psi->e));
+    RET(mlg_synchronize(c));
+  }
+  if (adj) RET(mlg_hyp3adj(c, psi, chi, b));
+  return 0;
+}
+// steps before the next event, at most `left`
+static int steps_to_event(const mlg_solver *s, int left) {
+  int k = left;
+  if (s->rmv_sw != 0 && s->rmv_int > 0) k = std::min(k, s->rmv_int - s->n % s->rmv_int);
+  if (s->adj_sw != 0 && s->adj_int > 0) k = std::min(k, s->adj_int - s->n % s->adj_int);
+  return k;
+}
+
 extern "C" int mlg_time_step(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b) {
   if (!s || !psi || !chi || !b) return MLG_E_ARG;
   RET(one_step(s, psi, chi, b));
+  RET(march_events(s, psi, chi, b));
   return mlg_synchronize(s->ctx);
 }
 
 extern "C" int mlg_time_steps(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, int nsteps) {
   if (!s || !psi || !chi || !b || nsteps < 0) return MLG_E_ARG;
-  for (int i = 0; i < nsteps; i++) RET(one_step(s, psi, chi, b));
+  for (int done = 0; done < nsteps;) {
+    const int k = steps_to_event(s, nsteps - done);
+    for (int i = 0; i < k; i++) RET(one_step(s, psi, chi, b));
+    done += k;
+    RET(march_events(s, psi, chi, b));
+  }
   return mlg_synchronize(s->ctx);
 }
 
@@ -1672,8 +1730,21 @@ static void field_addrs(const mlg_field *psi, const mlg_field *chi, const mlg_fi
 // nothing.  After capture the graph owns the pooled temporaries (they leave
 // the pool, so no later call can take one over as its own storage) and pins
 // the factor sets in the cache.
+static int steps_graph(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, int nsteps);
 extern "C" int mlg_time_steps_graph(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, int nsteps) {
   if (!s || !psi || !chi || !b || nsteps < 0) return MLG_E_ARG;
+  for (int done = 0; done < nsteps;) {
+    const int k = steps_to_event(s, nsteps - done);
+    RET(steps_graph(s, psi, chi, b, k));
+    done += k;
+    RET(march_events(s, psi, chi, b));
+  }
+  return 0;
+}
+
+// the captured steps between two events: an eager step and the capture
+// when the graph is missing or stale, then replays
+static int steps_graph(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b, int nsteps) {
   if (nsteps == 0) return 0;
   mlg_ctx *c = s->ctx;
   int done = 0;

@@ -2050,6 +2050,7 @@ struct EtdArgs {
   const double *hv, *hd;             // filter tables [hr | ht | hx] (or null: no filter)
   int nrc;
   double tf;
+  int post;                          // the filter also acts on the updated u, b
 };
 
 // HYPERV3 factor h^tf of one coefficient (0 outside the truncation); tf is
@@ -2072,7 +2073,7 @@ __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, E
   GRID_STRIDE(fff_count(g)) {
     FIdx f = fdecode(g, idx);
     const int nn = s_nrchops(g.nrchop, f.mm * g.minc);
-    if (A.hv && !A.b0 && !(f.n < nn && f.mm < g.ntchop)) {
+    if (A.hv && A.post && !A.b0 && !(f.n < nn && f.mm < g.ntchop)) {
       // outside the truncation the filter's factor is zero: every value
       // written below is the filtered one (zero), so nothing is read
       A.rur[idx] = zero2();
@@ -2146,7 +2147,7 @@ __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, E
       }
       cv[i] = add2(s1, s2);
     }
-    if (A.hv) {
+    if (A.hv && A.post) {
 #pragma unroll
       for (int q = 0; q < 3; q++) cv[q] = filt_apply(g, f, nn, fv, cv[q]);
       cv[3] = filt_apply(g, f, nn, fd, cv[3]);
@@ -2171,7 +2172,8 @@ __global__ void __launch_bounds__(256) k_etd_sub(KGrid g, Mat4x2 M, double dt, E
 int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt, double2 *rur, double2 *rup, double2 *uz,
                    double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_, double2 *hpr,
                    double2 *hpp, double2 *hpz, double2 *hnr, double2 *hnp, double2 *hnz, double2 *hnb, int ab, int save,
-                   double2 *b0, double2 *bout, int filter, double tf, double *const *lnd, const double *const *lns) {
+                   double2 *b0, double2 *bout, int filter, double tf, double *const *lnd, const double *const *lns,
+                   int post_filter) {
   KGrid g = kgrid(c);
   Mat4x2 M;
   for (int q = 0; q < 16; q++) {
@@ -2181,7 +2183,7 @@ int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt,
   EtdArgs A{rur, rup, uz, b, nr_, np_, nz_, nb_, hpr, hpp, hpz, hnr, hnp, hnz, hnb, b0, bout ? bout : b,
             {lnd ? lnd[0] : nullptr, lnd ? lnd[1] : nullptr, lnd ? lnd[2] : nullptr},
             {lns ? lns[0] : nullptr, lns ? lns[1] : nullptr, lns ? lns[2] : nullptr},
-            nullptr, nullptr, 0, tf};
+            nullptr, nullptr, 0, tf, post_filter};
   if (filter) {
     const mlg::Grid &G = c->g;
     size_t nr1 = (size_t)G.p.nrchop * G.ntchopdim, nt = G.ntchopdim, nx = G.nxchopdim;
@@ -2193,7 +2195,7 @@ int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt,
   // (save bits); b0 out.  With the filter (and no b0) only the coefficients
   // inside the truncation are read.
   const double nin = 8.0 + (ab ? 7.0 : 0.0), nout = 4.0 + ((save & 1) ? 4.0 : 0.0) + ((save & 2) ? 3.0 : 0.0) + (b0 ? 1.0 : 0.0);
-  const double cin = (A.hv && !b0) ? (double)sum_nrchops(c) * g.nxchopdim : (double)fff_count(g);
+  const double cin = (A.hv && A.post && !b0) ? (double)sum_nrchops(c) * g.nxchopdim : (double)fff_count(g);
   KScope ks(c, K_ETD, 16.0 * (cin * nin + (double)fff_count(g) * nout));
   hipLaunchKernelGGL(k_etd_sub, dim3(nblocks(fff_count(g))), dim3(256), 0, c->stream, g, M, dt, A, ab, save);
   MLG_CHECK_HIP(hipGetLastError());

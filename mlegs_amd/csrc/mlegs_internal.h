@@ -27,6 +27,9 @@ struct Grid {
 };
 
 int build_grid(const mlg_params &p, Grid &g);
+// HYP3SET (src/mod_march.f90:3320-3380) with per-direction coefficients
+void hyp3set(const Grid &g, double nu_r, double nu_t, double nu_x, std::vector<double> &hr, std::vector<double> &ht,
+             std::vector<double> &hx);
 void twiddles(int n, int sign, double *re, double *im);
 void band_xm(int ni, int m, const double *ln, double *b);
 void band_xxdx(int ni, int m, const double *ln, double *b);
@@ -125,6 +128,17 @@ struct mlg_ctx {
   // set while a sweep records this context's step (RTRAN backward products
   // are then deferred to the sweep's grouped launches)
   mlg::SweepRec *rec = nullptr;
+  // axial sponge (RMV%SW = 2; remove.hip): TOPHAT_X over the z points, tau,
+  // and the device damping per z plane for dt, dt/2 and one other step
+  int rmv_sw = 0;
+  std::vector<double> tophat;
+  double sp_tau = 0.0;
+  double *d_sponge = nullptr;
+  // HYP3ADJ: NUP_R, NUP_T, NUP_X, KAPPAP_R, KAPPAP_T, KAPPAP_X and the
+  // controllers' baselines and histories (CURVMO, CURVKO, SLOPMO, SLOPKO)
+  double hv_coef[6] = {0, 0, 0, 0, 0, 0};
+  double nupb = -1.0, kappapb = -1.0;
+  double hist[4] = {0, 0, 0, 0}, phist[4] = {0, 0, 0, 0};
 };
 
 struct mlg_solver {
@@ -149,6 +163,8 @@ struct mlg_solver {
   int gnrchop = -1;
   int gsplit = -1;
   hipStream_t gstream = nullptr;
+  // TIME_STEPPING's REMOVE and HYP3ADJ cadence (mlg_set_march_controls)
+  int rmv_sw = 0, rmv_int = 0, adj_sw = 0, adj_int = 0;
 };
 
 // Kernel-side view of the current grid (passed by value to kernels).
@@ -231,6 +247,12 @@ int launch_rtran_bwd_grouped(hipStream_t st, const BwdJob *jobs, const int *b0, 
 int sweep_defer_bwd(mlg_ctx *c, const KGrid &g, const TransformJobs &jobs, int nj);
 // one AB2-CN / ETD2 step of the solver (api.hip), launches only
 int step_once(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *b);
+// ---- vorticity removal, sponge, adaptive hyperviscosity (remove.hip)
+// CALC_BOUSSI_ETD_SPONGE on the ETD update's u_z and b (no-op unless the
+// sponge is on)
+int etd_sponge(mlg_ctx *c, mlg_field *uz, mlg_field *b, double dt);
+// the filter tables of the context's current coefficients to the device
+int upload_hyperv(mlg_ctx *c);
 // dz, dm: bit j set = job j transforms d/dz (i k times) or d/dtheta (i m
 // times) of its input (four-step lengths, NX <= 256, only); zl[0..nzl-1]:
 // LN coefficients the launch sets to zero (nzl <= MAX_TRANSFORM_JOBS)
@@ -399,7 +421,8 @@ int launch_etd_sub(mlg_ctx *c, const double *e16, const double *nl16, double dt,
                    double2 *b, const double2 *nr_, const double2 *np_, const double2 *nz_, const double2 *nb_, double2 *hpr,
                    double2 *hpp, double2 *hpz, double2 *hnr, double2 *hnp, double2 *hnz, double2 *hnb, int ab, int save,
                    double2 *b0, double2 *bout, int filter, double tf, double *const *lnd = nullptr,
-                   const double *const *lns = nullptr);  // lnd, lns: 3 entries, *lnd[i] = *lns[i] (null lns[i]: 0)
+                   const double *const *lns = nullptr,  // lnd, lns: 3 entries, *lnd[i] = *lns[i] (null lns[i]: 0)
+                   int post_filter = 1);  // 0: the filter acts on the nonlinear terms only (the caller filters u, b)
 int launch_hyp3(mlg_ctx *c, double2 *a, int diff, double tf);
 int launch_diagnost(mlg_ctx *c, const double2 *psi);
 int launch_energy_terms(mlg_ctx *c, const double2 *a, const double2 *b, int l2, double *out_all);

@@ -543,6 +543,8 @@ extern "C" int mlg_sweep_create(int nruns, mlg_solver *const *solvers, mlg_field
     mlg_ctx *c = s->ctx;
     if (c->device != dev || psi[r]->ctx != c || chi[r]->ctx != c || b[r]->ctx != c) return MLG_E_ARG;
     if ((flags & MLG_SWEEP_GROUPED) && c->leg_split) return MLG_E_ARG;
+    // the sweep steps without TIME_STEPPING's REMOVE / HYP3ADJ events
+    if ((s->rmv_sw != 0 && s->rmv_int > 0) || (s->adj_sw != 0 && s->adj_int > 0)) return MLG_E_ARG;
     for (int q = 0; q < r; q++)
       if (solvers[q] == s || solvers[q]->ctx == c) return MLG_E_ARG;  // one run per context
   }

@@ -168,8 +168,9 @@ static void hyp3mask(const double *kv, int n, double ratio, double steep, double
   }
 }
 
-// hr[nn + nrc*mm], ht[mm], hx[kk]
-void hyp3set(const Grid &g, double nu, std::vector<double> &hr, std::vector<double> &ht, std::vector<double> &hx) {
+// hr[nn + nrc*mm], ht[mm], hx[kk]; nu_r, nu_t, nu_x per direction
+void hyp3set(const Grid &g, double nu_r, double nu_t, double nu_x, std::vector<double> &hr, std::vector<double> &ht,
+             std::vector<double> &hx) {
   int nrc = g.p.nrchop, ntd = g.ntchopdim, nxd = g.nxchopdim, ph = g.p.visc_p / 2;
   hr.assign((size_t)nrc * ntd, 0.0);
   ht.assign((size_t)ntd, 0.0);
@@ -189,14 +190,14 @@ void hyp3set(const Grid &g, double nu, std::vector<double> &hr, std::vector<doub
   for (int mm = 0; mm < ntd; mm++) k2t = std::fmax(k2t, (double)(g.m[mm] * g.m[mm]));
   for (int kk = 0; kk < nxd; kk++) k2x = std::fmax(k2x, g.ak[(size_t)ntd * kk] * g.ak[(size_t)ntd * kk]);
   for (int mm = 0; mm < ntd; mm++) {
-    ht[mm] = std::exp(-nu * mt[mm] * std::pow((double)(g.m[mm] * g.m[mm]) / k2t, ph));
+    ht[mm] = std::exp(-nu_t * mt[mm] * std::pow((double)(g.m[mm] * g.m[mm]) / k2t, ph));
     for (int nn = 0; nn < nrc; nn++)
       hr[nn + (size_t)nrc * mm] =
-          std::exp(-nu * mr[nn + (size_t)nrc * mm] * std::pow(1.5 * 1.5 * (double)(nn * (nn + 1)) / g.ell2 / k2r, ph));
+          std::exp(-nu_r * mr[nn + (size_t)nrc * mm] * std::pow(1.5 * 1.5 * (double)(nn * (nn + 1)) / g.ell2 / k2r, ph));
   }
   for (int kk = 0; kk < nxd; kk++) {
     double a = g.ak[(size_t)ntd * kk];
-    hx[kk] = std::exp(-nu * mx[kk] * std::pow(a * a / k2x, ph));
+    hx[kk] = std::exp(-nu_x * mx[kk] * std::pow(a * a / k2x, ph));
   }
 }
 
@@ -314,8 +315,8 @@ int build_grid(const mlg_params &p, Grid &g) {
     }
   }
   if (p.visc_sw == 3 || p.hyper_cn) {
-    hyp3set(g, p.nup, g.hv_r, g.hv_t, g.hv_x);
-    hyp3set(g, p.kappap, g.hd_r, g.hd_t, g.hd_x);
+    hyp3set(g, p.nup, p.nup, p.nup, g.hv_r, g.hv_t, g.hv_x);
+    hyp3set(g, p.kappap, p.kappap, p.kappap, g.hd_r, g.hd_t, g.hd_x);
   }
   return 0;
 }

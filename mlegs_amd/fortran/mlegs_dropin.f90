@@ -24,9 +24,9 @@
 !   MOD_DIAGNOSTICS   present for the drivers' USE statements
 !   MOD_EVP           SAVE_PERTURB stops: the eigenvalue solvers are not part
 !                     of this library
-! Not provided: vortex removal / sponge (REMOVE) and the hyperviscosity
-! adjustment (NUP_ADJUST): a run whose read.input switches them on stops
-! with code 30 when the first such step comes.
+! REMOVE (with the ETD axial sponge for RMV%SW = 2) and the hyperviscosity
+! adjustment (NUP_ADJUST: HYP3ADJ + HYP3SET) run at the cadences read.input
+! gives them, in TIME_STEPPING's order.
 
 module mpi
   implicit none
@@ -451,6 +451,19 @@ module mod_march
       import :: c_int, c_ptr
       type(c_ptr), value :: psi, chi
     end function
+    integer(c_int) function mlg_set_march_controls(s, rmv_sw, rmv_int, adj_sw, adj_int) bind(c)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: s
+      integer(c_int), value :: rmv_sw, rmv_int, adj_sw, adj_int
+    end function
+    integer(c_int) function mlg_remove(psi, chi, b) bind(c)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: psi, chi, b
+    end function
+    integer(c_int) function mlg_hyp3adj(c, psi, chi, b) bind(c)
+      import :: c_int, c_ptr
+      type(c_ptr), value :: c, psi, chi, b
+    end function
     integer(c_int) function mlg_field_m0_columns(f, nk, out, nrad) bind(c)
       import :: c_int, c_ptr
       type(c_ptr), value :: f
@@ -475,6 +488,9 @@ contains
     call msave(chi, trim(adjustl(files%savedir))//files%chi0)
     call msave(b, trim(adjustl(files%savedir))//files%b0)
     if (.not. c_associated(this%h)) call check(mlg_solver_create(mlegs_ctx(), this%h), 'SOLVER_T%INITIALIZE')
+    ! SPNGINIT for RMV%SW = 2 (:209-215); the REMOVE and HYP3ADJ cadences are
+    ! kept here, in TIME_STEPPING's order
+    call check(mlg_set_march_controls(this%h, int(rmv%sw, c_int), 0_c_int, 0_c_int, 0_c_int), 'SOLVER_T%INITIALIZE')
     call check(mlg_initialize(this%h, psi%h, chi%h, b%h), 'SOLVER_T%INITIALIZE')
     tim%n = mlg_solver_nstep(this%h)
     tim%t = mlg_solver_time(this%h)
@@ -495,15 +511,19 @@ contains
     end if
     if (rmv%sw /= 0 .and. rmv%int > 0) then
       if (mod(tim%n, rmv%int) == 0) then
-        write (*, *) 'REMOVE (vortex removal / sponge) is not provided by this library'
-        error stop 30
+        ! REMOVE, then the Richardson restart (one step) and its checks
+        call check(mlg_remove(psi%h, chi%h, b%h), 'REMOVE')
+        call check(mlg_initialize(this%h, psi%h, chi%h, b%h), 'SOLVER_T%TIME_STEPPING')
+        tim%n = mlg_solver_nstep(this%h)
+        tim%t = mlg_solver_time(this%h)
+        call diagnost(psi, chi)
+        if (diagv%it1 > 0) then
+          if (mod(tim%n, diagv%it1) == 0) call calc_boussi_energy(psi, chi, b, tim%t, files%savedir)
+        end if
       end if
     end if
     if (visc%adjsw /= 0 .and. visc%adjint > 0) then
-      if (mod(tim%n, visc%adjint) == 0) then
-        write (*, *) 'NUP_ADJUST (hyperviscosity adjustment) is not provided by this library'
-        error stop 30
-      end if
+      if (mod(tim%n, visc%adjint) == 0) call check(mlg_hyp3adj(mlegs_ctx(), psi%h, chi%h, b%h), 'HYP3ADJ')
     end if
     if (this%save_variable .and. files%ne > 0 .and. files%n <= files%ne) then
       if (files%t(files%n) <= tim%t) then

@@ -155,6 +155,15 @@ SIGNATURES = {
     "mlg_ke_gradient": ([_V, _V, _I, _V, _V], _I),
     "mlg_solver_history": ([_V, C.POINTER(_V), C.POINTER(_V), C.POINTER(_V)], _I),
     "mlg_adjoint_op": ([_I, C.POINTER(_V), _I, _D], _I),
+    "mlg_set_march_controls": ([_V, _I, _I, _I, _I], _I),
+    "mlg_remove": ([_V, _V, _V], _I),
+    "mlg_ringcut": ([_V, _I, _I, _I], _I),
+    "mlg_sponge": ([_V, _V, _V, _D], _I),
+    "mlg_enemon": ([_V, _V, _PD, _PD], _I),
+    "mlg_hyp3adj": ([_V, _V, _V, _V], _I),
+    "mlg_hyperv_coeffs": ([_V, _PD], _I),
+    "mlg_set_hyperv_coeffs": ([_V, _PD], _I),
+    "mlg_sponge_table": ([_V, _PD, _I], _I),
     "mlg_sweep_create": ([_I, C.POINTER(_V), C.POINTER(_V), C.POINTER(_V), C.POINTER(_V), _I, C.POINTER(_V)], _I),
     "mlg_sweep_steps": ([_V, _I], _I),
     "mlg_sweep_info": ([_V, C.POINTER(_I)], _I),
@@ -310,6 +319,23 @@ class Grid:
     def set_split_fp64_modes(self, nmodes):
         """Split precision: the modes m < nmodes (>= 2; default 5) keep FP64 products."""
         _call("mlg_set_split_fp64_modes", self.h, int(nmodes))
+
+    @property
+    def hyperv_coeffs(self):
+        """(NUP_R, NUP_T, NUP_X, KAPPAP_R, KAPPAP_T, KAPPAP_X) of the filters."""
+        o = (_D * 6)()
+        _call("mlg_hyperv_coeffs", self.h, o)
+        return np.array(o)
+
+    @hyperv_coeffs.setter
+    def hyperv_coeffs(self, c):
+        _call("mlg_set_hyperv_coeffs", self.h, (_D * 6)(*[float(v) for v in c]))
+
+    def sponge_table(self):
+        """TOPHAT_X of the axial sponge over the z points (zeros when off)."""
+        o = (_D * self.ndimx)()
+        _call("mlg_sponge_table", self.h, o, self.ndimx)
+        return np.array(o)
 
     def timer(self, on):
         """Start (clearing earlier records) or stop the per-kernel HIP event timer."""
@@ -556,6 +582,37 @@ def calc_boussi_energy(psi, chi, b):
     return f(ke), f(pe), f(dke)
 
 
+def remove(psi, chi, b=None):
+    """REMOVE (mod_march.f90:2503-2551): far-field vorticity removal of psi,
+    chi (and b), in place."""
+    _call("mlg_remove", psi.h, chi.h, b.h if b is not None else None)
+
+
+def ringcut(wk, ns0, ns, divxm2):
+    """RINGCUT on a PFF field (1-based ns0, ns as in the reference)."""
+    _call("mlg_ringcut", wk.h, int(ns0), int(ns), 1 if divxm2 else 0)
+
+
+def sponge(psi, chi, b, dt):
+    """SPONGE (mod_march.f90:2380-2420) over dt; b may be None."""
+    _call("mlg_sponge", psi.h, chi.h, b.h if b is not None else None, float(dt))
+
+
+def enemon(psi, chi):
+    """ENEMON at the current truncation: (EM[NTCHOP], EK[NXCHOP])."""
+    cfg = psi.grid.cfg
+    em, ek = (_D * cfg["ntchop"])(), (_D * cfg["nxchop"])()
+    _call("mlg_enemon", psi.h, chi.h, em, ek)
+    return np.array(em), np.array(ek)
+
+
+def hyp3adj(grid, psi, chi, b=None):
+    """HYP3ADJ then HYP3SET: returns the new (NUP_R, NUP_T, NUP_X, KAPPAP_R,
+    KAPPAP_T, KAPPAP_X)."""
+    _call("mlg_hyp3adj", grid.h, psi.h, chi.h, b.h if b is not None else None)
+    return grid.hyperv_coeffs
+
+
 def force_adjoint(psi, chi, b, nbar, ubar):
     """ubar (3 fields) += CALC_BOUSSI_FORCE'(psi, chi, b)^T nbar (3 fields)."""
     _call("mlg_force_adjoint", psi.h, chi.h, b.h, *[f.h for f in nbar], *[f.h for f in ubar])
@@ -616,6 +673,12 @@ class Solver:
         """NSTEPS steps, one synchronisation at the end (device checks are sticky).
         graph=True replays a captured step (same results, fewer launches)."""
         _call("mlg_time_steps_graph" if graph else "mlg_time_steps", self.h, psi.h, chi.h, b.h, int(nsteps))
+
+    def set_march_controls(self, rmv_sw=0, rmv_int=0, adj_sw=0, adj_int=0):
+        """TIME_STEPPING's REMOVE (RMV%SW, RMV%INT) and HYP3ADJ (VISC%ADJSW,
+        VISC%ADJINT) cadence; rmv_sw=2 also turns on the ETD axial sponge
+        (call before initialize)."""
+        _call("mlg_set_march_controls", self.h, int(rmv_sw), int(rmv_int), int(adj_sw), int(adj_int))
 
     def history(self):
         """The solver's own AB2 history fields (N_psi, N_chi, N_b), borrowed."""

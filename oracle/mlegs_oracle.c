@@ -10,6 +10,11 @@
  * observable behaviour (chop state, log-term bookkeeping, summation
  * structure); floating-point evaluation order follows the reference
  * expressions where that is visible in the source.
+ *
+ * Parity unpinned: SPESLOP's overall scale.  Its structure is pinned (odd
+ * and linear in log E, clamps, floors), but the factor a pure power law's
+ * slope comes out with (about 4.55 at 120 modes) is the reference's own
+ * discretisation and has no check independent of the formula.
  */
 #include "mlegs_oracle.h"
 
@@ -47,6 +52,14 @@ struct og_grid {
   double *thr, *thi, *z;
   /* exponential hyperviscosity filters (HYP3SET) */
   double *hv_r, *hv_t, *hv_x, *hd_r, *hd_t, *hd_x;
+  /* HYP3ADJ's directional coefficients and controller history (VISC%,
+   * mod_march.f90:75-76) */
+  double nup_r, nup_t, nup_x, kap_r, kap_t, kap_x;
+  double nupb, curvmo, curvko, slopmo, slopko;
+  double kappapb, pcurvmo, pcurvko, pslopmo, pslopko;
+  /* axial sponge (RMV%SW = 2): TOPHAT_X over the NX z points, tau */
+  int rmv_sw;
+  double *tophat_x, sp_tau;
 };
 
 struct og_solver {
@@ -56,6 +69,8 @@ struct og_solver {
   og_field *n_psi, *n_chi, *n_b;                       /* AB2 history */
   og_field *r_n_r, *r_n_p, *n_z, *r_pi_r, *r_pi_p, *pi_z; /* ETD history */
   double etd_e[16], etd_nl[16];
+  /* TIME_STEPPING's REMOVE and HYP3ADJ cadence (RMV%SW/INT, VISC%ADJSW/ADJINT) */
+  int rmv_sw, rmv_int, adj_sw, adj_int;
 };
 
 #define FIDX(g, n, mm, kk) ((size_t)(n) + (size_t)(g)->nrchopdim * ((size_t)(mm) + (size_t)(g)->ntchopdim * (size_t)(kk)))
@@ -417,6 +432,10 @@ og_grid *og_init(const og_params *p) {
     g->thr[i] = 2 * pi / g->minc / g->nth * i;
     g->thi[i] = 2 * pi / g->minc / g->nth * (i + 0.5);
   }
+  g->nup_r = g->nup_t = g->nup_x = p->nup;
+  g->kap_r = g->kap_t = g->kap_x = p->kappap;
+  g->nupb = -1.0;
+  g->kappapb = -1.0;
   if (p->visc_sw == 3 || p->hyper_cn) hyp3_init(g);
   return g;
 }
@@ -427,6 +446,7 @@ void og_free(og_grid *g) {
   free(g->norm); free(g->lognorm); free(g->pf); free(g->at0); free(g->at1);
   free(g->z); free(g->thr); free(g->thi);
   free(g->hv_r); free(g->hv_t); free(g->hv_x); free(g->hd_r); free(g->hd_t); free(g->hd_x);
+  free(g->tophat_x);
   free(g);
 }
 
@@ -448,6 +468,14 @@ const double *og_table(const og_grid *g, const char *name, int *n) {
   if (!strcmp(name, "at0")) { *n = g->nrchopdim; return g->at0; }
   if (!strcmp(name, "at1")) { *n = g->nrchopdim; return g->at1; }
   if (!strcmp(name, "ak")) { *n = g->ntchopdim * g->nxchopdim; return g->ak; }
+  if (!strcmp(name, "z")) { *n = g->nx + 1; return g->z; }
+  if (!strcmp(name, "tophat_x")) { *n = g->tophat_x ? g->ndimx : 0; return g->tophat_x; }
+  if (!strcmp(name, "hv_r")) { *n = g->hv_r ? g->p.nrchop * g->ntchopdim : 0; return g->hv_r; }
+  if (!strcmp(name, "hv_t")) { *n = g->hv_t ? g->ntchopdim : 0; return g->hv_t; }
+  if (!strcmp(name, "hv_x")) { *n = g->hv_x ? g->nxchopdim : 0; return g->hv_x; }
+  if (!strcmp(name, "hd_r")) { *n = g->hd_r ? g->p.nrchop * g->ntchopdim : 0; return g->hd_r; }
+  if (!strcmp(name, "hd_t")) { *n = g->hd_t ? g->ntchopdim : 0; return g->hd_t; }
+  if (!strcmp(name, "hd_x")) { *n = g->hd_x ? g->nxchopdim : 0; return g->hd_x; }
   if (!strcmp(name, "hv_r") && g->hv_r) { *n = g->p.nrchop * g->ntchopdim; return g->hv_r; }
   if (!strcmp(name, "hv_t") && g->hv_t) { *n = g->ntchopdim; return g->hv_t; }
   if (!strcmp(name, "hv_x") && g->hv_x) { *n = g->nxchopdim; return g->hv_x; }
@@ -1721,8 +1749,9 @@ static void hyp3mask(const double *kv, int n, double ratio, double steep, double
   }
 }
 
-static void hyp3set(og_grid *g, double nu, double **hr, double **ht, double **hx) {
+static void hyp3set(og_grid *g, double nu_r, double nu_t, double nu_x, double **hr, double **ht, double **hx) {
   int nrc = g->p.nrchop, ntd = g->ntchopdim, nxd = g->nxchopdim, ph = g->p.visc_p / 2;
+  free(*hr); free(*ht); free(*hx);
   *hr = (double *)calloc((size_t)nrc * ntd, sizeof(double));
   *ht = (double *)calloc((size_t)ntd, sizeof(double));
   *hx = (double *)calloc((size_t)nxd, sizeof(double));
@@ -1743,17 +1772,18 @@ static void hyp3set(og_grid *g, double nu, double **hr, double **ht, double **hx
   for (int mm = 0; mm < ntd; mm++) k2t = fmax(k2t, (double)(g->m[mm] * g->m[mm]));
   for (int kk = 0; kk < nxd; kk++) k2x = fmax(k2x, AK(g, 0, kk) * AK(g, 0, kk));
   for (int mm = 0; mm < ntd; mm++) {
-    (*ht)[mm] = exp(-nu * mt[mm] * pow((double)(g->m[mm] * g->m[mm]) / k2t, ph));
+    (*ht)[mm] = exp(-nu_t * mt[mm] * pow((double)(g->m[mm] * g->m[mm]) / k2t, ph));
     for (int nn = 0; nn < nrc; nn++)
-      (*hr)[nn + (size_t)nrc * mm] = exp(-nu * mr[nn + (size_t)nrc * mm] * pow(1.5 * 1.5 * (double)(nn * (nn + 1)) / g->ell2 / k2r, ph));
+      (*hr)[nn + (size_t)nrc * mm] = exp(-nu_r * mr[nn + (size_t)nrc * mm] * pow(1.5 * 1.5 * (double)(nn * (nn + 1)) / g->ell2 / k2r, ph));
   }
-  for (int kk = 0; kk < nxd; kk++) (*hx)[kk] = exp(-nu * mx[kk] * pow(AK(g, 0, kk) * AK(g, 0, kk) / k2x, ph));
+  for (int kk = 0; kk < nxd; kk++) (*hx)[kk] = exp(-nu_x * mx[kk] * pow(AK(g, 0, kk) * AK(g, 0, kk) / k2x, ph));
   free(mr); free(mt); free(mx); free(kv);
 }
 
+/* HYP3INIT (mod_march.f90:3463-3480): every direction at NUP, KAPPAP */
 static void hyp3_init(og_grid *g) {
-  hyp3set(g, g->p.nup, &g->hv_r, &g->hv_t, &g->hv_x);
-  hyp3set(g, g->p.kappap, &g->hd_r, &g->hd_t, &g->hd_x);
+  hyp3set(g, g->nup_r, g->nup_t, g->nup_x, &g->hv_r, &g->hv_t, &g->hv_x);
+  hyp3set(g, g->kap_r, g->kap_t, g->kap_x, &g->hd_r, &g->hd_t, &g->hd_x);
 }
 
 /* multiply n < NRCHOPS(m) by (H_r H_t H_x)^tf: HYPERV3 / CALC_BOUSSI_ETD_HYPERV */
@@ -1771,6 +1801,396 @@ static void hyp3_apply(const og_grid *g, og_field *a, int diff, double tf) {
 }
 
 static int filter_active(const og_grid *g) { return g->p.visc_sw == 3 || g->p.hyper_cn; }
+
+/* ===================================================================== */
+/* Vorticity removal, axial sponge and adaptive hyperviscosity            */
+/* ===================================================================== */
+
+/* TOPHAT: mod_march.f90:3704-3712 */
+static double tophat(double x, double hw, double st) { return 0.5 * (tanh((x + hw) / st) - tanh((x - hw) / st)); }
+
+/* SPNGINIT: mod_march.f90:2292-2323 (N_TAU = 10, RATIO_L = 0.86,
+ * RATIO_S = 0.04; the reference compiles default reals as REAL(8)).
+ * rmv_sw != 2 turns the sponge off. */
+void og_set_sponge(og_grid *g, int rmv_sw) {
+  g->rmv_sw = rmv_sw;
+  free(g->tophat_x);
+  g->tophat_x = NULL;
+  if (rmv_sw != 2) return;
+  const double ratio_l = 0.86, ratio_s = 0.04;
+  const double hlfz = g->zlen / 2.0;
+  g->sp_tau = 10 * g->p.dt;
+  const double hw = g->zlen * ratio_l / 2.0, st = g->zlen * ratio_s;
+  g->tophat_x = (double *)calloc((size_t)g->ndimx, sizeof(double));
+  for (int k = 0; k < g->nx; k++) g->tophat_x[k] = tophat(g->z[k] - hlfz, hw, st);
+}
+
+/* the sponge's damping of every z plane over one application of length dt */
+static void sponge_damp(const og_grid *g, double dt, double *damp) {
+  for (int k = 0; k < g->ndimx; k++) damp[k] = exp(-(1.0 - g->tophat_x[k]) / g->sp_tau * dt);
+}
+
+/* PFP field (r, m < NTCHOPDIM in the NDIMTH slots, z): z plane k times
+ * damp[k], then m = 0 made real */
+static void sponge_planes(const og_grid *g, og_field *f, const double *damp) {
+  for (int z = 0; z < g->ndimx; z++)
+    for (int mm = 0; mm < g->ntchopdim; mm++)
+      for (int r = 0; r < g->ndimr; r++) f->e[PIDX(g, r, mm, z)] = f->e[PIDX(g, r, mm, z)] * damp[z];
+  for (int z = 0; z < g->ndimx; z++)
+    for (int r = 0; r < g->ndimr; r++) f->e[PIDX(g, r, 0, z)] = creal(f->e[PIDX(g, r, 0, z)]);
+}
+
+/* CALC_BOUSSI_ETD_SPONGE: mod_march.f90:1436-1469 */
+static int etd_sponge(og_grid *g, og_field *uz, og_field *b, double dt) {
+  if (g->rmv_sw != 2) return E_OK;
+  int rc = E_OK;
+  double *damp = (double *)calloc((size_t)g->ndimx, sizeof(double));
+  sponge_damp(g, dt, damp);
+  og_field *fs[2] = {uz, b};
+  for (int i = 0; i < 2 && !rc; i++) {
+    rc = og_rtran(g, fs[i], 1);
+    if (!rc) rc = og_verfft(g, fs[i], 1);
+  }
+  for (int i = 0; i < 2 && !rc; i++) sponge_planes(g, fs[i], damp);
+  for (int i = 0; i < 2 && !rc; i++) rc = og_toff(g, fs[i]);
+  free(damp);
+  return rc;
+}
+
+/* SPONGE: mod_march.f90:2380-2420 (b may be NULL) */
+int og_sponge(og_grid *g, og_field *psi, og_field *chi, og_field *b, double dt) {
+  if (g->rmv_sw != 2) return E_OK;
+  if (psi->space != OG_FFF) return E_TRANSFORM;
+  int rc = E_OK;
+  const double psi_ln = psi->ln, chi_ln = chi->ln;
+  psi->ln = 0.0;
+  chi->ln = 0.0;
+  double *damp = (double *)calloc((size_t)g->ndimx, sizeof(double));
+  sponge_damp(g, dt, damp);
+  og_field *fs[3] = {psi, chi, b};
+  const int nf = b ? 3 : 2;
+  for (int i = 0; i < nf && !rc; i++) {
+    rc = og_rtran(g, fs[i], 1);
+    if (!rc) rc = og_verfft(g, fs[i], 1);
+  }
+  for (int i = 0; i < nf && !rc; i++) sponge_planes(g, fs[i], damp);
+  for (int i = 0; i < nf && !rc; i++) rc = og_toff(g, fs[i]);
+  psi->ln = psi_ln;
+  chi->ln = chi_ln;
+  free(damp);
+  return rc;
+}
+
+/* SMOOTH: mod_march.f90:3562-3600 (ni >= 3); t: ni scratch values */
+static void smooth(cplx *a, int ni, cplx *t) {
+  static const double fae[3] = {0.2, 0.3, 0.5};
+  for (int i = 0; i < ni; i++) t[i] = 0;
+  t[0] = a[0];
+  for (int j = 0; j < 3; j++) t[ni - 3 + j] = t[ni - 3 + j] + a[ni - 1] * fae[j];
+  for (int i = 2; i <= ni - 1; i++) { /* the reference's 1-based index */
+    const double f = (1 + ((ni - i) / (ni - 1.0))) * 0.5;
+    t[i - 2] = t[i - 2] + a[i - 1] * (1 - f) / 2;
+    t[i - 1] = t[i - 1] + a[i - 1] * f;
+    t[i] = t[i] + a[i - 1] * (1 - f) / 2;
+  }
+  memcpy(a, t, sizeof(cplx) * (size_t)ni);
+}
+
+/* SMOOTH applied `passes` times to ni interleaved complex values (tests) */
+int og_smooth(double *v, int ni, int passes) {
+  if (ni < 3) return E_CALC;
+  cplx *a = (cplx *)calloc((size_t)ni, sizeof(cplx)), *t = (cplx *)calloc((size_t)ni, sizeof(cplx));
+  for (int i = 0; i < ni; i++) a[i] = v[2 * i] + I * v[2 * i + 1];
+  for (int p = 0; p < passes; p++) smooth(a, ni, t);
+  for (int i = 0; i < ni; i++) {
+    v[2 * i] = creal(a[i]);
+    v[2 * i + 1] = cimag(a[i]);
+  }
+  free(a);
+  free(t);
+  return E_OK;
+}
+
+/* RINGCUT: mod_march.f90:2403-2445, on a PFF field; ns0, ns are the
+ * reference's 1-based radial indices.  Rows ns0..NR are zeroed except at
+ * (m, k) = (0, 0); then rows ns..NR of every (m, k) column are weighted by
+ * w (divxm2: w / (1-x)^2), smoothed 5 times and unweighted. */
+int og_ringcut(og_grid *g, og_field *wk, int ns0, int ns, int divxm2) {
+  const int nr = g->nr, ni = nr - ns + 1;
+  if (wk->space != OG_PFF || ns < 1 || ns0 < 1 || ni < 3) return E_TRANSFORM;
+  for (int kk = 0; kk < g->nxchopdim; kk++)
+    for (int mm = 0; mm < g->ntchopdim; mm++) {
+      if (mm == 0 && (kk == 0 || g->nx <= 1)) continue;
+      for (int r = ns0 - 1; r < nr; r++) wk->e[QIDX(g, r, mm, kk)] = 0;
+    }
+  double *wf = (double *)calloc((size_t)ni, sizeof(double));
+  for (int i = 0; i < ni; i++) {
+    const int r = ns - 1 + i;
+    wf[i] = divxm2 ? g->w[r] / ((1.0 - g->x[r]) * (1.0 - g->x[r])) : g->w[r];
+  }
+#pragma omp parallel for collapse(2)
+  for (int kk = 0; kk < g->nxchopdim; kk++)
+    for (int mm = 0; mm < g->ntchopdim; mm++) {
+      cplx t[ni], *col = wk->e + QIDX(g, ns - 1, mm, kk);
+      for (int i = 0; i < ni; i++) col[i] = col[i] * wf[i];
+      for (int p = 0; p < 5; p++) smooth(col, ni, t);
+      for (int i = 0; i < ni; i++) col[i] = col[i] / wf[i];
+    }
+  free(wf);
+  return E_OK;
+}
+
+/* REMOVE: mod_march.f90:2503-2551 (b may be NULL) */
+int og_remove(og_grid *g, og_field *psi, og_field *chi, og_field *b) {
+  const int ns = g->nr * 2 / 3, ns0 = imin(ns + 4, g->nr);
+  og_field *wk = og_field_new(g, OG_FFF);
+  int rc = og_delsqh(g, psi, wk);
+  if (!rc) rc = og_rtran(g, wk, 1);
+  if (!rc) rc = og_ringcut(g, wk, ns0, ns, 0);
+  if (!rc) rc = og_rtran(g, wk, -1);
+  if (!rc) rc = og_idelsqh(g, wk, psi);
+  if (!rc) {
+    og_chopset(g, 2);
+    realloc_space(g, wk, OG_FFF);
+    rc = og_del2(g, chi, wk);
+    if (!rc) rc = og_rtran(g, wk, 1);
+    if (!rc) rc = og_ringcut(g, wk, ns0, ns, 1);
+    og_chopset(g, -2);
+    if (!rc) rc = og_rtran(g, wk, -1);
+    const double ln = chi->ln;
+    if (!rc) rc = og_idel2(g, wk, chi, 1, ln);
+  }
+  og_field_free(wk);
+  if (!rc && b) {
+    rc = og_rtran(g, b, 1);
+    if (!rc) rc = og_ringcut(g, b, ns0, ns, 1);
+    if (!rc) rc = og_rtran(g, b, -1);
+  }
+  return rc;
+}
+
+/* ENEMON: mod_march.f90:2581-2617 at the caller's truncation; em[NTCHOP],
+ * ek[NXCHOP] */
+int og_enemon(og_grid *g, const og_field *psi, const og_field *chi, double *em, double *ek) {
+  const int nt = g->ntchop, nxc = g->nxchop;
+  og_field *w1 = og_field_new(g, OG_FFF), *w2 = og_field_new(g, OG_FFF);
+  double *tmp = (double *)calloc((size_t)nt * nxc, sizeof(double));
+  og_copy(g, w1, psi);
+  w1->ln = 2 * psi->ln;
+  int rc = og_delsqh(g, w1, w2);
+  if (!rc) rc = og_rtran(g, w2, 1);
+  og_copy(g, w1, psi);
+  w1->ln = 0.0;
+  if (!rc) rc = og_rtran(g, w1, 1);
+  if (!rc) {
+    prodct_halfk(g, w2, w1, 0, tmp);
+    for (int mm = 0; mm < nt; mm++) {
+      double a = 0.0;
+      for (int kk = 0; kk < nxc; kk++) a = a + tmp[mm + (size_t)nt * kk];
+      em[mm] = a;
+    }
+    for (int kk = 0; kk < nxc; kk++) {
+      double a = 0.0;
+      for (int mm = 0; mm < nt; mm++) a = a + tmp[mm + (size_t)nt * kk];
+      ek[kk] = a;
+    }
+    realloc_space(g, w1, OG_FFF);
+    realloc_space(g, w2, OG_FFF);
+    rc = og_del2(g, chi, w1);
+  }
+  if (!rc) rc = og_rtran(g, w1, 1);
+  if (!rc) rc = og_delsqh(g, chi, w2);
+  if (!rc) rc = og_rtran(g, w2, 1);
+  if (!rc) {
+    prodct_halfk(g, w2, w1, 0, tmp);
+    for (int mm = 0; mm < nt; mm++) {
+      double a = 0.0;
+      for (int kk = 0; kk < nxc; kk++) a = a + tmp[mm + (size_t)nt * kk];
+      em[mm] = -em[mm] + a;
+    }
+    for (int kk = 0; kk < nxc; kk++) {
+      double a = 0.0;
+      for (int mm = 0; mm < nt; mm++) a = a + tmp[mm + (size_t)nt * kk];
+      ek[kk] = -ek[kk] + a;
+    }
+  }
+  og_field_free(w1);
+  og_field_free(w2);
+  free(tmp);
+  return rc;
+}
+
+/* SPECURV: mod_march.f90:2642-2689; espec[0..n-1] */
+double og_specurv(const double *espec, int n, double ratio) {
+  const double floor_ = 1.0e-22, target = -0.05;
+  int ie = n, is = ie - imax(1, (int)(ie * ratio));
+  while (ie > is && espec[ie - 1] < floor_) ie--;
+  const double rtrunc = 1.0 - (double)(ie - is) / (double)(n - is);
+  const int nn = ie - is + 1;
+  if (nn < 10 || rtrunc > 0.2) return target;
+  double curv = 0.0;
+  const double dx = 2.0 / (nn - 1);
+  for (int i = 1; i <= nn; i++) {
+    const double x = -1.0 + 2.0 * (i - 1) / (nn - 1);
+    const double fac = (i == 1 || i == nn) ? 0.5 : 1.0;
+    curv = curv + fac * log10(espec[is + i - 2]) * dx * sqrt(5.0 / 8.0) * (3.0 * (x * x) - 1.0);
+  }
+  return fmin(10.0, fmax(-10.0, curv));
+}
+
+/* SPESLOP: mod_march.f90:2729-2800; espec[0..n-1] */
+double og_speslop(const double *espec, int n, double ratio) {
+  const double floor_ = 1.0e-24, drop = 1.0e10;
+  int ie = n, is = ie - imax(1, (int)(ie * ratio));
+  const int k_start = is;
+  double peak = espec[is - 1];
+  for (int i = is; i <= ie; i++) peak = fmax(peak, espec[i - 1]);
+  const double rel = peak / drop;
+  while (ie > is && (espec[ie - 1] < floor_ || espec[ie - 1] < rel)) ie--;
+  const int nn = ie - is + 1;
+  if (nn < 3) return -3.0;
+  const double xi_min = log10((double)k_start), xi_max = log10((double)(k_start + nn - 1));
+  double slope = 0.0;
+  const double dx = 2.0 / (nn - 1);
+  for (int i = 1; i <= nn; i++) {
+    const double xi = log10((double)(k_start + i - 1));
+    const double x = -1.0 + 2.0 * (xi - xi_min) / (xi_max - xi_min);
+    const double fac = (i == 1 || i == nn) ? 0.5 : 1.0;
+    slope = slope + fac * log10(espec[is + i - 2]) * dx * sqrt(3.0 / 2.0) * x;
+  }
+  slope = slope * 2.0 / (xi_max - xi_min);
+  return fmin(5.0, fmax(-15.0, slope));
+}
+
+/* PDAJUST: mod_march.f90:2841-2867; params = [KP, KD, OFFSET, SCALE];
+ * has_thr: the over-damping threshold is given */
+void og_pdajust(double *coeff, double cur, double prev, double base, const double *params, int has_thr, double thr) {
+  double control = params[0] * (cur - prev) + params[1] * cur + params[2];
+  if (has_thr && cur < thr && prev < thr) {
+    const double severity = fmin(5.0, fabs(cur - thr));
+    control = -(0.1 + 0.05 * severity);
+  }
+  control = fmin(1.0, fmax(-0.4, params[3] * control));
+  *coeff = (1.0 + control) * *coeff;
+  if (*coeff < base) *coeff = base;
+}
+
+/* the dual controllers of one coefficient set (HYP3ADJ's velocity or
+ * density part): c[0..2] = (R, T, X) coefficients, hist[0..3] = (CURVMO,
+ * CURVKO, SLOPMO, SLOPKO), base the baseline */
+static void hyp3_controllers(double *c, double *hist, double base, double smc, double skc, double sms, double sks) {
+  static const double cp[4] = {2.0, 1.0, 0.05, 0.8}, sp[4] = {1.5, 0.5, 1.5, 1.0};
+  double tc = c[1], xc = c[2], ts = c[1], xs = c[2];
+  og_pdajust(&tc, smc, hist[0], base, cp, 1, -1.0);
+  og_pdajust(&xc, skc, hist[1], base, cp, 1, -1.0);
+  og_pdajust(&ts, sms, hist[2], base, sp, 0, 0.0);
+  og_pdajust(&xs, sks, hist[3], base, sp, 0, 0.0);
+  c[1] = fmax(tc, ts);
+  c[2] = fmax(xc, xs);
+  double rco, rcn, rso, rsn;
+  if (smc > skc) {
+    rco = (3.0 * hist[0] + hist[1]) / 4.0;
+    rcn = (3.0 * smc + skc) / 4.0;
+  } else {
+    rco = (hist[0] + 3.0 * hist[1]) / 4.0;
+    rcn = (smc + 3.0 * skc) / 4.0;
+  }
+  double rc = c[0];
+  og_pdajust(&rc, rcn, rco, base, cp, 1, -1.0);
+  if (sms > sks) {
+    rso = (3.0 * hist[2] + hist[3]) / 4.0;
+    rsn = (3.0 * sms + sks) / 4.0;
+  } else {
+    rso = (hist[2] + 3.0 * hist[3]) / 4.0;
+    rsn = (sms + 3.0 * sks) / 4.0;
+  }
+  double rs = c[0];
+  og_pdajust(&rs, rsn, rso, base, sp, 0, 0.0);
+  c[0] = fmin(fmax(rc, rs), 2.0);
+  hist[0] = smc;
+  hist[1] = skc;
+  hist[2] = sms;
+  hist[3] = sks;
+}
+
+/* HYP3ADJ (mod_march.f90:3000-3182) followed by the two HYP3SETs that
+ * TIME_STEPPING makes after it (:282-287); b may be NULL */
+int og_hyp3adj(og_grid *g, const og_field *psi, const og_field *chi, const og_field *b) {
+  const int nt = g->ntchop, nxc = g->nxchop;
+  double *em = (double *)calloc((size_t)nt, sizeof(double)), *ek = (double *)calloc((size_t)nxc, sizeof(double));
+  og_chopset(g, 2);
+  int rc = og_enemon(g, psi, chi, em, ek);
+  og_chopset(g, -2);
+  if (!rc) {
+    const double smc = og_specurv(em, nt - 2, 1.0 / 4.0), skc = og_specurv(ek, nxc, 1.0 / 4.0);
+    const double sms = og_speslop(em, nt - 2, 1.0 / 3.0), sks = og_speslop(ek, nxc, 1.0 / 3.0);
+    if (g->nupb < 0.0) {
+      g->curvmo = smc;
+      g->curvko = skc;
+      g->slopmo = sms;
+      g->slopko = sks;
+      g->nupb = 0.8;
+    }
+    double c[3] = {g->nup_r, g->nup_t, g->nup_x}, h[4] = {g->curvmo, g->curvko, g->slopmo, g->slopko};
+    hyp3_controllers(c, h, g->nupb, smc, skc, sms, sks);
+    g->nup_r = c[0]; g->nup_t = c[1]; g->nup_x = c[2];
+    g->curvmo = h[0]; g->curvko = h[1]; g->slopmo = h[2]; g->slopko = h[3];
+  }
+  if (!rc && b) {
+    /* CALC_BOUSSI_PE (mod_boussinesq.f90:112-135) at the current truncation */
+    og_field *w = og_field_new(g, OG_FFF);
+    double *pe = (double *)calloc((size_t)nt * nxc, sizeof(double));
+    double *pm = (double *)calloc((size_t)nt, sizeof(double)), *pk = (double *)calloc((size_t)nxc, sizeof(double));
+    og_copy(g, w, b);
+    w->ln = 0.0;
+    rc = og_rtran(g, w, 1);
+    if (!rc) {
+      prodct_halfk(g, w, w, 1, pe);
+      for (int i = 0; i < nt * nxc; i++) pe[i] = pe[i] / 2.0 / (g->p.bv0 * g->p.bv0);
+      for (int mm = 0; mm < nt; mm++) {
+        double a = 0.0;
+        for (int kk = 0; kk < nxc; kk++) a = a + pe[mm + (size_t)nt * kk];
+        pm[mm] = a;
+      }
+      for (int kk = 0; kk < nxc; kk++) {
+        double a = 0.0;
+        for (int mm = 0; mm < nt; mm++) a = a + pe[mm + (size_t)nt * kk];
+        pk[kk] = a;
+      }
+      const double smc = og_specurv(pm, nt - 2, 1.0 / 4.0), skc = og_specurv(pk, nxc, 1.0 / 4.0);
+      const double sms = og_speslop(pm, nt - 2, 1.0 / 3.0), sks = og_speslop(pk, nxc, 1.0 / 3.0);
+      if (g->kappapb < 0.0) {
+        g->pcurvmo = smc;
+        g->pcurvko = skc;
+        g->pslopmo = sms;
+        g->pslopko = sks;
+        g->kappapb = g->nupb;
+      }
+      double c[3] = {g->kap_r, g->kap_t, g->kap_x}, h[4] = {g->pcurvmo, g->pcurvko, g->pslopmo, g->pslopko};
+      hyp3_controllers(c, h, g->kappapb, smc, skc, sms, sks);
+      g->kap_r = c[0]; g->kap_t = c[1]; g->kap_x = c[2];
+      g->pcurvmo = h[0]; g->pcurvko = h[1]; g->pslopmo = h[2]; g->pslopko = h[3];
+    }
+    og_field_free(w);
+    free(pe); free(pm); free(pk);
+  }
+  if (!rc) hyp3_init(g);
+  free(em);
+  free(ek);
+  return rc;
+}
+
+/* the six directional coefficients (NUP_R, NUP_T, NUP_X, KAPPAP_R,
+ * KAPPAP_T, KAPPAP_X); set also rebuilds the filters (HYP3SET) */
+void og_hyperv_coeffs(const og_grid *g, double *out) {
+  out[0] = g->nup_r; out[1] = g->nup_t; out[2] = g->nup_x;
+  out[3] = g->kap_r; out[4] = g->kap_t; out[5] = g->kap_x;
+}
+void og_set_hyperv_coeffs(og_grid *g, const double *in) {
+  g->nup_r = in[0]; g->nup_t = in[1]; g->nup_x = in[2];
+  g->kap_r = in[3]; g->kap_t = in[4]; g->kap_x = in[5];
+  hyp3_init(g);
+}
 
 /* ===================================================================== */
 /* Time march: mod_march.f90                                              */
@@ -2044,6 +2464,7 @@ static int step_etdfe_be(og_solver *s, og_field *psi, og_field *chi, og_field *b
   etd_hyperv(g, rorn, ropn, ozn, bn, dt);
   pc2vel3(g, psi, chi, rurn, rupn, uzn);
   etd_sub(g, e16, nl16, dt, rurn, rupn, uzn, b, rorn, ropn, ozn, bn, NULL);
+  if ((rc = etd_sponge(g, uzn, b, dt))) goto done;
   etd_hyperv(g, rurn, rupn, uzn, b, dt);
   if (save) {
     memcpy(s->n_b->e, bn->e, sizeof(cplx) * (size_t)n);
@@ -2129,6 +2550,7 @@ static int step_etdab_cn(og_solver *s, og_field *psi, og_field *chi, og_field *b
   memcpy(s->r_n_r->e, rorn->e, sizeof(cplx) * (size_t)n);
   memcpy(s->r_n_p->e, ropn->e, sizeof(cplx) * (size_t)n);
   memcpy(s->n_z->e, ozn->e, sizeof(cplx) * (size_t)n);
+  if ((rc = etd_sponge(g, uzn, b, dt))) goto done;
   etd_hyperv(g, rurn, rupn, uzn, b, dt);
   memcpy(s->r_pi_r->e, rurn->e, sizeof(cplx) * (size_t)n);
   memcpy(s->r_pi_p->e, rupn->e, sizeof(cplx) * (size_t)n);
@@ -2191,11 +2613,33 @@ int og_solver_prepare(og_solver *s) {
   return E_OK;
 }
 
-/* SOLVER_T%TIME_STEPPING: mod_march.f90:243-307 (step + per-step checks) */
+/* TIME_STEPPING's REMOVE and HYP3ADJ cadence (RMV%SW, RMV%INT, VISC%ADJSW,
+ * VISC%ADJINT); rmv_sw = 2 also turns on the ETD steps' axial sponge
+ * (SPNGINIT, made by INITIALIZE: set before og_initialize) */
+void og_set_march_controls(og_solver *s, int rmv_sw, int rmv_int, int adj_sw, int adj_int) {
+  s->rmv_sw = rmv_sw;
+  s->rmv_int = rmv_int;
+  s->adj_sw = adj_sw;
+  s->adj_int = adj_int;
+  og_set_sponge(s->g, rmv_sw);
+}
+
+/* SOLVER_T%TIME_STEPPING: mod_march.f90:243-307 (step, per-step checks,
+ * REMOVE with the Richardson restart every RMV%INT steps, HYP3ADJ and
+ * HYP3SET every ADJINT steps; file output omitted) */
 int og_time_step(og_solver *s, og_field *psi, og_field *chi, og_field *b) {
-  int rc = s->g->p.adams ? step_ab_cn(s, psi, chi, b) : step_etdab_cn(s, psi, chi, b);
+  og_grid *g = s->g;
+  int rc = g->p.adams ? step_ab_cn(s, psi, chi, b) : step_etdab_cn(s, psi, chi, b);
   if (rc) return rc;
-  return og_diagnost(s->g, psi, chi);
+  if ((rc = og_diagnost(g, psi, chi))) return rc;
+  if (s->rmv_sw != 0 && s->rmv_int > 0 && s->n % s->rmv_int == 0) {
+    if ((rc = og_remove(g, psi, chi, b))) return rc;
+    rc = g->p.adams ? rich_fe_be(s, psi, chi, b) : rich_etdfe_be(s, psi, chi, b);
+    if (rc) return rc;
+    if ((rc = og_diagnost(g, psi, chi))) return rc;
+  }
+  if (s->adj_sw != 0 && s->adj_int > 0 && s->n % s->adj_int == 0) rc = og_hyp3adj(g, psi, chi, b);
+  return rc;
 }
 
 /* QVORTEX (single centred Gaussian, H = 0): mod_layout.f90:449-610 with

@@ -115,6 +115,21 @@ int og_diagnost(const og_grid *g, const og_field *psi, const og_field *chi);
 double og_time(const og_solver *s);
 int og_nstep(const og_solver *s);
 
+/* vorticity removal, axial sponge, adaptive hyperviscosity (mod_march.f90) */
+void og_set_sponge(og_grid *g, int rmv_sw);                      /* SPNGINIT */
+int og_sponge(og_grid *g, og_field *psi, og_field *chi, og_field *b, double dt); /* SPONGE, b may be NULL */
+int og_ringcut(og_grid *g, og_field *wk, int ns0, int ns, int divxm2); /* PFF; 1-based ns0, ns */
+int og_smooth(double *v, int ni, int passes);                        /* SMOOTH on interleaved complex (tests) */
+int og_remove(og_grid *g, og_field *psi, og_field *chi, og_field *b);   /* b may be NULL */
+int og_enemon(og_grid *g, const og_field *psi, const og_field *chi, double *em, double *ek);
+double og_specurv(const double *espec, int n, double ratio);
+double og_speslop(const double *espec, int n, double ratio);
+void og_pdajust(double *coeff, double cur, double prev, double base, const double *params, int has_thr, double thr);
+int og_hyp3adj(og_grid *g, const og_field *psi, const og_field *chi, const og_field *b); /* + HYP3SET */
+void og_hyperv_coeffs(const og_grid *g, double *out);       /* NUP_R,T,X, KAPPAP_R,T,X */
+void og_set_hyperv_coeffs(og_grid *g, const double *in);
+void og_set_march_controls(og_solver *s, int rmv_sw, int rmv_int, int adj_sw, int adj_int);
+
 /* set-up helpers */
 int og_qvortex(og_grid *g, double q, og_field *psi, og_field *chi);
 void og_set_threads(int n);

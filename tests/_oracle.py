@@ -92,6 +92,20 @@ def lib():
             "og_time": ([C.c_void_p], C.c_double),
             "og_nstep": ([C.c_void_p], C.c_int),
             "og_qvortex": ([g, C.c_double, P, P], C.c_int),
+            "og_set_sponge": ([g, C.c_int], None),
+            "og_sponge": ([g, P, P, P, C.c_double], C.c_int),
+            "og_ringcut": ([g, P, C.c_int, C.c_int, C.c_int], C.c_int),
+            "og_smooth": ([C.POINTER(C.c_double), C.c_int, C.c_int], C.c_int),
+            "og_remove": ([g, P, P, P], C.c_int),
+            "og_enemon": ([g, P, P, C.POINTER(C.c_double), C.POINTER(C.c_double)], C.c_int),
+            "og_specurv": ([C.POINTER(C.c_double), C.c_int, C.c_double], C.c_double),
+            "og_speslop": ([C.POINTER(C.c_double), C.c_int, C.c_double], C.c_double),
+            "og_pdajust": ([C.POINTER(C.c_double), C.c_double, C.c_double, C.c_double, C.POINTER(C.c_double),
+                            C.c_int, C.c_double], None),
+            "og_hyp3adj": ([g, P, P, P], C.c_int),
+            "og_hyperv_coeffs": ([g, C.POINTER(C.c_double)], None),
+            "og_set_hyperv_coeffs": ([g, C.POINTER(C.c_double)], None),
+            "og_set_march_controls": ([C.c_void_p, C.c_int, C.c_int, C.c_int, C.c_int], None),
             "og_set_threads": ([C.c_int], None),
             "og_set_sum_order": ([C.c_int], None),
             "og_set_mutation": ([C.c_int], None),
@@ -240,6 +254,59 @@ class OGrid:
         return f(ke), f(pe), f(dke)
 
 
+    def enemon(self, psi, chi):
+        """ENEMON at the current truncation: (EM[NTCHOP], EK[NXCHOP])."""
+        nt, nxc = self.cfg["ntchop"], self.cfg["nxchop"]
+        em, ek = (C.c_double * nt)(), (C.c_double * nxc)()
+        rc = lib().og_enemon(self.g, psi.ptr, chi.ptr, em, ek)
+        if rc:
+            raise RuntimeError(f"og_enemon returned {rc}")
+        return np.array(em), np.array(ek)
+
+    def hyperv_coeffs(self):
+        o = (C.c_double * 6)()
+        lib().og_hyperv_coeffs(self.g, o)
+        return np.array(o)
+
+    def set_hyperv_coeffs(self, c):
+        lib().og_set_hyperv_coeffs(self.g, (C.c_double * 6)(*[float(v) for v in c]))
+
+    def hyperv_tables(self):
+        """(H_R [NRCHOP x NTCHOPDIM], H_T, H_X, D_R, D_T, D_X) of the filters."""
+        nrc, ntd, nxd = self.cfg["nrchop"], self.ntchopdim, self.nxchopdim
+        out = []
+        for name, n in (("hv_r", nrc * ntd), ("hv_t", ntd), ("hv_x", nxd), ("hd_r", nrc * ntd), ("hd_t", ntd),
+                        ("hd_x", nxd)):
+            out.append(self.table(name)[:n].copy())
+        return out
+
+
+def smooth(a, passes=1):
+    """SMOOTH applied `passes` times to a complex vector."""
+    v = np.ascontiguousarray(np.asarray(a, dtype=np.complex128)).view(np.float64).copy()
+    rc = lib().og_smooth(v.ctypes.data_as(C.POINTER(C.c_double)), len(v) // 2, passes)
+    if rc:
+        raise RuntimeError(f"og_smooth returned {rc}")
+    return v.view(np.complex128)
+
+
+def specurv(espec, ratio):
+    a = np.ascontiguousarray(espec, dtype=np.float64)
+    return lib().og_specurv(a.ctypes.data_as(C.POINTER(C.c_double)), len(a), float(ratio))
+
+
+def speslop(espec, ratio=1.0 / 3.0):
+    a = np.ascontiguousarray(espec, dtype=np.float64)
+    return lib().og_speslop(a.ctypes.data_as(C.POINTER(C.c_double)), len(a), float(ratio))
+
+
+def pdajust(coeff, cur, prev, base, params, thr=None):
+    c = C.c_double(coeff)
+    lib().og_pdajust(C.byref(c), cur, prev, base, (C.c_double * 4)(*params), 0 if thr is None else 1,
+                     0.0 if thr is None else float(thr))
+    return c.value
+
+
 class OSolver:
     def __init__(self, grid):
         self.grid = grid
@@ -258,6 +325,9 @@ class OSolver:
 
     def step(self, psi, chi, b):
         return lib().og_time_step(self.s, psi.ptr, chi.ptr, b.ptr)
+
+    def set_march_controls(self, rmv_sw=0, rmv_int=0, adj_sw=0, adj_int=0):
+        lib().og_set_march_controls(self.s, rmv_sw, rmv_int, adj_sw, adj_int)
 
     @property
     def t(self):

@@ -366,13 +366,16 @@ def test_energy(P):
 
 
 # --------------------------------------------------------------- time march
-def oracle_trajectory(cfg, st, nsteps, reverse=False):
-    """Oracle states (psi, chi, b as (array, LN)) after the start and each step."""
+def oracle_trajectory(cfg, st, nsteps, reverse=False, controls=None):
+    """Oracle states (psi, chi, b as (array, LN)) after the start and each step
+    (controls: TIME_STEPPING's (RMV%SW, RMV%INT, ADJSW, ADJINT))."""
     O.lib().og_set_sum_order(1 if reverse else 0)
     o = O.OGrid(cfg)
     try:
         fs = [o.field(O.FFF, a, ln) for a, ln in st]
         sv = O.OSolver(o)
+        if controls:
+            sv.set_march_controls(*controls)
         sv.initialize(*fs)
         out = [[(f.get(), f.ln) for f in fs]]
         for _ in range(nsteps):
@@ -385,7 +388,7 @@ def oracle_trajectory(cfg, st, nsteps, reverse=False):
         o.close()
 
 
-def run_steps(cfg, nsteps, amp=1e-4):
+def run_steps(cfg, nsteps, amp=1e-4, controls=None, graph=False):
     """GPU vs oracle over a start and `nsteps` steps.
 
     Bar per field and step: err <= max(1e-12, 6 s), s being the largest
@@ -400,15 +403,20 @@ def run_steps(cfg, nsteps, amp=1e-4):
     p = Pair(cfg)
     try:
         st = state(p, amp)
-        ref = oracle_trajectory(cfg, st, nsteps)
-        rev = oracle_trajectory(cfg, st, nsteps, reverse=True)
+        ref = oracle_trajectory(cfg, st, nsteps, controls=controls)
+        rev = oracle_trajectory(cfg, st, nsteps, reverse=True, controls=controls)
         gf = [p.g.scalar(O.FFF, a, ln) for a, ln in st]
         gs = H.Solver(p.g)
+        if controls:
+            gs.set_march_controls(*controls)
         gs.initialize(*gf)
         report, lines = [], []
         for step in range(nsteps + 1):
             if step:
-                gs.time_step(*gf)
+                if graph:
+                    gs.time_stepping(*gf, 1, graph=True)
+                else:
+                    gs.time_step(*gf)
             row = []
             for nm, g, (ra, rl), (va, vl) in zip(("psi", "chi", "b"), gf, ref[step], rev[step]):
                 ga, _, gl = g.download()
@@ -437,7 +445,8 @@ def run_steps(cfg, nsteps, amp=1e-4):
                 assert ex <= max(TOL, SENS_FACTOR * sens_ex), f"step {step} {nm}: err excl. gauge {ex:.3e}\n" + "\n".join(lines)
                 assert e <= max(TOL, SENS_FACTOR * sens), f"step {step} {nm}: err {e:.3e} > bar\n" + "\n".join(lines)
                 assert eln <= max(TOL, SENS_FACTOR * lsens), f"step {step} {nm}: LN\n" + "\n".join(lines)
-        assert gs.nstep == nsteps + 1
+        if not controls:
+            assert gs.nstep == nsteps + 1
         return report
     finally:
         p.close()
