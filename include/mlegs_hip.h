@@ -304,6 +304,17 @@ int mlg_solver_history(mlg_solver *s, mlg_field **hpsi, mlg_field **hchi, mlg_fi
 int mlg_adjoint_op(int op, mlg_field *const *f, int nf, double alp);
 
 /* ------------------------------------------------------------------ */
+/* DIAGNOST's every-IT1 section (src/mod_march.f90:1797-1961) with CALCXY
+ * (:1967-2047), at NRCHOP+3 as the reference forms it.  out[0] VALVOR
+ * (INTEG of omega_z/(1-x)^2), [1] VALZMO (of u_z/(1-x)^2), [2] VALENE,
+ * [3] VALHEL, [4] VALANG (-INTEG_P(XXDX(psi))), [5] |PSIAT1| at k = 0 and
+ * [6] max - min of |PSIAT1| over k (CALCAT1), [7], [8] the same for chi,
+ * [9] <X>, [10] <Y>, [11] <R> (the enstrophy-weighted centre and radius of
+ * CALCXY).  The printing, coresize.dat and the ADV history stay with the
+ * caller. */
+int mlg_diagnost_it1(const mlg_field *psi, const mlg_field *chi, double out[12]);
+
+/* ------------------------------------------------------------------ */
 /* Vorticity removal, axial sponge and adaptive hyperviscosity
  * (src/mod_march.f90), with TIME_STEPPING's cadence.
  *   mlg_set_march_controls: RMV%SW, RMV%INT, VISC%ADJSW, VISC%ADJINT

@@ -391,6 +391,7 @@ module mod_march
   type, public :: diagdata
     logical :: is_initialized = .false.
     integer :: it1 = 40, it2 = 40
+    real(p8) :: vori = 0.0_p8, zmomi = 0.0_p8, enei = 0.0_p8, heli = 0.0_p8, angi = 0.0_p8
   end type diagdata
   type, public :: removal
     integer :: sw = 0, int = 10000
@@ -451,6 +452,11 @@ module mod_march
       import :: c_int, c_ptr
       type(c_ptr), value :: psi, chi
     end function
+    integer(c_int) function mlg_diagnost_it1(psi, chi, out) bind(c)
+      import :: c_int, c_ptr, c_double
+      type(c_ptr), value :: psi, chi
+      real(c_double), intent(out) :: out(12)
+    end function
     integer(c_int) function mlg_set_march_controls(s, rmv_sw, rmv_int, adj_sw, adj_int) bind(c)
       import :: c_int, c_ptr
       type(c_ptr), value :: s
@@ -506,6 +512,7 @@ contains
     call check(mlg_time_step(this%h, psi%h, chi%h, b%h), 'SOLVER_T%TIME_STEPPING')
     tim%n = mlg_solver_nstep(this%h)
     tim%t = mlg_solver_time(this%h)
+    call diagnost(psi, chi)
     if (diagv%it1 > 0) then
       if (mod(tim%n, diagv%it1) == 0) call calc_boussi_energy(psi, chi, b, tim%t, files%savedir)
     end if
@@ -558,11 +565,61 @@ contains
     end if
   end subroutine finalize
 
-  ! DIAGNOST (src/mod_march.f90:1726-1791), the per-step checks: 11
-  ! (dealiasing), 12 (divergence) end the run as the reference's MPI_ABORT
+  ! DIAGNOST (src/mod_march.f90:1726-1961): the per-step checks (11
+  ! dealiasing, 12 divergence end the run as the reference's MPI_ABORT) and,
+  ! every DIAGV%IT1 steps, the integrals of vorticity, z-momentum, energy,
+  ! helicity and angular momentum with their change since step 0, psi and
+  ! chi at r = infinity, and CALCXY's vortex centre and radius (also
+  ! appended to SAVEDIR/coresize.dat)
   subroutine diagnost(psi, chi)
     type(scalar), intent(in) :: psi, chi
+    real(c_double) :: d(12)
+    integer :: u, st
     call check(mlg_diagnost(psi%h, chi%h), 'DIAGNOST')
+    if (diagv%it1 <= 0) return
+    if (mod(tim%n, diagv%it1) /= 0) return
+    call check(mlg_diagnost_it1(psi%h, chi%h, d), 'DIAGNOST')
+    if (tim%n == 0) then
+      diagv%is_initialized = .true.
+      diagv%vori = d(1)
+      diagv%zmomi = d(2)
+      diagv%enei = d(3)
+      diagv%heli = d(4)
+      diagv%angi = d(5)
+    end if
+    if (mpi_rank /= 0) return
+    write (*, *) '<<< DIAGNOSTICS >>>'
+    write (6, '(A,E13.4,A,I8)') 'TIM%T=', tim%t, ' TIM%N=', tim%n
+    write (6, '(A,E13.5,A,E13.5)') 'ADV%X=', 0.0_p8, '  ADV%Y=', 0.0_p8
+    if (d(7) > 1.0e-10_p8 .or. d(9) > 1.0e-10_p8) then
+      write (6, '(A,E18.9,A,E18.9)') 'PSIAT1(KK=1)=', d(6), ' MAXMIN=', d(7)
+      write (6, '(A,E18.9,A,E18.9)') 'CHIAT1(KK=1)=', d(8), ' MAXMIN=', d(9)
+    end if
+    write (6, *) 'CFL#=', 0.0_p8
+    call change_line('VORTICITY', d(1), diagv%vori, .false.)
+    call change_line('Z-MOMENTUM', d(2), diagv%zmomi, .false.)
+    call change_line('ENERGY', d(3), diagv%enei, .true.)
+    call change_line('HELICITY', d(4), diagv%heli, .false.)
+    call change_line('INTEG_ANGMOM ', d(5), diagv%angi, .false.)
+    write (6, '(A,E10.3,A,E10.3,A,E10.3,A,E10.3)') 'T= ', tim%t, ' <X>= ', d(10), ' <Y>= ', d(11), ' <R>= ', d(12)
+    open (newunit=u, file=trim(adjustl(files%savedir))//'coresize.dat', position='append', iostat=st)
+    if (st == 0) then
+      write (u, '(4E14.6)') tim%t, d(10), d(11), d(12)
+      close (u)
+    end if
+  contains
+    ! a value with its fractional change from step 0 (always for the energy)
+    subroutine change_line(name, v, v0, always)
+      character(len=*), intent(in) :: name
+      real(c_double), intent(in) :: v
+      real(p8), intent(in) :: v0
+      logical, intent(in) :: always
+      if (always .or. v0 /= 0.0_p8) then
+        write (6, '(A12,A,E23.15,A,E23.15)') name, '=', v, '  FRACTIONAL CHANGE=', -(v0 - v) / v0
+      else
+        write (6, '(A12,A,E23.15)') name, '=', v
+      end if
+    end subroutine change_line
   end subroutine diagnost
 
   ! INSPECT (src/mod_march.f90:3743-3775): appends TIM%T and the m = 0

@@ -155,6 +155,7 @@ SIGNATURES = {
     "mlg_ke_gradient": ([_V, _V, _I, _V, _V], _I),
     "mlg_solver_history": ([_V, C.POINTER(_V), C.POINTER(_V), C.POINTER(_V)], _I),
     "mlg_adjoint_op": ([_I, C.POINTER(_V), _I, _D], _I),
+    "mlg_diagnost_it1": ([_V, _V, _PD], _I),
     "mlg_set_march_controls": ([_V, _I, _I, _I, _I], _I),
     "mlg_remove": ([_V, _V, _V], _I),
     "mlg_ringcut": ([_V, _I, _I, _I], _I),
@@ -580,6 +581,19 @@ def calc_boussi_energy(psi, chi, b):
     _call("mlg_energy", psi.h, chi.h, b.h, ke, pe, dke)
     f = lambda a: np.array(a).reshape(nt, nxc, order="F")
     return f(ke), f(pe), f(dke)
+
+
+DIAGNOST_IT1_KEYS = ("valvor", "valzmo", "valene", "valhel", "valang", "psiat1", "psiat1_spread", "chiat1",
+                     "chiat1_spread", "x", "y", "r")
+
+
+def diagnost_it1(psi, chi):
+    """DIAGNOST's every-IT1 quantities (vorticity, z-momentum, energy,
+    helicity, angular momentum, psi/chi at r = infinity, the vortex centre
+    <X>, <Y> and radius <R>) as a dict."""
+    o = (_D * 12)()
+    _call("mlg_diagnost_it1", psi.h, chi.h, o)
+    return dict(zip(DIAGNOST_IT1_KEYS, list(o)))
 
 
 def remove(psi, chi, b=None):

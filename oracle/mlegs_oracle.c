@@ -14,7 +14,10 @@
  * Parity unpinned: SPESLOP's overall scale.  Its structure is pinned (odd
  * and linear in log E, clamps, floors), but the factor a pure power law's
  * slope comes out with (about 4.55 at 120 modes) is the reference's own
- * discretisation and has no check independent of the formula.
+ * discretisation and has no check independent of the formula.  DIAGNOST's
+ * VALANG (INTEG_P's storage-order read of FFF coefficients) and the
+ * CALCAT1 spreads (the pinned AT1 table summed against the m = 0 column)
+ * have no further check either.
  */
 #include "mlegs_oracle.h"
 
@@ -2571,6 +2574,171 @@ done:
   og_field_free(rorn); og_field_free(ropn); og_field_free(ozn);
   og_field_free(bn); og_field_free(w);
   og_field_free(psi0); og_field_free(chi0); og_field_free(b0);
+  return rc;
+}
+
+/* ===================================================================== */
+/* DIAGNOST's every-IT1 section: mod_march.f90:1797-1961, CALCXY :1967-2047 */
+/* ===================================================================== */
+
+/* INTEG: mod_scalar3.f90:312-353 */
+static double integ(const og_grid *g, const og_field *f) {
+  const double pi = acos(-1.0);
+  return 4.0 * pi * g->zlen * g->ell2 * creal(f->e[0]) * NORM(g, 0, 0);
+}
+
+/* PRODCT: mod_scalar3.f90:398-468 (PROD(r) over k then m, then the
+ * weighted sum over r; m = 0 as a complex product, its real part kept) */
+static double prodct(const og_grid *g, const og_field *a, const og_field *b) {
+  const double pi = acos(-1.0);
+  cplx *prod = (cplx *)calloc((size_t)g->nr, sizeof(cplx));
+  for (int kk = 0; kk < g->nxchopdim; kk++)
+    for (int mm = 0; mm < g->ntchopdim; mm++)
+      for (int r = 0; r < g->nr; r++) {
+        const cplx av = a->e[QIDX(g, r, mm, kk)], bv = b->e[QIDX(g, r, mm, kk)];
+        if (mm == 0) prod[r] = prod[r] + av * conj(bv);
+        else prod[r] = prod[r] + 2.0 * (creal(av) * creal(bv) + cimag(av) * cimag(bv));
+      }
+  cplx s = 0;
+  for (int r = 0; r < g->nr; r++) s = s + prod[r] * g->w[r];
+  free(prod);
+  return 2 * pi * g->zlen * g->ell2 * creal(s);
+}
+
+/* |CALCAT1| (mod_scalar3.f90:286-310) at the current truncation: at k = 0
+ * and max - min over k */
+static void calcat1_range(const og_grid *g, const og_field *f, double *at0, double *spread) {
+  double mx = -1.0, mn = 1e300;
+  for (int kk = 0; kk < g->nxchopdim; kk++) {
+    const double a = cabs(calcat1(g, f, kk));
+    if (kk == 0) *at0 = a;
+    mx = fmax(mx, a);
+    mn = fmin(mn, a);
+  }
+  *spread = mx - mn;
+}
+
+/* ABSVEC2: mod_march.f90:2050-2099 (a may be w) */
+static void absvec2(const og_grid *g, const og_field *u, const og_field *v, const og_field *w, og_field *a) {
+  for (int kk = 0; kk < g->ndimx; kk++)
+    for (int j = 0; j < g->nth; j++)
+      for (int i = 0; i < g->nr; i++) {
+        const size_t id = PIDX(g, i, j, kk);
+        const double r = g->r[i], f = (1 - g->x[i]) * (1 - g->x[i]);
+        const cplx uu = u->e[id], vv = v->e[id], ww = w->e[id];
+        const double aaa = ((creal(uu) * creal(uu) + creal(vv) * creal(vv)) / (r * r) + creal(ww) * creal(ww));
+        const double bbb = ((cimag(uu) * cimag(uu) + cimag(vv) * cimag(vv)) / (r * r) + cimag(ww) * cimag(ww));
+        a->e[id] = aaa / f + I * (bbb / f);
+      }
+}
+
+/* MULRCOSTH (q = 0), MULRSINTH (1), MULR2 (2): mod_march.f90:2102-2225 */
+static void mul_rtheta(const og_grid *g, og_field *a, int q) {
+  for (int kk = 0; kk < g->ndimx; kk++)
+    for (int j = 0; j < g->nth; j++)
+      for (int i = 0; i < g->nr; i++) {
+        const size_t id = PIDX(g, i, j, kk);
+        const double r = g->r[i];
+        double fr, fi;
+        if (q == 0) fr = r * cos(g->thr[j]), fi = r * cos(g->thi[j]);
+        else if (q == 1) fr = r * sin(g->thr[j]), fi = r * sin(g->thi[j]);
+        else fr = fi = r * r;
+        a->e[id] = fr * creal(a->e[id]) + I * (fi * cimag(a->e[id]));
+      }
+}
+
+/* out: VALVOR, VALZMO, VALENE, VALHEL, VALANG, |PSIAT1(k=0)|, spread,
+ * |CHIAT1(k=0)|, spread, <X>, <Y>, <R> */
+int og_diagnost_it1(og_grid *g, const og_field *psi, const og_field *chi, double *out) {
+  const double pi = acos(-1.0);
+  og_field *rur = og_field_new(g, OG_FFF), *rup = og_field_new(g, OG_FFF), *uzh = og_field_new(g, OG_FFF);
+  og_field *ror = og_field_new(g, OG_FFF), *rop = og_field_new(g, OG_FFF), *ozh = og_field_new(g, OG_FFF);
+  long n = og_field_size(g, OG_FFF);
+  og_chopset(g, 3);
+  /* PC2VEL(..., 'H'): UZ/(1-x)^2 = -DELSQH(chi) (elements only) */
+  int rc = og_delsqh(g, chi, uzh);
+  for (long i = 0; i < n && !rc; i++) uzh->e[i] = -uzh->e[i];
+  /* PC2VOR(..., 'H'): ROR, ROP and OZ/(1-x)^2 = -DELSQH(psi) */
+  if (!rc) rc = og_pc2vor(g, psi, chi, ror, rop, ozh);
+  if (!rc) rc = og_delsqh(g, psi, ozh);
+  for (long i = 0; i < n && !rc; i++) ozh->e[i] = -ozh->e[i];
+  if (!rc) {
+    out[0] = integ(g, ozh);
+    out[1] = integ(g, uzh);
+    /* VALANG = -INTEG_P(XXDX(psi, LN = 0)): INTEG_P (mod_march.f90:3649-3682)
+     * reads F%E(:NR,1,1) of its FFF argument, the first NR stored
+     * coefficients in storage order */
+    og_copy(g, rur, psi);
+    rur->ln = 0.0;
+    rc = og_xxdx(g, rur, rup);
+  }
+  if (!rc) {
+    double dot = 0.0;
+    for (int i = 0; i < g->nr; i++) dot = dot + creal(rup->e[i]) / ((1 - g->x[i]) * (1 - g->x[i])) * g->w[i];
+    out[4] = -(2.0 * pi * g->zlen * g->ell2 * dot);
+    og_copy(g, rup, psi);
+    rup->ln = 2 * rup->ln;
+    realloc_space(g, rur, OG_FFF);
+    rc = og_delsqh(g, rup, rur);
+  }
+  if (!rc) rc = og_rtran(g, rur, 1);
+  if (!rc) {
+    og_copy(g, rup, psi);
+    rup->ln = 0.0;
+    rc = og_rtran(g, rup, 1);
+  }
+  double ene1 = 0, ene2 = 0;
+  if (!rc) {
+    ene1 = prodct(g, rur, rup);
+    realloc_space(g, rup, OG_FFF);
+    rc = og_del2(g, chi, rup);
+  }
+  if (!rc) rc = og_rtran(g, rup, 1);
+  if (!rc) {
+    realloc_space(g, rur, OG_FFF);
+    rc = og_delsqh(g, chi, rur);
+  }
+  if (!rc) rc = og_rtran(g, rur, 1);
+  if (!rc) {
+    ene2 = prodct(g, rur, rup);
+    out[2] = -ene1 + ene2;
+    realloc_space(g, rur, OG_FFF);
+    rc = og_delsqh(g, psi, rur);
+  }
+  if (!rc) rc = og_rtran(g, rur, 1);
+  if (!rc) {
+    out[3] = 2 * prodct(g, rur, rup);
+    calcat1_range(g, psi, &out[5], &out[6]);
+    calcat1_range(g, chi, &out[7], &out[8]);
+    realloc_space(g, rur, OG_FFF);
+    rc = og_mulxm(g, ozh, rur);
+  }
+  if (!rc) rc = og_mulxm(g, rur, ozh);
+  /* CALCXY */
+  if (!rc) rc = og_tofp(g, ror);
+  if (!rc) rc = og_tofp(g, rop);
+  if (!rc) rc = og_tofp(g, ozh);
+  double valens = 0.0, mom[3] = {0, 0, 0};
+  if (!rc) {
+    absvec2(g, ror, rop, ozh, ozh);
+    og_copy(g, ror, ozh);
+    rc = og_toff(g, ror);
+    valens = integ(g, ror);
+  }
+  for (int q = 0; q < 3 && !rc; q++) {
+    og_copy(g, ror, ozh);
+    mul_rtheta(g, ror, q);
+    rc = og_toff(g, ror);
+    mom[q] = integ(g, ror);
+  }
+  if (!rc) {
+    out[9] = mom[0] / valens;
+    out[10] = mom[1] / valens;
+    out[11] = sqrt(mom[2] / valens);
+  }
+  og_chopset(g, -3);
+  og_field_free(rur); og_field_free(rup); og_field_free(uzh);
+  og_field_free(ror); og_field_free(rop); og_field_free(ozh);
   return rc;
 }
 

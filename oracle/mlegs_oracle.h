@@ -112,6 +112,8 @@ int og_initialize(og_solver *s, og_field *psi, og_field *chi, og_field *b);
 int og_solver_prepare(og_solver *s);
 int og_time_step(og_solver *s, og_field *psi, og_field *chi, og_field *b);
 int og_diagnost(const og_grid *g, const og_field *psi, const og_field *chi);
+/* DIAGNOST's every-IT1 quantities (see mlegs_oracle.c): out[12] */
+int og_diagnost_it1(og_grid *g, const og_field *psi, const og_field *chi, double *out);
 double og_time(const og_solver *s);
 int og_nstep(const og_solver *s);
 

@@ -92,6 +92,7 @@ def lib():
             "og_time": ([C.c_void_p], C.c_double),
             "og_nstep": ([C.c_void_p], C.c_int),
             "og_qvortex": ([g, C.c_double, P, P], C.c_int),
+            "og_diagnost_it1": ([g, P, P, C.POINTER(C.c_double)], C.c_int),
             "og_set_sponge": ([g, C.c_int], None),
             "og_sponge": ([g, P, P, P, C.c_double], C.c_int),
             "og_ringcut": ([g, P, C.c_int, C.c_int, C.c_int], C.c_int),
@@ -262,6 +263,14 @@ class OGrid:
         if rc:
             raise RuntimeError(f"og_enemon returned {rc}")
         return np.array(em), np.array(ek)
+
+    def diagnost_it1(self, psi, chi):
+        """DIAGNOST's every-IT1 quantities, in mlegs_amd.hip.DIAGNOST_IT1_KEYS order."""
+        o = (C.c_double * 12)()
+        rc = lib().og_diagnost_it1(self.g, psi.ptr, chi.ptr, o)
+        if rc:
+            raise RuntimeError(f"og_diagnost_it1 returned {rc}")
+        return np.array(o)
 
     def hyperv_coeffs(self):
         o = (C.c_double * 6)()

@@ -151,6 +151,13 @@ def test_bsnsq_new_call_sequence_against_oracle(tmp_path):
         okf, _, _ = p.o.energy(*of)
         totalf = sum(float(x) for x in ke[-1].split()[2:])
         assert abs(totalf - okf.sum()) <= 1e-10 * abs(okf.sum())
+        # DIAGNOST's IT1 section at n = 0 (the driver's call before INITIALIZE)
+        # and in TIME_STEPPING at n = 2, 4: printed, and CALCXY's line in coresize.dat
+        assert r.stdout.count("<<< DIAGNOSTICS >>>") == 3 and "FRACTIONAL CHANGE=" in r.stdout
+        core = [[float(x) for x in ln.split()] for ln in (out / "coresize.dat").read_text().splitlines()]
+        assert len(core) == 3
+        d0 = dict(zip(H.DIAGNOST_IT1_KEYS, p.o.diagnost_it1(*[p.o.field(O.FFF, a, ln) for a, ln in st[:2]])))
+        assert core[0][0] == 0.0 and abs(core[0][3] - d0["r"]) <= 1e-5 * d0["r"]
         # INSPECT: three fields per step, each a time line and one k line
         ins = (out / "inspect.output").read_text().splitlines()
         assert len(ins) == 3 * nsteps * 2
