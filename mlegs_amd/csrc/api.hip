@@ -1347,9 +1347,11 @@ static int visc_cn(mlg_ctx *c, mlg_field *const *f, const mlg_field *const *fa, 
     wk[i]->space = MLG_FFF;
     jobs.j[i] = Visc2Job{fo[i]->e, fo[i]->ln, fa[i]->e, fa[i]->ln, alp[i], wk[i]->e, wk[i]->ln};
   }
-  RET(launch_visc2_rhs_il(c, n, jobs));  // WK row-interleaved for the solve
+  // WK row-interleaved for the one-thread-per-system solve, in columns for the partitioned one
+  const int il = band_lanes(kgrid(c)) == 0;
+  RET(launch_visc2_rhs_il(c, n, jobs, il));
   mlg_field *out[3] = {f[0], f[1], f[2]};
-  return ihelm_n(n, wk, out, alp, 1);
+  return ihelm_n(n, wk, out, alp, il);
 }
 static int visc_cn(mlg_ctx *c, mlg_field *psi, mlg_field *chi, mlg_field *b, const mlg_field *psio, const mlg_field *chio,
                    const mlg_field *bo) {

@@ -385,15 +385,17 @@ int launch_del2(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, do
 //   WK%LN = -ALP*A%LN
 // with DEL2 evaluated on the truncation NRCHOP+2 (nrchop2) at the positions
 // the base truncation keeps, and DEL2's top-mode check of AO at NRCHOP+2.
-// WK is written row-interleaved over the systems: row n of system
+// IL: WK is written row-interleaved over the systems: row n of system
 // s = m + NTCHOPDIM k at wk[n * NSYS + s] (NSYS = NTCHOPDIM * NXCHOPDIM), the
 // layout in which the band substitution, one thread per system, reads it
-// coalesced.  A block takes VR rows of 64 consecutive
-// systems (grid: systems, row chunks, jobs): four threads per column form
-// VR/4 consecutive rows each (DEL2's row as del2_row forms it, then
-// -WK - ALP*A), the tile goes through LDS, and each wave writes whole rows of
-// the 64 systems.
-template <int VR>
+// coalesced; otherwise in the FFF columns, which the partitioned
+// substitution stages with coalesced loads.  A block takes VR rows of 64
+// consecutive systems (grid: systems, row chunks, jobs): four threads per
+// column form VR/4 consecutive rows each (DEL2's row as del2_row forms it,
+// then -WK - ALP*A), and the tile goes through LDS: interleaved, each wave
+// writes whole rows of the 64 systems; in columns, each wave writes VR rows
+// of 64 / VR columns.
+template <int VR, bool IL>
 __global__ void __launch_bounds__(256) k_visc2_rhs_il(KGrid g, int nrchop2, const double *__restrict__ bd,
                                                       const double *__restrict__ ak, Visc2Jobs jobs, int *err) {
   constexpr int RPT = VR / 4;
@@ -487,11 +489,21 @@ __global__ void __launch_bounds__(256) k_visc2_rhs_il(KGrid g, int nrchop2, cons
       so[cs][sub * RPT + i] = sub2(make_double2(-v.x, -v.y), axc(alp, a));
     }
     __syncthreads();
-    if (sys < nsys) {
+    if (IL) {
+      if (sys < nsys) {
 #pragma unroll
-      for (int q = 0; q < RPT; q++) {
-        const int rl = rq * RPT + q, n = r0 + rl;
-        if (n < g.nrchopdim) J.wk[(size_t)n * nsys + sys] = so[c][rl];
+        for (int q = 0; q < RPT; q++) {
+          const int rl = rq * RPT + q, n = r0 + rl;
+          if (n < g.nrchopdim) J.wk[(size_t)n * nsys + sys] = so[c][rl];
+        }
+      }
+    } else {
+      // a wave writes VR consecutive rows of 64 / VR columns per instruction
+#pragma unroll
+      for (int q = 0; q < VR / 4; q++) {
+        const int idx = q * 256 + t, cc = idx / VR, rl = idx % VR, n = r0 + rl;
+        const long sc = s0 + cc;
+        if (sc < nsys && n < g.nrchopdim) J.wk[(size_t)g.nrchopdim * (size_t)sc + n] = so[cc][rl];
       }
     }
   }
@@ -506,7 +518,7 @@ __global__ void __launch_bounds__(256) k_visc2_rhs_il(KGrid g, int nrchop2, cons
     }
   }
 }
-int launch_visc2_rhs_il(mlg_ctx *c, int n, const Visc2Jobs &jobs) {
+int launch_visc2_rhs_il(mlg_ctx *c, int n, const Visc2Jobs &jobs, int il) {
   if (n < 1 || n > 3) return MLG_E_ARG;
   KGrid g = kgrid(c);
   const int nrchop2 = g.nrchop + 2;
@@ -514,8 +526,12 @@ int launch_visc2_rhs_il(mlg_ctx *c, int n, const Visc2Jobs &jobs) {
   const long nsys = (long)g.ntchopdim * g.nxchopdim;
   constexpr int VR = 16;
   const dim3 grid((unsigned)((nsys + 63) / 64), (unsigned)((g.nrchopdim + VR - 1) / VR), n);
-  hipLaunchKernelGGL(k_visc2_rhs_il<VR>, grid, dim3(256), 0, c->stream, g, nrchop2, c->d.bdel2h,
-                     c->d.ak, jobs, c->d.err);
+  if (il)
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_visc2_rhs_il<VR, true>), grid, dim3(256), 0, c->stream, g, nrchop2, c->d.bdel2h,
+                       c->d.ak, jobs, c->d.err);
+  else
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_visc2_rhs_il<VR, false>), grid, dim3(256), 0, c->stream, g, nrchop2,
+                       c->d.bdel2h, c->d.ak, jobs, c->d.err);
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
@@ -1361,6 +1377,20 @@ struct BandJobs {
   int gjob[MAX_BAND_JOBS][MAX_BAND_JOBS];  // and their indices
 };
 
+// LN of the m = 0, k = 0 results, set by the thread that wrote their row 0.
+template <int NJ>
+__device__ void penta_ln(const KGrid &g, int op, int has_ln, const BandJobs &jobs, int grp, double2 *const (&x)[NJ]) {
+#pragma unroll
+  for (int jj = 0; jj < NJ; jj++) {
+    double *ln_a = jobs.j[jobs.gjob[grp][jj]].ln_a;
+    if (op == 0) ln_a[0] = has_ln ? x[jj][0].x * g.norm0 : 0.0;
+    if (op == 1) {
+      ln_a[0] = x[jj][0].x * g.norm0;
+      x[jj][0] = zero2();
+    }
+  }
+}
+
 // The systems of one (m, k) for the NJ jobs of group grp (one factor set).
 template <int NJ>
 __device__ void apply_penta_n(const KGrid &g, int op, int has_ln, const BandJobs &jobs, int grp, long sys) {
@@ -1384,15 +1414,272 @@ __device__ void apply_penta_n(const KGrid &g, int op, int has_ln, const BandJobs
   const FactorView FV = factor_view(jobs.j[jobs.gjob[grp][0]].F, g.nrchopdim, ncls);
   band_apply_n<NJ, 5, 2, (NJ == 1 ? 16 : 8)>(nn, xin, jobs.bil ? nsys : 1L, xt, x, nsys, FV, ncls,
                                              mm + (long)g.ntchopdim * kabs);
-  if (mm == 0 && k == 0) {
+  if (mm == 0 && k == 0) penta_ln<NJ>(g, op, has_ln, jobs, grp, x);
+}
+
+// LDS-staged partitioned substitution.  A block (one wave) takes W = 64 / P
+// consecutive systems of one group, P lanes each: lane j P + seg owns rows
+// [seg PART_LS, (seg + 1) PART_LS) of system j.
+//  - The systems' right-hand sides come into LDS with coalesced loads, each
+//    lane's rows in a slot of PART_LS + 1 (the pad spreads the lanes' rows
+//    over the banks); the factor rows of a lane's segment come into its
+//    registers.
+//  - Each pass is a second-order linear recurrence in the unscaled values
+//    (forward: v_n = b_n - l1_n v_{n-1} - l2_n v_{n-2}; back: x_n = y_n -
+//    u1_n x_{n+1} - u2_n x_{n+2}), so a segment's end state is E = p + G s
+//    for its start state s: p from a run over the segment's rows from the
+//    zero state, G's columns from runs from the unit states without right-
+//    hand side.  The start states pass from lane to lane (part_carry: P - 1
+//    steps of DPP moves), then every lane runs its rows again from its own
+//    start state.  The forward results overwrite the right-hand side in
+//    LDS, the back substitution's results overwrite those, and the block
+//    stores the columns with coalesced writes.
+//  - Every lane runs all PART_LS rows with no branch: rows at or beyond the
+//    system's end, and the couplings LUBKSBB skips (rows 0, 1 forward, the
+//    last two rows back), take zero factors and right-hand sides, which
+//    carry a zero state through exactly.
+//  - The first two segments of each pass are LUBKSBB's arithmetic exactly
+//    (their start states are exact); the others differ from it by the
+//    rounding of the carried states.
+#define PART_LS 8
+__host__ __device__ constexpr size_t part_lds_bytes(int nj) { return sizeof(double2) * nj * 64 * (PART_LS + 1); }
+
+// Rows of penta-diagonal system sys under operator op (0: not one, or no
+// rows), with its m index and k slot.
+__device__ __forceinline__ int penta_rows(const KGrid &g, int op, long sys, int &mm, int &k) {
+  mm = k = 0;
+  if (sys >= (long)g.ntchopdim * g.nxchopdim) return 0;
+  const int m = (int)(sys % g.ntchopdim), kk = (int)(sys / g.ntchopdim);
+  const int nn = s_nrchops(g.nrchop, m * g.minc);
+  if (nn < 1 || m >= g.ntchop || solve_kind(op, m, kk) != 0) return 0;
+  mm = m;
+  k = kk;
+  return nn;
+}
+
+__device__ __forceinline__ double2 sel2(bool c, double2 v) { return c ? v : zero2(); }
+// Compiler-only fence: the rows of a block are read from LDS where the block
+// starts, not kept in registers from an earlier pass or hoisted ahead.
+__device__ __forceinline__ void lds_fence() { asm volatile("" ::: "memory"); }
+
+// One pass over a lane's PART_LS rows, in the direction the caller's row
+// order gives: state (h0, h1) = the last two values, c1, c2 the couplings.
+template <int NJ>
+__device__ __forceinline__ void part_row(double c1, double c2, const double2 (&r)[NJ], double2 (&h0)[NJ],
+                                         double2 (&h1)[NJ]) {
+#pragma unroll
+  for (int jj = 0; jj < NJ; jj++) {
+    double2 v = r[jj];
+    v = sub2(v, axc(c1, h0[jj]));
+    v = sub2(v, axc(c2, h1[jj]));
+    h1[jj] = h0[jj];
+    h0[jj] = v;
+  }
+}
+__device__ __forceinline__ void part_unit(double c1, double c2, double &g0, double &g1) {
+  double t = 0.0;
+  t = t - c1 * g0;
+  t = t - c2 * g1;
+  g1 = g0;
+  g0 = t;
+}
+
+// Values of the lane below (dir = 1) or above (dir = -1) in the wave, moved
+// by DPP (wave_shr:1 / wave_shl:1); lane 0, resp. 63, gets zero.
+template <int DIR>
+__device__ __forceinline__ double dpp_next(double v) {
+  constexpr int ctrl = DIR > 0 ? 0x138 : 0x130;
+  const long long bits = __double_as_longlong(v);
+  const int lo = __builtin_amdgcn_update_dpp(0, (int)(bits & 0xffffffff), ctrl, 0xf, 0xf, false);
+  const int hi = __builtin_amdgcn_update_dpp(0, (int)(bits >> 32), ctrl, 0xf, 0xf, false);
+  return __longlong_as_double(((long long)hi << 32) | (long long)(unsigned)lo);
+}
+template <int DIR>
+__device__ __forceinline__ double2 dpp_next2(double2 v) {
+  return make_double2(dpp_next<DIR>(v.x), dpp_next<DIR>(v.y));
+}
+
+// The start states of the segments, passed along the P lanes of each system
+// in turn (lanes seg + 1 after seg forward, seg - 1 after seg back): every
+// lane forms its segment's end state E = p + G s from its current start
+// state, and at step t the lane of segment t (forward) or P - 1 - t (back)
+// takes its neighbour's.  Each carried value is a state of the recurrence,
+// as in the serial substitution.
+template <int NJ, int DIR>
+__device__ __forceinline__ void part_carry(int P, int seg, const double2 (&p0)[NJ], const double2 (&p1)[NJ],
+                                           const double (&G)[4], double2 (&s0)[NJ], double2 (&s1)[NJ]) {
+  for (int t = 1; t < P; t++) {
+    const int take = DIR > 0 ? t : P - 1 - t;
+#pragma unroll
+    for (int jj = 0; jj < NJ; jj++) {
+      const double2 e0 = add2(p0[jj], add2(axc(G[0], s0[jj]), axc(G[1], s1[jj])));
+      const double2 e1 = add2(p1[jj], add2(axc(G[2], s0[jj]), axc(G[3], s1[jj])));
+      const double2 r0 = dpp_next2<DIR>(e0), r1 = dpp_next2<DIR>(e1);
+      if (seg == take) {
+        s0[jj] = r0;
+        s1[jj] = r1;
+      }
+    }
+  }
+}
+
+template <int NJ>
+__device__ void apply_penta_part(const KGrid &g, int op, int has_ln, const BandJobs &jobs, int grp, long sys0, int P,
+                                 double2 *lds) {
+  constexpr int LS = PART_LS, SB = 4, SLOT = PART_LS + 1;
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  const long ncls = (long)g.ntchopdim * g.nxchop;
+  const int lane = threadIdx.x, W = 64 / P;
+  const int j = lane / P, seg = lane % P;
+  int mm, k;
+  const int nd = penta_rows(g, op, sys0 + j, mm, k);
+  const int a = seg * LS;
+  const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
+  const long cls = mm + (long)g.ntchopdim * kabs;
+  const FactorView FV = factor_view(jobs.j[jobs.gjob[grp][0]].F, g.nrchopdim, ncls);
+  const int nlast = max(nd - 1, 0);
+  double l1[LS], l2[LS], rp[LS];
+#pragma unroll
+  for (int q = 0; q < LS; q++) {
+    const int n = a + q;
+    const double *ff = FV.ff + ((size_t)min(n, nlast) * ncls + cls) * 3;
+    l1[q] = (n >= 1 && n < nd) ? ff[0] : 0.0;
+    l2[q] = (n >= 2 && n < nd) ? ff[1] : 0.0;
+    rp[q] = ff[2];
+  }
+  // right-hand sides into LDS: system jp's row n at slot (jp P + n / LS), row n % LS
+  const size_t fstride = (size_t)64 * SLOT;
+  if (jobs.bil) {
+    // row n of system s at b[n nsys + s]: lanes over W consecutive systems
+    int m2, k2;
+    const int jp = lane % W;
+    const int ndp = penta_rows(g, op, sys0 + jp, m2, k2);
+    for (int n = lane / W; n < ndp; n += P) {
+      const int sl = (jp * P + n / LS) * SLOT + n % LS;
+#pragma unroll
+      for (int jj = 0; jj < NJ; jj++) lds[jj * fstride + sl] = jobs.j[jobs.gjob[grp][jj]].b[(size_t)n * nsys + sys0 + jp];
+    }
+  } else {
+    for (int jp = 0; jp < W; jp++) {
+      int m2, k2;
+      const int ndp = penta_rows(g, op, sys0 + jp, m2, k2);
+      const size_t colp = (size_t)g.nrchopdim * ((size_t)m2 + (size_t)g.ntchopdim * k2);
+      for (int n = lane; n < ndp; n += 64) {
+        const int sl = (jp * P + n / LS) * SLOT + n % LS;
+#pragma unroll
+        for (int jj = 0; jj < NJ; jj++) lds[jj * fstride + sl] = jobs.j[jobs.gjob[grp][jj]].b[colp + n];
+      }
+    }
+  }
+  __syncthreads();
+  double2 *my = lds + (size_t)lane * SLOT;
+  double2 s0[NJ], s1[NJ], p0[NJ], p1[NJ];
+  // ---- forward pass (rows ascending)
+#pragma unroll
+  for (int jj = 0; jj < NJ; jj++) s0[jj] = s1[jj] = p0[jj] = p1[jj] = zero2();
+  {
+    double ga0 = 1.0, ga1 = 0.0, gb0 = 0.0, gb1 = 1.0;
+#pragma unroll
+    for (int qb = 0; qb < LS; qb += SB) {
+      double2 r[SB][NJ];
+      lds_fence();
+#pragma unroll
+      for (int q = 0; q < SB; q++)
+#pragma unroll
+        for (int jj = 0; jj < NJ; jj++) r[q][jj] = sel2(a + qb + q < nd, my[jj * fstride + qb + q]);
+#pragma unroll
+      for (int q = 0; q < SB; q++) {
+        part_unit(l1[qb + q], l2[qb + q], ga0, ga1);
+        part_unit(l1[qb + q], l2[qb + q], gb0, gb1);
+        part_row<NJ>(l1[qb + q], l2[qb + q], r[q], p0, p1);
+      }
+    }
+    const double G[4] = {ga0, gb0, ga1, gb1};
+    part_carry<NJ, 1>(P, seg, p0, p1, G, s0, s1);
+  }
+#pragma unroll
+  for (int qb = 0; qb < LS; qb += SB) {
+    double2 r[SB][NJ];
+    lds_fence();
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+#pragma unroll
+      for (int jj = 0; jj < NJ; jj++) r[q][jj] = sel2(a + qb + q < nd, my[jj * fstride + qb + q]);
+#pragma unroll
+    for (int q = 0; q < SB; q++) {
+      part_row<NJ>(l1[qb + q], l2[qb + q], r[q], s0, s1);
+#pragma unroll
+      for (int jj = 0; jj < NJ; jj++) my[jj * fstride + qb + q] = axc(rp[qb + q], s0[jj]);
+    }
+  }
+  // ---- back substitution (rows descending)
+  double u1[LS], u2[LS];
+#pragma unroll
+  for (int q = 0; q < LS; q++) {
+    const int n = a + q;
+    const double *fb = FV.fb + ((size_t)min(n, nlast) * ncls + cls) * 2;
+    u1[q] = n + 1 < nd ? fb[0] : 0.0;
+    u2[q] = n + 2 < nd ? fb[1] : 0.0;
+  }
+#pragma unroll
+  for (int jj = 0; jj < NJ; jj++) s0[jj] = s1[jj] = p0[jj] = p1[jj] = zero2();
+  {
+    double ga0 = 1.0, ga1 = 0.0, gb0 = 0.0, gb1 = 1.0;
+#pragma unroll
+    for (int qb = LS - SB; qb >= 0; qb -= SB) {
+      double2 r[SB][NJ];
+      lds_fence();
+#pragma unroll
+      for (int q = 0; q < SB; q++)
+#pragma unroll
+        for (int jj = 0; jj < NJ; jj++) r[q][jj] = sel2(a + qb + q < nd, my[jj * fstride + qb + q]);
+#pragma unroll
+      for (int q = SB - 1; q >= 0; q--) {
+        part_unit(u1[qb + q], u2[qb + q], ga0, ga1);
+        part_unit(u1[qb + q], u2[qb + q], gb0, gb1);
+        part_row<NJ>(u1[qb + q], u2[qb + q], r[q], p0, p1);
+      }
+    }
+    const double G[4] = {ga0, gb0, ga1, gb1};
+    part_carry<NJ, -1>(P, seg, p0, p1, G, s0, s1);
+  }
+#pragma unroll
+  for (int qb = LS - SB; qb >= 0; qb -= SB) {
+    double2 r[SB][NJ];
+    lds_fence();
+#pragma unroll
+    for (int q = 0; q < SB; q++)
+#pragma unroll
+      for (int jj = 0; jj < NJ; jj++) r[q][jj] = sel2(a + qb + q < nd, my[jj * fstride + qb + q]);
+#pragma unroll
+    for (int q = SB - 1; q >= 0; q--) {
+      part_row<NJ>(u1[qb + q], u2[qb + q], r[q], s0, s1);
+#pragma unroll
+      for (int jj = 0; jj < NJ; jj++) my[jj * fstride + qb + q] = s0[jj];
+    }
+  }
+  __syncthreads();
+  // LN of the m = 0, k = 0 results (row 0 of segment 0), then the columns out
+  if (seg == 0 && nd > 0 && mm == 0 && k == 0) {
 #pragma unroll
     for (int jj = 0; jj < NJ; jj++) {
       double *ln_a = jobs.j[jobs.gjob[grp][jj]].ln_a;
-      if (op == 0) ln_a[0] = has_ln ? x[jj][0].x * g.norm0 : 0.0;
+      if (op == 0) ln_a[0] = has_ln ? my[jj * fstride].x * g.norm0 : 0.0;
       if (op == 1) {
-        ln_a[0] = x[jj][0].x * g.norm0;
-        x[jj][0] = zero2();
+        ln_a[0] = my[jj * fstride].x * g.norm0;
+        my[jj * fstride] = zero2();
       }
+    }
+  }
+  __syncthreads();
+  for (int jp = 0; jp < W; jp++) {
+    int m2, k2;
+    const int ndp = penta_rows(g, op, sys0 + jp, m2, k2);
+    const size_t colp = (size_t)g.nrchopdim * ((size_t)m2 + (size_t)g.ntchopdim * k2);
+    for (int n = lane; n < ndp; n += 64) {
+      const int sl = (jp * P + n / LS) * SLOT + n % LS;
+#pragma unroll
+      for (int jj = 0; jj < NJ; jj++) jobs.j[jobs.gjob[grp][jj]].a[colp + n] = lds[jj * fstride + sl];
     }
   }
 }
@@ -1465,27 +1752,55 @@ __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, 
 
 // One launch: blocks [0, nm0) take the m = 0 systems (one block each, first
 // because they are the longest), the remaining blocks the penta-diagonal
-// systems (one thread each), so both sets run side by side; grid row y
-// solves field y of a batch of independent solves.
-// blocks [0, nm0 n): the m = 0 systems, one block per (k, job); then npenta
-// blocks per group of jobs that share a factor set, one thread per (m, k)
-// solving that group's right-hand sides together
+// systems, so both sets run side by side; grid row y solves field y of a
+// batch of independent solves.  k_band_apply: one thread per penta-diagonal
+// system; k_band_apply_part: P lanes per system, 64 / P systems per block
+// (apply_penta_part), at three waves per SIMD.
+__device__ __forceinline__ bool band_m0_block(const KGrid &g, const double *bxm, int op, int has_ln, double lnval,
+                                              const BandJobs &jobs, int nm0, double *m0s) {
+  const int bid = blockIdx.x;
+  if (bid >= nm0 * jobs.n) return false;
+  const BandJob &J = jobs.j[bid / nm0];
+  apply_m0(g, bxm, op, J.alp, has_ln, lnval, J.b, J.a, J.ln_a, J.ln_b, J.F, bid % nm0, m0s, jobs.bil);
+  return true;
+}
+
 __global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, int op, int has_ln, double lnval, BandJobs jobs,
                                                    int nm0, int npenta) {
   extern __shared__ __align__(16) double m0s[];
-  const int bid = blockIdx.x;
-  if (bid < nm0 * jobs.n) {
-    const BandJob &J = jobs.j[bid / nm0];
-    apply_m0(g, bxm, op, J.alp, has_ln, lnval, J.b, J.a, J.ln_a, J.ln_b, J.F, bid % nm0, m0s, jobs.bil);
-    return;
-  }
-  const int pb = bid - nm0 * jobs.n, grp = pb / npenta;
+  if (band_m0_block(g, bxm, op, has_ln, lnval, jobs, nm0, m0s)) return;
+  const int pb = blockIdx.x - nm0 * jobs.n, grp = pb / npenta;
   const long sys = (long)(pb - grp * npenta) * blockDim.x + threadIdx.x;
   switch (jobs.gcount[grp]) {
     case 1: apply_penta_n<1>(g, op, has_ln, jobs, grp, sys); break;
     case 2: apply_penta_n<2>(g, op, has_ln, jobs, grp, sys); break;
     default: apply_penta_n<3>(g, op, has_ln, jobs, grp, sys); break;
   }
+}
+
+__global__ void __launch_bounds__(64, 3) k_band_apply_part(KGrid g, const double *bxm, int op, int has_ln, double lnval,
+                                                           BandJobs jobs, int nm0, int npenta, int P) {
+  extern __shared__ __align__(16) double m0s[];
+  if (band_m0_block(g, bxm, op, has_ln, lnval, jobs, nm0, m0s)) return;
+  const int pb = blockIdx.x - nm0 * jobs.n, grp = pb / npenta;
+  const long sys0 = (long)(pb - grp * npenta) * (64 / P);
+  double2 *lds = reinterpret_cast<double2 *>(m0s);
+  switch (jobs.gcount[grp]) {
+    case 1: apply_penta_part<1>(g, op, has_ln, jobs, grp, sys0, P, lds); break;
+    case 2: apply_penta_part<2>(g, op, has_ln, jobs, grp, sys0, P, lds); break;
+    default: apply_penta_part<3>(g, op, has_ln, jobs, grp, sys0, P, lds); break;
+  }
+}
+
+// Lanes per penta-diagonal system of the partitioned solve: the smallest
+// power of two, at least 4, whose segments of PART_LS rows cover NRCHOP.
+// 0: one thread per system (NRCHOP below 64, where systems are short, or
+// above 64 PART_LS); that path is LUBKSBB's arithmetic, bitwise.
+int band_lanes(const KGrid &g) {
+  if (g.nrchop < 64 || g.nrchop > 64 * PART_LS) return 0;
+  int p = 4;
+  while (p * PART_LS < g.nrchop) p *= 2;
+  return p;
 }
 
 // Factor sets are cached per (operator, alp, truncation): the matrices
@@ -1553,9 +1868,21 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, doub
     jobs.gjob[gi][jobs.gcount[gi]++] = i;
   }
   KScope ks(c, K_BAND_SOLVE, n * 2.0 * 16.0 * fff_count(g));  // the coefficients read and written once
-  const int npenta = (int)((nsys + 63) / 64), nm0 = (op != 1) ? g.nxchopdim : 0;
-  hipLaunchKernelGGL(k_band_apply, dim3(nm0 * n + npenta * jobs.ng), dim3(64), nm0 ? m0_lds_bytes(g) : 0, c->stream, g,
-                     c->d.bxm, op, has_ln, ln, jobs, nm0, npenta);
+  const int lanes = band_lanes(g);
+  const int nm0 = (op != 1) ? g.nxchopdim : 0;
+  size_t lds = nm0 ? m0_lds_bytes(g) : 0;
+  if (lanes) {
+    int njmax = 1;
+    for (int i = 0; i < jobs.ng; i++) njmax = std::max(njmax, jobs.gcount[i]);
+    lds = std::max(lds, part_lds_bytes(njmax));
+    const int npenta = (int)((nsys + 64 / lanes - 1) / (64 / lanes));
+    hipLaunchKernelGGL(k_band_apply_part, dim3(nm0 * n + npenta * jobs.ng), dim3(64), lds, c->stream, g, c->d.bxm, op,
+                       has_ln, ln, jobs, nm0, npenta, lanes);
+  } else {
+    const int npenta = (int)((nsys + 63) / 64);
+    hipLaunchKernelGGL(k_band_apply, dim3(nm0 * n + npenta * jobs.ng), dim3(64), lds, c->stream, g, c->d.bxm, op, has_ln,
+                       ln, jobs, nm0, npenta);
+  }
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }

@@ -342,7 +342,12 @@ struct Visc2Jobs {
 // n (<= 3) independent right-hand sides in one launch, WK written
 // row-interleaved over the systems (row n of system s = m + NTCHOPDIM k at
 // wk[n * NTCHOPDIM * NXCHOPDIM + s]), as the band solve reads it
-int launch_visc2_rhs_il(mlg_ctx *c, int n, const Visc2Jobs &jobs);
+// il: WK row-interleaved over the systems (the one-thread-per-system
+// substitution), else in the FFF columns (the partitioned substitution)
+int launch_visc2_rhs_il(mlg_ctx *c, int n, const Visc2Jobs &jobs, int il);
+// lanes per penta-diagonal system of the band substitution (0: one thread
+// per system, which reads row-interleaved right-hand sides coalesced)
+int band_lanes(const KGrid &g);
 int launch_set_ln(mlg_ctx *c, double *dst, const double *src, double scale, double add);
 int launch_zero_lns(mlg_ctx *c, double *const *ptrs, int n);  // LN = 0 for n fields, one launch per 32
 int launch_delsqh(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, double *ln_b);
