@@ -1523,56 +1523,21 @@ __device__ __forceinline__ void part_carry(int P, int seg, const double2 (&p0)[N
   }
 }
 
-template <int NJ>
-__device__ void apply_penta_part(const KGrid &g, int op, int has_ln, const BandJobs &jobs, int grp, long sys0, int P,
-                                 double2 *lds) {
-  constexpr int LS = PART_LS, SB = 4, SLOT = PART_LS + 1;
-  const long nsys = (long)g.ntchopdim * g.nxchopdim;
-  const long ncls = (long)g.ntchopdim * g.nxchop;
-  const int lane = threadIdx.x, W = 64 / P;
-  const int j = lane / P, seg = lane % P;
-  int mm, k;
-  const int nd = penta_rows(g, op, sys0 + j, mm, k);
-  const int a = seg * LS;
-  const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
-  const long cls = mm + (long)g.ntchopdim * kabs;
-  const FactorView FV = factor_view(jobs.j[jobs.gjob[grp][0]].F, g.nrchopdim, ncls);
-  const int nlast = max(nd - 1, 0);
+// The passes of the partitioned substitution on rows staged in LDS (my:
+// this lane's slot of PART_LS + 1 rows; field jj at my + jj fstride), for a
+// lane owning rows a .. a + PART_LS - 1 of a system of nd rows.  fwd(n, c1,
+// c2, sc) gives the couplings of row n of the forward pass (to the values of
+// rows n - 1, n - 2) and the scale of its result, back(n, c1, c2) those of
+// the back substitution (to rows n + 1, n + 2); both give zero couplings
+// where the serial substitution has none.  The back pass's couplings are
+// loaded after the forward pass, when the forward ones are dead.
+template <int NJ, class Fwd, class Back>
+__device__ __forceinline__ void part_passes(double2 *my, size_t fstride, int a, int nd, int P, int seg, Fwd fwd,
+                                            Back back) {
+  constexpr int LS = PART_LS, SB = 4;
   double l1[LS], l2[LS], rp[LS];
 #pragma unroll
-  for (int q = 0; q < LS; q++) {
-    const int n = a + q;
-    const double *ff = FV.ff + ((size_t)min(n, nlast) * ncls + cls) * 3;
-    l1[q] = (n >= 1 && n < nd) ? ff[0] : 0.0;
-    l2[q] = (n >= 2 && n < nd) ? ff[1] : 0.0;
-    rp[q] = ff[2];
-  }
-  // right-hand sides into LDS: system jp's row n at slot (jp P + n / LS), row n % LS
-  const size_t fstride = (size_t)64 * SLOT;
-  if (jobs.bil) {
-    // row n of system s at b[n nsys + s]: lanes over W consecutive systems
-    int m2, k2;
-    const int jp = lane % W;
-    const int ndp = penta_rows(g, op, sys0 + jp, m2, k2);
-    for (int n = lane / W; n < ndp; n += P) {
-      const int sl = (jp * P + n / LS) * SLOT + n % LS;
-#pragma unroll
-      for (int jj = 0; jj < NJ; jj++) lds[jj * fstride + sl] = jobs.j[jobs.gjob[grp][jj]].b[(size_t)n * nsys + sys0 + jp];
-    }
-  } else {
-    for (int jp = 0; jp < W; jp++) {
-      int m2, k2;
-      const int ndp = penta_rows(g, op, sys0 + jp, m2, k2);
-      const size_t colp = (size_t)g.nrchopdim * ((size_t)m2 + (size_t)g.ntchopdim * k2);
-      for (int n = lane; n < ndp; n += 64) {
-        const int sl = (jp * P + n / LS) * SLOT + n % LS;
-#pragma unroll
-        for (int jj = 0; jj < NJ; jj++) lds[jj * fstride + sl] = jobs.j[jobs.gjob[grp][jj]].b[colp + n];
-      }
-    }
-  }
-  __syncthreads();
-  double2 *my = lds + (size_t)lane * SLOT;
+  for (int q = 0; q < LS; q++) fwd(a + q, l1[q], l2[q], rp[q]);
   double2 s0[NJ], s1[NJ], p0[NJ], p1[NJ];
   // ---- forward pass (rows ascending)
 #pragma unroll
@@ -1615,12 +1580,7 @@ __device__ void apply_penta_part(const KGrid &g, int op, int has_ln, const BandJ
   // ---- back substitution (rows descending)
   double u1[LS], u2[LS];
 #pragma unroll
-  for (int q = 0; q < LS; q++) {
-    const int n = a + q;
-    const double *fb = FV.fb + ((size_t)min(n, nlast) * ncls + cls) * 2;
-    u1[q] = n + 1 < nd ? fb[0] : 0.0;
-    u2[q] = n + 2 < nd ? fb[1] : 0.0;
-  }
+  for (int q = 0; q < LS; q++) back(a + q, u1[q], u2[q]);
 #pragma unroll
   for (int jj = 0; jj < NJ; jj++) s0[jj] = s1[jj] = p0[jj] = p1[jj] = zero2();
   {
@@ -1658,6 +1618,63 @@ __device__ void apply_penta_part(const KGrid &g, int op, int has_ln, const BandJ
       for (int jj = 0; jj < NJ; jj++) my[jj * fstride + qb + q] = s0[jj];
     }
   }
+}
+
+template <int NJ>
+__device__ void apply_penta_part(const KGrid &g, int op, int has_ln, const BandJobs &jobs, int grp, long sys0, int P,
+                                 double2 *lds) {
+  constexpr int LS = PART_LS, SLOT = PART_LS + 1;
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  const long ncls = (long)g.ntchopdim * g.nxchop;
+  const int lane = threadIdx.x, W = 64 / P;
+  const int j = lane / P, seg = lane % P;
+  int mm, k;
+  const int nd = penta_rows(g, op, sys0 + j, mm, k);
+  const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
+  const long cls = mm + (long)g.ntchopdim * kabs;
+  const FactorView FV = factor_view(jobs.j[jobs.gjob[grp][0]].F, g.nrchopdim, ncls);
+  const int nlast = max(nd - 1, 0);
+  // right-hand sides into LDS: system jp's row n at slot (jp P + n / LS), row n % LS
+  const size_t fstride = (size_t)64 * SLOT;
+  if (jobs.bil) {
+    // row n of system s at b[n nsys + s]: lanes over W consecutive systems
+    int m2, k2;
+    const int jp = lane % W;
+    const int ndp = penta_rows(g, op, sys0 + jp, m2, k2);
+    for (int n = lane / W; n < ndp; n += P) {
+      const int sl = (jp * P + n / LS) * SLOT + n % LS;
+#pragma unroll
+      for (int jj = 0; jj < NJ; jj++) lds[jj * fstride + sl] = jobs.j[jobs.gjob[grp][jj]].b[(size_t)n * nsys + sys0 + jp];
+    }
+  } else {
+    for (int jp = 0; jp < W; jp++) {
+      int m2, k2;
+      const int ndp = penta_rows(g, op, sys0 + jp, m2, k2);
+      const size_t colp = (size_t)g.nrchopdim * ((size_t)m2 + (size_t)g.ntchopdim * k2);
+      for (int n = lane; n < ndp; n += 64) {
+        const int sl = (jp * P + n / LS) * SLOT + n % LS;
+#pragma unroll
+        for (int jj = 0; jj < NJ; jj++) lds[jj * fstride + sl] = jobs.j[jobs.gjob[grp][jj]].b[colp + n];
+      }
+    }
+  }
+  __syncthreads();
+  double2 *my = lds + (size_t)lane * SLOT;
+  // LUBKSBB: row n's multipliers against rows n-1, n-2 and 1/pivot, then its
+  // normalised upper entries against rows n+1, n+2
+  part_passes<NJ>(
+      my, fstride, seg * LS, nd, P, seg,
+      [&](int n, double &c1, double &c2, double &sc) {
+        const double *ff = FV.ff + ((size_t)min(n, nlast) * ncls + cls) * 3;
+        c1 = (n >= 1 && n < nd) ? ff[0] : 0.0;
+        c2 = (n >= 2 && n < nd) ? ff[1] : 0.0;
+        sc = ff[2];
+      },
+      [&](int n, double &c1, double &c2) {
+        const double *fb = FV.fb + ((size_t)min(n, nlast) * ncls + cls) * 2;
+        c1 = n + 1 < nd ? fb[0] : 0.0;
+        c2 = n + 2 < nd ? fb[1] : 0.0;
+      });
   __syncthreads();
   // LN of the m = 0, k = 0 results (row 0 of segment 0), then the columns out
   if (seg == 0 && nd > 0 && mm == 0 && k == 0) {
@@ -2068,6 +2085,87 @@ __global__ void __launch_bounds__(64) k_band_apply_t(KGrid g, const double *bxm,
     if (sys0 + i / R < nsys) J.out[base + i] = xs[i];
 }
 
+// The transposed solve with P lanes per system (band_lanes): A^T = U^T D L^T
+// on the partitioned passes, forward over the rows with row n's couplings
+// U(n-1, n), U(n-2, n) and scale 1/pivot(n), back with L(n+1, n), L(n+2, n).
+// The columns are staged in LDS chopped to NRCHOPS(m) (IDEL2LN's m = 0
+// columns with the transpose of the fix at infinity, and at k = 0 the LN
+// adjoint in row 0) and written back whole, zeros beyond NRCHOPS(m)
+// included.  IHELM's m = 0 systems (their own form, XM^T and LN terms) take
+// blocks [0, nm0), one per k, solved by one thread from LDS as in
+// k_band_apply_t.  in may equal out.
+__global__ void __launch_bounds__(64, 3) k_band_apply_t_part(KGrid g, const double *bxm, const double *at1, int op,
+                                                             BandTJobs jobs, int nm0, int P) {
+  extern __shared__ __align__(16) double2 xs[];
+  constexpr int LS = PART_LS, SLOT = PART_LS + 1;
+  const BandTJob &J = jobs.j[blockIdx.y];
+  const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  const long ncls = (long)g.ntchopdim * g.nxchop;
+  const int R = g.nrchopdim, lane = threadIdx.x;
+  if ((int)blockIdx.x < nm0) {
+    const int k = blockIdx.x;
+    const size_t base = (size_t)R * ((size_t)g.ntchopdim * k);
+    const int nn = g.ntchop > 0 ? s_nrchops(g.nrchop, 0) : 0;
+    for (int i = lane; i < R; i += 64) xs[i] = i < nn ? J.in[base + i] : zero2();
+    __syncthreads();
+    if (lane == 0 && nn >= 1)
+      solve_t_system(g, bxm, at1, op, J.alp, factor_view(J.F, R, ncls), ncls, 0, k, nn, xs, J.ln_in, J.ln_out);
+    __syncthreads();
+    for (int i = lane; i < R; i += 64) J.out[base + i] = xs[i];
+    return;
+  }
+  const int W = 64 / P;
+  const long sys0 = (long)(blockIdx.x - nm0) * W;
+  const size_t fstride = (size_t)64 * SLOT;
+  for (int jp = 0; jp < W; jp++) {
+    int m2, k2;
+    const int ndp = penta_rows(g, op, sys0 + jp, m2, k2);
+    const size_t colp = (size_t)R * ((size_t)m2 + (size_t)g.ntchopdim * k2);
+    const bool fix = op == 1 && m2 == 0 && ndp > 0;
+    const double2 a0 = fix ? J.in[colp] : zero2();
+    for (int n = lane; n < ndp; n += 64) {
+      double2 v = J.in[colp + n];
+      if (fix) {
+        v = sub2(v, axc(at1[n] / at1[0], a0));
+        if (k2 == 0 && n == 0) v = make_double2(g.norm0 * J.ln_in[0], 0.0);
+      }
+      xs[(jp * P + n / LS) * SLOT + n % LS] = v;
+    }
+  }
+  __syncthreads();
+  {
+    const int j = lane / P, seg = lane % P;
+    int mm, k;
+    const int nd = penta_rows(g, op, sys0 + j, mm, k);
+    const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
+    const long cls = mm + (long)g.ntchopdim * kabs;
+    const FactorView FV = factor_view(J.F, R, ncls);
+    const int nlast = max(nd - 1, 0);
+    auto row = [&](int n) { return (size_t)min(max(n, 0), nlast) * ncls + cls; };
+    part_passes<1>(
+        xs + (size_t)lane * SLOT, fstride, seg * LS, nd, P, seg,
+        [&](int n, double &c1, double &c2, double &sc) {
+          c1 = (n >= 1 && n < nd) ? FV.fb[row(n - 1) * 2] : 0.0;
+          c2 = (n >= 2 && n < nd) ? FV.fb[row(n - 2) * 2 + 1] : 0.0;
+          sc = FV.ff[row(n) * 3 + 2];
+        },
+        [&](int n, double &c1, double &c2) {
+          c1 = n + 1 < nd ? FV.ff[row(n + 1) * 3] : 0.0;
+          c2 = n + 2 < nd ? FV.ff[row(n + 2) * 3 + 1] : 0.0;
+        });
+  }
+  __syncthreads();
+  for (int jp = 0; jp < W; jp++) {
+    const long sys = sys0 + jp;
+    if (sys >= nsys) break;
+    const int m2 = (int)(sys % g.ntchopdim), k2 = (int)(sys / g.ntchopdim);
+    if (solve_kind(op, m2, k2) != 0) continue;  // IHELM's m = 0: its own block
+    const int ndp = m2 < g.ntchop ? s_nrchops(g.nrchop, m2 * g.minc) : 0;
+    const size_t colp = (size_t)R * (size_t)sys;
+    for (int n = lane; n < R; n += 64) J.out[colp + n] = n < ndp ? xs[(jp * P + n / LS) * SLOT + n % LS] : zero2();
+  }
+}
+
 int launch_band_solve_t_n(mlg_ctx *c, int op, int n, const BandTJob *in) {
   if ((op != 1 && op != 2) || n < 1 || n > MAX_BAND_JOBS) return MLG_E_ARG;
   KGrid g = kgrid(c);
@@ -2083,9 +2181,19 @@ int launch_band_solve_t_n(mlg_ctx *c, int op, int n, const BandTJob *in) {
     jobs.j[i].F = F;
   }
   const long nsys = (long)g.ntchopdim * g.nxchopdim;
+  KScope ks(c, K_BAND_SOLVE, n * 2.0 * 16.0 * fff_count(g));
+  const int lanes = band_lanes(g);
+  if (lanes) {
+    const int nm0 = op == 2 ? g.nxchopdim : 0;
+    const long npart = (nsys + 64 / lanes - 1) / (64 / lanes);
+    const size_t lds = std::max(part_lds_bytes(1), sizeof(double2) * (size_t)g.nrchopdim);
+    hipLaunchKernelGGL(k_band_apply_t_part, dim3((unsigned)(nm0 + npart), n), dim3(64), lds, c->stream, g, c->d.bxm,
+                       c->d.at1, op, jobs, nm0, lanes);
+    MLG_CHECK_HIP(hipGetLastError());
+    return 0;
+  }
   int spb = (int)((48 * 1024) / ((size_t)g.nrchopdim * sizeof(double2)));
   spb = spb < 1 ? 1 : (spb > 64 ? 64 : spb);
-  KScope ks(c, K_BAND_SOLVE, n * 2.0 * 16.0 * fff_count(g));
   hipLaunchKernelGGL(k_band_apply_t, dim3((unsigned)((nsys + spb - 1) / spb), n), dim3(64),
                      sizeof(double2) * (size_t)spb * g.nrchopdim, c->stream, g, c->d.bxm, c->d.at1, op, jobs, spb);
   MLG_CHECK_HIP(hipGetLastError());
