@@ -1265,10 +1265,6 @@ __device__ void band_apply_n(int nd, const double2 *const (&xin)[NJ], long sin, 
   }
 }
 
-// x[i] = x[i-1] for i = n-1 .. 1 (x in LDS)
-__device__ void shift_down_one(double2 *x, int n) {
-  for (int i = n - 1; i >= 1; i--) x[i] = x[i - 1];
-}
 
 // x <- XM x on nn rows (tri-band, ascending, from the original values)
 template <int SB>
@@ -1462,26 +1458,30 @@ __device__ __forceinline__ double2 sel2(bool c, double2 v) { return c ? v : zero
 // starts, not kept in registers from an earlier pass or hoisted ahead.
 __device__ __forceinline__ void lds_fence() { asm volatile("" ::: "memory"); }
 
-// One pass over a lane's PART_LS rows, in the direction the caller's row
-// order gives: state (h0, h1) = the last two values, c1, c2 the couplings.
-template <int NJ>
-__device__ __forceinline__ void part_row(double c1, double c2, const double2 (&r)[NJ], double2 (&h0)[NJ],
-                                         double2 (&h1)[NJ]) {
+// One row of a pass with C couplings: v = r - c_0 h_0 - ... - c_{C-1} h_{C-1},
+// h_t the value t + 1 rows back in the pass's direction, then the state
+// shifts.
+template <int NJ, int C>
+__device__ __forceinline__ void part_row(const double (&c)[C], const double2 (&r)[NJ], double2 (&h)[NJ][C]) {
 #pragma unroll
   for (int jj = 0; jj < NJ; jj++) {
     double2 v = r[jj];
-    v = sub2(v, axc(c1, h0[jj]));
-    v = sub2(v, axc(c2, h1[jj]));
-    h1[jj] = h0[jj];
-    h0[jj] = v;
+#pragma unroll
+    for (int t = 0; t < C; t++) v = sub2(v, axc(c[t], h[jj][t]));
+#pragma unroll
+    for (int t = C - 1; t >= 1; t--) h[jj][t] = h[jj][t - 1];
+    h[jj][0] = v;
   }
 }
-__device__ __forceinline__ void part_unit(double c1, double c2, double &g0, double &g1) {
+// The same row on a unit-state run (no right-hand side, real)
+template <int C>
+__device__ __forceinline__ void part_unit(const double (&c)[C], double (&g)[C]) {
   double t = 0.0;
-  t = t - c1 * g0;
-  t = t - c2 * g1;
-  g1 = g0;
-  g0 = t;
+#pragma unroll
+  for (int u = 0; u < C; u++) t = t - c[u] * g[u];
+#pragma unroll
+  for (int u = C - 1; u >= 1; u--) g[u] = g[u - 1];
+  g[0] = t;
 }
 
 // Values of the lane below (dir = 1) or above (dir = -1) in the wave, moved
@@ -1502,22 +1502,28 @@ __device__ __forceinline__ double2 dpp_next2(double2 v) {
 // The start states of the segments, passed along the P lanes of each system
 // in turn (lanes seg + 1 after seg forward, seg - 1 after seg back): every
 // lane forms its segment's end state E = p + G s from its current start
-// state, and at step t the lane of segment t (forward) or P - 1 - t (back)
-// takes its neighbour's.  Each carried value is a state of the recurrence,
-// as in the serial substitution.
-template <int NJ, int DIR>
-__device__ __forceinline__ void part_carry(int P, int seg, const double2 (&p0)[NJ], const double2 (&p1)[NJ],
-                                           const double (&G)[4], double2 (&s0)[NJ], double2 (&s1)[NJ]) {
+// state (G[u]: the end state of the run from unit state u), and at step t
+// the lane of segment t (forward) or P - 1 - t (back) takes its
+// neighbour's.  Each carried value is a state of the recurrence, as in the
+// serial substitution.
+template <int NJ, int C, int DIR>
+__device__ __forceinline__ void part_carry(int P, int seg, const double2 (&p)[NJ][C], const double (&G)[C][C],
+                                           double2 (&st)[NJ][C]) {
   for (int t = 1; t < P; t++) {
     const int take = DIR > 0 ? t : P - 1 - t;
 #pragma unroll
     for (int jj = 0; jj < NJ; jj++) {
-      const double2 e0 = add2(p0[jj], add2(axc(G[0], s0[jj]), axc(G[1], s1[jj])));
-      const double2 e1 = add2(p1[jj], add2(axc(G[2], s0[jj]), axc(G[3], s1[jj])));
-      const double2 r0 = dpp_next2<DIR>(e0), r1 = dpp_next2<DIR>(e1);
+      double2 r[C];
+#pragma unroll
+      for (int q = 0; q < C; q++) {
+        double2 acc = axc(G[0][q], st[jj][0]);
+#pragma unroll
+        for (int u = 1; u < C; u++) acc = add2(acc, axc(G[u][q], st[jj][u]));
+        r[q] = dpp_next2<DIR>(add2(p[jj][q], acc));
+      }
       if (seg == take) {
-        s0[jj] = r0;
-        s1[jj] = r1;
+#pragma unroll
+        for (int q = 0; q < C; q++) st[jj][q] = r[q];
       }
     }
   }
@@ -1525,25 +1531,32 @@ __device__ __forceinline__ void part_carry(int P, int seg, const double2 (&p0)[N
 
 // The passes of the partitioned substitution on rows staged in LDS (my:
 // this lane's slot of PART_LS + 1 rows; field jj at my + jj fstride), for a
-// lane owning rows a .. a + PART_LS - 1 of a system of nd rows.  fwd(n, c1,
-// c2, sc) gives the couplings of row n of the forward pass (to the values of
-// rows n - 1, n - 2) and the scale of its result, back(n, c1, c2) those of
-// the back substitution (to rows n + 1, n + 2); both give zero couplings
-// where the serial substitution has none.  The back pass's couplings are
-// loaded after the forward pass, when the forward ones are dead.
-template <int NJ, class Fwd, class Back>
+// lane owning rows a .. a + PART_LS - 1 of a system of nd rows, C couplings
+// per row.  fwd(n, c, sc) gives the couplings of row n of the forward pass
+// (c[t] to the value of row n - 1 - t) and the scale of its result,
+// back(n, c) those of the back substitution (c[t] to row n + 1 + t); both
+// give zero couplings where the serial substitution has none.  The back
+// pass's couplings are loaded after the forward pass, when the forward ones
+// are dead.
+template <int NJ, int C, class Fwd, class Back>
 __device__ __forceinline__ void part_passes(double2 *my, size_t fstride, int a, int nd, int P, int seg, Fwd fwd,
                                             Back back) {
   constexpr int LS = PART_LS, SB = 4;
-  double l1[LS], l2[LS], rp[LS];
+  double cf[LS][C], rp[LS];
 #pragma unroll
-  for (int q = 0; q < LS; q++) fwd(a + q, l1[q], l2[q], rp[q]);
-  double2 s0[NJ], s1[NJ], p0[NJ], p1[NJ];
+  for (int q = 0; q < LS; q++) fwd(a + q, cf[q], rp[q]);
+  double2 st[NJ][C], p[NJ][C];
   // ---- forward pass (rows ascending)
 #pragma unroll
-  for (int jj = 0; jj < NJ; jj++) s0[jj] = s1[jj] = p0[jj] = p1[jj] = zero2();
+  for (int jj = 0; jj < NJ; jj++)
+#pragma unroll
+    for (int t = 0; t < C; t++) st[jj][t] = p[jj][t] = zero2();
   {
-    double ga0 = 1.0, ga1 = 0.0, gb0 = 0.0, gb1 = 1.0;
+    double G[C][C];
+#pragma unroll
+    for (int u = 0; u < C; u++)
+#pragma unroll
+      for (int t = 0; t < C; t++) G[u][t] = u == t ? 1.0 : 0.0;
 #pragma unroll
     for (int qb = 0; qb < LS; qb += SB) {
       double2 r[SB][NJ];
@@ -1554,13 +1567,12 @@ __device__ __forceinline__ void part_passes(double2 *my, size_t fstride, int a, 
         for (int jj = 0; jj < NJ; jj++) r[q][jj] = sel2(a + qb + q < nd, my[jj * fstride + qb + q]);
 #pragma unroll
       for (int q = 0; q < SB; q++) {
-        part_unit(l1[qb + q], l2[qb + q], ga0, ga1);
-        part_unit(l1[qb + q], l2[qb + q], gb0, gb1);
-        part_row<NJ>(l1[qb + q], l2[qb + q], r[q], p0, p1);
+#pragma unroll
+        for (int u = 0; u < C; u++) part_unit<C>(cf[qb + q], G[u]);
+        part_row<NJ, C>(cf[qb + q], r[q], p);
       }
     }
-    const double G[4] = {ga0, gb0, ga1, gb1};
-    part_carry<NJ, 1>(P, seg, p0, p1, G, s0, s1);
+    part_carry<NJ, C, 1>(P, seg, p, G, st);
   }
 #pragma unroll
   for (int qb = 0; qb < LS; qb += SB) {
@@ -1572,19 +1584,25 @@ __device__ __forceinline__ void part_passes(double2 *my, size_t fstride, int a, 
       for (int jj = 0; jj < NJ; jj++) r[q][jj] = sel2(a + qb + q < nd, my[jj * fstride + qb + q]);
 #pragma unroll
     for (int q = 0; q < SB; q++) {
-      part_row<NJ>(l1[qb + q], l2[qb + q], r[q], s0, s1);
+      part_row<NJ, C>(cf[qb + q], r[q], st);
 #pragma unroll
-      for (int jj = 0; jj < NJ; jj++) my[jj * fstride + qb + q] = axc(rp[qb + q], s0[jj]);
+      for (int jj = 0; jj < NJ; jj++) my[jj * fstride + qb + q] = axc(rp[qb + q], st[jj][0]);
     }
   }
   // ---- back substitution (rows descending)
-  double u1[LS], u2[LS];
+  double cb[LS][C];
 #pragma unroll
-  for (int q = 0; q < LS; q++) back(a + q, u1[q], u2[q]);
+  for (int q = 0; q < LS; q++) back(a + q, cb[q]);
 #pragma unroll
-  for (int jj = 0; jj < NJ; jj++) s0[jj] = s1[jj] = p0[jj] = p1[jj] = zero2();
+  for (int jj = 0; jj < NJ; jj++)
+#pragma unroll
+    for (int t = 0; t < C; t++) st[jj][t] = p[jj][t] = zero2();
   {
-    double ga0 = 1.0, ga1 = 0.0, gb0 = 0.0, gb1 = 1.0;
+    double G[C][C];
+#pragma unroll
+    for (int u = 0; u < C; u++)
+#pragma unroll
+      for (int t = 0; t < C; t++) G[u][t] = u == t ? 1.0 : 0.0;
 #pragma unroll
     for (int qb = LS - SB; qb >= 0; qb -= SB) {
       double2 r[SB][NJ];
@@ -1595,13 +1613,12 @@ __device__ __forceinline__ void part_passes(double2 *my, size_t fstride, int a, 
         for (int jj = 0; jj < NJ; jj++) r[q][jj] = sel2(a + qb + q < nd, my[jj * fstride + qb + q]);
 #pragma unroll
       for (int q = SB - 1; q >= 0; q--) {
-        part_unit(u1[qb + q], u2[qb + q], ga0, ga1);
-        part_unit(u1[qb + q], u2[qb + q], gb0, gb1);
-        part_row<NJ>(u1[qb + q], u2[qb + q], r[q], p0, p1);
+#pragma unroll
+        for (int u = 0; u < C; u++) part_unit<C>(cb[qb + q], G[u]);
+        part_row<NJ, C>(cb[qb + q], r[q], p);
       }
     }
-    const double G[4] = {ga0, gb0, ga1, gb1};
-    part_carry<NJ, -1>(P, seg, p0, p1, G, s0, s1);
+    part_carry<NJ, C, -1>(P, seg, p, G, st);
   }
 #pragma unroll
   for (int qb = LS - SB; qb >= 0; qb -= SB) {
@@ -1613,9 +1630,9 @@ __device__ __forceinline__ void part_passes(double2 *my, size_t fstride, int a, 
       for (int jj = 0; jj < NJ; jj++) r[q][jj] = sel2(a + qb + q < nd, my[jj * fstride + qb + q]);
 #pragma unroll
     for (int q = SB - 1; q >= 0; q--) {
-      part_row<NJ>(u1[qb + q], u2[qb + q], r[q], s0, s1);
+      part_row<NJ, C>(cb[qb + q], r[q], st);
 #pragma unroll
-      for (int jj = 0; jj < NJ; jj++) my[jj * fstride + qb + q] = s0[jj];
+      for (int jj = 0; jj < NJ; jj++) my[jj * fstride + qb + q] = st[jj][0];
     }
   }
 }
@@ -1662,18 +1679,18 @@ __device__ void apply_penta_part(const KGrid &g, int op, int has_ln, const BandJ
   double2 *my = lds + (size_t)lane * SLOT;
   // LUBKSBB: row n's multipliers against rows n-1, n-2 and 1/pivot, then its
   // normalised upper entries against rows n+1, n+2
-  part_passes<NJ>(
+  part_passes<NJ, 2>(
       my, fstride, seg * LS, nd, P, seg,
-      [&](int n, double &c1, double &c2, double &sc) {
+      [&](int n, double(&c)[2], double &sc) {
         const double *ff = FV.ff + ((size_t)min(n, nlast) * ncls + cls) * 3;
-        c1 = (n >= 1 && n < nd) ? ff[0] : 0.0;
-        c2 = (n >= 2 && n < nd) ? ff[1] : 0.0;
+        c[0] = (n >= 1 && n < nd) ? ff[0] : 0.0;
+        c[1] = (n >= 2 && n < nd) ? ff[1] : 0.0;
         sc = ff[2];
       },
-      [&](int n, double &c1, double &c2) {
+      [&](int n, double(&c)[2]) {
         const double *fb = FV.fb + ((size_t)min(n, nlast) * ncls + cls) * 2;
-        c1 = n + 1 < nd ? fb[0] : 0.0;
-        c2 = n + 2 < nd ? fb[1] : 0.0;
+        c[0] = n + 1 < nd ? fb[0] : 0.0;
+        c[1] = n + 2 < nd ? fb[1] : 0.0;
       });
   __syncthreads();
   // LN of the m = 0, k = 0 results (row 0 of segment 0), then the columns out
@@ -1706,7 +1723,7 @@ __device__ void apply_penta_part(const KGrid &g, int op, int has_ln, const BandJ
 // right-hand side and XM rows in LDS with coalesced loads; lane 0 then runs
 // the substitutions from LDS (the same arithmetic as from memory) and the
 // block writes the column back.
-#define M0_SB 4
+#define M0_SB 8
 size_t m0_lds_bytes(const KGrid &g) { return sizeof(double) * 12 * (size_t)g.nrchopdim; }
 
 __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, int has_ln, double lnval,
@@ -1729,8 +1746,12 @@ __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, 
   double *Fs = sm + 2 * g.nrchopdim;             // [nrchopdim][FSTRIDE]
   double *xt = Fs + FSTRIDE * g.nrchopdim;       // [nrchopdim][3]
   const int nd = kind == 1 ? nn : nn - 1;        // factorised rows
+  // IDEL2 m0k0: the first row is the log-term (Dirichlet) row, the right-
+  // hand side moves down one row under it
+  if (kind == 1 && threadIdx.x == 0) x[0] = make_double2(has_ln ? lnval / g.norm0 : 0.0, 0.0);
   for (int row = threadIdx.x; row < nn; row += blockDim.x) {
-    x[row] = xb[row * bstride];
+    if (kind == 2) x[row] = xb[row * bstride];
+    else if (row + 1 < nn) x[row + 1] = xb[row * bstride];
     if (row < nd) {
       const double *fr = factor_view(F, g.nrchopdim, ncls).f7 + ((size_t)row * ncls + cls) * FSTRIDE;
 #pragma unroll
@@ -1744,9 +1765,6 @@ __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, 
   __syncthreads();
   if (threadIdx.x == 0) {
     if (kind == 1) {
-      // IDEL2 m0k0: first row is the log-term (Dirichlet) row
-      shift_down_one(x, nn);
-      x[0] = make_double2(has_ln ? lnval / g.norm0 : 0.0, 0.0);
       band_apply<7, 3, M0_SB>(nn, x, x, Fs, 1, 0);
       ln_a[0] = has_ln ? x[0].x * g.norm0 : 0.0;
     } else {
@@ -1795,10 +1813,96 @@ __global__ void __launch_bounds__(64) k_band_apply(KGrid g, const double *bxm, i
   }
 }
 
+// The m = 0 systems of apply_m0 on the partitioned passes, all 64 lanes of
+// the block on one system (lane l owns rows 8 l .. 8 l + 7; NRCHOP <= 512):
+// IHELM's (1-x)-factorised 7-band system on NN-1 rows followed by XM, with
+// its LN terms; three couplings per row.  IDEL2's Dirichlet log-row system
+// (k = 0) stays on apply_m0's serial substitution: carried states lose up to
+// four digits on it (its homogeneous solutions grow fast), against 1e-14 on
+// IHELM's.
+__device__ void apply_m0_part(const KGrid &g, const double *bxm, int op, double alp, int has_ln, double lnval,
+                              const double2 *b, double2 *a, double *ln_a, const double *ln_b, const double *__restrict__ F,
+                              int k, double2 *lds, int bil) {
+  if (solve_kind(op, 0, k) == 1) {
+    apply_m0(g, bxm, op, alp, has_ln, lnval, b, a, ln_a, ln_b, F, k, reinterpret_cast<double *>(lds), bil);
+    return;
+  }
+  constexpr int LS = PART_LS, SLOT = PART_LS + 1;
+  const long ncls = (long)g.ntchopdim * g.nxchop;
+  const int lane = threadIdx.x;
+  if (k >= g.nxchopdim) return;
+  // IHELM: A%LN starts as B%LN (COPY), replaced below by -B%LN/ALP
+  if (op == 2 && k == 0 && lane == 0) ln_a[0] = ln_b[0] * 1.0 + 0.0;
+  const int nn = s_nrchops(g.nrchop, 0);
+  const int kind = solve_kind(op, 0, k);
+  if (nn < 1 || g.ntchop < 1 || kind == 0) return;  // the whole block
+  const int kabs = k < g.nxchop ? k : g.nxchopdim - k;
+  const long cls = (long)g.ntchopdim * kabs;
+  const size_t bstride = bil ? (size_t)g.ntchopdim * g.nxchopdim : 1;
+  const double2 *xb = bil ? b + (size_t)g.ntchopdim * k : b + (size_t)g.nrchopdim * ((size_t)g.ntchopdim * k);
+  double2 *xg = a + (size_t)g.nrchopdim * ((size_t)g.ntchopdim * k);
+  auto slot = [&](int n) { return (n / LS) * SLOT + n % LS; };
+  const double lnA = k == 0 ? -ln_b[0] / alp : 0.0;
+  for (int n = lane; n < nn; n += 64) {
+    double2 v = xb[n * bstride];
+    if (k == 0) {
+      if (n == 0) v.x = v.x - lnA * (4.0 / 3 / g.ell2) / g.norm0;
+      if (n == 1) v.x = v.x - lnA * (-2 / g.norm1 / g.ell2);
+      if (n == 2) v.x = v.x - lnA * (+2.0 / 3 / g.norm2 / g.ell2);
+    }
+    lds[slot(n)] = v;
+  }
+  if (k == 0 && lane == 0) ln_a[0] = lnA;
+  __syncthreads();
+  const int nd = nn - 1;  // factorised rows
+  const double *f7 = factor_view(F, g.nrchopdim, ncls).f7;
+  const int nlast = max(nd - 1, 0);
+  auto rec = [&](int n) { return f7 + ((size_t)min(max(n, 0), nlast) * ncls + cls) * FSTRIDE; };
+  part_passes<1, 3>(
+      lds + (size_t)lane * SLOT, 0, lane * LS, nd, 64, lane,
+      [&](int n, double(&c)[3], double &sc) {
+        const double *fr = rec(n);
+#pragma unroll
+        for (int t = 0; t < 3; t++) c[t] = (n >= t + 1 && n < nd) ? fr[t] : 0.0;
+        sc = fr[6];
+      },
+      [&](int n, double(&c)[3]) {
+        const double *fr = rec(n);
+#pragma unroll
+        for (int t = 0; t < 3; t++) c[t] = n + t + 1 < nd ? fr[3 + t] : 0.0;
+      });
+  __syncthreads();
+  // the (1-x) y solve ends with row NN-1 zero, then A = XM y on NN
+  // rows (tri-band, from the solved values, ascending sums as mulxm_column)
+  double2 o[LS];
+#pragma unroll
+  for (int q = 0; q < LS; q++) {
+    const int c = lane * LS + q;
+    o[q] = zero2();
+    if (c < nn) {
+      const double2 cur = c < nn - 1 ? lds[slot(c)] : zero2();
+      double2 t = axc(bxm[(size_t)c * 3 + 1], cur);
+      if (c >= 1) t = add2(t, axc(bxm[(size_t)c * 3 + 0], lds[slot(c - 1)]));
+      if (c + 1 < nn) t = add2(t, axc(bxm[(size_t)c * 3 + 2], c + 1 < nn - 1 ? lds[slot(c + 1)] : zero2()));
+      o[q] = t;
+    }
+  }
+#pragma unroll
+  for (int q = 0; q < LS; q++) {
+    const int c = lane * LS + q;
+    if (c < nn) xg[c] = o[q];
+  }
+}
+
 __global__ void __launch_bounds__(64, 3) k_band_apply_part(KGrid g, const double *bxm, int op, int has_ln, double lnval,
                                                            BandJobs jobs, int nm0, int npenta, int P) {
   extern __shared__ __align__(16) double m0s[];
-  if (band_m0_block(g, bxm, op, has_ln, lnval, jobs, nm0, m0s)) return;
+  if ((int)blockIdx.x < nm0 * jobs.n) {
+    const BandJob &J = jobs.j[blockIdx.x / nm0];
+    apply_m0_part(g, bxm, op, J.alp, has_ln, lnval, J.b, J.a, J.ln_a, J.ln_b, J.F, blockIdx.x % nm0,
+                  reinterpret_cast<double2 *>(m0s), jobs.bil);
+    return;
+  }
   const int pb = blockIdx.x - nm0 * jobs.n, grp = pb / npenta;
   const long sys0 = (long)(pb - grp * npenta) * (64 / P);
   double2 *lds = reinterpret_cast<double2 *>(m0s);
@@ -1891,7 +1995,8 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, doub
   if (lanes) {
     int njmax = 1;
     for (int i = 0; i < jobs.ng; i++) njmax = std::max(njmax, jobs.gcount[i]);
-    lds = std::max(lds, part_lds_bytes(njmax));
+    // the partitioned m = 0 blocks use one field's; IDEL2's serial one its own
+    lds = std::max(part_lds_bytes(njmax), op == 0 ? m0_lds_bytes(g) : (size_t)0);
     const int npenta = (int)((nsys + 64 / lanes - 1) / (64 / lanes));
     hipLaunchKernelGGL(k_band_apply_part, dim3(nm0 * n + npenta * jobs.ng), dim3(64), lds, c->stream, g, c->d.bxm, op,
                        has_ln, ln, jobs, nm0, npenta, lanes);
@@ -2142,16 +2247,16 @@ __global__ void __launch_bounds__(64, 3) k_band_apply_t_part(KGrid g, const doub
     const FactorView FV = factor_view(J.F, R, ncls);
     const int nlast = max(nd - 1, 0);
     auto row = [&](int n) { return (size_t)min(max(n, 0), nlast) * ncls + cls; };
-    part_passes<1>(
+    part_passes<1, 2>(
         xs + (size_t)lane * SLOT, fstride, seg * LS, nd, P, seg,
-        [&](int n, double &c1, double &c2, double &sc) {
-          c1 = (n >= 1 && n < nd) ? FV.fb[row(n - 1) * 2] : 0.0;
-          c2 = (n >= 2 && n < nd) ? FV.fb[row(n - 2) * 2 + 1] : 0.0;
+        [&](int n, double(&c)[2], double &sc) {
+          c[0] = (n >= 1 && n < nd) ? FV.fb[row(n - 1) * 2] : 0.0;
+          c[1] = (n >= 2 && n < nd) ? FV.fb[row(n - 2) * 2 + 1] : 0.0;
           sc = FV.ff[row(n) * 3 + 2];
         },
-        [&](int n, double &c1, double &c2) {
-          c1 = n + 1 < nd ? FV.ff[row(n + 1) * 3] : 0.0;
-          c2 = n + 2 < nd ? FV.ff[row(n + 2) * 3 + 1] : 0.0;
+        [&](int n, double(&c)[2]) {
+          c[0] = n + 1 < nd ? FV.ff[row(n + 1) * 3] : 0.0;
+          c[1] = n + 2 < nd ? FV.ff[row(n + 2) * 3 + 1] : 0.0;
         });
   }
   __syncthreads();
