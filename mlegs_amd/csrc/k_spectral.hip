@@ -2208,15 +2208,59 @@ __global__ void __launch_bounds__(64, 3) k_band_apply_t_part(KGrid g, const doub
   const long ncls = (long)g.ntchopdim * g.nxchop;
   const int R = g.nrchopdim, lane = threadIdx.x;
   if ((int)blockIdx.x < nm0) {
+    // IHELM's m = 0 system of slot k on all 64 lanes (lane l owns rows
+    // 8 l .. 8 l + 7): XM^T, then (U^T D L^T) on NN-1 rows with three
+    // couplings per row, row NN-1 zero, then the LN term's adjoint
     const int k = blockIdx.x;
     const size_t base = (size_t)R * ((size_t)g.ntchopdim * k);
     const int nn = g.ntchop > 0 ? s_nrchops(g.nrchop, 0) : 0;
-    for (int i = lane; i < R; i += 64) xs[i] = i < nn ? J.in[base + i] : zero2();
-    __syncthreads();
-    if (lane == 0 && nn >= 1)
-      solve_t_system(g, bxm, at1, op, J.alp, factor_view(J.F, R, ncls), ncls, 0, k, nn, xs, J.ln_in, J.ln_out);
-    __syncthreads();
-    for (int i = lane; i < R; i += 64) J.out[base + i] = xs[i];
+    auto slot = [&](int n) { return (n / LS) * SLOT + n % LS; };
+    if (nn >= 1) {
+      for (int n = lane; n < nn; n += 64) xs[slot(n)] = J.in[base + n];
+      __syncthreads();
+      // x <- XM^T x on NN rows, from the original values (solve_t_system's sums)
+      double2 o[LS];
+#pragma unroll
+      for (int q = 0; q < LS; q++) {
+        const int c = lane * LS + q;
+        o[q] = zero2();
+        if (c < nn) {
+          double2 t = axc(bxm[(size_t)c * 3 + 1], xs[slot(c)]);
+          if (c + 1 < nn) t = add2(t, axc(bxm[(size_t)(c + 1) * 3 + 0], xs[slot(c + 1)]));
+          if (c >= 1) t = add2(t, axc(bxm[(size_t)(c - 1) * 3 + 2], xs[slot(c - 1)]));
+          o[q] = t;
+        }
+      }
+      __syncthreads();
+#pragma unroll
+      for (int q = 0; q < LS; q++)
+        if (lane * LS + q < nn) xs[slot(lane * LS + q)] = o[q];
+      __syncthreads();
+      const int nd = nn - 1;
+      const double *f7 = factor_view(J.F, R, ncls).f7;
+      const long cls = (long)g.ntchopdim * (k < g.nxchop ? k : g.nxchopdim - k);
+      const int nlast = max(nd - 1, 0);
+      auto rec = [&](int n) { return f7 + ((size_t)min(max(n, 0), nlast) * ncls + cls) * FSTRIDE; };
+      part_passes<1, 3>(
+          xs + (size_t)lane * SLOT, 0, lane * LS, nd, 64, lane,
+          [&](int n, double(&c)[3], double &sc) {
+#pragma unroll
+            for (int t = 0; t < 3; t++) c[t] = (n >= t + 1 && n < nd) ? rec(n - 1 - t)[3 + t] : 0.0;
+            sc = rec(n)[6];
+          },
+          [&](int n, double(&c)[3]) {
+#pragma unroll
+            for (int t = 0; t < 3; t++) c[t] = n + t + 1 < nd ? rec(n + 1 + t)[t] : 0.0;
+          });
+      __syncthreads();
+      if (lane == 0 && k == 0) {
+        const double2 c0 = xs[slot(0)], c1 = nn > 1 ? xs[slot(1)] : zero2(), c2 = nn > 2 ? xs[slot(2)] : zero2();
+        const double lbar = J.ln_in[0] - (c0.x * ((4.0 / 3 / g.ell2) / g.norm0) + c1.x * (-2 / g.norm1 / g.ell2) +
+                                          c2.x * (+2.0 / 3 / g.norm2 / g.ell2));
+        J.ln_out[0] = -lbar / J.alp;
+      }
+    }
+    for (int n = lane; n < R; n += 64) J.out[base + n] = n < nn - 1 ? xs[slot(n)] : zero2();
     return;
   }
   const int W = 64 / P;
