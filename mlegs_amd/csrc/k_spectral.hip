@@ -9,6 +9,7 @@
 // Composite operators (PC2VEL, PC2VOR, GRAD) are fused into one pass that
 // recomputes the intermediate bands per output row.
 #include "mlegs_internal.h"
+#include "xcd.h"
 
 namespace mlg {
 
@@ -1903,7 +1904,11 @@ __global__ void __launch_bounds__(64, 3) k_band_apply_part(KGrid g, const double
                   reinterpret_cast<double2 *>(m0s), jobs.bil);
     return;
   }
-  const int pb = blockIdx.x - nm0 * jobs.n, grp = pb / npenta;
+  // the penta-diagonal blocks from the next multiple of 8 on, XCD-aware: the
+  // blocks of one XCD take consecutive systems and share factor-row lines
+  const int pb = xcd_logical(npenta * jobs.ng, xcd_grid(nm0 * jobs.n));
+  if (pb < 0) return;
+  const int grp = pb / npenta;
   const long sys0 = (long)(pb - grp * npenta) * (64 / P);
   double2 *lds = reinterpret_cast<double2 *>(m0s);
   switch (jobs.gcount[grp]) {
@@ -1998,8 +2003,8 @@ int launch_band_solve_n(mlg_ctx *c, int op, int n, const double2 *const *b, doub
     // the partitioned m = 0 blocks use one field's; IDEL2's serial one its own
     lds = std::max(part_lds_bytes(njmax), op == 0 ? m0_lds_bytes(g) : (size_t)0);
     const int npenta = (int)((nsys + 64 / lanes - 1) / (64 / lanes));
-    hipLaunchKernelGGL(k_band_apply_part, dim3(nm0 * n + npenta * jobs.ng), dim3(64), lds, c->stream, g, c->d.bxm, op,
-                       has_ln, ln, jobs, nm0, npenta, lanes);
+    hipLaunchKernelGGL(k_band_apply_part, dim3(xcd_grid(nm0 * n) + xcd_grid(npenta * jobs.ng)), dim3(64), lds, c->stream,
+                       g, c->d.bxm, op, has_ln, ln, jobs, nm0, npenta, lanes);
   } else {
     const int npenta = (int)((nsys + 63) / 64);
     hipLaunchKernelGGL(k_band_apply, dim3(nm0 * n + npenta * jobs.ng), dim3(64), lds, c->stream, g, c->d.bxm, op, has_ln,
@@ -2200,7 +2205,7 @@ __global__ void __launch_bounds__(64) k_band_apply_t(KGrid g, const double *bxm,
 // blocks [0, nm0), one per k, solved by one thread from LDS as in
 // k_band_apply_t.  in may equal out.
 __global__ void __launch_bounds__(64, 3) k_band_apply_t_part(KGrid g, const double *bxm, const double *at1, int op,
-                                                             BandTJobs jobs, int nm0, int P) {
+                                                             BandTJobs jobs, int nm0, int npart, int P) {
   extern __shared__ __align__(16) double2 xs[];
   constexpr int LS = PART_LS, SLOT = PART_LS + 1;
   const BandTJob &J = jobs.j[blockIdx.y];
@@ -2264,7 +2269,10 @@ __global__ void __launch_bounds__(64, 3) k_band_apply_t_part(KGrid g, const doub
     return;
   }
   const int W = 64 / P;
-  const long sys0 = (long)(blockIdx.x - nm0) * W;
+  // XCD-aware from the next multiple of 8 on (as k_band_apply_part)
+  const int pb = xcd_logical(npart, xcd_grid(nm0));
+  if (pb < 0) return;
+  const long sys0 = (long)pb * W;
   const size_t fstride = (size_t)64 * SLOT;
   for (int jp = 0; jp < W; jp++) {
     int m2, k2;
@@ -2336,8 +2344,8 @@ int launch_band_solve_t_n(mlg_ctx *c, int op, int n, const BandTJob *in) {
     const int nm0 = op == 2 ? g.nxchopdim : 0;
     const long npart = (nsys + 64 / lanes - 1) / (64 / lanes);
     const size_t lds = std::max(part_lds_bytes(1), sizeof(double2) * (size_t)g.nrchopdim);
-    hipLaunchKernelGGL(k_band_apply_t_part, dim3((unsigned)(nm0 + npart), n), dim3(64), lds, c->stream, g, c->d.bxm,
-                       c->d.at1, op, jobs, nm0, lanes);
+    hipLaunchKernelGGL(k_band_apply_t_part, dim3((unsigned)(xcd_grid(nm0) + xcd_grid((int)npart)), n), dim3(64), lds,
+                       c->stream, g, c->d.bxm, c->d.at1, op, jobs, nm0, (int)npart, lanes);
     MLG_CHECK_HIP(hipGetLastError());
     return 0;
   }

@@ -11,12 +11,15 @@ namespace mlg {
 
 // Logical block id of this block in a grid of nblk logical blocks launched
 // as xcd_grid(nblk) blocks; -1 for a padding block.
-__device__ __forceinline__ int xcd_logical(int nblk) {
+// first: the linear id where the remapped range starts (a multiple of 8;
+// the blocks before it keep their own order and get -1 here).
+__device__ __forceinline__ int xcd_logical(int nblk, int first = 0) {
   const int nblkp = (nblk + 7) & ~7;
-  const int bid = blockIdx.x;
+  const int bid = (int)blockIdx.x - first;
+  if (bid < 0) return -1;
   const int logical = (bid & 7) * (nblkp >> 3) + (bid >> 3);
   return logical < nblk ? logical : -1;
 }
-static inline int xcd_grid(int nblk) { return (nblk + 7) & ~7; }
+__host__ __device__ inline int xcd_grid(int nblk) { return (nblk + 7) & ~7; }
 
 }  // namespace mlg

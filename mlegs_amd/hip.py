@@ -186,6 +186,15 @@ def _lib():
     if _LIB is None:
         if not os.path.exists(LIB_PATH):
             raise RuntimeError(f"libmlegs_hip.so not built ({LIB_PATH}); run `make -C mlegs_amd`")
+        # PyTorch ships its own HIP runtime, and a process keeps one copy per
+        # soname: with torch's loaded first the library runs on it, and
+        # torch's device work (tensors over the library's memory, its streams)
+        # stays usable after the library is loaded.  Loaded the other way
+        # round, torch finds no HIP device.
+        try:
+            import torch  # noqa: F401
+        except ImportError:
+            pass
         L = C.CDLL(LIB_PATH)
         for name, (args, res) in SIGNATURES.items():
             fn = getattr(L, name)
