@@ -1724,7 +1724,7 @@ __device__ void apply_penta_part(const KGrid &g, int op, int has_ln, const BandJ
 // right-hand side and XM rows in LDS with coalesced loads; lane 0 then runs
 // the substitutions from LDS (the same arithmetic as from memory) and the
 // block writes the column back.
-#define M0_SB 8
+#define M0_SB 4
 size_t m0_lds_bytes(const KGrid &g) { return sizeof(double) * 12 * (size_t)g.nrchopdim; }
 
 __device__ void apply_m0(const KGrid &g, const double *bxm, int op, double alp, int has_ln, double lnval,
