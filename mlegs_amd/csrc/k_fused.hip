@@ -621,6 +621,268 @@ k_fused_products_256(KGrid g, const double *__restrict__ rsq, const double2 *tw,
   forward(x, g, A, out(3), lo, inr, live);
 }
 
+// ---------------------------------------------------------------------------
+// NTH = 128: register-resident variant.
+//
+// 16 lanes own one theta line (four lines per wave, sixteen adjacent radial
+// nodes per block).  Each lane keeps 8 of the line's 128 points of every
+// field in registers.  The 128-point transform is 8 x 4 x 4:
+//   j = 16 a + b (lane b holds a = 0..7)         -> 8-point DFT over a, twiddle
+//   exchange: lane 4 c2 + f (b = 4 e + f) holds e = 0..3 of c = c2, c2 + 4
+//                                                 -> 4-point DFTs over e, twiddle
+//   exchange: lane 4 c2 + g holds f = 0..3 of c = c2, c2 + 4
+//                                                 -> 4-point DFTs over f
+// so lane 4 c2 + g ends with the points k = c + 8 g + 32 h, c = c2, c2 + 4,
+// h = 0..3.  The forward transform runs the stages backwards with
+// conjugate roots.  Same products, waves and LDS discipline as NTH = 256.
+// ---------------------------------------------------------------------------
+namespace w128 {
+
+constexpr int N = 128;
+constexpr int LP = 160;  // LDS double2 per line (the two exchange layouts, padded against bank conflicts)
+constexpr int MMAX48 = 48;  // NTCHOPDIM bound: slots a = 3, 4 hold no mode
+__device__ __forceinline__ int e1(int c, int b) { return c * 20 + b; }
+__device__ __forceinline__ int e2(int c, int f, int g) { return c * 20 + f * 5 + g; }
+using w256::cadd;
+using w256::cj;
+using w256::csub;
+using w256::dft4s;
+using w256::dft8s;
+using w256::dft8s_z34;
+using w256::wave_sync;
+using w256::Ctx;  // t1[16 c + b] = W_N^{b c}, t2[4 g + f] = W_N^{8 f g}, th[k] = exp(-i pi k / N)
+
+// HORFFT backward of one PFP line (element m at src[lo + ndimr m]); lane b
+// ends with the physical points P[c + 8 g + 32 h] in v[4 s + h], c = c2 + 4 s,
+// c2 = b >> 2, g = b & 3.
+__device__ __forceinline__ void inverse(const Ctx &x, const KGrid &g, const double2 *__restrict__ src, int lo,
+                                        bool live, double2 (&v)[8]) {
+  const int b = x.b, mmax = g.ntchopdim;
+  auto X = [&](int m, bool use) -> double2 {
+    use = use && live && m < g.ntchop;
+    double2 t = src[lo + (use ? g.ndimr * m : 0)];
+    if (!use) t = make_double2(0.0, 0.0);
+    if (m == 0) t.y = 0.0;
+    return t;
+  };
+  // Hermitian merge with one term per point, as at NTH = 256
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    if (a == 3 || a == 4) {
+      v[a] = make_double2(0.0, 0.0);
+      continue;
+    }
+    const int j = 16 * a + b, jc = N - j;
+    const double2 t = x.th[a < 3 ? j : jc];
+    const double2 xv = a < 3 ? X(j, j < mmax) : fz_conj(X(jc, jc < mmax));
+    const double2 q = a < 3 ? make_double2(1.0 + t.y, t.x) : make_double2(1.0 - t.y, t.x);
+    v[a] = fz_cmul(xv, q);
+  }
+  dft8s_z34<1>(v);
+#pragma unroll
+  for (int c = 1; c < 8; c++) v[c] = fz_cmul(v[c], cj(x.t1[16 * c + b], true));
+  // exchange 1: lane b holds y_b[c]; lane 4 c2 + f takes y_{4 e + f}[c], c = c2, c2 + 4
+#pragma unroll
+  for (int c = 0; c < 8; c++) x.s[e1(c, b)] = v[c];
+  wave_sync();
+  const int c2 = b >> 2, f0 = b & 3;
+  double2 t[2][4];
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++)
+#pragma unroll
+    for (int e = 0; e < 4; e++) t[sg][e] = x.s[e1(c2 + 4 * sg, 4 * e + f0)];
+  wave_sync();
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++) {
+    dft4s<1>(t[sg]);
+#pragma unroll
+    for (int gg = 1; gg < 4; gg++) t[sg][gg] = fz_cmul(t[sg][gg], cj(x.t2[4 * gg + f0], true));
+  }
+  // exchange 2: lane 4 c2 + f holds z_{c,f}[g]; lane 4 c2 + g takes f = 0..3
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++)
+#pragma unroll
+    for (int gg = 0; gg < 4; gg++) x.s[e2(c2 + 4 * sg, f0, gg)] = t[sg][gg];
+  wave_sync();
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++)
+#pragma unroll
+    for (int f = 0; f < 4; f++) t[sg][f] = x.s[e2(c2 + 4 * sg, f, f0)];
+  wave_sync();
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++) {
+    dft4s<1>(t[sg]);
+#pragma unroll
+    for (int h = 0; h < 4; h++) v[4 * sg + h] = t[sg][h];
+  }
+}
+
+// HORFFT forward of this lane's physical points v (positions as left by
+// `inverse`) into one PFP line (element m at dst[lo + ndimr m]).
+__device__ __forceinline__ void forward(const Ctx &x, const KGrid &g, const double2 (&v)[8], double2 *__restrict__ dst,
+                                        int lo, bool inr, bool live) {
+  const int b = x.b, c2 = b >> 2, g0 = b & 3;
+  double2 t[2][4];
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++) {
+#pragma unroll
+    for (int h = 0; h < 4; h++) t[sg][h] = v[4 * sg + h];
+    dft4s<-1>(t[sg]);
+#pragma unroll
+    for (int f = 1; f < 4; f++) t[sg][f] = fz_cmul(t[sg][f], x.t2[4 * g0 + f]);
+  }
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++)
+#pragma unroll
+    for (int f = 0; f < 4; f++) x.s[e2(c2 + 4 * sg, f, g0)] = t[sg][f];
+  wave_sync();
+  // lane 4 c2 + f takes g = 0..3
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++)
+#pragma unroll
+    for (int gg = 0; gg < 4; gg++) t[sg][gg] = x.s[e2(c2 + 4 * sg, g0, gg)];
+  wave_sync();
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++) {
+    dft4s<-1>(t[sg]);
+#pragma unroll
+    for (int e = 0; e < 4; e++) x.s[e1(c2 + 4 * sg, 4 * e + g0)] = t[sg][e];
+  }
+  wave_sync();
+  double2 u[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) u[c] = x.s[e1(c, b)];
+  wave_sync();
+#pragma unroll
+  for (int c = 1; c < 8; c++) u[c] = fz_cmul(u[c], x.t1[16 * c + b]);
+  dft8s<-1>(u);
+  // u[a] = Y[16 a + b]; Y[N - m] sits in lane (16 - b) mod 16 at a' = 7 - a
+  // (lane 0: a' = (8 - a) mod 8)
+  const double den = (double)(2 * N);
+  const int src = (16 - b) & 15;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (16 * a >= g.ntchopdim) break;
+    const int m = 16 * a + b;
+    double2 pc;
+    pc.x = __shfl(u[7 - a].x, src, 16);
+    pc.y = __shfl(u[7 - a].y, src, 16);
+    if (b == 0) pc = u[(8 - a) & 7];
+    if (m < g.ntchopdim && inr) {
+      double2 zm = u[a], zc = fz_conj(pc);
+      double2 e = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+      double2 d = make_double2(zm.x - zc.x, zm.y - zc.y);
+      double2 wd = fz_cmul(x.th[m], d);
+      double2 o = make_double2(e.x + 0.5 * wd.y, e.y - 0.5 * wd.x);
+      o = make_double2(o.x / den, o.y / den);
+      if (m == 0) o.y = 0.0;
+      if (m >= g.ntchop || !live) o = make_double2(0.0, 0.0);
+      dst[lo + g.ndimr * m] = o;
+    }
+  }
+}
+
+}  // namespace w128
+
+// The NTH = 256 kernel's products and field order on the NTH = 128 lines.
+template <bool LINEAR>
+__global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
+k_fused_products_128(KGrid g, const double *__restrict__ rsq, const double2 *tw, const double2 *twh, FusedIO io,
+                     double two_omega, double n2) {
+  if (blockIdx.x == 0 && threadIdx.x == 0 && io.zl) io.zl[0] = 0.0;
+  using namespace w128;
+  __shared__ double2 S[16 * LP];
+  __shared__ double2 Y[16 * N];  // [line][point i][lane]
+  __shared__ double2 T1[128], T2[16], TH[N + 1];
+  const int nrb = (g.ndimr + 15) / 16;
+  const int lid = xcd_logical(nrb * g.ndimx);
+  if (lid < 0) return;
+  {
+    const int q = threadIdx.x;
+    if (q < 128) T1[q] = tw[(q & 15) * (q >> 4)];
+    if (q < 16) T2[q] = tw[8 * (q & 3) * (q >> 2)];
+    if (q <= N) TH[q] = twh[q];
+  }
+  __syncthreads();
+  const int z = lid / nrb, line = threadIdx.x >> 4;
+  const int rr = (lid % nrb) * 16 + line;
+  const bool inr = rr < g.ndimr, live = rr < g.nr;
+  Ctx x;
+  x.t1 = T1;
+  x.t2 = T2;
+  x.th = TH;
+  x.s = S + line * LP;
+  x.b = threadIdx.x & 15;
+  double2 *ys = Y + line * N + x.b;
+  const size_t off = (size_t)(rr - line) + (size_t)g.ndimr * g.ndimth * z;  // same for the block
+  const int lo = inr ? line : 0;
+  auto in = [&](int q) { return io.in[q] + off; };
+  auto out = [&](int q) { return io.out[q] + off; };
+  const double r2 = live ? rsq[rr] : 1.0;
+
+  double2 A[8], B[8], C[8], U[8], V[8], T[8];
+  w128::inverse(x, g, in(0), lo, live, A);  // rur
+  w128::inverse(x, g, in(1), lo, live, B);  // rup
+  w128::inverse(x, g, in(2), lo, live, C);  // uz
+  // oz: U = rup oz, V = rur oz
+  w128::inverse(x, g, in(5), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    U[i] = make_double2(B[i].x * T[i].x, B[i].y * T[i].y);
+    V[i] = make_double2(A[i].x * T[i].x, A[i].y * T[i].y);
+  }
+  // ror: Y = uz ror - rur oz (to LDS); V = rup ror
+  w128::inverse(x, g, in(3), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    ys[16 * i] = make_double2(C[i].x * T[i].x - V[i].x, C[i].y * T[i].y - V[i].y);
+    V[i] = make_double2(B[i].x * T[i].x, B[i].y * T[i].y);
+  }
+  // rop: X = rup oz - uz rop in U; Z = (rur rop - rup ror) / r^2 in V
+  w128::inverse(x, g, in(4), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    U[i] = make_double2(U[i].x - C[i].x * T[i].x, U[i].y - C[i].y * T[i].y);
+    V[i] = make_double2((A[i].x * T[i].x - V[i].x) / r2, (A[i].y * T[i].y - V[i].y) / r2);
+  }
+  // ror, rop, oz are read: the outputs may now overwrite them
+  if (LINEAR) {
+#pragma unroll
+    for (int i = 0; i < 8; i++) U[i] = make_double2(U[i].x + two_omega * B[i].x, U[i].y + two_omega * B[i].y);
+  }
+  w128::forward(x, g, U, out(0), lo, inr, live);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    T[i] = ys[16 * i];
+    if (LINEAR) T[i] = make_double2(T[i].x - two_omega * A[i].x, T[i].y - two_omega * A[i].y);
+  }
+  w128::forward(x, g, T, out(1), lo, inr, live);
+  if (LINEAR) {
+    // W: N_z = Z - W
+    w128::inverse(x, g, in(9), lo, live, T);
+#pragma unroll
+    for (int i = 0; i < 8; i++) V[i] = make_double2(V[i].x - T[i].x, V[i].y - T[i].y);
+  }
+  w128::forward(x, g, V, out(2), lo, inr, live);
+  // rbr, rbp: A = rur rbr + rup rbp
+  w128::inverse(x, g, in(6), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) A[i] = make_double2(A[i].x * T[i].x, A[i].y * T[i].y);
+  w128::inverse(x, g, in(7), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) A[i] = make_double2(A[i].x + B[i].x * T[i].x, A[i].y + B[i].y * T[i].y);
+  // bz: N_b = -(T / r^2 + uz bz) (+ Nbar^2 uz)
+  w128::inverse(x, g, in(8), lo, live, T);
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const double vr = A[i].x / r2 + C[i].x * T[i].x, vi = A[i].y / r2 + C[i].y * T[i].y;
+    double2 nb = make_double2(-vr, -vi);
+    if (LINEAR) nb = make_double2(nb.x + n2 * C[i].x, nb.y + n2 * C[i].y);
+    A[i] = nb;
+  }
+  w128::forward(x, g, A, out(3), lo, inr, live);
+}
+
 bool fused_products_supported(const mlg_ctx *c) {
   const int n = c->g.nth;
   return n >= 2 && n <= 256 && (n & (n - 1)) == 0;
@@ -635,6 +897,17 @@ int launch_fused_products(mlg_ctx *c, const double2 *const in[10], double2 *cons
   io.zl = zl;
   // PFP traffic: 9 (10) live input fields read, 4 written
   KScope ks(c, K_PRODUCTS, 16.0 * g.nr * g.ntchopdim * (double)g.nx * ((linear ? 10 : 9) + 4));
+  if (g.nth == 128 && g.ntchopdim <= w128::MMAX48) {
+    dim3 grid(xcd_grid((g.ndimr + 15) / 16 * g.ndimx));
+    if (linear)
+      hipLaunchKernelGGL(k_fused_products_128<true>, grid, dim3(256), 0, c->stream, g, c->d.r, c->d.tw_t, c->d.tw_h, io,
+                         two_omega, n2);
+    else
+      hipLaunchKernelGGL(k_fused_products_128<false>, grid, dim3(256), 0, c->stream, g, c->d.r, c->d.tw_t, c->d.tw_h,
+                         io, two_omega, n2);
+    MLG_CHECK_HIP(hipGetLastError());
+    return 0;
+  }
   if (g.nth == 256 && g.ntchopdim <= w256::MMAX96) {
     dim3 grid(xcd_grid((g.ndimr + 7) / 8 * g.ndimx));
     if (linear)

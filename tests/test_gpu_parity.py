@@ -26,6 +26,10 @@ SMALL_ETD = dict(SMALL, adams=0, hyper_cn=1, nup=1e-3, kappap=1e-3, visc_p=4, om
 # NTH = 256 with the production theta truncation: the theta transforms and the
 # fused products take their NTH = 256 kernels here
 THETA256 = dict(CFG1, nr=16, nth=256, nx=8, nrchop=12, ntchop=86, nxchop=3)
+# NTH = 128 with the sweep's theta truncation (NTCHOP = NTH/3 + 1): the
+# register-resident NTH = 128 products kernel; NR = 20 leaves a partial
+# block of radial lines
+THETA128 = dict(CFG1, nr=20, nth=128, nx=8, nrchop=14, ntchop=43, nxchop=3)
 # NTH and NX above 256: the general radix-2 transforms, the separate product
 # kernels and the non-fused NONLIN sequence
 WIDE_THETA = dict(CFG1, nr=16, nth=512, nx=8, nrchop=12, ntchop=100, nxchop=3)
@@ -100,9 +104,10 @@ class Pair:
         return np.asfortranarray(z)
 
 
-@pytest.fixture(scope="module", params=["small", "cfg1", "theta256", "wide_theta", "wide_z", "theta204", "z608"])
+@pytest.fixture(scope="module", params=["small", "cfg1", "theta256", "theta128", "wide_theta", "wide_z", "theta204",
+                                        "z608"])
 def P(request):
-    p = Pair({"small": SMALL, "cfg1": CFG1, "theta256": THETA256, "wide_theta": WIDE_THETA,
+    p = Pair({"small": SMALL, "cfg1": CFG1, "theta256": THETA256, "theta128": THETA128, "wide_theta": WIDE_THETA,
               "wide_z": WIDE_Z, "theta204": THETA204, "z608": Z608}[request.param])
     yield p
     p.close()
@@ -470,6 +475,14 @@ def test_steps_wide_transforms(cfg):
     24 (not powers of two): the direct DFT kernels, with the fused theta
     products where NTH is a power of two."""
     run_steps(cfg, 2)
+
+
+@pytest.mark.parametrize("cfg", [THETA128, dict(THETA128, adams=0, hyper_cn=1, nup=1e-3, kappap=1e-3, visc_p=4)],
+                         ids=["ab2cn", "etd2"])
+def test_steps_theta128(cfg):
+    """NTH = 128 steps: the register-resident NTH = 128 products kernel inside
+    FORCE and NONLIN."""
+    run_steps(cfg, 3)
 
 
 @pytest.mark.parametrize("p", [4, 6])
