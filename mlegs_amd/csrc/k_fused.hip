@@ -784,35 +784,167 @@ __device__ __forceinline__ void forward(const Ctx &x, const KGrid &g, const doub
 
 }  // namespace w128
 
-// The NTH = 256 kernel's products and field order on the NTH = 128 lines.
-template <bool LINEAR>
+// ---------------------------------------------------------------------------
+// NTH = 64: 8 lanes own one theta line (eight lines per wave), each keeping
+// 8 of its 64 points.  The 64-point transform is 8 x 8 with one exchange:
+//   j = 8 a + b (lane b holds a = 0..7) -> 8-point DFT over a, twiddle
+//   exchange: lane c holds b = 0..7     -> 8-point DFT over b
+// so lane c ends with the points k = c + 8 d, d = 0..7.
+// ---------------------------------------------------------------------------
+namespace w64 {
+
+constexpr int N = 64;
+constexpr int LP = 72;      // LDS double2 per line (rows of 9 against bank conflicts)
+constexpr int MMAX24 = 24;  // NTCHOPDIM bound: slots a = 3, 4 hold no mode
+__device__ __forceinline__ int e1(int c, int b) { return c * 9 + b; }
+using w256::cj;
+using w256::Ctx;  // t1[8 c + b] = W_N^{b c}, th[k] = exp(-i pi k / N)
+using w256::dft8s;
+using w256::dft8s_z34;
+using w256::wave_sync;
+
+__device__ __forceinline__ void inverse(const Ctx &x, const KGrid &g, const double2 *__restrict__ src, int lo,
+                                        bool live, double2 (&v)[8]) {
+  const int b = x.b, mmax = g.ntchopdim;
+  auto X = [&](int m, bool use) -> double2 {
+    use = use && live && m < g.ntchop;
+    double2 t = src[lo + (use ? g.ndimr * m : 0)];
+    if (!use) t = make_double2(0.0, 0.0);
+    if (m == 0) t.y = 0.0;
+    return t;
+  };
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    if (a == 3 || a == 4) {
+      v[a] = make_double2(0.0, 0.0);
+      continue;
+    }
+    const int j = 8 * a + b, jc = N - j;
+    const double2 t = x.th[a < 3 ? j : jc];
+    const double2 xv = a < 3 ? X(j, j < mmax) : fz_conj(X(jc, jc < mmax));
+    const double2 q = a < 3 ? make_double2(1.0 + t.y, t.x) : make_double2(1.0 - t.y, t.x);
+    v[a] = fz_cmul(xv, q);
+  }
+  dft8s_z34<1>(v);
+#pragma unroll
+  for (int c = 1; c < 8; c++) v[c] = fz_cmul(v[c], cj(x.t1[8 * c + b], true));
+#pragma unroll
+  for (int c = 0; c < 8; c++) x.s[e1(c, b)] = v[c];
+  wave_sync();
+#pragma unroll
+  for (int bb = 0; bb < 8; bb++) v[bb] = x.s[e1(b, bb)];
+  wave_sync();
+  dft8s<1>(v);
+}
+
+__device__ __forceinline__ void forward(const Ctx &x, const KGrid &g, const double2 (&v)[8], double2 *__restrict__ dst,
+                                        int lo, bool inr, bool live) {
+  const int b = x.b;
+  double2 u[8];
+#pragma unroll
+  for (int d = 0; d < 8; d++) u[d] = v[d];
+  dft8s<-1>(u);
+#pragma unroll
+  for (int bb = 0; bb < 8; bb++) x.s[e1(b, bb)] = u[bb];
+  wave_sync();
+#pragma unroll
+  for (int c = 0; c < 8; c++) u[c] = x.s[e1(c, b)];
+  wave_sync();
+#pragma unroll
+  for (int c = 1; c < 8; c++) u[c] = fz_cmul(u[c], x.t1[8 * c + b]);
+  dft8s<-1>(u);
+  // u[a] = Y[8 a + b]; Y[N - m] sits in lane (8 - b) mod 8 at a' = 7 - a
+  // (lane 0: a' = (8 - a) mod 8)
+  const double den = (double)(2 * N);
+  const int src = (8 - b) & 7;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (8 * a >= g.ntchopdim) break;
+    const int m = 8 * a + b;
+    double2 pc;
+    pc.x = __shfl(u[7 - a].x, src, 8);
+    pc.y = __shfl(u[7 - a].y, src, 8);
+    if (b == 0) pc = u[(8 - a) & 7];
+    if (m < g.ntchopdim && inr) {
+      double2 zm = u[a], zc = fz_conj(pc);
+      double2 e = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+      double2 d = make_double2(zm.x - zc.x, zm.y - zc.y);
+      double2 wd = fz_cmul(x.th[m], d);
+      double2 o = make_double2(e.x + 0.5 * wd.y, e.y - 0.5 * wd.x);
+      o = make_double2(o.x / den, o.y / den);
+      if (m == 0) o.y = 0.0;
+      if (m >= g.ntchop || !live) o = make_double2(0.0, 0.0);
+      dst[lo + g.ndimr * m] = o;
+    }
+  }
+}
+
+}  // namespace w64
+
+// Line shapes of the register-resident kernel below: lanes per theta line,
+// LDS per line, the NTCHOPDIM bound of the pruned first stage, and the
+// per-lane twiddle tables (T1[LANES c + b] = W_N^{b c}; T2 of NTH = 128's
+// middle stage, T2[4 g + f] = W_N^{8 f g}).
+struct Line128 {
+  static constexpr int N = 128, LANES = 16, LP = w128::LP, MMAX = w128::MMAX48, NT2 = 16;
+  __device__ static void tables(int q, double2 *T1, double2 *T2, const double2 *tw) {
+    if (q < 128) T1[q] = tw[(q & 15) * (q >> 4)];
+    if (q < 16) T2[q] = tw[8 * (q & 3) * (q >> 2)];
+  }
+  __device__ static void inverse(const w256::Ctx &x, const KGrid &g, const double2 *src, int lo, bool live,
+                                 double2 (&v)[8]) {
+    w128::inverse(x, g, src, lo, live, v);
+  }
+  __device__ static void forward(const w256::Ctx &x, const KGrid &g, const double2 (&v)[8], double2 *dst, int lo, bool inr,
+                                 bool live) {
+    w128::forward(x, g, v, dst, lo, inr, live);
+  }
+};
+struct Line64 {
+  static constexpr int N = 64, LANES = 8, LP = w64::LP, MMAX = w64::MMAX24, NT2 = 1;
+  __device__ static void tables(int q, double2 *T1, double2 *, const double2 *tw) {
+    if (q < 64) T1[q] = tw[(q & 7) * (q >> 3)];
+  }
+  __device__ static void inverse(const w256::Ctx &x, const KGrid &g, const double2 *src, int lo, bool live,
+                                 double2 (&v)[8]) {
+    w64::inverse(x, g, src, lo, live, v);
+  }
+  __device__ static void forward(const w256::Ctx &x, const KGrid &g, const double2 (&v)[8], double2 *dst, int lo, bool inr,
+                                 bool live) {
+    w64::forward(x, g, v, dst, lo, inr, live);
+  }
+};
+
+// The NTH = 256 kernel's products and field order on the lines of NTH = 128
+// (Line128: 16 lanes, 16 lines per block) or NTH = 64 (Line64: 8 lanes, 32
+// lines per block); a block takes adjacent radial nodes of one z plane.
+template <class L, bool LINEAR>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
-k_fused_products_128(KGrid g, const double *__restrict__ rsq, const double2 *tw, const double2 *twh, FusedIO io,
+k_fused_products_reg(KGrid g, const double *__restrict__ rsq, const double2 *tw, const double2 *twh, FusedIO io,
                      double two_omega, double n2) {
   if (blockIdx.x == 0 && threadIdx.x == 0 && io.zl) io.zl[0] = 0.0;
-  using namespace w128;
-  __shared__ double2 S[16 * LP];
-  __shared__ double2 Y[16 * N];  // [line][point i][lane]
-  __shared__ double2 T1[128], T2[16], TH[N + 1];
-  const int nrb = (g.ndimr + 15) / 16;
+  constexpr int N = L::N, LANES = L::LANES, LINES = 256 / LANES;
+  __shared__ double2 S[LINES * L::LP];
+  __shared__ double2 Y[LINES * N];  // [line][point i][lane]
+  __shared__ double2 T1[N], T2[L::NT2], TH[N + 1];
+  const int nrb = (g.ndimr + LINES - 1) / LINES;
   const int lid = xcd_logical(nrb * g.ndimx);
   if (lid < 0) return;
   {
     const int q = threadIdx.x;
-    if (q < 128) T1[q] = tw[(q & 15) * (q >> 4)];
-    if (q < 16) T2[q] = tw[8 * (q & 3) * (q >> 2)];
+    L::tables(q, T1, T2, tw);
     if (q <= N) TH[q] = twh[q];
   }
   __syncthreads();
-  const int z = lid / nrb, line = threadIdx.x >> 4;
-  const int rr = (lid % nrb) * 16 + line;
+  const int z = lid / nrb, line = threadIdx.x / LANES;
+  const int rr = (lid % nrb) * LINES + line;
   const bool inr = rr < g.ndimr, live = rr < g.nr;
-  Ctx x;
+  w256::Ctx x;
   x.t1 = T1;
   x.t2 = T2;
   x.th = TH;
-  x.s = S + line * LP;
-  x.b = threadIdx.x & 15;
+  x.s = S + line * L::LP;
+  x.b = threadIdx.x % LANES;
   double2 *ys = Y + line * N + x.b;
   const size_t off = (size_t)(rr - line) + (size_t)g.ndimr * g.ndimth * z;  // same for the block
   const int lo = inr ? line : 0;
@@ -821,25 +953,25 @@ k_fused_products_128(KGrid g, const double *__restrict__ rsq, const double2 *tw,
   const double r2 = live ? rsq[rr] : 1.0;
 
   double2 A[8], B[8], C[8], U[8], V[8], T[8];
-  w128::inverse(x, g, in(0), lo, live, A);  // rur
-  w128::inverse(x, g, in(1), lo, live, B);  // rup
-  w128::inverse(x, g, in(2), lo, live, C);  // uz
+  L::inverse(x, g, in(0), lo, live, A);  // rur
+  L::inverse(x, g, in(1), lo, live, B);  // rup
+  L::inverse(x, g, in(2), lo, live, C);  // uz
   // oz: U = rup oz, V = rur oz
-  w128::inverse(x, g, in(5), lo, live, T);
+  L::inverse(x, g, in(5), lo, live, T);
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     U[i] = make_double2(B[i].x * T[i].x, B[i].y * T[i].y);
     V[i] = make_double2(A[i].x * T[i].x, A[i].y * T[i].y);
   }
   // ror: Y = uz ror - rur oz (to LDS); V = rup ror
-  w128::inverse(x, g, in(3), lo, live, T);
+  L::inverse(x, g, in(3), lo, live, T);
 #pragma unroll
   for (int i = 0; i < 8; i++) {
-    ys[16 * i] = make_double2(C[i].x * T[i].x - V[i].x, C[i].y * T[i].y - V[i].y);
+    ys[LANES * i] = make_double2(C[i].x * T[i].x - V[i].x, C[i].y * T[i].y - V[i].y);
     V[i] = make_double2(B[i].x * T[i].x, B[i].y * T[i].y);
   }
   // rop: X = rup oz - uz rop in U; Z = (rur rop - rup ror) / r^2 in V
-  w128::inverse(x, g, in(4), lo, live, T);
+  L::inverse(x, g, in(4), lo, live, T);
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     U[i] = make_double2(U[i].x - C[i].x * T[i].x, U[i].y - C[i].y * T[i].y);
@@ -850,29 +982,29 @@ k_fused_products_128(KGrid g, const double *__restrict__ rsq, const double2 *tw,
 #pragma unroll
     for (int i = 0; i < 8; i++) U[i] = make_double2(U[i].x + two_omega * B[i].x, U[i].y + two_omega * B[i].y);
   }
-  w128::forward(x, g, U, out(0), lo, inr, live);
+  L::forward(x, g, U, out(0), lo, inr, live);
 #pragma unroll
   for (int i = 0; i < 8; i++) {
-    T[i] = ys[16 * i];
+    T[i] = ys[LANES * i];
     if (LINEAR) T[i] = make_double2(T[i].x - two_omega * A[i].x, T[i].y - two_omega * A[i].y);
   }
-  w128::forward(x, g, T, out(1), lo, inr, live);
+  L::forward(x, g, T, out(1), lo, inr, live);
   if (LINEAR) {
     // W: N_z = Z - W
-    w128::inverse(x, g, in(9), lo, live, T);
+    L::inverse(x, g, in(9), lo, live, T);
 #pragma unroll
     for (int i = 0; i < 8; i++) V[i] = make_double2(V[i].x - T[i].x, V[i].y - T[i].y);
   }
-  w128::forward(x, g, V, out(2), lo, inr, live);
+  L::forward(x, g, V, out(2), lo, inr, live);
   // rbr, rbp: A = rur rbr + rup rbp
-  w128::inverse(x, g, in(6), lo, live, T);
+  L::inverse(x, g, in(6), lo, live, T);
 #pragma unroll
   for (int i = 0; i < 8; i++) A[i] = make_double2(A[i].x * T[i].x, A[i].y * T[i].y);
-  w128::inverse(x, g, in(7), lo, live, T);
+  L::inverse(x, g, in(7), lo, live, T);
 #pragma unroll
   for (int i = 0; i < 8; i++) A[i] = make_double2(A[i].x + B[i].x * T[i].x, A[i].y + B[i].y * T[i].y);
   // bz: N_b = -(T / r^2 + uz bz) (+ Nbar^2 uz)
-  w128::inverse(x, g, in(8), lo, live, T);
+  L::inverse(x, g, in(8), lo, live, T);
 #pragma unroll
   for (int i = 0; i < 8; i++) {
     const double vr = A[i].x / r2 + C[i].x * T[i].x, vi = A[i].y / r2 + C[i].y * T[i].y;
@@ -880,7 +1012,7 @@ k_fused_products_128(KGrid g, const double *__restrict__ rsq, const double2 *tw,
     if (LINEAR) nb = make_double2(nb.x + n2 * C[i].x, nb.y + n2 * C[i].y);
     A[i] = nb;
   }
-  w128::forward(x, g, A, out(3), lo, inr, live);
+  L::forward(x, g, A, out(3), lo, inr, live);
 }
 
 bool fused_products_supported(const mlg_ctx *c) {
@@ -897,17 +1029,21 @@ int launch_fused_products(mlg_ctx *c, const double2 *const in[10], double2 *cons
   io.zl = zl;
   // PFP traffic: 9 (10) live input fields read, 4 written
   KScope ks(c, K_PRODUCTS, 16.0 * g.nr * g.ntchopdim * (double)g.nx * ((linear ? 10 : 9) + 4));
-  if (g.nth == 128 && g.ntchopdim <= w128::MMAX48) {
-    dim3 grid(xcd_grid((g.ndimr + 15) / 16 * g.ndimx));
-    if (linear)
-      hipLaunchKernelGGL(k_fused_products_128<true>, grid, dim3(256), 0, c->stream, g, c->d.r, c->d.tw_t, c->d.tw_h, io,
-                         two_omega, n2);
-    else
-      hipLaunchKernelGGL(k_fused_products_128<false>, grid, dim3(256), 0, c->stream, g, c->d.r, c->d.tw_t, c->d.tw_h,
-                         io, two_omega, n2);
-    MLG_CHECK_HIP(hipGetLastError());
-    return 0;
-  }
+#define L_REG(L_)                                                                                                    \
+  do {                                                                                                               \
+    dim3 grid(xcd_grid((g.ndimr + 256 / L_::LANES - 1) / (256 / L_::LANES) * g.ndimx));                             \
+    if (linear)                                                                                                      \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fused_products_reg<L_, true>), grid, dim3(256), 0, c->stream, g, c->d.r,  \
+                         c->d.tw_t, c->d.tw_h, io, two_omega, n2);                                                   \
+    else                                                                                                             \
+      hipLaunchKernelGGL(HIP_KERNEL_NAME(k_fused_products_reg<L_, false>), grid, dim3(256), 0, c->stream, g, c->d.r, \
+                         c->d.tw_t, c->d.tw_h, io, two_omega, n2);                                                   \
+    MLG_CHECK_HIP(hipGetLastError());                                                                                \
+    return 0;                                                                                                        \
+  } while (0)
+  if (g.nth == 128 && g.ntchopdim <= Line128::MMAX) L_REG(Line128);
+  if (g.nth == 64 && g.ntchopdim <= Line64::MMAX) L_REG(Line64);
+#undef L_REG
   if (g.nth == 256 && g.ntchopdim <= w256::MMAX96) {
     dim3 grid(xcd_grid((g.ndimr + 7) / 8 * g.ndimx));
     if (linear)

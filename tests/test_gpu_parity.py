@@ -30,6 +30,10 @@ THETA256 = dict(CFG1, nr=16, nth=256, nx=8, nrchop=12, ntchop=86, nxchop=3)
 # register-resident NTH = 128 products kernel; NR = 20 leaves a partial
 # block of radial lines
 THETA128 = dict(CFG1, nr=20, nth=128, nx=8, nrchop=14, ntchop=43, nxchop=3)
+# NTH = 64 (NTCHOP 22 as in the 128x64x64 workload): the register-resident
+# NTH = 64 products kernel (32 radial lines per block; NR = 36 leaves a
+# partial one)
+THETA64 = dict(CFG1, nr=36, nth=64, nx=8, nrchop=24, ntchop=22, nxchop=3)
 # NTH and NX above 256: the general radix-2 transforms, the separate product
 # kernels and the non-fused NONLIN sequence
 WIDE_THETA = dict(CFG1, nr=16, nth=512, nx=8, nrchop=12, ntchop=100, nxchop=3)
@@ -104,11 +108,11 @@ class Pair:
         return np.asfortranarray(z)
 
 
-@pytest.fixture(scope="module", params=["small", "cfg1", "theta256", "theta128", "wide_theta", "wide_z", "theta204",
-                                        "z608"])
+@pytest.fixture(scope="module", params=["small", "cfg1", "theta256", "theta128", "theta64", "wide_theta", "wide_z",
+                                        "theta204", "z608"])
 def P(request):
-    p = Pair({"small": SMALL, "cfg1": CFG1, "theta256": THETA256, "theta128": THETA128, "wide_theta": WIDE_THETA,
-              "wide_z": WIDE_Z, "theta204": THETA204, "z608": Z608}[request.param])
+    p = Pair({"small": SMALL, "cfg1": CFG1, "theta256": THETA256, "theta128": THETA128, "theta64": THETA64,
+              "wide_theta": WIDE_THETA, "wide_z": WIDE_Z, "theta204": THETA204, "z608": Z608}[request.param])
     yield p
     p.close()
 
@@ -477,10 +481,13 @@ def test_steps_wide_transforms(cfg):
     run_steps(cfg, 2)
 
 
-@pytest.mark.parametrize("cfg", [THETA128, dict(THETA128, adams=0, hyper_cn=1, nup=1e-3, kappap=1e-3, visc_p=4)],
-                         ids=["ab2cn", "etd2"])
-def test_steps_theta128(cfg):
-    """NTH = 128 steps: the register-resident NTH = 128 products kernel inside
+ETD_FILTER = dict(adams=0, hyper_cn=1, nup=1e-3, kappap=1e-3, visc_p=4)
+
+
+@pytest.mark.parametrize("cfg", [THETA128, dict(THETA128, **ETD_FILTER), THETA64, dict(THETA64, **ETD_FILTER)],
+                         ids=["nth128_ab2cn", "nth128_etd2", "nth64_ab2cn", "nth64_etd2"])
+def test_steps_register_products(cfg):
+    """NTH = 128 and 64 steps: the register-resident products kernels inside
     FORCE and NONLIN."""
     run_steps(cfg, 3)
 
