@@ -881,6 +881,116 @@ __device__ __forceinline__ void forward(const Ctx &x, const KGrid &g, const doub
 
 }  // namespace w64
 
+// ---------------------------------------------------------------------------
+// NTH = 32: 4 lanes own one theta line (sixteen lines per wave), each keeping
+// 8 of its 32 points.  The 32-point transform is 8 x 4 with one exchange:
+//   j = 4 a + b (lane b holds a = 0..7)          -> 8-point DFT over a, twiddle
+//   exchange: lane L holds b = 0..3 of c = L, L + 4 -> 4-point DFTs over b
+// so lane L ends with the points k = c + 8 d, c = L, L + 4, d = 0..3.
+// ---------------------------------------------------------------------------
+namespace w32 {
+
+constexpr int N = 32;
+constexpr int LP = 40;      // LDS double2 per line (rows of 5 against bank conflicts)
+constexpr int MMAX12 = 12;  // NTCHOPDIM bound: slots a = 3, 4 hold no mode
+__device__ __forceinline__ int e1(int c, int b) { return c * 5 + b; }
+using w256::cj;
+using w256::Ctx;  // t1[4 c + b] = W_N^{b c}, th[k] = exp(-i pi k / N)
+using w256::dft4s;
+using w256::dft8s;
+using w256::dft8s_z34;
+using w256::wave_sync;
+
+__device__ __forceinline__ void inverse(const Ctx &x, const KGrid &g, const double2 *__restrict__ src, int lo,
+                                        bool live, double2 (&v)[8]) {
+  const int b = x.b, mmax = g.ntchopdim;
+  auto X = [&](int m, bool use) -> double2 {
+    use = use && live && m < g.ntchop;
+    double2 t = src[lo + (use ? g.ndimr * m : 0)];
+    if (!use) t = make_double2(0.0, 0.0);
+    if (m == 0) t.y = 0.0;
+    return t;
+  };
+#pragma unroll
+  for (int a = 0; a < 8; a++) {
+    if (a == 3 || a == 4) {
+      v[a] = make_double2(0.0, 0.0);
+      continue;
+    }
+    const int j = 4 * a + b, jc = N - j;
+    const double2 t = x.th[a < 3 ? j : jc];
+    const double2 xv = a < 3 ? X(j, j < mmax) : fz_conj(X(jc, jc < mmax));
+    const double2 q = a < 3 ? make_double2(1.0 + t.y, t.x) : make_double2(1.0 - t.y, t.x);
+    v[a] = fz_cmul(xv, q);
+  }
+  dft8s_z34<1>(v);
+#pragma unroll
+  for (int c = 1; c < 8; c++) v[c] = fz_cmul(v[c], cj(x.t1[4 * c + b], true));
+#pragma unroll
+  for (int c = 0; c < 8; c++) x.s[e1(c, b)] = v[c];
+  wave_sync();
+  double2 t[2][4];
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++)
+#pragma unroll
+    for (int bb = 0; bb < 4; bb++) t[sg][bb] = x.s[e1(b + 4 * sg, bb)];
+  wave_sync();
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++) {
+    dft4s<1>(t[sg]);
+#pragma unroll
+    for (int d = 0; d < 4; d++) v[4 * sg + d] = t[sg][d];
+  }
+}
+
+__device__ __forceinline__ void forward(const Ctx &x, const KGrid &g, const double2 (&v)[8], double2 *__restrict__ dst,
+                                        int lo, bool inr, bool live) {
+  const int b = x.b;
+  double2 t[2][4];
+#pragma unroll
+  for (int sg = 0; sg < 2; sg++) {
+#pragma unroll
+    for (int d = 0; d < 4; d++) t[sg][d] = v[4 * sg + d];
+    dft4s<-1>(t[sg]);
+#pragma unroll
+    for (int bb = 0; bb < 4; bb++) x.s[e1(b + 4 * sg, bb)] = t[sg][bb];
+  }
+  wave_sync();
+  double2 u[8];
+#pragma unroll
+  for (int c = 0; c < 8; c++) u[c] = x.s[e1(c, b)];
+  wave_sync();
+#pragma unroll
+  for (int c = 1; c < 8; c++) u[c] = fz_cmul(u[c], x.t1[4 * c + b]);
+  dft8s<-1>(u);
+  // u[a] = Y[4 a + b]; Y[N - m] sits in lane (4 - b) mod 4 at a' = 7 - a
+  // (lane 0: a' = (8 - a) mod 8)
+  const double den = (double)(2 * N);
+  const int src = (4 - b) & 3;
+#pragma unroll
+  for (int a = 0; a < 3; a++) {
+    if (4 * a >= g.ntchopdim) break;
+    const int m = 4 * a + b;
+    double2 pc;
+    pc.x = __shfl(u[7 - a].x, src, 4);
+    pc.y = __shfl(u[7 - a].y, src, 4);
+    if (b == 0) pc = u[(8 - a) & 7];
+    if (m < g.ntchopdim && inr) {
+      double2 zm = u[a], zc = fz_conj(pc);
+      double2 e = make_double2(0.5 * (zm.x + zc.x), 0.5 * (zm.y + zc.y));
+      double2 d = make_double2(zm.x - zc.x, zm.y - zc.y);
+      double2 wd = fz_cmul(x.th[m], d);
+      double2 o = make_double2(e.x + 0.5 * wd.y, e.y - 0.5 * wd.x);
+      o = make_double2(o.x / den, o.y / den);
+      if (m == 0) o.y = 0.0;
+      if (m >= g.ntchop || !live) o = make_double2(0.0, 0.0);
+      dst[lo + g.ndimr * m] = o;
+    }
+  }
+}
+
+}  // namespace w32
+
 // Line shapes of the register-resident kernel below: lanes per theta line,
 // LDS per line, the NTCHOPDIM bound of the pruned first stage, and the
 // per-lane twiddle tables (T1[LANES c + b] = W_N^{b c}; T2 of NTH = 128's
@@ -900,6 +1010,20 @@ struct Line128 {
     w128::forward(x, g, v, dst, lo, inr, live);
   }
 };
+struct Line32 {
+  static constexpr int N = 32, LANES = 4, LP = w32::LP, MMAX = w32::MMAX12, NT2 = 1;
+  __device__ static void tables(int q, double2 *T1, double2 *, const double2 *tw) {
+    if (q < 32) T1[q] = tw[(q & 3) * (q >> 2)];
+  }
+  __device__ static void inverse(const w256::Ctx &x, const KGrid &g, const double2 *src, int lo, bool live,
+                                 double2 (&v)[8]) {
+    w32::inverse(x, g, src, lo, live, v);
+  }
+  __device__ static void forward(const w256::Ctx &x, const KGrid &g, const double2 (&v)[8], double2 *dst, int lo, bool inr,
+                                 bool live) {
+    w32::forward(x, g, v, dst, lo, inr, live);
+  }
+};
 struct Line64 {
   static constexpr int N = 64, LANES = 8, LP = w64::LP, MMAX = w64::MMAX24, NT2 = 1;
   __device__ static void tables(int q, double2 *T1, double2 *, const double2 *tw) {
@@ -916,8 +1040,9 @@ struct Line64 {
 };
 
 // The NTH = 256 kernel's products and field order on the lines of NTH = 128
-// (Line128: 16 lanes, 16 lines per block) or NTH = 64 (Line64: 8 lanes, 32
-// lines per block); a block takes adjacent radial nodes of one z plane.
+// (Line128: 16 lanes, 16 lines per block), NTH = 64 (Line64: 8 lanes, 32
+// lines) or NTH = 32 (Line32: 4 lanes, 64 lines); a block takes adjacent
+// radial nodes of one z plane.
 template <class L, bool LINEAR>
 __global__ void __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(2, 2)))
 k_fused_products_reg(KGrid g, const double *__restrict__ rsq, const double2 *tw, const double2 *twh, FusedIO io,
@@ -1043,6 +1168,7 @@ int launch_fused_products(mlg_ctx *c, const double2 *const in[10], double2 *cons
   } while (0)
   if (g.nth == 128 && g.ntchopdim <= Line128::MMAX) L_REG(Line128);
   if (g.nth == 64 && g.ntchopdim <= Line64::MMAX) L_REG(Line64);
+  if (g.nth == 32 && g.ntchopdim <= Line32::MMAX) L_REG(Line32);
 #undef L_REG
   if (g.nth == 256 && g.ntchopdim <= w256::MMAX96) {
     dim3 grid(xcd_grid((g.ndimr + 7) / 8 * g.ndimx));
