@@ -146,7 +146,9 @@ __global__ void k_adj_products(KGrid g, const double *r2, double two_omega, doub
 //   u_z:   row j  Te a5 + To a5,                   row NR-1-j  (Te - To) a5
 // A block takes one m and ADJ_PKB consecutive k: the factors go to LDS, each
 // thread a node j, and a table row is loaded once for the ADJ_PKB columns.
-#define ADJ_PKB 4
+// One column per block: at 128x64x64 that is 946 one-wave blocks (28 us),
+// where four columns per block left 242 (70 us).
+#define ADJ_PKB 1
 __global__ void __launch_bounds__(64) k_adj_project(KGrid g, const double *vdt, const double *ak, const double *at1,
                                                     const double2 *psib, const double2 *chib, double2 *vr, double2 *vp,
                                                     double2 *vz) {
