@@ -484,6 +484,15 @@ def test_steps_wide_transforms(cfg):
 ETD_FILTER = dict(adams=0, hyper_cn=1, nup=1e-3, kappap=1e-3, visc_p=4)
 
 
+@pytest.mark.parametrize("cfg", [dict(THETA128, ntchop=60), dict(THETA64, ntchop=30), dict(CFG1, nr=16, nrchop=12,
+                                                                                            ntchop=14, nxchop=3, nx=8)],
+                         ids=["nth128_ntchop60", "nth64_ntchop30", "nth32_ntchop14"])
+def test_steps_lds_products_fallback(cfg):
+    """theta truncations above the register-resident kernels' bounds (48,
+    24, 12 modes at NTH = 128, 64, 32): the two-pencil LDS products kernel."""
+    run_steps(cfg, 2)
+
+
 @pytest.mark.parametrize("cfg", [THETA128, dict(THETA128, **ETD_FILTER), THETA64, dict(THETA64, **ETD_FILTER)],
                          ids=["nth128_ab2cn", "nth128_etd2", "nth64_ab2cn", "nth64_etd2"])
 def test_steps_register_products(cfg):
