@@ -1436,15 +1436,23 @@ static int step_ab_cn(mlg_solver *s, mlg_field *psi, mlg_field *chi, mlg_field *
   mlg_field *psin = T.get(), *chin = T.get(), *bn = T.get(), *psi0 = T.get(), *chi0 = T.get(), *b0 = T.get();
   RET(T.ready());
   RET(mlg_force(psi, chi, b, psin, chin, bn));
-  RET(copy0(psi0, psi));
-  RET(copy0(chi0, chi));
-  RET(copy0(b0, b));
-  RET(ab2(psi, dt, psin, s->n_psi, true));
-  RET(ab2(chi, dt, chin, s->n_chi, true));
-  RET(ab2(b, dt, bn, s->n_b, false));
-  RET(copy0(s->n_psi, psin));
-  RET(copy0(s->n_chi, chin));
-  RET(copy0(s->n_b, bn));
+  // PSI0 = PSI ; AB2 ; N history = the new N, for the three fields in one launch
+  mlg_field *y[3] = {psi, chi, b}, *y0[3] = {psi0, chi0, b0}, *h[3] = {s->n_psi, s->n_chi, s->n_b};
+  const mlg_field *x[3] = {psin, chin, bn};
+  bool fff = true;
+  for (int i = 0; i < 3; i++) fff = fff && y[i]->space == MLG_FFF && x[i]->space == MLG_FFF;
+  if (fff) {
+    Ab2Jobs jobs;
+    for (int i = 0; i < 3; i++) {
+      y0[i]->space = h[i]->space = MLG_FFF;
+      jobs.j[i] = Ab2Job{y[i]->e, y0[i]->e, h[i]->e, x[i]->e, y[i]->ln, y0[i]->ln, h[i]->ln, x[i]->ln, i < 2};
+    }
+    RET(launch_ab2_hist(c, 3, jobs, dt));
+  } else {
+    for (int i = 0; i < 3; i++) RET(copy0(y0[i], y[i]));
+    for (int i = 0; i < 3; i++) RET(ab2(y[i], dt, x[i], h[i], i < 2));
+    for (int i = 0; i < 3; i++) RET(copy0(h[i], x[i]));
+  }
   RET(visc_cn(c, psi, chi, b, psi0, chi0, b0));
   return hyperv_state(c, psi, chi, b, dt);
 }

@@ -2609,6 +2609,37 @@ __global__ void k_ln_axpby(int mode, double *y, double a, const double *x1, cons
   else v = x1[0] - x2[0];
   y[0] = v;
 }
+// The middle of STEP_BOUSSI_AB_CN (src/mod_march.f90:609-661) for up to
+// three fields in one pass over the FFF elements: Y0 = COPY0(Y) (chopped),
+// Y = Y + DT (1.5 X - 0.5 H) (AB2, every element), H = COPY0(X); LN
+// likewise (the AB2 update of LN only where ln is set).  Element by element
+// these are k_copy0_fff's and k_axpby's operations.
+__global__ void k_ab2_hist(KGrid g, Ab2Jobs jobs, double dt, long nfff) {
+  const Ab2Job &J = jobs.j[blockIdx.y];
+  GRID_STRIDE(nfff) {
+    const int n = (int)(idx % g.nrchopdim), mm = (int)((idx / g.nrchopdim) % g.ntchopdim);
+    const int nn = mm < g.ntchop ? s_nrchops(g.nrchop, mm * g.minc) : 0;
+    const bool in = n < nn;
+    const double2 y = J.y[idx], x = J.x[idx], h = J.h[idx];
+    J.y0[idx] = in ? y : zero2();
+    J.y[idx] = add2(y, axc(dt, sub2(axc(1.5, x), axc(0.5, h))));
+    J.h[idx] = in ? x : zero2();
+  }
+  if (blockIdx.x == 0 && threadIdx.x == 0) {
+    const double yl = J.y_ln[0], xl = J.x_ln[0], hl = J.h_ln[0];
+    J.y0_ln[0] = yl * 1.0 + 0.0;
+    if (J.ln) J.y_ln[0] = yl + dt * (1.5 * xl - 0.5 * hl);
+    J.h_ln[0] = xl * 1.0 + 0.0;
+  }
+}
+int launch_ab2_hist(mlg_ctx *c, int n, const Ab2Jobs &jobs, double dt) {
+  if (n < 1 || n > 3) return MLG_E_ARG;
+  KGrid g = kgrid(c);
+  hipLaunchKernelGGL(k_ab2_hist, dim3(nblocks(c->n_fff), n), dim3(256), 0, c->stream, g, jobs, dt, c->n_fff);
+  MLG_CHECK_HIP(hipGetLastError());
+  return 0;
+}
+
 int launch_axpby(mlg_ctx *c, int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n) {
   hipLaunchKernelGGL(k_axpby, dim3(nblocks(n)), dim3(256), 0, c->stream, mode, y, a, x1, x2, x3, n);
   MLG_CHECK_HIP(hipGetLastError());

@@ -416,6 +416,20 @@ struct BandTJob {
 };
 int launch_band_solve_t_n(mlg_ctx *c, int op, int n, const BandTJob *jobs);
 int launch_axpby(mlg_ctx *c, int mode, double2 *y, double a, const double2 *x1, const double2 *x2, const double2 *x3, long n);
+// STEP_BOUSSI_AB_CN's middle for n (<= 3) FFF fields in one launch: y0 =
+// COPY0(y), y = AB2(y, x, h), h = COPY0(x), with their LN (y's AB2 only
+// where ln is set)
+struct Ab2Job {
+  double2 *y, *y0, *h;
+  const double2 *x;
+  double *y_ln, *y0_ln, *h_ln;
+  const double *x_ln;
+  int ln;
+};
+struct Ab2Jobs {
+  Ab2Job j[3];
+};
+int launch_ab2_hist(mlg_ctx *c, int n, const Ab2Jobs &jobs, double dt);
 int launch_ln_axpby(mlg_ctx *c, int mode, double *y, double a, const double *x1, const double *x2, const double *x3);
 // ETD update fused with the HYPERV3 filters (filter != 0), the copy of b
 // before the update (b0 != null) and the history stores (save bit 0: the
