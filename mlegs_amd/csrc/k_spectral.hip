@@ -381,22 +381,33 @@ int launch_del2(mlg_ctx *c, const double2 *a, const double *ln_a, double2 *b, do
   return 0;
 }
 
+// WK%LN = -ALP*A%LN, and DEL2's top-mode check of AO at NRCHOP+2
+// (one thread of each job)
+__device__ __forceinline__ void visc2_ln_check(const KGrid &g, int nrchop2, const Visc2Job &J, double alp, int *err) {
+  J.ln_wk[0] = J.ln_a[0] * -alp + 0.0;
+  if (g.ntchopdim >= 2) {
+    const int nt = s_nrchops(nrchop2, g.minc);
+    if (nt >= 1 && g.ntchop > 1 && nt - 1 < s_nrchops(g.nrchop, g.minc)) {
+      double2 tv = J.ao[(size_t)(nt - 1) + (size_t)g.nrchopdim * 1];
+      if (hypot(tv.x, tv.y) > 1.0e-15) atomicCAS(err, 0, 3);
+    }
+  }
+}
+
 // VISC2's right-hand side in one pass (mod_march.f90:1579-1636):
 //   CHOPSET(2); WK = DEL2(AO); CHOPSET(-2); CHOPDO(WK); WK = -WK - ALP*A;
 //   WK%LN = -ALP*A%LN
 // with DEL2 evaluated on the truncation NRCHOP+2 (nrchop2) at the positions
 // the base truncation keeps, and DEL2's top-mode check of AO at NRCHOP+2.
-// IL: WK is written row-interleaved over the systems: row n of system
-// s = m + NTCHOPDIM k at wk[n * NSYS + s] (NSYS = NTCHOPDIM * NXCHOPDIM), the
-// layout in which the band substitution, one thread per system, reads it
-// coalesced; otherwise in the FFF columns, which the partitioned
-// substitution stages with coalesced loads.  A block takes VR rows of 64
-// consecutive systems (grid: systems, row chunks, jobs): four threads per
-// column form VR/4 consecutive rows each (DEL2's row as del2_row forms it,
-// then -WK - ALP*A), and the tile goes through LDS: interleaved, each wave
-// writes whole rows of the 64 systems; in columns, each wave writes VR rows
-// of 64 / VR columns.
-template <int VR, bool IL>
+// k_visc2_rhs_il writes WK row-interleaved over the systems: row n of
+// system s = m + NTCHOPDIM k at wk[n * NSYS + s] (NSYS = NTCHOPDIM *
+// NXCHOPDIM), the layout in which the band substitution, one thread per
+// system, reads it coalesced.  A block takes VR rows of 64 consecutive
+// systems (grid: systems, row chunks, jobs): four threads per column form
+// VR/4 consecutive rows each (DEL2's row as del2_row forms it, then
+// -WK - ALP*A), and the tile goes through LDS, each wave writing whole rows
+// of the 64 systems.
+template <int VR>
 __global__ void __launch_bounds__(256) k_visc2_rhs_il(KGrid g, int nrchop2, const double *__restrict__ bd,
                                                       const double *__restrict__ ak, Visc2Jobs jobs, int *err) {
   constexpr int RPT = VR / 4;
@@ -490,34 +501,101 @@ __global__ void __launch_bounds__(256) k_visc2_rhs_il(KGrid g, int nrchop2, cons
       so[cs][sub * RPT + i] = sub2(make_double2(-v.x, -v.y), axc(alp, a));
     }
     __syncthreads();
-    if (IL) {
-      if (sys < nsys) {
+    if (sys < nsys) {
 #pragma unroll
-        for (int q = 0; q < RPT; q++) {
-          const int rl = rq * RPT + q, n = r0 + rl;
-          if (n < g.nrchopdim) J.wk[(size_t)n * nsys + sys] = so[c][rl];
+      for (int q = 0; q < RPT; q++) {
+        const int rl = rq * RPT + q, n = r0 + rl;
+        if (n < g.nrchopdim) J.wk[(size_t)n * nsys + sys] = so[c][rl];
+      }
+    }
+  }
+  if (s0 == 0 && blockIdx.y == 0 && t == 0) visc2_ln_check(g, nrchop2, J, alp, err);
+}
+
+// The same right-hand sides written in the FFF columns, the layout that the
+// partitioned substitution stages with coalesced loads.  A block takes VR
+// rows of up to 64 columns of one m (k = k0 .. k0+63), so DEL2's band rows
+// of that m are loaded once for the block; the AO rows r0-2 .. r0+VR+1 of
+// its columns go through LDS, loaded with consecutive lanes on consecutive
+// rows of a column.  The grid is 1D and XCD-ordered with the row chunks of
+// one column group consecutive, so a chunk's halo rows come from the L2
+// that its neighbours filled.  Four threads per column form VR/4
+// consecutive rows each by k_visc2_rhs_il's operations; each wave writes VR
+// consecutive rows of 64 / VR columns, adding -ALP*A on the way.
+template <int VR>
+__global__ void __launch_bounds__(256) k_visc2_rhs_col(KGrid g, int nrchop2, const double *__restrict__ bd,
+                                                       const double *__restrict__ ak, Visc2Jobs jobs, int nrc, int *err) {
+  constexpr int RPT = VR / 4, XR = VR + 4, XP = XR + 1;
+  static_assert(64 * XR % 256 == 0 && 64 * VR % 256 == 0, "whole tile rounds per block");
+  __shared__ double2 sx[64 * XP];
+  __shared__ double sb[VR * 5];
+  const int nkc = (g.nxchopdim + 63) / 64;
+  const int lid = xcd_logical(nrc * g.ntchopdim * nkc);
+  if (lid < 0) return;
+  const Visc2Job &J = jobs.j[blockIdx.y];
+  const int rc = lid % nrc, cg = lid / nrc;
+  const int ms = cg % g.ntchopdim, k0 = (cg / g.ntchopdim) * 64;
+  const int r0 = rc * VR, t = threadIdx.x, nk = min(64, g.nxchopdim - k0), nlast = g.nrchopdim - 1;
+  const int nn = ms < g.ntchop ? s_nrchops(g.nrchop, ms * g.minc) : 0;  // AO rows >= NRCHOPS read as zero
+  const int nn2 = s_nrchops(nrchop2, ms * g.minc);
+  const size_t kstride = (size_t)g.nrchopdim * g.ntchopdim;             // column (m, k) to (m, k + 1)
+  const size_t cb0 = (size_t)g.nrchopdim * ms + kstride * (size_t)k0;  // column (m, k0)
+  const double ln = J.ln_ao[0], alp = J.alp;
+  const bool rows = r0 - 2 < nn;  // the chunk holds rows of the truncation (uniform over the block)
+  if (rows) {
+    const double *bdm = bd + (size_t)ms * g.nband * 5;  // DEL2's table holds NRCHOPDIM+2 rows per mode
+    if (t < VR * 5) sb[t] = bdm[(size_t)min(r0 + t / 5, nlast) * 5 + t % 5];
+#pragma unroll
+    for (int q = 0; q < 64 * XR / 256; q++) {
+      const int e = q * 256 + t, col = e / XR, row = e % XR, n = r0 - 2 + row;
+      double2 v = zero2();
+      if (col < nk && n >= 0 && n < nn) v = J.ao[cb0 + kstride * col + n];
+      sx[col * XP + row] = v;
+    }
+  }
+  __syncthreads();
+  const int cs = t >> 2, sub = t & 3, ks = k0 + cs;
+  double2 res[RPT];
+  {
+    const double kv = ak[ms + (size_t)g.ntchopdim * (cs < nk ? ks : 0)], k2 = kv * kv;
+    const bool m0k0 = ms == 0 && ks == 0;
+#pragma unroll
+    for (int i = 0; i < RPT; i++) {
+      const int rl = sub * RPT + i, n = r0 + rl;
+      double2 v = zero2();
+      if (rows && n < nn) {  // band5_row, then -k^2 and the m0k0 log image (del2_row)
+        const double *r = sb + rl * 5;
+        const double2 *x = sx + cs * XP + rl;  // x[j]: row n - 2 + j
+        double2 o = axc(r[2], x[2]);
+        if (n >= 2) o = add2(o, axc(r[0], x[0]));
+        if (n >= 1) o = add2(o, axc(r[1], x[1]));
+        if (n + 1 < nn2) o = add2(o, axc(r[3], x[3]));
+        if (n + 2 < nn2) o = add2(o, axc(r[4], x[4]));
+        o = sub2(o, axc(k2, x[2]));
+        if (m0k0) {
+          if (n == 0) o.x = o.x + 4.0 / 3 * ln / g.ell2 / g.norm0;
+          if (n == 1) o.x = o.x - 2.0 * ln / g.ell2 / g.norm1;
+          if (n == 2) o.x = o.x + 2.0 / 3 * ln / g.ell2 / g.norm2;
         }
+        v = o;
       }
-    } else {
-      // a wave writes VR consecutive rows of 64 / VR columns per instruction
+      res[i] = v;
+    }
+  }
+  __syncthreads();  // every read of the AO tile is done
 #pragma unroll
-      for (int q = 0; q < VR / 4; q++) {
-        const int idx = q * 256 + t, cc = idx / VR, rl = idx % VR, n = r0 + rl;
-        const long sc = s0 + cc;
-        if (sc < nsys && n < g.nrchopdim) J.wk[(size_t)g.nrchopdim * (size_t)sc + n] = so[cc][rl];
-      }
+  for (int i = 0; i < RPT; i++) sx[cs * XP + sub * RPT + i] = res[i];
+  __syncthreads();
+#pragma unroll
+  for (int q = 0; q < 64 * VR / 256; q++) {
+    const int e = q * 256 + t, cc = e / VR, rl = e % VR, n = r0 + rl;
+    if (cc < nk && n < g.nrchopdim) {
+      const size_t o = cb0 + kstride * cc + n;
+      const double2 v = sx[cc * XP + rl];
+      J.wk[o] = sub2(make_double2(-v.x, -v.y), axc(alp, J.a[o]));
     }
   }
-  if (s0 == 0 && blockIdx.y == 0 && t == 0) {
-    J.ln_wk[0] = J.ln_a[0] * -alp + 0.0;
-    if (g.ntchopdim >= 2) {
-      const int nt = s_nrchops(nrchop2, g.minc);
-      if (nt >= 1 && g.ntchop > 1 && nt - 1 < s_nrchops(g.nrchop, g.minc)) {
-        double2 tv = J.ao[(size_t)(nt - 1) + (size_t)g.nrchopdim * 1];
-        if (hypot(tv.x, tv.y) > 1.0e-15) atomicCAS(err, 0, 3);
-      }
-    }
-  }
+  if (lid == 0 && t == 0) visc2_ln_check(g, nrchop2, J, alp, err);
 }
 int launch_visc2_rhs_il(mlg_ctx *c, int n, const Visc2Jobs &jobs, int il) {
   if (n < 1 || n > 3) return MLG_E_ARG;
@@ -526,13 +604,18 @@ int launch_visc2_rhs_il(mlg_ctx *c, int n, const Visc2Jobs &jobs, int il) {
   if (nrchop2 > g.nrchopdim) return 1;  // CHOPSET: NRCHOP TOO LARGE (mod_scalar3.f90:183-190)
   const long nsys = (long)g.ntchopdim * g.nxchopdim;
   constexpr int VR = 16;
-  const dim3 grid((unsigned)((nsys + 63) / 64), (unsigned)((g.nrchopdim + VR - 1) / VR), n);
-  if (il)
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_visc2_rhs_il<VR, true>), grid, dim3(256), 0, c->stream, g, nrchop2, c->d.bdel2h,
+  const int nrc = (g.nrchopdim + VR - 1) / VR;
+  if (il) {
+    const dim3 grid((unsigned)((nsys + 63) / 64), (unsigned)nrc, n);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_visc2_rhs_il<VR>), grid, dim3(256), 0, c->stream, g, nrchop2, c->d.bdel2h,
                        c->d.ak, jobs, c->d.err);
-  else
-    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_visc2_rhs_il<VR, false>), grid, dim3(256), 0, c->stream, g, nrchop2,
-                       c->d.bdel2h, c->d.ak, jobs, c->d.err);
+  } else {
+    const long nblk = (long)nrc * g.ntchopdim * ((g.nxchopdim + 63) / 64);
+    if (nblk > INT_MAX / 2) return MLG_E_ARG;
+    const dim3 grid((unsigned)xcd_grid((int)nblk), n);
+    hipLaunchKernelGGL(HIP_KERNEL_NAME(k_visc2_rhs_col<VR>), grid, dim3(256), 0, c->stream, g, nrchop2, c->d.bdel2h,
+                       c->d.ak, jobs, nrc, c->d.err);
+  }
   MLG_CHECK_HIP(hipGetLastError());
   return 0;
 }
