@@ -49,6 +49,8 @@ def parse():
                     help="skip the batched uneven sweep measurement (48 AB2-CN runs)")
     ap.add_argument("--sweep-only", action="store_true",
                     help="only run the sweep measurement (the modes of --sweep-modes) and print its JSON")
+    ap.add_argument("--adjoint-only", action="store_true",
+                    help="only run the adjoint measurement (128x64x64 AB2-CN step and its transpose) and print its JSON")
     ap.add_argument("--sweep-steps", type=int, default=50, help="steps per run of the sweep (2 untimed)")
     ap.add_argument("--sweep-modes", default=",".join(SWEEP_MODES),
                     help="comma-separated sweep modes to time (--sweep-only)")
@@ -396,6 +398,9 @@ def main():
         return
     if a.sweep_only:
         print(json.dumps(sweep_measurement(H, a.sweep_steps, modes=tuple(a.sweep_modes.split(",")))), flush=True)
+        return
+    if a.adjoint_only:
+        print(json.dumps(adjoint_measurement(H)), flush=True)
         return
     g = H.Grid(cfg)
     psi, chi, b = initial_state(H, g, cfg)
